@@ -1,0 +1,112 @@
+"""ctypes bindings of the HIP libraries built into qtqp_amd/_lib.
+
+Loading fails loudly (OSError) when a library has not been built or no HIP
+device is usable: there is no CPU fallback anywhere in the product path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import threading
+
+import numpy as np
+
+LIBDIR = pathlib.Path(__file__).resolve().parent / "_lib"
+
+QTQP_OK = 0
+QTQP_ERR_ARG = 1
+QTQP_ERR_HIP = 2
+QTQP_ERR_FACTOR = 3
+QTQP_ERR_NAN = 4
+QTQP_ERR_STATE = 5
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_lock = threading.Lock()
+_libs: dict[str, ctypes.CDLL] = {}
+
+
+class IrStats(ctypes.Structure):
+  _fields_ = [("solves", ctypes.c_int32), ("status", ctypes.c_int32),
+              ("final_residual_norm", ctypes.c_double), ("rhs_norm", ctypes.c_double),
+              ("tolerance", ctypes.c_double)]
+
+
+class KktInfo(ctypes.Structure):
+  _fields_ = [("precision", ctypes.c_int32), ("factor_precision", ctypes.c_int32),
+              ("factorizations", ctypes.c_int64), ("fp64_fallbacks", ctypes.c_int64),
+              ("backend_solves", ctypes.c_int64), ("inner_steps", ctypes.c_int64),
+              ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+
+# Exported symbol -> (restype, argtypes), matching include/qtqp_kkt.h.
+KKT_SYMBOLS = {
+    "qtqp_kkt_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _c_double_p, _c_double_p,
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_kkt_update_diag": (ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    "qtqp_kkt_factorize": (ctypes.c_int, [ctypes.c_void_p]),
+    "qtqp_kkt_solve": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    "qtqp_kkt_matvec": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    "qtqp_kkt_set_correction": (ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    "qtqp_kkt_solve_ir": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int32,
+                                         ctypes.c_double, ctypes.c_double, _c_double_p,
+                                         ctypes.POINTER(IrStats)]),
+    "qtqp_kkt_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KktInfo)]),
+    "qtqp_kkt_free": (None, [ctypes.c_void_p]),
+    "qtqp_last_error": (ctypes.c_char_p, []),
+    "qtqp_device_count": (ctypes.c_int, []),
+}
+
+
+def lib_path(name: str) -> pathlib.Path:
+  return LIBDIR / name
+
+
+def load(name: str, symbols: dict) -> ctypes.CDLL:
+  """Loads a built library and declares its C signatures (raises OSError)."""
+  with _lock:
+    if name in _libs:
+      return _libs[name]
+    path = lib_path(name)
+    if not path.exists():
+      raise OSError(f"{path} is not built; run `python -m qtqp_amd.build` "
+                    "(or __graft_entry__.build()) first")
+    lib = ctypes.CDLL(str(path))
+    for sym, (res, args) in symbols.items():
+      fn = getattr(lib, sym)
+      fn.restype = res
+      fn.argtypes = args
+    _libs[name] = lib
+    return lib
+
+
+def kkt_lib() -> ctypes.CDLL:
+  return load("libqtqp_kkt.so", KKT_SYMBOLS)
+
+
+def require_device(lib: ctypes.CDLL) -> None:
+  if lib.qtqp_device_count() < 1:
+    raise OSError("no HIP device is visible; the qtqp_amd HIP backends need an MI355X GPU")
+
+
+def dptr(a: np.ndarray):
+  return a.ctypes.data_as(_c_double_p)
+
+
+def as_f64(a) -> np.ndarray:
+  return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def check(lib: ctypes.CDLL, rc: int) -> None:
+  """Maps a C status to the exception the reference backends raise."""
+  if rc == QTQP_OK:
+    return
+  msg = (lib.qtqp_last_error() or b"").decode(errors="replace")
+  if rc == QTQP_ERR_FACTOR:
+    raise np.linalg.LinAlgError(msg)
+  if rc in (QTQP_ERR_ARG, QTQP_ERR_NAN):
+    raise ValueError(msg)
+  raise RuntimeError(f"qtqp_amd native error {rc}: {msg}")

@@ -1,0 +1,72 @@
+"""Builds the HIP shared libraries of qtqp_amd in-tree for gfx950.
+
+Each library is one translation unit under qtqp_amd/csrc compiled with hipcc
+into qtqp_amd/_lib/.  Usage: ``python -m qtqp_amd.build`` (or
+``__graft_entry__.build()``).
+"""
+
+from __future__ import annotations
+
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+
+PKG = pathlib.Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIBDIR = PKG / "_lib"
+INCLUDE = ROOT / "include"
+
+# name -> source file
+LIBRARIES = {
+    "libqtqp_kkt.so": "kkt.hip",
+    "libqtqp_batch.so": "batch.hip",
+    "libqtqp_mlp.so": "mlp.hip",
+}
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # Packed FP32 VALU forms are kept out of every kernel.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def hipcc() -> str:
+  for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+    if cand and pathlib.Path(cand).exists():
+      return cand
+  raise RuntimeError("hipcc not found; the HIP libraries cannot be built")
+
+
+def build(names=None, verbose: bool = True) -> list[pathlib.Path]:
+  LIBDIR.mkdir(exist_ok=True)
+  built = []
+  for lib, src in LIBRARIES.items():
+    if names and lib not in names:
+      continue
+    srcp = CSRC / src
+    if not srcp.exists():
+      continue
+    out = LIBDIR / lib
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", str(srcp), "-o", str(tmp)]
+    if verbose:
+      print("[build]", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+      sys.stderr.write(res.stdout + res.stderr)
+      raise RuntimeError(f"hipcc failed for {src}")
+    os.replace(tmp, out)
+    built.append(out)
+  return built
+
+
+if __name__ == "__main__":
+  build(names=sys.argv[1:] or None)

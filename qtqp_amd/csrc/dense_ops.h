@@ -1,0 +1,347 @@
+// Workgroup-level dense building blocks for the KKT engine on gfx950 (CDNA4).
+//
+// Conventions: one workgroup = WG = 256 threads = 4 wavefronts of 64 lanes.
+// Symmetric n x n matrices (the Gram block G and its factor L) are stored
+// column-major with leading dimension ld and only the lower triangle is
+// referenced, so column k of L is contiguous in the row index -- the same
+// access pattern as a row of the row-major constraint block A.  That makes
+// the Gram product A' W A and the LDL' trailing update L21 L21' the same
+// symmetric rank-k tile routine, X' diag(w) X with X read row by row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qk {
+
+constexpr int NB = 64;   // panel width of the blocked factorisation, tile edge
+constexpr int WG = 256;  // threads per workgroup
+constexpr int KC = 32;   // k-chunk staged through LDS by the rank-k tile
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+// v_mfma_f64_16x16x4_f64: A lane l holds A[l%16][l/16], B lane l holds
+// B[l/16][l%16], D lane l register r holds D[4*(l/16)+r][l%16].
+template <> struct Mfma<double> {
+  using acc_t = dbl4;
+  __device__ static inline acc_t mac(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+// v_mfma_f32_16x16x4_f32: same operand/result layout as the f64 form.
+template <> struct Mfma<float> {
+  using acc_t = flt4;
+  __device__ static inline acc_t mac(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// Block-wide sum / max; `red` is LDS scratch of at least 4 doubles.  All
+// threads of the workgroup must call; every thread gets the result.
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double r = red[0] + red[1] + red[2] + red[3];
+  return r;
+}
+__device__ inline double block_max(double v, double* red) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// Non-negative doubles compare like their bit patterns, so an inf-norm can be
+// accumulated with an integer atomic max (order independent, deterministic).
+__device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
+  if (!(v >= 0.0)) v = __longlong_as_double(0x7ff8000000000000ll);  // NaN wins
+  atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
+// Symmetric rank-k tile on MFMA:
+//   acc(j, i) = sum_{k in [kbeg,kend)} X[k*ldx + i] * w(k) * X[k*ldx + j]
+// for the 64x64 tile with row block i0 and column block j0 (global indices,
+// rows >= ilim / cols >= jlim read as zero).  Wave v owns the 32x32 sub-tile
+// j in [j0+32*(v>>1), +32), i in [i0+32*(v&1), +32) as 2x2 MFMA 16x16 blocks:
+// acc[a][b][r] is element j = j0+32*(v>>1)+16a+4*(lane>>4)+r,
+// i = i0+32*(v&1)+16b+(lane&15), so stores into a column-major matrix
+// (index j*ld+i) are contiguous across lanes.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RankKTile {
+  using acc_t = typename Mfma<T>::acc_t;
+  acc_t acc[2][2];
+
+  __device__ inline void zero() {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[a][b][r] = T(0);
+  }
+
+  // si/sj: LDS staging, KC rows of (NB+1) elements each.
+  template <typename WFun>
+  __device__ inline void run(const T* __restrict__ X, int64_t ldx, int kbeg,
+                             int kend, int i0, int j0, int ilim, int jlim,
+                             WFun wfun, T (*si)[NB + 1], T (*sj)[NB + 1]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = tid & 63, kr = tid >> 6;
+    const int jb = 32 * (wave >> 1), ib = 32 * (wave & 1);
+    for (int kc = kbeg; kc < kend; kc += KC) {
+#pragma unroll
+      for (int q = 0; q < KC / 4; ++q) {
+        const int kk = kr + 4 * q;
+        const int k = kc + kk;
+        T vi = T(0), vj = T(0);
+        if (k < kend) {
+          const T* row = X + (int64_t)k * ldx;
+          const T wk = wfun(k);
+          if (i0 + c < ilim) vi = row[i0 + c];
+          if (j0 + c < jlim) vj = row[j0 + c] * wk;
+        }
+        si[kk][c] = vi;
+        sj[kk][c] = vj;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < KC; kk += 4) {
+        const int kq = kk + (lane >> 4), cl = lane & 15;
+        T av0 = sj[kq][jb + cl], av1 = sj[kq][jb + 16 + cl];
+        T bv0 = si[kq][ib + cl], bv1 = si[kq][ib + 16 + cl];
+        acc[0][0] = Mfma<T>::mac(av0, bv0, acc[0][0]);
+        acc[0][1] = Mfma<T>::mac(av0, bv1, acc[0][1]);
+        acc[1][0] = Mfma<T>::mac(av1, bv0, acc[1][0]);
+        acc[1][1] = Mfma<T>::mac(av1, bv1, acc[1][1]);
+      }
+      __syncthreads();
+    }
+  }
+
+  // Calls f(i, j, value) for every accumulated element of this thread.
+  template <typename F>
+  __device__ inline void for_each(int i0, int j0, F f) const {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jb = j0 + 32 * (wave >> 1), ib = i0 + 32 * (wave & 1);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          f(ib + 16 * b + (lane & 15), jb + 16 * a + 4 * (lane >> 4) + r, acc[a][b][r]);
+  }
+};
+
+// Maps a linear index over the lower-triangular tiles (ti >= tj) of an
+// nt x nt tile grid to (ti, tj).
+__device__ inline void lower_tile(int t, int* ti, int* tj) {
+  int r = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  *ti = r;
+  *tj = t - r * (r + 1) / 2;
+}
+
+// ---------------------------------------------------------------------------
+// LDL' pieces.  The Gram block is positive definite, so its LDL' factor is
+// stored in the Cholesky form L D^{1/2} (one triangle, no separate D).
+// ---------------------------------------------------------------------------
+
+// Loads the nb x nb diagonal block at (k0, k0) of column-major G into S
+// (row-major S[i][j], lower part; padding rows/cols become the identity).
+template <typename T>
+__device__ inline void load_diag_block(const T* G, int64_t ld, int k0, int nb,
+                                       T (*S)[NB + 1]) {
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
+    const int i = idx & (NB - 1), j = idx >> 6;
+    T v;
+    if (i < nb && j < nb)
+      v = (i >= j) ? G[(int64_t)(k0 + j) * ld + k0 + i] : T(0);
+    else
+      v = (i == j) ? T(1) : T(0);
+    S[i][j] = v;
+  }
+}
+
+// In-LDS Cholesky of the diagonal block (all WG threads; 3 barriers/column).
+// A non-positive or NaN pivot records `kglob + j + 1` in *info (first one
+// wins) and is replaced by 1 so the sweep finishes without NaNs.
+template <typename T>
+__device__ inline void potrf_lds(T (*S)[NB + 1], T* sinv, int nb, int* info,
+                                 int kglob) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = 0; j < nb; ++j) {
+    const T piv = S[j][j];
+    const bool bad = !(piv > T(0));
+    const T d = bad ? T(1) : (T)sqrt((double)piv);
+    __syncthreads();
+    if (tid == 0) {
+      if (bad && info) atomicCAS(info, 0, kglob + j + 1);
+      S[j][j] = d;
+      sinv[j] = T(1) / d;
+    }
+    if (tid > j && tid < nb) S[tid][j] = S[tid][j] / d;
+    __syncthreads();
+    const int i = lane;
+    if (i > j && i < nb) {
+      const T lij = S[i][j];
+      for (int k = j + 1 + wave; k <= i; k += 4) S[i][k] -= lij * S[k][j];
+    }
+    __syncthreads();
+  }
+  for (int j = nb + tid; j < NB; j += WG) sinv[j] = T(1);
+  __syncthreads();
+}
+
+// Writes the factored lower diagonal block back into G.
+template <typename T>
+__device__ inline void store_diag_block(T* G, int64_t ld, int k0, int nb,
+                                        const T (*S)[NB + 1]) {
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
+    const int i = idx & (NB - 1), j = idx >> 6;
+    if (i < nb && j < nb && i >= j) G[(int64_t)(k0 + j) * ld + k0 + i] = S[i][j];
+  }
+}
+
+// Li = L11^{-1} for the factored diagonal block S (lower, row-major, padding
+// = identity).  Wave 0, lane j computes column j by forward substitution;
+// entries above the diagonal are zero.  Caller syncs afterwards.
+template <typename T>
+__device__ inline void tri_inverse_lds(const T (*S)[NB + 1], const T* sinv,
+                                       T (*Li)[NB + 1]) {
+  const int j = threadIdx.x;
+  if (j >= NB) return;
+  for (int i = 0; i < j; ++i) Li[i][j] = T(0);
+  Li[j][j] = sinv[j];
+  for (int i = j + 1; i < NB; ++i) {
+    T s = T(0);
+    for (int k = j; k < i; ++k) s += S[i][k] * Li[k][j];
+    Li[i][j] = -s * sinv[i];
+  }
+}
+
+// Panel solve on MFMA for rows [rbeg, rbeg+WG) below the diagonal block:
+// L21 = A21 L11^{-T}, computed as L21' = Li A21' so that results land
+// contiguously in the column-major G.  Wave v handles 64 rows as four
+// 16-row strips; each strip is 4 MFMA 16x16 tiles over the 64 columns.
+template <typename T>
+__device__ inline void trsm_mfma_rows(T* G, int64_t ld, int n, int k0, int nb,
+                                      int rbeg, const T (*Li)[NB + 1]) {
+  using acc_t = typename Mfma<T>::acc_t;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cl = lane & 15, kq = lane >> 4;
+  for (int s = 0; s < 4; ++s) {
+    const int r0 = rbeg + 64 * wave + 16 * s;
+    if (r0 >= n) break;
+    acc_t acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[ct][r] = T(0);
+    const int row = r0 + cl;
+#pragma unroll 4
+    for (int kk = 0; kk < NB; kk += 4) {
+      const int k = kk + kq;
+      const T b = (row < n && k < nb) ? G[(int64_t)(k0 + k) * ld + row] : T(0);
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) acc[ct] = Mfma<T>::mac(Li[16 * ct + cl][k], b, acc[ct]);
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = 16 * ct + 4 * kq + r;
+        if (c < nb && row < n) G[(int64_t)(k0 + c) * ld + row] = acc[ct][r];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with the factor, one workgroup, vector v in LDS.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline T lane_bcast(T v, int src) {
+  return __shfl(v, src, 64);
+}
+
+// Solves L v = v in place.
+template <typename T>
+__device__ inline void trsv_lower(const T* L, int64_t ld, int n, T* v,
+                                  T (*S)[NB + 1]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int nb = min(NB, n - j0);
+    load_diag_block(L, ld, j0, nb, S);
+    __syncthreads();
+    if (wave == 0) {
+      T zi = lane < nb ? v[j0 + lane] : T(0);
+      for (int c = 0; c < nb; ++c) {
+        const T zc = lane_bcast(zi, c) / S[c][c];
+        if (lane == c) zi = zc;
+        else if (lane > c) zi -= S[lane][c] * zc;
+      }
+      if (lane < nb) v[j0 + lane] = zi;
+    }
+    __syncthreads();
+    for (int i = j0 + nb + tid; i < n; i += WG) {
+      T acc = T(0);
+      const T* col = L + (int64_t)j0 * ld + i;
+      for (int c = 0; c < nb; ++c) acc += col[(int64_t)c * ld] * v[j0 + c];
+      v[i] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Solves L' v = v in place.
+template <typename T>
+__device__ inline void trsv_upper(const T* L, int64_t ld, int n, T* v,
+                                  T (*S)[NB + 1]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + NB - 1) / NB;
+  for (int jb = nblk - 1; jb >= 0; --jb) {
+    const int j0 = jb * NB, nb = min(NB, n - j0);
+    for (int c = wave; c < nb; c += 4) {
+      double acc = 0.0;
+      const T* col = L + (int64_t)(j0 + c) * ld;
+      for (int i = j0 + nb + lane; i < n; i += 64) acc += (double)col[i] * (double)v[i];
+      acc = wave_sum(acc);
+      if (lane == 0) v[j0 + c] -= (T)acc;
+    }
+    load_diag_block(L, ld, j0, nb, S);
+    __syncthreads();
+    if (wave == 0) {
+      T zi = lane < nb ? v[j0 + lane] : T(0);
+      for (int c = nb - 1; c >= 0; --c) {
+        const T xc = lane_bcast(zi, c) / S[c][c];
+        if (lane == c) zi = xc;
+        else if (lane < c) zi -= S[c][lane] * xc;
+      }
+      if (lane < nb) v[j0 + lane] = zi;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace qk

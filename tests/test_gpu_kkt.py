@@ -1,0 +1,196 @@
+"""GPU parity of the single-instance HIP KKT engine against the CPU oracle.
+
+Bars: KKT block equations to 1e-10 (tests/test_qtqp.py:740-771), matvec to
+1e-12 of the assembled K, end-to-end QTQP solves with the same status and
+x/y/s within 1e-6 relative of the oracle (FP64 solves on identical inputs).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import qtqp_amd
+from qtqp_amd import direct, problems
+from oracle import qtqp_oracle as O
+import golden_cases
+import qp_generators as G
+
+pytestmark = pytest.mark.gpu
+
+BACKENDS = [direct.HipDenseSolver, direct.HipDenseFp64Solver]
+ENUMS = [qtqp_amd.LinearSolver.HIP_DENSE, qtqp_amd.LinearSolver.HIP_DENSE_FP64]
+TRAJ = golden_cases.load_trajectories()
+
+
+def _rel(u, v):
+  return float(np.max(np.abs(u - v)) / max(1.0, np.max(np.abs(v))))
+
+
+def _kkt_inputs(m, n, z, seed, density=0.3):
+  a, b, c, p = G.gen_feasible(m, n, z, seed, density=density)
+  rng = np.random.default_rng(seed + 7)
+  mu = rng.uniform(0.05, 1.0)
+  s = rng.uniform(size=m)
+  y = rng.uniform(size=m)
+  s[:z] = 0.0
+  return a, b, c, p, mu, s, y
+
+
+class _HostLoop(direct.LinearSolver):
+  """Hides solve_refined so DirectKktSolver runs its host refinement loop
+  over the HIP backend's protocol methods (the reference's call pattern)."""
+
+  def __init__(self, inner):
+    self.inner = inner
+
+  def set_dims(self, n, m, z):
+    self.inner.set_dims(n, m, z)
+
+  def set_kkt(self, kkt):
+    self.inner.set_kkt(kkt)
+
+  def update_diag(self, diag):
+    self.inner.update_diag(diag)
+
+  def factorize(self):
+    self.inner.factorize()
+
+  def solve(self, rhs):
+    return self.inner.solve(rhs)
+
+  def __matmul__(self, x):
+    return self.inner @ x
+
+  def format(self):
+    return "csr"
+
+  def free(self):
+    self.inner.free()
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("mnz", [(6, 4, 1), (150, 100, 10), (700, 330, 40)])
+@pytest.mark.parametrize("host_loop", [False, True])
+def test_kkt_block_equations(require_gpu, backend, mnz, host_loop):
+  m, n, z = mnz
+  a, b, c, p, mu, s, y = _kkt_inputs(m, n, z, seed=842 + n)
+  be = backend()
+  kkt = direct.DirectKktSolver(a=sp.csc_matrix(a), p=sp.csc_matrix(p), z=z,
+                               min_static_regularization=1e-8,
+                               max_iterative_refinement_steps=10, atol=1e-12, rtol=1e-12,
+                               solver=_HostLoop(be) if host_loop else be)
+  kkt.update(mu=mu, s=s, y=y)
+  sol, st = kkt.solve(rhs=np.concatenate([c, b]), warm_start=np.zeros(n + m))
+  kkt.free()
+  d = np.concatenate([np.zeros(z), s[z:] / y[z:]])
+  np.testing.assert_allclose(p @ sol[:n] + mu * sol[:n] + a.T @ sol[n:], c, atol=1e-10, rtol=1e-10)
+  np.testing.assert_allclose(-a @ sol[:n] + (d + mu) * sol[n:], b, atol=1e-10, rtol=1e-10)
+  assert st["status"] == "converged"
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("mn", [(5, 3), (90, 70), (300, 129)])
+def test_backend_solve_and_matvec_match_numpy(require_gpu, backend, mn):
+  m, n = mn
+  rng = np.random.default_rng(m)
+  a = rng.standard_normal((m, n))
+  r = rng.standard_normal((n, n))
+  p = r @ r.T / n
+  dx = np.diag(p) + rng.uniform(0.1, 1.0, n)
+  dy = -rng.uniform(0.05, 2.0, m)
+  k = np.block([[p - np.diag(np.diag(p)) + np.diag(dx), a.T], [a, np.diag(dy)]])
+  be = backend()
+  be.set_dims(n, m, 0)
+  be.set_dense_blocks(a, p)
+  be.update_diag(np.concatenate([dx, dy]))
+  be.factorize()
+  v = rng.standard_normal(n + m)
+  np.testing.assert_allclose(be @ v, k @ v, rtol=1e-12, atol=1e-12)
+  sol = be.solve(v)
+  ref = np.linalg.solve(k, v)
+  assert _rel(sol, ref) < 1e-10
+  assert np.linalg.norm(k @ sol - v, np.inf) < 1e-10 * (1 + np.linalg.norm(v, np.inf))
+  info = be.info()
+  assert info["factorizations"] == 1 and info["backend_solves"] >= 1
+  be.free()
+  be.free()  # idempotent
+
+
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+@pytest.mark.parametrize("key", sorted(TRAJ))
+def test_end_to_end_matches_reference_fixtures(require_gpu, linear_solver, key):
+  case = TRAJ[key]
+  a, b, c, z, p = golden_cases.inputs(case["spec"])
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+      linear_solver=linear_solver, equilibrate=case["equilibrate"], verbose=False,
+      collect_stats=True)
+  assert sol.status.value == case["status"]
+  assert abs(len(sol.stats) - case["iters"]) <= 1
+  ref = O.solve_qp(a, b, c, z, p, equilibrate=case["equilibrate"])
+  unique = case["spec"]["kind"] in ("readme", "dense", "mpc") or case["status"] != "solved"
+  for k in "xys":
+    got, want = getattr(sol, k), ref[k]
+    assert np.array_equal(np.isnan(got), np.isnan(want)), k
+    ok = ~np.isnan(want)
+    if ok.any():
+      assert _rel(got[ok], want[ok]) < (1e-6 if unique else 1e-4), k
+
+
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+@pytest.mark.parametrize("kind", ["feasible", "infeasible", "unbounded"])
+@pytest.mark.parametrize("seed", [42, 43])
+def test_reference_problem_families(require_gpu, linear_solver, kind, seed):
+  m, n, z = 150, 100, 10
+  gen = {"feasible": G.gen_feasible, "infeasible": G.gen_infeasible,
+         "unbounded": G.gen_unbounded}[kind]
+  a, b, c, p = gen(m, n, z, seed)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+      linear_solver=linear_solver, verbose=False)
+  if kind == "feasible":
+    G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, z)
+  elif kind == "infeasible":
+    assert sol.status == qtqp_amd.SolutionStatus.INFEASIBLE
+    G.check_infeasible(sol.x, sol.y, sol.s, a, b, z)
+  else:
+    assert sol.status == qtqp_amd.SolutionStatus.UNBOUNDED
+    G.check_unbounded(sol.x, sol.y, sol.s, a, c, p, z)
+
+
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+def test_dense_generator_planted_optimum(require_gpu, linear_solver):
+  qp = problems.random_dense_qp(200, 300, 50, seed=0)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p)).solve(
+      linear_solver=linear_solver, verbose=False, atol=1e-10, rtol=1e-11)
+  assert sol.status == qtqp_amd.SolutionStatus.SOLVED
+  assert _rel(sol.x, qp.x_star) < 1e-6 and _rel(sol.y, qp.y_star) < 1e-6
+  assert _rel(sol.s, qp.s_star) < 1e-5
+
+
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+def test_condensed_mpc_matches_oracle(require_gpu, linear_solver):
+  qp = problems.mpc_qp(32, 8, 3, seed=11)
+  ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p)).solve(
+      linear_solver=linear_solver, verbose=False)
+  assert sol.status.value == ref["status"]
+  for k in "xys":
+    assert _rel(getattr(sol, k), ref[k]) < 1e-6, k
+
+
+def test_min_iterative_refinement_steps(require_gpu):
+  # tests/test_qtqp.py:1862-1873: one backend solve per KKT solve must suffice.
+  a, b, c, p = G.gen_feasible(150, 100, 10, 7)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=10, p=sp.csc_matrix(p)).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE, max_iterative_refinement_steps=1,
+      verbose=False)
+  G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, 10)
+
+
+def test_protocol_errors(require_gpu):
+  be = direct.HipDenseFp64Solver()
+  be.set_dims(3, 4, 0)
+  be.set_dense_blocks(np.ones((4, 3)), np.eye(3))
+  with pytest.raises(RuntimeError):
+    be.factorize()  # before update_diag
+  with pytest.raises(ValueError):
+    be.update_diag(np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0]))  # y-block not negative
+  be.free()
