@@ -1,0 +1,233 @@
+"""Benchmark: QPs/sec of the batched mixed-precision device IPM on one MI355X.
+
+Workload (one "step"): solve a batch of 4096 uneven condensed-MPC QPs
+(T ~ U{8..64}, nx ~ U{4..16}, nu ~ U{1..6}, instance i from seed 1000+i) to
+qtqp's default tolerances (atol 1e-7, rtol 1e-8), inputs resident in HBM,
+device-side equilibration included.  Printed as one JSON line with the
+roofline of the dominant kernel, the plain FP64 GPU path (one QP per call,
+FP64 factor and solves, same kernels, no batching, no graph) on a sample,
+and the CPU baseline (the FP64 oracle, a process pool on this host's cores)
+on a bounded sample of the same workload.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--count 4096]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "QPs/sec at 1e-8 rel KKT residual: mixed-prec batched vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+  print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def _cpu_worker(args):
+  seed_lo, seed_hi = args
+  os.environ["OMP_NUM_THREADS"] = "1"
+  from qtqp_amd import problems
+  from oracle import qtqp_oracle
+  qps = [problems.uneven_mpc_batch(1, base_seed=s)[0] for s in range(seed_lo, seed_hi)]
+  t0 = time.perf_counter()
+  st = [qtqp_oracle.solve_qp(q.a, q.b, q.c, q.z, q.p)["status"] for q in qps]
+  return time.perf_counter() - t0, len(qps), st
+
+
+def cpu_baseline(base_seed: int, budget_s: float = 20.0):
+  """FP64 oracle (dense Gram/Cholesky IPM) over a pool of this host's cores."""
+  try:
+    cores = len(os.sched_getaffinity(0))
+  except AttributeError:
+    cores = os.cpu_count() or 1
+  cores = max(1, min(cores, 16))
+  # calibrate on one core, then size the sample to ~budget_s of pool time
+  dt, k, _ = _cpu_worker((base_seed, base_seed + 4))
+  per_qp = dt / k
+  n_total = int(max(cores, min(4096, budget_s * cores / max(per_qp, 1e-4))))
+  per_worker = max(1, n_total // cores)
+  jobs = [(base_seed + 100 + w * per_worker, base_seed + 100 + (w + 1) * per_worker)
+          for w in range(cores)]
+  env_threads = os.environ.get("OMP_NUM_THREADS")
+  os.environ["OMP_NUM_THREADS"] = "1"
+  ctx = mp.get_context("spawn")
+  t0 = time.perf_counter()
+  with ctx.Pool(cores) as pool:
+    res = pool.map(_cpu_worker, jobs)
+  wall = time.perf_counter() - t0
+  if env_threads is None:
+    os.environ.pop("OMP_NUM_THREADS", None)
+  else:
+    os.environ["OMP_NUM_THREADS"] = env_threads
+  solved = sum(r[1] for r in res)
+  solve_time = max(r[0] for r in res)
+  return {"value": solved / solve_time, "unit": "QPs/s", "cores": cores, "kind": "port",
+          "sample": f"{solved} QPs of the same uneven-MPC family (seeds {base_seed + 100}..), "
+                    f"FP64 oracle IPM + dense Gram/Cholesky (scipy LAPACK), one process per core, "
+                    f"BLAS threads 1; wall {wall:.1f}s incl. pool start-up",
+          "cpu_model": _cpu_model()}
+
+
+def _cpu_model():
+  try:
+    for line in open("/proc/cpuinfo"):
+      if line.startswith("model name"):
+        return line.split(":", 1)[1].strip()
+  except OSError:
+    pass
+  return "unknown"
+
+
+def algorithmic_work(n, m, counts):
+  """Algorithmic FLOPs and HBM bytes of the executed work of every QP.
+
+  Per QP: each factorisation costs m n^2 + n^3/3 flops (Gram product plus
+  LDL') and reads A once; each K_reg product reads A twice (A x and A' y
+  from the row- and column-major copies) and P once in FP64; each
+  factor solve reads A twice (FP32 or FP64) and the triangular factor twice
+  (n^2/2 each way).
+  """
+  f32, f64, kmv, fs32, fs64 = (counts[:, k].astype(np.float64) for k in range(5))
+  mn, nn = m.astype(np.float64) * n, n.astype(np.float64) ** 2
+  fac_flops = m.astype(np.float64) * nn + nn * n / 3.0
+  flops32 = float(np.sum(f32 * fac_flops))
+  flops64 = float(np.sum(f64 * fac_flops))
+  bytes_ = np.sum(f32 * 4 * mn + f64 * 8 * mn + kmv * (16 * mn + 8 * nn)
+                  + fs32 * (8 * mn + 4 * nn) + fs64 * (16 * mn + 8 * nn))
+  return flops32, flops64, float(bytes_)
+
+
+def main():
+  ap = argparse.ArgumentParser()
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=3)
+  ap.add_argument("--warmup", type=int, default=1)
+  ap.add_argument("--count", type=int, default=4096)
+  ap.add_argument("--base-seed", type=int, default=1000)
+  ap.add_argument("--plain-sample", type=int, default=32)
+  ap.add_argument("--cpu-budget", type=float, default=20.0)
+  ap.add_argument("--no-cpu", action="store_true")
+  args = ap.parse_args()
+  if args.gpus != 1:
+    print(json.dumps({"error": "this build measures one GPU only; the batched engine shards "
+                               "QPs across GPUs by design but no multi-GPU run is supported"}))
+    sys.exit(2)
+
+  from qtqp_amd import batch, problems
+
+  log(f"generating {args.count} uneven MPC QPs")
+  t0 = time.perf_counter()
+  qps = problems.uneven_mpc_batch(args.count, base_seed=args.base_seed)
+  n = np.array([q.n for q in qps])
+  m = np.array([q.m for q in qps])
+  log(f"generated in {time.perf_counter() - t0:.1f}s; uploading")
+  bq = batch.BatchQP(qps)
+  log("warmup")
+  for _ in range(args.warmup):
+    bq.solve_device(precision="mixed")
+  log("timed steps")
+  times, cnts = [], []
+  for k in range(args.steps):
+    t0 = time.perf_counter()
+    cnt = bq.solve_device(precision="mixed")
+    times.append(time.perf_counter() - t0)
+    cnts.append(cnt)
+    log(f"step {k}: {times[-1] * 1e3:.1f} ms wall, {cnt['device_ms']:.1f} ms device, "
+        f"{cnt['iterations']} iterations")
+  sols = bq.results()
+  work = bq.work_counts()
+  ms_per_step = 1e3 * float(np.mean(times))
+  value = args.count / (ms_per_step / 1e3)
+  statuses = [s.status.value for s in sols]
+  iters = np.array([s.iters for s in sols])
+  launch_ms = bq.launch_times_ms()
+
+  # roofline of the dominant kernel (the per-iteration IPM kernel)
+  flops32, flops64, bytes_ = algorithmic_work(n, m, work)
+  iter_ms = float(np.sum(launch_ms))
+  t_mem = bytes_ / (HBM_PEAK_GBS * 1e9) * 1e3
+  t_mfma = flops32 / (FP32_PEAK_TFLOPS * 1e12) * 1e3 + flops64 / (FP64_PEAK_TFLOPS * 1e12) * 1e3
+  nl = max(1, len(launch_ms))
+  if t_mem >= t_mfma:
+    achieved = bytes_ / nl / (iter_ms / nl / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+  else:
+    achieved = (flops32 + flops64) / nl / (iter_ms / nl / 1e3) / 1e12
+    roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+  roof.update({"kernel": "k_ipm_iter", "launches": nl, "avg_launch_ms": iter_ms / nl,
+               "algorithmic_bytes_per_launch": bytes_ / nl,
+               "algorithmic_flops_per_launch": (flops32 + flops64) / nl,
+               "combined_frac": (t_mem + t_mfma) / iter_ms})
+
+  # plain FP64 GPU path: one QP per call, FP64 factor/solves, no batching, no graph
+  sample = qps[: args.plain_sample]
+  singles = [batch.BatchQP([q]) for q in sample]
+  for s in singles[:2]:
+    s.solve_device(precision="fp64", use_graph=False)
+  t0 = time.perf_counter()
+  for s in singles:
+    s.solve_device(precision="fp64", use_graph=False)
+  plain_s = time.perf_counter() - t0
+  plain_qps = len(sample) / plain_s
+  plain_status_match = all(s.results()[0].status.value == statuses[i]
+                           for i, s in enumerate(singles))
+  for s in singles:
+    s.free()
+
+  # accuracy against the FP64 oracle on a sample
+  from oracle import qtqp_oracle
+  errs = []
+  status_ok = True
+  for i in range(0, args.count, max(1, args.count // 16)):
+    q = qps[i]
+    ref = qtqp_oracle.solve_qp(q.a, q.b, q.c, q.z, q.p)
+    status_ok &= ref["status"] == statuses[i]
+    for k in "xys":
+      got, want = getattr(sols[i], k), ref[k]
+      errs.append(float(np.max(np.abs(got - want)) / max(1.0, np.max(np.abs(want)))))
+
+  line = {
+      "metric": METRIC, "value": value, "unit": "QPs/s", "n_gpus": 1, "steps": args.steps,
+      "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+      "scaling": "weak", "vs_baseline": None, "dtype": "f32 factor + f64 refinement",
+      "data": "synthetic (seeded condensed MPC)",
+      "config": {"workload": f"batched uneven condensed MPC, {args.count} QPs, T~U{{8..64}}, "
+                             f"nx~U{{4..16}}, nu~U{{1..6}}, seeds {args.base_seed}..",
+                 "count": args.count, "sum_mn": int(np.sum(m * n)),
+                 "mean_N": float(np.mean(n + m)), "precision": "mixed",
+                 "tolerances": "atol 1e-7, rtol 1e-8"},
+      "roofline": roof,
+      "plain_fp64_gpu": {"value": plain_qps, "unit": "QPs/s", "sample": len(sample),
+                         "status_match": plain_status_match,
+                         "speedup_of_batched_mixed": value / plain_qps},
+      "iterations": {"mean": float(iters.mean()), "max": int(iters.max())},
+      "statuses": {s: statuses.count(s) for s in set(statuses)},
+      "oracle_check": {"sample": len(errs) // 3, "status_match": bool(status_ok),
+                       "max_rel_err_xys": max(errs) if errs else None},
+      "counters": cnts[-1],
+  }
+  if not args.no_cpu:
+    log("cpu baseline")
+    line["cpu_baseline"] = cpu_baseline(args.base_seed, args.cpu_budget)
+  bq.free()
+  print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+  main()

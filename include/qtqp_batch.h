@@ -1,0 +1,108 @@
+/* C ABI of the batched device interior-point engine (MI355X, gfx950).
+ *
+ * Solves many independent dense QPs of uneven sizes with qtqp's algorithm
+ * (Mehrotra predictor-corrector on the homogeneous embedding), every
+ * per-iteration step on the GPU: one workgroup runs one whole IPM iteration
+ * of one QP (KKT update, MFMA Gram product and LDL', three refined KKT
+ * solves, tau, step, sigma, termination), a persistent grid pulls QPs from
+ * a work queue in longest-first order, and the per-iteration launch pair is
+ * replayed from a HIP graph until every QP has a status.
+ *
+ * Reference behaviour restated per QP (qtqp v0.0.5, paths relative to its
+ * checkout): src/qtqp/__init__.py:351-614 (QTQP._solve_impl) and 671-1008;
+ * src/qtqp/direct.py:183-307 (regularised diagonal, iterative refinement);
+ * src/qtqp/solvers_dense.py:99-142 (Gram/Schur reduction); Ruiz
+ * equilibration (__init__.py:616-669) also runs on the device.  Presolve and
+ * the final status-dependent scaling stay on the host (qtqp_amd/batch.py),
+ * as in __init__.py:248-297 and 580-614.
+ *
+ * Host pointers are caller-owned C-contiguous float64 arrays, copied during
+ * the call.  Calls are synchronous.  A handle is not thread-safe.
+ */
+#ifndef QTQP_BATCH_H_
+#define QTQP_BATCH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QTQP_BATCH_UNFINISHED 0
+#define QTQP_BATCH_SOLVED 1
+#define QTQP_BATCH_INFEASIBLE 2
+#define QTQP_BATCH_UNBOUNDED 3
+#define QTQP_BATCH_ERROR 5      /* FP64 factor breakdown (the reference raises LinAlgError) */
+
+typedef struct qtqp_batch qtqp_batch;
+
+/* Solve options: the keyword arguments of QTQP.solve (__init__.py:309-326)
+ * plus the engine's precision mode (QTQP_PREC_FP64 / QTQP_PREC_MIXED). */
+typedef struct qtqp_batch_opts {
+  double atol, rtol, atol_infeas, rtol_infeas;
+  double step_size_scale, min_static_regularization;
+  double linear_solver_atol, linear_solver_rtol;
+  int32_t max_iter, max_iterative_refinement_steps;
+  int32_t precision;
+  int32_t use_graph;    /* replay the per-iteration launches from a hipGraph */
+  int32_t equilibrate;  /* Ruiz scaling on the device (__init__.py:616-663) */
+} qtqp_batch_opts;
+
+/* Engine counters of one qtqp_batch_solve call. */
+typedef struct qtqp_batch_counters {
+  int64_t backend_solves;  /* K_reg^{-1} applications over all QPs */
+  int64_t inner_steps;     /* FP32 refinement steps over all QPs */
+  int64_t fp64_fallbacks;  /* QPs whose factor fell back to FP64 */
+  int64_t ir_not_converged;/* KKT solves that ended stalled / non-converged */
+  int32_t iterations;      /* IPM iterations launched (max over QPs) */
+  double device_ms;        /* device time of the whole solve (HIP events) */
+} qtqp_batch_counters;
+
+/* Packs `count` QPs onto the device (no compute).  QP i has sizes n[i],
+ * m[i], z[i] (1 <= n[i] <= 1024, 0 <= z[i] < m[i]) and its unscaled,
+ * presolved data at a + a_off[i] (m x n row-major), p + p_off[i] (n x n
+ * symmetric), b + m_off[i], c + n_off[i].  Packed vectors x/y/s used by
+ * set_start and results follow n_off / m_off. */
+int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                      const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                      const int64_t* m_off, const double* a, const double* p,
+                      const double* b, const double* c, qtqp_batch** out);
+
+/* Sets the starting point of the next solve for every QP: x, y, s in the
+ * unscaled coordinates (equilibrated on the device like _init_variables,
+ * __init__.py:299-307 and 668-669) and tau.  Without this call the default
+ * start x = 0, y[z:] = s[z:] = 1, tau = 1 is used. */
+int qtqp_batch_set_start(qtqp_batch* h, const double* x, const double* y, const double* s,
+                         const double* tau);
+
+/* Equilibrates on the device and runs the IPM on every QP until each has a
+ * status or max_iter iterations.  Inputs stay on the device, so a handle can
+ * be solved repeatedly. */
+int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_counters* counters);
+
+/* Copies back the final iterates in the unscaled coordinates (x = e x_eq,
+ * y = d y_eq, s = s_eq / d; not yet divided by tau), tau, status
+ * (QTQP_BATCH_*; UNFINISHED after max_iter), IPM iteration count and the
+ * final pres/dres/gap.  Any output pointer may be NULL. */
+int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* tau,
+                       int32_t* status, int32_t* iters, double* pres, double* dres,
+                       double* gap);
+
+/* Device time of each IPM-iteration launch of the last solve (HIP events on
+ * the launch stream).  Writes up to `cap` values; returns how many exist. */
+int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap);
+
+/* Executed work of the last solve per QP, 5 counts each: FP32 and FP64
+ * factorisations, K_reg products, FP32 and FP64 factor solves. */
+int qtqp_batch_work(qtqp_batch* h, int32_t* counts);
+
+void qtqp_batch_free(qtqp_batch* h);
+
+/* Message of the calling thread's last error ("" if none). */
+const char* qtqp_batch_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QTQP_BATCH_H_ */

@@ -1,0 +1,275 @@
+"""Batched device IPM: many uneven dense QPs solved per launch on one MI355X.
+
+``BatchQP(qps).solve(**kwargs)`` runs qtqp's algorithm (the same iteration as
+``qtqp_amd.QTQP``) for every QP with every per-iteration step on the GPU
+(include/qtqp_batch.h).  Presolve (qtqp __init__.py:248-270) and the
+status-dependent scaling of the result (580-614) run on the host here.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from typing import Sequence
+
+import numpy as np
+
+from . import _native
+from . import SolutionStatus
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class BatchOpts(ctypes.Structure):
+  _fields_ = [("atol", ctypes.c_double), ("rtol", ctypes.c_double),
+              ("atol_infeas", ctypes.c_double), ("rtol_infeas", ctypes.c_double),
+              ("step_size_scale", ctypes.c_double),
+              ("min_static_regularization", ctypes.c_double),
+              ("linear_solver_atol", ctypes.c_double), ("linear_solver_rtol", ctypes.c_double),
+              ("max_iter", ctypes.c_int32), ("max_iterative_refinement_steps", ctypes.c_int32),
+              ("precision", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+              ("equilibrate", ctypes.c_int32)]
+
+
+class BatchCounters(ctypes.Structure):
+  _fields_ = [("backend_solves", ctypes.c_int64), ("inner_steps", ctypes.c_int64),
+              ("fp64_fallbacks", ctypes.c_int64), ("ir_not_converged", ctypes.c_int64),
+              ("iterations", ctypes.c_int32), ("device_ms", ctypes.c_double)]
+
+
+BATCH_SYMBOLS = {
+    "qtqp_batch_create": (ctypes.c_int, [ctypes.c_int32, _i32p, _i32p, _i32p, _i64p, _i64p, _i64p,
+                                         _i64p, _dp, _dp, _dp, _dp,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_batch_set_start": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "qtqp_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts),
+                                        ctypes.POINTER(BatchCounters)]),
+    "qtqp_batch_results": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _i32p, _i32p,
+                                          _dp, _dp, _dp]),
+    "qtqp_batch_launch_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int32]),
+    "qtqp_batch_work": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
+    "qtqp_batch_free": (None, [ctypes.c_void_p]),
+    "qtqp_batch_last_error": (ctypes.c_char_p, []),
+}
+
+_STATUS = {1: SolutionStatus.SOLVED, 2: SolutionStatus.INFEASIBLE,
+           3: SolutionStatus.UNBOUNDED}
+
+
+def batch_lib() -> ctypes.CDLL:
+  lib = _native.load("libqtqp_batch.so", BATCH_SYMBOLS)
+  _native.require_device(_native.kkt_lib())
+  return lib
+
+
+def _check(lib, rc):
+  if rc != 0:
+    msg = (lib.qtqp_batch_last_error() or b"").decode(errors="replace")
+    if rc == _native.QTQP_ERR_ARG:
+      raise ValueError(msg)
+    raise RuntimeError(f"qtqp_amd batch error {rc}: {msg}")
+
+
+@dataclasses.dataclass
+class BatchSolution:
+  x: np.ndarray
+  y: np.ndarray
+  s: np.ndarray
+  status: SolutionStatus
+  iters: int
+  pres: float
+  dres: float
+  gap: float
+
+
+@dataclasses.dataclass
+class BatchResult:
+  solutions: list
+  device_ms: float
+  counters: dict
+
+  @property
+  def statuses(self):
+    return [s.status for s in self.solutions]
+
+  @property
+  def iterations(self) -> np.ndarray:
+    return np.array([s.iters for s in self.solutions])
+
+
+def _ptr(a, t):
+  return a.ctypes.data_as(t)
+
+
+class BatchQP:
+  """A batch of dense QPs ``(a, b, c, z, p)`` resident on the GPU.
+
+  Each QP follows qtqp's conventions: ``min 1/2 x'Px + c'x  s.t.
+  Ax + s = b, s[:z] = 0, s[z:] >= 0``; ``p`` may be None (LP).
+  """
+
+  def __init__(self, qps: Sequence):
+    self._lib = batch_lib()
+    self._h = None
+    ns, ms, zs, a_parts, p_parts, b_parts, c_parts = [], [], [], [], [], [], []
+    self._presolve = []
+    for q in qps:
+      a, b, c, z, p = (q.a, q.b, q.c, q.z, q.p) if hasattr(q, "a") else q
+      a = np.asarray(a, dtype=np.float64)
+      m, n = a.shape
+      b = np.array(b, dtype=np.float64)
+      c = np.asarray(c, dtype=np.float64)
+      p = np.zeros((n, n)) if p is None else np.asarray(p, dtype=np.float64)
+      if b.shape != (m,) or c.shape != (n,) or p.shape != (n, n) or not 0 <= z <= m:
+        raise ValueError("inconsistent QP shapes")
+      if not np.all(np.isfinite(b[:z])):
+        raise ValueError("Equality RHS entries in 'b' must be finite.")
+      if np.any(np.isnan(b[z:]) | np.isneginf(b[z:])):
+        raise ValueError("Inequality RHS entries in 'b' must be finite, +inf, or >= inf_bound.")
+      keep = np.ones(m, dtype=bool)
+      keep[z:] = b[z:] < 1e20
+      self._presolve.append((keep, a[~keep], b[~keep]) if not keep.all() else None)
+      a, b = a[keep], b[keep]
+      if z == a.shape[0]:
+        raise ValueError("effective z == m after presolve; some rows may have been dropped")
+      ns.append(n)
+      ms.append(a.shape[0])
+      zs.append(z)
+      a_parts.append(a.ravel())
+      p_parts.append(p.ravel())
+      b_parts.append(b)
+      c_parts.append(c)
+    self.count = len(ns)
+    self.n = np.array(ns, dtype=np.int32)
+    self.m = np.array(ms, dtype=np.int32)
+    self.z = np.array(zs, dtype=np.int32)
+    off = lambda sizes: np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    self.a_off = off([x.size for x in a_parts])
+    self.p_off = off([x.size for x in p_parts])
+    self.n_off = off(ns)
+    self.m_off = off(ms)
+    self._b = [b for b in b_parts]
+    self._c = [c for c in c_parts]
+    a_all = np.concatenate(a_parts)
+    p_all = np.concatenate(p_parts)
+    b_all = np.concatenate(b_parts)
+    c_all = np.concatenate(c_parts)
+    h = ctypes.c_void_p()
+    rc = self._lib.qtqp_batch_create(self.count, _ptr(self.n, _i32p), _ptr(self.m, _i32p),
+                                     _ptr(self.z, _i32p), _ptr(self.a_off, _i64p),
+                                     _ptr(self.p_off, _i64p), _ptr(self.n_off, _i64p),
+                                     _ptr(self.m_off, _i64p), _ptr(a_all, _dp), _ptr(p_all, _dp),
+                                     _ptr(b_all, _dp), _ptr(c_all, _dp), ctypes.byref(h))
+    _check(self._lib, rc)
+    self._h = h
+
+  def set_start(self, x, y, s, tau) -> None:
+    """Starting point of the next solve (unscaled coordinates), packed or lists."""
+    pack = lambda v: np.ascontiguousarray(np.concatenate(v) if isinstance(v, (list, tuple)) else v,
+                                          dtype=np.float64)
+    x, y, s, t = pack(x), pack(y), pack(s), np.ascontiguousarray(tau, dtype=np.float64)
+    if x.size != self.n.sum() or y.size != self.m.sum() or s.size != self.m.sum() or t.size != self.count:
+      raise ValueError("start has the wrong packed sizes")
+    _check(self._lib, self._lib.qtqp_batch_set_start(self._h, _ptr(x, _dp), _ptr(y, _dp),
+                                                     _ptr(s, _dp), _ptr(t, _dp)))
+
+  def solve_device(self, *, atol=1e-7, rtol=1e-8, atol_infeas=1e-8, rtol_infeas=1e-9,
+                   max_iter=100, step_size_scale=0.99, min_static_regularization=1e-8,
+                   max_iterative_refinement_steps=20, linear_solver_atol=1e-12,
+                   linear_solver_rtol=1e-12, equilibrate=True, precision="mixed",
+                   use_graph=True) -> dict:
+    """Runs the device IPM only (results stay on the GPU); returns counters."""
+    assert atol >= 0 and rtol >= 0 and atol_infeas >= 0 and rtol_infeas >= 0
+    assert max_iter > 0 and 0 < step_size_scale < 1 and max_iterative_refinement_steps >= 1
+    prec = {"mixed": _native.PREC_MIXED, "fp64": _native.PREC_FP64}[precision]
+    o = BatchOpts(atol, rtol, atol_infeas, rtol_infeas, step_size_scale,
+                  min_static_regularization, linear_solver_atol, linear_solver_rtol, max_iter,
+                  max_iterative_refinement_steps, prec, int(bool(use_graph)), int(bool(equilibrate)))
+    cnt = BatchCounters()
+    _check(self._lib, self._lib.qtqp_batch_solve(self._h, ctypes.byref(o), ctypes.byref(cnt)))
+    self._max_iter = max_iter
+    return {k: getattr(cnt, k) for k, _ in cnt._fields_}
+
+  def results(self) -> list:
+    tn, tm = int(self.n.sum()), int(self.m.sum())
+    x, y, s = np.empty(tn), np.empty(tm), np.empty(tm)
+    tau, pres, dres, gap = (np.empty(self.count) for _ in range(4))
+    st = np.empty(self.count, dtype=np.int32)
+    its = np.empty(self.count, dtype=np.int32)
+    _check(self._lib, self._lib.qtqp_batch_results(
+        self._h, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(tau, _dp), _ptr(st, _i32p),
+        _ptr(its, _i32p), _ptr(pres, _dp), _ptr(dres, _dp), _ptr(gap, _dp)))
+    out = []
+    for q in range(self.count):
+      n, m = int(self.n[q]), int(self.m[q])
+      xq = x[self.n_off[q]:self.n_off[q] + n].copy()
+      yq = y[self.m_off[q]:self.m_off[q] + m].copy()
+      sq = s[self.m_off[q]:self.m_off[q] + m].copy()
+      code = int(st[q])
+      if code == 5:
+        raise np.linalg.LinAlgError(f"QP {q}: FP64 factorisation broke down")
+      status = _STATUS.get(code, SolutionStatus.HIT_MAX_ITER)
+      xq, yq, sq = self._finish(q, status, xq, yq, sq, float(tau[q]))
+      out.append(BatchSolution(xq, yq, sq, status, int(its[q]), float(pres[q]), float(dres[q]),
+                               float(gap[q])))
+    return out
+
+  def _finish(self, q, status, x, y, s, tau):
+    """Status-dependent scaling and presolve restore (__init__.py:580-614, 272-297)."""
+    ps = self._presolve[q]
+    def restore(yv, sv, y_drop, s_drop):
+      if ps is None:
+        return yv, sv
+      keep = ps[0]
+      yf = np.empty(keep.size)
+      sf = np.empty(keep.size)
+      yf[keep], yf[~keep] = yv, y_drop
+      sf[keep], sf[~keep] = sv, s_drop
+      return yf, sf
+    if status is SolutionStatus.INFEASIBLE:
+      x[:] = np.nan
+      s[:] = np.nan
+      y = y / abs(self._b[q] @ y)
+      y, s = restore(y, s, 0.0, np.nan)
+      return x, y, s
+    if status is SolutionStatus.UNBOUNDED:
+      y[:] = np.nan
+      k = abs(self._c[q] @ x)
+      x, s = x / k, s / k
+      y, s = restore(y, s, np.nan, np.nan)
+      return x, y, s
+    x, y, s = x / tau, y / tau, s / tau
+    drop_s = 0.0 if ps is None else ps[2] - ps[1] @ x
+    y, s = restore(y, s, 0.0, drop_s)
+    return x, y, s
+
+  def launch_times_ms(self) -> np.ndarray:
+    """HIP-event durations of each IPM-iteration launch of the last solve."""
+    k = self._lib.qtqp_batch_launch_times(self._h, None, 0)
+    out = np.zeros(max(k, 0))
+    if k > 0:
+      self._lib.qtqp_batch_launch_times(self._h, _ptr(out, _dp), k)
+    return out
+
+  def work_counts(self) -> np.ndarray:
+    """(count, 5): FP32/FP64 factorisations, K products, FP32/FP64 factor solves."""
+    out = np.zeros((self.count, 5), dtype=np.int32)
+    _check(self._lib, self._lib.qtqp_batch_work(self._h, _ptr(out, _i32p)))
+    return out
+
+  def solve(self, **kwargs) -> BatchResult:
+    cnt = self.solve_device(**kwargs)
+    return BatchResult(self.results(), cnt["device_ms"], cnt)
+
+  def free(self) -> None:
+    if getattr(self, "_h", None) is not None:
+      self._lib.qtqp_batch_free(self._h)
+      self._h = None
+
+  def __del__(self):
+    try:
+      self.free()
+    except Exception:  # pylint: disable=broad-except
+      pass

@@ -1,0 +1,1259 @@
+// Batched device interior-point engine for many uneven dense QPs (gfx950).
+//
+// Layout in HBM (one allocation per array kind, instances packed back to
+// back in input order, every per-instance array 256-byte aligned):
+//   raw A (m x n row-major, as uploaded), A_eq and A_eq' (row- and
+//   column-major copies of the equilibrated block, FP64), the same two in
+//   FP32 for the FP32 factor path, raw and equilibrated P (n x n) in FP64
+//   and FP32, vectors b, c, d, e, the Gram/factor workspace G (n x n FP64,
+//   whose front half holds the FP32 factor), iterates and scratch vectors.
+// Two copies of A let every pass over it be coalesced: A' u as a column sum
+// of row-major A, A x as a column sum of column-major A.
+//
+// Execution: k_prepare (one workgroup per QP: transpose, Ruiz scaling, FP32
+// copies, default start) once per solve, then per IPM iteration the pair
+// {k_reset_queue, k_ipm_iter} replayed from a HIP graph.  k_ipm_iter is a
+// persistent grid: each workgroup takes the next QP from an atomic queue
+// (QPs sorted by descending m*n^2, so large ones start first) and runs one
+// whole IPM iteration of it; every workgroup exits when the queue is empty.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "dense_ops.h"
+#include "qtqp_batch.h"
+#include "qtqp_kkt.h"
+
+using namespace qk;
+
+namespace {
+
+thread_local std::string g_err;
+constexpr double kEps = 1e-15;  // qtqp _EPS (__init__.py:51)
+constexpr int kMaxInner = 12;   // FP32 refinement steps inside one backend solve
+constexpr int kStatusError = 5; // factorisation breakdown in FP64 (LinAlgError)
+
+enum { S_TAU = 0, S_MU, S_SIGMA, S_ALPHA, S_PRES, S_DRES, S_GAP, S_NSCAL = 8 };
+enum { I_STATUS = 0, I_ITERS, I_FORCE64, I_FALLBACK, I_F32, I_F64, I_KMV, I_FS32, I_FS64,
+       I_NINT = 12 };
+
+struct Inst {
+  int n, m, z, has_p;
+  // raw data
+  const double *Araw, *Praw, *braw, *craw;
+  // equilibrated data
+  double *A, *AT, *P, *b, *c, *d, *e;
+  float *A32, *AT32, *P32;
+  double normb, normc;
+  // workspace
+  double* G;
+  double *x, *y, *s, *kq, *dg, *corr, *krhs, *sol, *res, *iacc, *ires, *r, *xp;
+  double *um, *un, *ds, *cm;
+  double* sd;
+  int* si;
+};
+
+struct Opts {
+  double atol, rtol, atol_inf, rtol_inf, step_scale, min_reg, ls_atol, ls_rtol;
+  int max_iter, max_ir, precision, equilibrate;
+};
+
+struct Counters {
+  unsigned long long backend_solves, inner_steps, fallbacks, ir_bad;
+  int done;
+};
+
+struct Smem {
+  unsigned char* u;  // union scratch: {S, Li, sinv} or {si, sj}
+  double* v;         // triangular-solve vector (n values of T)
+  double* red;       // WG + 8 doubles
+};
+
+__device__ inline Smem carve(double* base, int nmax) {
+  Smem s;
+  s.u = reinterpret_cast<unsigned char*>(base);
+  s.v = base + (2 * NB * (NB + 1) + NB);
+  s.red = s.v + ((nmax + 1) & ~1);
+  return s;
+}
+
+template <typename T>
+__device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
+  return reinterpret_cast<T (*)[NB + 1]>(p) + row_off;
+}
+
+// -------------------------------------------------------------------------
+// Block-level vector kernels (every thread of the workgroup calls; each ends
+// with a barrier so results are visible to the whole workgroup).
+// -------------------------------------------------------------------------
+
+// epi(j, sum_{i<nrow} M[i*ncol+j] * u[i]) for j < ncol; exactly one thread
+// calls epi per j.  Deterministic summation order.
+template <typename T, typename EF>
+__device__ void colsum(const T* __restrict__ M, int nrow, int ncol, const double* __restrict__ u,
+                       EF epi, double* red) {
+  const int tid = threadIdx.x;
+  if (ncol >= WG / 2) {
+    for (int j = tid; j < ncol; j += WG) {
+      double a0 = 0.0, a1 = 0.0;
+      int i = 0;
+      for (; i + 1 < nrow; i += 2) {
+        a0 += (double)M[(int64_t)i * ncol + j] * u[i];
+        a1 += (double)M[(int64_t)(i + 1) * ncol + j] * u[i + 1];
+      }
+      if (i < nrow) a0 += (double)M[(int64_t)i * ncol + j] * u[i];
+      epi(j, a0 + a1);
+    }
+    __syncthreads();
+    return;
+  }
+  const int nph = WG / ncol;
+  const int j = tid % ncol, ph = tid / ncol;
+  double acc = 0.0;
+  if (ph < nph)
+    for (int i = ph; i < nrow; i += nph) acc += (double)M[(int64_t)i * ncol + j] * u[i];
+  red[tid] = acc;
+  __syncthreads();
+  if (tid < ncol) {
+    double s = 0.0;
+    for (int p = 0; p < nph; ++p) s += red[p * ncol + tid];
+    epi(tid, s);
+  }
+  __syncthreads();
+}
+
+// Column-wise max |M[i][j]| * wr[i] * wc[j] (wr/wc may be null = ones).
+template <typename EF>
+__device__ void colmax(const double* __restrict__ M, int nrow, int ncol,
+                       const double* __restrict__ wr, EF epi, double* red) {
+  const int tid = threadIdx.x;
+  const int nph = ncol >= WG / 2 ? 1 : WG / ncol;
+  for (int j0 = 0; j0 < ncol; j0 += (nph == 1 ? WG : ncol)) {
+    const int j = j0 + (nph == 1 ? tid : tid % ncol), ph = nph == 1 ? 0 : tid / ncol;
+    double mx = 0.0;
+    if (j < ncol && ph < nph)
+      for (int i = ph; i < nrow; i += nph) {
+        const double v = fabs(M[(int64_t)i * ncol + j]) * (wr ? wr[i] : 1.0);
+        mx = fmax(mx, v);
+      }
+    if (nph == 1) {
+      if (j < ncol) epi(j, mx);
+    } else {
+      red[tid] = mx;
+      __syncthreads();
+      if (tid < ncol) {
+        double s = 0.0;
+        for (int p = 0; p < nph; ++p) s = fmax(s, red[p * ncol + tid]);
+        epi(tid, s);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ inline double bsum(double v, double* red) { return block_sum(v, red + WG); }
+__device__ inline double bmax(double v, double* red) { return block_max(v, red + WG); }
+__device__ inline double bmin(double v, double* red) { return -block_max(-v, red + WG); }
+
+__device__ inline double dot(const double* a, const double* b, int len, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < len; i += WG) s += a[i] * b[i];
+  return bsum(s, red);
+}
+
+// -------------------------------------------------------------------------
+// Factorisation of G = P_off + diag(d_x) + A' W A (one workgroup).
+// -------------------------------------------------------------------------
+template <typename T>
+__device__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
+  const int n = I.n, m = I.m;
+  const int nt = (n + NB - 1) / NB;
+  const double* dy = I.dg + n;
+  T (*si)[NB + 1] = tile_ptr<T>(sm.u, 0);
+  T (*sj)[NB + 1] = tile_ptr<T>(sm.u, KC);
+  for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
+    int ti, tj;
+    lower_tile(t, &ti, &tj);
+    const int i0 = ti * NB, j0 = tj * NB;
+    RankKTile<T> acc;
+    acc.zero();
+    acc.run(A, n, 0, m, i0, j0, n, n, [dy](int k) { return (T)(-1.0 / dy[k]); }, si, sj);
+    acc.for_each(i0, j0, [&](int i, int j, T v) {
+      if (i < n && j < n && i >= j)
+        G[(int64_t)j * n + i] = v + ((i == j) ? (T)I.dg[i] : P[(int64_t)j * n + i]);
+    });
+  }
+  __syncthreads();
+  // diag(G) += 1e-14 * max(diag(G))  (solvers_dense.py:107-112)
+  double mx = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += WG) mx = fmax(mx, (double)G[(int64_t)i * n + i]);
+  const double jit = 1e-14 * bmax(mx, sm.red);
+  for (int i = threadIdx.x; i < n; i += WG)
+    G[(int64_t)i * n + i] = (T)((double)G[(int64_t)i * n + i] + jit);
+  __syncthreads();
+}
+
+// Blocked right-looking LDL' (Cholesky form) in place; returns breakdown
+// column (1-based) or 0.
+template <typename T>
+__device__ int chol(T* G, int n, Smem& sm) {
+  __shared__ int info;
+  if (threadIdx.x == 0) info = 0;
+  __syncthreads();
+  T (*S)[NB + 1] = tile_ptr<T>(sm.u, 0);
+  T (*Li)[NB + 1] = tile_ptr<T>(sm.u, NB);
+  T* sinv = reinterpret_cast<T*>(tile_ptr<T>(sm.u, 2 * NB));
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = min(NB, n - k0);
+    load_diag_block(G, n, k0, nb, S);
+    __syncthreads();
+    potrf_lds(S, sinv, nb, &info, k0);
+    store_diag_block(G, n, k0, nb, S);
+    if (k0 + NB < n) {
+      tri_inverse_lds(S, sinv, Li);
+      __syncthreads();
+      for (int rb = k0 + NB; rb < n; rb += WG) trsm_mfma_rows(G, n, n, k0, nb, rb, Li);
+      __syncthreads();
+      const int t0 = k0 + NB;
+      const int tt = (n - t0 + NB - 1) / NB;
+      T (*si)[NB + 1] = tile_ptr<T>(sm.u, 0);
+      T (*sj)[NB + 1] = tile_ptr<T>(sm.u, KC);
+      for (int t = 0; t < tt * (tt + 1) / 2; ++t) {
+        int ti, tj;
+        lower_tile(t, &ti, &tj);
+        const int i0 = t0 + ti * NB, j0 = t0 + tj * NB;
+        RankKTile<T> acc;
+        acc.zero();
+        acc.run(G + (int64_t)k0 * n, n, 0, NB, i0, j0, n, n, [](int) { return T(1); }, si, sj);
+        acc.for_each(i0, j0, [&](int i, int j, T v) {
+          if (i < n && j < n && i >= j) G[(int64_t)j * n + i] -= v;
+        });
+      }
+    }
+    __syncthreads();
+  }
+  const int r = info;
+  __syncthreads();
+  return r;
+}
+
+// Factor in the instance's current precision; FP32 breakdown falls back to
+// FP64.  Returns false on an FP64 breakdown.
+__device__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
+  if (!I.si[I_FORCE64]) {
+    float* G = reinterpret_cast<float*>(I.G);
+    if (threadIdx.x == 0) I.si[I_F32] += 1;
+    gram_syrk<float>(I, I.A32, I.P32, G, sm);
+    if (chol<float>(G, I.n, sm) == 0) return true;
+    if (threadIdx.x == 0) {
+      I.si[I_FORCE64] = 1;
+      I.si[I_FALLBACK] += 1;
+      atomicAdd(&cnt->fallbacks, 1ull);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) I.si[I_F64] += 1;
+  gram_syrk<double>(I, I.A, I.P, I.G, sm);
+  return chol<double>(I.G, I.n, sm) == 0;
+}
+
+// -------------------------------------------------------------------------
+// KKT products and solves.
+// -------------------------------------------------------------------------
+
+// out = b - K_reg v + corr*v   (b == nullptr: out = K_reg v); returns
+// ||out||_inf with NaN reported as inf.
+__device__ double kmv(const Inst& I, const double* v, const double* b, const double* corr,
+                      double* out, Smem& sm) {
+  const int n = I.n, m = I.m;
+  if (threadIdx.x == 0) I.si[I_KMV] += 1;
+  auto fin = [&](int idx, double k) {
+    if (!b) return k;
+    double r = b[idx] - k;
+    if (corr) r += corr[idx] * v[idx];
+    return r;
+  };
+  colsum<double>(I.A, m, n, v + n, [&](int j, double s) { out[j] = s; }, sm.red);
+  colsum<double>(I.P, n, n, v, [&](int j, double s) {
+    const double kx = out[j] + (s - I.P[(int64_t)j * n + j] * v[j]) + I.dg[j] * v[j];
+    out[j] = fin(j, kx);
+  }, sm.red);
+  colsum<double>(I.AT, n, m, v, [&](int i, double t) {
+    out[n + i] = fin(n + i, t + I.dg[n + i] * v[n + i]);
+  }, sm.red);
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < n + m; k += WG) {
+    const double a = fabs(out[k]);
+    mx = fmax(mx, (a == a) ? a : INFINITY);
+  }
+  return bmax(mx, sm.red);
+}
+
+// out (+)= K_reg^{-1} r with the factor in precision T:
+// g = r_x + A'(W r_y);  G x = g;  y = W (A x - r_y),  W = -1/d_y.
+template <typename T>
+__device__ void fsolve(const Inst& I, const T* A, const T* AT, const T* L, const double* r,
+                       double* out, bool accumulate, Smem& sm) {
+  const int n = I.n, m = I.m;
+  if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
+  for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -r[n + i] / I.dg[n + i];
+  __syncthreads();
+  T* v = reinterpret_cast<T*>(sm.v);
+  colsum<T>(A, m, n, I.um, [&](int j, double s) { v[j] = (T)(r[j] + s); }, sm.red);
+  T (*S)[NB + 1] = tile_ptr<T>(sm.u, 0);
+  trsv_lower(L, n, n, v, S);
+  trsv_upper(L, n, n, v, S);
+  for (int j = threadIdx.x; j < n; j += WG) {
+    const double xv = (double)v[j];
+    I.un[j] = xv;
+    out[j] = accumulate ? out[j] + xv : xv;
+  }
+  __syncthreads();
+  colsum<T>(AT, n, m, I.un, [&](int i, double t) {
+    const double yv = -(t - r[n + i]) / I.dg[n + i];
+    out[n + i] = accumulate ? out[n + i] + yv : yv;
+  }, sm.red);
+}
+
+// out += K_reg^{-1} r to FP64 accuracy.  FP32 factor: refine against the
+// exact K_reg; poor contraction switches the instance to an FP64 factor.
+__device__ bool backend_solve(const Inst& I, const double* r, double* out, Smem& sm,
+                              Counters* cnt) {
+  const int N = I.n + I.m;
+  if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
+  if (I.si[I_FORCE64]) {
+    fsolve<double>(I, I.A, I.AT, I.G, r, out, true, sm);
+    return true;
+  }
+  double r0 = 0.0;
+  for (int k = threadIdx.x; k < N; k += WG) {
+    I.iacc[k] = 0.0;
+    I.ires[k] = r[k];
+    const double a = fabs(r[k]);
+    r0 = fmax(r0, (a == a) ? a : INFINITY);
+  }
+  r0 = bmax(r0, sm.red);
+  if (r0 == 0.0) return true;
+  double prev = r0;
+  for (int step = 0; step < kMaxInner; ++step) {
+    const bool f64 = I.si[I_FORCE64] != 0;
+    if (f64)
+      fsolve<double>(I, I.A, I.AT, I.G, I.ires, I.iacc, true, sm);
+    else
+      fsolve<float>(I, I.A32, I.AT32, reinterpret_cast<const float*>(I.G), I.ires, I.iacc, true,
+                    sm);
+    if (!f64 && threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
+    const double cur = kmv(I, I.iacc, r, nullptr, I.ires, sm);
+    if (!(cur == cur) || cur <= 1e-15 * r0) break;
+    if (!f64 && cur > 1e-2 * prev) {
+      if (threadIdx.x == 0) {
+        I.si[I_FORCE64] = 1;
+        I.si[I_FALLBACK] += 1;
+        atomicAdd(&cnt->fallbacks, 1ull);
+      }
+      if (threadIdx.x == 0) I.si[I_F64] += 1;
+      __syncthreads();
+      gram_syrk<double>(I, I.A, I.P, I.G, sm);
+      if (chol<double>(I.G, I.n, sm) != 0) return false;
+      continue;
+    }
+    if (cur > 0.5 * prev) break;
+    prev = cur;
+  }
+  for (int k = threadIdx.x; k < N; k += WG) out[k] += I.iacc[k];
+  __syncthreads();
+  return true;
+}
+
+struct IrResult {
+  int solves, status;  // 0 converged, 1 non-converged, 2 stalled
+  double res;
+  bool ok;
+};
+
+// DirectKktSolver.solve (direct.py:243-307): sol from warm, K_true = K_reg - diag(corr).
+__device__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm, double* sol,
+                             const Opts& o, Smem& sm, Counters* cnt) {
+  const int n = I.n, N = I.n + I.m;
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < N; k += WG) {
+    const double v = k < n ? rhs[k] : -rhs[k];
+    I.krhs[k] = v;
+    sol[k] = warm[k];
+    const double a = fabs(v);
+    mx = fmax(mx, (a == a) ? a : INFINITY);
+  }
+  const double rhs_norm = bmax(mx, sm.red);
+  const double tol = o.ls_atol + o.ls_rtol * rhs_norm;
+  double res = kmv(I, sol, I.krhs, I.corr, I.res, sm);
+  IrResult out{0, 1, res, true};
+  int solves = 0;
+  for (solves = 1; solves <= o.max_ir; ++solves) {
+    const double old = res;
+    if (!backend_solve(I, I.res, sol, sm, cnt)) {
+      out.ok = false;
+      return out;
+    }
+    res = kmv(I, sol, I.krhs, I.corr, I.res, sm);
+    if (res < tol) {
+      out.status = 0;
+      break;
+    }
+    if (res >= old) {
+      out.status = 2;
+      break;
+    }
+  }
+  out.solves = min(solves, o.max_ir);
+  out.res = res;
+  if (out.status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
+  return out;
+}
+
+// -------------------------------------------------------------------------
+// Scalar pieces of the IPM (__init__.py:671-857).
+// -------------------------------------------------------------------------
+__device__ double max_step(const double* v, const double* dv_a, const double* dv_b, int lo,
+                           int hi, double* red) {
+  // dv = dv_a - dv_b if dv_b else dv_a
+  double mn = INFINITY;
+  for (int i = lo + threadIdx.x; i < hi; i += WG) {
+    const double dv = dv_b ? dv_a[i] - dv_b[i] : dv_a[i];
+    if (dv < -kEps) mn = fmin(mn, -v[i] / dv);
+  }
+  mn = bmin(mn, red);
+  return mn == INFINITY ? 1.0 : fmin(1.0, mn);
+}
+
+// P u for the n-vector u into out (has_p only).
+__device__ void pmul(const Inst& I, const double* u, double* out, Smem& sm) {
+  colsum<double>(I.P, I.n, I.n, u, [&](int j, double s) { out[j] = s; }, sm.red);
+}
+
+// Quadratic tau (__init__.py:759-817); returns false where it raises.
+__device__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double mu_t,
+                              double r_tau, double* tau_out, Smem& sm) {
+  const int n = I.n, m = I.m;
+  double t_a = mu + (dot(I.kq, I.c, n, sm.red) + dot(I.kq + n, I.b, m, sm.red));
+  double t_b = -r_tau - (dot(kr, I.c, n, sm.red) + dot(kr + n, I.b, m, sm.red));
+  double t_c = -mu_t;
+  if (I.has_p) {
+    pmul(I, kr, I.un, sm);   // P kr_x
+    pmul(I, I.kq, I.um, sm); // P kq_x (um has >= n slots: m >= 1 ... sized max(n,m))
+    t_a -= dot(I.kq, I.um, n, sm.red);
+    t_b += dot(kr, I.um, n, sm.red) + dot(I.kq, I.un, n, sm.red);
+    t_c -= dot(kr, I.un, n, sm.red);
+  }
+  if (fabs(t_a) < kEps) {
+    if (fabs(t_b) < kEps) return false;
+    const double t = -t_c / t_b;
+    if (!isfinite(t) || t < -1e-10) return false;
+    *tau_out = fmax(0.0, t);
+    return true;
+  }
+  double disc = t_b * t_b - 4 * t_a * t_c;
+  if (disc < -1e-9) return false;
+  disc = fmax(0.0, disc);
+  const double sq = sqrt(disc);
+  const double t = t_b > 0 ? t_c / (-0.5 * (t_b + sq)) : (-0.5 * (t_b - sq)) / t_a;
+  if (!isfinite(t) || t < -1e-10) return false;
+  *tau_out = fmax(0.0, t);
+  return true;
+}
+
+// Linearised tau with trust region (__init__.py:819-857).
+__device__ double tau_linearized(const Inst& I, const double* kr, double mu, double mu_t,
+                                 double tau_c, double tau_anchor, Smem& sm) {
+  const int n = I.n, m = I.m;
+  double x_px = 0.0, px_kq = 0.0, px_kr = 0.0;
+  if (I.has_p) {
+    pmul(I, I.x, I.un, sm);
+    x_px = dot(I.x, I.un, n, sm.red);
+    px_kq = dot(I.un, I.kq, n, sm.red);
+    px_kr = dot(I.un, kr, n, sm.red);
+  }
+  const double q_z = dot(I.c, I.x, n, sm.red) + dot(I.b, I.y, m, sm.red);
+  const double q_kq = dot(I.c, I.kq, n, sm.red) + dot(I.b, I.kq + n, m, sm.red);
+  const double q_kr = dot(I.c, kr, n, sm.red) + dot(I.b, kr + n, m, sm.red);
+  const double q_rz = q_kr - tau_c * q_kq - q_z;
+  const double px_rz = px_kr - tau_c * px_kq - x_px;
+  const double g = mu * tau_c * tau_c + (mu_t - mu) * tau_anchor * tau_c - tau_c * q_z - mu_t - x_px;
+  const double num = g - tau_c * q_rz - 2.0 * px_rz;
+  const double den = 2.0 * mu * tau_c + (mu_t - mu) * tau_anchor - q_z + tau_c * q_kq + 2.0 * px_kq;
+  const double t = tau_c + (fabs(den) < 1e-16 ? 0.0 : -num / den);
+  if (!isfinite(t)) return tau_c;
+  return fmin(fmax(t, 0.1 * tau_c), 10.0 * tau_c);
+}
+
+// Newton step (__init__.py:708-757): writes [x+; y+] into outxy, returns tau+.
+__device__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
+                       double tau_anchor, const double* corr_m, double tau, double* outxy,
+                       double* tau_plus, const Opts& o, Smem& sm, Counters* cnt) {
+  const int n = I.n, z = I.z, N = I.n + I.m;
+  for (int k = threadIdx.x; k < N; k += WG) {
+    double v = (mu - mu_t) * anchor[k];
+    if (k >= n + z) {
+      const int i = k - n;
+      if (mu_t != 0.0) v += mu_t / I.y[i];
+      v += I.s[i];
+      if (corr_m) v += corr_m[i];
+    }
+    I.r[k] = v;
+  }
+  __syncthreads();
+  const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt);
+  if (!ir.ok) return false;
+  double t = 0.0;
+  bool have = false;
+  if (ir.status == 0 || ir.res < 1e-7)
+    have = tau_quadratic(I, I.sol, mu, mu_t, (mu - mu_t) * tau_anchor, &t, sm);
+  if (!have) t = tau_linearized(I, I.sol, mu, mu_t, tau, tau_anchor, sm);
+  for (int k = threadIdx.x; k < N; k += WG) outxy[k] = I.sol[k] - I.kq[k] * t;
+  __syncthreads();
+  *tau_plus = t;
+  return true;
+}
+
+// One IPM iteration of one QP (__init__.py:473-578).
+__device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* cnt) {
+  const int n = I.n, m = I.m, z = I.z, N = n + m;
+  const int tid = threadIdx.x;
+  double* red = sm.red;
+  double tau = I.sd[S_TAU];
+
+  // normalise so ||(x, y, tau)||^2 = m - z + 1
+  double nx2 = 0.0;
+  for (int k = tid; k < n; k += WG) nx2 += I.x[k] * I.x[k];
+  double ny2 = 0.0;
+  for (int k = tid; k < m; k += WG) ny2 += I.y[k] * I.y[k];
+  const double nrm = sqrt(bsum(nx2, red) + bsum(ny2, red) + tau * tau);
+  const double scale = sqrt((double)(m - z + 1)) / fmax(kEps, nrm);
+  for (int k = tid; k < n; k += WG) I.x[k] *= scale;
+  for (int k = tid; k < m; k += WG) {
+    I.y[k] *= scale;
+    I.s[k] *= scale;
+  }
+  tau *= scale;
+  __syncthreads();
+  const double mu = dot(I.y, I.s, m, red) / (m - z);
+
+  // regularised diagonal and correction (direct.py:196-216)
+  for (int k = tid; k < N; k += WG) {
+    double t;
+    if (k < n) t = I.P[(int64_t)k * n + k] + mu;
+    else if (k < n + z) t = mu;
+    else t = I.s[k - n] / I.y[k - n] + mu;
+    const double rg = fmax(t, o.min_reg);
+    if (k < n) {
+      I.dg[k] = rg;
+      I.corr[k] = rg - t;
+    } else {
+      I.dg[k] = -rg;
+      I.corr[k] = -rg + t;
+    }
+  }
+  __syncthreads();
+  if (!factor(I, sm, cnt)) {
+    if (tid == 0) I.si[I_STATUS] = kStatusError;
+    return;
+  }
+
+  // kinv_q = K^{-1} [c; b], warm-started from the previous kinv_q
+  for (int k = tid; k < N; k += WG) I.xp[k] = I.kq[k];
+  __syncthreads();
+  {
+    for (int k = tid; k < N; k += WG) I.r[k] = k < n ? I.c[k] : I.b[k - n];
+    __syncthreads();
+    const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt);
+    if (!ir.ok) {
+      if (tid == 0) I.si[I_STATUS] = kStatusError;
+      return;
+    }
+    for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
+    __syncthreads();
+  }
+
+  // predictor (mu_target = 0) anchored at [x; y]
+  for (int k = tid; k < N; k += WG) I.iacc[k] = k < n ? I.x[k] : I.y[k - n];
+  __syncthreads();
+  // iacc is reused inside backend_solve; keep the anchor in `res`-free slot xp
+  for (int k = tid; k < N; k += WG) I.xp[k] = I.iacc[k];
+  __syncthreads();
+  double tau_p;
+  if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt)) {
+    if (tid == 0) I.si[I_STATUS] = kStatusError;
+    return;
+  }
+  // d_s = -y_p s / y ; step, sigma
+  for (int i = tid; i < m; i += WG) I.ds[i] = i < z ? 0.0 : -I.xp[n + i] * I.s[i] / I.y[i];
+  __syncthreads();
+  const double a_s = max_step(I.s, I.ds, nullptr, z, m, red);
+  const double a_y = max_step(I.y, I.xp + n, I.y, z, m, red);
+  const double alpha_p = fmin(a_s, a_y);
+  double sx = 0.0, sy = 0.0, sys = 0.0;
+  for (int k = tid; k < n; k += WG) {
+    const double xa = I.x[k] + alpha_p * (I.xp[k] - I.x[k]);
+    sx += xa * xa;
+  }
+  for (int i = tid; i < m; i += WG) {
+    const double ya = I.y[i] + alpha_p * (I.xp[n + i] - I.y[i]);
+    const double sa = I.s[i] + alpha_p * I.ds[i];
+    sy += ya * ya;
+    sys += ya * sa;
+  }
+  const double ta = tau + alpha_p * (tau_p - tau);
+  const double nrm2 = bsum(sx, red) + bsum(sy, red) + ta * ta;
+  const double mu_aff = (double)(m - z + 1) / fmax(kEps * kEps, nrm2) * bsum(sys, red) / (m - z);
+  const double base = mu_aff / fmax(kEps, mu);
+  const double sigma = fmin(fmax(base * base * base, 0.0), 1.0);
+
+  // corrector: Mehrotra term, mu_target = sigma mu, anchored at [x_p; y_p]
+  for (int i = tid; i < m; i += WG)
+    I.cm[i] = i < z ? 0.0 : -I.ds[i] * (I.xp[n + i] - I.y[i]) / I.y[i];
+  __syncthreads();
+  double tau_c;
+  if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt)) {
+    if (tid == 0) I.si[I_STATUS] = kStatusError;
+    return;
+  }
+  for (int i = tid; i < m; i += WG)
+    I.ds[i] = i < z ? 0.0 : sigma * mu / I.y[i] + I.cm[i] - I.xp[n + i] * I.s[i] / I.y[i];
+  __syncthreads();
+  const double alpha = fmin(max_step(I.s, I.ds, nullptr, z, m, red),
+                            max_step(I.y, I.xp + n, I.y, z, m, red));
+  const double step = o.step_scale * alpha;
+  for (int k = tid; k < n; k += WG) I.x[k] += step * (I.xp[k] - I.x[k]);
+  for (int i = tid; i < m; i += WG) {
+    double yv = I.y[i] + step * (I.xp[n + i] - I.y[i]);
+    double sv = I.s[i] + step * I.ds[i];
+    if (i >= z) {
+      yv = fmax(yv, 1e-30);
+      sv = fmax(sv, 1e-30);
+    }
+    I.y[i] = yv;
+    I.s[i] = sv;
+  }
+  tau = fmax(tau + step * (tau_c - tau), 1e-30);
+  __syncthreads();
+
+  // termination on the unscaled problem (__init__.py:888-965):
+  // A_orig x_orig = (A x)/d, A_orig' y_orig = (A' y)/e, P_orig x_orig = (P x)/e.
+  const double inv_tau = 1.0 / fmax(tau, kEps);
+  double* ax = I.um;  // m
+  double* aty = I.un; // n
+  colsum<double>(I.AT, n, m, I.x, [&](int i, double t) { ax[i] = t / I.d[i]; }, red);
+  colsum<double>(I.A, m, n, I.y, [&](int j, double t) { aty[j] = t / I.e[j]; }, red);
+  double* px = I.iacc;  // n
+  if (I.has_p) colsum<double>(I.P, n, n, I.x, [&](int j, double t) { px[j] = t; }, red);
+  double l_xpx = 0.0, l_ctx = 0.0, l_px = 0.0, l_aty = 0.0, l_dres = 0.0, l_nx = 0.0;
+  for (int j = tid; j < n; j += WG) {
+    const double pxj = I.has_p ? px[j] : 0.0;
+    l_xpx += I.x[j] * pxj;
+    l_ctx += I.c[j] * I.x[j];
+    const double pxo = pxj / I.e[j];
+    l_px = fmax(l_px, fabs(pxo));
+    l_aty = fmax(l_aty, fabs(aty[j]));
+    l_dres = fmax(l_dres, fabs((pxo + aty[j]) * inv_tau + I.c[j] / I.e[j]));
+    l_nx = fmax(l_nx, fabs(I.e[j] * I.x[j]));
+  }
+  double l_bty = 0.0, l_ax = 0.0, l_s = 0.0, l_pres = 0.0, l_axs = 0.0, l_ny = 0.0;
+  for (int i = tid; i < m; i += WG) {
+    l_bty += I.b[i] * I.y[i];
+    const double so = I.s[i] / I.d[i];
+    l_ax = fmax(l_ax, fabs(ax[i]));
+    l_s = fmax(l_s, fabs(so));
+    l_pres = fmax(l_pres, fabs((ax[i] + so) * inv_tau - I.b[i] / I.d[i]));
+    l_axs = fmax(l_axs, fabs(ax[i] + so));
+    l_ny = fmax(l_ny, fabs(I.d[i] * I.y[i]));
+  }
+  const double xpx = I.has_p ? bsum(l_xpx, red) : 0.0;
+  const double ctx = bsum(l_ctx, red), bty = bsum(l_bty, red);
+  const double norm_px = bmax(l_px, red), norm_aty = bmax(l_aty, red);
+  const double dres = bmax(l_dres, red), pres = bmax(l_pres, red);
+  const double norm_ax = bmax(l_ax, red), norm_s = bmax(l_s, red), norm_axs = bmax(l_axs, red);
+  const double norm_x = bmax(l_nx, red), norm_y = bmax(l_ny, red);
+  const double pcost = (ctx + 0.5 * xpx * inv_tau) * inv_tau;
+  const double dcost = (-bty - 0.5 * xpx * inv_tau) * inv_tau;
+  const double gap = fabs((ctx + bty + xpx * inv_tau) * inv_tau);
+  const double dinf = fmax(norm_axs / (fabs(ctx) + kEps), norm_px / (fabs(ctx) + kEps));
+  const double pinf = norm_aty / (fabs(bty) + kEps);
+  const double prel = fmax(fmax(norm_ax * inv_tau, norm_s * inv_tau), I.normb);
+  const double drel = fmax(fmax(norm_px * inv_tau, norm_aty * inv_tau), I.normc);
+  int status = QTQP_BATCH_UNFINISHED;
+  if (gap < o.atol + o.rtol * fmin(fabs(pcost), fabs(dcost)) && pres < o.atol + o.rtol * prel &&
+      dres < o.atol + o.rtol * drel)
+    status = QTQP_BATCH_SOLVED;
+  else if (ctx < -kEps && dinf < o.atol_inf + o.rtol_inf * norm_x / fabs(ctx))
+    status = QTQP_BATCH_UNBOUNDED;
+  else if (bty < -kEps && pinf < o.atol_inf + o.rtol_inf * norm_y / fabs(bty))
+    status = QTQP_BATCH_INFEASIBLE;
+  if (tid == 0) {
+    I.sd[S_TAU] = tau;
+    I.sd[S_MU] = mu;
+    I.sd[S_SIGMA] = sigma;
+    I.sd[S_ALPHA] = alpha;
+    I.sd[S_PRES] = pres;
+    I.sd[S_DRES] = dres;
+    I.sd[S_GAP] = gap;
+    I.si[I_STATUS] = status;
+    I.si[I_ITERS] += 1;
+  }
+  __syncthreads();
+}
+
+// -------------------------------------------------------------------------
+// Kernels
+// -------------------------------------------------------------------------
+
+// Per QP: transpose, Ruiz equilibration (__init__.py:616-663), FP32 copies,
+// norms of the unscaled b/c, default start (__init__.py:299-307).
+__global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equilibrate,
+                                                int default_start, int force64) {
+  extern __shared__ double smem[];
+  double* red = smem;  // WG + 8
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+    const Inst& I = insts[q];
+    const int n = I.n, m = I.m, tid = threadIdx.x;
+    // raw transpose into AT (as scratch for row norms), ones for d/e
+    for (int64_t k = tid; k < (int64_t)m * n; k += WG) {
+      const int i = (int)(k / n), j = (int)(k % n);
+      I.AT[(int64_t)j * m + i] = I.Araw[k];
+    }
+    for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
+    for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
+    __syncthreads();
+    if (equilibrate) {
+      for (int pass = 0; pass < 10; ++pass) {
+        // row inf-norms of D A E -> um; column norms of A and P -> un
+        colmax(I.AT, n, m, I.e, [&](int i, double v) {
+          const double r = v * I.d[i];
+          I.um[i] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
+        }, red);
+        colmax(I.Araw, m, n, I.d, [&](int j, double v) { I.un[j] = v * I.e[j]; }, red);
+        colmax(I.Praw, n, n, I.e, [&](int j, double v) {
+          const double cn = fmax(I.un[j], v * I.e[j]);
+          I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
+        }, red);
+        for (int i = tid; i < m; i += WG) I.d[i] *= I.um[i];
+        for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
+        __syncthreads();
+      }
+    }
+    for (int64_t k = tid; k < (int64_t)m * n; k += WG) {
+      const int i = (int)(k / n), j = (int)(k % n);
+      const double v = I.Araw[k] * (I.d[i] * I.e[j]);
+      I.A[k] = v;
+      I.A32[k] = (float)v;
+      I.AT[(int64_t)j * m + i] = v;
+      I.AT32[(int64_t)j * m + i] = (float)v;
+    }
+    for (int64_t k = tid; k < (int64_t)n * n; k += WG) {
+      const int i = (int)(k / n), j = (int)(k % n);
+      const double v = I.Praw[k] * (I.e[i] * I.e[j]);
+      I.P[k] = v;
+      I.P32[k] = (float)v;
+    }
+    double lb = 0.0, lc = 0.0;
+    for (int i = tid; i < m; i += WG) {
+      I.b[i] = I.braw[i] * I.d[i];
+      lb = fmax(lb, fabs(I.braw[i]));
+    }
+    for (int j = tid; j < n; j += WG) {
+      I.c[j] = I.craw[j] * I.e[j];
+      lc = fmax(lc, fabs(I.craw[j]));
+    }
+    lb = block_max(lb, red + WG);
+    lc = block_max(lc, red + WG);
+    if (tid == 0) {
+      insts[q].normb = lb;
+      insts[q].normc = lc;
+      for (int k = 0; k < S_NSCAL; ++k) I.sd[k] = (k == S_TAU) ? I.sd[k] : 0.0;
+      I.si[I_STATUS] = QTQP_BATCH_UNFINISHED;
+      I.si[I_ITERS] = 0;
+      for (int k = 0; k < I_NINT; ++k) I.si[k] = 0;
+      I.si[I_STATUS] = QTQP_BATCH_UNFINISHED;
+      I.si[I_FORCE64] = force64;
+    }
+    if (default_start) {
+      for (int j = tid; j < n; j += WG) I.x[j] = 0.0;
+      for (int i = tid; i < m; i += WG) {
+        I.y[i] = i < I.z ? 0.0 : 1.0 / I.d[i];
+        I.s[i] = i < I.z ? 0.0 : 1.0 * I.d[i];
+      }
+      if (tid == 0) I.sd[S_TAU] = 1.0;
+    } else {
+      // caller's start in unscaled coordinates -> equilibrated (__init__.py:668-669)
+      for (int j = tid; j < n; j += WG) I.x[j] = I.x[j] / I.e[j];
+      for (int i = tid; i < m; i += WG) {
+        I.y[i] = I.y[i] / I.d[i];
+        I.s[i] = I.s[i] * I.d[i];
+      }
+    }
+    for (int k = tid; k < n + m; k += WG) I.kq[k] = 0.0;
+    __syncthreads();
+  }
+}
+
+// Packed copies between the per-QP iterates and caller layouts.
+__global__ __launch_bounds__(WG) void k_gather(const Inst* insts, int count, const int64_t* n_off,
+                                               const int64_t* m_off, double* x, double* y,
+                                               double* s, double* scal, int* ints) {
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+    const Inst& I = insts[q];
+    for (int j = threadIdx.x; j < I.n; j += WG) x[n_off[q] + j] = I.e[j] * I.x[j];
+    for (int i = threadIdx.x; i < I.m; i += WG) {
+      y[m_off[q] + i] = I.d[i] * I.y[i];
+      s[m_off[q] + i] = I.s[i] / I.d[i];
+    }
+    if (threadIdx.x < S_NSCAL) scal[(int64_t)q * S_NSCAL + threadIdx.x] = I.sd[threadIdx.x];
+    if (threadIdx.x < I_NINT) ints[(int64_t)q * I_NINT + threadIdx.x] = I.si[threadIdx.x];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_scatter_start(const Inst* insts, int count,
+                                                      const int64_t* n_off, const int64_t* m_off,
+                                                      const double* x, const double* y,
+                                                      const double* s, const double* tau) {
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+    const Inst& I = insts[q];
+    for (int j = threadIdx.x; j < I.n; j += WG) I.x[j] = x[n_off[q] + j];
+    for (int i = threadIdx.x; i < I.m; i += WG) {
+      I.y[i] = y[m_off[q] + i];
+      I.s[i] = s[m_off[q] + i];
+    }
+    if (threadIdx.x == 0) I.sd[S_TAU] = tau[q];
+  }
+}
+
+__global__ void k_reset_queue(int* queue) {
+  if (threadIdx.x == 0) *queue = 0;
+}
+
+__global__ __launch_bounds__(WG) void k_ipm_iter(const Inst* insts, const int* order, int count,
+                                                 int* queue, Opts o, int nmax, Counters* cnt) {
+  extern __shared__ double smem[];
+  Smem sm = carve(smem, nmax);
+  __shared__ int s_q;
+  for (;;) {
+    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+    __syncthreads();
+    const int q = s_q;
+    __syncthreads();
+    if (q >= count) break;
+    const Inst& I = insts[order[q]];
+    if (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter) continue;
+    ipm_iteration(I, o, sm, cnt);
+    if (threadIdx.x == 0 &&
+        (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter))
+      atomicAdd(&cnt->done, 1);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct qtqp_batch {
+  int count = 0, nmax = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<void*> allocs;
+  std::vector<Inst> hinst;
+  Inst* dinst = nullptr;
+  int* dorder = nullptr;
+  int* dqueue = nullptr;
+  Counters* dcnt = nullptr;
+  bool have_start = false;
+  std::vector<int64_t> n_off, m_off;
+  std::vector<int> n, m;
+  int64_t tot_n = 0, tot_m = 0;
+  int64_t *d_noff = nullptr, *d_moff = nullptr;
+  double *d_px = nullptr, *d_py = nullptr, *d_ps = nullptr, *d_scal = nullptr;
+  int* d_ints = nullptr;
+  size_t smem_iter = 0;
+  std::vector<hipEvent_t> iter_ev;  // per-iteration launch timing
+  std::vector<double> iter_ms;
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(QTQP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Bump allocator over one device allocation.
+struct Arena {
+  unsigned char* base = nullptr;
+  size_t used = 0;
+  template <typename T>
+  T* take(int64_t count) {
+    T* p = reinterpret_cast<T*>(base + used);
+    used += align256(sizeof(T) * (size_t)std::max<int64_t>(count, 1));
+    return p;
+  }
+};
+
+size_t inst_bytes(int n, int m) {
+  const int64_t N = n + m, mn = (int64_t)m * n, nn = (int64_t)n * n;
+  size_t b = 0;
+  b += 3 * align256(8 * mn);       // A, AT (FP64) + raw A
+  b += 2 * align256(4 * mn);       // A32, AT32
+  b += 2 * align256(8 * nn);       // raw P, P
+  b += align256(4 * nn);           // P32
+  b += align256(8 * nn);           // G
+  b += 2 * align256(8 * m) + 2 * align256(8 * n);  // raw b, c ; (b, c below)
+  b += 2 * align256(8 * m) + 2 * align256(8 * n);  // b, c, d, e
+  b += 3 * align256(8 * std::max<int64_t>(m, n));  // x/y/s slack (x n, y m, s m)
+  b += 10 * align256(8 * N);                       // kq dg corr krhs sol res iacc ires r xp
+  b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
+  b += align256(8 * S_NSCAL) + align256(4 * I_NINT);
+  return b + 4096;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* qtqp_batch_last_error(void) { return g_err.c_str(); }
+
+int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                      const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                      const int64_t* m_off, const double* a, const double* p, const double* b,
+                      const double* c, qtqp_batch** out) {
+  if (!out) return fail(QTQP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (count < 1) return fail(QTQP_ERR_ARG, "count must be >= 1");
+  if (!n || !m || !z || !a_off || !p_off || !n_off || !m_off || !a || !p || !b || !c)
+    return fail(QTQP_ERR_ARG, "NULL argument");
+  int nmax = 0;
+  for (int q = 0; q < count; ++q) {
+    if (n[q] < 1 || m[q] < 1 || z[q] < 0 || z[q] >= m[q])
+      return fail(QTQP_ERR_ARG, "instance " + std::to_string(q) + ": need n>=1, 0<=z<m");
+    if (n[q] > 1024) return fail(QTQP_ERR_ARG, "batched engine supports n <= 1024");
+    nmax = std::max(nmax, (int)n[q]);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(QTQP_ERR_HIP, "no HIP device available");
+  auto* h = new qtqp_batch();
+  auto cleanup = [&](int rc) {
+    for (void* ptr : h->allocs) (void)hipFree(ptr);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc;
+  };
+  h->count = count;
+  h->nmax = nmax;
+  h->n.assign(n, n + count);
+  h->m.assign(m, m + count);
+  h->n_off.assign(n_off, n_off + count);
+  h->m_off.assign(m_off, m_off + count);
+  if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(QTQP_ERR_HIP, "stream"));
+  if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, "event"));
+  size_t total = 0;
+  for (int q = 0; q < count; ++q) total += inst_bytes(n[q], m[q]);
+  void* mem = nullptr;
+  hipError_t err = hipMalloc(&mem, total);
+  if (err != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, std::string("hipMalloc batch arena: ") + hipGetErrorString(err)));
+  h->allocs.push_back(mem);
+  Arena ar{static_cast<unsigned char*>(mem), 0};
+  h->hinst.resize(count);
+  int64_t tot_mn = 0, tot_nn = 0, tot_n = 0, tot_m = 0;
+  for (int q = 0; q < count; ++q) {
+    tot_mn = std::max(tot_mn, a_off[q] + (int64_t)m[q] * n[q]);
+    tot_nn = std::max(tot_nn, p_off[q] + (int64_t)n[q] * n[q]);
+    tot_n = std::max(tot_n, n_off[q] + n[q]);
+    tot_m = std::max(tot_m, m_off[q] + m[q]);
+  }
+  for (int q = 0; q < count; ++q) {
+    Inst& I = h->hinst[q];
+    const int nq = n[q], mq = m[q];
+    const int64_t N = nq + mq, mn = (int64_t)mq * nq, nn = (int64_t)nq * nq;
+    const int64_t mx = std::max<int64_t>(mq, nq);
+    I.n = nq;
+    I.m = mq;
+    I.z = z[q];
+    I.has_p = 0;
+    for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
+    double* araw = ar.take<double>(mn);
+    I.Araw = araw;
+    I.A = ar.take<double>(mn);
+    I.AT = ar.take<double>(mn);
+    I.A32 = ar.take<float>(mn);
+    I.AT32 = ar.take<float>(mn);
+    double* praw = ar.take<double>(nn);
+    I.Praw = praw;
+    I.P = ar.take<double>(nn);
+    I.P32 = ar.take<float>(nn);
+    I.G = ar.take<double>(nn);
+    double* braw = ar.take<double>(mq);
+    double* craw = ar.take<double>(nq);
+    I.braw = braw;
+    I.craw = craw;
+    I.b = ar.take<double>(mq);
+    I.c = ar.take<double>(nq);
+    I.d = ar.take<double>(mq);
+    I.e = ar.take<double>(nq);
+    I.x = ar.take<double>(mx);
+    I.y = ar.take<double>(mx);
+    I.s = ar.take<double>(mx);
+    I.kq = ar.take<double>(N);
+    I.dg = ar.take<double>(N);
+    I.corr = ar.take<double>(N);
+    I.krhs = ar.take<double>(N);
+    I.sol = ar.take<double>(N);
+    I.res = ar.take<double>(N);
+    I.iacc = ar.take<double>(N);
+    I.ires = ar.take<double>(N);
+    I.r = ar.take<double>(N);
+    I.xp = ar.take<double>(N);
+    I.um = ar.take<double>(mx);
+    I.un = ar.take<double>(mx);
+    I.ds = ar.take<double>(mx);
+    I.cm = ar.take<double>(mx);
+    I.sd = ar.take<double>(S_NSCAL);
+    I.si = ar.take<int>(I_NINT);
+    I.normb = I.normc = 0.0;
+    if (hipMemcpyAsync(araw, a + a_off[q], 8 * mn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(praw, p + p_off[q], 8 * nn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(braw, b + m_off[q], 8 * mq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(craw, c + n_off[q], 8 * nq, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+      return cleanup(fail(QTQP_ERR_HIP, "upload failed"));
+  }
+  if (ar.used > total) return cleanup(fail(QTQP_ERR_HIP, "arena overflow"));
+  h->tot_n = tot_n;
+  h->tot_m = tot_m;
+  if (hipMalloc(&h->dinst, sizeof(Inst) * count) != hipSuccess ||
+      hipMalloc(&h->dorder, sizeof(int) * count) != hipSuccess ||
+      hipMalloc(&h->dqueue, sizeof(int)) != hipSuccess ||
+      hipMalloc(&h->dcnt, sizeof(Counters)) != hipSuccess ||
+      hipMalloc(&h->d_noff, sizeof(int64_t) * count) != hipSuccess ||
+      hipMalloc(&h->d_moff, sizeof(int64_t) * count) != hipSuccess ||
+      hipMalloc(&h->d_px, sizeof(double) * std::max<int64_t>(tot_n, 1)) != hipSuccess ||
+      hipMalloc(&h->d_py, sizeof(double) * std::max<int64_t>(tot_m, 1)) != hipSuccess ||
+      hipMalloc(&h->d_ps, sizeof(double) * std::max<int64_t>(tot_m, 1)) != hipSuccess ||
+      hipMalloc(&h->d_scal, sizeof(double) * S_NSCAL * count) != hipSuccess ||
+      hipMalloc(&h->d_ints, sizeof(int) * I_NINT * count) != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, "hipMalloc tables"));
+  for (void* ptr : {(void*)h->dinst, (void*)h->dorder, (void*)h->dqueue, (void*)h->dcnt,
+                    (void*)h->d_noff, (void*)h->d_moff, (void*)h->d_px, (void*)h->d_py,
+                    (void*)h->d_ps, (void*)h->d_scal, (void*)h->d_ints})
+    h->allocs.push_back(ptr);
+  if (hipMemcpyAsync(h->d_noff, n_off, sizeof(int64_t) * count, hipMemcpyHostToDevice,
+                     h->stream) != hipSuccess ||
+      hipMemcpyAsync(h->d_moff, m_off, sizeof(int64_t) * count, hipMemcpyHostToDevice,
+                     h->stream) != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, "offset upload"));
+  // longest-processing-time order: descending m*n^2 (+ n^3/3)
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
+    const double cu = (double)m[u] * n[u] * n[u] + (double)n[u] * n[u] * n[u] / 3.0;
+    const double cv = (double)m[v] * n[v] * n[v] + (double)n[v] * n[v] * n[v] / 3.0;
+    return cu > cv;
+  });
+  if (hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * count, hipMemcpyHostToDevice,
+                     h->stream) != hipSuccess ||
+      hipMemcpyAsync(h->dinst, h->hinst.data(), sizeof(Inst) * count, hipMemcpyHostToDevice,
+                     h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, "table upload"));
+  h->smem_iter = sizeof(double) * (2 * NB * (NB + 1) + NB + ((nmax + 1) & ~1) + WG + 8);
+  if (hipFuncSetAttribute((const void*)k_ipm_iter, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)h->smem_iter) != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, "LDS attribute"));
+  *out = h;
+  return QTQP_OK;
+}
+
+int qtqp_batch_set_start(qtqp_batch* h, const double* x, const double* y, const double* s,
+                         const double* tau) {
+  if (!h || !x || !y || !s || !tau) return fail(QTQP_ERR_ARG, "NULL argument");
+  double* dt = nullptr;
+  HIPC(hipMalloc(&dt, sizeof(double) * h->count));
+  HIPC(hipMemcpyAsync(h->d_px, x, 8 * h->tot_n, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipMemcpyAsync(h->d_py, y, 8 * h->tot_m, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipMemcpyAsync(h->d_ps, s, 8 * h->tot_m, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipMemcpyAsync(dt, tau, 8 * h->count, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_scatter_start, dim3(std::min(h->count, 2048)), dim3(WG), 0, h->stream,
+                     h->dinst, h->count, h->d_noff, h->d_moff, h->d_px, h->d_py, h->d_ps, dt);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(h->stream));
+  HIPC(hipFree(dt));
+  h->have_start = true;
+  return QTQP_OK;
+}
+
+int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_counters* counters) {
+  if (!h || !opts) return fail(QTQP_ERR_ARG, "NULL argument");
+  if (opts->max_iter < 1 || opts->max_iterative_refinement_steps < 1)
+    return fail(QTQP_ERR_ARG, "max_iter and max_iterative_refinement_steps must be >= 1");
+  Opts o{};
+  o.atol = opts->atol;
+  o.rtol = opts->rtol;
+  o.atol_inf = opts->atol_infeas;
+  o.rtol_inf = opts->rtol_infeas;
+  o.step_scale = opts->step_size_scale;
+  o.min_reg = opts->min_static_regularization;
+  o.ls_atol = opts->linear_solver_atol;
+  o.ls_rtol = opts->linear_solver_rtol;
+  o.max_iter = opts->max_iter;
+  o.max_ir = opts->max_iterative_refinement_steps;
+  o.precision = opts->precision;
+  HIPC(hipEventRecord(h->ev0, h->stream));
+  HIPC(hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream));
+  const size_t smem_prep = sizeof(double) * (WG + 8);
+  hipLaunchKernelGGL(k_prepare, dim3(std::min(h->count, 2048)), dim3(WG), smem_prep, h->stream,
+                     h->dinst, h->count, opts->equilibrate ? 1 : 0, h->have_start ? 0 : 1,
+                     o.precision == QTQP_PREC_FP64 ? 1 : 0);
+  HIPC(hipGetLastError());
+  int dev = 0, cus = 0;
+  HIPC(hipGetDevice(&dev));
+  HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int grid = std::min(h->count, std::max(1, cus * 2));
+  // graph of one iteration: reset queue, persistent iteration kernel
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  if (opts->use_graph) {
+    HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+    hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
+                       h->dorder, h->count, h->dqueue, o, h->nmax, h->dcnt);
+    HIPC(hipStreamEndCapture(h->stream, &graph));
+    HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  int it = 0;
+  int check_every = 2;
+  while ((int)h->iter_ev.size() < o.max_iter + 1) {
+    hipEvent_t ev;
+    HIPC(hipEventCreate(&ev));
+    h->iter_ev.push_back(ev);
+  }
+  HIPC(hipEventRecord(h->iter_ev[0], h->stream));
+  for (it = 0; it < o.max_iter; ++it) {
+    if (exec) {
+      HIPC(hipGraphLaunch(exec, h->stream));
+    } else {
+      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+      hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
+                         h->dorder, h->count, h->dqueue, o, h->nmax, h->dcnt);
+      HIPC(hipGetLastError());
+    }
+    HIPC(hipEventRecord(h->iter_ev[it + 1], h->stream));
+    if ((it + 1) % check_every == 0 || it + 1 == o.max_iter) {
+      Counters hc{};
+      HIPC(hipMemcpyAsync(&hc, h->dcnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+      HIPC(hipStreamSynchronize(h->stream));
+      if (hc.done >= h->count) {
+        ++it;
+        break;
+      }
+    }
+  }
+  HIPC(hipEventRecord(h->ev1, h->stream));
+  HIPC(hipEventSynchronize(h->ev1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  const int launched = std::min(it, o.max_iter);
+  h->iter_ms.assign(launched, 0.0);
+  for (int k = 0; k < launched; ++k) {
+    float t = 0.f;
+    HIPC(hipEventElapsedTime(&t, h->iter_ev[k], h->iter_ev[k + 1]));
+    h->iter_ms[k] = t;
+  }
+  if (counters) {
+    Counters hc{};
+    HIPC(hipMemcpy(&hc, h->dcnt, sizeof(Counters), hipMemcpyDeviceToHost));
+    counters->backend_solves = (int64_t)hc.backend_solves;
+    counters->inner_steps = (int64_t)hc.inner_steps;
+    counters->fp64_fallbacks = (int64_t)hc.fallbacks;
+    counters->ir_not_converged = (int64_t)hc.ir_bad;
+    counters->iterations = std::min(it, o.max_iter);
+    counters->device_ms = ms;
+  }
+  h->have_start = false;
+  return QTQP_OK;
+}
+
+int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* tau,
+                       int32_t* status, int32_t* iters, double* pres, double* dres, double* gap) {
+  if (!h) return fail(QTQP_ERR_ARG, "NULL handle");
+  hipLaunchKernelGGL(k_gather, dim3(std::min(h->count, 2048)), dim3(WG), 0, h->stream, h->dinst,
+                     h->count, h->d_noff, h->d_moff, h->d_px, h->d_py, h->d_ps, h->d_scal,
+                     h->d_ints);
+  HIPC(hipGetLastError());
+  std::vector<double> sd((size_t)S_NSCAL * h->count);
+  std::vector<int> si((size_t)I_NINT * h->count);
+  if (x) HIPC(hipMemcpyAsync(x, h->d_px, 8 * h->tot_n, hipMemcpyDeviceToHost, h->stream));
+  if (y) HIPC(hipMemcpyAsync(y, h->d_py, 8 * h->tot_m, hipMemcpyDeviceToHost, h->stream));
+  if (s) HIPC(hipMemcpyAsync(s, h->d_ps, 8 * h->tot_m, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipMemcpyAsync(sd.data(), h->d_scal, sd.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipMemcpyAsync(si.data(), h->d_ints, si.size() * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  for (int q = 0; q < h->count; ++q) {
+    const double* d = sd.data() + (size_t)q * S_NSCAL;
+    const int* iv = si.data() + (size_t)q * I_NINT;
+    if (tau) tau[q] = d[S_TAU];
+    if (pres) pres[q] = d[S_PRES];
+    if (dres) dres[q] = d[S_DRES];
+    if (gap) gap[q] = d[S_GAP];
+    if (status) status[q] = iv[I_STATUS];
+    if (iters) iters[q] = iv[I_ITERS];
+  }
+  return QTQP_OK;
+}
+
+int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap) {
+  if (!h) return -1;
+  const int k = (int)h->iter_ms.size();
+  for (int i = 0; i < std::min(k, (int)cap); ++i) ms[i] = h->iter_ms[i];
+  return k;
+}
+
+int qtqp_batch_work(qtqp_batch* h, int32_t* counts) {
+  if (!h || !counts) return fail(QTQP_ERR_ARG, "NULL argument");
+  hipLaunchKernelGGL(k_gather, dim3(std::min(h->count, 2048)), dim3(WG), 0, h->stream, h->dinst,
+                     h->count, h->d_noff, h->d_moff, h->d_px, h->d_py, h->d_ps, h->d_scal,
+                     h->d_ints);
+  HIPC(hipGetLastError());
+  std::vector<int> si((size_t)I_NINT * h->count);
+  HIPC(hipMemcpyAsync(si.data(), h->d_ints, si.size() * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  const int slots[5] = {I_F32, I_F64, I_KMV, I_FS32, I_FS64};
+  for (int q = 0; q < h->count; ++q)
+    for (int k = 0; k < 5; ++k) counts[q * 5 + k] = si[(size_t)q * I_NINT + slots[k]];
+  return QTQP_OK;
+}
+
+void qtqp_batch_free(qtqp_batch* h) {
+  if (!h) return;
+  for (hipEvent_t ev : h->iter_ev) (void)hipEventDestroy(ev);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+}  // extern "C"

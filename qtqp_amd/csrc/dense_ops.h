@@ -21,21 +21,25 @@ constexpr int KC = 32;   // k-chunk staged through LDS by the rank-k tile
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA tiles.  Operands: A lane l holds A[l%16][l/16] and B lane l
+// holds B[l/16][l%16] for both forms.  Results differ: row(l, r) gives the
+// row of D held by register r of lane l (the column is always l%16).
 template <typename T> struct Mfma;
-// v_mfma_f64_16x16x4_f64: A lane l holds A[l%16][l/16], B lane l holds
-// B[l/16][l%16], D lane l register r holds D[4*(l/16)+r][l%16].
+// v_mfma_f64_16x16x4_f64: register r holds row (l/16) + 4r.
 template <> struct Mfma<double> {
   using acc_t = dbl4;
   __device__ static inline acc_t mac(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
-// v_mfma_f32_16x16x4_f32: same operand/result layout as the f64 form.
+// v_mfma_f32_16x16x4_f32: register r holds row 4(l/16) + r.
 template <> struct Mfma<float> {
   using acc_t = flt4;
   __device__ static inline acc_t mac(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 // ---------------------------------------------------------------------------
@@ -82,7 +86,7 @@ __device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
 // for the 64x64 tile with row block i0 and column block j0 (global indices,
 // rows >= ilim / cols >= jlim read as zero).  Wave v owns the 32x32 sub-tile
 // j in [j0+32*(v>>1), +32), i in [i0+32*(v&1), +32) as 2x2 MFMA 16x16 blocks:
-// acc[a][b][r] is element j = j0+32*(v>>1)+16a+4*(lane>>4)+r,
+// acc[a][b][r] is element j = j0+32*(v>>1)+16a+Mfma<T>::row(lane,r),
 // i = i0+32*(v&1)+16b+(lane&15), so stores into a column-major matrix
 // (index j*ld+i) are contiguous across lanes.
 // ---------------------------------------------------------------------------
@@ -149,7 +153,7 @@ struct RankKTile {
       for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          f(ib + 16 * b + (lane & 15), jb + 16 * a + 4 * (lane >> 4) + r, acc[a][b][r]);
+          f(ib + 16 * b + (lane & 15), jb + 16 * a + Mfma<T>::row(lane, r), acc[a][b][r]);
   }
 };
 
@@ -271,7 +275,7 @@ __device__ inline void trsm_mfma_rows(T* G, int64_t ld, int n, int k0, int nb,
     for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int c = 16 * ct + 4 * kq + r;
+        const int c = 16 * ct + Mfma<T>::row(lane, r);
         if (c < nb && row < n) G[(int64_t)(k0 + c) * ld + row] = acc[ct][r];
       }
   }

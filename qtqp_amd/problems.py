@@ -96,11 +96,14 @@ def condensed_mpc(sys_: MpcSystem, horizon: int, x0: np.ndarray, r_weight=0.1,
   for _ in range(t_h):
     pows.append(ad @ pows[-1])
   phi = np.concatenate([pows[t + 1] for t in range(t_h)], axis=0)
-  gam = np.zeros((t_h * nx, t_h * nu))
-  blocks = [pw @ bd for pw in pows[:t_h]]
-  for t in range(t_h):
-    for k in range(t + 1):
-      gam[t * nx:(t + 1) * nx, k * nu:(k + 1) * nu] = blocks[t - k]
+  # Block-Toeplitz Gamma: block (t, k) = A^(t-k) B for k <= t, filled one
+  # block diagonal at a time.
+  blocks = np.stack([pw @ bd for pw in pows[:t_h]])  # (T, nx, nu)
+  g4 = np.zeros((t_h, nx, t_h, nu))
+  for lag in range(t_h):
+    idx = np.arange(t_h - lag)
+    g4[idx + lag, :, idx, :] = blocks[lag]
+  gam = g4.reshape(t_h * nx, t_h * nu)
   n = t_h * nu
   p = gam.T @ gam + r_weight * np.eye(n)
   p = 0.5 * (p + p.T)

@@ -1,0 +1,100 @@
+"""GPU parity of the batched device IPM against the CPU oracle.
+
+Bar: same status as the FP64 oracle on identical inputs, x/y/s within 1e-6
+relative (problems with a unique solution; 1e-4 for the degenerate sparse
+reference families), IPM iteration counts within one of the oracle's.
+"""
+import numpy as np
+import pytest
+
+import qtqp_amd
+from qtqp_amd import batch, problems
+from oracle import qtqp_oracle as O
+import golden_cases
+
+pytestmark = pytest.mark.gpu
+TRAJ = golden_cases.load_trajectories()
+
+
+def _rel(u, v):
+  ok = ~np.isnan(v)
+  if not ok.any():
+    return 0.0
+  return float(np.max(np.abs(u[ok] - v[ok])) / max(1.0, np.max(np.abs(v[ok]))))
+
+
+def _compare(sol, ref, tol):
+  assert sol.status.value == ref["status"], (sol.status, ref["status"])
+  for k in "xys":
+    got, want = getattr(sol, k), ref[k]
+    assert np.array_equal(np.isnan(got), np.isnan(want)), k
+    assert _rel(got, want) < tol, (k, _rel(got, want))
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("equilibrate", [True, False])
+def test_batch_matches_oracle_on_fixture_problems(require_gpu, precision, equilibrate):
+  keys = sorted(k for k in TRAJ if TRAJ[k]["equilibrate"] == equilibrate)
+  inputs = [golden_cases.inputs(TRAJ[k]["spec"]) for k in keys]
+  bq = batch.BatchQP(inputs)
+  res = bq.solve(equilibrate=equilibrate, precision=precision)
+  bq.free()
+  for k, inp, sol in zip(keys, inputs, res.solutions):
+    ref = O.solve_qp(*inp, equilibrate=equilibrate)
+    unique = TRAJ[k]["spec"]["kind"] in ("readme", "dense", "mpc") or ref["status"] != "solved"
+    _compare(sol, ref, 1e-6 if unique else 1e-4)
+    assert abs(sol.iters - ref["iters"]) <= 1, (k, sol.iters, ref["iters"])
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_uneven_mpc_batch_matches_oracle(require_gpu, precision):
+  qps = problems.uneven_mpc_batch(48, base_seed=500, t_range=(4, 24), nx_range=(2, 10),
+                                  nu_range=(1, 4))
+  bq = batch.BatchQP(qps)
+  res = bq.solve(precision=precision)
+  for qp, sol in zip(qps, res.solutions):
+    ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+    _compare(sol, ref, 1e-6)
+    assert abs(sol.iters - ref["iters"]) <= 1
+  # re-solving the same resident batch is bit-for-bit deterministic
+  res2 = bq.solve(precision=precision, use_graph=False)
+  for s1, s2 in zip(res.solutions, res2.solutions):
+    np.testing.assert_array_equal(s1.x, s2.x)
+    assert s1.iters == s2.iters
+  bq.free()
+
+
+def test_large_instances_in_batch(require_gpu):
+  # multi-panel factor (n > 64), long rows (m > 1000) and uneven neighbours
+  qps = [problems.mpc_qp(40, 12, 4, seed=3), problems.random_dense_qp(150, 220, 30, seed=4),
+         problems.mpc_qp(6, 3, 1, seed=5)]
+  bq = batch.BatchQP(qps)
+  res = bq.solve(precision="mixed")
+  for qp, sol in zip(qps, res.solutions):
+    _compare(sol, O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p), 1e-6)
+  bq.free()
+
+
+def test_warm_start_from_solution_cuts_iterations(require_gpu):
+  qps = problems.uneven_mpc_batch(8, base_seed=900, t_range=(8, 16), nx_range=(3, 6),
+                                  nu_range=(1, 3))
+  bq = batch.BatchQP(qps)
+  cold = bq.solve()
+  # a central, strictly interior point near the optimum: (x*, y* + 1e-3, s* + 1e-3)
+  xs = [s.x for s in cold.solutions]
+  ys = [s.y + 1e-3 for s in cold.solutions]
+  ss = [s.s + 1e-3 for s in cold.solutions]
+  bq.set_start(xs, ys, ss, np.ones(len(qps)))
+  warm = bq.solve()
+  bq.free()
+  for c, w in zip(cold.solutions, warm.solutions):
+    assert w.status == c.status == qtqp_amd.SolutionStatus.SOLVED
+    assert _rel(w.x, c.x) < 1e-6
+  assert warm.iterations.mean() < cold.iterations.mean()
+
+
+def test_batch_input_validation(require_gpu):
+  with pytest.raises(ValueError, match="effective z == m"):
+    batch.BatchQP([(np.ones((2, 2)), np.ones(2), np.ones(2), 2, None)])
+  with pytest.raises(ValueError):
+    batch.BatchQP([(np.ones((2, 2)), np.ones(3), np.ones(2), 0, None)])

@@ -176,13 +176,20 @@ def test_condensed_mpc_matches_oracle(require_gpu, linear_solver):
     assert _rel(getattr(sol, k), ref[k]) < 1e-6, k
 
 
-def test_min_iterative_refinement_steps(require_gpu):
-  # tests/test_qtqp.py:1862-1873: one backend solve per KKT solve must suffice.
-  a, b, c, p = G.gen_feasible(150, 100, 10, 7)
-  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=10, p=sp.csc_matrix(p)).solve(
-      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE, max_iterative_refinement_steps=1,
-      verbose=False)
-  G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, 10)
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+@pytest.mark.parametrize("mnzs", [(30, 20, 5, 42), (150, 100, 10, 7)])
+def test_min_iterative_refinement_steps(require_gpu, linear_solver, mnzs):
+  # tests/test_qtqp.py:1862-1873: one backend solve per KKT solve.  The
+  # (150, 100, 10, 7) instance hits max_iter with the FP64 oracle too; the
+  # bar is the oracle's status.
+  m, n, z, seed = mnzs
+  a, b, c, p = G.gen_feasible(m, n, z, seed)
+  ref = O.solve_qp(a, b, c, z, p, max_iterative_refinement_steps=1)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+      linear_solver=linear_solver, max_iterative_refinement_steps=1, verbose=False)
+  assert sol.status.value == ref["status"]
+  if ref["status"] == "solved":
+    G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, z)
 
 
 def test_protocol_errors(require_gpu):
