@@ -92,6 +92,14 @@ int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* t
  * the launch stream).  Writes up to `cap` values; returns how many exist. */
 int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap);
 
+/* Per-iteration trace of QP q for the last solve: 16 doubles per IPM
+ * iteration (mu, sigma, alpha, tau, tau_predictor, tau_corrector,
+ * alpha_predictor, then solves and final residual of the q / predictor /
+ * corrector KKT solves (solves + 1000 when stalled), pres, dres, gap) --
+ * the scalars of the reference's per-iteration stats (__init__.py:967-990).
+ * Copies min(rows, 128) rows. */
+int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows);
+
 /* Executed work of the last solve per QP, 5 counts each: FP32 and FP64
  * factorisations, K_reg products, FP32 and FP64 factor solves. */
 int qtqp_batch_work(qtqp_batch* h, int32_t* counts);

@@ -50,6 +50,7 @@ BATCH_SYMBOLS = {
                                           _dp, _dp, _dp]),
     "qtqp_batch_launch_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "qtqp_batch_work": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
+    "qtqp_batch_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32]),
     "qtqp_batch_free": (None, [ctypes.c_void_p]),
     "qtqp_batch_last_error": (ctypes.c_char_p, []),
 }
@@ -252,6 +253,19 @@ class BatchQP:
     if k > 0:
       self._lib.qtqp_batch_launch_times(self._h, _ptr(out, _dp), k)
     return out
+
+  TRACE_FIELDS = ("mu", "sigma", "alpha", "tau", "tau_p", "tau_c", "alpha_p", "q_solves",
+                  "q_res", "p_solves", "p_res", "c_solves", "c_res", "pres", "dres", "gap",
+                  "dbg_p_ta", "dbg_p_tb", "dbg_p_tc", "dbg_p_krmax", "dbg_c_ta", "dbg_c_tb",
+                  "dbg_c_tc", "dbg_c_krmax", "snap_sd_tau", "snap_tau_norm", "snap_scale",
+                  "snap_tau_after_factor", "snap_tau_before_pred", "s5", "s6", "s7")
+
+  def trace(self, q: int, iters: int) -> list:
+    """Per-iteration scalars of QP q in the last solve (list of dicts)."""
+    rows = min(int(iters), 128)
+    out = np.zeros((max(rows, 1), len(self.TRACE_FIELDS)))
+    _check(self._lib, self._lib.qtqp_batch_trace(self._h, int(q), _ptr(out, _dp), rows))
+    return [dict(zip(self.TRACE_FIELDS, row)) for row in out[:rows]]
 
   def work_counts(self) -> np.ndarray:
     """(count, 5): FP32/FP64 factorisations, K products, FP32/FP64 factor solves."""

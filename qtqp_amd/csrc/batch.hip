@@ -39,6 +39,11 @@ constexpr int kMaxInner = 12;   // FP32 refinement steps inside one backend solv
 constexpr int kStatusError = 5; // factorisation breakdown in FP64 (LinAlgError)
 
 enum { S_TAU = 0, S_MU, S_SIGMA, S_ALPHA, S_PRES, S_DRES, S_GAP, S_NSCAL = 8 };
+// Per-iteration trace row: the scalars of the reference's per-iteration stats.
+enum { T_MU = 0, T_SIGMA, T_ALPHA, T_TAU, T_TAU_P, T_TAU_C, T_ALPHA_P, T_Q_SOLVES, T_Q_RES,
+       T_P_SOLVES, T_P_RES, T_C_SOLVES, T_C_RES, T_PRES, T_DRES, T_GAP, T_DBG0,
+       kTraceFields = 24 };
+constexpr int kTraceRows = 128;
 enum { I_STATUS = 0, I_ITERS, I_FORCE64, I_FALLBACK, I_F32, I_F64, I_KMV, I_FS32, I_FS64,
        I_NINT = 12 };
 
@@ -56,6 +61,7 @@ struct Inst {
   double *um, *un, *ds, *cm;
   double* sd;
   int* si;
+  double* trace;  // kTraceFields doubles per IPM iteration (max_iter rows)
 };
 
 struct Opts {
@@ -170,7 +176,7 @@ __device__ inline double dot(const double* a, const double* b, int len, double* 
 // Factorisation of G = P_off + diag(d_x) + A' W A (one workgroup).
 // -------------------------------------------------------------------------
 template <typename T>
-__device__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
+__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
   const int n = I.n, m = I.m;
   const int nt = (n + NB - 1) / NB;
   const double* dy = I.dg + n;
@@ -201,7 +207,7 @@ __device__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm)
 // Blocked right-looking LDL' (Cholesky form) in place; returns breakdown
 // column (1-based) or 0.
 template <typename T>
-__device__ int chol(T* G, int n, Smem& sm) {
+__device__ __noinline__ int chol(T* G, int n, Smem& sm) {
   __shared__ int info;
   if (threadIdx.x == 0) info = 0;
   __syncthreads();
@@ -244,7 +250,7 @@ __device__ int chol(T* G, int n, Smem& sm) {
 
 // Factor in the instance's current precision; FP32 breakdown falls back to
 // FP64.  Returns false on an FP64 breakdown.
-__device__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
+__device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
   if (!I.si[I_FORCE64]) {
     float* G = reinterpret_cast<float*>(I.G);
     if (threadIdx.x == 0) I.si[I_F32] += 1;
@@ -268,7 +274,7 @@ __device__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
 
 // out = b - K_reg v + corr*v   (b == nullptr: out = K_reg v); returns
 // ||out||_inf with NaN reported as inf.
-__device__ double kmv(const Inst& I, const double* v, const double* b, const double* corr,
+__device__ __noinline__ double kmv(const Inst& I, const double* v, const double* b, const double* corr,
                       double* out, Smem& sm) {
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
@@ -297,7 +303,7 @@ __device__ double kmv(const Inst& I, const double* v, const double* b, const dou
 // out (+)= K_reg^{-1} r with the factor in precision T:
 // g = r_x + A'(W r_y);  G x = g;  y = W (A x - r_y),  W = -1/d_y.
 template <typename T>
-__device__ void fsolve(const Inst& I, const T* A, const T* AT, const T* L, const double* r,
+__device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* AT, const T* L, const double* r,
                        double* out, bool accumulate, Smem& sm) {
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
@@ -322,7 +328,7 @@ __device__ void fsolve(const Inst& I, const T* A, const T* AT, const T* L, const
 
 // out += K_reg^{-1} r to FP64 accuracy.  FP32 factor: refine against the
 // exact K_reg; poor contraction switches the instance to an FP64 factor.
-__device__ bool backend_solve(const Inst& I, const double* r, double* out, Smem& sm,
+__device__ __noinline__ bool backend_solve(const Inst& I, const double* r, double* out, Smem& sm,
                               Counters* cnt) {
   const int N = I.n + I.m;
   if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
@@ -377,8 +383,8 @@ struct IrResult {
 };
 
 // DirectKktSolver.solve (direct.py:243-307): sol from warm, K_true = K_reg - diag(corr).
-__device__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm, double* sol,
-                             const Opts& o, Smem& sm, Counters* cnt) {
+__device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm, double* sol,
+                             const Opts o, Smem& sm, Counters* cnt) {
   const int n = I.n, N = I.n + I.m;
   double mx = 0.0;
   for (int k = threadIdx.x; k < N; k += WG) {
@@ -436,9 +442,12 @@ __device__ void pmul(const Inst& I, const double* u, double* out, Smem& sm) {
 }
 
 // Quadratic tau (__init__.py:759-817); returns false where it raises.
-__device__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double mu_t,
+__device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double mu_t,
                               double r_tau, double* tau_out, Smem& sm) {
   const int n = I.n, m = I.m;
+  double kqmax = 0.0;
+  for (int k = threadIdx.x; k < n + m; k += WG) kqmax = fmax(kqmax, fabs(kr[k]));
+  const double dot_dbg_kq = bmax(kqmax, sm.red);
   double t_a = mu + (dot(I.kq, I.c, n, sm.red) + dot(I.kq + n, I.b, m, sm.red));
   double t_b = -r_tau - (dot(kr, I.c, n, sm.red) + dot(kr + n, I.b, m, sm.red));
   double t_c = -mu_t;
@@ -448,6 +457,16 @@ __device__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double
     t_a -= dot(I.kq, I.um, n, sm.red);
     t_b += dot(kr, I.um, n, sm.red) + dot(I.kq, I.un, n, sm.red);
     t_c -= dot(kr, I.un, n, sm.red);
+  }
+  if (threadIdx.x == 0 && I.trace) {
+    const int row = I.si[I_ITERS];
+    if (row < kTraceRows) {
+      double* tr = I.trace + (int64_t)row * kTraceFields + T_DBG0 + (mu_t == 0.0 ? 0 : 4);
+      tr[0] = t_a;
+      tr[1] = t_b;
+      tr[2] = t_c;
+      tr[3] = dot_dbg_kq;
+    }
   }
   if (fabs(t_a) < kEps) {
     if (fabs(t_b) < kEps) return false;
@@ -467,7 +486,7 @@ __device__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double
 }
 
 // Linearised tau with trust region (__init__.py:819-857).
-__device__ double tau_linearized(const Inst& I, const double* kr, double mu, double mu_t,
+__device__ __noinline__ double tau_linearized(const Inst& I, const double* kr, double mu, double mu_t,
                                  double tau_c, double tau_anchor, Smem& sm) {
   const int n = I.n, m = I.m;
   double x_px = 0.0, px_kq = 0.0, px_kr = 0.0;
@@ -491,9 +510,10 @@ __device__ double tau_linearized(const Inst& I, const double* kr, double mu, dou
 }
 
 // Newton step (__init__.py:708-757): writes [x+; y+] into outxy, returns tau+.
-__device__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
+__device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
                        double tau_anchor, const double* corr_m, double tau, double* outxy,
-                       double* tau_plus, const Opts& o, Smem& sm, Counters* cnt) {
+                       double* tau_plus, const Opts o, Smem& sm, Counters* cnt,
+                       IrResult* irout) {
   const int n = I.n, z = I.z, N = I.n + I.m;
   for (int k = threadIdx.x; k < N; k += WG) {
     double v = (mu - mu_t) * anchor[k];
@@ -507,6 +527,7 @@ __device__ bool newton(const Inst& I, double mu, double mu_t, const double* anch
   }
   __syncthreads();
   const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt);
+  *irout = ir;
   if (!ir.ok) return false;
   double t = 0.0;
   bool have = false;
@@ -520,7 +541,7 @@ __device__ bool newton(const Inst& I, double mu, double mu_t, const double* anch
 }
 
 // One IPM iteration of one QP (__init__.py:473-578).
-__device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* cnt) {
+__device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm, Counters* cnt) {
   const int n = I.n, m = I.m, z = I.z, N = n + m;
   const int tid = threadIdx.x;
   double* red = sm.red;
@@ -541,6 +562,7 @@ __device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* 
   tau *= scale;
   __syncthreads();
   const double mu = dot(I.y, I.s, m, red) / (m - z);
+
 
   // regularised diagonal and correction (direct.py:196-216)
   for (int k = tid; k < N; k += WG) {
@@ -563,6 +585,7 @@ __device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* 
     return;
   }
 
+  IrResult irq{}, irp{}, irc{};
   // kinv_q = K^{-1} [c; b], warm-started from the previous kinv_q
   for (int k = tid; k < N; k += WG) I.xp[k] = I.kq[k];
   __syncthreads();
@@ -570,6 +593,7 @@ __device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* 
     for (int k = tid; k < N; k += WG) I.r[k] = k < n ? I.c[k] : I.b[k - n];
     __syncthreads();
     const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt);
+    irq = ir;
     if (!ir.ok) {
       if (tid == 0) I.si[I_STATUS] = kStatusError;
       return;
@@ -585,7 +609,7 @@ __device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* 
   for (int k = tid; k < N; k += WG) I.xp[k] = I.iacc[k];
   __syncthreads();
   double tau_p;
-  if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt)) {
+  if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp)) {
     if (tid == 0) I.si[I_STATUS] = kStatusError;
     return;
   }
@@ -617,7 +641,7 @@ __device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* 
     I.cm[i] = i < z ? 0.0 : -I.ds[i] * (I.xp[n + i] - I.y[i]) / I.y[i];
   __syncthreads();
   double tau_c;
-  if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt)) {
+  if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt, &irc)) {
     if (tid == 0) I.si[I_STATUS] = kStatusError;
     return;
   }
@@ -693,6 +717,26 @@ __device__ void ipm_iteration(const Inst& I, const Opts& o, Smem& sm, Counters* 
   else if (bty < -kEps && pinf < o.atol_inf + o.rtol_inf * norm_y / fabs(bty))
     status = QTQP_BATCH_INFEASIBLE;
   if (tid == 0) {
+    const int row = I.si[I_ITERS];
+    if (row < kTraceRows) {
+      double* tr = I.trace + (int64_t)row * kTraceFields;
+      tr[T_MU] = mu;
+      tr[T_SIGMA] = sigma;
+      tr[T_ALPHA] = alpha;
+      tr[T_TAU] = tau;
+      tr[T_TAU_P] = tau_p;
+      tr[T_TAU_C] = tau_c;
+      tr[T_ALPHA_P] = alpha_p;
+      tr[T_Q_SOLVES] = irq.solves + (irq.status == 2 ? 1000 : 0);
+      tr[T_Q_RES] = irq.res;
+      tr[T_P_SOLVES] = irp.solves + (irp.status == 2 ? 1000 : 0);
+      tr[T_P_RES] = irp.res;
+      tr[T_C_SOLVES] = irc.solves + (irc.status == 2 ? 1000 : 0);
+      tr[T_C_RES] = irc.res;
+      tr[T_PRES] = pres;
+      tr[T_DRES] = dres;
+      tr[T_GAP] = gap;
+    }
     I.sd[S_TAU] = tau;
     I.sd[S_MU] = mu;
     I.sd[S_SIGMA] = sigma;
@@ -924,6 +968,7 @@ size_t inst_bytes(int n, int m) {
   b += 10 * align256(8 * N);                       // kq dg corr krhs sol res iacc ires r xp
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
   b += align256(8 * S_NSCAL) + align256(4 * I_NINT);
+  b += align256(8 * kTraceFields * kTraceRows);
   return b + 4096;
 }
 
@@ -1032,6 +1077,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.cm = ar.take<double>(mx);
     I.sd = ar.take<double>(S_NSCAL);
     I.si = ar.take<int>(I_NINT);
+    I.trace = ar.take<double>(kTraceFields * kTraceRows);
     I.normb = I.normc = 0.0;
     if (hipMemcpyAsync(araw, a + a_off[q], 8 * mn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipMemcpyAsync(praw, p + p_off[q], 8 * nn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
@@ -1229,6 +1275,13 @@ int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap) {
   const int k = (int)h->iter_ms.size();
   for (int i = 0; i < std::min(k, (int)cap); ++i) ms[i] = h->iter_ms[i];
   return k;
+}
+
+int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows) {
+  if (!h || !out || q < 0 || q >= h->count || rows < 0) return fail(QTQP_ERR_ARG, "bad argument");
+  const int r = std::min<int>(rows, kTraceRows);
+  HIPC(hipMemcpy(out, h->hinst[q].trace, sizeof(double) * kTraceFields * r, hipMemcpyDeviceToHost));
+  return QTQP_OK;
 }
 
 int qtqp_batch_work(qtqp_batch* h, int32_t* counts) {

@@ -82,25 +82,33 @@ __global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n) {
   for (int i = threadIdx.x; i < n; i += WG) G[(int64_t)i * ld + i] = (T)((double)G[(int64_t)i * ld + i] + jit);
 }
 
-// Panel k0: workgroup 0 factors the diagonal block and stores it; workgroup
-// b >= 1 re-factors it in LDS, inverts it and solves rows
-// k0+NB+(b-1)*WG .. +WG of the panel on MFMA.
+// Panel k0, diagonal block: factors it in LDS, stores L11 into G and
+// L11^{-1} into linv (64 x 64, row-major) for the panel solve.  One workgroup.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_potrf_panel(T* G, int64_t ld, int n, int k0, int* info) {
+__global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int k0, int* info,
+                                                  T* linv) {
   __shared__ T S[NB][NB + 1];
   __shared__ T Li[NB][NB + 1];
   __shared__ T sinv[NB];
   const int nb = min(NB, n - k0);
   load_diag_block(G, ld, k0, nb, S);
   __syncthreads();
-  potrf_lds(S, sinv, nb, blockIdx.x == 0 ? info : nullptr, k0);
-  if (blockIdx.x == 0) {
-    store_diag_block(G, ld, k0, nb, S);
-    return;
-  }
+  potrf_lds(S, sinv, nb, info, k0);
+  store_diag_block(G, ld, k0, nb, S);
   tri_inverse_lds(S, sinv, Li);
   __syncthreads();
-  trsm_mfma_rows(G, ld, n, k0, nb, k0 + NB + (blockIdx.x - 1) * WG, Li);
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) linv[idx] = Li[idx >> 6][idx & (NB - 1)];
+}
+
+// Panel k0, rows below the diagonal block: L21 = A21 L11^{-T} on MFMA,
+// workgroup b handling rows k0+NB+b*WG .. +WG.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_trsm_panel(T* G, int64_t ld, int n, int k0,
+                                                  const T* linv) {
+  __shared__ T Li[NB][NB + 1];
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) Li[idx >> 6][idx & (NB - 1)] = linv[idx];
+  __syncthreads();
+  trsm_mfma_rows(G, ld, n, k0, min(NB, n - k0), k0 + NB + blockIdx.x * WG, Li);
 }
 
 // Trailing update G[t0:, t0:] -= L[t0:, k0:t0] L[t0:, k0:t0]', t0 = k0 + NB.
@@ -291,6 +299,7 @@ struct qtqp_kkt {
   double *rin = nullptr, *racc = nullptr, *rres = nullptr, *rtmp = nullptr;
   double *tx = nullptr, *ty = nullptr, *tg = nullptr, *tpx = nullptr, *part = nullptr;
   double* tv = nullptr;    // global fallback for the triangular-solve vector
+  double* linv = nullptr;  // inverse of the current 64x64 diagonal block
   size_t lds_limit = 0;    // dynamic LDS bytes granted to k_trsv
   unsigned long long* dnorm = nullptr;
   int* dflag = nullptr;
@@ -364,11 +373,13 @@ int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
   HIPC(hipGetLastError());
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int below = std::max(0, n - k0 - NB);
-    const int grid = 1 + (below + WG - 1) / WG;
-    hipLaunchKernelGGL(k_potrf_panel<T>, dim3(grid), dim3(WG), 0, h->stream, G, (int64_t)n, n, k0,
-                       h->info);
+    hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, k0,
+                       h->info, reinterpret_cast<T*>(h->linv));
     HIPC(hipGetLastError());
     if (below > 0) {
+      hipLaunchKernelGGL(k_trsm_panel<T>, dim3((below + WG - 1) / WG), dim3(WG), 0, h->stream, G,
+                         (int64_t)n, n, k0, reinterpret_cast<const T*>(h->linv));
+      HIPC(hipGetLastError());
       const int tt = (below + NB - 1) / NB;
       hipLaunchKernelGGL(k_syrk_trail<T>, dim3(tt * (tt + 1) / 2), dim3(WG), 0, h->stream, G,
                          (int64_t)n, n, k0);
@@ -479,7 +490,7 @@ void release(qtqp_kkt* h) {
   void* bufs[] = {h->A64, h->Poff64, h->A32, h->Poff32, h->diag, h->corr, h->G64, h->G32,
                   h->info, h->vin, h->vout, h->vsol, h->vres, h->rin, h->racc, h->rres,
                   h->rtmp, h->tx, h->ty, h->tg, h->tpx, h->part, h->dnorm, h->dflag,
-                  h->tv};
+                  h->tv, h->linv};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -573,6 +584,7 @@ int qtqp_kkt_create(int64_t n, int64_t m, const double* a, const double* p, int3
   ALLOC(tg, double, n);
   ALLOC(tpx, double, n);
   ALLOC(tv, double, n);
+  ALLOC(linv, double, NB * NB);
   ALLOC(part, double, (int64_t)nblk_rows(m) * n);
   ALLOC(dnorm, unsigned long long, 1);
   ALLOC(dflag, int, 1);
