@@ -39,7 +39,6 @@ def log(*a):
 
 def _cpu_worker(args):
   seed_lo, seed_hi = args
-  os.environ["OMP_NUM_THREADS"] = "1"
   from qtqp_amd import problems
   from oracle import qtqp_oracle
   qps = [problems.uneven_mpc_batch(1, base_seed=s)[0] for s in range(seed_lo, seed_hi)]
@@ -48,37 +47,54 @@ def _cpu_worker(args):
   return time.perf_counter() - t0, len(qps), st
 
 
+_BLAS_VARS = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+
+
 def cpu_baseline(base_seed: int, budget_s: float = 20.0):
-  """FP64 oracle (dense Gram/Cholesky IPM) over a pool of this host's cores."""
+  """FP64 oracle (dense Gram/Cholesky IPM) over a pool of this host's cores.
+
+  One process per core with BLAS threads = 1; each worker solves a disjoint
+  slice of the same uneven-MPC family.  QPs/s = QPs / slowest worker time.
+  """
   try:
     cores = len(os.sched_getaffinity(0))
   except AttributeError:
     cores = os.cpu_count() or 1
   cores = max(1, min(cores, 16))
-  # calibrate on one core, then size the sample to ~budget_s of pool time
-  dt, k, _ = _cpu_worker((base_seed, base_seed + 4))
-  per_qp = dt / k
-  n_total = int(max(cores, min(4096, budget_s * cores / max(per_qp, 1e-4))))
-  per_worker = max(1, n_total // cores)
-  jobs = [(base_seed + 100 + w * per_worker, base_seed + 100 + (w + 1) * per_worker)
-          for w in range(cores)]
-  env_threads = os.environ.get("OMP_NUM_THREADS")
-  os.environ["OMP_NUM_THREADS"] = "1"
-  ctx = mp.get_context("spawn")
-  t0 = time.perf_counter()
-  with ctx.Pool(cores) as pool:
-    res = pool.map(_cpu_worker, jobs)
-  wall = time.perf_counter() - t0
-  if env_threads is None:
-    os.environ.pop("OMP_NUM_THREADS", None)
-  else:
-    os.environ["OMP_NUM_THREADS"] = env_threads
+  saved = {k: os.environ.get(k) for k in _BLAS_VARS}
+  for k in _BLAS_VARS:
+    os.environ[k] = "1"
+  try:
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores) as pool:
+      # calibrate: one small slice per worker
+      t0 = time.perf_counter()
+      cal = pool.map(_cpu_worker, [(base_seed + 100 + 2 * w, base_seed + 102 + 2 * w)
+                                   for w in range(cores)])
+      per_qp = max(r[0] for r in cal) / 2
+      log(f"cpu baseline: {cores} workers, ~{per_qp * 1e3:.0f} ms/QP, "
+          f"calibrated in {time.perf_counter() - t0:.1f}s")
+      per_worker = int(max(1, min(256, budget_s / max(per_qp, 1e-4))))
+      jobs = [(base_seed + 1000 + w * per_worker, base_seed + 1000 + (w + 1) * per_worker)
+              for w in range(cores)]
+      res = []
+      t0 = time.perf_counter()
+      for r in pool.imap_unordered(_cpu_worker, jobs):
+        res.append(r)
+        log(f"cpu baseline: worker done ({len(res)}/{cores}) in {r[0]:.1f}s")
+      wall = time.perf_counter() - t0
+  finally:
+    for k, v in saved.items():
+      if v is None:
+        os.environ.pop(k, None)
+      else:
+        os.environ[k] = v
   solved = sum(r[1] for r in res)
   solve_time = max(r[0] for r in res)
   return {"value": solved / solve_time, "unit": "QPs/s", "cores": cores, "kind": "port",
-          "sample": f"{solved} QPs of the same uneven-MPC family (seeds {base_seed + 100}..), "
+          "sample": f"{solved} QPs of the same uneven-MPC family (seeds {base_seed + 1000}..), "
                     f"FP64 oracle IPM + dense Gram/Cholesky (scipy LAPACK), one process per core, "
-                    f"BLAS threads 1; wall {wall:.1f}s incl. pool start-up",
+                    f"BLAS threads 1; pool wall {wall:.1f}s",
           "cpu_model": _cpu_model()}
 
 
@@ -92,21 +108,23 @@ def _cpu_model():
   return "unknown"
 
 
-def algorithmic_work(n, m, counts):
+def algorithmic_work(n, m, counts, iters):
   """Algorithmic FLOPs and HBM bytes of the executed work of every QP.
 
-  Per QP: each factorisation costs m n^2 + n^3/3 flops (Gram product plus
-  LDL') and reads A once; each K_reg product reads A twice (A x and A' y
-  from the row- and column-major copies) and P once in FP64; each
-  factor solve reads A twice (FP32 or FP64) and the triangular factor twice
-  (n^2/2 each way).
+  Per QP, from the engine's own counters: each factorisation costs
+  m n^2 + n^3/3 flops (Gram product + LDL') and reads A once (FP32 or
+  FP64); each K_reg product (FP64 refinement residual) reads A and P once in
+  FP64; each factor solve reads A twice and the triangular factor twice
+  (n^2/2 entries each way) in the factor's precision; each IPM iteration's
+  termination test reads A and P once in FP64.
   """
   f32, f64, kmv, fs32, fs64 = (counts[:, k].astype(np.float64) for k in range(5))
+  it = iters.astype(np.float64)
   mn, nn = m.astype(np.float64) * n, n.astype(np.float64) ** 2
   fac_flops = m.astype(np.float64) * nn + nn * n / 3.0
   flops32 = float(np.sum(f32 * fac_flops))
   flops64 = float(np.sum(f64 * fac_flops))
-  bytes_ = np.sum(f32 * 4 * mn + f64 * 8 * mn + kmv * (16 * mn + 8 * nn)
+  bytes_ = np.sum(f32 * 4 * mn + f64 * 8 * mn + (kmv + it) * (8 * mn + 8 * nn)
                   + fs32 * (8 * mn + 4 * nn) + fs64 * (16 * mn + 8 * nn))
   return flops32, flops64, float(bytes_)
 
@@ -155,9 +173,10 @@ def main():
   statuses = [s.status.value for s in sols]
   iters = np.array([s.iters for s in sols])
   launch_ms = bq.launch_times_ms()
+  log("per-launch ms: " + " ".join(f"{t:.1f}" for t in launch_ms))
 
   # roofline of the dominant kernel (the per-iteration IPM kernel)
-  flops32, flops64, bytes_ = algorithmic_work(n, m, work)
+  flops32, flops64, bytes_ = algorithmic_work(n, m, work, iters)
   iter_ms = float(np.sum(launch_ms))
   t_mem = bytes_ / (HBM_PEAK_GBS * 1e9) * 1e3
   t_mfma = flops32 / (FP32_PEAK_TFLOPS * 1e12) * 1e3 + flops64 / (FP64_PEAK_TFLOPS * 1e12) * 1e3

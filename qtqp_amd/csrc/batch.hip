@@ -2,16 +2,16 @@
 //
 // Layout in HBM (one allocation per array kind, instances packed back to
 // back in input order, every per-instance array 256-byte aligned):
-//   raw A (m x n row-major, as uploaded), A_eq and A_eq' (row- and
-//   column-major copies of the equilibrated block, FP64), the same two in
-//   FP32 for the FP32 factor path, raw and equilibrated P (n x n) in FP64
+//   raw A (m x n row-major, as uploaded), the equilibrated A in FP64 and in
+//   FP32 (for the FP32 factor path), raw and equilibrated P (n x n) in FP64
 //   and FP32, vectors b, c, d, e, the Gram/factor workspace G (n x n FP64,
 //   whose front half holds the FP32 factor), iterates and scratch vectors.
-// Two copies of A let every pass over it be coalesced: A' u as a column sum
-// of row-major A, A x as a column sum of column-major A.
+// Every pass over A is one coalesced sweep of its rows that yields both A x
+// (row dots, wavefront reductions) and A' u (column partials per lane,
+// combined across the four wavefronts in LDS).
 //
-// Execution: k_prepare (one workgroup per QP: transpose, Ruiz scaling, FP32
-// copies, default start) once per solve, then per IPM iteration the pair
+// Execution: k_prepare (one workgroup per QP: Ruiz scaling, FP32 copies,
+// default start) once per solve, then per IPM iteration the pair
 // {k_reset_queue, k_ipm_iter} replayed from a HIP graph.  k_ipm_iter is a
 // persistent grid: each workgroup takes the next QP from an atomic queue
 // (QPs sorted by descending m*n^2, so large ones start first) and runs one
@@ -49,11 +49,12 @@ enum { I_STATUS = 0, I_ITERS, I_FORCE64, I_FALLBACK, I_F32, I_F64, I_KMV, I_FS32
 
 struct Inst {
   int n, m, z, has_p;
+  int ld, pad_;  // leading dimension of A, A32, P, P32: n rounded up to 4
   // raw data
   const double *Araw, *Praw, *braw, *craw;
   // equilibrated data
-  double *A, *AT, *P, *b, *c, *d, *e;
-  float *A32, *AT32, *P32;
+  double *A, *P, *b, *c, *d, *e;
+  float *A32, *P32;
   double normb, normc;
   // workspace
   double* G;
@@ -98,68 +99,169 @@ __device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
 // with a barrier so results are visible to the whole workgroup).
 // -------------------------------------------------------------------------
 
-// epi(j, sum_{i<nrow} M[i*ncol+j] * u[i]) for j < ncol; exactly one thread
-// calls epi per j.  Deterministic summation order.
-template <typename T, typename EF>
-__device__ void colsum(const T* __restrict__ M, int nrow, int ncol, const double* __restrict__ u,
-                       EF epi, double* red) {
-  const int tid = threadIdx.x;
-  if (ncol >= WG / 2) {
-    for (int j = tid; j < ncol; j += WG) {
-      double a0 = 0.0, a1 = 0.0;
-      int i = 0;
-      for (; i + 1 < nrow; i += 2) {
-        a0 += (double)M[(int64_t)i * ncol + j] * u[i];
-        a1 += (double)M[(int64_t)(i + 1) * ncol + j] * u[i + 1];
-      }
-      if (i < nrow) a0 += (double)M[(int64_t)i * ncol + j] * u[i];
-      epi(j, a0 + a1);
-    }
-    __syncthreads();
-    return;
+constexpr int RPW = 8;     // rows in flight per wavefront in a row pass
+constexpr int MAXCW = 16;  // 64-column slots per lane in the scalar max pass (n <= 1024)
+
+template <typename T, int W> struct VecT;
+template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+
+// Completes 8 per-lane partial sums across the wavefront with 10 exchanges
+// (a transpose-reduce: each step halves the rows a lane still carries).  On
+// return every lane holds in d[0] the total of row
+// 4*((l>>5)&1) + 2*((l>>4)&1) + ((l>>3)&1).
+__device__ inline void reduce8(double* d, int lane) {
+  const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double send = h32 ? d[q] : d[q + 4];
+    const double keep = h32 ? d[q + 4] : d[q];
+    d[q] = keep + __shfl_xor(send, 32, 64);
   }
-  const int nph = WG / ncol;
-  const int j = tid % ncol, ph = tid / ncol;
-  double acc = 0.0;
-  if (ph < nph)
-    for (int i = ph; i < nrow; i += nph) acc += (double)M[(int64_t)i * ncol + j] * u[i];
-  red[tid] = acc;
-  __syncthreads();
-  if (tid < ncol) {
-    double s = 0.0;
-    for (int p = 0; p < nph; ++p) s += red[p * ncol + tid];
-    epi(tid, s);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double send = h16 ? d[q] : d[q + 2];
+    const double keep = h16 ? d[q + 2] : d[q];
+    d[q] = keep + __shfl_xor(send, 16, 64);
+  }
+  {
+    const double send = h8 ? d[0] : d[1];
+    const double keep = h8 ? d[1] : d[0];
+    d[0] = keep + __shfl_xor(send, 8, 64);
+  }
+  d[0] += __shfl_xor(d[0], 4, 64);
+  d[0] += __shfl_xor(d[0], 2, 64);
+  d[0] += __shfl_xor(d[0], 1, 64);
+}
+
+// One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
+// a multiple of W with zero padding; ld <= 1024), reading M once with
+// W-wide vector loads:
+//   HX: rowepi(i, sum_j M[i][j] xs[j])   (one lane per row)
+//   HU: colepi(j, sum_i M[i][j] u[i])    (one thread per j)
+// Wavefront w takes RPW = 8 consecutive rows at a time (rows w*8 + 32k),
+// lane l the columns s*64W + lW .. +W; column partials of the four
+// wavefronts meet in `scratch` (>= 4*ncol doubles of LDS).  Deterministic.
+template <typename T, int W, bool HX, bool HU, typename RE, typename CE>
+__device__ inline void rowpass(const T* __restrict__ M, int nrow, int ncol, int ld,
+                               const double* __restrict__ xs, const double* __restrict__ u,
+                               RE rowepi, CE colepi, double* scratch) {
+  using V = typename VecT<T, W>::type;
+  constexpr int NS = 1024 / (64 * W);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nslot = (ld + 64 * W - 1) / (64 * W);
+  double xr[NS][W], ca[NS][W];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+      const int col = s * 64 * W + lane * W + e;
+      xr[s][e] = (HX && s < nslot && col < ncol) ? xs[col] : 0.0;
+      ca[s][e] = 0.0;
+    }
+  for (int i0 = wave * RPW; i0 < nrow; i0 += 4 * RPW) {
+    double d[RPW], uu[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      d[r] = 0.0;
+      uu[r] = (HU && i0 + r < nrow) ? u[i0 + r] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int col0 = s * 64 * W + lane * W;
+      if (s < nslot && col0 < ld) {
+        V a[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+          a[r] = (i0 + r < nrow) ? *reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0)
+                                 : V(0);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+          for (int e = 0; e < W; ++e) {
+            const double av = (double)a[r][e];
+            if (HX) d[r] += av * xr[s][e];
+            if (HU) ca[s][e] += av * uu[r];
+          }
+      }
+    }
+    if (HX) {
+      reduce8(d, lane);
+      const int row = i0 + 4 * ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 3) & 1);
+      if ((lane & 7) == 0 && row < nrow) rowepi(row, d[0]);
+    }
+  }
+  if (HU) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        const int col = s * 64 * W + lane * W + e;
+        if (s < nslot && col < ncol) scratch[wave * ncol + col] = ca[s][e];
+      }
+    __syncthreads();
+    for (int j = threadIdx.x; j < ncol; j += WG)
+      colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]));
   }
   __syncthreads();
 }
 
-// Column-wise max |M[i][j]| * wr[i] * wc[j] (wr/wc may be null = ones).
-template <typename EF>
-__device__ void colmax(const double* __restrict__ M, int nrow, int ncol,
-                       const double* __restrict__ wr, EF epi, double* red) {
-  const int tid = threadIdx.x;
-  const int nph = ncol >= WG / 2 ? 1 : WG / ncol;
-  for (int j0 = 0; j0 < ncol; j0 += (nph == 1 ? WG : ncol)) {
-    const int j = j0 + (nph == 1 ? tid : tid % ncol), ph = nph == 1 ? 0 : tid / ncol;
-    double mx = 0.0;
-    if (j < ncol && ph < nph)
-      for (int i = ph; i < nrow; i += nph) {
-        const double v = fabs(M[(int64_t)i * ncol + j]) * (wr ? wr[i] : 1.0);
-        mx = fmax(mx, v);
-      }
-    if (nph == 1) {
-      if (j < ncol) epi(j, mx);
-    } else {
-      red[tid] = mx;
-      __syncthreads();
-      if (tid < ncol) {
-        double s = 0.0;
-        for (int p = 0; p < nph; ++p) s = fmax(s, red[p * ncol + tid]);
-        epi(tid, s);
+template <typename T> struct VW;
+template <> struct VW<double> { static constexpr int value = 2; };
+template <> struct VW<float> { static constexpr int value = 4; };
+
+// Row maxima max_j |M[i][j]| wc[j] and column maxima max_i |M[i][j]| wr[i]
+// of row-major M in one pass (same work split as rowpass).
+template <typename RE, typename CE>
+__device__ inline void rowpass_absmax(const double* __restrict__ M, int nrow, int ncol,
+                                      const double* __restrict__ wc,
+                                      const double* __restrict__ wr, RE rowepi, CE colepi,
+                                      double* scratch) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ncw = (ncol + 63) >> 6;
+  double wcr[MAXCW], cm[MAXCW];
+#pragma unroll
+  for (int c = 0; c < MAXCW; ++c) {
+    const int j = lane + 64 * c;
+    wcr[c] = (c < ncw && j < ncol) ? wc[j] : 0.0;
+    cm[c] = 0.0;
+  }
+  for (int i0 = wave * RPW; i0 < nrow; i0 += 4 * RPW) {
+    double d[RPW], ww[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      d[r] = 0.0;
+      ww[r] = (i0 + r < nrow) ? wr[i0 + r] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < MAXCW; ++c) {
+      const int j = lane + 64 * c;
+      if (c < ncw && j < ncol) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+          if (i0 + r < nrow) {
+            const double a = fabs(M[(int64_t)(i0 + r) * ncol + j]);
+            d[r] = fmax(d[r], a * wcr[c]);
+            cm[c] = fmax(cm[c], a * ww[r]);
+          }
+        }
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const double mxr = wave_max(d[r]);
+      if (lane == 0 && i0 + r < nrow) rowepi(i0 + r, mxr);
+    }
   }
+#pragma unroll
+  for (int c = 0; c < MAXCW; ++c) {
+    const int j = lane + 64 * c;
+    if (c < ncw && j < ncol) scratch[wave * ncol + j] = cm[c];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < ncol; j += WG)
+    colepi(j, fmax(fmax(scratch[j], scratch[ncol + j]), fmax(scratch[2 * ncol + j], scratch[3 * ncol + j])));
+  __syncthreads();
 }
 
 __device__ inline double bsum(double v, double* red) { return block_sum(v, red + WG); }
@@ -177,9 +279,11 @@ __device__ inline double dot(const double* a, const double* b, int len, double* 
 // -------------------------------------------------------------------------
 template <typename T>
 __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
-  const int n = I.n, m = I.m;
+  const int n = I.n, m = I.m, ld = I.ld;
   const int nt = (n + NB - 1) / NB;
-  const double* dy = I.dg + n;
+  for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -1.0 / I.dg[n + i];
+  __syncthreads();
+  const double* w = I.um;
   T (*si)[NB + 1] = tile_ptr<T>(sm.u, 0);
   T (*sj)[NB + 1] = tile_ptr<T>(sm.u, KC);
   for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
@@ -188,10 +292,10 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
     const int i0 = ti * NB, j0 = tj * NB;
     RankKTile<T> acc;
     acc.zero();
-    acc.run(A, n, 0, m, i0, j0, n, n, [dy](int k) { return (T)(-1.0 / dy[k]); }, si, sj);
+    acc.run(A, ld, 0, m, i0, j0, n, n, [w](int k) { return (T)w[k]; }, si, sj);
     acc.for_each(i0, j0, [&](int i, int j, T v) {
       if (i < n && j < n && i >= j)
-        G[(int64_t)j * n + i] = v + ((i == j) ? (T)I.dg[i] : P[(int64_t)j * n + i]);
+        G[(int64_t)j * n + i] = v + ((i == j) ? (T)I.dg[i] : P[(int64_t)j * ld + i]);
     });
   }
   __syncthreads();
@@ -273,25 +377,30 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
 // -------------------------------------------------------------------------
 
 // out = b - K_reg v + corr*v   (b == nullptr: out = K_reg v); returns
-// ||out||_inf with NaN reported as inf.
-__device__ __noinline__ double kmv(const Inst& I, const double* v, const double* b, const double* corr,
-                      double* out, Smem& sm) {
+// ||out||_inf with NaN reported as inf.  One pass over A and one over P.
+__device__ __noinline__ double kmv(const Inst& I, const double* v, const double* b,
+                                   const double* corr, double* out, Smem& sm) {
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
+  double* scratch = reinterpret_cast<double*>(sm.u);
   auto fin = [&](int idx, double k) {
     if (!b) return k;
     double r = b[idx] - k;
     if (corr) r += corr[idx] * v[idx];
     return r;
   };
-  colsum<double>(I.A, m, n, v + n, [&](int j, double s) { out[j] = s; }, sm.red);
-  colsum<double>(I.P, n, n, v, [&](int j, double s) {
-    const double kx = out[j] + (s - I.P[(int64_t)j * n + j] * v[j]) + I.dg[j] * v[j];
-    out[j] = fin(j, kx);
-  }, sm.red);
-  colsum<double>(I.AT, n, m, v, [&](int i, double t) {
-    out[n + i] = fin(n + i, t + I.dg[n + i] * v[n + i]);
-  }, sm.red);
+  // y-block A v_x + d_y v_y and, from the same rows, A' v_y (into un)
+  rowpass<double, 2, true, true>(
+      I.A, m, n, I.ld, v, v + n,
+      [&](int i, double t) { out[n + i] = fin(n + i, t + I.dg[n + i] * v[n + i]); },
+      [&](int j, double s) { I.un[j] = s; }, scratch);
+  // x-block P_off v_x + d_x v_x + A' v_y  (P symmetric: row j of P = column j)
+  rowpass<double, 2, true, false>(
+      I.P, n, n, I.ld, v, nullptr,
+      [&](int j, double t) {
+        out[j] = fin(j, I.un[j] + (t - I.P[(int64_t)j * I.ld + j] * v[j]) + I.dg[j] * v[j]);
+      },
+      [](int, double) {}, scratch);
   double mx = 0.0;
   for (int k = threadIdx.x; k < n + m; k += WG) {
     const double a = fabs(out[k]);
@@ -300,80 +409,53 @@ __device__ __noinline__ double kmv(const Inst& I, const double* v, const double*
   return bmax(mx, sm.red);
 }
 
-// out (+)= K_reg^{-1} r with the factor in precision T:
+// out = K_reg^{-1} r with the factor in precision T:
 // g = r_x + A'(W r_y);  G x = g;  y = W (A x - r_y),  W = -1/d_y.
 template <typename T>
-__device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* AT, const T* L, const double* r,
-                       double* out, bool accumulate, Smem& sm) {
+__device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* L, const double* r,
+                                    double* out, Smem& sm) {
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
   for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -r[n + i] / I.dg[n + i];
   __syncthreads();
   T* v = reinterpret_cast<T*>(sm.v);
-  colsum<T>(A, m, n, I.um, [&](int j, double s) { v[j] = (T)(r[j] + s); }, sm.red);
+  double* scratch = reinterpret_cast<double*>(sm.u);
+  rowpass<T, VW<T>::value, false, true>(A, m, n, I.ld, nullptr, I.um, [](int, double) {},
+                                        [&](int j, double s) { v[j] = (T)(r[j] + s); }, scratch);
   T (*S)[NB + 1] = tile_ptr<T>(sm.u, 0);
   trsv_lower(L, n, n, v, S);
   trsv_upper(L, n, n, v, S);
   for (int j = threadIdx.x; j < n; j += WG) {
     const double xv = (double)v[j];
     I.un[j] = xv;
-    out[j] = accumulate ? out[j] + xv : xv;
+    out[j] = xv;
   }
   __syncthreads();
-  colsum<T>(AT, n, m, I.un, [&](int i, double t) {
-    const double yv = -(t - r[n + i]) / I.dg[n + i];
-    out[n + i] = accumulate ? out[n + i] + yv : yv;
-  }, sm.red);
+  rowpass<T, VW<T>::value, true, false>(
+      A, m, n, I.ld, I.un, nullptr,
+      [&](int i, double t) { out[n + i] = -(t - r[n + i]) / I.dg[n + i]; }, [](int, double) {},
+      scratch);
 }
 
-// out += K_reg^{-1} r to FP64 accuracy.  FP32 factor: refine against the
-// exact K_reg; poor contraction switches the instance to an FP64 factor.
-__device__ __noinline__ bool backend_solve(const Inst& I, const double* r, double* out, Smem& sm,
-                              Counters* cnt) {
-  const int N = I.n + I.m;
-  if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
-  if (I.si[I_FORCE64]) {
-    fsolve<double>(I, I.A, I.AT, I.G, r, out, true, sm);
-    return true;
+// out = K_reg^{-1} r with the instance's current factor.
+__device__ inline void fsolve_current(const Inst& I, const double* r, double* out, Smem& sm) {
+  if (I.si[I_FORCE64])
+    fsolve<double>(I, I.A, I.G, r, out, sm);
+  else
+    fsolve<float>(I, I.A32, reinterpret_cast<const float*>(I.G), r, out, sm);
+}
+
+// Replaces an FP32 factor that refines too slowly by an FP64 one.
+__device__ bool fallback_fp64(const Inst& I, Smem& sm, Counters* cnt) {
+  if (threadIdx.x == 0) {
+    I.si[I_FORCE64] = 1;
+    I.si[I_FALLBACK] += 1;
+    I.si[I_F64] += 1;
+    atomicAdd(&cnt->fallbacks, 1ull);
   }
-  double r0 = 0.0;
-  for (int k = threadIdx.x; k < N; k += WG) {
-    I.iacc[k] = 0.0;
-    I.ires[k] = r[k];
-    const double a = fabs(r[k]);
-    r0 = fmax(r0, (a == a) ? a : INFINITY);
-  }
-  r0 = bmax(r0, sm.red);
-  if (r0 == 0.0) return true;
-  double prev = r0;
-  for (int step = 0; step < kMaxInner; ++step) {
-    const bool f64 = I.si[I_FORCE64] != 0;
-    if (f64)
-      fsolve<double>(I, I.A, I.AT, I.G, I.ires, I.iacc, true, sm);
-    else
-      fsolve<float>(I, I.A32, I.AT32, reinterpret_cast<const float*>(I.G), I.ires, I.iacc, true,
-                    sm);
-    if (!f64 && threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
-    const double cur = kmv(I, I.iacc, r, nullptr, I.ires, sm);
-    if (!(cur == cur) || cur <= 1e-15 * r0) break;
-    if (!f64 && cur > 1e-2 * prev) {
-      if (threadIdx.x == 0) {
-        I.si[I_FORCE64] = 1;
-        I.si[I_FALLBACK] += 1;
-        atomicAdd(&cnt->fallbacks, 1ull);
-      }
-      if (threadIdx.x == 0) I.si[I_F64] += 1;
-      __syncthreads();
-      gram_syrk<double>(I, I.A, I.P, I.G, sm);
-      if (chol<double>(I.G, I.n, sm) != 0) return false;
-      continue;
-    }
-    if (cur > 0.5 * prev) break;
-    prev = cur;
-  }
-  for (int k = threadIdx.x; k < N; k += WG) out[k] += I.iacc[k];
   __syncthreads();
-  return true;
+  gram_syrk<double>(I, I.A, I.P, I.G, sm);
+  return chol<double>(I.G, I.n, sm) == 0;
 }
 
 struct IrResult {
@@ -382,9 +464,16 @@ struct IrResult {
   bool ok;
 };
 
-// DirectKktSolver.solve (direct.py:243-307): sol from warm, K_true = K_reg - diag(corr).
-__device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm, double* sol,
-                             const Opts o, Smem& sm, Counters* cnt) {
+// DirectKktSolver.solve (direct.py:243-307) on K_true = K_reg - diag(corr):
+// sol from warm, each step sol += K_reg^{-1} res with the current factor and
+// res recomputed in FP64, until res < atol + rtol ||rhs||, a stall, or
+// max_steps.  An FP32 step that contracts the residual by less than
+// kPoorContraction is undone and the factor is redone in FP64 (it does not
+// count as a step); FP64 steps follow the reference rule exactly.
+constexpr double kPoorContraction = 3e-2;
+
+__device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm,
+                                          double* sol, const Opts o, Smem& sm, Counters* cnt) {
   const int n = I.n, N = I.n + I.m;
   double mx = 0.0;
   for (int k = threadIdx.x; k < N; k += WG) {
@@ -396,16 +485,39 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   }
   const double rhs_norm = bmax(mx, sm.red);
   const double tol = o.ls_atol + o.ls_rtol * rhs_norm;
-  double res = kmv(I, sol, I.krhs, I.corr, I.res, sm);
+  double* rc = I.res;   // current residual
+  double* rn = I.ires;  // trial residual
+  double res = kmv(I, sol, I.krhs, I.corr, rc, sm);
   IrResult out{0, 1, res, true};
   int solves = 0;
-  for (solves = 1; solves <= o.max_ir; ++solves) {
+  while (solves < o.max_ir) {
     const double old = res;
-    if (!backend_solve(I, I.res, sol, sm, cnt)) {
-      out.ok = false;
-      return out;
+    const bool f32 = I.si[I_FORCE64] == 0;
+    if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
+    fsolve_current(I, rc, I.iacc, sm);
+    for (int k = threadIdx.x; k < N; k += WG) {
+      if (f32) I.r[k] = sol[k];  // backup: the caller's rhs was consumed into krhs
+      sol[k] += I.iacc[k];
     }
-    res = kmv(I, sol, I.krhs, I.corr, I.res, sm);
+    __syncthreads();
+    const double nr = kmv(I, sol, I.krhs, I.corr, rn, sm);
+    if (f32) {
+      if (threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
+      if (!(nr < tol) && !(nr <= kPoorContraction * old)) {
+        for (int k = threadIdx.x; k < N; k += WG) sol[k] = I.r[k];
+        __syncthreads();
+        if (!fallback_fp64(I, sm, cnt)) {
+          out.ok = false;
+          return out;
+        }
+        continue;
+      }
+    }
+    ++solves;
+    double* t = rc;
+    rc = rn;
+    rn = t;
+    res = nr;
     if (res < tol) {
       out.status = 0;
       break;
@@ -415,7 +527,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
       break;
     }
   }
-  out.solves = min(solves, o.max_ir);
+  out.solves = max(solves, 1);
   out.res = res;
   if (out.status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
   return out;
@@ -438,7 +550,9 @@ __device__ double max_step(const double* v, const double* dv_a, const double* dv
 
 // P u for the n-vector u into out (has_p only).
 __device__ void pmul(const Inst& I, const double* u, double* out, Smem& sm) {
-  colsum<double>(I.P, I.n, I.n, u, [&](int j, double s) { out[j] = s; }, sm.red);
+  rowpass<double, 2, true, false>(I.P, I.n, I.n, I.ld, u, nullptr,
+                                  [&](int j, double t) { out[j] = t; }, [](int, double) {},
+                                  reinterpret_cast<double*>(sm.u));
 }
 
 // Quadratic tau (__init__.py:759-817); returns false where it raises.
@@ -567,7 +681,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   // regularised diagonal and correction (direct.py:196-216)
   for (int k = tid; k < N; k += WG) {
     double t;
-    if (k < n) t = I.P[(int64_t)k * n + k] + mu;
+    if (k < n) t = I.P[(int64_t)k * I.ld + k] + mu;
     else if (k < n + z) t = mu;
     else t = I.s[k - n] / I.y[k - n] + mu;
     const double rg = fmax(t, o.min_reg);
@@ -603,10 +717,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   }
 
   // predictor (mu_target = 0) anchored at [x; y]
-  for (int k = tid; k < N; k += WG) I.iacc[k] = k < n ? I.x[k] : I.y[k - n];
-  __syncthreads();
-  // iacc is reused inside backend_solve; keep the anchor in `res`-free slot xp
-  for (int k = tid; k < N; k += WG) I.xp[k] = I.iacc[k];
+  for (int k = tid; k < N; k += WG) I.xp[k] = k < n ? I.x[k] : I.y[k - n];
   __syncthreads();
   double tau_p;
   if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp)) {
@@ -670,10 +781,15 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   const double inv_tau = 1.0 / fmax(tau, kEps);
   double* ax = I.um;  // m
   double* aty = I.un; // n
-  colsum<double>(I.AT, n, m, I.x, [&](int i, double t) { ax[i] = t / I.d[i]; }, red);
-  colsum<double>(I.A, m, n, I.y, [&](int j, double t) { aty[j] = t / I.e[j]; }, red);
+  rowpass<double, 2, true, true>(I.A, m, n, I.ld, I.x, I.y,
+                                 [&](int i, double t) { ax[i] = t / I.d[i]; },
+                                 [&](int j, double t) { aty[j] = t / I.e[j]; },
+                                 reinterpret_cast<double*>(sm.u));
   double* px = I.iacc;  // n
-  if (I.has_p) colsum<double>(I.P, n, n, I.x, [&](int j, double t) { px[j] = t; }, red);
+  if (I.has_p)
+    rowpass<double, 2, true, false>(I.P, n, n, I.ld, I.x, nullptr,
+                                    [&](int j, double t) { px[j] = t; }, [](int, double) {},
+                                    reinterpret_cast<double*>(sm.u));
   double l_xpx = 0.0, l_ctx = 0.0, l_px = 0.0, l_aty = 0.0, l_dres = 0.0, l_nx = 0.0;
   for (int j = tid; j < n; j += WG) {
     const double pxj = I.has_p ? px[j] : 0.0;
@@ -758,47 +874,43 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 // norms of the unscaled b/c, default start (__init__.py:299-307).
 __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equilibrate,
                                                 int default_start, int force64) {
+  // dynamic LDS: (WG + 8 + 4 * nmax) doubles
   extern __shared__ double smem[];
-  double* red = smem;  // WG + 8
+  double* red = smem;           // WG + 8
+  double* scratch = smem + WG + 8;  // 4 * nmax
   for (int q = blockIdx.x; q < count; q += gridDim.x) {
     const Inst& I = insts[q];
     const int n = I.n, m = I.m, tid = threadIdx.x;
-    // raw transpose into AT (as scratch for row norms), ones for d/e
-    for (int64_t k = tid; k < (int64_t)m * n; k += WG) {
-      const int i = (int)(k / n), j = (int)(k % n);
-      I.AT[(int64_t)j * m + i] = I.Araw[k];
-    }
     for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
     for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
     __syncthreads();
     if (equilibrate) {
       for (int pass = 0; pass < 10; ++pass) {
-        // row inf-norms of D A E -> um; column norms of A and P -> un
-        colmax(I.AT, n, m, I.e, [&](int i, double v) {
+        // of D A E: row inf-norms -> um (as 1/sqrt scales), column norms -> un
+        rowpass_absmax(I.Araw, m, n, I.e, I.d, [&](int i, double v) {
           const double r = v * I.d[i];
           I.um[i] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
-        }, red);
-        colmax(I.Araw, m, n, I.d, [&](int j, double v) { I.un[j] = v * I.e[j]; }, red);
-        colmax(I.Praw, n, n, I.e, [&](int j, double v) {
+        }, [&](int j, double v) { I.un[j] = v * I.e[j]; }, scratch);
+        // and of E P E (P symmetric): column norms, combined with A's
+        rowpass_absmax(I.Praw, n, n, I.e, I.e, [](int, double) {}, [&](int j, double v) {
           const double cn = fmax(I.un[j], v * I.e[j]);
           I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
-        }, red);
+        }, scratch);
         for (int i = tid; i < m; i += WG) I.d[i] *= I.um[i];
         for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
         __syncthreads();
       }
     }
-    for (int64_t k = tid; k < (int64_t)m * n; k += WG) {
-      const int i = (int)(k / n), j = (int)(k % n);
-      const double v = I.Araw[k] * (I.d[i] * I.e[j]);
+    const int ld = I.ld;
+    for (int64_t k = tid; k < (int64_t)m * ld; k += WG) {
+      const int i = (int)(k / ld), j = (int)(k % ld);
+      const double v = j < n ? I.Araw[(int64_t)i * n + j] * (I.d[i] * I.e[j]) : 0.0;
       I.A[k] = v;
       I.A32[k] = (float)v;
-      I.AT[(int64_t)j * m + i] = v;
-      I.AT32[(int64_t)j * m + i] = (float)v;
     }
-    for (int64_t k = tid; k < (int64_t)n * n; k += WG) {
-      const int i = (int)(k / n), j = (int)(k % n);
-      const double v = I.Praw[k] * (I.e[i] * I.e[j]);
+    for (int64_t k = tid; k < (int64_t)n * ld; k += WG) {
+      const int i = (int)(k / ld), j = (int)(k % ld);
+      const double v = j < n ? I.Praw[(int64_t)i * n + j] * (I.e[i] * I.e[j]) : 0.0;
       I.P[k] = v;
       I.P32[k] = (float)v;
     }
@@ -878,7 +990,7 @@ __global__ void k_reset_queue(int* queue) {
   if (threadIdx.x == 0) *queue = 0;
 }
 
-__global__ __launch_bounds__(WG) void k_ipm_iter(const Inst* insts, const int* order, int count,
+__global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int* order, int count,
                                                  int* queue, Opts o, int nmax, Counters* cnt) {
   extern __shared__ double smem[];
   Smem sm = carve(smem, nmax);
@@ -955,10 +1067,11 @@ struct Arena {
 };
 
 size_t inst_bytes(int n, int m) {
-  const int64_t N = n + m, mn = (int64_t)m * n, nn = (int64_t)n * n;
+  const int64_t N = n + m, ld = (n + 3) & ~3;
+  const int64_t mn = (int64_t)m * ld, nn = (int64_t)n * ld;
   size_t b = 0;
-  b += 3 * align256(8 * mn);       // A, AT (FP64) + raw A
-  b += 2 * align256(4 * mn);       // A32, AT32
+  b += 2 * align256(8 * mn);       // raw A, A (FP64)
+  b += align256(4 * mn);           // A32
   b += 2 * align256(8 * nn);       // raw P, P
   b += align256(4 * nn);           // P32
   b += align256(8 * nn);           // G
@@ -1037,18 +1150,19 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.n = nq;
     I.m = mq;
     I.z = z[q];
+    I.ld = (nq + 3) & ~3;
+    I.pad_ = 0;
+    const int64_t mld = (int64_t)mq * I.ld, nld = (int64_t)nq * I.ld;
     I.has_p = 0;
     for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
     double* araw = ar.take<double>(mn);
     I.Araw = araw;
-    I.A = ar.take<double>(mn);
-    I.AT = ar.take<double>(mn);
-    I.A32 = ar.take<float>(mn);
-    I.AT32 = ar.take<float>(mn);
+    I.A = ar.take<double>(mld);
+    I.A32 = ar.take<float>(mld);
     double* praw = ar.take<double>(nn);
     I.Praw = praw;
-    I.P = ar.take<double>(nn);
-    I.P32 = ar.take<float>(nn);
+    I.P = ar.take<double>(nld);
+    I.P32 = ar.take<float>(nld);
     I.G = ar.take<double>(nn);
     double* braw = ar.take<double>(mq);
     double* craw = ar.take<double>(nq);
@@ -1167,7 +1281,7 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   o.precision = opts->precision;
   HIPC(hipEventRecord(h->ev0, h->stream));
   HIPC(hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream));
-  const size_t smem_prep = sizeof(double) * (WG + 8);
+  const size_t smem_prep = sizeof(double) * (WG + 8 + 4 * h->nmax);
   hipLaunchKernelGGL(k_prepare, dim3(std::min(h->count, 2048)), dim3(WG), smem_prep, h->stream,
                      h->dinst, h->count, opts->equilibrate ? 1 : 0, h->have_start ? 0 : 1,
                      o.precision == QTQP_PREC_FP64 ? 1 : 0);

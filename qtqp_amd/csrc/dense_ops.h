@@ -104,7 +104,8 @@ struct RankKTile {
         for (int r = 0; r < 4; ++r) acc[a][b][r] = T(0);
   }
 
-  // si/sj: LDS staging, KC rows of (NB+1) elements each.
+  // si/sj: LDS staging, KC rows of (NB+1) elements each.  The next k-chunk
+  // is fetched into registers while the MFMAs consume the current one.
   template <typename WFun>
   __device__ inline void run(const T* __restrict__ X, int64_t ldx, int kbeg,
                              int kend, int i0, int j0, int ilim, int jlim,
@@ -112,22 +113,30 @@ struct RankKTile {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = tid & 63, kr = tid >> 6;
     const int jb = 32 * (wave >> 1), ib = 32 * (wave & 1);
-    for (int kc = kbeg; kc < kend; kc += KC) {
+    T pi[KC / 4], pj[KC / 4];
+    auto fetch = [&](int kc) {
 #pragma unroll
       for (int q = 0; q < KC / 4; ++q) {
-        const int kk = kr + 4 * q;
-        const int k = kc + kk;
+        const int k = kc + kr + 4 * q;
         T vi = T(0), vj = T(0);
         if (k < kend) {
           const T* row = X + (int64_t)k * ldx;
-          const T wk = wfun(k);
           if (i0 + c < ilim) vi = row[i0 + c];
-          if (j0 + c < jlim) vj = row[j0 + c] * wk;
+          if (j0 + c < jlim) vj = row[j0 + c] * wfun(k);
         }
-        si[kk][c] = vi;
-        sj[kk][c] = vj;
+        pi[q] = vi;
+        pj[q] = vj;
+      }
+    };
+    if (kbeg < kend) fetch(kbeg);
+    for (int kc = kbeg; kc < kend; kc += KC) {
+#pragma unroll
+      for (int q = 0; q < KC / 4; ++q) {
+        si[kr + 4 * q][c] = pi[q];
+        sj[kr + 4 * q][c] = pj[q];
       }
       __syncthreads();
+      if (kc + KC < kend) fetch(kc + KC);
 #pragma unroll
       for (int kk = 0; kk < KC; kk += 4) {
         const int kq = kk + (lane >> 4), cl = lane & 15;
