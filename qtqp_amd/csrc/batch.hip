@@ -58,6 +58,7 @@ struct Inst {
   double normb, normc;
   // workspace
   double* G;
+  double* LiB;  // per 64-block of the factor: L_bb^{-1} column-major then row-major
   double *x, *y, *s, *kq, *dg, *corr, *krhs, *sol, *res, *iacc, *ires, *r, *xp;
   double *um, *un, *ds, *cm;
   double* sd;
@@ -311,7 +312,7 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
 // Blocked right-looking LDL' (Cholesky form) in place; returns breakdown
 // column (1-based) or 0.
 template <typename T>
-__device__ __noinline__ int chol(T* G, int n, Smem& sm) {
+__device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm) {
   __shared__ int info;
   if (threadIdx.x == 0) info = 0;
   __syncthreads();
@@ -324,9 +325,15 @@ __device__ __noinline__ int chol(T* G, int n, Smem& sm) {
     __syncthreads();
     potrf_lds(S, sinv, nb, &info, k0);
     store_diag_block(G, n, k0, nb, S);
+    tri_inverse_lds(S, sinv, Li);
+    __syncthreads();
+    // keep L_bb^{-1} for the triangular solves: column-major, then row-major
+    T* dst = LiB + (int64_t)(k0 / NB) * 2 * NB * NB;
+    for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
+      dst[idx] = Li[idx & (NB - 1)][idx >> 6];
+      dst[NB * NB + idx] = Li[idx >> 6][idx & (NB - 1)];
+    }
     if (k0 + NB < n) {
-      tri_inverse_lds(S, sinv, Li);
-      __syncthreads();
       for (int rb = k0 + NB; rb < n; rb += WG) trsm_mfma_rows(G, n, n, k0, nb, rb, Li);
       __syncthreads();
       const int t0 = k0 + NB;
@@ -352,6 +359,60 @@ __device__ __noinline__ int chol(T* G, int n, Smem& sm) {
   return r;
 }
 
+// L v = v in place with stored diagonal-block inverses: per 64-block a
+// parallel 64x64 product with L_bb^{-1}, then the rows below are updated.
+// red: >= 4*64 doubles of LDS.
+template <typename T>
+__device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int nb = min(NB, n - j0);
+    const T* LiT = LiB + (int64_t)(j0 / NB) * 2 * NB * NB;  // LiT[k*64 + i] = Li[i][k]
+    double acc = 0.0;
+    if (lane < nb)
+      for (int k = p; k < nb; k += 4) acc += (double)LiT[k * NB + lane] * (double)v[j0 + k];
+    red[p * NB + lane] = acc;
+    __syncthreads();
+    if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
+    __syncthreads();
+    for (int i = j0 + nb + tid; i < n; i += WG) {
+      const T* col = L + (int64_t)j0 * n + i;
+      double a = 0.0;
+      for (int c = 0; c < nb; ++c) a += (double)col[(int64_t)c * n] * (double)v[j0 + c];
+      v[i] = (T)((double)v[i] - a);
+    }
+    __syncthreads();
+  }
+}
+
+// L' v = v in place with stored diagonal-block inverses.  rb: 64 doubles.
+template <typename T>
+__device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* red, double* rb) {
+  const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;
+  const int nblk = (n + NB - 1) / NB;
+  for (int b = nblk - 1; b >= 0; --b) {
+    const int j0 = b * NB, nb = min(NB, n - j0);
+    const T* Li = LiB + (int64_t)b * 2 * NB * NB + NB * NB;  // Li[k*64 + c] (row-major)
+    double acc = 0.0;
+    if (lane < nb) {
+      const T* col = L + (int64_t)(j0 + lane) * n;
+      for (int i = j0 + nb + p; i < n; i += 4) acc += (double)col[i] * (double)v[i];
+    }
+    red[p * NB + lane] = acc;
+    __syncthreads();
+    if (tid < nb)
+      rb[tid] = (double)v[j0 + tid] - ((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
+    __syncthreads();
+    acc = 0.0;
+    if (lane < nb)
+      for (int k = p; k < nb; k += 4) acc += (double)Li[k * NB + lane] * rb[k];
+    red[p * NB + lane] = acc;
+    __syncthreads();
+    if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
+    __syncthreads();
+  }
+}
+
 // Factor in the instance's current precision; FP32 breakdown falls back to
 // FP64.  Returns false on an FP64 breakdown.
 __device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
@@ -359,7 +420,7 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
     float* G = reinterpret_cast<float*>(I.G);
     if (threadIdx.x == 0) I.si[I_F32] += 1;
     gram_syrk<float>(I, I.A32, I.P32, G, sm);
-    if (chol<float>(G, I.n, sm) == 0) return true;
+    if (chol<float>(G, I.n, reinterpret_cast<float*>(I.LiB), sm) == 0) return true;
     if (threadIdx.x == 0) {
       I.si[I_FORCE64] = 1;
       I.si[I_FALLBACK] += 1;
@@ -369,7 +430,7 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
   }
   if (threadIdx.x == 0) I.si[I_F64] += 1;
   gram_syrk<double>(I, I.A, I.P, I.G, sm);
-  return chol<double>(I.G, I.n, sm) == 0;
+  return chol<double>(I.G, I.n, I.LiB, sm) == 0;
 }
 
 // -------------------------------------------------------------------------
@@ -412,8 +473,8 @@ __device__ __noinline__ double kmv(const Inst& I, const double* v, const double*
 // out = K_reg^{-1} r with the factor in precision T:
 // g = r_x + A'(W r_y);  G x = g;  y = W (A x - r_y),  W = -1/d_y.
 template <typename T>
-__device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* L, const double* r,
-                                    double* out, Smem& sm) {
+__device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* L, const T* LiB,
+                                    const double* r, double* out, Smem& sm) {
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
   for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -r[n + i] / I.dg[n + i];
@@ -422,9 +483,8 @@ __device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* L, const
   double* scratch = reinterpret_cast<double*>(sm.u);
   rowpass<T, VW<T>::value, false, true>(A, m, n, I.ld, nullptr, I.um, [](int, double) {},
                                         [&](int j, double s) { v[j] = (T)(r[j] + s); }, scratch);
-  T (*S)[NB + 1] = tile_ptr<T>(sm.u, 0);
-  trsv_lower(L, n, n, v, S);
-  trsv_upper(L, n, n, v, S);
+  trsv_lower_inv(L, n, LiB, v, sm.red);
+  trsv_upper_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
   for (int j = threadIdx.x; j < n; j += WG) {
     const double xv = (double)v[j];
     I.un[j] = xv;
@@ -440,9 +500,10 @@ __device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* L, const
 // out = K_reg^{-1} r with the instance's current factor.
 __device__ inline void fsolve_current(const Inst& I, const double* r, double* out, Smem& sm) {
   if (I.si[I_FORCE64])
-    fsolve<double>(I, I.A, I.G, r, out, sm);
+    fsolve<double>(I, I.A, I.G, I.LiB, r, out, sm);
   else
-    fsolve<float>(I, I.A32, reinterpret_cast<const float*>(I.G), r, out, sm);
+    fsolve<float>(I, I.A32, reinterpret_cast<const float*>(I.G),
+                  reinterpret_cast<const float*>(I.LiB), r, out, sm);
 }
 
 // Replaces an FP32 factor that refines too slowly by an FP64 one.
@@ -455,7 +516,7 @@ __device__ bool fallback_fp64(const Inst& I, Smem& sm, Counters* cnt) {
   }
   __syncthreads();
   gram_syrk<double>(I, I.A, I.P, I.G, sm);
-  return chol<double>(I.G, I.n, sm) == 0;
+  return chol<double>(I.G, I.n, I.LiB, sm) == 0;
 }
 
 struct IrResult {
@@ -1075,6 +1136,7 @@ size_t inst_bytes(int n, int m) {
   b += 2 * align256(8 * nn);       // raw P, P
   b += align256(4 * nn);           // P32
   b += align256(8 * nn);           // G
+  b += align256(8 * 2 * 64 * 64 * ((n + 63) / 64));  // diagonal-block inverses
   b += 2 * align256(8 * m) + 2 * align256(8 * n);  // raw b, c ; (b, c below)
   b += 2 * align256(8 * m) + 2 * align256(8 * n);  // b, c, d, e
   b += 3 * align256(8 * std::max<int64_t>(m, n));  // x/y/s slack (x n, y m, s m)
@@ -1164,6 +1226,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.P = ar.take<double>(nld);
     I.P32 = ar.take<float>(nld);
     I.G = ar.take<double>(nn);
+    I.LiB = ar.take<double>((int64_t)2 * NB * NB * ((nq + NB - 1) / NB));
     double* braw = ar.take<double>(mq);
     double* craw = ar.take<double>(nq);
     I.braw = braw;
