@@ -193,6 +193,20 @@ class BatchQP:
     self._max_iter = max_iter
     return {k: getattr(cnt, k) for k, _ in cnt._fields_}
 
+  def raw_iterates(self) -> dict:
+    """Final iterates of the last solve in unscaled coordinates, not divided by
+    tau (x = e x_eq, y = d y_eq, s = s_eq / d), with tau, status code and
+    iteration count per QP; x/y/s packed by n_off / m_off."""
+    tn, tm = int(self.n.sum()), int(self.m.sum())
+    x, y, s = np.empty(tn), np.empty(tm), np.empty(tm)
+    tau = np.empty(self.count)
+    st = np.empty(self.count, dtype=np.int32)
+    its = np.empty(self.count, dtype=np.int32)
+    _check(self._lib, self._lib.qtqp_batch_results(
+        self._h, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(tau, _dp), _ptr(st, _i32p),
+        _ptr(its, _i32p), None, None, None))
+    return {"x": x, "y": y, "s": s, "tau": tau, "status": st, "iters": its}
+
   def results(self) -> list:
     tn, tm = int(self.n.sum()), int(self.m.sum())
     x, y, s = np.empty(tn), np.empty(tm), np.empty(tm)

@@ -1,0 +1,265 @@
+"""Learned warm start for the batched IPM on a parametric MPC family.
+
+The family fixes (A_d, B_d, Q, R, boxes, horizon) and varies the initial state
+x0, so every QP shares A and P and only b, c depend on x0.  The network maps x0
+to a starting point of the solver: its own normalised iterate (x, y, s, tau)
+after ``k_star`` IPM iterations of a default-start solve, plus kappa = mu/tau.
+It predicts x, log y, log s, log tau, log kappa, so a proposal is strictly
+interior by construction.
+
+Fallback rule (default start of qtqp __init__.py:299-307): a QP keeps the
+default start when its proposal is not finite; a warm-started QP still
+unfinished after ``cap`` iterations is re-solved from the default start and
+charged both runs' iterations.
+
+Training and inference run on the GPU through include/qtqp_mlp.h (BF16
+MFMA products with FP32 accumulation, Adam).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import time
+
+import numpy as np
+
+from . import _native, batch, problems
+from . import SolutionStatus
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+MLP_SYMBOLS = {
+    "qtqp_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_mlp_num_params": (ctypes.c_int64, [ctypes.c_void_p]),
+    "qtqp_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "qtqp_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "qtqp_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, _fp]),
+    "qtqp_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int32, _fp, _dp]),
+    "qtqp_mlp_train": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _fp]),
+    "qtqp_mlp_free": (None, [ctypes.c_void_p]),
+    "qtqp_mlp_last_error": (ctypes.c_char_p, []),
+}
+
+
+def mlp_lib() -> ctypes.CDLL:
+  lib = _native.load("libqtqp_mlp.so", MLP_SYMBOLS)
+  _native.require_device(_native.kkt_lib())
+  return lib
+
+
+def _f32(a):
+  return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class MLP:
+  """n_in -> hidden -> hidden -> n_out perceptron with GELU, on the GPU."""
+
+  def __init__(self, n_in: int, hidden: int, n_out: int, max_batch: int = 512, seed: int = 0):
+    self._lib = mlp_lib()
+    self.n_in, self.hidden, self.n_out, self.max_batch = n_in, hidden, n_out, max_batch
+    h = ctypes.c_void_p()
+    self._check(self._lib.qtqp_mlp_create(n_in, hidden, n_out, max_batch, ctypes.byref(h)))
+    self._h = h
+    self.num_params = int(self._lib.qtqp_mlp_num_params(h))
+    self.set_params(init_params(n_in, hidden, n_out, seed))
+
+  def _check(self, rc):
+    if rc != 0:
+      msg = (self._lib.qtqp_mlp_last_error() or b"").decode(errors="replace")
+      raise (ValueError if rc == 1 else RuntimeError)(f"qtqp_mlp error {rc}: {msg}")
+
+  def set_params(self, p):
+    p = _f32(p)
+    assert p.size == self.num_params
+    self._check(self._lib.qtqp_mlp_set_params(self._h, p.ctypes.data_as(_fp)))
+
+  def get_params(self) -> np.ndarray:
+    p = np.empty(self.num_params, dtype=np.float32)
+    self._check(self._lib.qtqp_mlp_get_params(self._h, p.ctypes.data_as(_fp)))
+    return p
+
+  def forward(self, x) -> np.ndarray:
+    x = _f32(x)
+    y = np.empty((x.shape[0], self.n_out), dtype=np.float32)
+    self._check(self._lib.qtqp_mlp_forward(self._h, x.ctypes.data_as(_fp), x.shape[0],
+                                           y.ctypes.data_as(_fp)))
+    return y
+
+  def loss_grad(self, x, t):
+    x, t = _f32(x), _f32(t)
+    g = np.empty(self.num_params, dtype=np.float32)
+    loss = ctypes.c_double()
+    self._check(self._lib.qtqp_mlp_loss_grad(self._h, x.ctypes.data_as(_fp),
+                                             t.ctypes.data_as(_fp), x.shape[0],
+                                             g.ctypes.data_as(_fp), ctypes.byref(loss)))
+    return float(loss.value), g
+
+  def train(self, x, t, *, batch=512, steps=2000, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+            seed=0) -> np.ndarray:
+    x, t = _f32(x), _f32(t)
+    losses = np.empty(steps, dtype=np.float32)
+    self._check(self._lib.qtqp_mlp_train(self._h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp),
+                                         x.shape[0], batch, steps, lr, beta1, beta2, eps, seed,
+                                         losses.ctypes.data_as(_fp)))
+    return losses
+
+  def free(self):
+    if getattr(self, "_h", None) is not None:
+      self._lib.qtqp_mlp_free(self._h)
+      self._h = None
+
+  def __del__(self):
+    try:
+      self.free()
+    except Exception:  # pylint: disable=broad-except
+      pass
+
+
+def init_params(n_in, hidden, n_out, seed=0) -> np.ndarray:
+  """He-normal weights, zero biases, packed as W1, b1, W2, b2, W3, b3."""
+  rng = np.random.default_rng(seed)
+  parts = []
+  for fan_in, fan_out in ((n_in, hidden), (hidden, hidden), (hidden, n_out)):
+    parts.append(rng.standard_normal(fan_in * fan_out) * np.sqrt(2.0 / fan_in))
+    parts.append(np.zeros(fan_out))
+  return np.concatenate(parts).astype(np.float32)
+
+
+@dataclasses.dataclass
+class Family:
+  """Parametric condensed MPC: shared A, P; b(x0), c(x0) affine in x0."""
+  horizon: int = 32
+  nx: int = 8
+  nu: int = 3
+  family_seed: int = 7
+
+  def __post_init__(self):
+    self.sys = problems.parametric_mpc_family(self.family_seed, self.horizon, self.nx, self.nu)
+    base = problems.condensed_mpc(self.sys, self.horizon, np.zeros(self.nx))
+    self.a, self.p, self.z = base.a, base.p, base.z
+    self.n, self.m = base.n, base.m
+    # b = b0 + Bx x0, c = Cx x0 (read off by linearity)
+    eye = np.eye(self.nx)
+    cols = [problems.condensed_mpc(self.sys, self.horizon, eye[k]) for k in range(self.nx)]
+    self.b0, self.c0 = base.b, base.c
+    self.bx = np.stack([q.b - base.b for q in cols], axis=1)
+    self.cx = np.stack([q.c - base.c for q in cols], axis=1)
+
+  def qps(self, x0s):
+    b = self.b0[None, :] + x0s @ self.bx.T
+    c = self.c0[None, :] + x0s @ self.cx.T
+    return [(self.a, b[i], c[i], self.z, self.p) for i in range(x0s.shape[0])]
+
+
+def _solve(fam: Family, x0s, *, start=None, max_iter=100, chunk=4096, precision="mixed"):
+  """Batched solves in chunks; returns dict of per-QP arrays (raw iterates)."""
+  n, m = fam.n, fam.m
+  out = {k: [] for k in ("x", "y", "s", "tau", "status", "iters", "sol_x", "sol_y", "sol_s")}
+  t_dev = 0.0
+  for c0 in range(0, x0s.shape[0], chunk):
+    qs = fam.qps(x0s[c0:c0 + chunk])
+    bq = batch.BatchQP(qs)
+    if start is not None:
+      sx, sy, ss, st = (v[c0:c0 + chunk] for v in start)
+      bq.set_start(sx.ravel(), sy.ravel(), ss.ravel(), st)
+    cnt = bq.solve_device(max_iter=max_iter, precision=precision)
+    t_dev += cnt["device_ms"]
+    raw = bq.raw_iterates()
+    sols = bq.results()
+    bq.free()
+    k = len(qs)
+    out["x"].append(raw["x"].reshape(k, n))
+    out["y"].append(raw["y"].reshape(k, m))
+    out["s"].append(raw["s"].reshape(k, m))
+    out["tau"].append(raw["tau"])
+    out["status"].append(raw["status"])
+    out["iters"].append(raw["iters"])
+    out["sol_x"].append(np.stack([s.x for s in sols]))
+    out["sol_y"].append(np.stack([s.y for s in sols]))
+    out["sol_s"].append(np.stack([s.s for s in sols]))
+  res = {k: np.concatenate(v) for k, v in out.items()}
+  res["device_ms"] = t_dev
+  return res
+
+
+class WarmStart:
+  """x0 -> starting point of the batched IPM for one Family."""
+
+  def __init__(self, fam: Family, hidden=256, k_star=6, seed=0):
+    self.fam, self.k_star = fam, k_star
+    self.n_out = fam.n + 2 * fam.m + 2
+    self.mlp = MLP(fam.nx, hidden, self.n_out, max_batch=512, seed=seed)
+    self.xm = self.xs = self.tm = self.ts = None
+
+  def targets(self, x0s, chunk=4096):
+    """Solver iterates after k_star iterations, as (x, log y, log s, log tau, log kappa)."""
+    r = _solve(self.fam, x0s, max_iter=self.k_star, chunk=chunk)
+    z = self.fam.z
+    y, s, tau = r["y"], r["s"], r["tau"]
+    mu = np.sum(y[:, z:] * s[:, z:], axis=1) / (self.fam.m - z)
+    t = np.concatenate([r["x"], np.log(np.maximum(y, 1e-300)), np.log(np.maximum(s, 1e-300)),
+                        np.log(tau)[:, None], np.log(mu / tau)[:, None]], axis=1)
+    return t, r
+
+  def fit(self, x0s, t, *, steps=3000, batch_size=512, lr=1e-3, seed=0):
+    self.xm, self.xs = x0s.mean(0), x0s.std(0) + 1e-12
+    self.tm, self.ts = t.mean(0), t.std(0) + 1e-6
+    return self.mlp.train((x0s - self.xm) / self.xs, (t - self.tm) / self.ts, batch=batch_size,
+                          steps=steps, lr=lr, seed=seed)
+
+  def propose(self, x0s):
+    """(x, y, s, tau, kappa, valid) for each x0; unscaled solver coordinates."""
+    raw = self.mlp.forward((x0s - self.xm) / self.xs).astype(np.float64) * self.ts + self.tm
+    n, m = self.fam.n, self.fam.m
+    x = raw[:, :n]
+    y = np.exp(np.clip(raw[:, n:n + m], -700, 700))
+    s = np.exp(np.clip(raw[:, n + m:n + 2 * m], -700, 700))
+    tau = np.exp(np.clip(raw[:, -2], -700, 700))
+    kappa = np.exp(np.clip(raw[:, -1], -700, 700))
+    z = self.fam.z
+    y[:, :z] = raw[:, n:n + z]  # free duals of equality rows (none in MPC)
+    s[:, :z] = 0.0
+    valid = (np.all(np.isfinite(raw), axis=1) & np.all(y[:, z:] > 0, axis=1)
+             & np.all(s[:, z:] > 0, axis=1) & (tau > 0) & (kappa > 0))
+    return x, y, s, tau, kappa, valid
+
+
+def default_start(fam: Family, count: int):
+  x = np.zeros((count, fam.n))
+  y = np.zeros((count, fam.m))
+  s = np.zeros((count, fam.m))
+  y[:, fam.z:] = 1.0
+  s[:, fam.z:] = 1.0
+  return x, y, s, np.ones(count)
+
+
+def solve_with_warm_start(ws: WarmStart, x0s, *, cap: int, chunk=4096):
+  """Warm-started batched solve with the default-start fallback rule."""
+  fam = ws.fam
+  x, y, s, tau, _, valid = ws.propose(x0s)
+  dx, dy, ds, dt = default_start(fam, x0s.shape[0])
+  x = np.where(valid[:, None], x, dx)
+  y = np.where(valid[:, None], y, dy)
+  s = np.where(valid[:, None], s, ds)
+  tau = np.where(valid, tau, dt)
+  t0 = time.perf_counter()
+  warm = _solve(fam, x0s, start=(x, y, s, tau), max_iter=cap, chunk=chunk)
+  unfinished = warm["status"] == 0
+  iters = warm["iters"].astype(np.int64).copy()
+  status = warm["status"].copy()
+  sol_x, sol_y, sol_s = warm["sol_x"], warm["sol_y"], warm["sol_s"]
+  device_ms = warm["device_ms"]
+  if unfinished.any():
+    redo = _solve(fam, x0s[unfinished], chunk=chunk)
+    iters[unfinished] += redo["iters"]
+    status[unfinished] = redo["status"]
+    sol_x[unfinished], sol_y[unfinished], sol_s[unfinished] = redo["sol_x"], redo["sol_y"], redo["sol_s"]
+    device_ms += redo["device_ms"]
+  return {"iters": iters, "status": status, "fallback": unfinished | ~valid,
+          "invalid": ~valid, "x": sol_x, "y": sol_y, "s": sol_s, "device_ms": device_ms,
+          "wall_s": time.perf_counter() - t0}

@@ -1,0 +1,31 @@
+"""Learned warm start on the parametric MPC family: what the trained network
+gains over the default start, with identical statuses and KKT-accurate
+solutions (checked with the reference's KKT assertion restated in
+tests/qp_generators.py)."""
+import numpy as np
+import pytest
+
+from qtqp_amd import problems, warmstart
+import qp_generators as G
+
+pytestmark = pytest.mark.gpu
+
+
+def test_warm_start_cuts_iterations_with_identical_statuses(require_gpu):
+  fam = warmstart.Family()
+  x_tr = problems.parametric_mpc_x0(8192, seed=11)
+  x_ev = problems.parametric_mpc_x0(1024, seed=22)
+  cold = warmstart._solve(fam, x_ev)
+  ws = warmstart.WarmStart(fam, k_star=7, seed=0)
+  t, _ = ws.targets(x_tr)
+  losses = ws.fit(x_tr, t, steps=2000)
+  assert losses[-50:].mean() < 0.05 * losses[:5].mean()
+  warm = warmstart.solve_with_warm_start(ws, x_ev, cap=int(np.percentile(cold["iters"], 99)))
+  np.testing.assert_array_equal(warm["status"], cold["status"])
+  cut = 1.0 - warm["iters"].mean() / cold["iters"].mean()
+  assert cut >= 0.30, cut
+  assert warm["fallback"].mean() < 0.05
+  qps = fam.qps(x_ev[:32])
+  for i, (a, b, c, z, p) in enumerate(qps):
+    G.check_solution(warm["x"][i], warm["y"][i], warm["s"][i], a, b, c, p, z)
+  ws.mlp.free()
