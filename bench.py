@@ -129,6 +129,79 @@ def algorithmic_work(n, m, counts, iters):
   return flops32, flops64, float(bytes_)
 
 
+def kkt_rel_residual(x, y, s, a, b, c, p, z):
+  """max(pres/prel, dres/drel, gap/min|cost|) of a solution (tests/test_qtqp.py:283-309 form)."""
+  pres = np.linalg.norm(a @ x + s - b, np.inf)
+  dres = np.linalg.norm(p @ x + a.T @ y + c, np.inf)
+  gap = abs(c @ x + b @ y + x @ p @ x)
+  prel = max(np.linalg.norm(a @ x, np.inf), np.linalg.norm(s, np.inf), np.linalg.norm(b, np.inf))
+  drel = max(np.linalg.norm(p @ x, np.inf), np.linalg.norm(a.T @ y, np.inf), np.linalg.norm(c, np.inf))
+  cost = min(abs(c @ x + 0.5 * x @ p @ x), abs(b @ y + 0.5 * x @ p @ x))
+  return max(pres / prel, dres / drel, gap / max(cost, 1e-300))
+
+
+def warm_start_leg(n_train, n_eval, steps, k_star=7):
+  """Learned warm start on the parametric MPC family (T=32, nx=8, nu=3)."""
+  from qtqp_amd import problems, warmstart
+  from oracle import qtqp_oracle
+  fam = warmstart.Family()
+  x_tr = problems.parametric_mpc_x0(n_train, seed=11)
+  x_ev = problems.parametric_mpc_x0(n_eval, seed=22)
+  t0 = time.perf_counter()
+  cold = warmstart._solve(fam, x_ev)
+  t_cold = time.perf_counter() - t0
+  log(f"warm start: cold solve of {n_eval} QPs: {cold['iters'].mean():.2f} iterations mean")
+  ws = warmstart.WarmStart(fam, k_star=k_star)
+  t0 = time.perf_counter()
+  t, _ = ws.targets(x_tr)
+  t_data = time.perf_counter() - t0
+  log(f"warm start: {n_train} solver targets in {t_data:.1f}s; training")
+  t0 = time.perf_counter()
+  losses = ws.fit(x_tr, t, steps=steps)
+  t_train = time.perf_counter() - t0
+  cap = int(np.percentile(cold["iters"], 99))
+  t0 = time.perf_counter()
+  warm = warmstart.solve_with_warm_start(ws, x_ev, cap=cap)
+  t_warm = time.perf_counter() - t0
+  qps = fam.qps(x_ev[:32])
+  res, dx = [], []
+  ok = True
+  for i, (a, b, c, z, p) in enumerate(qps):
+    ref = qtqp_oracle.solve_qp(a, b, c, z, p)
+    ok &= ref["status"] == ("solved" if warm["status"][i] == 1 else "other")
+    res.append(kkt_rel_residual(warm["x"][i], warm["y"][i], warm["s"][i], a, b, c, p, z))
+    dx.append(float(np.max(np.abs(warm["x"][i] - ref["x"])) / max(1.0, np.max(np.abs(ref["x"])))))
+  ws.mlp.free()
+  return {"family": "condensed MPC T=32 nx=8 nu=3, x0 ~ N(0, 0.5^2 I)", "k_star": k_star,
+          "train_qps": n_train, "eval_qps": n_eval, "mlp": "8-256-256-1506 GELU, BF16 MFMA, Adam",
+          "train_steps": steps, "loss_first_last": [float(losses[:10].mean()), float(losses[-50:].mean())],
+          "data_gen_s": t_data, "train_s": t_train,
+          "cold_iters_mean": float(cold["iters"].mean()), "cold_iters_p99": float(np.percentile(cold["iters"], 99)),
+          "warm_iters_mean": float(warm["iters"].mean()), "warm_iters_p99": float(np.percentile(warm["iters"], 99)),
+          "iteration_cut": float(1.0 - warm["iters"].mean() / cold["iters"].mean()),
+          "statuses_identical": bool(np.array_equal(warm["status"], cold["status"])),
+          "fallback_rate": float(warm["fallback"].mean()),
+          "cold_qps_per_s": n_eval / t_cold, "warm_qps_per_s_incl_inference": n_eval / t_warm,
+          "oracle_sample": len(qps), "oracle_status_match": bool(ok),
+          "max_rel_kkt_residual": float(max(res)), "max_rel_x_diff_vs_oracle": float(max(dx))}
+
+
+def dropin_fp64(qps, reps=1):
+  """The drop-in path: qtqp_amd.QTQP host driver + HIP_DENSE_FP64 backend, one QP per call."""
+  import scipy.sparse as sp
+  import qtqp_amd
+  q0 = qps[0]
+  qtqp_amd.QTQP(a=sp.csc_matrix(q0.a), b=q0.b, c=q0.c, z=q0.z, p=sp.csc_matrix(q0.p)).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE_FP64, verbose=False)
+  t0 = time.perf_counter()
+  st = []
+  for q in qps:
+    sol = qtqp_amd.QTQP(a=sp.csc_matrix(q.a), b=q.b, c=q.c, z=q.z, p=sp.csc_matrix(q.p)).solve(
+        linear_solver=qtqp_amd.LinearSolver.HIP_DENSE_FP64, verbose=False)
+    st.append(sol.status.value)
+  return len(qps) / (time.perf_counter() - t0), st
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +212,10 @@ def main():
   ap.add_argument("--plain-sample", type=int, default=32)
   ap.add_argument("--cpu-budget", type=float, default=20.0)
   ap.add_argument("--no-cpu", action="store_true")
+  ap.add_argument("--ws-train", type=int, default=100000)
+  ap.add_argument("--ws-eval", type=int, default=16384)
+  ap.add_argument("--ws-steps", type=int, default=8000)
+  ap.add_argument("--no-ws", action="store_true")
   args = ap.parse_args()
   if args.gpus != 1:
     print(json.dumps({"error": "this build measures one GPU only; the batched engine shards "
@@ -194,20 +271,24 @@ def main():
                "algorithmic_flops_per_launch": (flops32 + flops64) / nl,
                "combined_frac": (t_mem + t_mfma) / iter_ms})
 
-  # plain FP64 GPU path: one QP per call, FP64 factor/solves, no batching, no graph
+  # plain FP64 GPU paths, one QP per call, FP64 factor/solves, no batching/graph:
+  # (1) the batched engine's kernels on a 1-QP batch, (2) the drop-in driver.
   sample = qps[: args.plain_sample]
+  log(f"plain FP64: {len(sample)} QPs one per call")
   singles = [batch.BatchQP([q]) for q in sample]
-  for s in singles[:2]:
-    s.solve_device(precision="fp64", use_graph=False)
+  for s_ in singles[:2]:
+    s_.solve_device(precision="fp64", use_graph=False)
   t0 = time.perf_counter()
-  for s in singles:
-    s.solve_device(precision="fp64", use_graph=False)
-  plain_s = time.perf_counter() - t0
-  plain_qps = len(sample) / plain_s
-  plain_status_match = all(s.results()[0].status.value == statuses[i]
-                           for i, s in enumerate(singles))
-  for s in singles:
-    s.free()
+  for s_ in singles:
+    s_.solve_device(precision="fp64", use_graph=False)
+  plain_qps = len(sample) / (time.perf_counter() - t0)
+  plain_status_match = all(s_.results()[0].status.value == statuses[i]
+                           for i, s_ in enumerate(singles))
+  for s_ in singles:
+    s_.free()
+  drop_qps, drop_st = dropin_fp64(sample[:8])
+  drop_match = all(st == statuses[i] for i, st in enumerate(drop_st))
+  denom = max(plain_qps, drop_qps)
 
   # accuracy against the FP64 oracle on a sample
   from oracle import qtqp_oracle
@@ -232,15 +313,21 @@ def main():
                  "mean_N": float(np.mean(n + m)), "precision": "mixed",
                  "tolerances": "atol 1e-7, rtol 1e-8"},
       "roofline": roof,
-      "plain_fp64_gpu": {"value": plain_qps, "unit": "QPs/s", "sample": len(sample),
-                         "status_match": plain_status_match,
-                         "speedup_of_batched_mixed": value / plain_qps},
+      "plain_fp64_gpu": {"value": denom, "unit": "QPs/s",
+                         "device_1qp_per_call": {"value": plain_qps, "sample": len(sample),
+                                                 "status_match": plain_status_match},
+                         "dropin_driver_1qp_per_call": {"value": drop_qps, "sample": 8,
+                                                        "status_match": drop_match},
+                         "speedup_of_batched_mixed": value / denom},
       "iterations": {"mean": float(iters.mean()), "max": int(iters.max())},
       "statuses": {s: statuses.count(s) for s in set(statuses)},
       "oracle_check": {"sample": len(errs) // 3, "status_match": bool(status_ok),
                        "max_rel_err_xys": max(errs) if errs else None},
       "counters": cnts[-1],
   }
+  if not args.no_ws:
+    log("warm start leg")
+    line["warm_start"] = warm_start_leg(args.ws_train, args.ws_eval, args.ws_steps)
   if not args.no_cpu:
     log("cpu baseline")
     line["cpu_baseline"] = cpu_baseline(args.base_seed, args.cpu_budget)
