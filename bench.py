@@ -186,6 +186,77 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
           "max_rel_kkt_residual": float(max(res)), "max_rel_x_diff_vs_oracle": float(max(dx))}
 
 
+def single_instance_leg(name, qp, oracle_ref=None):
+  """One large QP through the drop-in driver, mixed vs plain FP64 HIP backends.
+
+  Reports per-iteration factor time (HIP events around the factorisation
+  kernels), its MFMA utilisation (executed Gram-route flops m n^2 + n^3/3
+  per factorisation over the FP32 or FP64 dense MFMA peak), refinement step
+  counts, and the x/y/s error against the planted optimum and/or the FP64
+  oracle.
+  """
+  import types
+  import scipy.sparse as sp
+  import qtqp_amd
+  from qtqp_amd import direct
+  n, m = qp.n, qp.m
+  flops = m * n * n + n ** 3 / 3
+  out = {"n": n, "m": m, "z": qp.z, "N": n + m, "factor_flops": flops}
+  a_csc, p_csc = sp.csc_matrix(qp.a), sp.csc_matrix(qp.p)
+  for prec, cls, peak in (("mixed", direct.HipDenseSolver, FP32_PEAK_TFLOPS),
+                          ("fp64", direct.HipDenseFp64Solver, FP64_PEAK_TFLOPS)):
+    infos = []
+
+    class Probe(cls):  # records the engine counters before the handle is released
+      def free(self):
+        if getattr(self, "_h", None) is not None:
+          infos.append(self.info())
+        super().free()
+
+    sel = types.SimpleNamespace(name=f"HIP_DENSE[{prec}]", value=Probe)
+    log(f"{name}: {prec} solve (n={n}, m={m})")
+    t0 = time.perf_counter()
+    sol = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc).solve(
+        linear_solver=sel, verbose=False, collect_stats=True)
+    wall = time.perf_counter() - t0
+    inf = infos[-1]
+    nf = max(1, inf["factorizations"])
+    f_ms = inf["factor_ms"] / nf
+    ir = [st.get("solves", 0) for it in sol.stats if isinstance(it, dict)
+          for k, st in it.items() if k.endswith("lin_sys_stats")]
+    r = {"status": sol.status.value, "iterations": len(sol.stats), "wall_s": wall,
+         "factor_ms_per_iter": f_ms, "factor_tflops": flops / (f_ms / 1e3) / 1e12,
+         "factor_mfma_frac": flops / (f_ms / 1e3) / 1e12 / peak,
+         "peak_tflops": peak, "solve_ms_total": inf["solve_ms"],
+         "refinement_solves_per_system": {"mean": float(np.mean(ir)) if ir else None,
+                                          "max": int(max(ir)) if ir else None},
+         "backend_solves": inf["backend_solves"], "fp32_inner_steps": inf["inner_steps"],
+         "fp64_fallbacks": inf["fp64_fallbacks"]}
+    for tag, ref in (("planted", (qp.x_star, qp.y_star, qp.s_star)),
+                     ("oracle", None if oracle_ref is None else
+                      (oracle_ref["x"], oracle_ref["y"], oracle_ref["s"]))):
+      if ref is None or ref[0] is None:
+        continue
+      r[f"max_rel_err_xys_vs_{tag}"] = max(
+          float(np.max(np.abs(g - w)) / max(1.0, np.max(np.abs(w))))
+          for g, w in zip((sol.x, sol.y, sol.s), ref))
+    r["rel_kkt_residual"] = kkt_rel_residual(sol.x, sol.y, sol.s, qp.a, qp.b, qp.c, qp.p, qp.z)
+    out[prec] = r
+  out["factor_speedup_mixed_vs_fp64"] = (out["fp64"]["factor_ms_per_iter"] /
+                                         out["mixed"]["factor_ms_per_iter"])
+  return out
+
+
+def single_legs(problems, qtqp_oracle):
+  out = {"single_large_dense": single_instance_leg(
+      "n=4000 m=6000 z=1000 dense QP (seed 0)", problems.random_dense_qp(4000, 6000, 1000, seed=0))}
+  q3 = problems.mpc_qp(64, 12, 4, seed=0)
+  log("oracle for MPC T=64 nx=12 nu=4")
+  out["single_mpc"] = single_instance_leg("condensed MPC T=64 nx=12 nu=4 (seed 0)", q3,
+                                          qtqp_oracle.solve_qp(q3.a, q3.b, q3.c, q3.z, q3.p))
+  return out
+
+
 def dropin_fp64(qps, reps=1):
   """The drop-in path: qtqp_amd.QTQP host driver + HIP_DENSE_FP64 backend, one QP per call."""
   import scipy.sparse as sp
@@ -202,6 +273,36 @@ def dropin_fp64(qps, reps=1):
   return len(qps) / (time.perf_counter() - t0), st
 
 
+def calibrate_copy(mib: int = 1024):
+  """A plain streaming kernel of known traffic for scaling HBM counters.
+
+  out = in * 1 over `mib` MiB of FP32: one elementwise kernel that reads and
+  writes exactly mib MiB each.
+  """
+  import torch
+  x = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+  y = torch.empty_like(x)
+  for _ in range(3):
+    torch.mul(x, 1.0, out=y)
+  torch.cuda.synchronize()
+  print(json.dumps({"copy_kernel": "torch.mul(x, 1.0, out=y), FP32", "bytes_read": mib << 20,
+                    "bytes_written": mib << 20, "dispatches": 3}), flush=True)
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "k_ipm_iter_mpc4096_traffic.md"
+
+
+def measured_traffic():
+  """HBM bytes per k_ipm_iter launch from the committed counter summary, or None."""
+  import re
+  try:
+    txt = TRAFFIC_FILE.read_text()
+  except OSError:
+    return None
+  mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+  return float(mt.group(1)) if mt else None
+
+
 def main():
   ap = argparse.ArgumentParser()
   ap.add_argument("--gpus", type=int, default=1)
@@ -216,13 +317,25 @@ def main():
   ap.add_argument("--ws-eval", type=int, default=16384)
   ap.add_argument("--ws-steps", type=int, default=8000)
   ap.add_argument("--no-ws", action="store_true")
+  ap.add_argument("--no-single", action="store_true")
+  ap.add_argument("--calibrate-copy", action="store_true")
+  ap.add_argument("--no-plain", action="store_true")
+  ap.add_argument("--single-only", action="store_true")
+  ap.add_argument("--oracle-sample", type=int, default=64)
   args = ap.parse_args()
   if args.gpus != 1:
     print(json.dumps({"error": "this build measures one GPU only; the batched engine shards "
                                "QPs across GPUs by design but no multi-GPU run is supported"}))
     sys.exit(2)
+  if args.calibrate_copy:
+    calibrate_copy()
+    return
 
   from qtqp_amd import batch, problems
+  from oracle import qtqp_oracle
+  if args.single_only:
+    print(json.dumps(single_legs(problems, qtqp_oracle)), flush=True)
+    return
 
   log(f"generating {args.count} uneven MPC QPs")
   t0 = time.perf_counter()
@@ -266,6 +379,7 @@ def main():
     achieved = (flops32 + flops64) / nl / (iter_ms / nl / 1e3) / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+  roof["traffic"] = measured_traffic()
   roof.update({"kernel": "k_ipm_iter", "launches": nl, "avg_launch_ms": iter_ms / nl,
                "algorithmic_bytes_per_launch": bytes_ / nl,
                "algorithmic_flops_per_launch": (flops32 + flops64) / nl,
@@ -273,34 +387,49 @@ def main():
 
   # plain FP64 GPU paths, one QP per call, FP64 factor/solves, no batching/graph:
   # (1) the batched engine's kernels on a 1-QP batch, (2) the drop-in driver.
-  sample = qps[: args.plain_sample]
-  log(f"plain FP64: {len(sample)} QPs one per call")
-  singles = [batch.BatchQP([q]) for q in sample]
-  for s_ in singles[:2]:
-    s_.solve_device(precision="fp64", use_graph=False)
-  t0 = time.perf_counter()
-  for s_ in singles:
-    s_.solve_device(precision="fp64", use_graph=False)
-  plain_qps = len(sample) / (time.perf_counter() - t0)
-  plain_status_match = all(s_.results()[0].status.value == statuses[i]
-                           for i, s_ in enumerate(singles))
-  for s_ in singles:
-    s_.free()
-  drop_qps, drop_st = dropin_fp64(sample[:8])
-  drop_match = all(st == statuses[i] for i, st in enumerate(drop_st))
-  denom = max(plain_qps, drop_qps)
+  if args.no_plain:
+    plain = None
+  else:
+    sample = qps[: args.plain_sample]
+    log(f"plain FP64: {len(sample)} QPs one per call")
+    singles = [batch.BatchQP([q]) for q in sample]
+    for s_ in singles[:2]:
+      s_.solve_device(precision="fp64", use_graph=False)
+    t0 = time.perf_counter()
+    for s_ in singles:
+      s_.solve_device(precision="fp64", use_graph=False)
+    plain_qps = len(sample) / (time.perf_counter() - t0)
+    plain_status_match = all(s_.results()[0].status.value == statuses[i]
+                             for i, s_ in enumerate(singles))
+    for s_ in singles:
+      s_.free()
+    drop_qps, drop_st = dropin_fp64(sample[:8])
+    drop_match = all(st == statuses[i] for i, st in enumerate(drop_st))
+    denom = max(plain_qps, drop_qps)
+    plain = {"value": denom, "unit": "QPs/s",
+             "device_1qp_per_call": {"value": plain_qps, "sample": len(sample),
+                                     "status_match": plain_status_match},
+             "dropin_driver_1qp_per_call": {"value": drop_qps, "sample": len(sample[:8]),
+                                            "status_match": drop_match},
+             "speedup_of_batched_mixed": value / denom}
 
   # accuracy against the FP64 oracle on a sample
-  from oracle import qtqp_oracle
+  log(f"oracle check on {args.oracle_sample} instances")
   errs = []
   status_ok = True
-  for i in range(0, args.count, max(1, args.count // 16)):
+  for i in np.linspace(0, args.count - 1, args.oracle_sample).astype(int):
     q = qps[i]
     ref = qtqp_oracle.solve_qp(q.a, q.b, q.c, q.z, q.p)
     status_ok &= ref["status"] == statuses[i]
     for k in "xys":
       got, want = getattr(sols[i], k), ref[k]
       errs.append(float(np.max(np.abs(got - want)) / max(1.0, np.max(np.abs(want)))))
+  # achieved relative KKT residual of every solved instance
+  kkt = [kkt_rel_residual(s_.x, s_.y, s_.s, q.a, q.b, q.c, q.p, q.z)
+         for s_, q in zip(sols, qps) if s_.status.value == "solved"]
+  # per-instance completion time: equilibration/start, then one launch per iteration
+  prep_ms = max(0.0, cnts[-1]["device_ms"] - float(np.sum(launch_ms)))
+  done_ms = prep_ms + np.cumsum(launch_ms)[np.clip(iters, 1, len(launch_ms)) - 1]
 
   line = {
       "metric": METRIC, "value": value, "unit": "QPs/s", "n_gpus": 1, "steps": args.steps,
@@ -313,25 +442,27 @@ def main():
                  "mean_N": float(np.mean(n + m)), "precision": "mixed",
                  "tolerances": "atol 1e-7, rtol 1e-8"},
       "roofline": roof,
-      "plain_fp64_gpu": {"value": denom, "unit": "QPs/s",
-                         "device_1qp_per_call": {"value": plain_qps, "sample": len(sample),
-                                                 "status_match": plain_status_match},
-                         "dropin_driver_1qp_per_call": {"value": drop_qps, "sample": 8,
-                                                        "status_match": drop_match},
-                         "speedup_of_batched_mixed": value / denom},
-      "iterations": {"mean": float(iters.mean()), "max": int(iters.max())},
+      "plain_fp64_gpu": plain,
+      "iterations": {"mean": float(iters.mean()), "p99": float(np.percentile(iters, 99)),
+                     "max": int(iters.max())},
+      "instance_completion_ms": {"p50": float(np.percentile(done_ms, 50)),
+                                 "p99": float(np.percentile(done_ms, 99)),
+                                 "prepare_ms": prep_ms},
+      "rel_kkt_residual_solved": {"max": float(np.max(kkt)), "median": float(np.median(kkt))},
       "statuses": {s: statuses.count(s) for s in set(statuses)},
       "oracle_check": {"sample": len(errs) // 3, "status_match": bool(status_ok),
                        "max_rel_err_xys": max(errs) if errs else None},
       "counters": cnts[-1],
   }
+  bq.free()
+  if not args.no_single:
+    line.update(single_legs(problems, qtqp_oracle))
   if not args.no_ws:
     log("warm start leg")
     line["warm_start"] = warm_start_leg(args.ws_train, args.ws_eval, args.ws_steps)
   if not args.no_cpu:
     log("cpu baseline")
     line["cpu_baseline"] = cpu_baseline(args.base_seed, args.cpu_budget)
-  bq.free()
   print(json.dumps(line), flush=True)
 
 
