@@ -186,44 +186,139 @@ __device__ inline void lower_tile(int t, int* ti, int* tj) {
 template <typename T>
 __device__ inline void load_diag_block(const T* G, int64_t ld, int k0, int nb,
                                        T (*S)[NB + 1]) {
-  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
+  // 16 loads per thread issued together (clamped in-block addresses, values
+  // selected afterwards) so their latencies overlap.
+  T v[NB * NB / WG];
+#pragma unroll
+  for (int u = 0; u < NB * NB / WG; ++u) {
+    const int idx = threadIdx.x + u * WG;
+    const int i = min(idx & (NB - 1), nb - 1), j = min(idx >> 6, nb - 1);
+    v[u] = G[(int64_t)(k0 + j) * ld + k0 + i];
+  }
+#pragma unroll
+  for (int u = 0; u < NB * NB / WG; ++u) {
+    const int idx = threadIdx.x + u * WG;
     const int i = idx & (NB - 1), j = idx >> 6;
-    T v;
-    if (i < nb && j < nb)
-      v = (i >= j) ? G[(int64_t)(k0 + j) * ld + k0 + i] : T(0);
-    else
-      v = (i == j) ? T(1) : T(0);
-    S[i][j] = v;
+    S[i][j] = (i < nb && j < nb) ? (i >= j ? v[u] : T(0)) : T(i == j);
   }
 }
 
-// In-LDS Cholesky of the diagonal block (all WG threads; 3 barriers/column).
-// A non-positive or NaN pivot records `kglob + j + 1` in *info (first one
-// wins) and is replaced by 1 so the sweep finishes without NaNs.
+// Reciprocal from the hardware estimate refined by Newton steps (within an
+// ulp or two of 1/x; used where a correctly rounded quotient is not needed).
+__device__ inline double approx_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * (2.0 - x * r);
+  return r * (2.0 - x * r);
+}
+__device__ inline float approx_rcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return r * (2.0f - x * r);
+}
+
+// In-LDS Cholesky of the diagonal block fused with the inverse of its factor
+// (all WG threads, one barrier per column).  On entry S holds the block
+// (lower part; padding = identity from load_diag_block).  On return S holds
+// L (lower, L[j][j] = sqrt(pivot), zeros above), X = L^{-1} (lower, zeros
+// above) and sinv[j] = 1 / L[j][j].
+//
+// Thread (i = lane, w = wave) keeps its 16 columns of row i in registers,
+// R[t] = S[i][16w+t] and Y[t] = X[i][16w+t].  Both sweeps run right-looking
+// on unscaled values:
+//   Cholesky step p:  S[i][k] -= S[i][p] S[k][p] / S[p][p]   (p < k <= i)
+//   inverse step j:   Y[i][c] -= S[i][j] / S[j][j] * Y[j][c] (c <= j < i)
+// with Y starting as the identity; row j of L^{-1} is Y[j][:] / L[j][j].
+// Phase p runs Cholesky step p and inverse step p-1: the owners publish
+// column p of S and row p-1 of Y into double-buffered LDS vectors (kept in
+// the first rows of X, which is written only at the end), one barrier, then
+// every thread updates its registers from broadcast reads.  A non-positive
+// or NaN pivot records `kglob + j + 1` in *info (the first such column) and
+// is replaced by 1 so the sweep finishes without NaNs.
 template <typename T>
-__device__ inline void potrf_lds(T (*S)[NB + 1], T* sinv, int nb, int* info,
-                                 int kglob) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int j = 0; j < nb; ++j) {
-    const T piv = S[j][j];
-    const bool bad = !(piv > T(0));
-    const T d = bad ? T(1) : (T)sqrt((double)piv);
-    __syncthreads();
-    if (tid == 0) {
-      if (bad && info) atomicCAS(info, 0, kglob + j + 1);
-      S[j][j] = d;
-      sinv[j] = T(1) / d;
-    }
-    if (tid > j && tid < nb) S[tid][j] = S[tid][j] / d;
-    __syncthreads();
-    const int i = lane;
-    if (i > j && i < nb) {
-      const T lij = S[i][j];
-      for (int k = j + 1 + wave; k <= i; k += 4) S[i][k] -= lij * S[k][j];
-    }
-    __syncthreads();
+__device__ inline void potrf_inv_lds(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, int nb,
+                                     int* info, int kglob) {
+  const int tid = threadIdx.x, i = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
+  T R[16], Y[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    R[t] = S[i][16 * w + t];
+    Y[t] = T((16 * w + t) == i);
   }
-  for (int j = nb + tid; j < NB; j += WG) sinv[j] = T(1);
+  T* col = &X[0][0];     // [2][64]: column p of S below the diagonal, zeros above
+  T* xrow = col + 128;   // [2][64]: row p-1 of Y
+  T* pivs = col + 256;   // [2]: S[p][p]
+  T fprev = T(0);
+  for (int p = 0; p <= NB; ++p) {
+    const int b = (p & 1) * NB;
+    if (p < NB && w == (p >> 4)) {
+      const T v = R[p & 15];
+      col[b + i] = i > p ? v : T(0);
+      if (i == p) pivs[p & 1] = v;
+    }
+    if (p >= 1 && i == p - 1) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) xrow[b + 16 * w + t] = Y[t];
+    }
+    __syncthreads();
+    // All updates are unconditional: the published column is zero in rows
+    // <= p, so rows i <= p get f = 0 and columns k <= p get cv = 0; row p-1
+    // of Y is zero right of the diagonal, and rows i <= p-1 get fprev = 0.
+    // Entries of R above the diagonal pick up values that are never
+    // published (zeros are) and are dropped at the end.
+    T cv[16], xv[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      cv[t] = col[b + 16 * w + t];
+      xv[t] = xrow[b + 16 * w + t];
+    }
+    const T piv = pivs[p & 1];
+    const T fcur = col[b + i] * approx_rcp(piv > T(0) ? piv : T(1));
+    if (p < NB) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) R[t] -= fcur * cv[t];
+    }
+    if (p >= 1) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) Y[t] -= fprev * xv[t];
+    }
+    fprev = fcur;
+  }
+  // Pivots are the final diagonal entries of R (raw): 1/sqrt, breakdowns.
+  __shared__ int first_bad;
+  if (tid == 0) first_bad = NB;
+  T pv = T(1);
+  bool own = false;
+#pragma unroll
+  for (int t = 0; t < 16; ++t)
+    if (16 * w + t == i) {
+      pv = R[t];
+      own = true;
+    }
+  __syncthreads();  // broadcast buffers (inside X) were read for the last time above
+  if (own) {
+    const bool ok = pv > T(0);
+    sinv[i] = ok ? (T)(1.0 / sqrt((double)pv)) : T(1);
+    if (!ok && i < nb) atomicMin(&first_bad, i);
+  }
+  __syncthreads();
+  if (tid == 0 && info && first_bad < nb) atomicCAS(info, 0, kglob + first_bad + 1);
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int k = 16 * w + t;
+    T sv, xv;
+    if (k < i) {
+      sv = R[t] * sinv[k];
+      xv = Y[t] * sinv[i];
+    } else if (k == i) {
+      sv = R[t] > T(0) ? (T)sqrt((double)R[t]) : T(1);
+      xv = sinv[i];
+    } else {
+      sv = T(0);
+      xv = T(0);
+    }
+    S[i][k] = sv;
+    X[i][k] = xv;
+  }
   __syncthreads();
 }
 
@@ -237,35 +332,18 @@ __device__ inline void store_diag_block(T* G, int64_t ld, int k0, int nb,
   }
 }
 
-// Li = L11^{-1} for the factored diagonal block S (lower, row-major, padding
-// = identity).  Wave 0, lane j computes column j by forward substitution;
-// entries above the diagonal are zero.  Caller syncs afterwards.
-template <typename T>
-__device__ inline void tri_inverse_lds(const T (*S)[NB + 1], const T* sinv,
-                                       T (*Li)[NB + 1]) {
-  const int j = threadIdx.x;
-  if (j >= NB) return;
-  for (int i = 0; i < j; ++i) Li[i][j] = T(0);
-  Li[j][j] = sinv[j];
-  for (int i = j + 1; i < NB; ++i) {
-    T s = T(0);
-    for (int k = j; k < i; ++k) s += S[i][k] * Li[k][j];
-    Li[i][j] = -s * sinv[i];
-  }
-}
-
-// Panel solve on MFMA for rows [rbeg, rbeg+WG) below the diagonal block:
-// L21 = A21 L11^{-T}, computed as L21' = Li A21' so that results land
-// contiguously in the column-major G.  Wave v handles 64 rows as four
+// Panel solve on MFMA for rows [rbeg, rbeg + 64*STRIPS) below the diagonal
+// block: L21 = A21 L11^{-T}, computed as L21' = Li A21' so that results land
+// contiguously in the column-major G.  Wave v handles 16*STRIPS rows as
 // 16-row strips; each strip is 4 MFMA 16x16 tiles over the 64 columns.
-template <typename T>
+template <typename T, int STRIPS = 4>
 __device__ inline void trsm_mfma_rows(T* G, int64_t ld, int n, int k0, int nb,
                                       int rbeg, const T (*Li)[NB + 1]) {
   using acc_t = typename Mfma<T>::acc_t;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int cl = lane & 15, kq = lane >> 4;
-  for (int s = 0; s < 4; ++s) {
-    const int r0 = rbeg + 64 * wave + 16 * s;
+  for (int s = 0; s < STRIPS; ++s) {
+    const int r0 = rbeg + 16 * STRIPS * wave + 16 * s;
     if (r0 >= n) break;
     acc_t acc[4];
 #pragma unroll
@@ -287,73 +365,6 @@ __device__ inline void trsm_mfma_rows(T* G, int64_t ld, int n, int k0, int nb,
         const int c = 16 * ct + Mfma<T>::row(lane, r);
         if (c < nb && row < n) G[(int64_t)(k0 + c) * ld + row] = acc[ct][r];
       }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Triangular solves with the factor, one workgroup, vector v in LDS.
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline T lane_bcast(T v, int src) {
-  return __shfl(v, src, 64);
-}
-
-// Solves L v = v in place.
-template <typename T>
-__device__ inline void trsv_lower(const T* L, int64_t ld, int n, T* v,
-                                  T (*S)[NB + 1]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int nb = min(NB, n - j0);
-    load_diag_block(L, ld, j0, nb, S);
-    __syncthreads();
-    if (wave == 0) {
-      T zi = lane < nb ? v[j0 + lane] : T(0);
-      for (int c = 0; c < nb; ++c) {
-        const T zc = lane_bcast(zi, c) / S[c][c];
-        if (lane == c) zi = zc;
-        else if (lane > c) zi -= S[lane][c] * zc;
-      }
-      if (lane < nb) v[j0 + lane] = zi;
-    }
-    __syncthreads();
-    for (int i = j0 + nb + tid; i < n; i += WG) {
-      T acc = T(0);
-      const T* col = L + (int64_t)j0 * ld + i;
-      for (int c = 0; c < nb; ++c) acc += col[(int64_t)c * ld] * v[j0 + c];
-      v[i] -= acc;
-    }
-    __syncthreads();
-  }
-}
-
-// Solves L' v = v in place.
-template <typename T>
-__device__ inline void trsv_upper(const T* L, int64_t ld, int n, T* v,
-                                  T (*S)[NB + 1]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nblk = (n + NB - 1) / NB;
-  for (int jb = nblk - 1; jb >= 0; --jb) {
-    const int j0 = jb * NB, nb = min(NB, n - j0);
-    for (int c = wave; c < nb; c += 4) {
-      double acc = 0.0;
-      const T* col = L + (int64_t)(j0 + c) * ld;
-      for (int i = j0 + nb + lane; i < n; i += 64) acc += (double)col[i] * (double)v[i];
-      acc = wave_sum(acc);
-      if (lane == 0) v[j0 + c] -= (T)acc;
-    }
-    load_diag_block(L, ld, j0, nb, S);
-    __syncthreads();
-    if (wave == 0) {
-      T zi = lane < nb ? v[j0 + lane] : T(0);
-      for (int c = nb - 1; c >= 0; --c) {
-        const T xc = lane_bcast(zi, c) / S[c][c];
-        if (lane == c) zi = xc;
-        else if (lane < c) zi -= S[c][lane] * xc;
-      }
-      if (lane < nb) v[j0 + lane] = zi;
-    }
-    __syncthreads();
   }
 }
 

@@ -83,7 +83,9 @@ __global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n) {
 }
 
 // Panel k0, diagonal block: factors it in LDS, stores L11 into G and
-// L11^{-1} into linv (64 x 64, row-major) for the panel solve.  One workgroup.
+// L11^{-1} into linv + (k0/64) * 2 * 64 * 64 twice: column-major (for the
+// forward sweep), then row-major (for the panel solve and the backward
+// sweep).  One workgroup.
 template <typename T>
 __global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int k0, int* info,
                                                   T* linv) {
@@ -93,22 +95,25 @@ __global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int 
   const int nb = min(NB, n - k0);
   load_diag_block(G, ld, k0, nb, S);
   __syncthreads();
-  potrf_lds(S, sinv, nb, info, k0);
+  potrf_inv_lds(S, Li, sinv, nb, info, k0);
   store_diag_block(G, ld, k0, nb, S);
-  tri_inverse_lds(S, sinv, Li);
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) linv[idx] = Li[idx >> 6][idx & (NB - 1)];
+  T* dst = linv + (int64_t)(k0 / NB) * 2 * NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
+    dst[idx] = Li[idx & (NB - 1)][idx >> 6];
+    dst[NB * NB + idx] = Li[idx >> 6][idx & (NB - 1)];
+  }
 }
 
 // Panel k0, rows below the diagonal block: L21 = A21 L11^{-T} on MFMA,
-// workgroup b handling rows k0+NB+b*WG .. +WG.
+// workgroup b handling rows k0+NB+b*64 .. +64 (one 16-row strip per wave).
 template <typename T>
 __global__ __launch_bounds__(WG) void k_trsm_panel(T* G, int64_t ld, int n, int k0,
                                                   const T* linv) {
   __shared__ T Li[NB][NB + 1];
-  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) Li[idx >> 6][idx & (NB - 1)] = linv[idx];
+  const T* src = linv + (int64_t)(k0 / NB) * 2 * NB * NB + NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) Li[idx >> 6][idx & (NB - 1)] = src[idx];
   __syncthreads();
-  trsm_mfma_rows(G, ld, n, k0, min(NB, n - k0), k0 + NB + blockIdx.x * WG, Li);
+  trsm_mfma_rows<T, 1>(G, ld, n, k0, min(NB, n - k0), k0 + NB + blockIdx.x * NB, Li);
 }
 
 // Trailing update G[t0:, t0:] -= L[t0:, k0:t0] L[t0:, k0:t0]', t0 = k0 + NB.
@@ -234,20 +239,179 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
   }
 }
 
-// One workgroup: v = G^{-1} g with the factor L.  v lives in LDS after the
-// diagonal-block scratch when it fits (vglob == nullptr), else in vglob.
-template <typename T>
-__global__ __launch_bounds__(WG) void k_trsv(const T* __restrict__ L, int64_t ld, int n,
-                                             const double* __restrict__ g,
-                                             double* __restrict__ xout, T* vglob) {
-  extern __shared__ double smem_d[];
-  T (*S)[NB + 1] = reinterpret_cast<T (*)[NB + 1]>(smem_d);
-  T* v = vglob ? vglob : reinterpret_cast<T*>(smem_d) + NB * (NB + 1);
-  for (int i = threadIdx.x; i < n; i += WG) v[i] = (T)g[i];
+// ---------------------------------------------------------------------------
+// Triangular sweeps G x = g, G = L L', across many workgroups.
+//
+// Workgroup "ticket" t (taken with an atomic counter, so every lower ticket
+// belongs to a workgroup that is already running) owns row block I = t
+// (forward) or I = nt-1-t (backward).  It streams the off-diagonal blocks
+// it needs, waiting per block on the flag of the block that produced the
+// matching piece of the solution, then applies the stored inverse of its
+// diagonal block and publishes its piece.  Flags hold the launch epoch so
+// no reset is needed between solves: sync[0], sync[1] are the forward and
+// backward ticket counters, sync[2..2+nt) and sync[2+nt..2+2nt) the block
+// flags.  A wait is bounded: past the limit the block writes NaN (which the
+// callers' NaN checks report) and still publishes, so every workgroup ends.
+// ---------------------------------------------------------------------------
+constexpr unsigned kSpinLimit = 1u << 21;
+
+__device__ inline unsigned take_ticket(unsigned* ctr, unsigned base, unsigned* sh) {
+  if (threadIdx.x == 0) *sh = atomicAdd(ctr, 1u) - base;
   __syncthreads();
-  trsv_lower(L, ld, n, v, S);
-  trsv_upper(L, ld, n, v, S);
-  for (int i = threadIdx.x; i < n; i += WG) xout[i] = (double)v[i];
+  return *sh;
+}
+
+// Lane 0 of the calling wave spins until *flag == want; returns false on
+// timeout.  The acquire fence makes the producer's stores visible.
+__device__ inline bool wave_wait(const unsigned* flag, unsigned want) {
+  int ok = 1;
+  if ((threadIdx.x & 63) == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+      if (++spins > kSpinLimit) {
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  ok = __shfl(ok, 0, 64);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok != 0;
+}
+
+__device__ inline void publish(unsigned* flag, unsigned val) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// out[r] = sum_c M[c*64 + r] w[c] for the 64 x 64 block M (r = lane); the
+// four waves split c and reduce through red[4][64].
+template <typename T>
+__device__ inline double block_gemv(const T* __restrict__ M, const double* w, double* red) {
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  double acc = 0.0;
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) {
+    const int c = q * 16 + cc;
+    acc += (double)M[c * NB + r] * w[c];
+  }
+  red[q * NB + r] = acc;
+  __syncthreads();
+  const double v = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
+  __syncthreads();
+  return v;
+}
+
+// Forward sweep L v = g.  Thread (r = lane, q = wave) accumulates
+// sum_{J<I} sum_{c in q's 16 columns} L[I*64+r][J*64+c] v[J*64+c].
+template <typename T>
+__global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_t ld, int n,
+                                                 const T* __restrict__ LiB,
+                                                 const double* __restrict__ g, T* v,
+                                                 unsigned* sync, unsigned base, unsigned epoch) {
+  __shared__ unsigned sh_ticket;
+  __shared__ double red[4 * NB];
+  __shared__ double w[NB];
+  __shared__ T sv[4][NB];
+  const int nt = gridDim.x;
+  const int I = (int)take_ticket(&sync[0], base, &sh_ticket);
+  unsigned* flags = sync + 2;
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i0 = I * NB, rows = min(NB, n - i0);
+  const bool live = r < rows;
+  bool ok = true;
+  double acc = 0.0;
+  T cur[16], nxt[16];
+  auto load = [&](int J, T* dst) {
+    const T* col = L + (int64_t)(J * NB + q * 16) * ld + i0 + r;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) dst[cc] = live ? col[(int64_t)cc * ld] : T(0);
+  };
+  if (I > 0) load(0, cur);
+  for (int J = 0; J < I; ++J) {
+    if (J + 1 < I) load(J + 1, nxt);
+    ok &= wave_wait(&flags[J], epoch);
+    sv[q][r] = v[J * NB + r];  // J < I: a full block
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) acc += (double)cur[cc] * (double)sv[q][q * 16 + cc];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+  }
+  red[q * NB + r] = acc;
+  __syncthreads();
+  if (q == 0) {
+    const double s = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
+    w[r] = live ? g[i0 + r] - s : 0.0;
+  }
+  __syncthreads();
+  const double vi = block_gemv(LiB + (int64_t)I * 2 * NB * NB, w, red);
+  if (q == 0 && live) v[i0 + r] = ok ? (T)vi : T(NAN);
+  publish(&flags[I], epoch);
+  (void)nt;
+}
+
+// Backward sweep L' x = v.  Thread (rr = lane, q = wave) accumulates, for
+// the 16 columns c of q, sum_{J>I} L[J*64+rr][I*64+c] x[J*64+rr].
+template <typename T>
+__global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_t ld, int n,
+                                                 const T* __restrict__ LiB,
+                                                 const T* __restrict__ v, double* x,
+                                                 unsigned* sync, unsigned base, unsigned epoch) {
+  __shared__ unsigned sh_ticket;
+  __shared__ double red[4 * NB];
+  __shared__ double part[NB][NB + 1];
+  __shared__ double w[NB];
+  const int nt = gridDim.x;
+  const int I = nt - 1 - (int)take_ticket(&sync[1], base, &sh_ticket);
+  unsigned* flags = sync + 2 + nt;
+  const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i0 = I * NB, rows = min(NB, n - i0);
+  bool ok = true;
+  double acc[16];
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
+  T cur[16], nxt[16];
+  auto load = [&](int J, T* dst) {
+    const int row = J * NB + rr;
+    const bool in = row < n;
+    const T* base_p = L + (int64_t)(i0 + q * 16) * ld + row;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc)
+      dst[cc] = (in && q * 16 + cc < rows) ? base_p[(int64_t)cc * ld] : T(0);
+  };
+  if (I + 1 < nt) load(nt - 1, cur);
+  for (int J = nt - 1; J > I; --J) {
+    if (J - 1 > I) load(J - 1, nxt);
+    ok &= wave_wait(&flags[J], epoch);
+    const int row = J * NB + rr;
+    const double xj = row < n ? x[row] : 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) acc[cc] += (double)cur[cc] * xj;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+  }
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) part[q * 16 + cc][rr] = acc[cc];
+  __syncthreads();
+  {
+    // column c = threadIdx.x / 4 summed by 4 threads over 16 lanes each
+    const int c = threadIdx.x >> 2, k = threadIdx.x & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) s += part[c][k * 16 + t];
+    red[k * NB + c] = s;
+  }
+  __syncthreads();
+  if (q == 0) {
+    const double s = (red[rr] + red[NB + rr]) + (red[2 * NB + rr] + red[3 * NB + rr]);
+    w[rr] = rr < rows ? (double)v[i0 + rr] - s : 0.0;
+  }
+  __syncthreads();
+  const double xi = block_gemv(LiB + (int64_t)I * 2 * NB * NB + NB * NB, w, red);
+  if (q == 0 && rr < rows) x[i0 + rr] = ok ? xi : (double)NAN;
+  publish(&flags[I], epoch);
 }
 
 // y = W (Ax - r_y) with W = -1/d_y; writes [x; y] into out (or adds it).
@@ -298,9 +462,10 @@ struct qtqp_kkt {
   double *vin = nullptr, *vout = nullptr, *vsol = nullptr, *vres = nullptr;
   double *rin = nullptr, *racc = nullptr, *rres = nullptr, *rtmp = nullptr;
   double *tx = nullptr, *ty = nullptr, *tg = nullptr, *tpx = nullptr, *part = nullptr;
-  double* tv = nullptr;    // global fallback for the triangular-solve vector
-  double* linv = nullptr;  // inverse of the current 64x64 diagonal block
-  size_t lds_limit = 0;    // dynamic LDS bytes granted to k_trsv
+  double* tv = nullptr;    // forward-sweep result of the triangular solves
+  double* linv = nullptr;  // inverses of the 64x64 diagonal blocks of the factor
+  unsigned* sync = nullptr;  // tickets and block flags of the triangular sweeps
+  unsigned trsv_epoch = 0;   // completed sweep launches
   unsigned long long* dnorm = nullptr;
   int* dflag = nullptr;
   // state
@@ -377,7 +542,7 @@ int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
                        h->info, reinterpret_cast<T*>(h->linv));
     HIPC(hipGetLastError());
     if (below > 0) {
-      hipLaunchKernelGGL(k_trsm_panel<T>, dim3((below + WG - 1) / WG), dim3(WG), 0, h->stream, G,
+      hipLaunchKernelGGL(k_trsm_panel<T>, dim3((below + NB - 1) / NB), dim3(WG), 0, h->stream, G,
                          (int64_t)n, n, k0, reinterpret_cast<const T*>(h->linv));
       HIPC(hipGetLastError());
       const int tt = (below + NB - 1) / NB;
@@ -417,12 +582,18 @@ int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out
   hipLaunchKernelGGL(k_colsum, dim3(blocks_for(n)), dim3(WG), 0, h->stream, h->part, nblk_rows(m),
                      (int)n, r, h->tg);
   HIPC(hipGetLastError());
-  const size_t full = sizeof(T) * (NB * (NB + 1) + n);
-  const bool in_lds = full <= h->lds_limit;
-  const size_t smem = in_lds ? full : sizeof(T) * NB * (NB + 1);
-  hipLaunchKernelGGL(k_trsv<T>, dim3(1), dim3(WG), smem, h->stream, L, (int64_t)n, (int)n, h->tg,
-                     h->tx, in_lds ? nullptr : reinterpret_cast<T*>(h->tv));
+  const int nt = (int)((n + NB - 1) / NB);
+  const unsigned base = h->trsv_epoch * (unsigned)nt;
+  const unsigned epoch = h->trsv_epoch + 1u;
+  const T* LiB = reinterpret_cast<const T*>(h->linv);
+  T* tv = reinterpret_cast<T*>(h->tv);
+  hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
+                     h->tg, tv, h->sync, base, epoch);
   HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
+                     tv, h->tx, h->sync, base, epoch);
+  HIPC(hipGetLastError());
+  h->trsv_epoch++;
   if ((rc = rowscols<T>(h, A, m, n, h->tx, h->ty, nullptr, nullptr, nullptr))) return rc;
   hipLaunchKernelGGL(k_solve_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m,
                      h->tx, h->ty, r, h->diag, out, accumulate);
@@ -441,9 +612,13 @@ int backend_solve(qtqp_kkt* h, const double* r, double* out) {
   const int64_t N = h->n + h->m;
   if (h->fprec == 64) return solve_current(h, r, out, 0);
   // FP32 factor: refine in FP64 against the exact K_reg.
-  constexpr int kMaxInner = 12;
+  // An FP32 factor of a Gram block with condition number k contracts the
+  // residual by about n * 6e-8 * k per step; past kPoorContraction it would
+  // need too many steps, and an FP64 refactor is cheaper.
+  constexpr int kMaxInner = 40;
   constexpr double kTarget = 1e-15;  // relative residual reached by an FP64 factor
-  constexpr double kPoorContraction = 1e-2;
+  constexpr double kPoorContraction = 0.25;
+  constexpr double kFloor = 1e-13;
   int rc;
   double r0 = 0.0, prev = 0.0, cur = 0.0;
   // ||r|| for the relative target
@@ -464,15 +639,16 @@ int backend_solve(qtqp_kkt* h, const double* r, double* out) {
     if ((rc = kmatvec(h, h->racc, r, nullptr, h->rres, &cur))) return rc;
     if (!(cur == cur)) break;
     if (cur <= kTarget * r0) break;
-    const bool poor = cur > kPoorContraction * prev;
-    if (poor && h->fprec == 32) {
-      // FP32 factor too inaccurate at this conditioning: refactor in FP64.
+    if (cur > kPoorContraction * prev) {
+      // Stalled.  Below kFloor the FP64 residual itself is at its rounding
+      // floor (an FP64 factor would stop there too); above it the FP32
+      // factor is too inaccurate at this conditioning: refactor in FP64.
+      if (cur <= kFloor * r0 || h->fprec == 64) break;
       h->info_acc.fp64_fallbacks++;
       h->force64 = true;
       if ((rc = factor64(h))) return rc;
       continue;  // keep the accumulated correction; refine with the FP64 factor
     }
-    if (cur > 0.5 * prev) break;  // stagnated at the rounding floor
     prev = cur;
   }
   HIPC(hipMemcpyAsync(out, h->racc, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
@@ -490,7 +666,7 @@ void release(qtqp_kkt* h) {
   void* bufs[] = {h->A64, h->Poff64, h->A32, h->Poff32, h->diag, h->corr, h->G64, h->G32,
                   h->info, h->vin, h->vout, h->vsol, h->vres, h->rin, h->racc, h->rres,
                   h->rtmp, h->tx, h->ty, h->tg, h->tpx, h->part, h->dnorm, h->dflag,
-                  h->tv, h->linv};
+                  h->tv, h->linv, h->sync};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -547,20 +723,6 @@ int qtqp_kkt_create(int64_t n, int64_t m, const double* a, const double* p, int3
     return fail(QTQP_ERR_HIP, msg);
   };
   if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad("stream");
-  {
-    int dev = 0, maxsh = 0;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return bad("device");
-    if ((e = hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, dev)) !=
-        hipSuccess)
-      return bad("attribute");
-    h->lds_limit = (size_t)maxsh;
-    if ((e = hipFuncSetAttribute((const void*)k_trsv<double>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, maxsh)) != hipSuccess)
-      return bad("trsv attribute");
-    if ((e = hipFuncSetAttribute((const void*)k_trsv<float>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, maxsh)) != hipSuccess)
-      return bad("trsv attribute");
-  }
   if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return bad("event");
   if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return bad("event");
 #define ALLOC(field, T, count) \
@@ -584,7 +746,9 @@ int qtqp_kkt_create(int64_t n, int64_t m, const double* a, const double* p, int3
   ALLOC(tg, double, n);
   ALLOC(tpx, double, n);
   ALLOC(tv, double, n);
-  ALLOC(linv, double, NB * NB);
+  const int64_t nt = (n + NB - 1) / NB;
+  ALLOC(linv, double, nt * 2 * NB * NB);
+  ALLOC(sync, unsigned, 2 + 2 * nt);
   ALLOC(part, double, (int64_t)nblk_rows(m) * n);
   ALLOC(dnorm, unsigned long long, 1);
   ALLOC(dflag, int, 1);
@@ -605,6 +769,8 @@ int qtqp_kkt_create(int64_t n, int64_t m, const double* a, const double* p, int3
                           h->stream)) != hipSuccess)
     return bad("P");
   if ((e = hipMemsetAsync(h->corr, 0, sizeof(double) * N, h->stream)) != hipSuccess) return bad("corr");
+  if ((e = hipMemsetAsync(h->sync, 0, sizeof(unsigned) * (2 + 2 * nt), h->stream)) != hipSuccess)
+    return bad("sync");
   if (precision == QTQP_PREC_MIXED) {
     hipLaunchKernelGGL(k_cast<float>, dim3(1024), dim3(WG), 0, h->stream, h->A64, h->A32, m * n);
     hipLaunchKernelGGL(k_cast<float>, dim3(256), dim3(WG), 0, h->stream, h->Poff64, h->Poff32, n * n);
