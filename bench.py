@@ -344,6 +344,11 @@ def main():
   m = np.array([q.m for q in qps])
   log(f"generated in {time.perf_counter() - t0:.1f}s; uploading")
   bq = batch.BatchQP(qps)
+  log("autotuning Gram tile shapes per size bucket")
+  t0 = time.perf_counter()
+  tuned = bq.autotune(sample=32, iters=3, precision="mixed")
+  tuned["wall_s"] = time.perf_counter() - t0
+  log(f"autotune {tuned['wall_s']:.2f}s: table {tuned['table']}")
   log("warmup")
   for _ in range(args.warmup):
     bq.solve_device(precision="mixed")
@@ -453,6 +458,7 @@ def main():
       "oracle_check": {"sample": len(errs) // 3, "status_match": bool(status_ok),
                        "max_rel_err_xys": max(errs) if errs else None},
       "counters": cnts[-1],
+      "autotune": tuned,
   }
   bq.free()
   if not args.no_single:

@@ -51,6 +51,9 @@ BATCH_SYMBOLS = {
     "qtqp_batch_launch_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "qtqp_batch_work": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "qtqp_batch_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32]),
+    "qtqp_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts),
+                                           ctypes.c_int32, ctypes.c_int32, _i32p, _dp]),
+    "qtqp_batch_set_tiles": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "qtqp_batch_free": (None, [ctypes.c_void_p]),
     "qtqp_batch_last_error": (ctypes.c_char_p, []),
 }
@@ -176,18 +179,53 @@ class BatchQP:
     _check(self._lib, self._lib.qtqp_batch_set_start(self._h, _ptr(x, _dp), _ptr(y, _dp),
                                                      _ptr(s, _dp), _ptr(t, _dp)))
 
-  def solve_device(self, *, atol=1e-7, rtol=1e-8, atol_infeas=1e-8, rtol_infeas=1e-9,
-                   max_iter=100, step_size_scale=0.99, min_static_regularization=1e-8,
-                   max_iterative_refinement_steps=20, linear_solver_atol=1e-12,
-                   linear_solver_rtol=1e-12, equilibrate=True, precision="mixed",
-                   use_graph=True) -> dict:
-    """Runs the device IPM only (results stay on the GPU); returns counters."""
+  @staticmethod
+  def _opts(*, atol=1e-7, rtol=1e-8, atol_infeas=1e-8, rtol_infeas=1e-9, max_iter=100,
+            step_size_scale=0.99, min_static_regularization=1e-8,
+            max_iterative_refinement_steps=20, linear_solver_atol=1e-12,
+            linear_solver_rtol=1e-12, equilibrate=True, precision="mixed",
+            use_graph=True) -> BatchOpts:
     assert atol >= 0 and rtol >= 0 and atol_infeas >= 0 and rtol_infeas >= 0
     assert max_iter > 0 and 0 < step_size_scale < 1 and max_iterative_refinement_steps >= 1
     prec = {"mixed": _native.PREC_MIXED, "fp64": _native.PREC_FP64}[precision]
-    o = BatchOpts(atol, rtol, atol_infeas, rtol_infeas, step_size_scale,
-                  min_static_regularization, linear_solver_atol, linear_solver_rtol, max_iter,
-                  max_iterative_refinement_steps, prec, int(bool(use_graph)), int(bool(equilibrate)))
+    return BatchOpts(atol, rtol, atol_infeas, rtol_infeas, step_size_scale,
+                     min_static_regularization, linear_solver_atol, linear_solver_rtol, max_iter,
+                     max_iterative_refinement_steps, prec, int(bool(use_graph)),
+                     int(bool(equilibrate)))
+
+  # Gram tile shapes and autotuner size buckets (include/qtqp_batch.h).
+  TILE_SHAPES = ("64x64", "128x128")
+  BUCKET_EDGES = (64, 128, 192, 256, 384, 512, 768, 1024)
+
+  def autotune(self, *, sample: int = 32, iters: int = 3, **opts) -> dict:
+    """Times each Gram tile shape on up to `sample` QPs of every size bucket
+    for `iters` IPM iterations and applies the fastest per bucket.  Returns
+    {"table": shape index per bucket (-1: empty), "ms": bucket x shape
+    timings, "buckets": upper n edges, "shapes": names}.  Discards a start
+    set with set_start."""
+    o = self._opts(**opts)
+    nb, nt = len(self.BUCKET_EDGES), len(self.TILE_SHAPES)
+    table = np.zeros(nb, dtype=np.int32)
+    ms = np.zeros(nb * nt)
+    _check(self._lib, self._lib.qtqp_batch_autotune(self._h, ctypes.byref(o), int(sample),
+                                                    int(iters), _ptr(table, _i32p),
+                                                    _ptr(ms, _dp)))
+    ms_rows = [[float(v) if np.isfinite(v) else None for v in row] for row in ms.reshape(nb, nt)]
+    return {"table": table.tolist(), "ms": ms_rows,
+            "buckets": list(self.BUCKET_EDGES), "shapes": list(self.TILE_SHAPES)}
+
+  def set_tiles(self, table) -> None:
+    """Applies a tile table (one shape index per size bucket)."""
+    t = np.ascontiguousarray(table, dtype=np.int32)
+    assert t.shape == (len(self.BUCKET_EDGES),)
+    _check(self._lib, self._lib.qtqp_batch_set_tiles(self._h, _ptr(t, _i32p)))
+
+  def solve_device(self, **opts) -> dict:
+    """Runs the device IPM only (results stay on the GPU); returns counters.
+    Keyword options as QTQP.solve, plus precision ("mixed" | "fp64") and
+    use_graph."""
+    o = self._opts(**opts)
+    max_iter = o.max_iter
     cnt = BatchCounters()
     _check(self._lib, self._lib.qtqp_batch_solve(self._h, ctypes.byref(o), ctypes.byref(cnt)))
     self._max_iter = max_iter

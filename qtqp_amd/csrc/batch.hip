@@ -49,7 +49,8 @@ enum { I_STATUS = 0, I_ITERS, I_FORCE64, I_FALLBACK, I_F32, I_F64, I_KMV, I_FS32
 
 struct Inst {
   int n, m, z, has_p;
-  int ld, pad_;  // leading dimension of A, A32, P, P32: n rounded up to 4
+  int ld;    // leading dimension of A, A32, P, P32: n rounded up to 4
+  int tile;  // Gram tile shape (QTQP_BATCH_TILE_*), chosen per size bucket
   // raw data
   const double *Araw, *Praw, *braw, *craw;
   // equilibrated data
@@ -278,20 +279,21 @@ __device__ inline double dot(const double* a, const double* b, int len, double* 
 // -------------------------------------------------------------------------
 // Factorisation of G = P_off + diag(d_x) + A' W A (one workgroup).
 // -------------------------------------------------------------------------
-template <typename T>
-__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
+// Lower tiles of the Gram block G = P_off + diag(d_x) + A' W A as
+// (64 MT) x (64 MT) rank-m MFMA tiles.
+template <typename T, int MT>
+__device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, const double* w,
+                                  Smem& sm) {
   const int n = I.n, m = I.m, ld = I.ld;
-  const int nt = (n + NB - 1) / NB;
-  for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -1.0 / I.dg[n + i];
-  __syncthreads();
-  const double* w = I.um;
-  T (*si)[NB + 1] = tile_ptr<T>(sm.u, 0);
-  T (*sj)[NB + 1] = tile_ptr<T>(sm.u, KC);
+  constexpr int TS = 64 * MT;
+  const int nt = (n + TS - 1) / TS;
+  T* si = reinterpret_cast<T*>(sm.u);
+  T* sj = si + KC * (TS + 1);
   for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
     int ti, tj;
     lower_tile(t, &ti, &tj);
-    const int i0 = ti * NB, j0 = tj * NB;
-    RankKTile<T> acc;
+    const int i0 = ti * TS, j0 = tj * TS;
+    RankKTileG<T, MT, MT> acc;
     acc.zero();
     acc.run(A, ld, 0, m, i0, j0, n, n, [w](int k) { return (T)w[k]; }, si, sj);
     acc.for_each(i0, j0, [&](int i, int j, T v) {
@@ -299,6 +301,19 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
         G[(int64_t)j * n + i] = v + ((i == j) ? (T)I.dg[i] : P[(int64_t)j * ld + i]);
     });
   }
+}
+
+template <typename T>
+__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
+  const int n = I.n, m = I.m;
+  for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -1.0 / I.dg[n + i];
+  __syncthreads();
+  // 128 x 128 tiles halve the re-reads of A for large n; they are built for
+  // the FP32 factor only (the FP64 accumulators would not fit in registers).
+  if (sizeof(T) == 4 && I.tile == QTQP_BATCH_TILE_128)
+    gram_tiles<T, 2>(I, A, P, G, I.um, sm);
+  else
+    gram_tiles<T, 1>(I, A, P, G, I.um, sm);
   __syncthreads();
   // diag(G) += 1e-14 * max(diag(G))  (solvers_dense.py:107-112)
   double mx = -INFINITY;
@@ -1094,6 +1109,7 @@ struct qtqp_batch {
   int* d_ints = nullptr;
   size_t smem_iter = 0;
   std::vector<hipEvent_t> iter_ev;  // per-iteration launch timing
+  std::vector<int> order;           // longest-first queue order
   std::vector<double> iter_ms;
 };
 
@@ -1112,6 +1128,30 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+Opts to_opts(const qtqp_batch_opts* opts) {
+  Opts o{};
+  o.atol = opts->atol;
+  o.rtol = opts->rtol;
+  o.atol_inf = opts->atol_infeas;
+  o.rtol_inf = opts->rtol_infeas;
+  o.step_scale = opts->step_size_scale;
+  o.min_reg = opts->min_static_regularization;
+  o.ls_atol = opts->linear_solver_atol;
+  o.ls_rtol = opts->linear_solver_rtol;
+  o.max_iter = opts->max_iter;
+  o.max_ir = opts->max_iterative_refinement_steps;
+  o.precision = opts->precision;
+  return o;
+}
+
+// Autotuner size buckets by n (upper edges; the last covers n <= 1024).
+int size_bucket(int n) {
+  static const int edge[QTQP_BATCH_NBUCKETS] = {64, 128, 192, 256, 384, 512, 768, 1024};
+  for (int b = 0; b < QTQP_BATCH_NBUCKETS; ++b)
+    if (n <= edge[b]) return b;
+  return QTQP_BATCH_NBUCKETS - 1;
+}
 
 // Bump allocator over one device allocation.
 struct Arena {
@@ -1211,7 +1251,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.m = mq;
     I.z = z[q];
     I.ld = (nq + 3) & ~3;
-    I.pad_ = 0;
+    I.tile = QTQP_BATCH_TILE_64;
     const int64_t mld = (int64_t)mq * I.ld, nld = (int64_t)nq * I.ld;
     I.has_p = 0;
     for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
@@ -1292,6 +1332,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     const double cv = (double)m[v] * n[v] * n[v] + (double)n[v] * n[v] * n[v] / 3.0;
     return cu > cv;
   });
+  h->order = order;
   if (hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * count, hipMemcpyHostToDevice,
                      h->stream) != hipSuccess ||
       hipMemcpyAsync(h->dinst, h->hinst.data(), sizeof(Inst) * count, hipMemcpyHostToDevice,
@@ -1328,18 +1369,7 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   if (!h || !opts) return fail(QTQP_ERR_ARG, "NULL argument");
   if (opts->max_iter < 1 || opts->max_iterative_refinement_steps < 1)
     return fail(QTQP_ERR_ARG, "max_iter and max_iterative_refinement_steps must be >= 1");
-  Opts o{};
-  o.atol = opts->atol;
-  o.rtol = opts->rtol;
-  o.atol_inf = opts->atol_infeas;
-  o.rtol_inf = opts->rtol_infeas;
-  o.step_scale = opts->step_size_scale;
-  o.min_reg = opts->min_static_regularization;
-  o.ls_atol = opts->linear_solver_atol;
-  o.ls_rtol = opts->linear_solver_rtol;
-  o.max_iter = opts->max_iter;
-  o.max_ir = opts->max_iterative_refinement_steps;
-  o.precision = opts->precision;
+  const Opts o = to_opts(opts);
   HIPC(hipEventRecord(h->ev0, h->stream));
   HIPC(hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream));
   const size_t smem_prep = sizeof(double) * (WG + 8 + 4 * h->nmax);
@@ -1414,6 +1444,90 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
     counters->device_ms = ms;
   }
   h->have_start = false;
+  return QTQP_OK;
+}
+
+int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t sample,
+                        int32_t iters, int32_t* table, double* ms) {
+  if (!h || !opts) return fail(QTQP_ERR_ARG, "NULL argument");
+  if (sample < 1 || iters < 1) return fail(QTQP_ERR_ARG, "sample and iters must be >= 1");
+  Opts o = to_opts(opts);
+  o.max_iter = std::max(o.max_iter, iters);
+  std::vector<int32_t> choice(QTQP_BATCH_NBUCKETS, -1);
+  std::vector<double> times((size_t)QTQP_BATCH_NBUCKETS * QTQP_BATCH_NTILES, NAN);
+  // QPs of each bucket in longest-first order
+  std::vector<std::vector<int>> members(QTQP_BATCH_NBUCKETS);
+  for (int q : h->order) members[size_bucket(h->n[q])].push_back(q);
+  Inst* tmp = nullptr;
+  int* tmp_order = nullptr;
+  HIPC(hipMalloc(&tmp, sizeof(Inst) * sample));
+  HIPC(hipMalloc(&tmp_order, sizeof(int) * sample));
+  auto release_tmp = [&]() {
+    (void)hipFree(tmp);
+    (void)hipFree(tmp_order);
+  };
+  std::vector<int> ident(sample);
+  std::iota(ident.begin(), ident.end(), 0);
+  hipError_t e = hipMemcpy(tmp_order, ident.data(), sizeof(int) * sample, hipMemcpyHostToDevice);
+  int dev = 0, cus = 0;
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t smem_prep = sizeof(double) * (WG + 8 + 4 * h->nmax);
+  for (int b = 0; b < QTQP_BATCH_NBUCKETS && e == hipSuccess; ++b) {
+    const std::vector<int>& mem = members[b];
+    if (mem.empty()) continue;
+    // an evenly spread sample of the bucket (it holds the longest QPs first)
+    std::vector<Inst> pick;
+    const int k = std::min<int>(sample, (int)mem.size());
+    for (int j = 0; j < k; ++j) pick.push_back(h->hinst[mem[(size_t)j * mem.size() / k]]);
+    double best = INFINITY;
+    for (int v = 0; v < QTQP_BATCH_NTILES && e == hipSuccess; ++v) {
+      for (Inst& I : pick) I.tile = v;
+      if ((e = hipMemcpyAsync(tmp, pick.data(), sizeof(Inst) * k, hipMemcpyHostToDevice,
+                              h->stream)) != hipSuccess)
+        break;
+      if ((e = hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream)) != hipSuccess) break;
+      hipLaunchKernelGGL(k_prepare, dim3(k), dim3(WG), smem_prep, h->stream, tmp, k,
+                         opts->equilibrate ? 1 : 0, 1, o.precision == QTQP_PREC_FP64 ? 1 : 0);
+      const int grid = std::min(k, std::max(1, cus * 2));
+      if ((e = hipEventRecord(h->ev0, h->stream)) != hipSuccess) break;
+      for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+        hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, tmp,
+                           tmp_order, k, h->dqueue, o, h->nmax, h->dcnt);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) break;
+      if ((e = hipEventRecord(h->ev1, h->stream)) != hipSuccess) break;
+      if ((e = hipEventSynchronize(h->ev1)) != hipSuccess) break;
+      float t = 0.f;
+      if ((e = hipEventElapsedTime(&t, h->ev0, h->ev1)) != hipSuccess) break;
+      times[(size_t)b * QTQP_BATCH_NTILES + v] = t;
+      if (t < best) {
+        best = t;
+        choice[b] = v;
+      }
+    }
+  }
+  release_tmp();
+  if (e != hipSuccess) return fail(QTQP_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+  // the timing runs re-initialised the iterates of the sampled QPs
+  h->have_start = false;
+  std::vector<int32_t> apply(choice);
+  for (int32_t& c : apply)
+    if (c < 0) c = QTQP_BATCH_TILE_64;
+  int rc = qtqp_batch_set_tiles(h, apply.data());
+  if (rc) return rc;
+  if (table) std::copy(choice.begin(), choice.end(), table);
+  if (ms) std::copy(times.begin(), times.end(), ms);
+  return QTQP_OK;
+}
+
+int qtqp_batch_set_tiles(qtqp_batch* h, const int32_t* table) {
+  if (!h || !table) return fail(QTQP_ERR_ARG, "NULL argument");
+  for (int b = 0; b < QTQP_BATCH_NBUCKETS; ++b)
+    if (table[b] < 0 || table[b] >= QTQP_BATCH_NTILES) return fail(QTQP_ERR_ARG, "unknown tile shape");
+  for (int q = 0; q < h->count; ++q) h->hinst[q].tile = table[size_bucket(h->n[q])];
+  HIPC(hipMemcpy(h->dinst, h->hinst.data(), sizeof(Inst) * h->count, hipMemcpyHostToDevice));
   return QTQP_OK;
 }
 

@@ -83,88 +83,111 @@ __device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
 // ---------------------------------------------------------------------------
 // Symmetric rank-k tile on MFMA:
 //   acc(j, i) = sum_{k in [kbeg,kend)} X[k*ldx + i] * w(k) * X[k*ldx + j]
-// for the 64x64 tile with row block i0 and column block j0 (global indices,
-// rows >= ilim / cols >= jlim read as zero).  Wave v owns the 32x32 sub-tile
-// j in [j0+32*(v>>1), +32), i in [i0+32*(v&1), +32) as 2x2 MFMA 16x16 blocks:
-// acc[a][b][r] is element j = j0+32*(v>>1)+16a+Mfma<T>::row(lane,r),
-// i = i0+32*(v&1)+16b+(lane&15), so stores into a column-major matrix
-// (index j*ld+i) are contiguous across lanes.
+// for the (64 MI) x (64 MJ) tile with row block i0 and column block j0
+// (global indices, rows >= ilim / cols >= jlim read as zero).  Wave v owns
+// the (32 MI) x (32 MJ) sub-tile i in [i0 + 32 MI (v&1), ...),
+// j in [j0 + 32 MJ (v>>1), ...) as (2 MJ) x (2 MI) MFMA 16x16 blocks:
+// acc[a][b][r] is element j = jb + 16a + Mfma<T>::row(lane,r),
+// i = ib + 16b + (lane&15), so stores into a column-major matrix (index
+// j*ld + i) are contiguous across lanes.  Larger tiles re-read X less
+// (each k-row of X is fetched once per tile) at the cost of registers and
+// of work wasted on partial tiles; the batched engine picks MI x MJ per
+// size bucket with its autotuner.
 // ---------------------------------------------------------------------------
-template <typename T>
-struct RankKTile {
+template <typename T, int MI = 1, int MJ = 1>
+struct RankKTileG {
   using acc_t = typename Mfma<T>::acc_t;
-  acc_t acc[2][2];
+  static constexpr int TI = 64 * MI, TJ = 64 * MJ;  // tile rows (i) and columns (j)
+  static constexpr int BI = 2 * MI, BJ = 2 * MJ;    // 16x16 blocks per wave
+  acc_t acc[BJ][BI];
 
   __device__ inline void zero() {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < BJ; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < BI; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[a][b][r] = T(0);
   }
 
-  // si/sj: LDS staging, KC rows of (NB+1) elements each.  The next k-chunk
-  // is fetched into registers while the MFMAs consume the current one.
+  // si: KC rows of (TI + 1) elements, sj: KC rows of (TJ + 1) elements of
+  // LDS staging.  The next k-chunk is fetched into registers while the
+  // MFMAs consume the current one.
   template <typename WFun>
-  __device__ inline void run(const T* __restrict__ X, int64_t ldx, int kbeg,
-                             int kend, int i0, int j0, int ilim, int jlim,
-                             WFun wfun, T (*si)[NB + 1], T (*sj)[NB + 1]) {
+  __device__ inline void run(const T* __restrict__ X, int64_t ldx, int kbeg, int kend, int i0,
+                             int j0, int ilim, int jlim, WFun wfun, T* si, T* sj) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = tid & 63, kr = tid >> 6;
-    const int jb = 32 * (wave >> 1), ib = 32 * (wave & 1);
-    T pi[KC / 4], pj[KC / 4];
+    const int jb = 32 * MJ * (wave >> 1), ib = 32 * MI * (wave & 1);
+    T pi[KC / 4][MI], pj[KC / 4][MJ];
     auto fetch = [&](int kc) {
 #pragma unroll
       for (int q = 0; q < KC / 4; ++q) {
         const int k = kc + kr + 4 * q;
-        T vi = T(0), vj = T(0);
-        if (k < kend) {
-          const T* row = X + (int64_t)k * ldx;
-          if (i0 + c < ilim) vi = row[i0 + c];
-          if (j0 + c < jlim) vj = row[j0 + c] * wfun(k);
-        }
-        pi[q] = vi;
-        pj[q] = vj;
+        const bool kin = k < kend;
+        const T* row = X + (int64_t)(kin ? k : kbeg) * ldx;
+        const T wk = kin ? wfun(k) : T(0);
+#pragma unroll
+        for (int h = 0; h < MI; ++h) pi[q][h] = (kin && i0 + c + 64 * h < ilim) ? row[i0 + c + 64 * h] : T(0);
+#pragma unroll
+        for (int h = 0; h < MJ; ++h)
+          pj[q][h] = (kin && j0 + c + 64 * h < jlim) ? row[j0 + c + 64 * h] * wk : T(0);
       }
     };
     if (kbeg < kend) fetch(kbeg);
     for (int kc = kbeg; kc < kend; kc += KC) {
 #pragma unroll
       for (int q = 0; q < KC / 4; ++q) {
-        si[kr + 4 * q][c] = pi[q];
-        sj[kr + 4 * q][c] = pj[q];
+#pragma unroll
+        for (int h = 0; h < MI; ++h) si[(kr + 4 * q) * (TI + 1) + c + 64 * h] = pi[q][h];
+#pragma unroll
+        for (int h = 0; h < MJ; ++h) sj[(kr + 4 * q) * (TJ + 1) + c + 64 * h] = pj[q][h];
       }
       __syncthreads();
       if (kc + KC < kend) fetch(kc + KC);
 #pragma unroll
       for (int kk = 0; kk < KC; kk += 4) {
         const int kq = kk + (lane >> 4), cl = lane & 15;
-        T av0 = sj[kq][jb + cl], av1 = sj[kq][jb + 16 + cl];
-        T bv0 = si[kq][ib + cl], bv1 = si[kq][ib + 16 + cl];
-        acc[0][0] = Mfma<T>::mac(av0, bv0, acc[0][0]);
-        acc[0][1] = Mfma<T>::mac(av0, bv1, acc[0][1]);
-        acc[1][0] = Mfma<T>::mac(av1, bv0, acc[1][0]);
-        acc[1][1] = Mfma<T>::mac(av1, bv1, acc[1][1]);
+        T av[BJ], bv[BI];
+#pragma unroll
+        for (int a = 0; a < BJ; ++a) av[a] = sj[kq * (TJ + 1) + jb + 16 * a + cl];
+#pragma unroll
+        for (int b = 0; b < BI; ++b) bv[b] = si[kq * (TI + 1) + ib + 16 * b + cl];
+#pragma unroll
+        for (int a = 0; a < BJ; ++a)
+#pragma unroll
+          for (int b = 0; b < BI; ++b) acc[a][b] = Mfma<T>::mac(av[a], bv[b], acc[a][b]);
       }
       __syncthreads();
     }
+  }
+
+  // Same, with the classic [KC][NB + 1] staging arrays of a 64 x 64 tile.
+  template <typename WFun>
+  __device__ inline void run(const T* __restrict__ X, int64_t ldx, int kbeg, int kend, int i0,
+                             int j0, int ilim, int jlim, WFun wfun, T (*si)[NB + 1],
+                             T (*sj)[NB + 1]) {
+    static_assert(MI == 1 && MJ == 1, "2-D staging arrays describe a 64 x 64 tile");
+    run(X, ldx, kbeg, kend, i0, j0, ilim, jlim, wfun, &si[0][0], &sj[0][0]);
   }
 
   // Calls f(i, j, value) for every accumulated element of this thread.
   template <typename F>
   __device__ inline void for_each(int i0, int j0, F f) const {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int jb = j0 + 32 * (wave >> 1), ib = i0 + 32 * (wave & 1);
+    const int jb = j0 + 32 * MJ * (wave >> 1), ib = i0 + 32 * MI * (wave & 1);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < BJ; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int b = 0; b < BI; ++b)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           f(ib + 16 * b + (lane & 15), jb + 16 * a + Mfma<T>::row(lane, r), acc[a][b][r]);
   }
 };
+
+template <typename T>
+using RankKTile = RankKTileG<T, 1, 1>;
 
 // Maps a linear index over the lower-triangular tiles (ti >= tj) of an
 // nt x nt tile grid to (ti, tj).

@@ -98,3 +98,33 @@ def test_batch_input_validation(require_gpu):
     batch.BatchQP([(np.ones((2, 2)), np.ones(2), np.ones(2), 2, None)])
   with pytest.raises(ValueError):
     batch.BatchQP([(np.ones((2, 2)), np.ones(3), np.ones(2), 0, None)])
+
+
+def test_tile_shapes_give_identical_results_and_autotune(require_gpu):
+  """Gram tile shapes change no result (each G entry sums the same products in
+  the same order), and the autotuner fills one choice per occupied bucket."""
+  qps = problems.uneven_mpc_batch(40, base_seed=3000)
+  bq = batch.BatchQP(qps)
+  nb = len(batch.BatchQP.BUCKET_EDGES)
+  outs = []
+  for shape in range(len(batch.BatchQP.TILE_SHAPES)):
+    bq.set_tiles([shape] * nb)
+    bq.solve_device(precision="mixed")
+    outs.append(bq.raw_iterates())
+  for k in ("x", "y", "s", "tau", "status", "iters"):
+    assert np.array_equal(outs[0][k], outs[1][k]), k
+  tuned = bq.autotune(sample=6, iters=2)
+  occupied = {int(np.searchsorted(batch.BatchQP.BUCKET_EDGES, q.n)) for q in qps}
+  for b, choice in enumerate(tuned["table"]):
+    if b in occupied:
+      assert choice in (0, 1)
+      assert all(v is not None and v > 0 for v in tuned["ms"][b])
+    else:
+      assert choice == -1 and tuned["ms"][b] == [None, None]
+  bq.solve_device(precision="mixed")
+  again = bq.raw_iterates()
+  for k in ("x", "y", "s", "status", "iters"):
+    assert np.array_equal(outs[0][k], again[k]), k
+  with pytest.raises(ValueError):
+    bq.set_tiles([5] * nb)
+  bq.free()
