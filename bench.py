@@ -113,10 +113,11 @@ def algorithmic_work(n, m, counts, iters):
 
   Per QP, from the engine's own counters: each factorisation costs
   m n^2 + n^3/3 flops (Gram product + LDL') and reads A once (FP32 or
-  FP64); each K_reg product (FP64 refinement residual) reads A and P once in
-  FP64; each factor solve reads A twice and the triangular factor twice
-  (n^2/2 entries each way) in the factor's precision; each IPM iteration's
-  termination test reads A and P once in FP64.
+  FP64); each refinement pass (the rest of a correction, the FP64 residual
+  and the next right-hand side, fused) reads A and P once in FP64; each
+  factor solve reads the triangular factor twice (n^2/2 entries each way)
+  in the factor's precision; each IPM iteration's termination test reads A
+  and P once in FP64.
   """
   f32, f64, kmv, fs32, fs64 = (counts[:, k].astype(np.float64) for k in range(5))
   it = iters.astype(np.float64)
@@ -125,7 +126,7 @@ def algorithmic_work(n, m, counts, iters):
   flops32 = float(np.sum(f32 * fac_flops))
   flops64 = float(np.sum(f64 * fac_flops))
   bytes_ = np.sum(f32 * 4 * mn + f64 * 8 * mn + (kmv + it) * (8 * mn + 8 * nn)
-                  + fs32 * (8 * mn + 4 * nn) + fs64 * (16 * mn + 8 * nn))
+                  + fs32 * 4 * nn + fs64 * 8 * nn)
   return flops32, flops64, float(bytes_)
 
 

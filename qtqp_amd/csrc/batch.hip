@@ -62,6 +62,7 @@ struct Inst {
   double* LiB;  // per 64-block of the factor: L_bb^{-1} column-major then row-major
   double *x, *y, *s, *kq, *dg, *corr, *krhs, *sol, *res, *iacc, *ires, *r, *xp;
   double *um, *un, *ds, *cm;
+  double* irb;  // refinement state, two copies: A x (m), A'y, P_off x, g (n each)
   double* sd;
   int* si;
   double* trace;  // kTraceFields doubles per IPM iteration (max_iter rows)
@@ -447,76 +448,201 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
 }
 
 // -------------------------------------------------------------------------
-// KKT products and solves.
+// KKT solves with iterative refinement, one pass over A per step.
 // -------------------------------------------------------------------------
 
-// out = b - K_reg v + corr*v   (b == nullptr: out = K_reg v); returns
-// ||out||_inf with NaN reported as inf.  One pass over A and one over P.
-__device__ __noinline__ double kmv(const Inst& I, const double* v, const double* b,
-                                   const double* corr, double* out, Smem& sm) {
+// One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
+// zero-padded to even; ld <= 128 NS): for every row i,
+//   (c1_i, c2_i) = rowepi(i, M[i,:] . xs, writer, &c1, &c2)
+// and then colepi(j, sum_i M[i][j] c1_i, sum_i M[i][j] c2_i).  A wavefront
+// takes 8 rows at a time: it reads them with 16-byte loads while forming the
+// dot products, completes those with reduce8, evaluates the epilogue (the 8
+// lanes holding a row's total compute it; `writer` is true for one of
+// them), broadcasts the coefficients, and reads the rows again (from cache)
+// for the column sums.  scratch: >= 8 * ncol doubles of LDS.
+template <int NS, typename RE, typename CE>
+__device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
+                                const double* __restrict__ xs, RE rowepi, CE colepi,
+                                double* scratch) {
+  using V = VecT<double, 2>::type;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double xr[NS][2], c1a[NS][2], c2a[NS][2];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int col = s * 128 + lane * 2 + e;
+      xr[s][e] = col < ncol ? xs[col] : 0.0;
+      c1a[s][e] = 0.0;
+      c2a[s][e] = 0.0;
+    }
+  for (int i0 = wave * RPW; i0 < nrow; i0 += 4 * RPW) {
+    double d[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) d[r] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int col0 = s * 128 + lane * 2;
+      if (col0 < ld) {
+        V a[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+          a[r] = (i0 + r < nrow) ? *reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0)
+                                 : V(0);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) d[r] += a[r][0] * xr[s][0] + a[r][1] * xr[s][1];
+      }
+    }
+    reduce8(d, lane);
+    const int row = i0 + 4 * ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 3) & 1);
+    double c1 = 0.0, c2 = 0.0;
+    if (row < nrow) rowepi(row, d[0], (lane & 7) == 0, &c1, &c2);
+    double k1[RPW], k2[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      const int src = 32 * ((r >> 2) & 1) + 16 * ((r >> 1) & 1) + 8 * (r & 1);
+      k1[r] = __shfl(c1, src, 64);
+      k2[r] = __shfl(c2, src, 64);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int col0 = s * 128 + lane * 2;
+      if (col0 < ld) {
+        V a[RPW];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+          a[r] = (i0 + r < nrow) ? *reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0)
+                                 : V(0);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            c1a[s][e] += a[r][e] * k1[r];
+            c2a[s][e] += a[r][e] * k2[r];
+          }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int col = s * 128 + lane * 2 + e;
+      if (col < ncol) {
+        scratch[wave * ncol + col] = c1a[s][e];
+        scratch[(4 + wave) * ncol + col] = c2a[s][e];
+      }
+    }
+  __syncthreads();
+  for (int j = threadIdx.x; j < ncol; j += WG)
+    colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]),
+           (scratch[4 * ncol + j] + scratch[5 * ncol + j]) +
+               (scratch[6 * ncol + j] + scratch[7 * ncol + j]));
+  __syncthreads();
+}
+
+// State of a refined solve for the current iterate sol: the products
+// A sol_x (m), A' sol_y, P_off sol_x (n), the residual
+// r = krhs - K_reg sol + corr .* sol (N) and the factor-solve right-hand
+// side g = r_x + A'(W r_y), W = -1/d_y (n) of the next correction.
+struct IrState {
+  double *sol, *ax, *aty, *px, *r, *g;
+};
+
+// Fills S1 from S1.sol (init), or sets S1 = S0 advanced by the correction
+// whose x-part is xc:  y_c = W (A x_c - r0_y), sol1 = sol0 + (x_c, y_c),
+// with the products updated incrementally.  One pass over A (both A x and
+// A' y, A' u in it), one over P.  Returns ||r1||_inf (NaN reported as inf).
+template <int NS>
+__device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const IrState S1,
+                                       const double* xc, bool init, Smem& sm) {
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
-  auto fin = [&](int idx, double k) {
-    if (!b) return k;
-    double r = b[idx] - k;
-    if (corr) r += corr[idx] * v[idx];
-    return r;
-  };
-  // y-block A v_x + d_y v_y and, from the same rows, A' v_y (into un)
-  rowpass<double, 2, true, true>(
-      I.A, m, n, I.ld, v, v + n,
-      [&](int i, double t) { out[n + i] = fin(n + i, t + I.dg[n + i] * v[n + i]); },
-      [&](int j, double s) { I.un[j] = s; }, scratch);
-  // x-block P_off v_x + d_x v_x + A' v_y  (P symmetric: row j of P = column j)
-  rowpass<double, 2, true, false>(
-      I.P, n, n, I.ld, v, nullptr,
-      [&](int j, double t) {
-        out[j] = fin(j, I.un[j] + (t - I.P[(int64_t)j * I.ld + j] * v[j]) + I.dg[j] * v[j]);
-      },
-      [](int, double) {}, scratch);
+  const double* dy = I.dg + n;
+  const double* ky = I.krhs + n;
+  const double* cy = I.corr + n;
   double mx = 0.0;
-  for (int k = threadIdx.x; k < n + m; k += WG) {
-    const double a = fabs(out[k]);
+  rowpass2<NS>(
+      I.A, m, n, I.ld, init ? S1.sol : xc,
+      [&](int i, double t, bool writer, double* c1, double* c2) {
+        double yv, axv, yc = 0.0;
+        if (init) {
+          yv = S1.sol[n + i];
+          axv = t;
+        } else {
+          yc = -(t - S0.r[n + i]) / dy[i];
+          yv = S0.sol[n + i] + yc;
+          axv = S0.ax[i] + t;
+        }
+        const double ry = ky[i] - (axv + dy[i] * yv) + cy[i] * yv;
+        *c1 = init ? yv : yc;
+        *c2 = -ry / dy[i];
+        if (writer) {
+          if (!init) S1.sol[n + i] = yv;
+          S1.ax[i] = axv;
+          S1.r[n + i] = ry;
+          const double a = fabs(ry);
+          mx = fmax(mx, (a == a) ? a : INFINITY);
+        }
+      },
+      [&](int j, double s1, double s2) {
+        S1.aty[j] = init ? s1 : S0.aty[j] + s1;
+        I.um[j] = s2;  // A'(W r1_y)
+      },
+      scratch);
+  if (I.has_p) {
+    rowpass<double, 2, true, false>(
+        I.P, n, n, I.ld, init ? S1.sol : xc, nullptr,
+        [&](int j, double t) {
+          const double v = t - I.P[(int64_t)j * I.ld + j] * (init ? S1.sol[j] : xc[j]);
+          S1.px[j] = init ? v : S0.px[j] + v;
+        },
+        [](int, double) {}, scratch);
+  } else {
+    for (int j = threadIdx.x; j < n; j += WG) S1.px[j] = 0.0;
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < n; j += WG) {
+    const double xv = init ? S1.sol[j] : S0.sol[j] + xc[j];
+    if (!init) S1.sol[j] = xv;
+    const double rx = I.krhs[j] - (S1.px[j] + I.dg[j] * xv + S1.aty[j]) + I.corr[j] * xv;
+    S1.r[j] = rx;
+    S1.g[j] = rx + I.um[j];
+    const double a = fabs(rx);
     mx = fmax(mx, (a == a) ? a : INFINITY);
   }
   return bmax(mx, sm.red);
 }
 
-// out = K_reg^{-1} r with the factor in precision T:
-// g = r_x + A'(W r_y);  G x = g;  y = W (A x - r_y),  W = -1/d_y.
-template <typename T>
-__device__ __noinline__ void fsolve(const Inst& I, const T* A, const T* L, const T* LiB,
-                                    const double* r, double* out, Smem& sm) {
-  const int n = I.n, m = I.m;
-  if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
-  for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -r[n + i] / I.dg[n + i];
-  __syncthreads();
-  T* v = reinterpret_cast<T*>(sm.v);
-  double* scratch = reinterpret_cast<double*>(sm.u);
-  rowpass<T, VW<T>::value, false, true>(A, m, n, I.ld, nullptr, I.um, [](int, double) {},
-                                        [&](int j, double s) { v[j] = (T)(r[j] + s); }, scratch);
-  trsv_lower_inv(L, n, LiB, v, sm.red);
-  trsv_upper_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
-  for (int j = threadIdx.x; j < n; j += WG) {
-    const double xv = (double)v[j];
-    I.un[j] = xv;
-    out[j] = xv;
-  }
-  __syncthreads();
-  rowpass<T, VW<T>::value, true, false>(
-      A, m, n, I.ld, I.un, nullptr,
-      [&](int i, double t) { out[n + i] = -(t - r[n + i]) / I.dg[n + i]; }, [](int, double) {},
-      scratch);
+__device__ inline double ir_pass_any(const Inst& I, const IrState& S0, const IrState& S1,
+                                     const double* xc, bool init, Smem& sm) {
+  if (I.ld <= 256) return ir_pass<2>(I, S0, S1, xc, init, sm);
+  if (I.ld <= 512) return ir_pass<4>(I, S0, S1, xc, init, sm);
+  return ir_pass<8>(I, S0, S1, xc, init, sm);
 }
 
-// out = K_reg^{-1} r with the instance's current factor.
-__device__ inline void fsolve_current(const Inst& I, const double* r, double* out, Smem& sm) {
+// xc = G^{-1} g with the factor in precision T.
+template <typename T>
+__device__ __noinline__ void factor_solve(const Inst& I, const T* L, const T* LiB, const double* g,
+                                          double* xc, Smem& sm) {
+  const int n = I.n;
+  if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
+  T* v = reinterpret_cast<T*>(sm.v);
+  for (int j = threadIdx.x; j < n; j += WG) v[j] = (T)g[j];
+  __syncthreads();
+  trsv_lower_inv(L, n, LiB, v, sm.red);
+  trsv_upper_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
+  for (int j = threadIdx.x; j < n; j += WG) xc[j] = (double)v[j];
+  __syncthreads();
+}
+
+__device__ inline void factor_solve_current(const Inst& I, const double* g, double* xc, Smem& sm) {
   if (I.si[I_FORCE64])
-    fsolve<double>(I, I.A, I.G, I.LiB, r, out, sm);
+    factor_solve<double>(I, I.G, I.LiB, g, xc, sm);
   else
-    fsolve<float>(I, I.A32, reinterpret_cast<const float*>(I.G),
-                  reinterpret_cast<const float*>(I.LiB), r, out, sm);
+    factor_solve<float>(I, reinterpret_cast<const float*>(I.G),
+                        reinterpret_cast<const float*>(I.LiB), g, xc, sm);
 }
 
 // Replaces an FP32 factor that refines too slowly by an FP64 one.
@@ -541,14 +667,18 @@ struct IrResult {
 // DirectKktSolver.solve (direct.py:243-307) on K_true = K_reg - diag(corr):
 // sol from warm, each step sol += K_reg^{-1} res with the current factor and
 // res recomputed in FP64, until res < atol + rtol ||rhs||, a stall, or
-// max_steps.  An FP32 step that contracts the residual by less than
-// kPoorContraction is undone and the factor is redone in FP64 (it does not
-// count as a step); FP64 steps follow the reference rule exactly.
+// max_steps.  A step is: the triangular solves for x_c from g, then one
+// ir_pass that completes the correction (y_c), advances sol and the
+// products, and forms the next residual and g.  An FP32 step that contracts
+// the residual by less than kPoorContraction is dropped and the factor is
+// redone in FP64 (it does not count as a step); FP64 steps follow the
+// reference rule exactly.  rhs may alias I.r (it is consumed into krhs
+// first).
 constexpr double kPoorContraction = 3e-2;
 
 __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm,
                                           double* sol, const Opts o, Smem& sm, Counters* cnt) {
-  const int n = I.n, N = I.n + I.m;
+  const int n = I.n, m = I.m, N = I.n + I.m;
   double mx = 0.0;
   for (int k = threadIdx.x; k < N; k += WG) {
     const double v = k < n ? rhs[k] : -rhs[k];
@@ -559,38 +689,33 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   }
   const double rhs_norm = bmax(mx, sm.red);
   const double tol = o.ls_atol + o.ls_rtol * rhs_norm;
-  double* rc = I.res;   // current residual
-  double* rn = I.ires;  // trial residual
-  double res = kmv(I, sol, I.krhs, I.corr, rc, sm);
+  double* w = I.irb;
+  IrState S0{sol, w, w + 2 * m, w + 2 * m + 2 * n, I.res, w + 2 * m + 4 * n};
+  IrState S1{I.r, w + m, w + 2 * m + n, w + 2 * m + 3 * n, I.ires, w + 2 * m + 5 * n};
+  double* xc = I.iacc;
+  double res = ir_pass_any(I, S1, S0, nullptr, true, sm);
   IrResult out{0, 1, res, true};
   int solves = 0;
   while (solves < o.max_ir) {
     const double old = res;
     const bool f32 = I.si[I_FORCE64] == 0;
     if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
-    fsolve_current(I, rc, I.iacc, sm);
-    for (int k = threadIdx.x; k < N; k += WG) {
-      if (f32) I.r[k] = sol[k];  // backup: the caller's rhs was consumed into krhs
-      sol[k] += I.iacc[k];
-    }
-    __syncthreads();
-    const double nr = kmv(I, sol, I.krhs, I.corr, rn, sm);
+    factor_solve_current(I, S0.g, xc, sm);
+    const double nr = ir_pass_any(I, S0, S1, xc, false, sm);
     if (f32) {
       if (threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
       if (!(nr < tol) && !(nr <= kPoorContraction * old)) {
-        for (int k = threadIdx.x; k < N; k += WG) sol[k] = I.r[k];
-        __syncthreads();
         if (!fallback_fp64(I, sm, cnt)) {
           out.ok = false;
           return out;
         }
-        continue;
+        continue;  // S0 is untouched: redo the step with the FP64 factor
       }
     }
     ++solves;
-    double* t = rc;
-    rc = rn;
-    rn = t;
+    const IrState t = S0;
+    S0 = S1;
+    S1 = t;
     res = nr;
     if (res < tol) {
       out.status = 0;
@@ -600,6 +725,10 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
       out.status = 2;
       break;
     }
+  }
+  if (S0.sol != sol) {
+    for (int k = threadIdx.x; k < N; k += WG) sol[k] = S0.sol[k];
+    __syncthreads();
   }
   out.solves = max(solves, 1);
   out.res = res;
@@ -1180,6 +1309,7 @@ size_t inst_bytes(int n, int m) {
   b += 3 * align256(8 * std::max<int64_t>(m, n));  // x/y/s slack (x n, y m, s m)
   b += 10 * align256(8 * N);                       // kq dg corr krhs sol res iacc ires r xp
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
+  b += align256(8 * (2 * (int64_t)m + 6 * (int64_t)n));  // refinement state
   b += align256(8 * S_NSCAL) + align256(4 * I_NINT);
   b += align256(8 * kTraceFields * kTraceRows);
   return b + 4096;
@@ -1290,6 +1420,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.un = ar.take<double>(mx);
     I.ds = ar.take<double>(mx);
     I.cm = ar.take<double>(mx);
+    I.irb = ar.take<double>(2 * (int64_t)mq + 6 * (int64_t)nq);
     I.sd = ar.take<double>(S_NSCAL);
     I.si = ar.take<int>(I_NINT);
     I.trace = ar.take<double>(kTraceFields * kTraceRows);
