@@ -92,6 +92,28 @@ __device__ inline Smem carve(double* base, int nmax) {
   return s;
 }
 
+// The IPM kernel's LDS: one static array (77 KB, 2 workgroups per CU) laid
+// out as {union scratch of two 64 x 65 tiles + 64 | vector of n <= 1024 |
+// reduction slots}.  Being a named LDS object, every __noinline__ phase can
+// re-derive its pointers from it, so their accesses compile to LDS
+// instructions: a pointer that arrives through a reference is generic and
+// would be accessed with flat instructions, whose completion is counted
+// with the global loads (an LDS wait would then drain every prefetch).
+constexpr int kLdsU = 2 * NB * (NB + 1) + NB;  // doubles
+constexpr int kLdsV = 1024;
+constexpr int kLdsRed = WG + 8;
+__shared__ double g_lds[kLdsU + kLdsV + kLdsRed];
+
+__device__ inline Smem lds_smem() {
+  Smem s;
+  s.u = reinterpret_cast<unsigned char*>(g_lds);
+  s.v = g_lds + kLdsU;
+  s.red = g_lds + kLdsU + kLdsV;
+  return s;
+}
+
+__device__ inline Smem lds_view(const Smem&) { return lds_smem(); }
+
 template <typename T>
 __device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
   return reinterpret_cast<T (*)[NB + 1]>(p) + row_off;
@@ -176,7 +198,7 @@ __device__ inline void rowpass(const T* __restrict__ M, int nrow, int ncol, int 
         V a[RPW];
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
-          a[r] = (i0 + r < nrow) ? *reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0)
+          a[r] = (i0 + r < nrow) ? ldg(reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0))
                                  : V(0);
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
@@ -243,7 +265,7 @@ __device__ inline void rowpass_absmax(const double* __restrict__ M, int nrow, in
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
           if (i0 + r < nrow) {
-            const double a = fabs(M[(int64_t)(i0 + r) * ncol + j]);
+            const double a = fabs(ldg(M + (int64_t)(i0 + r) * ncol + j));
             d[r] = fmax(d[r], a * wcr[c]);
             cm[c] = fmax(cm[c], a * ww[r]);
           }
@@ -305,7 +327,8 @@ __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, c
 }
 
 template <typename T>
-__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm) {
+__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
   for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -1.0 / I.dg[n + i];
   __syncthreads();
@@ -328,7 +351,8 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
 // Blocked right-looking LDL' (Cholesky form) in place; returns breakdown
 // column (1-based) or 0.
 template <typename T>
-__device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm) {
+__device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   __shared__ int info;
   if (threadIdx.x == 0) info = 0;
   __syncthreads();
@@ -384,7 +408,7 @@ __device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* re
     const T* LiT = LiB + (int64_t)(j0 / NB) * 2 * NB * NB;  // LiT[k*64 + i] = Li[i][k]
     double acc = 0.0;
     if (lane < nb)
-      for (int k = p; k < nb; k += 4) acc += (double)LiT[k * NB + lane] * (double)v[j0 + k];
+      for (int k = p; k < nb; k += 4) acc += (double)ldg(LiT + k * NB + lane) * (double)v[j0 + k];
     red[p * NB + lane] = acc;
     __syncthreads();
     if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
@@ -392,7 +416,7 @@ __device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* re
     for (int i = j0 + nb + tid; i < n; i += WG) {
       const T* col = L + (int64_t)j0 * n + i;
       double a = 0.0;
-      for (int c = 0; c < nb; ++c) a += (double)col[(int64_t)c * n] * (double)v[j0 + c];
+      for (int c = 0; c < nb; ++c) a += (double)ldg(col + (int64_t)c * n) * (double)v[j0 + c];
       v[i] = (T)((double)v[i] - a);
     }
     __syncthreads();
@@ -410,7 +434,7 @@ __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* re
     double acc = 0.0;
     if (lane < nb) {
       const T* col = L + (int64_t)(j0 + lane) * n;
-      for (int i = j0 + nb + p; i < n; i += 4) acc += (double)col[i] * (double)v[i];
+      for (int i = j0 + nb + p; i < n; i += 4) acc += (double)ldg(col + i) * (double)v[i];
     }
     red[p * NB + lane] = acc;
     __syncthreads();
@@ -419,7 +443,7 @@ __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* re
     __syncthreads();
     acc = 0.0;
     if (lane < nb)
-      for (int k = p; k < nb; k += 4) acc += (double)Li[k * NB + lane] * rb[k];
+      for (int k = p; k < nb; k += 4) acc += (double)ldg(Li + k * NB + lane) * rb[k];
     red[p * NB + lane] = acc;
     __syncthreads();
     if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
@@ -429,7 +453,8 @@ __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* re
 
 // Factor in the instance's current precision; FP32 breakdown falls back to
 // FP64.  Returns false on an FP64 breakdown.
-__device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
+__device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
+  Smem sm = lds_view(sm_in);
   if (!I.si[I_FORCE64]) {
     float* G = reinterpret_cast<float*>(I.G);
     if (threadIdx.x == 0) I.si[I_F32] += 1;
@@ -451,20 +476,48 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm, Counters* cnt) {
 // KKT solves with iterative refinement, one pass over A per step.
 // -------------------------------------------------------------------------
 
+// Completes R per-lane partial sums across the wavefront (R = 2, 4 or 8): a
+// transpose-reduce that halves the rows a lane carries at each of log2(R)
+// exchanges, then sums the rest.  Lane l ends with the total of row
+// l >> (6 - log2 R) in d[0].
+template <int R>
+__device__ inline void reduce_rows(double* d) {
+  const int lane = threadIdx.x & 63;
+  int cnt = R;
+#pragma unroll
+  for (int bit = 32; bit >= 64 / R; bit >>= 1) {
+    const int half = cnt / 2;
+    const bool hi = lane & bit;
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      if (q < half) {
+        const double send = hi ? d[q] : d[q + half];
+        const double keep = hi ? d[q + half] : d[q];
+        d[q] = keep + __shfl_xor(send, bit, 64);
+      }
+    }
+    cnt = half;
+  }
+#pragma unroll
+  for (int bit = 32 / R; bit >= 1; bit >>= 1) d[0] += __shfl_xor(d[0], bit, 64);
+}
+
 // One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
 // zero-padded to even; ld <= 128 NS): for every row i,
 //   (c1_i, c2_i) = rowepi(i, M[i,:] . xs, writer, &c1, &c2)
 // and then colepi(j, sum_i M[i][j] c1_i, sum_i M[i][j] c2_i).  A wavefront
-// takes 8 rows at a time: it reads them with 16-byte loads while forming the
-// dot products, completes those with reduce8, evaluates the epilogue (the 8
+// takes R rows at a time and keeps them in registers: it forms the dot
+// products, completes them with reduce_rows, evaluates the epilogue (the
 // lanes holding a row's total compute it; `writer` is true for one of
-// them), broadcasts the coefficients, and reads the rows again (from cache)
-// for the column sums.  scratch: >= 8 * ncol doubles of LDS.
-template <int NS, typename RE, typename CE>
+// them), broadcasts the coefficients and accumulates the column sums from
+// the same registers.  The next R rows are fetched while the current ones
+// are processed.  scratch: >= 8 * ncol doubles of LDS.
+template <int NS, int R, typename RE, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
                                 const double* __restrict__ xs, RE rowepi, CE colepi,
                                 double* scratch) {
   using V = VecT<double, 2>::type;
+  constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double xr[NS][2], c1a[NS][2], c2a[NS][2];
 #pragma unroll
@@ -476,52 +529,48 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       c1a[s][e] = 0.0;
       c2a[s][e] = 0.0;
     }
-  for (int i0 = wave * RPW; i0 < nrow; i0 += 4 * RPW) {
-    double d[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) d[r] = 0.0;
+  V cur[R][NS], nxt[R][NS];
+  auto fetch = [&](int i0, V (*dst)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       const int col0 = s * 128 + lane * 2;
-      if (col0 < ld) {
-        V a[RPW];
 #pragma unroll
-        for (int r = 0; r < RPW; ++r)
-          a[r] = (i0 + r < nrow) ? *reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0)
-                                 : V(0);
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) d[r] += a[r][0] * xr[s][0] + a[r][1] * xr[s][1];
-      }
+      for (int r = 0; r < R; ++r)
+        dst[r][s] = (col0 < ld && i0 + r < nrow)
+                        ? ldg(reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0))
+                        : V(0);
     }
-    reduce8(d, lane);
-    const int row = i0 + 4 * ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 3) & 1);
+  };
+  fetch(wave * R, cur);
+  for (int i0 = wave * R; i0 < nrow; i0 += 4 * R) {
+    if (i0 + 4 * R < nrow) fetch(i0 + 4 * R, nxt);
+    double d[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      d[r] = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) d[r] += cur[r][s][0] * xr[s][0] + cur[r][s][1] * xr[s][1];
+    }
+    reduce_rows<R>(d);
+    const int row = i0 + (lane >> (6 - LR));
     double c1 = 0.0, c2 = 0.0;
-    if (row < nrow) rowepi(row, d[0], (lane & 7) == 0, &c1, &c2);
-    double k1[RPW], k2[RPW];
+    if (row < nrow) rowepi(row, d[0], (lane & ((64 >> LR) - 1)) == 0, &c1, &c2);
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      const int src = 32 * ((r >> 2) & 1) + 16 * ((r >> 1) & 1) + 8 * (r & 1);
-      k1[r] = __shfl(c1, src, 64);
-      k2[r] = __shfl(c2, src, 64);
+    for (int r = 0; r < R; ++r) {
+      const double k1 = __shfl(c1, r << (6 - LR), 64);
+      const double k2 = __shfl(c2, r << (6 - LR), 64);
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          c1a[s][e] += cur[r][s][e] * k1;
+          c2a[s][e] += cur[r][s][e] * k2;
+        }
     }
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int col0 = s * 128 + lane * 2;
-      if (col0 < ld) {
-        V a[RPW];
+    for (int r = 0; r < R; ++r)
 #pragma unroll
-        for (int r = 0; r < RPW; ++r)
-          a[r] = (i0 + r < nrow) ? *reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0)
-                                 : V(0);
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-#pragma unroll
-          for (int e = 0; e < 2; ++e) {
-            c1a[s][e] += a[r][e] * k1[r];
-            c2a[s][e] += a[r][e] * k2[r];
-          }
-      }
-    }
+      for (int s = 0; s < NS; ++s) cur[r][s] = nxt[r][s];
   }
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -553,9 +602,10 @@ struct IrState {
 // whose x-part is xc:  y_c = W (A x_c - r0_y), sol1 = sol0 + (x_c, y_c),
 // with the products updated incrementally.  One pass over A (both A x and
 // A' y, A' u in it), one over P.  Returns ||r1||_inf (NaN reported as inf).
-template <int NS>
+template <int NS, int R>
 __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const IrState S1,
-                                       const double* xc, bool init, Smem& sm) {
+                                       const double* xc, bool init, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
@@ -563,7 +613,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   const double* ky = I.krhs + n;
   const double* cy = I.corr + n;
   double mx = 0.0;
-  rowpass2<NS>(
+  rowpass2<NS, R>(
       I.A, m, n, I.ld, init ? S1.sol : xc,
       [&](int i, double t, bool writer, double* c1, double* c2) {
         double yv, axv, yc = 0.0;
@@ -617,15 +667,18 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
 
 __device__ inline double ir_pass_any(const Inst& I, const IrState& S0, const IrState& S1,
                                      const double* xc, bool init, Smem& sm) {
-  if (I.ld <= 256) return ir_pass<2>(I, S0, S1, xc, init, sm);
-  if (I.ld <= 512) return ir_pass<4>(I, S0, S1, xc, init, sm);
-  return ir_pass<8>(I, S0, S1, xc, init, sm);
+  // rows in flight per wavefront x 128-column slots kept within 16 vectors
+  if (I.ld <= 128) return ir_pass<1, 8>(I, S0, S1, xc, init, sm);
+  if (I.ld <= 256) return ir_pass<2, 8>(I, S0, S1, xc, init, sm);
+  if (I.ld <= 512) return ir_pass<4, 4>(I, S0, S1, xc, init, sm);
+  return ir_pass<8, 2>(I, S0, S1, xc, init, sm);
 }
 
 // xc = G^{-1} g with the factor in precision T.
 template <typename T>
 __device__ __noinline__ void factor_solve(const Inst& I, const T* L, const T* LiB, const double* g,
-                                          double* xc, Smem& sm) {
+                                          double* xc, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n;
   if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
   T* v = reinterpret_cast<T*>(sm.v);
@@ -637,7 +690,8 @@ __device__ __noinline__ void factor_solve(const Inst& I, const T* L, const T* Li
   __syncthreads();
 }
 
-__device__ inline void factor_solve_current(const Inst& I, const double* g, double* xc, Smem& sm) {
+__device__ inline void factor_solve_current(const Inst& I, const double* g, double* xc, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   if (I.si[I_FORCE64])
     factor_solve<double>(I, I.G, I.LiB, g, xc, sm);
   else
@@ -646,7 +700,8 @@ __device__ inline void factor_solve_current(const Inst& I, const double* g, doub
 }
 
 // Replaces an FP32 factor that refines too slowly by an FP64 one.
-__device__ bool fallback_fp64(const Inst& I, Smem& sm, Counters* cnt) {
+__device__ inline bool fallback_fp64(const Inst& I, Smem& sm_in, Counters* cnt) {
+  Smem sm = lds_view(sm_in);
   if (threadIdx.x == 0) {
     I.si[I_FORCE64] = 1;
     I.si[I_FALLBACK] += 1;
@@ -677,7 +732,8 @@ struct IrResult {
 constexpr double kPoorContraction = 3e-2;
 
 __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm,
-                                          double* sol, const Opts o, Smem& sm, Counters* cnt) {
+                                          double* sol, const Opts o, Smem& sm_in, Counters* cnt) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m, N = I.n + I.m;
   double mx = 0.0;
   for (int k = threadIdx.x; k < N; k += WG) {
@@ -752,7 +808,8 @@ __device__ double max_step(const double* v, const double* dv_a, const double* dv
 }
 
 // P u for the n-vector u into out (has_p only).
-__device__ void pmul(const Inst& I, const double* u, double* out, Smem& sm) {
+__device__ inline void pmul(const Inst& I, const double* u, double* out, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   rowpass<double, 2, true, false>(I.P, I.n, I.n, I.ld, u, nullptr,
                                   [&](int j, double t) { out[j] = t; }, [](int, double) {},
                                   reinterpret_cast<double*>(sm.u));
@@ -760,7 +817,8 @@ __device__ void pmul(const Inst& I, const double* u, double* out, Smem& sm) {
 
 // Quadratic tau (__init__.py:759-817); returns false where it raises.
 __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double mu_t,
-                              double r_tau, double* tau_out, Smem& sm) {
+                              double r_tau, double* tau_out, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
   double kqmax = 0.0;
   for (int k = threadIdx.x; k < n + m; k += WG) kqmax = fmax(kqmax, fabs(kr[k]));
@@ -804,7 +862,8 @@ __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, doub
 
 // Linearised tau with trust region (__init__.py:819-857).
 __device__ __noinline__ double tau_linearized(const Inst& I, const double* kr, double mu, double mu_t,
-                                 double tau_c, double tau_anchor, Smem& sm) {
+                                 double tau_c, double tau_anchor, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
   double x_px = 0.0, px_kq = 0.0, px_kr = 0.0;
   if (I.has_p) {
@@ -829,8 +888,9 @@ __device__ __noinline__ double tau_linearized(const Inst& I, const double* kr, d
 // Newton step (__init__.py:708-757): writes [x+; y+] into outxy, returns tau+.
 __device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
                        double tau_anchor, const double* corr_m, double tau, double* outxy,
-                       double* tau_plus, const Opts o, Smem& sm, Counters* cnt,
+                       double* tau_plus, const Opts o, Smem& sm_in, Counters* cnt,
                        IrResult* irout) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, z = I.z, N = I.n + I.m;
   for (int k = threadIdx.x; k < N; k += WG) {
     double v = (mu - mu_t) * anchor[k];
@@ -858,7 +918,8 @@ __device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const
 }
 
 // One IPM iteration of one QP (__init__.py:473-578).
-__device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm, Counters* cnt) {
+__device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm_in, Counters* cnt) {
+  Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m, z = I.z, N = n + m;
   const int tid = threadIdx.x;
   double* red = sm.red;
@@ -1195,8 +1256,8 @@ __global__ void k_reset_queue(int* queue) {
 
 __global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int* order, int count,
                                                  int* queue, Opts o, int nmax, Counters* cnt) {
-  extern __shared__ double smem[];
-  Smem sm = carve(smem, nmax);
+  Smem sm = lds_smem();
+  (void)nmax;
   __shared__ int s_q;
   for (;;) {
     if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
@@ -1470,10 +1531,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
                      h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess)
     return cleanup(fail(QTQP_ERR_HIP, "table upload"));
-  h->smem_iter = sizeof(double) * (2 * NB * (NB + 1) + NB + ((nmax + 1) & ~1) + WG + 8);
-  if (hipFuncSetAttribute((const void*)k_ipm_iter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)h->smem_iter) != hipSuccess)
-    return cleanup(fail(QTQP_ERR_HIP, "LDS attribute"));
+  h->smem_iter = 0;  // the IPM kernel's LDS is static (g_lds)
   *out = h;
   return QTQP_OK;
 }

@@ -73,6 +73,22 @@ __device__ inline double block_max(double v, double* red) {
   return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// Loads through a pointer known to address global memory (device buffers
+// reached through tables in memory are otherwise accessed with flat
+// instructions, whose completion is also counted with LDS operations, so an
+// LDS wait would drain every outstanding prefetch).
+template <typename T>
+__device__ inline T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// The same pointer, marked as addressing LDS (a round trip through the LDS
+// address space that the compiler keeps, so accesses become ds_* ops).
+template <typename T>
+__device__ inline T* in_lds(T* p) {
+  return (T*)(__attribute__((address_space(3))) T*)p;
+}
+
 // Non-negative doubles compare like their bit patterns, so an inf-norm can be
 // accumulated with an integer atomic max (order independent, deterministic).
 __device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
@@ -128,10 +144,10 @@ struct RankKTileG {
         const T* row = X + (int64_t)(kin ? k : kbeg) * ldx;
         const T wk = kin ? wfun(k) : T(0);
 #pragma unroll
-        for (int h = 0; h < MI; ++h) pi[q][h] = (kin && i0 + c + 64 * h < ilim) ? row[i0 + c + 64 * h] : T(0);
+        for (int h = 0; h < MI; ++h) pi[q][h] = (kin && i0 + c + 64 * h < ilim) ? ldg(row + i0 + c + 64 * h) : T(0);
 #pragma unroll
         for (int h = 0; h < MJ; ++h)
-          pj[q][h] = (kin && j0 + c + 64 * h < jlim) ? row[j0 + c + 64 * h] * wk : T(0);
+          pj[q][h] = (kin && j0 + c + 64 * h < jlim) ? ldg(row + j0 + c + 64 * h) * wk : T(0);
       }
     };
     if (kbeg < kend) fetch(kbeg);
@@ -216,7 +232,7 @@ __device__ inline void load_diag_block(const T* G, int64_t ld, int k0, int nb,
   for (int u = 0; u < NB * NB / WG; ++u) {
     const int idx = threadIdx.x + u * WG;
     const int i = min(idx & (NB - 1), nb - 1), j = min(idx >> 6, nb - 1);
-    v[u] = G[(int64_t)(k0 + j) * ld + k0 + i];
+    v[u] = ldg(G + (int64_t)(k0 + j) * ld + k0 + i);
   }
 #pragma unroll
   for (int u = 0; u < NB * NB / WG; ++u) {
@@ -377,7 +393,7 @@ __device__ inline void trsm_mfma_rows(T* G, int64_t ld, int n, int k0, int nb,
 #pragma unroll 4
     for (int kk = 0; kk < NB; kk += 4) {
       const int k = kk + kq;
-      const T b = (row < n && k < nb) ? G[(int64_t)(k0 + k) * ld + row] : T(0);
+      const T b = (row < n && k < nb) ? ldg(G + (int64_t)(k0 + k) * ld + row) : T(0);
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) acc[ct] = Mfma<T>::mac(Li[16 * ct + cl][k], b, acc[ct]);
     }
