@@ -310,15 +310,17 @@ __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, c
   const int n = I.n, m = I.m, ld = I.ld;
   constexpr int TS = 64 * MT;
   const int nt = (n + TS - 1) / TS;
+  using Tile = RankKTileG<T, MT, MT>;
   T* si = reinterpret_cast<T*>(sm.u);
-  T* sj = si + KC * (TS + 1);
+  T* sj = si + Tile::KCH * (TS + 1);
   for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
     int ti, tj;
     lower_tile(t, &ti, &tj);
     const int i0 = ti * TS, j0 = tj * TS;
-    RankKTileG<T, MT, MT> acc;
+    Tile acc;
     acc.zero();
-    acc.run(A, ld, 0, m, i0, j0, n, n, [w](int k) { return (T)w[k]; }, si, sj);
+    // rows of A are 16-byte aligned and zero-padded to ld (a multiple of 4)
+    acc.run_vec(A, ld, 0, m, i0, j0, [w](int k) { return (T)w[k]; }, si, sj);
     acc.for_each(i0, j0, [&](int i, int j, T v) {
       if (i < n && j < n && i >= j)
         G[(int64_t)j * n + i] = v + ((i == j) ? (T)I.dg[i] : P[(int64_t)j * ld + i]);

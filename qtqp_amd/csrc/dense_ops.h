@@ -178,6 +178,71 @@ struct RankKTileG {
     }
   }
 
+  // Rows of X staged per LDS round by run_vec.
+  static constexpr int KCH = sizeof(T) == 4 ? 64 : 32;
+
+  // run() for a row-major X whose rows are 16-byte aligned and zero-padded
+  // to ldx (a multiple of 16 / sizeof(T) elements; columns >= ilim / jlim
+  // must read as zero, which the padding provides): every thread fetches
+  // 16-byte vectors, KCH rows per round, one round ahead of the MFMAs.
+  // si: KCH rows of (TI + 1), sj: KCH rows of (TJ + 1) elements of LDS.
+  template <typename WFun>
+  __device__ inline void run_vec(const T* __restrict__ X, int64_t ldx, int kbeg, int kend, int i0,
+                                 int j0, WFun wfun, T* si, T* sj) {
+    constexpr int VW = 16 / sizeof(T);
+    typedef T VT __attribute__((ext_vector_type(VW)));
+    constexpr int VI = TI / VW, VJ = TJ / VW;  // vectors per staged row
+    constexpr int RI = WG / VI, RJ = WG / VJ;  // rows per pass of the workgroup
+    constexpr int PI = KCH / RI, PJ = KCH / RJ;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int jb = 32 * MJ * (wave >> 1), ib = 32 * MI * (wave & 1);
+    const int vi = tid % VI, ri = tid / VI, vj = tid % VJ, rj = tid / VJ;
+    const bool ci = i0 + vi * VW < ldx, cj = j0 + vj * VW < ldx;
+    VT pi[PI], pj[PJ];
+    auto fetch = [&](int kc) {
+#pragma unroll
+      for (int q = 0; q < PI; ++q) {
+        const int k = kc + ri + RI * q;
+        pi[q] = (k < kend && ci) ? ldg(reinterpret_cast<const VT*>(X + (int64_t)k * ldx + i0 + vi * VW))
+                                 : VT(0);
+      }
+#pragma unroll
+      for (int q = 0; q < PJ; ++q) {
+        const int k = kc + rj + RJ * q;
+        pj[q] = (k < kend && cj)
+                    ? ldg(reinterpret_cast<const VT*>(X + (int64_t)k * ldx + j0 + vj * VW)) * wfun(k)
+                    : VT(0);
+      }
+    };
+    if (kbeg < kend) fetch(kbeg);
+    for (int kc = kbeg; kc < kend; kc += KCH) {
+#pragma unroll
+      for (int q = 0; q < PI; ++q)
+#pragma unroll
+        for (int e = 0; e < VW; ++e) si[(ri + RI * q) * (TI + 1) + vi * VW + e] = pi[q][e];
+#pragma unroll
+      for (int q = 0; q < PJ; ++q)
+#pragma unroll
+        for (int e = 0; e < VW; ++e) sj[(rj + RJ * q) * (TJ + 1) + vj * VW + e] = pj[q][e];
+      __syncthreads();
+      if (kc + KCH < kend) fetch(kc + KCH);
+#pragma unroll
+      for (int kk = 0; kk < KCH; kk += 4) {
+        const int kq = kk + (lane >> 4), cl = lane & 15;
+        T av[BJ], bv[BI];
+#pragma unroll
+        for (int a = 0; a < BJ; ++a) av[a] = sj[kq * (TJ + 1) + jb + 16 * a + cl];
+#pragma unroll
+        for (int b = 0; b < BI; ++b) bv[b] = si[kq * (TI + 1) + ib + 16 * b + cl];
+#pragma unroll
+        for (int a = 0; a < BJ; ++a)
+#pragma unroll
+          for (int b = 0; b < BI; ++b) acc[a][b] = Mfma<T>::mac(av[a], bv[b], acc[a][b]);
+      }
+      __syncthreads();
+    }
+  }
+
   // Same, with the classic [KC][NB + 1] staging arrays of a 64 x 64 tile.
   template <typename WFun>
   __device__ inline void run(const T* __restrict__ X, int64_t ldx, int kbeg, int kend, int i0,
