@@ -125,7 +125,6 @@ __device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
 // -------------------------------------------------------------------------
 
 constexpr int RPW = 8;     // rows in flight per wavefront in a row pass
-constexpr int MAXCW = 16;  // 64-column slots per lane in the scalar max pass (n <= 1024)
 
 template <typename T, int W> struct VecT;
 template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
@@ -234,60 +233,6 @@ __device__ inline void rowpass(const T* __restrict__ M, int nrow, int ncol, int 
 template <typename T> struct VW;
 template <> struct VW<double> { static constexpr int value = 2; };
 template <> struct VW<float> { static constexpr int value = 4; };
-
-// Row maxima max_j |M[i][j]| wc[j] and column maxima max_i |M[i][j]| wr[i]
-// of row-major M in one pass (same work split as rowpass).
-template <typename RE, typename CE>
-__device__ inline void rowpass_absmax(const double* __restrict__ M, int nrow, int ncol,
-                                      const double* __restrict__ wc,
-                                      const double* __restrict__ wr, RE rowepi, CE colepi,
-                                      double* scratch) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ncw = (ncol + 63) >> 6;
-  double wcr[MAXCW], cm[MAXCW];
-#pragma unroll
-  for (int c = 0; c < MAXCW; ++c) {
-    const int j = lane + 64 * c;
-    wcr[c] = (c < ncw && j < ncol) ? wc[j] : 0.0;
-    cm[c] = 0.0;
-  }
-  for (int i0 = wave * RPW; i0 < nrow; i0 += 4 * RPW) {
-    double d[RPW], ww[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      d[r] = 0.0;
-      ww[r] = (i0 + r < nrow) ? wr[i0 + r] : 0.0;
-    }
-#pragma unroll
-    for (int c = 0; c < MAXCW; ++c) {
-      const int j = lane + 64 * c;
-      if (c < ncw && j < ncol) {
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-          if (i0 + r < nrow) {
-            const double a = fabs(ldg(M + (int64_t)(i0 + r) * ncol + j));
-            d[r] = fmax(d[r], a * wcr[c]);
-            cm[c] = fmax(cm[c], a * ww[r]);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      const double mxr = wave_max(d[r]);
-      if (lane == 0 && i0 + r < nrow) rowepi(i0 + r, mxr);
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < MAXCW; ++c) {
-    const int j = lane + 64 * c;
-    if (c < ncw && j < ncol) scratch[wave * ncol + j] = cm[c];
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < ncol; j += WG)
-    colepi(j, fmax(fmax(scratch[j], scratch[ncol + j]), fmax(scratch[2 * ncol + j], scratch[3 * ncol + j])));
-  __syncthreads();
-}
 
 __device__ inline double bsum(double v, double* red) { return block_sum(v, red + WG); }
 __device__ inline double bmax(double v, double* red) { return block_max(v, red + WG); }
@@ -478,11 +423,11 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
 // KKT solves with iterative refinement, one pass over A per step.
 // -------------------------------------------------------------------------
 
-// Completes R per-lane partial sums across the wavefront (R = 2, 4 or 8): a
-// transpose-reduce that halves the rows a lane carries at each of log2(R)
-// exchanges, then sums the rest.  Lane l ends with the total of row
-// l >> (6 - log2 R) in d[0].
-template <int R>
+// Completes R per-lane partial sums (or maxima, MAX) across the wavefront
+// (R = 2, 4 or 8): a transpose-reduce that halves the rows a lane carries
+// at each of log2(R) exchanges, then combines the rest.  Lane l ends with
+// the result of row l >> (6 - log2 R) in d[0].
+template <int R, bool MAX = false>
 __device__ inline void reduce_rows(double* d) {
   const int lane = threadIdx.x & 63;
   int cnt = R;
@@ -495,13 +440,89 @@ __device__ inline void reduce_rows(double* d) {
       if (q < half) {
         const double send = hi ? d[q] : d[q + half];
         const double keep = hi ? d[q + half] : d[q];
-        d[q] = keep + __shfl_xor(send, bit, 64);
+        const double got = __shfl_xor(send, bit, 64);
+        d[q] = MAX ? fmax(keep, got) : keep + got;
       }
     }
     cnt = half;
   }
 #pragma unroll
-  for (int bit = 32 / R; bit >= 1; bit >>= 1) d[0] += __shfl_xor(d[0], bit, 64);
+  for (int bit = 32 / R; bit >= 1; bit >>= 1) {
+    const double got = __shfl_xor(d[0], bit, 64);
+    d[0] = MAX ? fmax(d[0], got) : d[0] + got;
+  }
+}
+
+// Row maxima max_j |M[i][j]| wc[j] and column maxima max_i |M[i][j]| wr[i]
+// of row-major M (leading dimension ld, zero-padded, 16-byte aligned rows;
+// ld <= 128 NS) in one pass: R rows per wavefront in registers, the next R
+// fetched while the current ones are reduced.  scratch: >= 4 * ncol
+// doubles.
+template <int NS, int R, typename RE, typename CE>
+__device__ inline void rowpass_absmax_v(const double* __restrict__ M, int nrow, int ncol, int ld,
+                                        const double* __restrict__ wc,
+                                        const double* __restrict__ wr, RE rowepi, CE colepi,
+                                        double* scratch) {
+  using V = VecT<double, 2>::type;
+  constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double wcr[NS][2], cm[NS][2];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int col = s * 128 + lane * 2 + e;
+      wcr[s][e] = col < ncol ? wc[col] : 0.0;
+      cm[s][e] = 0.0;
+    }
+  V cur[R][NS], nxt[R][NS];
+  auto fetch = [&](int i0, V (*dst)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int col0 = s * 128 + lane * 2;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        dst[r][s] = (col0 < ld && i0 + r < nrow)
+                        ? ldg(reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0))
+                        : V(0);
+    }
+  };
+  fetch(wave * R, cur);
+  for (int i0 = wave * R; i0 < nrow; i0 += 4 * R) {
+    if (i0 + 4 * R < nrow) fetch(i0 + 4 * R, nxt);
+    double d[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double w = i0 + r < nrow ? wr[i0 + r] : 0.0;
+      d[r] = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const double a = fabs(cur[r][s][e]);
+          d[r] = fmax(d[r], a * wcr[s][e]);
+          cm[s][e] = fmax(cm[s][e], a * w);
+        }
+    }
+    reduce_rows<R, true>(d);
+    const int row = i0 + (lane >> (6 - LR));
+    if (row < nrow && (lane & ((64 >> LR) - 1)) == 0) rowepi(row, d[0]);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) cur[r][s] = nxt[r][s];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int col = s * 128 + lane * 2 + e;
+      if (col < ncol) scratch[wave * ncol + col] = cm[s][e];
+    }
+  __syncthreads();
+  for (int j = threadIdx.x; j < ncol; j += WG)
+    colepi(j, fmax(fmax(scratch[j], scratch[ncol + j]), fmax(scratch[2 * ncol + j], scratch[3 * ncol + j])));
+  __syncthreads();
 }
 
 // One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
@@ -1138,6 +1159,28 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 
 // Per QP: transpose, Ruiz equilibration (__init__.py:616-663), FP32 copies,
 // norms of the unscaled b/c, default start (__init__.py:299-307).
+// Ruiz equilibration (__init__.py:616-669): 10 passes, each taking the row
+// and column inf-norms of D A E and E P E (one pass over each) and scaling
+// d by 1/sqrt(row norm), e by 1/sqrt(max(column norms)), clipped to
+// [1e-3, 1e3].  Reads the unscaled padded copies I.A and I.P.
+template <int NS, int R>
+__device__ inline void ruiz(const Inst& I, double* scratch) {
+  const int n = I.n, m = I.m, ld = I.ld, tid = threadIdx.x;
+  for (int pass = 0; pass < 10; ++pass) {
+    rowpass_absmax_v<NS, R>(I.A, m, n, ld, I.e, I.d, [&](int i, double v) {
+      const double r = v * I.d[i];
+      I.um[i] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
+    }, [&](int j, double v) { I.un[j] = v * I.e[j]; }, scratch);
+    rowpass_absmax_v<NS, R>(I.P, n, n, ld, I.e, I.e, [](int, double) {}, [&](int j, double v) {
+      const double cn = fmax(I.un[j], v * I.e[j]);
+      I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
+    }, scratch);
+    for (int i = tid; i < m; i += WG) I.d[i] *= I.um[i];
+    for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equilibrate,
                                                 int default_start, int force64) {
   // dynamic LDS: (WG + 8 + 4 * nmax) doubles
@@ -1149,36 +1192,43 @@ __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equi
     const int n = I.n, m = I.m, tid = threadIdx.x;
     for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
     for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
+    // unscaled copies with the padded leading dimension (zeros in the pad)
+    const int ld = I.ld, lane = tid & 63, wave = tid >> 6;
+    for (int i = wave; i < m; i += 4)
+      for (int j = lane; j < ld; j += 64)
+        I.A[(int64_t)i * ld + j] = j < n ? ldg(I.Araw + (int64_t)i * n + j) : 0.0;
+    for (int i = wave; i < n; i += 4)
+      for (int j = lane; j < ld; j += 64)
+        I.P[(int64_t)i * ld + j] = j < n ? ldg(I.Praw + (int64_t)i * n + j) : 0.0;
     __syncthreads();
     if (equilibrate) {
-      for (int pass = 0; pass < 10; ++pass) {
-        // of D A E: row inf-norms -> um (as 1/sqrt scales), column norms -> un
-        rowpass_absmax(I.Araw, m, n, I.e, I.d, [&](int i, double v) {
-          const double r = v * I.d[i];
-          I.um[i] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
-        }, [&](int j, double v) { I.un[j] = v * I.e[j]; }, scratch);
-        // and of E P E (P symmetric): column norms, combined with A's
-        rowpass_absmax(I.Praw, n, n, I.e, I.e, [](int, double) {}, [&](int j, double v) {
-          const double cn = fmax(I.un[j], v * I.e[j]);
-          I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
-        }, scratch);
-        for (int i = tid; i < m; i += WG) I.d[i] *= I.um[i];
-        for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
-        __syncthreads();
+      if (ld <= 128)
+        ruiz<1, 8>(I, scratch);
+      else if (ld <= 256)
+        ruiz<2, 8>(I, scratch);
+      else if (ld <= 512)
+        ruiz<4, 4>(I, scratch);
+      else
+        ruiz<8, 2>(I, scratch);
+    }
+    // D A E and E P E in place, and their FP32 copies
+    for (int i = wave; i < m; i += 4) {
+      const double di = I.d[i];
+      for (int j = lane; j < ld; j += 64) {
+        const int64_t k = (int64_t)i * ld + j;
+        const double v = j < n ? I.A[k] * (di * I.e[j]) : 0.0;
+        I.A[k] = v;
+        I.A32[k] = (float)v;
       }
     }
-    const int ld = I.ld;
-    for (int64_t k = tid; k < (int64_t)m * ld; k += WG) {
-      const int i = (int)(k / ld), j = (int)(k % ld);
-      const double v = j < n ? I.Araw[(int64_t)i * n + j] * (I.d[i] * I.e[j]) : 0.0;
-      I.A[k] = v;
-      I.A32[k] = (float)v;
-    }
-    for (int64_t k = tid; k < (int64_t)n * ld; k += WG) {
-      const int i = (int)(k / ld), j = (int)(k % ld);
-      const double v = j < n ? I.Praw[(int64_t)i * n + j] * (I.e[i] * I.e[j]) : 0.0;
-      I.P[k] = v;
-      I.P32[k] = (float)v;
+    for (int i = wave; i < n; i += 4) {
+      const double ei = I.e[i];
+      for (int j = lane; j < ld; j += 64) {
+        const int64_t k = (int64_t)i * ld + j;
+        const double v = j < n ? I.P[k] * (ei * I.e[j]) : 0.0;
+        I.P[k] = v;
+        I.P32[k] = (float)v;
+      }
     }
     double lb = 0.0, lc = 0.0;
     for (int i = tid; i < m; i += WG) {
