@@ -535,9 +535,9 @@ __device__ inline void rowpass_absmax_v(const double* __restrict__ M, int nrow, 
 // them), broadcasts the coefficients and accumulates the column sums from
 // the same registers.  The next R rows are fetched while the current ones
 // are processed.  scratch: >= 8 * ncol doubles of LDS.
-template <int NS, int R, typename RE, typename CE>
+template <int NS, int R, typename RL, typename RE, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
-                                const double* __restrict__ xs, RE rowepi, CE colepi,
+                                const double* __restrict__ xs, RL rowload, RE rowepi, CE colepi,
                                 double* scratch) {
   using V = VecT<double, 2>::type;
   constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
@@ -566,6 +566,10 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
   };
   fetch(wave * R, cur);
   for (int i0 = wave * R; i0 < nrow; i0 += 4 * R) {
+    // The epilogue's per-row inputs are loaded before the next rows are
+    // requested, so waiting for them does not wait for the prefetch.
+    const int row = i0 + (lane >> (6 - LR));
+    const auto in = rowload(min(row, nrow - 1));
     if (i0 + 4 * R < nrow) fetch(i0 + 4 * R, nxt);
     double d[R];
 #pragma unroll
@@ -575,9 +579,8 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       for (int s = 0; s < NS; ++s) d[r] += cur[r][s][0] * xr[s][0] + cur[r][s][1] * xr[s][1];
     }
     reduce_rows<R>(d);
-    const int row = i0 + (lane >> (6 - LR));
     double c1 = 0.0, c2 = 0.0;
-    if (row < nrow) rowepi(row, d[0], (lane & ((64 >> LR) - 1)) == 0, &c1, &c2);
+    if (row < nrow) rowepi(row, d[0], (lane & ((64 >> LR) - 1)) == 0, in, &c1, &c2);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double k1 = __shfl(c1, r << (6 - LR), 64);
@@ -636,25 +639,38 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   const double* ky = I.krhs + n;
   const double* cy = I.corr + n;
   double mx = 0.0;
+  struct RowIn {
+    double sy, r0, ax0, dy, ky, cy;
+  };
   rowpass2<NS, R>(
       I.A, m, n, I.ld, init ? S1.sol : xc,
-      [&](int i, double t, bool writer, double* c1, double* c2) {
+      [&](int i) {
+        RowIn in;
+        in.sy = ldg((init ? S1.sol : S0.sol) + n + i);
+        in.r0 = init ? 0.0 : ldg(S0.r + n + i);
+        in.ax0 = init ? 0.0 : ldg(S0.ax + i);
+        in.dy = ldg(dy + i);
+        in.ky = ldg(ky + i);
+        in.cy = ldg(cy + i);
+        return in;
+      },
+      [&](int i, double t, bool writer, const RowIn& in, double* c1, double* c2) {
         double yv, axv, yc = 0.0;
         if (init) {
-          yv = S1.sol[n + i];
+          yv = in.sy;
           axv = t;
         } else {
-          yc = -(t - S0.r[n + i]) / dy[i];
-          yv = S0.sol[n + i] + yc;
-          axv = S0.ax[i] + t;
+          yc = -(t - in.r0) / in.dy;
+          yv = in.sy + yc;
+          axv = in.ax0 + t;
         }
-        const double ry = ky[i] - (axv + dy[i] * yv) + cy[i] * yv;
+        const double ry = in.ky - (axv + in.dy * yv) + in.cy * yv;
         *c1 = init ? yv : yc;
-        *c2 = -ry / dy[i];
+        *c2 = -ry / in.dy;
         if (writer) {
-          if (!init) S1.sol[n + i] = yv;
-          S1.ax[i] = axv;
-          S1.r[n + i] = ry;
+          if (!init) stg(S1.sol + n + i, yv);
+          stg(S1.ax + i, axv);
+          stg(S1.r + n + i, ry);
           const double a = fabs(ry);
           mx = fmax(mx, (a == a) ? a : INFINITY);
         }

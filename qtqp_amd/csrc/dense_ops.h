@@ -82,6 +82,11 @@ __device__ inline T ldg(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
 
+template <typename T>
+__device__ inline void stg(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)p = v;
+}
+
 // The same pointer, marked as addressing LDS (a round trip through the LDS
 // address space that the compiler keeps, so accesses become ds_* ops).
 template <typename T>
