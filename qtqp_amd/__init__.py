@@ -61,6 +61,31 @@ def _auto_linear_solver_order() -> list[LinearSolver]:
   return [LinearSolver.HIP_DENSE, LinearSolver.HIP_DENSE_FP64]
 
 
+class _ProductMat:
+  """A matrix used only in products: a dense copy (BLAS) when at least a
+  quarter of it is stored, the scipy.sparse matrix otherwise."""
+
+  def __init__(self, mat):
+    self.shape, self.nnz = mat.shape, mat.nnz
+    cells = mat.shape[0] * mat.shape[1]
+    self._m = mat.toarray() if cells and mat.nnz >= 0.25 * cells and cells <= (1 << 28) else mat
+
+  def __matmul__(self, v):
+    return self._m @ v
+
+  def rmatvec(self, v):
+    return self._m.T @ v
+
+
+def _segment_max(vals: np.ndarray, indptr: np.ndarray) -> np.ndarray:
+  """Max of vals over each compressed segment [indptr[k], indptr[k+1]); 0 if empty."""
+  out = np.zeros(len(indptr) - 1)
+  nonempty = np.diff(indptr) > 0
+  if vals.size:
+    out[nonempty] = np.maximum.reduceat(vals, indptr[:-1][nonempty])
+  return out
+
+
 def _resolve_linear_solver(linear_solver: LinearSolver):
   """(enum, backend instance); mirrors __init__.py:107-126."""
   if linear_solver is not LinearSolver.AUTO:
@@ -237,6 +262,8 @@ class QTQP:
     else:
       a, p, b, c, self.d, self.e = self.a, self.p, self.b, self.c, None, None
     self.q = np.concatenate([c, b])
+    self._a_op, self._p_op = _ProductMat(self.a), _ProductMat(self.p)
+    p_op = _ProductMat(p)
     self._norm_b = _inf_norm(self.b)
     self._norm_c = _inf_norm(self.c)
     self._linear_solver = direct.DirectKktSolver(
@@ -265,7 +292,7 @@ class QTQP:
       # Predictor: mu_target = 0, anchored at the current point.
       anchor[:n], anchor[n:] = x, y
       x_p, y_p, tau_p, st["predictor_lin_sys_stats"] = self._newton_step(
-          p=p, mu=mu, mu_target=0.0, r_anchor=anchor, tau_anchor=tau, x=x, y=y, s=s,
+          p=p_op, mu=mu, mu_target=0.0, r_anchor=anchor, tau_anchor=tau, x=x, y=y, s=s,
           tau=tau, correction=None)
       d_x_p, d_y_p, d_tau_p = x_p - x, y_p - y, tau_p - tau
       d_s[z:] = -y_p[z:] * s[z:] / y[z:]
@@ -276,7 +303,7 @@ class QTQP:
       correction = -d_s[z:] * d_y_p[z:] / y[z:]
       anchor[:n], anchor[n:] = x_p, y_p
       x_c, y_c, tau_c, st["corrector_lin_sys_stats"] = self._newton_step(
-          p=p, mu=mu, mu_target=sigma * mu, r_anchor=anchor, tau_anchor=tau_p, x=x, y=y,
+          p=p_op, mu=mu, mu_target=sigma * mu, r_anchor=anchor, tau_anchor=tau_p, x=x, y=y,
           s=s, tau=tau, correction=correction)
       d_x, d_y, d_tau = x_c - x, y_c - y, tau_c - tau
       d_s[z:] = sigma * mu / y[z:] + correction - y_c[z:] * s[z:] / y[z:]
@@ -330,15 +357,25 @@ class QTQP:
 
   # -- equilibration (__init__.py:616-669) ---------------------------------
   def _equilibrate(self, num_iters=10, min_scale=1e-3, max_scale=1e3):
-    """Ruiz scaling: A <- D A E, P <- E P E, b <- D b, c <- E c."""
+    """Ruiz scaling: A <- D A E, P <- E P E, b <- D b, c <- E c (__init__.py:616-669).
+
+    Same passes and the same floating-point products as the reference.  The
+    inf-norms are segment maxima over CSC (columns) and a CSR copy of A
+    (rows) that is scaled alongside, instead of format conversions inside
+    every norm call; maxima are exact, so d and e are bitwise the same.
+    """
     a, p = self.a.copy().tocsc(), self.p.copy().tocsc()
+    ar = a.tocsr()
+    ar_rows = np.repeat(np.arange(self.m), np.diff(ar.indptr))
     d, e = np.ones(self.m), np.ones(self.n)
     for _ in range(num_iters):
-      row = sp.linalg.norm(a, np.inf, axis=1)
+      row = _segment_max(np.abs(ar.data), ar.indptr)
       d_i = np.clip(1.0 / np.sqrt(np.where(row == 0.0, 1.0, row)), min_scale, max_scale)
-      col = np.maximum(sp.linalg.norm(a, np.inf, axis=0), sp.linalg.norm(p, np.inf, axis=0))
+      col = np.maximum(_segment_max(np.abs(a.data), a.indptr),
+                       _segment_max(np.abs(p.data), p.indptr))
       e_i = np.clip(1.0 / np.sqrt(np.where(col == 0.0, 1.0, col)), min_scale, max_scale)
       a.data *= d_i[a.indices] * np.repeat(e_i, np.diff(a.indptr))
+      ar.data *= d_i[ar_rows] * e_i[ar.indices]
       if p.nnz > 0:
         p.data *= e_i[p.indices] * np.repeat(e_i, np.diff(p.indptr))
       d *= d_i
@@ -461,12 +498,12 @@ class QTQP:
     if self.equilibrate:
       x, y, s = self._unequilibrate_iterates(x, y, s)
     inv_tau = 1.0 / max(tau, _EPS)
-    ax = self.a @ x
-    aty = self.a.T @ y
-    if self.p.nnz == 0:
+    ax = self._a_op @ x
+    aty = self._a_op.rmatvec(y)
+    if self._p_op.nnz == 0:
       px, xpx = np.zeros(self.n), 0.0
     else:
-      px = self.p @ x
+      px = self._p_op @ x
       xpx = x @ px
     ctx, bty = self.c @ x, self.b @ y
     pcost = (ctx + 0.5 * xpx * inv_tau) * inv_tau
