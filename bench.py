@@ -248,13 +248,43 @@ def single_instance_leg(name, qp, oracle_ref=None):
   return out
 
 
+def timed_oracle(qtqp_oracle, qp, threads=None):
+  """The FP64 oracle (the reference algorithm on scipy LAPACK) on one QP: (result, seconds).
+
+  threads=None uses every BLAS thread the host offers; 1 pins BLAS to one thread.
+  """
+  from threadpoolctl import threadpool_limits
+  with threadpool_limits(limits=threads):
+    t0 = time.perf_counter()
+    ref = qtqp_oracle.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+    return ref, time.perf_counter() - t0
+
+
 def single_legs(problems, qtqp_oracle):
-  out = {"single_large_dense": single_instance_leg(
-      "n=4000 m=6000 z=1000 dense QP (seed 0)", problems.random_dense_qp(4000, 6000, 1000, seed=0))}
-  q3 = problems.mpc_qp(64, 12, 4, seed=0)
-  log("oracle for MPC T=64 nx=12 nu=4")
-  out["single_mpc"] = single_instance_leg("condensed MPC T=64 nx=12 nu=4 (seed 0)", q3,
-                                          qtqp_oracle.solve_qp(q3.a, q3.b, q3.c, q3.z, q3.p))
+  """The single-instance configurations: the n=200 dense QP (CPU reference case),
+  the n=4000 dense QP and the T=64 condensed MPC QP, each with the FP64 oracle's
+  CPU time (all BLAS threads and one) and the GPU drop-in backends."""
+  try:
+    cores = len(os.sched_getaffinity(0))
+  except AttributeError:
+    cores = os.cpu_count()
+  out = {}
+  for key, label, qp in (
+      ("single_small_dense", "n=200 m=300 z=50 dense QP (seed 0)",
+       problems.random_dense_qp(200, 300, 50, seed=0)),
+      ("single_large_dense", "n=4000 m=6000 z=1000 dense QP (seed 0)",
+       problems.random_dense_qp(4000, 6000, 1000, seed=0)),
+      ("single_mpc", "condensed MPC T=64 nx=12 nu=4 (seed 0)", problems.mpc_qp(64, 12, 4, seed=0))):
+    log(f"{label}: FP64 oracle on the CPU")
+    ref, t_all = timed_oracle(qtqp_oracle, qp)
+    cpu = {"status": ref["status"], "iterations": ref["iters"], "seconds_all_threads": t_all,
+           "threads": cores}
+    if qp.n <= 1000:
+      _, t_one = timed_oracle(qtqp_oracle, qp, threads=1)
+      cpu["seconds_one_thread"] = t_one
+    leg = single_instance_leg(label, qp, ref)
+    leg["cpu_oracle"] = cpu
+    out[key] = leg
   return out
 
 
