@@ -128,3 +128,46 @@ def test_tile_shapes_give_identical_results_and_autotune(require_gpu):
   with pytest.raises(ValueError):
     bq.set_tiles([5] * nb)
   bq.free()
+
+
+def test_maximum_size_instance_in_batch(require_gpu):
+  """n near the engine's limit (1024) beside small instances: the widest row
+  passes (8 slots of 128 columns), 16 diagonal blocks, both Gram tile shapes."""
+  qps = [problems.random_dense_qp(1000, 1100, 100, seed=6), problems.mpc_qp(6, 3, 1, seed=7),
+         problems.random_dense_qp(70, 90, 10, seed=8)]
+  refs = [O.solve_qp(q.a, q.b, q.c, q.z, q.p) for q in qps]
+  bq = batch.BatchQP(qps)
+  nb = len(batch.BatchQP.BUCKET_EDGES)
+  for shape in (0, 1):
+    bq.set_tiles([shape] * nb)
+    res = bq.solve(precision="mixed")
+    for sol, ref in zip(res.solutions, refs):
+      _compare(sol, ref, 1e-6)
+  bq.free()
+
+
+def test_full_size_uneven_mpc_batch(require_gpu):
+  """The 4096-QP uneven MPC workload at full size: every instance ends solved or
+  infeasible, solved ones meet the stopping tests on the unscaled data
+  (size-independent KKT property), and a spread sample matches the oracle."""
+  qps = problems.uneven_mpc_batch(4096, base_seed=1000)
+  bq = batch.BatchQP(qps)
+  bq.autotune(sample=16, iters=2)
+  res = bq.solve(precision="mixed")
+  assert {s.status for s in res.solutions} <= {qtqp_amd.SolutionStatus.SOLVED,
+                                               qtqp_amd.SolutionStatus.INFEASIBLE}
+  for q, s in zip(qps, res.solutions):
+    if s.status != qtqp_amd.SolutionStatus.SOLVED:
+      continue
+    ax, aty, px = q.a @ s.x, q.a.T @ s.y, q.p @ s.x
+    pres = np.max(np.abs(ax + s.s - q.b))
+    dres = np.max(np.abs(px + aty + q.c))
+    prel = max(np.max(np.abs(ax)), np.max(np.abs(s.s)), np.max(np.abs(q.b)))
+    drel = max(np.max(np.abs(px)), np.max(np.abs(aty)), np.max(np.abs(q.c)))
+    # the engine stops at atol 1e-7 + rtol 1e-8 * rel on its own (tau-scaled)
+    # residuals; the recomputed ones may differ by rounding, hence 1e-6 here
+    assert pres <= 1e-6 * max(1.0, prel) and dres <= 1e-6 * max(1.0, drel)
+  for i in range(0, 4096, 512):
+    q = qps[i]
+    _compare(res.solutions[i], O.solve_qp(q.a, q.b, q.c, q.z, q.p), 1e-6)
+  bq.free()

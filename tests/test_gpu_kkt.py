@@ -201,3 +201,15 @@ def test_protocol_errors(require_gpu):
   with pytest.raises(ValueError):
     be.update_diag(np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0]))  # y-block not negative
   be.free()
+
+
+def test_full_size_dense_qp_matches_oracle(require_gpu):
+  """The n=4000, m=6000, z=1000 dense QP (a 10000 x 10000 KKT system) through the
+  drop-in driver with the mixed-precision backend: oracle status and x/y/s."""
+  qp = problems.random_dense_qp(4000, 6000, 1000, seed=0)
+  ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p)).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE, verbose=False)
+  assert sol.status.value == ref["status"] == "solved"
+  for k in "xys":
+    assert _rel(getattr(sol, k), ref[k]) < 1e-6, k
