@@ -101,7 +101,9 @@ int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap);
 int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows);
 
 /* Executed work of the last solve per QP, 5 counts each: FP32 and FP64
- * factorisations, K_reg products, FP32 and FP64 factor solves. */
+ * factorisations, refinement passes (one sweep of A each: the rest of a
+ * correction, the FP64 K_reg residual and the next right-hand side), FP32
+ * and FP64 triangular factor solves. */
 int qtqp_batch_work(qtqp_batch* h, int32_t* counts);
 
 /* Gram tile shapes of the FP32 factor.  Larger tiles read A fewer times but

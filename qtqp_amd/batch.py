@@ -107,6 +107,57 @@ def _ptr(a, t):
   return a.ctypes.data_as(t)
 
 
+# -- packed batch-instance files ----------------------------------------------
+# One .npz archive (no pickled objects): per-instance sizes n, m, z, element
+# offsets of each QP's dense row-major A (m x n) and P (n x n) and of its b, c
+# in the concatenated arrays, optional generator seeds (-1 = none), and a
+# format tag.  The same packing the C ABI takes (qtqp_batch_create).
+_INSTANCE_FORMAT = "qtqp_amd.batch-instances.v1"
+
+
+def save_instances(path, qps: Sequence, seeds=None) -> None:
+  """Writes dense QPs ``(a, b, c, z, p)`` (or objects with those fields)."""
+  ns, ms, zs, a_parts, p_parts, b_parts, c_parts = [], [], [], [], [], [], []
+  for q in qps:
+    a, b, c, z, p = (q.a, q.b, q.c, q.z, q.p) if hasattr(q, "a") else q
+    a = np.asarray(a, dtype=np.float64)
+    m, n = a.shape
+    p = np.zeros((n, n)) if p is None else np.asarray(p, dtype=np.float64)
+    ns.append(n)
+    ms.append(m)
+    zs.append(int(z))
+    a_parts.append(a.ravel())
+    p_parts.append(p.ravel())
+    b_parts.append(np.asarray(b, dtype=np.float64))
+    c_parts.append(np.asarray(c, dtype=np.float64))
+  off = lambda parts: np.concatenate([[0], np.cumsum([x.size for x in parts])[:-1]]).astype(np.int64)
+  seeds = np.full(len(ns), -1, dtype=np.int64) if seeds is None else np.asarray(seeds, np.int64)
+  if seeds.shape != (len(ns),):
+    raise ValueError("one seed per instance")
+  np.savez(path, format=np.array(_INSTANCE_FORMAT), n=np.array(ns, np.int32),
+           m=np.array(ms, np.int32), z=np.array(zs, np.int32), a_off=off(a_parts),
+           p_off=off(p_parts), b_off=off(b_parts), c_off=off(c_parts),
+           a=np.concatenate(a_parts), p=np.concatenate(p_parts), b=np.concatenate(b_parts),
+           c=np.concatenate(c_parts), seeds=seeds)
+
+
+def load_instances(path) -> tuple[list, np.ndarray]:
+  """Reads a file written by save_instances: ([(a, b, c, z, p), ...], seeds)."""
+  with np.load(path, allow_pickle=False) as f:
+    if str(f["format"]) != _INSTANCE_FORMAT:
+      raise ValueError(f"not a {_INSTANCE_FORMAT} file")
+    n, m, z = f["n"], f["m"], f["z"]
+    a, p, b, c = f["a"], f["p"], f["b"], f["c"]
+    ao, po, bo, co = f["a_off"], f["p_off"], f["b_off"], f["c_off"]
+    qps = []
+    for i in range(len(n)):
+      ni, mi = int(n[i]), int(m[i])
+      qps.append((a[ao[i]:ao[i] + mi * ni].reshape(mi, ni).copy(), b[bo[i]:bo[i] + mi].copy(),
+                  c[co[i]:co[i] + ni].copy(), int(z[i]),
+                  p[po[i]:po[i] + ni * ni].reshape(ni, ni).copy()))
+    return qps, f["seeds"].copy()
+
+
 class BatchQP:
   """A batch of dense QPs ``(a, b, c, z, p)`` resident on the GPU.
 

@@ -212,6 +212,35 @@ class WarmStart:
     return self.mlp.train((x0s - self.xm) / self.xs, (t - self.tm) / self.ts, batch=batch_size,
                           steps=steps, lr=lr, seed=seed)
 
+  # -- checkpoint: one .npz (no pickled objects) -----------------------------
+  _FORMAT = "qtqp_amd.warmstart.v1"
+
+  def save(self, path) -> None:
+    """Writes the family, k*, the network's packed FP32 parameters and the
+    input/target standardisation."""
+    if self.xm is None:
+      raise ValueError("fit() before save()")
+    f = self.fam
+    np.savez(path, format=np.array(self._FORMAT),
+             family=np.array([f.horizon, f.nx, f.nu, f.family_seed], np.int64),
+             shape=np.array([self.mlp.n_in, self.mlp.hidden, self.n_out, self.k_star], np.int64),
+             params=self.mlp.get_params(), xm=self.xm, xs=self.xs, tm=self.tm, ts=self.ts)
+
+  @classmethod
+  def load(cls, path) -> "WarmStart":
+    with np.load(path, allow_pickle=False) as d:
+      if str(d["format"]) != cls._FORMAT:
+        raise ValueError(f"not a {cls._FORMAT} file")
+      horizon, nx, nu, fseed = (int(v) for v in d["family"])
+      n_in, hidden, n_out, k_star = (int(v) for v in d["shape"])
+      ws = cls(Family(horizon=horizon, nx=nx, nu=nu, family_seed=fseed), hidden=hidden,
+               k_star=k_star)
+      if ws.n_out != n_out or ws.mlp.n_in != n_in:
+        raise ValueError("checkpoint does not match its family")
+      ws.mlp.set_params(d["params"])
+      ws.xm, ws.xs, ws.tm, ws.ts = (d[k].copy() for k in ("xm", "xs", "tm", "ts"))
+    return ws
+
   def propose(self, x0s):
     """(x, y, s, tau, kappa, valid) for each x0; unscaled solver coordinates."""
     raw = self.mlp.forward((x0s - self.xm) / self.xs).astype(np.float64) * self.ts + self.tm

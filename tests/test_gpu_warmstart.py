@@ -29,3 +29,20 @@ def test_warm_start_cuts_iterations_with_identical_statuses(require_gpu):
   for i, (a, b, c, z, p) in enumerate(qps):
     G.check_solution(warm["x"][i], warm["y"][i], warm["s"][i], a, b, c, p, z)
   ws.mlp.free()
+
+
+def test_checkpoint_round_trip(require_gpu, tmp_path):
+  """A saved warm start reloads to the same proposals."""
+  fam = warmstart.Family()
+  ws = warmstart.WarmStart(fam, hidden=64, k_star=3)
+  x0 = problems.parametric_mpc_x0(512, seed=5)
+  t, _ = ws.targets(x0)
+  ws.fit(x0, t, steps=50)
+  path = tmp_path / "ws.npz"
+  ws.save(path)
+  ws2 = warmstart.WarmStart.load(path)
+  a, b = ws.propose(x0[:64]), ws2.propose(x0[:64])
+  for u, v in zip(a, b):
+    np.testing.assert_array_equal(u, v)
+  ws.mlp.free()
+  ws2.mlp.free()
