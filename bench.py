@@ -188,7 +188,8 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
 
 
 def single_instance_leg(name, qp, oracle_ref=None):
-  """One large QP through the drop-in driver, mixed vs plain FP64 HIP backends.
+  """One QP through the drop-in QTQP.solve, mixed vs plain FP64 HIP backends (the
+  device-resident iteration loop; the mixed solve is repeated with the host loop).
 
   Reports per-iteration factor time (HIP events around the factorisation
   kernels), its MFMA utilisation (executed Gram-route flops m n^2 + n^3/3
@@ -208,7 +209,18 @@ def single_instance_leg(name, qp, oracle_ref=None):
                           ("fp64", direct.HipDenseFp64Solver, FP64_PEAK_TFLOPS)):
     infos = []
 
-    class Probe(cls):  # records the engine counters before the handle is released
+    class Probe(cls):  # records the engine counters before a handle is released
+      def device_ipm(self, *args, **kw):
+        ipm = super().device_ipm(*args, **kw)
+        release = ipm.free
+
+        def free():
+          if ipm._h is not None:
+            infos.append(ipm.info())
+          release()
+        ipm.free = free
+        return ipm
+
       def free(self):
         if getattr(self, "_h", None) is not None:
           infos.append(self.info())
@@ -220,7 +232,21 @@ def single_instance_leg(name, qp, oracle_ref=None):
     sol = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc).solve(
         linear_solver=sel, verbose=False, collect_stats=True)
     wall = time.perf_counter() - t0
-    inf = infos[-1]
+    host_wall = None
+    if prec == "mixed":
+      # The same solve with the iteration loop on the host (the reference's
+      # call pattern over the LinearSolver protocol), for comparison.
+      Probe.device_driver = False
+      log(f"{name}: {prec} solve, host iteration loop")
+      t1 = time.perf_counter()
+      hsol = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc).solve(
+          linear_solver=sel, verbose=False)
+      host_wall = {"wall_s": time.perf_counter() - t1, "status": hsol.status.value,
+                   "max_rel_diff_xys_vs_device_loop": max(
+                       float(np.max(np.abs(g - w)) / max(1.0, np.max(np.abs(w))))
+                       for g, w in zip((hsol.x, hsol.y, hsol.s), (sol.x, sol.y, sol.s)))}
+      infos.pop()
+    inf = infos[0] if prec == "mixed" else infos[-1]
     nf = max(1, inf["factorizations"])
     f_ms = inf["factor_ms"] / nf
     ir = [st.get("solves", 0) for it in sol.stats if isinstance(it, dict)
@@ -233,6 +259,8 @@ def single_instance_leg(name, qp, oracle_ref=None):
                                           "max": int(max(ir)) if ir else None},
          "backend_solves": inf["backend_solves"], "fp32_inner_steps": inf["inner_steps"],
          "fp64_fallbacks": inf["fp64_fallbacks"]}
+    if host_wall is not None:
+      r["host_iteration_loop"] = host_wall
     for tag, ref in (("planted", (qp.x_star, qp.y_star, qp.s_star)),
                      ("oracle", None if oracle_ref is None else
                       (oracle_ref["x"], oracle_ref["y"], oracle_ref["s"]))):
@@ -289,7 +317,8 @@ def single_legs(problems, qtqp_oracle):
 
 
 def dropin_fp64(qps, reps=1):
-  """The drop-in path: qtqp_amd.QTQP host driver + HIP_DENSE_FP64 backend, one QP per call."""
+  """The drop-in path: qtqp_amd.QTQP.solve with the HIP_DENSE_FP64 backend, one QP per
+  call (the device-resident iteration loop)."""
   import scipy.sparse as sp
   import qtqp_amd
   q0 = qps[0]

@@ -28,7 +28,7 @@ from typing import Any, Dict, List
 import numpy as np
 import scipy.sparse as sp
 
-from . import direct
+from . import _native, direct
 
 __version__ = "0.0.5"
 _HEADER = ("| iter |      pcost |      dcost |     pres |     dres |      gap |   infeas |"
@@ -257,6 +257,16 @@ class QTQP:
     if verbose:
       print(f"| QTQP v{__version__}: m={self.m}, n={self.n}, z={self.z}, nnz(A)={self.a.nnz},"
             f" nnz(P)={self.p.nnz}, linear_solver={resolved.name}")
+    if getattr(backend, "device_driver", False):
+      return self._solve_on_device(
+          backend, max_iter=max_iter, collect_stats=collect_stats,
+          opts=_native.IpmOpts(
+              atol=atol, rtol=rtol, atol_infeas=atol_infeas, rtol_infeas=rtol_infeas,
+              step_size_scale=step_size_scale,
+              min_static_regularization=min_static_regularization,
+              linear_solver_atol=linear_solver_atol, linear_solver_rtol=linear_solver_rtol,
+              max_iterative_refinement_steps=max_iterative_refinement_steps,
+              collect_stats=int(bool(collect_stats))))
     if equilibrate:
       a, p, b, c, self.d, self.e = self._equilibrate()
     else:
@@ -330,6 +340,68 @@ class QTQP:
 
     if self.equilibrate:
       x, y, s = self._unequilibrate_iterates(x, y, s)
+    return self._finish(x, y, s, tau, status, stats)
+
+  def _solve_on_device(self, backend, *, max_iter, collect_stats, opts) -> Solution:
+    """The iteration loop of _solve_impl (__init__.py:473-578) with the whole
+    iteration on the GPU: equilibration, normalisation, KKT update and
+    factor, the three refined solves, tau, sigma, step and termination run
+    as kernels on data held in HBM; each iteration returns one record."""
+    has_p = self.p.nnz > 0
+    ipm = backend.device_ipm(self.a.toarray(), self.p.toarray() if has_p else None, has_p,
+                             self.b, self.c, self.z, self.equilibrate)
+    self._linear_solver = ipm
+    stats: list = []
+    status = SolutionStatus.UNFINISHED
+    self._log_header()
+    for self.it in range(max_iter):
+      st = self._record_stats(ipm.iterate(opts), collect_stats)
+      status = st["status"]
+      self._log_iteration(st)
+      if collect_stats:
+        stats.append(st)
+      if status != SolutionStatus.UNFINISHED:
+        break
+    else:
+      status = SolutionStatus.HIT_MAX_ITER
+      if collect_stats:
+        stats[-1]["status"] = status
+    x, y, s, tau = ipm.iterates()
+    return self._finish(x, y, s, tau, status, stats)
+
+  _DEVICE_STATUS = {0: SolutionStatus.UNFINISHED, 1: SolutionStatus.SOLVED,
+                    2: SolutionStatus.INFEASIBLE, 3: SolutionStatus.UNBOUNDED}
+
+  def _record_stats(self, rec, collect_stats) -> dict:
+    """The stats dict of one iteration (__init__.py:479-566, 966-1008) from a
+    device record."""
+    def lin(i, tau_method=None):
+      ls = rec.lin[i]
+      status = direct._IR_STATUS[ls.status]
+      d = {"solves": int(ls.solves), "final_residual_norm": float(ls.final_residual_norm),
+           "rhs_norm": float(ls.rhs_norm), "tolerance": float(ls.tolerance),
+           "converged": status == "converged", "status": status}
+      if tau_method is not None:
+        d["tau_method"] = "quadratic" if tau_method else "linearized"
+      return d
+
+    st = {"q_lin_sys_stats": lin(0), "predictor_lin_sys_stats": lin(1, rec.tau_method[0]),
+          "corrector_lin_sys_stats": lin(2, rec.tau_method[1])}
+    st.update({k: float(getattr(rec, k)) for k in (
+        "ctx", "bty", "pcost", "dcost", "pres", "dres", "gap", "pinfeas", "dinfeas", "dinfeas_a",
+        "dinfeas_p", "mu", "sigma", "alpha", "tau", "norm_x", "norm_y")})
+    st.update({"iter": self.it, "status": self._DEVICE_STATUS[rec.status],
+               "time": timeit.default_timer() - self.start_time,
+               "prelrhs": float(rec.prelrhs), "drelrhs": float(rec.drelrhs)})
+    if collect_stats:
+      st.update({k: float(getattr(rec, k)) for k in (
+          "complementarity", "norm_s", "max_sy", "min_sy", "std_sy", "max_s_over_y",
+          "min_s_over_y", "mean_s_over_y", "std_s_over_y")})
+    return st
+
+  def _finish(self, x, y, s, tau, status, stats) -> Solution:
+    """Certificate or tau scaling and postsolve of the unequilibrated iterate
+    (__init__.py:580-614)."""
     if status is SolutionStatus.INFEASIBLE:
       self._log_footer("Primal infeasible / dual unbounded")
       x.fill(np.nan)

@@ -42,6 +42,29 @@ class KktInfo(ctypes.Structure):
               ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
 
 
+class IpmOpts(ctypes.Structure):
+  """qtqp_ipm_opts (include/qtqp_ipm.h)."""
+  _fields_ = [("atol", ctypes.c_double), ("rtol", ctypes.c_double),
+              ("atol_infeas", ctypes.c_double), ("rtol_infeas", ctypes.c_double),
+              ("step_size_scale", ctypes.c_double), ("min_static_regularization", ctypes.c_double),
+              ("linear_solver_atol", ctypes.c_double), ("linear_solver_rtol", ctypes.c_double),
+              ("max_iterative_refinement_steps", ctypes.c_int32),
+              ("collect_stats", ctypes.c_int32)]
+
+
+IPM_RECORD_DOUBLES = ("ctx", "bty", "pcost", "dcost", "pres", "dres", "gap", "pinfeas", "dinfeas",
+                      "dinfeas_a", "dinfeas_p", "mu", "sigma", "alpha", "tau", "norm_x", "norm_y",
+                      "prelrhs", "drelrhs", "complementarity", "norm_s", "max_sy", "min_sy",
+                      "std_sy", "max_s_over_y", "min_s_over_y", "mean_s_over_y", "std_s_over_y")
+
+
+class IpmRecord(ctypes.Structure):
+  """qtqp_ipm_record (include/qtqp_ipm.h)."""
+  _fields_ = ([("status", ctypes.c_int32), ("tau_method", ctypes.c_int32 * 2),
+               ("reserved", ctypes.c_int32), ("lin", IrStats * 3)]
+              + [(name, ctypes.c_double) for name in IPM_RECORD_DOUBLES])
+
+
 # Exported symbol -> (restype, argtypes), matching include/qtqp_kkt.h.
 KKT_SYMBOLS = {
     "qtqp_kkt_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, _c_double_p, _c_double_p,
@@ -58,6 +81,21 @@ KKT_SYMBOLS = {
     "qtqp_kkt_free": (None, [ctypes.c_void_p]),
     "qtqp_last_error": (ctypes.c_char_p, []),
     "qtqp_device_count": (ctypes.c_int, []),
+}
+
+
+# Exported symbol -> (restype, argtypes), matching include/qtqp_ipm.h (same library).
+IPM_SYMBOLS = {
+    "qtqp_ipm_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _c_double_p,
+                                       _c_double_p, ctypes.c_int32, ctypes.c_int32, _c_double_p,
+                                       _c_double_p, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_ipm_iterate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(IpmOpts),
+                                        ctypes.POINTER(IpmRecord)]),
+    "qtqp_ipm_get_iterates": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, _c_double_p,
+                                             _c_double_p]),
+    "qtqp_ipm_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(KktInfo)]),
+    "qtqp_ipm_free": (None, [ctypes.c_void_p]),
 }
 
 
@@ -84,7 +122,7 @@ def load(name: str, symbols: dict) -> ctypes.CDLL:
 
 
 def kkt_lib() -> ctypes.CDLL:
-  return load("libqtqp_kkt.so", KKT_SYMBOLS)
+  return load("libqtqp_kkt.so", {**KKT_SYMBOLS, **IPM_SYMBOLS})
 
 
 def require_device(lib: ctypes.CDLL) -> None:

@@ -1,7 +1,7 @@
 """Builds the HIP shared libraries of qtqp_amd in-tree for gfx950.
 
-Each library is one translation unit under qtqp_amd/csrc compiled with hipcc
-into qtqp_amd/_lib/.  Usage: ``python -m qtqp_amd.build`` (or
+Each library is compiled by hipcc from its sources under qtqp_amd/csrc into
+qtqp_amd/_lib/.  Usage: ``python -m qtqp_amd.build`` (or
 ``__graft_entry__.build()``).
 """
 
@@ -19,11 +19,11 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "_lib"
 INCLUDE = ROOT / "include"
 
-# name -> source file
+# name -> source files
 LIBRARIES = {
-    "libqtqp_kkt.so": "kkt.hip",
-    "libqtqp_batch.so": "batch.hip",
-    "libqtqp_mlp.so": "mlp.hip",
+    "libqtqp_kkt.so": ("kkt.hip", "ipm.hip"),
+    "libqtqp_batch.so": ("batch.hip",),
+    "libqtqp_mlp.so": ("mlp.hip",),
 }
 
 HIPCC_FLAGS = [
@@ -48,21 +48,21 @@ def hipcc() -> str:
 def build(names=None, verbose: bool = True) -> list[pathlib.Path]:
   LIBDIR.mkdir(exist_ok=True)
   built = []
-  for lib, src in LIBRARIES.items():
+  for lib, srcs in LIBRARIES.items():
     if names and lib not in names:
       continue
-    srcp = CSRC / src
-    if not srcp.exists():
+    paths = [CSRC / src for src in srcs]
+    if not all(p.exists() for p in paths):
       continue
     out = LIBDIR / lib
     tmp = out.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", str(srcp), "-o", str(tmp)]
+    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *map(str, paths), "-o", str(tmp)]
     if verbose:
       print("[build]", " ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
       sys.stderr.write(res.stdout + res.stderr)
-      raise RuntimeError(f"hipcc failed for {src}")
+      raise RuntimeError(f"hipcc failed for {', '.join(srcs)}")
     os.replace(tmp, out)
     built.append(out)
   return built

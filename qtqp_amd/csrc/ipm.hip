@@ -1,0 +1,1006 @@
+// Device-resident interior-point driver for one dense QP (MI355X, gfx950).
+//
+// qtqp's iteration loop (src/qtqp/__init__.py:473-578) with the KKT engine of
+// kkt_engine.h underneath: the iterate (x, y, tau, s), K^{-1}q, the Newton
+// directions and every scalar of an iteration (mu, tau+, alpha, sigma, the
+// termination quantities) live in HBM.  The host launches one iteration's
+// kernels and reads back one record; the only other host synchronisations are
+// the factor's breakdown flag and the refinement loop's residual norms.
+//
+// Each vector phase is a grid-stride kernel whose workgroups write partial
+// sums / maxima / minima to `vpart` (kSlots per workgroup), followed by a
+// one-workgroup "fin" kernel that combines them in a fixed order and applies
+// the scalar logic of the reference, storing the result in the device
+// scalars `Scal`.  Reductions are deterministic for a given problem size.
+//
+// Termination uses the equilibrated data: with A_eq = D A E, P_eq = E P E and
+// the equilibrated iterate, A x_u = (A_eq x)/d, A' y_u = (A_eq' y)/e and
+// P x_u = (P_eq x)/e, while b and c are the unscaled vectors.
+#include "kkt_engine.h"
+#include "qtqp_ipm.h"
+
+using namespace qk;
+
+namespace {
+
+constexpr double kEps = 1e-15;  // _EPS (__init__.py:40)
+constexpr int kSlots = 24;      // partial results per workgroup
+constexpr int kMaxVecBlocks = 240;
+
+// Scalars of the iteration in progress (device memory).
+struct Scal {
+  double tau, scale, mu, sigma, alpha_p, alpha, step, tau_p, tau_c;
+  double mean_sy, mean_soy;
+  double norm_b, norm_c;  // ||b||_inf, ||c||_inf of the unscaled data
+};
+
+__device__ inline void put_sum(double v, double* part, int k, double* red) {
+  v = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x * kSlots + k] = v;
+}
+__device__ inline void put_max(double v, double* part, int k, double* red) {
+  v = block_max(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x * kSlots + k] = v;
+}
+__device__ inline void put_min(double v, double* part, int k, double* red) {
+  v = -block_max(-v, red);
+  if (threadIdx.x == 0) part[blockIdx.x * kSlots + k] = v;
+}
+__device__ inline double fin_sum(const double* part, int nb, int k, double* red) {
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += WG) v += part[b * kSlots + k];
+  return block_sum(v, red);
+}
+__device__ inline double fin_max(const double* part, int nb, int k, double* red) {
+  double v = -INFINITY;
+  for (int b = threadIdx.x; b < nb; b += WG) v = fmax(v, part[b * kSlots + k]);
+  return block_max(v, red);
+}
+__device__ inline double fin_min(const double* part, int nb, int k, double* red) {
+  double v = INFINITY;
+  for (int b = threadIdx.x; b < nb; b += WG) v = fmin(v, part[b * kSlots + k]);
+  return -block_max(-v, red);
+}
+// |v| for an inf-norm; NaN is reported as inf (fmax drops NaN).
+__device__ inline double absn(double v) { return v != v ? INFINITY : fabs(v); }
+
+#define GRID_STRIDE(k, count) \
+  for (int64_t k = blockIdx.x * (int64_t)WG + threadIdx.x; k < (count); k += (int64_t)gridDim.x * WG)
+
+// ---------------------------------------------------------------------------
+// Equilibration (__init__.py:616-663) and start (__init__.py:299-307)
+// ---------------------------------------------------------------------------
+
+// Workgroup b takes rows b*RB .. +RB of the row-major M (rows x ncol): the
+// row inf-norms into rowmax (if given) and the column maxima over its rows
+// into cpart[b * ncol + j].
+__global__ __launch_bounds__(WG) void k_ruiz_rows(const double* __restrict__ M, int rows, int ncol,
+                                                  double* __restrict__ rowmax,
+                                                  double* __restrict__ cpart) {
+  __shared__ double red[RB][4];
+  const int r0 = blockIdx.x * RB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double rm[RB];
+#pragma unroll
+  for (int q = 0; q < RB; ++q) rm[q] = 0.0;
+  for (int j = tid; j < ncol; j += WG) {
+    double cm = 0.0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      if (r0 + q < rows) {
+        const double a = fabs(M[(int64_t)(r0 + q) * ncol + j]);
+        rm[q] = fmax(rm[q], a);
+        cm = fmax(cm, a);
+      }
+    }
+    cpart[(int64_t)blockIdx.x * ncol + j] = cm;
+  }
+  if (rowmax) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const double v = wave_max(rm[q]);
+      if (lane == 0) red[q][wave] = v;
+    }
+    __syncthreads();
+    if (tid < RB && r0 + tid < rows)
+      rowmax[r0 + tid] = fmax(fmax(red[tid][0], red[tid][1]), fmax(red[tid][2], red[tid][3]));
+  }
+}
+
+__device__ inline double ruiz_factor(double nrm, double lo, double hi) {
+  // np.clip(1 / sqrt(where(nrm == 0, 1, nrm)), lo, hi)
+  const double v = 1.0 / sqrt(nrm == 0.0 ? 1.0 : nrm);
+  return fmin(fmax(v, lo), hi);
+}
+
+// One pass's scaling factors: d_i from the row norms of A, e_i from the
+// column norms of A and P (P = P_off + diag(pdiag)); d *= d_i, e *= e_i.
+__global__ __launch_bounds__(WG) void k_ruiz_factors(int m, int n, const double* __restrict__ rowmax,
+                                                     const double* __restrict__ cpa, int nba,
+                                                     const double* __restrict__ cpp, int nbp,
+                                                     const double* __restrict__ pdiag,
+                                                     double* __restrict__ di, double* __restrict__ ei,
+                                                     double* __restrict__ d, double* __restrict__ e) {
+  GRID_STRIDE(k, (int64_t)n + m) {
+    if (k < n) {
+      double col = fabs(pdiag[k]);
+      for (int b = 0; b < nba; ++b) col = fmax(col, cpa[(int64_t)b * n + k]);
+      for (int b = 0; b < nbp; ++b) col = fmax(col, cpp[(int64_t)b * n + k]);
+      const double f = ruiz_factor(col, 1e-3, 1e3);
+      ei[k] = f;
+      e[k] *= f;
+    } else {
+      const int i = (int)(k - n);
+      const double f = ruiz_factor(rowmax[i], 1e-3, 1e3);
+      di[i] = f;
+      d[i] *= f;
+    }
+  }
+}
+
+// M[i][j] *= (fr[i] * fc[j])  (the reference's a.data *= d_i[rows] * e_i[cols]);
+// workgroups take rows in turn, threads stride along the row.
+__global__ void k_scale_rows_cols(double* __restrict__ M, int rows, int ncol,
+                                  const double* __restrict__ fr, const double* __restrict__ fc) {
+  for (int i = blockIdx.x; i < rows; i += gridDim.x) {
+    double* row = M + (int64_t)i * ncol;
+    const double f = fr[i];
+    for (int j = threadIdx.x; j < ncol; j += WG) row[j] *= f * fc[j];
+  }
+}
+
+__global__ void k_scale_pdiag(double* __restrict__ pdiag, int n, const double* __restrict__ ei) {
+  GRID_STRIDE(k, n) pdiag[k] *= ei[k] * ei[k];
+}
+
+// q = [c*e; b*d]; default start x = 0, y[z:] = 1/d, s[z:] = d (the
+// equilibrated image of y = s = 1), tau = 1; K^{-1}q = 0.
+__global__ void k_start(int n, int m, int z, const double* __restrict__ b,
+                        const double* __restrict__ c, const double* __restrict__ d,
+                        const double* __restrict__ e, double* __restrict__ q,
+                        double* __restrict__ xy, double* __restrict__ s, double* __restrict__ kq,
+                        Scal* S, double norm_b, double norm_c) {
+  GRID_STRIDE(k, (int64_t)n + m) {
+    if (k < n) {
+      q[k] = c[k] * e[k];
+      xy[k] = 0.0 / e[k];
+    } else {
+      const int i = (int)(k - n);
+      q[k] = b[i] * d[i];
+      const double one = i < z ? 0.0 : 1.0;
+      xy[k] = one / d[i];
+      s[i] = one * d[i];
+    }
+    kq[k] = 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->tau = 1.0;
+    S->norm_b = norm_b;
+    S->norm_c = norm_c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One iteration
+// ---------------------------------------------------------------------------
+
+// ||x||^2, ||y||^2, y's  (__init__.py:859-880, 477).
+__global__ __launch_bounds__(WG) void k_norm_part(int n, int m, const double* __restrict__ xy,
+                                                  const double* __restrict__ s, double* part) {
+  __shared__ double red[4];
+  double xx = 0.0, yy = 0.0, ys = 0.0;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    const double v = xy[k];
+    if (k < n) {
+      xx += v * v;
+    } else {
+      yy += v * v;
+      ys += v * s[k - n];
+    }
+  }
+  put_sum(xx, part, 0, red);
+  put_sum(yy, part, 1, red);
+  put_sum(ys, part, 2, red);
+}
+
+// scale = sqrt(m - z + 1) / max(eps, ||(x, y, tau)||), tau *= scale,
+// mu = (scale y)'(scale s) / (m - z).
+__global__ __launch_bounds__(WG) void k_norm_fin(int m, int z, const double* part, int nb, Scal* S) {
+  __shared__ double red[4];
+  const double xx = fin_sum(part, nb, 0, red);
+  const double yy = fin_sum(part, nb, 1, red);
+  const double ys = fin_sum(part, nb, 2, red);
+  if (threadIdx.x == 0) {
+    const double tau = S->tau;
+    const double scale = sqrt((double)(m - z + 1)) / fmax(kEps, sqrt(xx + yy + tau * tau));
+    S->scale = scale;
+    S->tau = tau * scale;
+    S->mu = ys * (scale * scale) / (m - z);
+  }
+}
+
+// Applies the normalisation and forms the regularised KKT diagonal and its
+// correction (DirectKktSolver.update, direct.py:196-216):
+//   t = [diag(P) + mu; mu (z rows); s/y + mu],  reg = max(t, min_reg),
+//   diag = [reg_x; -reg_y],  corr = diag - [t_x; -t_y].
+__global__ void k_scale_diag(int n, int m, int z, double* __restrict__ xy, double* __restrict__ s,
+                             const double* __restrict__ pdiag, double min_reg, const Scal* S,
+                             double* __restrict__ diag, double* __restrict__ corr) {
+  const double sc = S->scale, mu = S->mu;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    double t;
+    if (k < n) {
+      xy[k] *= sc;
+      t = pdiag[k] + mu;
+    } else {
+      const int i = (int)(k - n);
+      const double yv = xy[k] * sc, sv = s[i] * sc;
+      xy[k] = yv;
+      s[i] = sv;
+      t = i < z ? mu : sv / yv + mu;
+    }
+    const double rg = fmax(t, min_reg);
+    if (k < n) {
+      diag[k] = rg;
+      corr[k] = rg - t;
+    } else {
+      diag[k] = -rg;
+      corr[k] = -rg - (-t);
+    }
+  }
+}
+
+// out0 = P v0 (and out1 = P v1), P = P_off + diag(pdiag), one wave per row.
+template <int NV>
+__global__ __launch_bounds__(WG) void k_pmul(const double* __restrict__ Poff,
+                                             const double* __restrict__ pdiag, int n,
+                                             const double* __restrict__ v0, double* __restrict__ o0,
+                                             const double* __restrict__ v1, double* __restrict__ o1) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const double* pr = Poff + (int64_t)row * n;
+  double a0 = 0.0, a1 = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double p = pr[j];
+    a0 += p * v0[j];
+    if (NV == 2) a1 += p * v1[j];
+  }
+  a0 = wave_sum(a0);
+  if (NV == 2) a1 = wave_sum(a1);
+  if (lane == 0) {
+    o0[row] = a0 + pdiag[row] * v0[row];
+    if (NV == 2) o1[row] = a1 + pdiag[row] * v1[row];
+  }
+}
+
+// Right-hand side of a Newton step (__init__.py:722-731):
+//   r = (mu - mu_t) anchor;  r[n+z:] += mu_t / y (if mu_t != 0) + s + corr.
+// Predictor (corrector = 0): mu_t = 0, anchor = [x; y], copied into xp.
+// Corrector: mu_t = sigma mu, anchor = xp = [x_p; y_p], and the Mehrotra term
+// corr = -d_s (y_p - y) / y (__init__.py:528) is formed into cm.
+__global__ void k_newton_rhs(int n, int m, int z, int corrector, const double* __restrict__ xy,
+                             const double* __restrict__ s, const double* __restrict__ ds,
+                             double* __restrict__ xp, double* __restrict__ cm,
+                             double* __restrict__ r, const Scal* S) {
+  const double mu = S->mu;
+  const double mu_t = corrector ? S->sigma * mu : 0.0;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    double a;
+    if (corrector) {
+      a = xp[k];
+    } else {
+      a = xy[k];
+      xp[k] = a;
+    }
+    double v = (mu - mu_t) * a;
+    if (k >= n) {
+      const int i = (int)(k - n);
+      if (i >= z) {
+        const double yv = xy[k];
+        if (mu_t != 0.0) v += mu_t / yv;
+        v += s[i];
+        if (corrector) {
+          const double c = -ds[i] * (a - yv) / yv;
+          cm[i] = c;
+          v += c;
+        }
+      } else if (corrector) {
+        cm[i] = 0.0;
+      }
+    }
+    r[k] = v;
+  }
+}
+
+// Dot products of the tau equations (__init__.py:759-857):
+//   0 kq.q  1 kr.q  2 kq_x.pq  3 kr_x.pq  4 kq_x.pr  5 kr_x.pr
+//   6 q_x.x  7 q_y.y  8 x.px  9 px.kq_x  10 px.kr_x
+// with pq = P kq_x, pr = P kr_x, px = P x (slots 2-5, 8-10 only with P).
+__global__ __launch_bounds__(WG) void k_tau_part(int n, int m, int has_p, const double* __restrict__ q,
+                                                 const double* __restrict__ kq,
+                                                 const double* __restrict__ kr,
+                                                 const double* __restrict__ xy,
+                                                 const double* __restrict__ pq,
+                                                 const double* __restrict__ pr,
+                                                 const double* __restrict__ px, double* part) {
+  __shared__ double red[4];
+  double v[11];
+#pragma unroll
+  for (int t = 0; t < 11; ++t) v[t] = 0.0;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    const double qk = q[k], kqk = kq[k], krk = kr[k], xk = xy[k];
+    v[0] += kqk * qk;
+    v[1] += krk * qk;
+    if (k < n) {
+      v[6] += qk * xk;
+      if (has_p) {
+        v[2] += kqk * pq[k];
+        v[3] += krk * pq[k];
+        v[4] += kqk * pr[k];
+        v[5] += krk * pr[k];
+        v[8] += xk * px[k];
+        v[9] += px[k] * kqk;
+        v[10] += px[k] * krk;
+      }
+    } else {
+      v[7] += qk * xk;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 11; ++t) put_sum(v[t], part, t, red);
+}
+
+// tau+ of a Newton step: the stable root of t_a tau^2 + t_b tau + t_c = 0
+// (_solve_for_tau, __init__.py:759-817) when `quadratic` is allowed and the
+// root is valid, else the linearised step with its [0.1, 10] trust region
+// (_solve_for_tau_linearized_fallback, __init__.py:819-857).
+__global__ __launch_bounds__(WG) void k_tau_fin(int corrector, int quadratic, int has_p,
+                                                const double* part, int nb, Scal* S,
+                                                qtqp_ipm_record* rec) {
+  __shared__ double red[4];
+  double v[11];
+#pragma unroll
+  for (int t = 0; t < 11; ++t) v[t] = fin_sum(part, nb, t, red);
+  if (threadIdx.x != 0) return;
+  const double mu = S->mu, tau = S->tau;
+  const double mu_t = corrector ? S->sigma * mu : 0.0;
+  const double tau_anchor = corrector ? S->tau_p : tau;
+  const double kq_q = v[0], kr_q = v[1];
+  bool have = false;
+  double tp = 0.0;
+  if (quadratic) {
+    const double r_tau = (mu - mu_t) * tau_anchor;
+    double t_a = mu + kq_q;
+    double t_b = -r_tau - kr_q;
+    double t_c = -mu_t;
+    if (has_p) {
+      t_a -= v[2];
+      t_b += v[3] + v[4];
+      t_c -= v[5];
+    }
+    if (fabs(t_a) < kEps) {
+      if (!(fabs(t_b) < kEps)) {
+        const double root = -t_c / t_b;
+        if (isfinite(root) && !(root < -1e-10)) {
+          tp = fmax(0.0, root);
+          have = true;
+        }
+      }
+    } else {
+      const double disc = t_b * t_b - 4 * t_a * t_c;
+      if (!(disc < -1e-9)) {
+        const double sq = sqrt(fmax(0.0, disc));
+        const double root = t_b > 0 ? t_c / (-0.5 * (t_b + sq)) : (-0.5 * (t_b - sq)) / t_a;
+        if (isfinite(root) && !(root < -1e-10)) {
+          tp = fmax(0.0, root);
+          have = true;
+        }
+      }
+    }
+  }
+  if (!have) {
+    const double x_px = has_p ? v[8] : 0.0, px_kq = has_p ? v[9] : 0.0, px_kr = has_p ? v[10] : 0.0;
+    const double q_z = v[6] + v[7];
+    const double q_kq = kq_q;
+    const double q_rz = kr_q - tau * q_kq - q_z;
+    const double px_rz = px_kr - tau * px_kq - x_px;
+    const double g = mu * tau * tau + (mu_t - mu) * tau_anchor * tau - tau * q_z - mu_t - x_px;
+    const double num = g - tau * q_rz - 2.0 * px_rz;
+    const double den = 2.0 * mu * tau + (mu_t - mu) * tau_anchor - q_z + tau * q_kq + 2.0 * px_kq;
+    const double tn = tau + (fabs(den) < 1e-16 ? 0.0 : -num / den);
+    tp = !isfinite(tn) ? tau : fmin(fmax(tn, 0.1 * tau), 10.0 * tau);
+  }
+  if (corrector) S->tau_c = tp;
+  else S->tau_p = tp;
+  rec->tau_method[corrector] = have ? 1 : 0;
+}
+
+// [x+; y+] = kr - kq tau+  (__init__.py:755-757).
+__global__ void k_newton_fin(int64_t count, int corrector, const double* __restrict__ kr,
+                             const double* __restrict__ kq, double* __restrict__ xp, const Scal* S) {
+  const double tp = corrector ? S->tau_c : S->tau_p;
+  GRID_STRIDE(k, count) xp[k] = kr[k] - kq[k] * tp;
+}
+
+// d_s over the inequality rows and the fraction-to-boundary minima
+// (__init__.py:507-510, 546-547, 671-680):
+//   predictor: d_s = -y_p s / y;  corrector: d_s = sigma mu / y + corr - y_c s / y;
+//   slot 0: min -s/d_s over d_s < -eps;  slot 1: min -y/d_y over d_y < -eps.
+__global__ __launch_bounds__(WG) void k_step_part(int n, int m, int z, int corrector,
+                                                  const double* __restrict__ xy,
+                                                  const double* __restrict__ s,
+                                                  const double* __restrict__ xp,
+                                                  const double* __restrict__ cm,
+                                                  double* __restrict__ ds, const Scal* S,
+                                                  double* part) {
+  __shared__ double red[4];
+  const double smu = corrector ? S->sigma * S->mu : 0.0;
+  double mn_s = INFINITY, mn_y = INFINITY;
+  GRID_STRIDE(i, m) {
+    double d = 0.0;
+    if (i >= z) {
+      const double yv = xy[n + i], sv = s[i], yn = xp[n + i];
+      d = corrector ? smu / yv + cm[i] - yn * sv / yv : -yn * sv / yv;
+      if (d < -kEps) mn_s = fmin(mn_s, -sv / d);
+      const double dy = yn - yv;
+      if (dy < -kEps) mn_y = fmin(mn_y, -yv / dy);
+    }
+    ds[i] = d;
+  }
+  put_min(mn_s, part, 0, red);
+  put_min(mn_y, part, 1, red);
+}
+
+// alpha = min(max step on s, max step on y) (__init__.py:882-886); for the
+// corrector also step = scale * alpha and tau = max(tau + step (tau_c - tau), 1e-30).
+__global__ __launch_bounds__(WG) void k_alpha_fin(int corrector, double step_scale,
+                                                  const double* part, int nb, Scal* S) {
+  __shared__ double red[4];
+  const double ms = fin_min(part, nb, 0, red), my = fin_min(part, nb, 1, red);
+  if (threadIdx.x != 0) return;
+  const double as = ms == INFINITY ? 1.0 : fmin(1.0, ms);
+  const double ay = my == INFINITY ? 1.0 : fmin(1.0, my);
+  const double alpha = fmin(as, ay);
+  if (!corrector) {
+    S->alpha_p = alpha;
+    return;
+  }
+  S->alpha = alpha;
+  const double step = step_scale * alpha;
+  S->step = step;
+  S->tau = fmax(S->tau + step * (S->tau_c - S->tau), 1e-30);
+}
+
+// Affine trial point norms for sigma (__init__.py:682-706).
+__global__ __launch_bounds__(WG) void k_sigma_part(int n, int m, const double* __restrict__ xy,
+                                                   const double* __restrict__ s,
+                                                   const double* __restrict__ xp,
+                                                   const double* __restrict__ ds, const Scal* S,
+                                                   double* part) {
+  __shared__ double red[4];
+  const double al = S->alpha_p;
+  double sx = 0.0, sy = 0.0, sys = 0.0;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    const double v = xy[k];
+    const double va = v + al * (xp[k] - v);
+    if (k < n) {
+      sx += va * va;
+    } else {
+      const int i = (int)(k - n);
+      const double sa = s[i] + al * ds[i];
+      sy += va * va;
+      sys += va * sa;
+    }
+  }
+  put_sum(sx, part, 0, red);
+  put_sum(sy, part, 1, red);
+  put_sum(sys, part, 2, red);
+}
+
+__global__ __launch_bounds__(WG) void k_sigma_fin(int m, int z, const double* part, int nb, Scal* S) {
+  __shared__ double red[4];
+  const double sx = fin_sum(part, nb, 0, red), sy = fin_sum(part, nb, 1, red);
+  const double sys = fin_sum(part, nb, 2, red);
+  if (threadIdx.x != 0) return;
+  const double tau = S->tau, al = S->alpha_p;
+  const double ta = tau + al * (S->tau_p - tau);
+  const double nrm2 = sx + sy + ta * ta;
+  const int k = m - z;
+  const double mu_aff = (k + 1) / fmax(kEps * kEps, nrm2) * sys / k;
+  const double base = mu_aff / fmax(kEps, S->mu);
+  S->sigma = fmin(fmax(base * base * base, 0.0), 1.0);
+}
+
+// x += step (x_c - x), y += step (y_c - y), s += step d_s, clamps
+// (__init__.py:556-565).
+__global__ void k_update(int n, int m, int z, double* __restrict__ xy, double* __restrict__ s,
+                         const double* __restrict__ xp, const double* __restrict__ ds,
+                         const Scal* S) {
+  const double step = S->step;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    const double v = xy[k];
+    double nv = v + step * (xp[k] - v);
+    if (k >= n) {
+      const int i = (int)(k - n);
+      double sv = s[i] + step * ds[i];
+      if (i >= z) {
+        nv = fmax(nv, 1e-30);
+        sv = fmax(sv, 1e-30);
+      }
+      s[i] = sv;
+    }
+    xy[k] = nv;
+  }
+}
+
+// Termination quantities on the unscaled problem (__init__.py:888-965).
+// ax = A_eq x (m), aty = A_eq' y (n), pxe = P_eq x (n).
+//   sums: 0 x'Px  1 c'x  2 b'y  3 y's     maxima: 4 |Px| 5 |A'y| 6 dres
+//   7 |x| 8 |Ax| 9 |s| 10 pres 11 |Ax+s| 12 |y|
+//   collect_stats, rows z..m of sy = s*y, soy = s/max(eps, y):
+//   13 max sy 14 min sy 15 sum sy 16 max soy 17 min soy 18 sum soy
+__global__ __launch_bounds__(WG) void k_term_part(int n, int m, int z, int has_p, int collect,
+                                                  const double* __restrict__ xy,
+                                                  const double* __restrict__ s,
+                                                  const double* __restrict__ d,
+                                                  const double* __restrict__ e,
+                                                  const double* __restrict__ ax,
+                                                  const double* __restrict__ aty,
+                                                  const double* __restrict__ pxe,
+                                                  const double* __restrict__ b,
+                                                  const double* __restrict__ c, const Scal* S,
+                                                  double* part) {
+  __shared__ double red[4];
+  const double inv_tau = 1.0 / fmax(S->tau, kEps);
+  double xpx = 0.0, ctx = 0.0, bty = 0.0, ys = 0.0;
+  double n_px = 0.0, n_aty = 0.0, dres = 0.0, n_x = 0.0;
+  double n_ax = 0.0, n_s = 0.0, pres = 0.0, n_axs = 0.0, n_y = 0.0;
+  double sy_max = -INFINITY, sy_min = INFINITY, sy_sum = 0.0;
+  double so_max = -INFINITY, so_min = INFINITY, so_sum = 0.0;
+  GRID_STRIDE(k, (int64_t)n + m) {
+    if (k < n) {
+      const double xu = e[k] * xy[k];
+      const double pxu = has_p ? pxe[k] / e[k] : 0.0;
+      const double atyu = aty[k] / e[k];
+      xpx += xu * pxu;
+      ctx += c[k] * xu;
+      n_px = fmax(n_px, absn(pxu));
+      n_aty = fmax(n_aty, absn(atyu));
+      dres = fmax(dres, absn((pxu + atyu) * inv_tau + c[k]));
+      n_x = fmax(n_x, absn(xu));
+    } else {
+      const int i = (int)(k - n);
+      const double yu = d[i] * xy[k], su = s[i] / d[i], axu = ax[i] / d[i];
+      bty += b[i] * yu;
+      ys += yu * su;
+      n_ax = fmax(n_ax, absn(axu));
+      n_s = fmax(n_s, absn(su));
+      pres = fmax(pres, absn((axu + su) * inv_tau - b[i]));
+      n_axs = fmax(n_axs, absn(axu + su));
+      n_y = fmax(n_y, absn(yu));
+      if (collect && i >= z) {
+        const double sy = su * yu, so = su / fmax(kEps, yu);
+        sy_max = fmax(sy_max, sy);
+        sy_min = fmin(sy_min, sy);
+        sy_sum += sy;
+        so_max = fmax(so_max, so);
+        so_min = fmin(so_min, so);
+        so_sum += so;
+      }
+    }
+  }
+  put_sum(xpx, part, 0, red);
+  put_sum(ctx, part, 1, red);
+  put_sum(bty, part, 2, red);
+  put_sum(ys, part, 3, red);
+  put_max(n_px, part, 4, red);
+  put_max(n_aty, part, 5, red);
+  put_max(dres, part, 6, red);
+  put_max(n_x, part, 7, red);
+  put_max(n_ax, part, 8, red);
+  put_max(n_s, part, 9, red);
+  put_max(pres, part, 10, red);
+  put_max(n_axs, part, 11, red);
+  put_max(n_y, part, 12, red);
+  if (collect) {
+    put_max(sy_max, part, 13, red);
+    put_min(sy_min, part, 14, red);
+    put_sum(sy_sum, part, 15, red);
+    put_max(so_max, part, 16, red);
+    put_min(so_min, part, 17, red);
+    put_sum(so_sum, part, 18, red);
+  }
+}
+
+struct Tol {
+  double atol, rtol, atol_inf, rtol_inf;
+};
+
+// Costs, residuals, certificates and the status (__init__.py:905-1008).
+__global__ __launch_bounds__(WG) void k_term_fin(int m, int z, int has_p, int collect, Tol tl,
+                                                 const double* part, int nb, Scal* S,
+                                                 qtqp_ipm_record* rec) {
+  __shared__ double red[4];
+  double v[19];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) v[t] = fin_sum(part, nb, t, red);
+#pragma unroll
+  for (int t = 4; t < 13; ++t) v[t] = fin_max(part, nb, t, red);
+  if (collect) {
+    v[13] = fin_max(part, nb, 13, red);
+    v[14] = fin_min(part, nb, 14, red);
+    v[15] = fin_sum(part, nb, 15, red);
+    v[16] = fin_max(part, nb, 16, red);
+    v[17] = fin_min(part, nb, 17, red);
+    v[18] = fin_sum(part, nb, 18, red);
+  }
+  if (threadIdx.x != 0) return;
+  const double tau = S->tau;
+  const double inv_tau = 1.0 / fmax(tau, kEps);
+  const double xpx = has_p ? v[0] : 0.0, ctx = v[1], bty = v[2];
+  const double norm_px = v[4], norm_aty = v[5], dres = v[6], norm_x = v[7];
+  const double norm_ax = v[8], norm_s = v[9], pres = v[10], norm_axs = v[11], norm_y = v[12];
+  const double pcost = (ctx + 0.5 * xpx * inv_tau) * inv_tau;
+  const double dcost = (-bty - 0.5 * xpx * inv_tau) * inv_tau;
+  const double gap = fabs((ctx + bty + xpx * inv_tau) * inv_tau);
+  const double dinfeas_a = norm_axs / (fabs(ctx) + kEps);
+  const double dinfeas_p = norm_px / (fabs(ctx) + kEps);
+  const double dinfeas = fmax(dinfeas_a, dinfeas_p);
+  const double pinfeas = norm_aty / (fabs(bty) + kEps);
+  const double prelrhs = fmax(fmax(norm_ax * inv_tau, norm_s * inv_tau), S->norm_b);
+  const double drelrhs = fmax(fmax(norm_px * inv_tau, norm_aty * inv_tau), S->norm_c);
+  int status = QTQP_IPM_UNFINISHED;
+  if (gap < tl.atol + tl.rtol * fmin(fabs(pcost), fabs(dcost)) &&
+      pres < tl.atol + tl.rtol * prelrhs && dres < tl.atol + tl.rtol * drelrhs)
+    status = QTQP_IPM_SOLVED;
+  else if (ctx < -kEps && dinfeas < tl.atol_inf + tl.rtol_inf * norm_x / fabs(ctx))
+    status = QTQP_IPM_UNBOUNDED;
+  else if (bty < -kEps && pinfeas < tl.atol_inf + tl.rtol_inf * norm_y / fabs(bty))
+    status = QTQP_IPM_INFEASIBLE;
+  rec->status = status;
+  rec->ctx = ctx;
+  rec->bty = bty;
+  rec->pcost = pcost;
+  rec->dcost = dcost;
+  rec->pres = pres;
+  rec->dres = dres;
+  rec->gap = gap;
+  rec->pinfeas = pinfeas;
+  rec->dinfeas = dinfeas;
+  rec->dinfeas_a = dinfeas_a;
+  rec->dinfeas_p = dinfeas_p;
+  rec->mu = S->mu;
+  rec->sigma = S->sigma;
+  rec->alpha = S->alpha;
+  rec->tau = tau;
+  rec->norm_x = norm_x;
+  rec->norm_y = norm_y;
+  rec->prelrhs = prelrhs;
+  rec->drelrhs = drelrhs;
+  if (collect) {
+    const int k = m - z;
+    rec->complementarity = fabs(v[3] * inv_tau * inv_tau);
+    rec->norm_s = norm_s;
+    rec->max_sy = v[13];
+    rec->min_sy = v[14];
+    rec->max_s_over_y = v[16];
+    rec->min_s_over_y = v[17];
+    S->mean_sy = v[15] / k;
+    S->mean_soy = v[18] / k;
+    rec->mean_s_over_y = S->mean_soy;
+  }
+}
+
+// Second pass of the standard deviations (np.std: sqrt(mean(|v - mean|^2))).
+__global__ __launch_bounds__(WG) void k_std_part(int n, int m, int z, const double* __restrict__ xy,
+                                                 const double* __restrict__ s,
+                                                 const double* __restrict__ d, const Scal* S,
+                                                 double* part) {
+  __shared__ double red[4];
+  const double msy = S->mean_sy, mso = S->mean_soy;
+  double a = 0.0, b = 0.0;
+  GRID_STRIDE(i, m) {
+    if (i >= z) {
+      const double yu = d[i] * xy[n + i], su = s[i] / d[i];
+      const double dsy = su * yu - msy, dso = su / fmax(kEps, yu) - mso;
+      a += dsy * dsy;
+      b += dso * dso;
+    }
+  }
+  put_sum(a, part, 0, red);
+  put_sum(b, part, 1, red);
+}
+
+__global__ __launch_bounds__(WG) void k_std_fin(int m, int z, const double* part, int nb,
+                                                qtqp_ipm_record* rec) {
+  __shared__ double red[4];
+  const double a = fin_sum(part, nb, 0, red), b = fin_sum(part, nb, 1, red);
+  if (threadIdx.x != 0) return;
+  rec->std_sy = sqrt(a / (m - z));
+  rec->std_s_over_y = sqrt(b / (m - z));
+}
+
+}  // namespace
+
+struct qtqp_ipm {
+  qtqp_kkt* k = nullptr;  // KKT engine: equilibrated A and P_off, factor, refinement buffers
+  int64_t n = 0, m = 0, z = 0;
+  int has_p = 0;
+  int nv = 1;  // workgroups of the vector kernels
+  double *pdiag = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
+  double *q = nullptr, *xy = nullptr, *s = nullptr, *kq = nullptr, *xp = nullptr, *kr = nullptr;
+  double *r = nullptr, *ds = nullptr, *cm = nullptr, *px = nullptr, *pq = nullptr, *pr = nullptr;
+  double *ax = nullptr, *aty = nullptr, *pxe = nullptr;
+  double *vpart = nullptr, *cpartp = nullptr, *rowmax = nullptr, *di = nullptr, *ei = nullptr;
+  Scal* S = nullptr;
+  qtqp_ipm_record* drec = nullptr;
+};
+
+namespace {
+
+void ipm_release(qtqp_ipm* h) {
+  void* bufs[] = {h->pdiag, h->b,  h->c,  h->d,  h->e,   h->q,      h->xy,     h->s,
+                  h->kq,    h->xp, h->kr, h->r,  h->ds,  h->cm,     h->px,     h->pq,
+                  h->pr,    h->ax, h->aty, h->pxe, h->vpart, h->cpartp, h->rowmax, h->di,
+                  h->ei,    h->S,  h->drec};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (h->k) {
+    release(h->k);
+    delete h->k;
+  }
+}
+
+unsigned pmul_blocks(int64_t n) { return (unsigned)((n + WG / 64 - 1) / (WG / 64)); }
+
+// Device Ruiz equilibration (_equilibrate, __init__.py:616-663), in place on
+// the engine's A and P_off and on pdiag; d and e accumulate the scalings.
+int ruiz(qtqp_ipm* h) {
+  qtqp_kkt* k = h->k;
+  const int n = (int)h->n, m = (int)h->m;
+  const int nba = nblk_rows(m), nbp = nblk_rows(n);
+  for (int pass = 0; pass < 10; ++pass) {
+    hipLaunchKernelGGL(k_ruiz_rows, dim3(nba), dim3(WG), 0, k->stream, k->A64, m, n, h->rowmax,
+                       k->part);
+    hipLaunchKernelGGL(k_ruiz_rows, dim3(nbp), dim3(WG), 0, k->stream, k->Poff64, n, n, nullptr,
+                       h->cpartp);
+    hipLaunchKernelGGL(k_ruiz_factors, dim3(blocks_for(n + m)), dim3(WG), 0, k->stream, m, n,
+                       h->rowmax, k->part, nba, h->cpartp, nbp, h->pdiag, h->di, h->ei, h->d, h->e);
+    hipLaunchKernelGGL(k_scale_rows_cols, dim3(2048), dim3(WG), 0, k->stream, k->A64, m, n, h->di,
+                       h->ei);
+    hipLaunchKernelGGL(k_scale_rows_cols, dim3(1024), dim3(WG), 0, k->stream, k->Poff64, n, n,
+                       h->ei, h->ei);
+    hipLaunchKernelGGL(k_scale_pdiag, dim3(blocks_for(n)), dim3(WG), 0, k->stream, h->pdiag, n,
+                       h->ei);
+    HIPC(hipGetLastError());
+  }
+  return QTQP_OK;
+}
+
+__global__ void k_fill(double* v, int64_t count, double val) {
+  GRID_STRIDE(k, count) v[k] = val;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qtqp_ipm_create(int64_t n, int64_t m, int64_t z, const double* a, const double* p,
+                    int32_t col_major, int32_t has_p, const double* b, const double* c,
+                    int32_t precision, int32_t equilibrate, qtqp_ipm** out) {
+  if (!out) return fail(QTQP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (!b || !c) return fail(QTQP_ERR_ARG, "b and c are required");
+  if (z < 0 || z >= m) return fail(QTQP_ERR_ARG, "need 0 <= z < m");
+  if (m > (int64_t)1 << 30) return fail(QTQP_ERR_ARG, "problem too large");
+  qtqp_kkt* k = nullptr;
+  int rc = engine_create(n, m, a, p, precision, false, &k, col_major != 0);
+  if (rc) return rc;
+  auto* h = new qtqp_ipm();
+  h->k = k;
+  h->n = n;
+  h->m = m;
+  h->z = z;
+  h->has_p = has_p ? 1 : 0;
+  const int64_t N = n + m;
+  h->nv = (int)std::min<int64_t>(kMaxVecBlocks, std::max<int64_t>(1, (N + 4 * WG - 1) / (4 * WG)));
+  hipError_t e = hipSuccess;
+  auto bad = [&](const char* what) {
+    std::string msg = std::string("allocation/copy failed: ") + what + ": " + hipGetErrorString(e);
+    ipm_release(h);
+    delete h;
+    return fail(QTQP_ERR_HIP, msg);
+  };
+#define ALLOC(field, T, count) \
+  if ((h->field = dalloc<T>((count), &e)), e != hipSuccess) return bad(#field)
+  ALLOC(pdiag, double, n);
+  ALLOC(b, double, m);
+  ALLOC(c, double, n);
+  ALLOC(d, double, m);
+  ALLOC(e, double, n);
+  ALLOC(q, double, N);
+  ALLOC(xy, double, N);
+  ALLOC(s, double, m);
+  ALLOC(kq, double, N);
+  ALLOC(xp, double, N);
+  ALLOC(kr, double, N);
+  ALLOC(r, double, N);
+  ALLOC(ds, double, m);
+  ALLOC(cm, double, m);
+  ALLOC(px, double, n);
+  ALLOC(pq, double, n);
+  ALLOC(pr, double, n);
+  ALLOC(ax, double, m);
+  ALLOC(aty, double, n);
+  ALLOC(pxe, double, n);
+  ALLOC(vpart, double, (int64_t)kMaxVecBlocks * kSlots);
+  ALLOC(cpartp, double, (int64_t)nblk_rows(n) * n);
+  ALLOC(rowmax, double, m);
+  ALLOC(di, double, m);
+  ALLOC(ei, double, n);
+  ALLOC(S, Scal, 1);
+  ALLOC(drec, qtqp_ipm_record, 1);
+#undef ALLOC
+  const hipStream_t st = k->stream;
+  std::vector<double> pd((size_t)n, 0.0);
+  if (p)
+    for (int64_t i = 0; i < n; ++i) pd[i] = p[i * n + i];
+  double norm_b = 0.0, norm_c = 0.0;
+  for (int64_t i = 0; i < m; ++i) norm_b = std::max(norm_b, std::fabs(b[i]));
+  for (int64_t j = 0; j < n; ++j) norm_c = std::max(norm_c, std::fabs(c[j]));
+  if ((e = hipMemcpyAsync(h->pdiag, pd.data(), sizeof(double) * n, hipMemcpyHostToDevice, st)))
+    return bad("pdiag");
+  if ((e = hipMemcpyAsync(h->b, b, sizeof(double) * m, hipMemcpyHostToDevice, st))) return bad("b");
+  if ((e = hipMemcpyAsync(h->c, c, sizeof(double) * n, hipMemcpyHostToDevice, st))) return bad("c");
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(m)), dim3(WG), 0, st, h->d, m, 1.0);
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(WG), 0, st, h->e, n, 1.0);
+  if ((e = hipMemsetAsync(h->S, 0, sizeof(Scal), st))) return bad("scalars");
+  if ((e = hipMemsetAsync(h->drec, 0, sizeof(qtqp_ipm_record), st))) return bad("record");
+  if ((e = hipGetLastError())) return bad("fill");
+  if (equilibrate && (rc = ruiz(h))) {
+    ipm_release(h);
+    delete h;
+    return rc;
+  }
+  if ((rc = cast_fp32(k))) {
+    ipm_release(h);
+    delete h;
+    return rc;
+  }
+  hipLaunchKernelGGL(k_start, dim3(blocks_for(N)), dim3(WG), 0, st, (int)n, (int)m, (int)z, h->b,
+                     h->c, h->d, h->e, h->q, h->xy, h->s, h->kq, h->S, norm_b, norm_c);
+  if ((e = hipGetLastError())) return bad("start");
+  if ((e = hipStreamSynchronize(st))) return bad("sync");
+  *out = h;
+  return QTQP_OK;
+}
+
+int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
+  if (!h || !o || !rec) return fail(QTQP_ERR_ARG, "NULL argument");
+  if (o->max_iterative_refinement_steps < 1)
+    return fail(QTQP_ERR_ARG, "max_iterative_refinement_steps must be >= 1");
+  qtqp_kkt* k = h->k;
+  const hipStream_t st = k->stream;
+  const int n = (int)h->n, m = (int)h->m, z = (int)h->z;
+  const int64_t N = h->n + h->m;
+  const int nv = h->nv;
+  const dim3 gv(nv), gb(WG), g1(1);
+  const unsigned gN = blocks_for(N);
+  double* part = h->vpart;
+  int rc;
+  qtqp_ipm_record out{};
+
+  // normalise, mu, regularised diagonal (__init__.py:474-478, direct.py:183-216)
+  hipLaunchKernelGGL(k_norm_part, gv, gb, 0, st, n, m, h->xy, h->s, part);
+  hipLaunchKernelGGL(k_norm_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+  hipLaunchKernelGGL(k_scale_diag, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->pdiag,
+                     o->min_static_regularization, h->S, k->diag, k->corr);
+  HIPC(hipGetLastError());
+  k->have_diag = true;
+  k->have_factor = false;
+  if ((rc = engine_factorize(k))) return rc;
+
+  const int steps = o->max_iterative_refinement_steps;
+  const double latol = o->linear_solver_atol, lrtol = o->linear_solver_rtol;
+  // kinv_q = K^{-1} q, warm-started from the previous one (__init__.py:479-481)
+  if ((rc = ir_refine(k, h->q, h->kq, steps, latol, lrtol, &out.lin[0]))) return rc;
+  if (h->has_p)
+    hipLaunchKernelGGL(k_pmul<2>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
+                       h->px, h->kq, h->pq);
+
+  for (int corrector = 0; corrector < 2; ++corrector) {
+    if (corrector) {
+      // sigma from the affine trial point (__init__.py:503-520)
+      hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 0, h->xy, h->s, h->xp, h->cm, h->ds,
+                         h->S, part);
+      hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 0, o->step_size_scale, part, nv, h->S);
+      hipLaunchKernelGGL(k_sigma_part, gv, gb, 0, st, n, m, h->xy, h->s, h->xp, h->ds, h->S, part);
+      hipLaunchKernelGGL(k_sigma_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+    }
+    // Newton step (__init__.py:708-757)
+    hipLaunchKernelGGL(k_newton_rhs, dim3(gN), gb, 0, st, n, m, z, corrector, h->xy, h->s, h->ds,
+                       h->xp, h->cm, h->r, h->S);
+    HIPC(hipMemcpyAsync(h->kr, h->xp, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    qtqp_ir_stats& ls = out.lin[1 + corrector];
+    if ((rc = ir_refine(k, h->r, h->kr, steps, latol, lrtol, &ls))) return rc;
+    const int quadratic = ls.status == 0 || ls.final_residual_norm < 1e-7;
+    if (h->has_p)
+      hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->kr,
+                         h->pr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_tau_part, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
+                       h->pr, h->px, part);
+    hipLaunchKernelGGL(k_tau_fin, g1, gb, 0, st, corrector, quadratic, h->has_p, part, nv, h->S,
+                       h->drec);
+    hipLaunchKernelGGL(k_newton_fin, dim3(gN), gb, 0, st, N, corrector, h->kr, h->kq, h->xp, h->S);
+    HIPC(hipGetLastError());
+  }
+  // step and update (__init__.py:546-565)
+  hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 1, h->xy, h->s, h->xp, h->cm, h->ds, h->S,
+                     part);
+  hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 1, o->step_size_scale, part, nv, h->S);
+  hipLaunchKernelGGL(k_update, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->xp, h->ds, h->S);
+  HIPC(hipGetLastError());
+
+  // termination (__init__.py:888-1008)
+  if ((rc = rowscols<double>(k, k->A64, m, n, h->xy, h->ax, h->xy + n, nullptr, k->part))) return rc;
+  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(n)), gb, 0, st, k->part, nblk_rows(m), n, nullptr,
+                     h->aty);
+  if (h->has_p)
+    hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
+                       h->pxe, nullptr, nullptr);
+  const int collect = o->collect_stats ? 1 : 0;
+  hipLaunchKernelGGL(k_term_part, gv, gb, 0, st, n, m, z, h->has_p, collect, h->xy, h->s, h->d,
+                     h->e, h->ax, h->aty, h->pxe, h->b, h->c, h->S, part);
+  const Tol tl{o->atol, o->rtol, o->atol_infeas, o->rtol_infeas};
+  hipLaunchKernelGGL(k_term_fin, g1, gb, 0, st, m, z, h->has_p, collect, tl, part, nv, h->S,
+                     h->drec);
+  if (collect) {
+    hipLaunchKernelGGL(k_std_part, gv, gb, 0, st, n, m, z, h->xy, h->s, h->d, h->S, part);
+    hipLaunchKernelGGL(k_std_fin, g1, gb, 0, st, m, z, part, nv, h->drec);
+  }
+  HIPC(hipGetLastError());
+  qtqp_ipm_record dev{};
+  HIPC(hipMemcpyAsync(&dev, h->drec, sizeof(dev), hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  for (int t = 0; t < 3; ++t) dev.lin[t] = out.lin[t];
+  *rec = dev;
+  return QTQP_OK;
+}
+
+int qtqp_ipm_get_iterates(qtqp_ipm* h, double* x, double* y, double* s, double* tau) {
+  if (!h || !x || !y || !s || !tau) return fail(QTQP_ERR_ARG, "NULL argument");
+  const int64_t n = h->n, m = h->m;
+  const hipStream_t st = h->k->stream;
+  std::vector<double> d(m), e(n);
+  Scal sc{};
+  HIPC(hipMemcpyAsync(x, h->xy, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(y, h->xy + n, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(s, h->s, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(d.data(), h->d, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(e.data(), h->e, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(&sc, h->S, sizeof(sc), hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  for (int64_t j = 0; j < n; ++j) x[j] = e[j] * x[j];
+  for (int64_t i = 0; i < m; ++i) {
+    y[i] = d[i] * y[i];
+    s[i] = s[i] / d[i];
+  }
+  *tau = sc.tau;
+  return QTQP_OK;
+}
+
+int qtqp_ipm_get_info(qtqp_ipm* h, qtqp_kkt_info* info) {
+  if (!h || !info) return fail(QTQP_ERR_ARG, "NULL argument");
+  *info = h->k->info_acc;
+  info->factor_precision = h->k->fprec;
+  return QTQP_OK;
+}
+
+void qtqp_ipm_free(qtqp_ipm* h) {
+  if (!h) return;
+  ipm_release(h);
+  delete h;
+}
+
+}  // extern "C"

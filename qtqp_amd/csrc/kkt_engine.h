@@ -1,0 +1,930 @@
+// Internals of the single-instance dense KKT engine (MI355X, gfx950), shared
+// by the LinearSolver backend (kkt.hip) and the device-resident IPM driver
+// (ipm.hip), which are compiled into the same library.
+//
+// K_reg = [ P_off + diag(d_x)   A'       ]     d_x = diag[:n] > 0
+//         [ A                   diag(d_y) ]     d_y = diag[n:] < 0
+//
+// Factorisation: quasi-definite LDL' with the y-block pivoted first.  Its
+// pivots are d_y and its trailing update into the x-block is one symmetric
+// rank-m MFMA product, giving the positive definite Gram block
+//   G = P_off + diag(d_x) + A' W A,   W = diag(-1/d_y)
+// (the Schur complement ScipyDenseSolver forms, solvers_dense.py:99-116),
+// which is then factored by a blocked right-looking LDL' whose trailing
+// updates run on MFMA.  A solve is: g = r_x + A'(W r_y); G x = g by two
+// triangular sweeps; y = W(A x - r_y)  (solvers_dense.py:130-142).
+//
+// Precision modes: FP64 everywhere, or an FP32 factor (FP32 copies of A and
+// P_off, FP32 MFMA) whose solves are refined in FP64 against the exact K_reg
+// until they are as accurate as an FP64 factor; when the FP32 factor breaks
+// down or contracts too slowly the handle refactors in FP64 and stays there.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dense_ops.h"
+#include "qtqp_kkt.h"
+
+// Message of the calling thread's last failure (defined in kkt.hip).
+extern thread_local std::string g_qtqp_last_error;
+
+using namespace qk;
+
+namespace {
+
+
+constexpr int RB = 16;  // rows per workgroup in the row/column passes over A
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_cast(const double* __restrict__ src, T* __restrict__ dst, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x; i < count; i += (int64_t)gridDim.x * WG)
+    dst[i] = (T)src[i];
+}
+
+// G (lower tiles) = P_off + diag(d_x) + A' W A.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
+                                                  const T* __restrict__ Poff,
+                                                  const double* __restrict__ diag,
+                                                  int n, int m, T* __restrict__ G,
+                                                  int64_t ld) {
+  __shared__ T si[KC][NB + 1];
+  __shared__ T sj[KC][NB + 1];
+  int ti, tj;
+  lower_tile(blockIdx.x, &ti, &tj);
+  const int i0 = ti * NB, j0 = tj * NB;
+  RankKTile<T> t;
+  t.zero();
+  const double* dy = diag + n;
+  t.run(A, n, 0, m, i0, j0, n, n, [dy](int k) { return (T)(-1.0 / dy[k]); }, si, sj);
+  t.for_each(i0, j0, [&](int i, int j, T v) {
+    if (i < n && j < n && i >= j) {
+      T g = v + Poff[(int64_t)j * n + i];
+      if (i == j) g += (T)diag[i];
+      G[(int64_t)j * ld + i] = g;
+    }
+  });
+}
+
+// diag(G) += 1e-14 * max(diag(G))  (solvers_dense.py:107-112).  One workgroup.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n) {
+  __shared__ double red[4];
+  double mx = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += WG) mx = fmax(mx, (double)G[(int64_t)i * ld + i]);
+  mx = block_max(mx, red);
+  const double jit = 1e-14 * mx;
+  for (int i = threadIdx.x; i < n; i += WG) G[(int64_t)i * ld + i] = (T)((double)G[(int64_t)i * ld + i] + jit);
+}
+
+// Panel k0, diagonal block: factors it in LDS, stores L11 into G and
+// L11^{-1} into linv + (k0/64) * 2 * 64 * 64 twice: column-major (for the
+// forward sweep), then row-major (for the panel solve and the backward
+// sweep).  One workgroup.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int k0, int* info,
+                                                  T* linv) {
+  __shared__ T S[NB][NB + 1];
+  __shared__ T Li[NB][NB + 1];
+  __shared__ T sinv[NB];
+  const int nb = min(NB, n - k0);
+  load_diag_block(G, ld, k0, nb, S);
+  __syncthreads();
+  potrf_inv_lds(S, Li, sinv, nb, info, k0);
+  store_diag_block(G, ld, k0, nb, S);
+  T* dst = linv + (int64_t)(k0 / NB) * 2 * NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
+    dst[idx] = Li[idx & (NB - 1)][idx >> 6];
+    dst[NB * NB + idx] = Li[idx >> 6][idx & (NB - 1)];
+  }
+}
+
+// Panel k0, rows below the diagonal block: L21 = A21 L11^{-T} on MFMA,
+// workgroup b handling rows k0+NB+b*64 .. +64 (one 16-row strip per wave).
+template <typename T>
+__global__ __launch_bounds__(WG) void k_trsm_panel(T* G, int64_t ld, int n, int k0,
+                                                  const T* linv) {
+  __shared__ T Li[NB][NB + 1];
+  const T* src = linv + (int64_t)(k0 / NB) * 2 * NB * NB + NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += WG) Li[idx >> 6][idx & (NB - 1)] = src[idx];
+  __syncthreads();
+  trsm_mfma_rows<T, 1>(G, ld, n, k0, min(NB, n - k0), k0 + NB + blockIdx.x * NB, Li);
+}
+
+// Trailing update G[t0:, t0:] -= L[t0:, k0:t0] L[t0:, k0:t0]', t0 = k0 + NB.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int k0) {
+  __shared__ T si[KC][NB + 1];
+  __shared__ T sj[KC][NB + 1];
+  int ti, tj;
+  lower_tile(blockIdx.x, &ti, &tj);
+  const int t0 = k0 + NB;
+  const int i0 = t0 + ti * NB, j0 = t0 + tj * NB;
+  RankKTile<T> t;
+  t.zero();
+  // X[k*ldx + i] = L[i][k0+k] (column k0+k of the column-major factor).
+  t.run(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, n, n, [](int) { return T(1); }, si, sj);
+  t.for_each(i0, j0, [&](int i, int j, T v) {
+    if (i < n && j < n && i >= j) G[(int64_t)j * ld + i] -= v;
+  });
+}
+
+// One pass over RB rows of a row-major matrix M (rows x ncol):
+//   rowout[r] = M[r,:] . xv                 (if xv)
+//   part[blk][j] = sum_{r in blk} M[r][j] * u_r   (if uv), with
+//   u_r = uv[r] * (wdiag ? -1/wdiag[r] : 1).
+template <typename T>
+__global__ __launch_bounds__(WG) void k_rowscols(const T* __restrict__ M, int rows, int ncol,
+                                                 const double* __restrict__ xv,
+                                                 double* __restrict__ rowout,
+                                                 const double* __restrict__ uv,
+                                                 const double* __restrict__ wdiag,
+                                                 double* __restrict__ part) {
+  __shared__ double red[RB][4];
+  __shared__ double su[RB];
+  const int r0 = blockIdx.x * RB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (uv && tid < RB) {
+    const int r = r0 + tid;
+    double u = 0.0;
+    if (r < rows) {
+      u = uv[r];
+      if (wdiag) u *= -1.0 / wdiag[r];
+    }
+    su[tid] = u;
+  }
+  __syncthreads();
+  double dots[RB];
+#pragma unroll
+  for (int q = 0; q < RB; ++q) dots[q] = 0.0;
+  for (int j = tid; j < ncol; j += WG) {
+    const double xj = xv ? xv[j] : 0.0;
+    double colacc = 0.0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const int r = r0 + q;
+      if (r < rows) {
+        const double a = (double)M[(int64_t)r * ncol + j];
+        dots[q] += a * xj;
+        if (uv) colacc += a * su[q];
+      }
+    }
+    if (uv) part[(int64_t)blockIdx.x * ncol + j] = colacc;
+  }
+  if (xv) {
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+      const double s = wave_sum(dots[q]);
+      if (lane == 0) red[q][wave] = s;
+    }
+    __syncthreads();
+    if (tid < RB && r0 + tid < rows)
+      rowout[r0 + tid] = red[tid][0] + red[tid][1] + red[tid][2] + red[tid][3];
+  }
+}
+
+// out[j] = base[j] + sum_b part[b][j]  for j < ncol.
+__global__ __launch_bounds__(WG) void k_colsum(const double* __restrict__ part, int nblk, int ncol,
+                                               const double* __restrict__ base,
+                                               double* __restrict__ out) {
+  const int j = blockIdx.x * WG + threadIdx.x;
+  if (j >= ncol) return;
+  double s = base ? base[j] : 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * ncol + j];
+  out[j] = s;
+}
+
+// K_reg x (or a residual):  kx = [Poffx + d_x*x_x + Atx ; Ax + d_y*x_y].
+// If rhs: out = rhs - kx (+ corr*x if corr), and *norm gets max |out|.
+__global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* __restrict__ x,
+                                                   const double* __restrict__ diag,
+                                                   const double* __restrict__ poffx,
+                                                   const double* __restrict__ part, int nblk,
+                                                   const double* __restrict__ ax,
+                                                   const double* __restrict__ rhs,
+                                                   const double* __restrict__ corr,
+                                                   double* __restrict__ out,
+                                                   unsigned long long* norm) {
+  __shared__ double red[4];
+  const int idx = blockIdx.x * WG + threadIdx.x;
+  double v = 0.0;
+  bool live = idx < n + m;
+  if (live) {
+    double kx;
+    if (idx < n) {
+      double s = poffx[idx] + diag[idx] * x[idx];
+      for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * n + idx];
+      kx = s;
+    } else {
+      kx = ax[idx - n] + diag[idx] * x[idx];
+    }
+    if (rhs) {
+      v = rhs[idx] - kx;
+      if (corr) v += corr[idx] * x[idx];
+    } else {
+      v = kx;
+    }
+    out[idx] = v;
+  }
+  if (norm) {
+    double a = live ? fabs(v) : 0.0;
+    if (live && v != v) a = INFINITY;  // fmax drops NaN; report it as inf
+    a = block_max(a, red);
+    if (threadIdx.x == 0) atomic_max_nonneg(norm, a);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Triangular sweeps G x = g, G = L L', across many workgroups.
+//
+// Workgroup "ticket" t (taken with an atomic counter, so every lower ticket
+// belongs to a workgroup that is already running) owns row block I = t
+// (forward) or I = nt-1-t (backward).  It streams the off-diagonal blocks
+// it needs, waiting per block on the flag of the block that produced the
+// matching piece of the solution, then applies the stored inverse of its
+// diagonal block and publishes its piece.  Flags hold the launch epoch so
+// no reset is needed between solves: sync[0], sync[1] are the forward and
+// backward ticket counters, sync[2..2+nt) and sync[2+nt..2+2nt) the block
+// flags.  A wait is bounded: past the limit the block writes NaN (which the
+// callers' NaN checks report) and still publishes, so every workgroup ends.
+// ---------------------------------------------------------------------------
+constexpr unsigned kSpinLimit = 1u << 21;
+
+__device__ inline unsigned take_ticket(unsigned* ctr, unsigned base, unsigned* sh) {
+  if (threadIdx.x == 0) *sh = atomicAdd(ctr, 1u) - base;
+  __syncthreads();
+  return *sh;
+}
+
+// Lane 0 of the calling wave spins until *flag == want; returns false on
+// timeout.  The acquire fence makes the producer's stores visible.
+__device__ inline bool wave_wait(const unsigned* flag, unsigned want) {
+  int ok = 1;
+  if ((threadIdx.x & 63) == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+      if (++spins > kSpinLimit) {
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  ok = __shfl(ok, 0, 64);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok != 0;
+}
+
+__device__ inline void publish(unsigned* flag, unsigned val) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// out[r] = sum_c M[c*64 + r] w[c] for the 64 x 64 block M (r = lane); the
+// four waves split c and reduce through red[4][64].
+template <typename T>
+__device__ inline double block_gemv(const T* __restrict__ M, const double* w, double* red) {
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  double acc = 0.0;
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) {
+    const int c = q * 16 + cc;
+    acc += (double)M[c * NB + r] * w[c];
+  }
+  red[q * NB + r] = acc;
+  __syncthreads();
+  const double v = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
+  __syncthreads();
+  return v;
+}
+
+// Forward sweep L v = g.  Thread (r = lane, q = wave) accumulates
+// sum_{J<I} sum_{c in q's 16 columns} L[I*64+r][J*64+c] v[J*64+c].
+template <typename T>
+__global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_t ld, int n,
+                                                 const T* __restrict__ LiB,
+                                                 const double* __restrict__ g, T* v,
+                                                 unsigned* sync, unsigned base, unsigned epoch) {
+  __shared__ unsigned sh_ticket;
+  __shared__ double red[4 * NB];
+  __shared__ double w[NB];
+  __shared__ T sv[4][NB];
+  const int nt = gridDim.x;
+  const int I = (int)take_ticket(&sync[0], base, &sh_ticket);
+  unsigned* flags = sync + 2;
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i0 = I * NB, rows = min(NB, n - i0);
+  const bool live = r < rows;
+  bool ok = true;
+  double acc = 0.0;
+  T cur[16], nxt[16];
+  auto load = [&](int J, T* dst) {
+    const T* col = L + (int64_t)(J * NB + q * 16) * ld + i0 + r;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) dst[cc] = live ? col[(int64_t)cc * ld] : T(0);
+  };
+  if (I > 0) load(0, cur);
+  for (int J = 0; J < I; ++J) {
+    if (J + 1 < I) load(J + 1, nxt);
+    ok &= wave_wait(&flags[J], epoch);
+    sv[q][r] = v[J * NB + r];  // J < I: a full block
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) acc += (double)cur[cc] * (double)sv[q][q * 16 + cc];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+  }
+  red[q * NB + r] = acc;
+  __syncthreads();
+  if (q == 0) {
+    const double s = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
+    w[r] = live ? g[i0 + r] - s : 0.0;
+  }
+  __syncthreads();
+  const double vi = block_gemv(LiB + (int64_t)I * 2 * NB * NB, w, red);
+  if (q == 0 && live) v[i0 + r] = ok ? (T)vi : T(NAN);
+  publish(&flags[I], epoch);
+  (void)nt;
+}
+
+// Backward sweep L' x = v.  Thread (rr = lane, q = wave) accumulates, for
+// the 16 columns c of q, sum_{J>I} L[J*64+rr][I*64+c] x[J*64+rr].
+template <typename T>
+__global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_t ld, int n,
+                                                 const T* __restrict__ LiB,
+                                                 const T* __restrict__ v, double* x,
+                                                 unsigned* sync, unsigned base, unsigned epoch) {
+  __shared__ unsigned sh_ticket;
+  __shared__ double red[4 * NB];
+  __shared__ double part[NB][NB + 1];
+  __shared__ double w[NB];
+  const int nt = gridDim.x;
+  const int I = nt - 1 - (int)take_ticket(&sync[1], base, &sh_ticket);
+  unsigned* flags = sync + 2 + nt;
+  const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int i0 = I * NB, rows = min(NB, n - i0);
+  bool ok = true;
+  double acc[16];
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
+  T cur[16], nxt[16];
+  auto load = [&](int J, T* dst) {
+    const int row = J * NB + rr;
+    const bool in = row < n;
+    const T* base_p = L + (int64_t)(i0 + q * 16) * ld + row;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc)
+      dst[cc] = (in && q * 16 + cc < rows) ? base_p[(int64_t)cc * ld] : T(0);
+  };
+  if (I + 1 < nt) load(nt - 1, cur);
+  for (int J = nt - 1; J > I; --J) {
+    if (J - 1 > I) load(J - 1, nxt);
+    ok &= wave_wait(&flags[J], epoch);
+    const int row = J * NB + rr;
+    const double xj = row < n ? x[row] : 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) acc[cc] += (double)cur[cc] * xj;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+  }
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) part[q * 16 + cc][rr] = acc[cc];
+  __syncthreads();
+  {
+    // column c = threadIdx.x / 4 summed by 4 threads over 16 lanes each
+    const int c = threadIdx.x >> 2, k = threadIdx.x & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) s += part[c][k * 16 + t];
+    red[k * NB + c] = s;
+  }
+  __syncthreads();
+  if (q == 0) {
+    const double s = (red[rr] + red[NB + rr]) + (red[2 * NB + rr] + red[3 * NB + rr]);
+    w[rr] = rr < rows ? (double)v[i0 + rr] - s : 0.0;
+  }
+  __syncthreads();
+  const double xi = block_gemv(LiB + (int64_t)I * 2 * NB * NB + NB * NB, w, red);
+  if (q == 0 && rr < rows) x[i0 + rr] = ok ? xi : (double)NAN;
+  publish(&flags[I], epoch);
+}
+
+// y = W (Ax - r_y) with W = -1/d_y; writes [x; y] into out (or adds it).
+__global__ __launch_bounds__(WG) void k_solve_finish(int n, int m, const double* __restrict__ x,
+                                                     const double* __restrict__ ax,
+                                                     const double* __restrict__ r,
+                                                     const double* __restrict__ diag,
+                                                     double* __restrict__ out, int accumulate) {
+  const int idx = blockIdx.x * WG + threadIdx.x;
+  if (idx >= n + m) return;
+  double v;
+  if (idx < n) v = x[idx];
+  else v = (-1.0 / diag[idx]) * (ax[idx - n] - r[idx]);
+  out[idx] = accumulate ? out[idx] + v : v;
+}
+
+__global__ void k_axpy(int64_t count, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
+  if (i < count) y[i] += alpha * x[i];
+}
+
+__global__ void k_nan_count(int64_t count, const double* __restrict__ x, int* flag) {
+  const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
+  if (i < count && x[i] != x[i]) atomicOr(flag, 1);
+}
+
+// krhs = [r_x; -r_y] (direct.py:245-248) and ||krhs||_inf into *norm (a NaN
+// entry is reported as inf, since fmax drops NaN).
+__global__ __launch_bounds__(WG) void k_negate_y(int64_t count, int64_t n,
+                                                 const double* __restrict__ r,
+                                                 double* __restrict__ krhs,
+                                                 unsigned long long* norm) {
+  __shared__ double red[4];
+  const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
+  double a = 0.0;
+  if (i < count) {
+    const double v = i < n ? r[i] : -r[i];
+    krhs[i] = v;
+    a = v != v ? INFINITY : fabs(v);
+  }
+  a = block_max(a, red);
+  if (threadIdx.x == 0) atomic_max_nonneg(norm, a);
+}
+
+// dst (rows x cols, row-major) = src', where src is cols x rows row-major
+// (i.e. the column-major rows x cols matrix).  64 x 64 tiles through LDS;
+// blockIdx.x walks row tiles, blockIdx.y column tiles.
+__global__ __launch_bounds__(WG) void k_transpose(const double* __restrict__ src,
+                                                  double* __restrict__ dst, int64_t rows,
+                                                  int64_t cols) {
+  __shared__ double t[NB][NB + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * NB, c0 = (int64_t)blockIdx.y * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += WG) {
+    const int a = e >> 6, b = e & (NB - 1);  // src row c0 + a, src column r0 + b
+    if (c0 + a < cols && r0 + b < rows) t[a][b] = src[(c0 + a) * rows + r0 + b];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < NB * NB; e += WG) {
+    const int a = e >> 6, b = e & (NB - 1);  // dst row r0 + a, dst column c0 + b
+    if (r0 + a < rows && c0 + b < cols) dst[(r0 + a) * cols + c0 + b] = t[b][a];
+  }
+}
+
+// Zeroes the diagonal of the row-major n x n matrix M.
+__global__ void k_zero_diag(double* M, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
+  if (i < n) M[i * n + i] = 0.0;
+}
+
+inline unsigned blocks_for(int64_t count) { return (unsigned)((count + WG - 1) / WG); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+struct qtqp_kkt {
+  int64_t n = 0, m = 0;
+  int precision = QTQP_PREC_FP64;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // problem data
+  double *A64 = nullptr, *Poff64 = nullptr;
+  float *A32 = nullptr, *Poff32 = nullptr;
+  double *diag = nullptr, *corr = nullptr;
+  // factor storage (column-major, ld = n)
+  double* G64 = nullptr;
+  float* G32 = nullptr;
+  int* info = nullptr;
+  // vectors
+  double *vin = nullptr, *vout = nullptr, *vsol = nullptr, *vres = nullptr;
+  double *rin = nullptr, *racc = nullptr, *rres = nullptr, *rtmp = nullptr;
+  double *tx = nullptr, *ty = nullptr, *tg = nullptr, *tpx = nullptr, *part = nullptr;
+  double* tv = nullptr;    // forward-sweep result of the triangular solves
+  double* linv = nullptr;  // inverses of the 64x64 diagonal blocks of the factor
+  unsigned* sync = nullptr;  // tickets and block flags of the triangular sweeps
+  unsigned trsv_epoch = 0;   // completed sweep launches
+  unsigned long long* dnorm = nullptr;
+  int* dflag = nullptr;
+  // state
+  bool have_diag = false, have_factor = false, force64 = false;
+  int fprec = 64;
+  qtqp_kkt_info info_acc{};
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+  g_qtqp_last_error = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(QTQP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+int nblk_rows(int64_t rows) { return (int)((rows + RB - 1) / RB); }
+
+// t = M x (rows) and/or part = column partials of M' u.
+template <typename T>
+int rowscols(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* xv,
+             double* rowout, const double* uv, const double* wdiag, double* part) {
+  if (rows == 0) return QTQP_OK;
+  hipLaunchKernelGGL(k_rowscols<T>, dim3(nblk_rows(rows)), dim3(WG), 0, h->stream, M, (int)rows,
+                     (int)ncol, xv, rowout, uv, wdiag, part);
+  HIPC(hipGetLastError());
+  return QTQP_OK;
+}
+
+// out = K_reg x, or out = rhs - K_reg x + corr*x with ||out||_inf into *norm_host.
+//  With rhs_norm_host, also reads back dnorm[1] (set by k_negate_y).
+int kmatvec(qtqp_kkt* h, const double* x, const double* rhs, const double* corr, double* out,
+            double* norm_host, double* rhs_norm_host = nullptr) {
+  const int64_t n = h->n, m = h->m, N = n + m;
+  int rc;
+  if ((rc = rowscols<double>(h, h->A64, m, n, x, h->ty, x + n, nullptr, h->part))) return rc;
+  if ((rc = rowscols<double>(h, h->Poff64, n, n, x, h->tpx, nullptr, nullptr, nullptr))) return rc;
+  if (norm_host) HIPC(hipMemsetAsync(h->dnorm, 0, sizeof(unsigned long long), h->stream));
+  hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
+                     h->diag, h->tpx, h->part, nblk_rows(m), h->ty, rhs, corr, out,
+                     norm_host ? h->dnorm : nullptr);
+  HIPC(hipGetLastError());
+  if (norm_host) {
+    unsigned long long bits[2] = {0, 0};
+    const int cnt = rhs_norm_host ? 2 : 1;
+    HIPC(hipMemcpyAsync(bits, h->dnorm, sizeof(bits[0]) * cnt, hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    std::memcpy(norm_host, &bits[0], sizeof(double));
+    if (rhs_norm_host) std::memcpy(rhs_norm_host, &bits[1], sizeof(double));
+  }
+  return QTQP_OK;
+}
+
+// Factor G in precision T: Gram SYRK, jitter, blocked LDL'. Returns the
+// breakdown column (1-based) or 0 in *brk.
+template <typename T>
+int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
+  const int n = (int)h->n, m = (int)h->m;
+  const int nt = (n + NB - 1) / NB;
+  HIPC(hipMemsetAsync(h->info, 0, sizeof(int), h->stream));
+  hipLaunchKernelGGL(k_gram_syrk<T>, dim3(nt * (nt + 1) / 2), dim3(WG), 0, h->stream, A, Poff,
+                     h->diag, n, m, G, (int64_t)n);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n);
+  HIPC(hipGetLastError());
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int below = std::max(0, n - k0 - NB);
+    hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, k0,
+                       h->info, reinterpret_cast<T*>(h->linv));
+    HIPC(hipGetLastError());
+    if (below > 0) {
+      hipLaunchKernelGGL(k_trsm_panel<T>, dim3((below + NB - 1) / NB), dim3(WG), 0, h->stream, G,
+                         (int64_t)n, n, k0, reinterpret_cast<const T*>(h->linv));
+      HIPC(hipGetLastError());
+      const int tt = (below + NB - 1) / NB;
+      hipLaunchKernelGGL(k_syrk_trail<T>, dim3(tt * (tt + 1) / 2), dim3(WG), 0, h->stream, G,
+                         (int64_t)n, n, k0);
+      HIPC(hipGetLastError());
+    }
+  }
+  int hinfo = 0;
+  HIPC(hipMemcpyAsync(&hinfo, h->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  *brk = hinfo;
+  return QTQP_OK;
+}
+
+int factor64(qtqp_kkt* h) {
+  int brk = 0, rc;
+  if ((rc = factor_prec<double>(h, h->A64, h->Poff64, h->G64, &brk))) return rc;
+  if (brk != 0) {
+    h->have_factor = false;
+    char buf[128];
+    snprintf(buf, sizeof(buf), "LDL' breakdown of the FP64 Gram block at column %d", brk);
+    return fail(QTQP_ERR_FACTOR, buf);
+  }
+  h->fprec = 64;
+  h->have_factor = true;
+  return QTQP_OK;
+}
+
+// d = G^{-1}-based K_reg solve of r with the current factor: [x; y] into out.
+template <typename T>
+int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out, int accumulate) {
+  const int64_t n = h->n, m = h->m, N = n + m;
+  int rc;
+  // g = r_x + A'(W r_y)
+  if ((rc = rowscols<T>(h, A, m, n, nullptr, nullptr, r + n, h->diag + n, h->part))) return rc;
+  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(n)), dim3(WG), 0, h->stream, h->part, nblk_rows(m),
+                     (int)n, r, h->tg);
+  HIPC(hipGetLastError());
+  const int nt = (int)((n + NB - 1) / NB);
+  const unsigned base = h->trsv_epoch * (unsigned)nt;
+  const unsigned epoch = h->trsv_epoch + 1u;
+  const T* LiB = reinterpret_cast<const T*>(h->linv);
+  T* tv = reinterpret_cast<T*>(h->tv);
+  hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
+                     h->tg, tv, h->sync, base, epoch);
+  HIPC(hipGetLastError());
+  hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
+                     tv, h->tx, h->sync, base, epoch);
+  HIPC(hipGetLastError());
+  h->trsv_epoch++;
+  if ((rc = rowscols<T>(h, A, m, n, h->tx, h->ty, nullptr, nullptr, nullptr))) return rc;
+  hipLaunchKernelGGL(k_solve_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m,
+                     h->tx, h->ty, r, h->diag, out, accumulate);
+  HIPC(hipGetLastError());
+  h->info_acc.backend_solves++;
+  return QTQP_OK;
+}
+
+int solve_current(qtqp_kkt* h, const double* r, double* out, int accumulate) {
+  if (h->fprec == 64) return solve_with<double>(h, h->A64, h->G64, r, out, accumulate);
+  return solve_with<float>(h, h->A32, h->G32, r, out, accumulate);
+}
+
+// Backend solve: out = K_reg^{-1} r to FP64 accuracy (device buffers).
+int backend_solve(qtqp_kkt* h, const double* r, double* out) {
+  const int64_t N = h->n + h->m;
+  if (h->fprec == 64) return solve_current(h, r, out, 0);
+  // FP32 factor: refine in FP64 against the exact K_reg.
+  // An FP32 factor of a Gram block with condition number k contracts the
+  // residual by about n * 6e-8 * k per step; past kPoorContraction it would
+  // need too many steps, and an FP64 refactor is cheaper.
+  constexpr int kMaxInner = 40;
+  constexpr double kTarget = 1e-15;  // relative residual reached by an FP64 factor
+  constexpr double kPoorContraction = 0.25;
+  constexpr double kFloor = 1e-13;
+  int rc;
+  double r0 = 0.0, prev = 0.0, cur = 0.0;
+  // ||r|| for the relative target
+  {
+    HIPC(hipMemsetAsync(h->dnorm, 0, sizeof(unsigned long long), h->stream));
+    // reuse k_kmv_finish-free path: a residual with x = 0 is r itself
+    HIPC(hipMemsetAsync(h->racc, 0, sizeof(double) * N, h->stream));
+    if ((rc = kmatvec(h, h->racc, r, nullptr, h->rres, &r0))) return rc;
+  }
+  if (r0 == 0.0) {
+    HIPC(hipMemsetAsync(out, 0, sizeof(double) * N, h->stream));
+    return QTQP_OK;
+  }
+  prev = r0;
+  for (int step = 0; step < kMaxInner; ++step) {
+    if ((rc = solve_current(h, h->rres, h->racc, 1))) return rc;
+    if (h->fprec == 32) h->info_acc.inner_steps++;
+    if ((rc = kmatvec(h, h->racc, r, nullptr, h->rres, &cur))) return rc;
+    if (!(cur == cur)) break;
+    if (cur <= kTarget * r0) break;
+    if (cur > kPoorContraction * prev) {
+      // Stalled.  Below kFloor the FP64 residual itself is at its rounding
+      // floor (an FP64 factor would stop there too); above it the FP32
+      // factor is too inaccurate at this conditioning: refactor in FP64.
+      if (cur <= kFloor * r0 || h->fprec == 64) break;
+      h->info_acc.fp64_fallbacks++;
+      h->force64 = true;
+      if ((rc = factor64(h))) return rc;
+      continue;  // keep the accumulated correction; refine with the FP64 factor
+    }
+    prev = cur;
+  }
+  HIPC(hipMemcpyAsync(out, h->racc, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
+  return QTQP_OK;
+}
+
+template <typename T>
+T* dalloc(int64_t count, hipError_t* err) {
+  void* p = nullptr;
+  *err = hipMalloc(&p, sizeof(T) * std::max<int64_t>(count, 1));
+  return static_cast<T*>(p);
+}
+
+void release(qtqp_kkt* h) {
+  void* bufs[] = {h->A64, h->Poff64, h->A32, h->Poff32, h->diag, h->corr, h->G64, h->G32,
+                  h->info, h->vin, h->vout, h->vsol, h->vres, h->rin, h->racc, h->rres,
+                  h->rtmp, h->tx, h->ty, h->tg, h->tpx, h->part, h->dnorm, h->dflag,
+                  h->tv, h->linv, h->sync};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+double elapsed_ms(qtqp_kkt* h) {
+  float ms = 0.f;
+  if (hipEventRecord(h->ev1, h->stream) != hipSuccess) return 0.0;
+  if (hipEventSynchronize(h->ev1) != hipSuccess) return 0.0;
+  if (hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) return 0.0;
+  return ms;
+}
+
+// Allocates a handle and uploads A (m x n row-major) and P (n x n row-major,
+// its diagonal dropped; NULL for P = 0).  With cast, the FP32 copies of a
+// mixed-precision handle are made now; otherwise the caller makes them
+// (after rescaling the FP64 data) with cast_fp32.
+int cast_fp32(qtqp_kkt* h) {
+  if (h->precision != QTQP_PREC_MIXED) return QTQP_OK;
+  hipLaunchKernelGGL(k_cast<float>, dim3(1024), dim3(WG), 0, h->stream, h->A64, h->A32,
+                     h->m * h->n);
+  hipLaunchKernelGGL(k_cast<float>, dim3(256), dim3(WG), 0, h->stream, h->Poff64, h->Poff32,
+                     h->n * h->n);
+  HIPC(hipGetLastError());
+  return QTQP_OK;
+}
+
+//  With col_major, a and p are given column-major (transposed on the device).
+int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_t precision,
+                  bool cast, qtqp_kkt** out, bool col_major = false) {
+  if (!out) return fail(QTQP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n < 1 || m < 1) return fail(QTQP_ERR_ARG, "n and m must be >= 1");
+  if (n > 46340 || n * m > (int64_t)1 << 40) return fail(QTQP_ERR_ARG, "problem too large");
+  if (!a) return fail(QTQP_ERR_ARG, "a is NULL");
+  if (precision != QTQP_PREC_FP64 && precision != QTQP_PREC_MIXED)
+    return fail(QTQP_ERR_ARG, "unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(QTQP_ERR_HIP, "no HIP device available");
+  auto* h = new qtqp_kkt();
+  h->n = n;
+  h->m = m;
+  h->precision = precision;
+  h->info_acc.precision = precision;
+  const int64_t N = n + m;
+  hipError_t e = hipSuccess;
+  auto bad = [&](const char* what) {
+    std::string msg = std::string("allocation/copy failed: ") + what + ": " + hipGetErrorString(e);
+    release(h);
+    delete h;
+    return fail(QTQP_ERR_HIP, msg);
+  };
+  if ((e = hipStreamCreate(&h->stream)) != hipSuccess) return bad("stream");
+  if ((e = hipEventCreate(&h->ev0)) != hipSuccess) return bad("event");
+  if ((e = hipEventCreate(&h->ev1)) != hipSuccess) return bad("event");
+#define ALLOC(field, T, count) \
+  if ((h->field = dalloc<T>((count), &e)), e != hipSuccess) return bad(#field)
+  ALLOC(A64, double, m * n);
+  ALLOC(Poff64, double, n * n);
+  ALLOC(diag, double, N);
+  ALLOC(corr, double, N);
+  ALLOC(G64, double, n * n);
+  ALLOC(info, int, 1);
+  ALLOC(vin, double, N);
+  ALLOC(vout, double, N);
+  ALLOC(vsol, double, N);
+  ALLOC(vres, double, N);
+  ALLOC(rin, double, N);
+  ALLOC(racc, double, N);
+  ALLOC(rres, double, N);
+  ALLOC(rtmp, double, N);
+  ALLOC(tx, double, n);
+  ALLOC(ty, double, std::max(m, n));
+  ALLOC(tg, double, n);
+  ALLOC(tpx, double, n);
+  ALLOC(tv, double, n);
+  const int64_t nt = (n + NB - 1) / NB;
+  ALLOC(linv, double, nt * 2 * NB * NB);
+  ALLOC(sync, unsigned, 2 + 2 * nt);
+  ALLOC(part, double, (int64_t)nblk_rows(std::max(m, n)) * n);
+  ALLOC(dnorm, unsigned long long, 2);
+  ALLOC(dflag, int, 1);
+  if (precision == QTQP_PREC_MIXED) {
+    ALLOC(A32, float, m * n);
+    ALLOC(Poff32, float, n * n);
+    ALLOC(G32, float, n * n);
+  }
+#undef ALLOC
+  // Column-major inputs are staged in G64 (P) and a temporary buffer (A).
+  double* stage = nullptr;
+  if (col_major) {
+    if ((e = hipMalloc(&stage, sizeof(double) * m * n)) != hipSuccess) return bad("staging");
+    if ((e = hipMemcpyAsync(stage, a, sizeof(double) * m * n, hipMemcpyHostToDevice, h->stream)) ==
+        hipSuccess) {
+      hipLaunchKernelGGL(k_transpose, dim3((m + NB - 1) / NB, (n + NB - 1) / NB), dim3(WG), 0,
+                         h->stream, stage, h->A64, m, n);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return bad("A");
+  } else if ((e = hipMemcpyAsync(h->A64, a, sizeof(double) * m * n, hipMemcpyHostToDevice,
+                                 h->stream)) != hipSuccess) {
+    return bad("A");
+  }
+  if (p) {
+    double* pdst = col_major ? h->G64 : h->Poff64;
+    if ((e = hipMemcpyAsync(pdst, p, sizeof(double) * n * n, hipMemcpyHostToDevice, h->stream)) !=
+        hipSuccess)
+      return bad("P");
+    if (col_major) {
+      hipLaunchKernelGGL(k_transpose, dim3((n + NB - 1) / NB, (n + NB - 1) / NB), dim3(WG), 0,
+                         h->stream, h->G64, h->Poff64, n, n);
+      if ((e = hipGetLastError()) != hipSuccess) return bad("P transpose");
+    }
+    hipLaunchKernelGGL(k_zero_diag, dim3(blocks_for(n)), dim3(WG), 0, h->stream, h->Poff64, n);
+    if ((e = hipGetLastError()) != hipSuccess) return bad("P diagonal");
+  } else if ((e = hipMemsetAsync(h->Poff64, 0, sizeof(double) * n * n, h->stream)) != hipSuccess) {
+    return bad("P");
+  }
+  if ((e = hipMemsetAsync(h->corr, 0, sizeof(double) * N, h->stream)) != hipSuccess) return bad("corr");
+  if ((e = hipMemsetAsync(h->sync, 0, sizeof(unsigned) * (2 + 2 * nt), h->stream)) != hipSuccess)
+    return bad("sync");
+  if (cast && cast_fp32(h) != QTQP_OK) {
+    e = hipGetLastError();
+    return bad("cast");
+  }
+  if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bad("sync");
+  *out = h;
+  return QTQP_OK;
+}
+
+// LinearSolver.factorize (solvers_dense.py:99-116) on the diagonal in
+// h->diag: FP32 factor for a mixed handle (FP64 refactor on breakdown, and
+// FP64 from then on), FP64 otherwise.
+int engine_factorize(qtqp_kkt* h) {
+  HIPC(hipEventRecord(h->ev0, h->stream));
+  int rc = QTQP_OK;
+  h->info_acc.factorizations++;
+  if (h->precision == QTQP_PREC_MIXED && !h->force64) {
+    int brk = 0;
+    rc = factor_prec<float>(h, h->A32, h->Poff32, h->G32, &brk);
+    if (rc == QTQP_OK && brk == 0) {
+      h->fprec = 32;
+      h->have_factor = true;
+    } else if (rc == QTQP_OK) {
+      h->info_acc.fp64_fallbacks++;
+      h->force64 = true;
+      rc = factor64(h);
+    }
+  } else {
+    rc = factor64(h);
+  }
+  h->info_acc.factor_ms += elapsed_ms(h);
+  return rc;
+}
+
+// DirectKktSolver.solve (direct.py:218-307) on device buffers: refines sol
+// (warm start in, solution out) against K_true = K_reg - diag(corr) for the
+// right-hand side [rhs_x; -rhs_y] until ||res||_inf < atol + rtol ||rhs||_inf,
+// a stall (the residual stops decreasing), or max_steps backend solves.
+// Returns QTQP_ERR_NAN when sol has NaNs (direct.py:293-294).
+int ir_refine(qtqp_kkt* h, const double* rhs, double* sol, int max_steps, double atol,
+              double rtol, qtqp_ir_stats* st) {
+  const int64_t n = h->n, N = h->n + h->m;
+  HIPC(hipMemsetAsync(h->dnorm + 1, 0, sizeof(unsigned long long), h->stream));
+  hipLaunchKernelGGL(k_negate_y, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, n, rhs, h->vin,
+                     h->dnorm + 1);
+  HIPC(hipGetLastError());
+  double res = 0.0, old = 0.0, rhs_norm = 0.0;
+  int rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res, &rhs_norm);
+  if (rc) return rc;
+  const double tol = atol + rtol * rhs_norm;
+  int status = 1, solves = 0;
+  for (solves = 1; solves <= max_steps; ++solves) {
+    old = res;
+    if ((rc = backend_solve(h, h->vres, h->vout))) return rc;
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, 1.0, h->vout, sol);
+    HIPC(hipGetLastError());
+    if ((rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res))) return rc;
+    if (res < tol) {
+      status = 0;
+      break;
+    }
+    if (res >= old) {
+      status = 2;
+      break;
+    }
+  }
+  if (solves > max_steps) solves = max_steps;
+  HIPC(hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
+  hipLaunchKernelGGL(k_nan_count, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, sol, h->dflag);
+  HIPC(hipGetLastError());
+  int nanflag = 0;
+  HIPC(hipMemcpyAsync(&nanflag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  if (st) {
+    st->solves = solves;
+    st->status = status;
+    st->final_residual_norm = res;
+    st->rhs_norm = rhs_norm;
+    st->tolerance = tol;
+  }
+  if (nanflag) return fail(QTQP_ERR_NAN, "Linear solver returned NaNs.");
+  return QTQP_OK;
+}
+
+}  // namespace

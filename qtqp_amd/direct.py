@@ -83,6 +83,9 @@ class _HipDenseBase(LinearSolver):
   """
 
   _precision = _native.PREC_FP64
+  # QTQP.solve runs the whole iteration loop on the device (DeviceIpm) for
+  # backends that set this; the protocol methods below stay available.
+  device_driver = True
 
   def __init__(self):
     self._lib = _native.kkt_lib()
@@ -142,6 +145,10 @@ class _HipDenseBase(LinearSolver):
   def free(self) -> None:
     self._release()
 
+  def device_ipm(self, a, p, has_p, b, c, z, equilibrate) -> DeviceIpm:
+    """The device-resident IPM driver on this backend's engine and precision."""
+    return DeviceIpm(self._lib, self._precision, a, p, has_p, b, c, z, equilibrate)
+
   # -- device-resident refinement (used by DirectKktSolver when present) ---
   def set_correction(self, corr: np.ndarray) -> None:
     c = _native.as_f64(corr)
@@ -174,6 +181,67 @@ class _HipDenseBase(LinearSolver):
   def __del__(self):
     try:
       self._release()
+    except Exception:  # pylint: disable=broad-except
+      pass
+
+
+class DeviceIpm:
+  """One QP held on the device for the device-resident IPM driver
+  (include/qtqp_ipm.h): ``iterate`` runs one whole IPM iteration on the GPU
+  and returns its record; the iterate never leaves HBM until ``iterates``.
+  """
+
+  def __init__(self, lib, precision: int, a: np.ndarray, p: np.ndarray | None, has_p: bool,
+               b: np.ndarray, c: np.ndarray, z: int, equilibrate: bool):
+    self._lib = lib
+    self._h = None
+    self.m, self.n = a.shape
+    # Column-major inputs (what scipy's CSC toarray() returns) go up as they
+    # are and are transposed on the device.
+    col_major = (a.dtype == np.float64 and a.flags.f_contiguous and not a.flags.c_contiguous and
+                 (p is None or (p.dtype == np.float64 and p.flags.f_contiguous)))
+    if col_major:
+      a, p = a.T, (None if p is None else p.T)
+    a = _native.as_f64(a)
+    pp = None if p is None else _native.as_f64(p)
+    bb, cc = _native.as_f64(b), _native.as_f64(c)
+    h = _native.ctypes.c_void_p()
+    rc = lib.qtqp_ipm_create(self.n, self.m, int(z), _native.dptr(a),
+                             None if pp is None else _native.dptr(pp), int(col_major),
+                             int(bool(has_p)),
+                             _native.dptr(bb), _native.dptr(cc), precision, int(bool(equilibrate)),
+                             _native.ctypes.byref(h))
+    _native.check(lib, rc)
+    self._h = h
+    self._rec = _native.IpmRecord()
+
+  def iterate(self, opts: "_native.IpmOpts") -> "_native.IpmRecord":
+    _native.check(self._lib, self._lib.qtqp_ipm_iterate(self._h, _native.ctypes.byref(opts),
+                                                        _native.ctypes.byref(self._rec)))
+    return self._rec
+
+  def iterates(self):
+    """Unequilibrated (x, y, s, tau) of the current iterate."""
+    x, y, s = np.empty(self.n), np.empty(self.m), np.empty(self.m)
+    tau = _native.ctypes.c_double()
+    _native.check(self._lib, self._lib.qtqp_ipm_get_iterates(
+        self._h, _native.dptr(x), _native.dptr(y), _native.dptr(s),
+        _native.ctypes.cast(_native.ctypes.byref(tau), _native._c_double_p)))
+    return x, y, s, float(tau.value)
+
+  def info(self) -> dict[str, Any]:
+    inf = _native.KktInfo()
+    _native.check(self._lib, self._lib.qtqp_ipm_get_info(self._h, _native.ctypes.byref(inf)))
+    return {k: getattr(inf, k) for k, _ in inf._fields_}
+
+  def free(self) -> None:
+    if getattr(self, "_h", None) is not None:
+      self._lib.qtqp_ipm_free(self._h)
+      self._h = None
+
+  def __del__(self):
+    try:
+      self.free()
     except Exception:  # pylint: disable=broad-except
       pass
 

@@ -3,14 +3,16 @@
 import ctypes
 import pathlib
 import re
+import shutil
+import subprocess
 
 import pytest
 
 from qtqp_amd import _native
 
 INCLUDE = pathlib.Path(__file__).resolve().parents[1] / "include"
-HEADER_TO_LIB = {"qtqp_kkt.h": "libqtqp_kkt.so", "qtqp_batch.h": "libqtqp_batch.so",
-                 "qtqp_mlp.h": "libqtqp_mlp.so"}
+HEADER_TO_LIB = {"qtqp_kkt.h": "libqtqp_kkt.so", "qtqp_ipm.h": "libqtqp_kkt.so",
+                 "qtqp_batch.h": "libqtqp_batch.so", "qtqp_mlp.h": "libqtqp_mlp.so"}
 
 
 def declared(header: pathlib.Path):
@@ -31,6 +33,31 @@ def test_library_exports_every_declared_symbol(header):
 
 def test_kkt_bindings_cover_header():
   assert set(declared(INCLUDE / "qtqp_kkt.h")) == set(_native.KKT_SYMBOLS)
+  assert set(declared(INCLUDE / "qtqp_ipm.h")) == set(_native.IPM_SYMBOLS)
+
+
+def _c_layout(tmp_path, header, struct, fields):
+  """sizeof and field offsets of a C struct, from gcc on the header itself."""
+  prog = tmp_path / "layout.c"
+  body = "".join(f'  printf("%zu\\n", offsetof({struct}, {f}));\n' for f in fields)
+  prog.write_text(f'#include <stddef.h>\n#include <stdio.h>\n#include "{header}"\n'
+                  f'int main(void) {{\n  printf("%zu\\n", sizeof({struct}));\n{body}  return 0;\n}}\n')
+  exe = tmp_path / "layout"
+  subprocess.run(["gcc", f"-I{INCLUDE}", str(prog), "-o", str(exe)], check=True)
+  out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+  return [int(v) for v in out]
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+@pytest.mark.parametrize("cls,struct", [(_native.IpmOpts, "qtqp_ipm_opts"),
+                                        (_native.IpmRecord, "qtqp_ipm_record"),
+                                        (_native.IrStats, "qtqp_ir_stats"),
+                                        (_native.KktInfo, "qtqp_kkt_info")])
+def test_ctypes_structs_match_c_layout(tmp_path, cls, struct):
+  names = [f for f, _ in cls._fields_]
+  c = _c_layout(tmp_path, "qtqp_ipm.h", struct, names)
+  assert c[0] == ctypes.sizeof(cls)
+  assert c[1:] == [getattr(cls, f).offset for f in names]
 
 
 def test_device_count_without_gpu_is_safe():

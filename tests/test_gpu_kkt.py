@@ -21,6 +21,14 @@ ENUMS = [qtqp_amd.LinearSolver.HIP_DENSE, qtqp_amd.LinearSolver.HIP_DENSE_FP64]
 TRAJ = golden_cases.load_trajectories()
 
 
+@pytest.fixture(params=["device", "host"])
+def driver(request, monkeypatch):
+  """QTQP.solve with the HIP backends either runs the whole iteration on the
+  device (the default) or the reference's host loop over DirectKktSolver."""
+  monkeypatch.setattr(direct._HipDenseBase, "device_driver", request.param == "device")
+  return request.param
+
+
 def _rel(u, v):
   return float(np.max(np.abs(u - v)) / max(1.0, np.max(np.abs(v))))
 
@@ -117,7 +125,7 @@ def test_backend_solve_and_matvec_match_numpy(require_gpu, backend, mn):
 
 @pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
 @pytest.mark.parametrize("key", sorted(TRAJ))
-def test_end_to_end_matches_reference_fixtures(require_gpu, linear_solver, key):
+def test_end_to_end_matches_reference_fixtures(require_gpu, driver, linear_solver, key):
   case = TRAJ[key]
   a, b, c, z, p = golden_cases.inputs(case["spec"])
   sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
@@ -138,7 +146,7 @@ def test_end_to_end_matches_reference_fixtures(require_gpu, linear_solver, key):
 @pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
 @pytest.mark.parametrize("kind", ["feasible", "infeasible", "unbounded"])
 @pytest.mark.parametrize("seed", [42, 43])
-def test_reference_problem_families(require_gpu, linear_solver, kind, seed):
+def test_reference_problem_families(require_gpu, driver, linear_solver, kind, seed):
   m, n, z = 150, 100, 10
   gen = {"feasible": G.gen_feasible, "infeasible": G.gen_infeasible,
          "unbounded": G.gen_unbounded}[kind]
@@ -156,7 +164,7 @@ def test_reference_problem_families(require_gpu, linear_solver, kind, seed):
 
 
 @pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
-def test_dense_generator_planted_optimum(require_gpu, linear_solver):
+def test_dense_generator_planted_optimum(require_gpu, driver, linear_solver):
   qp = problems.random_dense_qp(200, 300, 50, seed=0)
   sol = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p)).solve(
       linear_solver=linear_solver, verbose=False, atol=1e-10, rtol=1e-11)
@@ -166,7 +174,7 @@ def test_dense_generator_planted_optimum(require_gpu, linear_solver):
 
 
 @pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
-def test_condensed_mpc_matches_oracle(require_gpu, linear_solver):
+def test_condensed_mpc_matches_oracle(require_gpu, driver, linear_solver):
   qp = problems.mpc_qp(32, 8, 3, seed=11)
   ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
   sol = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p)).solve(
@@ -178,7 +186,7 @@ def test_condensed_mpc_matches_oracle(require_gpu, linear_solver):
 
 @pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
 @pytest.mark.parametrize("mnzs", [(30, 20, 5, 42), (150, 100, 10, 7)])
-def test_min_iterative_refinement_steps(require_gpu, linear_solver, mnzs):
+def test_min_iterative_refinement_steps(require_gpu, driver, linear_solver, mnzs):
   # tests/test_qtqp.py:1862-1873: one backend solve per KKT solve.  The
   # (150, 100, 10, 7) instance hits max_iter with the FP64 oracle too; the
   # bar is the oracle's status.
@@ -213,3 +221,68 @@ def test_full_size_dense_qp_matches_oracle(require_gpu):
   assert sol.status.value == ref["status"] == "solved"
   for k in "xys":
     assert _rel(getattr(sol, k), ref[k]) < 1e-6, k
+
+
+STAT_KEYS = {"iter", "ctx", "bty", "pcost", "dcost", "pres", "dres", "gap", "pinfeas", "dinfeas",
+             "dinfeas_a", "dinfeas_p", "mu", "sigma", "alpha", "tau", "norm_x", "norm_y", "status",
+             "time", "prelrhs", "drelrhs", "complementarity", "norm_s", "max_sy", "min_sy",
+             "std_sy", "max_s_over_y", "min_s_over_y", "mean_s_over_y", "std_s_over_y",
+             "q_lin_sys_stats", "predictor_lin_sys_stats", "corrector_lin_sys_stats"}
+
+
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+@pytest.mark.parametrize("equilibrate", [True, False])
+def test_device_driver_tracks_host_driver(require_gpu, monkeypatch, linear_solver, equilibrate):
+  """The device-resident loop and the host loop over the same kernels take the
+  same path: same statuses and iteration counts, per-iteration mu, tau,
+  sigma and alpha to 1e-6 while mu is above 1e-6, the same stats keys, and
+  IR solve counts that differ only where a residual sits at its tolerance."""
+  a, b, c, p = G.gen_feasible(150, 100, 10, 42)
+  runs = {}
+  for dev in (True, False):
+    monkeypatch.setattr(direct._HipDenseBase, "device_driver", dev)
+    runs[dev] = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=10, p=sp.csc_matrix(p)).solve(
+        linear_solver=linear_solver, equilibrate=equilibrate, verbose=False, collect_stats=True)
+  d, h = runs[True], runs[False]
+  assert d.status == h.status == qtqp_amd.SolutionStatus.SOLVED
+  assert len(d.stats) == len(h.stats)
+  for sd, sh in zip(d.stats, h.stats):
+    assert set(sd) == set(sh) == STAT_KEYS
+    assert set(sd["predictor_lin_sys_stats"]) == set(sh["predictor_lin_sys_stats"])
+    if sh["mu"] > 1e-6:
+      for k in ("mu", "tau", "sigma", "alpha", "pcost", "dcost"):
+        assert abs(sd[k] - sh[k]) <= 1e-6 * max(1.0, abs(sh[k])), (sh["iter"], k)
+  for k in "xys":
+    assert _rel(getattr(d, k), getattr(h, k)) < 1e-6, k
+  G.check_solution(d.x, d.y, d.s, a, b, c, p, 10)
+
+
+def test_device_driver_verbose_log(require_gpu, capsys):
+  """Banner, header, one line per iteration and the footer, as the host loop prints."""
+  a, b, c, p = G.gen_feasible(30, 20, 5, 3)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=5, p=sp.csc_matrix(p)).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE, verbose=True, collect_stats=True)
+  out = capsys.readouterr().out.splitlines()
+  assert out[0].startswith("| QTQP v") and "linear_solver=HIP_DENSE" in out[0]
+  rows = [ln for ln in out if ln.startswith("|") and ln[2:6].strip().isdigit()]
+  assert len(rows) == len(sol.stats)
+  assert out[-1] == "| Solved"
+
+
+def test_device_driver_without_equality_rows_and_p(require_gpu):
+  """z = 0 and P = 0 (an LP): the device loop skips every P product."""
+  rng = np.random.default_rng(5)
+  m, n = 60, 25
+  a = rng.standard_normal((m, n))
+  x0 = rng.standard_normal(n)
+  s0 = np.abs(rng.standard_normal(m))
+  y0 = np.where(rng.uniform(size=m) < 0.5, np.abs(rng.standard_normal(m)), 0.0)
+  s0[y0 > 0] = 0.0
+  b = a @ x0 + s0
+  c = -a.T @ y0
+  ref = O.solve_qp(a, b, c, 0, np.zeros((n, n)))
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=0).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE_FP64, verbose=False)
+  assert sol.status.value == ref["status"]
+  if ref["status"] == "solved":
+    assert abs(c @ sol.x - c @ ref["x"]) <= 1e-6 * max(1.0, abs(c @ ref["x"]))
