@@ -73,6 +73,36 @@ __device__ inline double block_max(double v, double* red) {
   return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// Completes R per-lane partial sums (or maxima, MAX) across the wavefront
+// (R a power of two up to 32): a transpose-reduce that halves the rows a lane carries
+// at each of log2(R) exchanges, then combines the rest.  Lane l ends with
+// the result of row l >> (6 - log2 R) in d[0].
+template <int R, bool MAX = false>
+__device__ inline void reduce_rows(double* d) {
+  const int lane = threadIdx.x & 63;
+  int cnt = R;
+#pragma unroll
+  for (int bit = 32; bit >= 64 / R; bit >>= 1) {
+    const int half = cnt / 2;
+    const bool hi = lane & bit;
+#pragma unroll
+    for (int q = 0; q < R / 2; ++q) {
+      if (q < half) {
+        const double send = hi ? d[q] : d[q + half];
+        const double keep = hi ? d[q + half] : d[q];
+        const double got = __shfl_xor(send, bit, 64);
+        d[q] = MAX ? fmax(keep, got) : keep + got;
+      }
+    }
+    cnt = half;
+  }
+#pragma unroll
+  for (int bit = 32 / R; bit >= 1; bit >>= 1) {
+    const double got = __shfl_xor(d[0], bit, 64);
+    d[0] = MAX ? fmax(d[0], got) : d[0] + got;
+  }
+}
+
 // Loads through a pointer known to address global memory (device buffers
 // reached through tables in memory are otherwise accessed with flat
 // instructions, whose completion is also counted with LDS operations, so an

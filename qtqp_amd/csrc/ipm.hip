@@ -943,8 +943,9 @@ int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) 
   HIPC(hipGetLastError());
 
   // termination (__init__.py:888-1008)
-  if ((rc = rowscols<double>(k, k->A64, m, n, h->xy, h->ax, h->xy + n, nullptr, k->part))) return rc;
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(n)), gb, 0, st, k->part, nblk_rows(m), n, nullptr,
+  if ((rc = tile_pass<double>(k, k->A64, m, n, h->xy, k->rpa, h->xy + n, nullptr))) return rc;
+  hipLaunchKernelGGL(k_rowfin, dim3(blocks_for(m)), gb, 0, st, k->rpa, tiles_c(n), m, h->ax);
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), gb, 0, st, k->part, tiles_r(m), n, nullptr,
                      h->aty);
   if (h->has_p)
     hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,

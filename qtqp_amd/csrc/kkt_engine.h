@@ -140,78 +140,143 @@ __global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int 
   });
 }
 
-// One pass over RB rows of a row-major matrix M (rows x ncol):
-//   rowout[r] = M[r,:] . xv                 (if xv)
-//   part[blk][j] = sum_{r in blk} M[r][j] * u_r   (if uv), with
-//   u_r = uv[r] * (wdiag ? -1/wdiag[r] : 1).
-template <typename T>
-__global__ __launch_bounds__(WG) void k_rowscols(const T* __restrict__ M, int rows, int ncol,
-                                                 const double* __restrict__ xv,
-                                                 double* __restrict__ rowout,
-                                                 const double* __restrict__ uv,
-                                                 const double* __restrict__ wdiag,
-                                                 double* __restrict__ part) {
-  __shared__ double red[RB][4];
-  __shared__ double su[RB];
-  const int r0 = blockIdx.x * RB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (uv && tid < RB) {
-    const int r = r0 + tid;
-    double u = 0.0;
-    if (r < rows) {
-      u = uv[r];
-      if (wdiag) u *= -1.0 / wdiag[r];
-    }
-    su[tid] = u;
+// Row and column passes over a row-major matrix M (rows x ncol) read it once
+// in TR x TC tiles, one tile per workgroup, grid (ceil(rows/TR), ceil(ncol/TC)):
+//   HX: rpart[blockIdx.y * rows + r] = sum_{j in tile} M[r][j] x[j]
+//   HU: cpart[blockIdx.x * ncol + j] = sum_{r in tile} M[r][j] u_r,
+//       u_r = u[r] * (w ? -1/w[r] : 1)
+// Wave v takes tile rows 16v .. 16v+15 and lane l the four consecutive
+// columns 4l .. 4l+3, fetched as one 16-byte vector (two for FP64) when rows
+// are 16-byte aligned (VEC).  The row sums are completed across the lanes by
+// a transpose-reduce, the column sums across the four waves in LDS.  The
+// finish kernels below add the partials in a fixed order, so results are
+// deterministic.
+constexpr int TR = 64;
+constexpr int TC = 256;
+inline int tiles_r(int64_t rows) { return (int)((rows + TR - 1) / TR); }
+inline int tiles_c(int64_t ncol) { return (int)((ncol + TC - 1) / TC); }
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  __device__ static void load(const float* p, double* d) {
+    const type v = *reinterpret_cast<const type*>(p);
+    d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
   }
-  __syncthreads();
-  double dots[RB];
+};
+template <> struct Vec4<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  __device__ static void load(const double* p, double* d) {
+    const type v0 = reinterpret_cast<const type*>(p)[0];
+    const type v1 = reinterpret_cast<const type*>(p)[1];
+    d[0] = v0.x; d[1] = v0.y; d[2] = v1.x; d[3] = v1.y;
+  }
+};
+
+template <typename T, bool HX, bool HU, bool VEC>
+__global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int rows, int ncol,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ rpart,
+                                                  const double* __restrict__ u,
+                                                  const double* __restrict__ w,
+                                                  double* __restrict__ cpart) {
+  constexpr int RPW = 16;  // rows per wave
+  __shared__ double cs[4][TC];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * TR + wave * RPW;
+  const int j0 = blockIdx.y * TC + 4 * lane;  // this lane's first column
+  const bool full = j0 + 3 < ncol;
+  double a[RPW][4];
 #pragma unroll
-  for (int q = 0; q < RB; ++q) dots[q] = 0.0;
-  for (int j = tid; j < ncol; j += WG) {
-    const double xj = xv ? xv[j] : 0.0;
-    double colacc = 0.0;
+  for (int q = 0; q < RPW; ++q) {
+    const int r = r0 + q;
+    const T* row = M + (int64_t)r * ncol + j0;
+    if (VEC && full && r < rows) {
+      Vec4<T>::load(row, a[q]);
+    } else {
 #pragma unroll
-    for (int q = 0; q < RB; ++q) {
+      for (int c = 0; c < 4; ++c) a[q][c] = (r < rows && j0 + c < ncol) ? (double)row[c] : 0.0;
+    }
+  }
+  if (HX) {
+    double xv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xv[c] = j0 + c < ncol ? x[j0 + c] : 0.0;
+    double rd[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      rd[q] = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) rd[q] += a[q][c] * xv[c];
+    }
+    reduce_rows<RPW>(rd);  // lane l: row l >> 2
+    const int q = lane >> 2;
+    if ((lane & 3) == 0 && r0 + q < rows) rpart[(int64_t)blockIdx.y * rows + r0 + q] = rd[0];
+  }
+  if (HU) {
+    double cacc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
       const int r = r0 + q;
+      double ur = 0.0;
       if (r < rows) {
-        const double a = (double)M[(int64_t)r * ncol + j];
-        dots[q] += a * xj;
-        if (uv) colacc += a * su[q];
+        ur = u[r];
+        if (w) ur *= -1.0 / w[r];
       }
-    }
-    if (uv) part[(int64_t)blockIdx.x * ncol + j] = colacc;
-  }
-  if (xv) {
 #pragma unroll
-    for (int q = 0; q < RB; ++q) {
-      const double s = wave_sum(dots[q]);
-      if (lane == 0) red[q][wave] = s;
+      for (int c = 0; c < 4; ++c) cacc[c] += a[q][c] * ur;
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cs[wave][4 * lane + c] = cacc[c];
     __syncthreads();
-    if (tid < RB && r0 + tid < rows)
-      rowout[r0 + tid] = red[tid][0] + red[tid][1] + red[tid][2] + red[tid][3];
+    const int j = blockIdx.y * TC + threadIdx.x;
+    if (j < ncol)
+      cpart[(int64_t)blockIdx.x * ncol + j] =
+          (cs[0][threadIdx.x] + cs[1][threadIdx.x]) + (cs[2][threadIdx.x] + cs[3][threadIdx.x]);
   }
 }
 
-// out[j] = base[j] + sum_b part[b][j]  for j < ncol.
-__global__ __launch_bounds__(WG) void k_colsum(const double* __restrict__ part, int nblk, int ncol,
+// out[j] = (base ? base[j] : 0) + sum_{b < nb} cpart[b * ncol + j].  A workgroup
+// takes 64 columns; its four waves add every fourth partial, then meet in LDS.
+__global__ __launch_bounds__(WG) void k_colfin(const double* __restrict__ cpart, int nb, int ncol,
                                                const double* __restrict__ base,
                                                double* __restrict__ out) {
-  const int j = blockIdx.x * WG + threadIdx.x;
-  if (j >= ncol) return;
-  double s = base ? base[j] : 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * ncol + j];
-  out[j] = s;
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (j < ncol)
+    for (int b = wave; b < nb; b += 4) s += cpart[(int64_t)b * ncol + j];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && j < ncol) {
+    const double t = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    out[j] = base ? base[j] + t : t;
+  }
 }
 
-// K_reg x (or a residual):  kx = [Poffx + d_x*x_x + Atx ; Ax + d_y*x_y].
-// If rhs: out = rhs - kx (+ corr*x if corr), and *norm gets max |out|.
+// sum_{b < nb} rpart[b * rows + r]
+__device__ inline double row_total(const double* __restrict__ rpart, int nb, int rows, int r) {
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += rpart[(int64_t)b * rows + r];
+  return s;
+}
+
+__global__ void k_rowfin(const double* __restrict__ rpart, int nb, int rows,
+                         double* __restrict__ out) {
+  const int r = blockIdx.x * WG + threadIdx.x;
+  if (r < rows) out[r] = row_total(rpart, nb, rows, r);
+}
+
+// K_reg x (or a residual):  kx = [P_off x_x + d_x*x_x + A'x_y ; A x_x + d_y*x_y]
+// from the row partials of P_off x_x (rpp) and A x_x (rpa, ncb tiles each)
+// and aty = A'x_y.  If rhs: out = rhs - kx (+ corr*x if corr), and *norm gets
+// max |out|.
 __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* __restrict__ x,
                                                    const double* __restrict__ diag,
-                                                   const double* __restrict__ poffx,
-                                                   const double* __restrict__ part, int nblk,
-                                                   const double* __restrict__ ax,
+                                                   const double* __restrict__ rpp,
+                                                   const double* __restrict__ rpa, int ncb,
+                                                   const double* __restrict__ aty,
                                                    const double* __restrict__ rhs,
                                                    const double* __restrict__ corr,
                                                    double* __restrict__ out,
@@ -222,13 +287,8 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
   bool live = idx < n + m;
   if (live) {
     double kx;
-    if (idx < n) {
-      double s = poffx[idx] + diag[idx] * x[idx];
-      for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * n + idx];
-      kx = s;
-    } else {
-      kx = ax[idx - n] + diag[idx] * x[idx];
-    }
+    if (idx < n) kx = row_total(rpp, ncb, n, idx) + diag[idx] * x[idx] + aty[idx];
+    else kx = row_total(rpa, ncb, m, idx - n) + diag[idx] * x[idx];
     if (rhs) {
       v = rhs[idx] - kx;
       if (corr) v += corr[idx] * x[idx];
@@ -292,17 +352,23 @@ __device__ inline void publish(unsigned* flag, unsigned val) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// out[r] = sum_c M[c*64 + r] w[c] for the 64 x 64 block M (r = lane); the
-// four waves split c and reduce through red[4][64].
+// Thread (r = lane, q = wave) part of the 64 x 64 block M: M[c*64 + r] for
+// its 16 columns c = 16q .. 16q+15.
 template <typename T>
-__device__ inline double block_gemv(const T* __restrict__ M, const double* w, double* red) {
+__device__ inline void load_block_part(const T* __restrict__ M, T* mv) {
+  const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) mv[cc] = M[(q * 16 + cc) * NB + r];
+}
+
+// out[r] = sum_c M[c*64 + r] w[c] with the block held as load_block_part
+// leaves it; the four waves split c and reduce through red[4][64].
+template <typename T>
+__device__ inline double block_gemv(const T* mv, const double* w, double* red) {
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
   double acc = 0.0;
 #pragma unroll
-  for (int cc = 0; cc < 16; ++cc) {
-    const int c = q * 16 + cc;
-    acc += (double)M[c * NB + r] * w[c];
-  }
+  for (int cc = 0; cc < 16; ++cc) acc += (double)mv[cc] * w[q * 16 + cc];
   red[q * NB + r] = acc;
   __syncthreads();
   const double v = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
@@ -329,7 +395,9 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
   const bool live = r < rows;
   bool ok = true;
   double acc = 0.0;
-  T cur[16], nxt[16];
+  T cur[16], nxt[16], inv[16];
+  // the block inverse is needed only after the last wait: fetch it now
+  load_block_part(LiB + (int64_t)I * 2 * NB * NB, inv);
   auto load = [&](int J, T* dst) {
     const T* col = L + (int64_t)(J * NB + q * 16) * ld + i0 + r;
 #pragma unroll
@@ -352,7 +420,7 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
     w[r] = live ? g[i0 + r] - s : 0.0;
   }
   __syncthreads();
-  const double vi = block_gemv(LiB + (int64_t)I * 2 * NB * NB, w, red);
+  const double vi = block_gemv(inv, w, red);
   if (q == 0 && live) v[i0 + r] = ok ? (T)vi : T(NAN);
   publish(&flags[I], epoch);
   (void)nt;
@@ -378,7 +446,8 @@ __global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_
   double acc[16];
 #pragma unroll
   for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
-  T cur[16], nxt[16];
+  T cur[16], nxt[16], inv[16];
+  load_block_part(LiB + (int64_t)I * 2 * NB * NB + NB * NB, inv);
   auto load = [&](int J, T* dst) {
     const int row = J * NB + rr;
     const bool in = row < n;
@@ -415,14 +484,15 @@ __global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_
     w[rr] = rr < rows ? (double)v[i0 + rr] - s : 0.0;
   }
   __syncthreads();
-  const double xi = block_gemv(LiB + (int64_t)I * 2 * NB * NB + NB * NB, w, red);
+  const double xi = block_gemv(inv, w, red);
   if (q == 0 && rr < rows) x[i0 + rr] = ok ? xi : (double)NAN;
   publish(&flags[I], epoch);
 }
 
-// y = W (Ax - r_y) with W = -1/d_y; writes [x; y] into out (or adds it).
+// y = W (Ax - r_y) with W = -1/d_y, A x from its row partials (rpa, ncb
+// tiles); writes [x; y] into out (or adds it).
 __global__ __launch_bounds__(WG) void k_solve_finish(int n, int m, const double* __restrict__ x,
-                                                     const double* __restrict__ ax,
+                                                     const double* __restrict__ rpa, int ncb,
                                                      const double* __restrict__ r,
                                                      const double* __restrict__ diag,
                                                      double* __restrict__ out, int accumulate) {
@@ -430,7 +500,7 @@ __global__ __launch_bounds__(WG) void k_solve_finish(int n, int m, const double*
   if (idx >= n + m) return;
   double v;
   if (idx < n) v = x[idx];
-  else v = (-1.0 / diag[idx]) * (ax[idx - n] - r[idx]);
+  else v = (-1.0 / diag[idx]) * (row_total(rpa, ncb, m, idx - n) - r[idx]);
   out[idx] = accumulate ? out[idx] + v : v;
 }
 
@@ -481,6 +551,22 @@ __global__ __launch_bounds__(WG) void k_transpose(const double* __restrict__ src
   }
 }
 
+// dst = src and ||src||_inf into *norm (NaN reported as inf).
+__global__ __launch_bounds__(WG) void k_copy_norm(int64_t count, const double* __restrict__ src,
+                                                  double* __restrict__ dst,
+                                                  unsigned long long* norm) {
+  __shared__ double red[4];
+  const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
+  double a = 0.0;
+  if (i < count) {
+    const double v = src[i];
+    dst[i] = v;
+    a = v != v ? INFINITY : fabs(v);
+  }
+  a = block_max(a, red);
+  if (threadIdx.x == 0) atomic_max_nonneg(norm, a);
+}
+
 // Zeroes the diagonal of the row-major n x n matrix M.
 __global__ void k_zero_diag(double* M, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
@@ -510,7 +596,8 @@ struct qtqp_kkt {
   // vectors
   double *vin = nullptr, *vout = nullptr, *vsol = nullptr, *vres = nullptr;
   double *rin = nullptr, *racc = nullptr, *rres = nullptr, *rtmp = nullptr;
-  double *tx = nullptr, *ty = nullptr, *tg = nullptr, *tpx = nullptr, *part = nullptr;
+  double *tx = nullptr, *tg = nullptr, *part = nullptr;
+  double *rpa = nullptr, *rpp = nullptr, *aty = nullptr;  // row partials of A, P_off; A'v
   double* tv = nullptr;    // forward-sweep result of the triangular solves
   double* linv = nullptr;  // inverses of the 64x64 diagonal blocks of the factor
   unsigned* sync = nullptr;  // tickets and block flags of the triangular sweeps
@@ -539,16 +626,38 @@ int fail(int code, const std::string& msg) {
 
 int nblk_rows(int64_t rows) { return (int)((rows + RB - 1) / RB); }
 
-// t = M x (rows) and/or part = column partials of M' u.
+// One tile pass over M (k_tile_pass): row partials into rpart (if x) and
+// column partials into h->part (if u).
+template <typename T, bool VEC>
+void launch_tile_pass(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* x,
+                      double* rpart, const double* u, const double* w) {
+  const dim3 grid(tiles_r(rows), tiles_c(ncol));
+  if (x && u)
+    hipLaunchKernelGGL((k_tile_pass<T, true, true, VEC>), grid, dim3(WG), 0, h->stream, M,
+                       (int)rows, (int)ncol, x, rpart, u, w, h->part);
+  else if (x)
+    hipLaunchKernelGGL((k_tile_pass<T, true, false, VEC>), grid, dim3(WG), 0, h->stream, M,
+                       (int)rows, (int)ncol, x, rpart, u, w, h->part);
+  else
+    hipLaunchKernelGGL((k_tile_pass<T, false, true, VEC>), grid, dim3(WG), 0, h->stream, M,
+                       (int)rows, (int)ncol, x, rpart, u, w, h->part);
+}
+
 template <typename T>
-int rowscols(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* xv,
-             double* rowout, const double* uv, const double* wdiag, double* part) {
+int tile_pass(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* x,
+              double* rpart, const double* u, const double* w) {
   if (rows == 0) return QTQP_OK;
-  hipLaunchKernelGGL(k_rowscols<T>, dim3(nblk_rows(rows)), dim3(WG), 0, h->stream, M, (int)rows,
-                     (int)ncol, xv, rowout, uv, wdiag, part);
+  // rows start on 16-byte boundaries when ncol * sizeof(T) is a multiple of 16
+  // (device allocations are 256-byte aligned)
+  const bool vec = (ncol * (int64_t)sizeof(T)) % 16 == 0 &&
+                   (reinterpret_cast<uintptr_t>(M) & 15) == 0;
+  if (vec) launch_tile_pass<T, true>(h, M, rows, ncol, x, rpart, u, w);
+  else launch_tile_pass<T, false>(h, M, rows, ncol, x, rpart, u, w);
   HIPC(hipGetLastError());
   return QTQP_OK;
 }
+
+inline unsigned colfin_blocks(int64_t ncol) { return (unsigned)((ncol + 63) / 64); }
 
 // out = K_reg x, or out = rhs - K_reg x + corr*x with ||out||_inf into *norm_host.
 //  With rhs_norm_host, also reads back dnorm[1] (set by k_negate_y).
@@ -556,11 +665,13 @@ int kmatvec(qtqp_kkt* h, const double* x, const double* rhs, const double* corr,
             double* norm_host, double* rhs_norm_host = nullptr) {
   const int64_t n = h->n, m = h->m, N = n + m;
   int rc;
-  if ((rc = rowscols<double>(h, h->A64, m, n, x, h->ty, x + n, nullptr, h->part))) return rc;
-  if ((rc = rowscols<double>(h, h->Poff64, n, n, x, h->tpx, nullptr, nullptr, nullptr))) return rc;
+  if ((rc = tile_pass<double>(h, h->A64, m, n, x, h->rpa, x + n, nullptr))) return rc;
+  if ((rc = tile_pass<double>(h, h->Poff64, n, n, x, h->rpp, nullptr, nullptr))) return rc;
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
+                     tiles_r(m), (int)n, nullptr, h->aty);
   if (norm_host) HIPC(hipMemsetAsync(h->dnorm, 0, sizeof(unsigned long long), h->stream));
   hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
-                     h->diag, h->tpx, h->part, nblk_rows(m), h->ty, rhs, corr, out,
+                     h->diag, h->rpp, h->rpa, tiles_c(n), h->aty, rhs, corr, out,
                      norm_host ? h->dnorm : nullptr);
   HIPC(hipGetLastError());
   if (norm_host) {
@@ -628,9 +739,9 @@ int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out
   const int64_t n = h->n, m = h->m, N = n + m;
   int rc;
   // g = r_x + A'(W r_y)
-  if ((rc = rowscols<T>(h, A, m, n, nullptr, nullptr, r + n, h->diag + n, h->part))) return rc;
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(n)), dim3(WG), 0, h->stream, h->part, nblk_rows(m),
-                     (int)n, r, h->tg);
+  if ((rc = tile_pass<T>(h, A, m, n, nullptr, nullptr, r + n, h->diag + n))) return rc;
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
+                     tiles_r(m), (int)n, r, h->tg);
   HIPC(hipGetLastError());
   const int nt = (int)((n + NB - 1) / NB);
   const unsigned base = h->trsv_epoch * (unsigned)nt;
@@ -644,9 +755,9 @@ int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out
                      tv, h->tx, h->sync, base, epoch);
   HIPC(hipGetLastError());
   h->trsv_epoch++;
-  if ((rc = rowscols<T>(h, A, m, n, h->tx, h->ty, nullptr, nullptr, nullptr))) return rc;
+  if ((rc = tile_pass<T>(h, A, m, n, h->tx, h->rpa, nullptr, nullptr))) return rc;
   hipLaunchKernelGGL(k_solve_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m,
-                     h->tx, h->ty, r, h->diag, out, accumulate);
+                     h->tx, h->rpa, tiles_c(n), r, h->diag, out, accumulate);
   HIPC(hipGetLastError());
   h->info_acc.backend_solves++;
   return QTQP_OK;
@@ -657,8 +768,11 @@ int solve_current(qtqp_kkt* h, const double* r, double* out, int accumulate) {
   return solve_with<float>(h, h->A32, h->G32, r, out, accumulate);
 }
 
-// Backend solve: out = K_reg^{-1} r to FP64 accuracy (device buffers).
-int backend_solve(qtqp_kkt* h, const double* r, double* out) {
+// Backend solve: out = K_reg^{-1} r to FP64 accuracy (device buffers).  With
+// abs_target > 0 the FP32 refinement may stop once ||r - K_reg out||_inf is
+// below it: a caller refining against a looser tolerance gains nothing from
+// the last digits.
+int backend_solve(qtqp_kkt* h, const double* r, double* out, double abs_target = 0.0) {
   const int64_t N = h->n + h->m;
   if (h->fprec == 64) return solve_current(h, r, out, 0);
   // FP32 factor: refine in FP64 against the exact K_reg.
@@ -671,12 +785,17 @@ int backend_solve(qtqp_kkt* h, const double* r, double* out) {
   constexpr double kFloor = 1e-13;
   int rc;
   double r0 = 0.0, prev = 0.0, cur = 0.0;
-  // ||r|| for the relative target
+  // the residual of x = 0 is r itself: rres = r, r0 = ||r||_inf, racc = 0
   {
     HIPC(hipMemsetAsync(h->dnorm, 0, sizeof(unsigned long long), h->stream));
-    // reuse k_kmv_finish-free path: a residual with x = 0 is r itself
     HIPC(hipMemsetAsync(h->racc, 0, sizeof(double) * N, h->stream));
-    if ((rc = kmatvec(h, h->racc, r, nullptr, h->rres, &r0))) return rc;
+    hipLaunchKernelGGL(k_copy_norm, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, r, h->rres,
+                       h->dnorm);
+    HIPC(hipGetLastError());
+    unsigned long long bits = 0;
+    HIPC(hipMemcpyAsync(&bits, h->dnorm, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+    HIPC(hipStreamSynchronize(h->stream));
+    std::memcpy(&r0, &bits, sizeof(double));
   }
   if (r0 == 0.0) {
     HIPC(hipMemsetAsync(out, 0, sizeof(double) * N, h->stream));
@@ -688,7 +807,7 @@ int backend_solve(qtqp_kkt* h, const double* r, double* out) {
     if (h->fprec == 32) h->info_acc.inner_steps++;
     if ((rc = kmatvec(h, h->racc, r, nullptr, h->rres, &cur))) return rc;
     if (!(cur == cur)) break;
-    if (cur <= kTarget * r0) break;
+    if (cur <= kTarget * r0 || cur <= abs_target) break;
     if (cur > kPoorContraction * prev) {
       // Stalled.  Below kFloor the FP64 residual itself is at its rounding
       // floor (an FP64 factor would stop there too); above it the FP32
@@ -715,8 +834,8 @@ T* dalloc(int64_t count, hipError_t* err) {
 void release(qtqp_kkt* h) {
   void* bufs[] = {h->A64, h->Poff64, h->A32, h->Poff32, h->diag, h->corr, h->G64, h->G32,
                   h->info, h->vin, h->vout, h->vsol, h->vres, h->rin, h->racc, h->rres,
-                  h->rtmp, h->tx, h->ty, h->tg, h->tpx, h->part, h->dnorm, h->dflag,
-                  h->tv, h->linv, h->sync};
+                  h->rtmp, h->tx, h->tg, h->part, h->dnorm, h->dflag,
+                  h->tv, h->linv, h->sync, h->rpa, h->rpp, h->aty};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -792,14 +911,15 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
   ALLOC(rres, double, N);
   ALLOC(rtmp, double, N);
   ALLOC(tx, double, n);
-  ALLOC(ty, double, std::max(m, n));
   ALLOC(tg, double, n);
-  ALLOC(tpx, double, n);
   ALLOC(tv, double, n);
   const int64_t nt = (n + NB - 1) / NB;
   ALLOC(linv, double, nt * 2 * NB * NB);
   ALLOC(sync, unsigned, 2 + 2 * nt);
   ALLOC(part, double, (int64_t)nblk_rows(std::max(m, n)) * n);
+  ALLOC(rpa, double, m * tiles_c(n));
+  ALLOC(rpp, double, n * tiles_c(n));
+  ALLOC(aty, double, n);
   ALLOC(dnorm, unsigned long long, 2);
   ALLOC(dflag, int, 1);
   if (precision == QTQP_PREC_MIXED) {
@@ -896,7 +1016,9 @@ int ir_refine(qtqp_kkt* h, const double* rhs, double* sol, int max_steps, double
   int status = 1, solves = 0;
   for (solves = 1; solves <= max_steps; ++solves) {
     old = res;
-    if ((rc = backend_solve(h, h->vres, h->vout))) return rc;
+    // An inner residual a tenth of the tolerance leaves the next outer
+    // residual below it (up to the correction term).
+    if ((rc = backend_solve(h, h->vres, h->vout, 0.1 * tol))) return rc;
     hipLaunchKernelGGL(k_axpy, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, 1.0, h->vout, sol);
     HIPC(hipGetLastError());
     if ((rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res))) return rc;
