@@ -228,9 +228,9 @@ def single_instance_leg(name, qp, oracle_ref=None):
 
     sel = types.SimpleNamespace(name=f"HIP_DENSE[{prec}]", value=Probe)
     log(f"{name}: {prec} solve (n={n}, m={m})")
+    prob = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc)
     t0 = time.perf_counter()
-    sol = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc).solve(
-        linear_solver=sel, verbose=False, collect_stats=True)
+    sol = prob.solve(linear_solver=sel, verbose=False, collect_stats=True)
     wall = time.perf_counter() - t0
     host_wall = None
     if prec == "mixed":
@@ -238,9 +238,9 @@ def single_instance_leg(name, qp, oracle_ref=None):
       # call pattern over the LinearSolver protocol), for comparison.
       Probe.device_driver = False
       log(f"{name}: {prec} solve, host iteration loop")
+      prob = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc)
       t1 = time.perf_counter()
-      hsol = qtqp_amd.QTQP(a=a_csc, b=qp.b, c=qp.c, z=qp.z, p=p_csc).solve(
-          linear_solver=sel, verbose=False)
+      hsol = prob.solve(linear_solver=sel, verbose=False)
       host_wall = {"wall_s": time.perf_counter() - t1, "status": hsol.status.value,
                    "max_rel_diff_xys_vs_device_loop": max(
                        float(np.max(np.abs(g - w)) / max(1.0, np.max(np.abs(w))))
@@ -297,6 +297,13 @@ def single_legs(problems, qtqp_oracle):
   except AttributeError:
     cores = os.cpu_count()
   out = {}
+  # first use of the GPU libraries (code-object loading) outside the timings
+  import scipy.sparse as sp
+  import qtqp_amd
+  w = problems.random_dense_qp(40, 60, 10, seed=1)
+  for ls in (qtqp_amd.LinearSolver.HIP_DENSE, qtqp_amd.LinearSolver.HIP_DENSE_FP64):
+    qtqp_amd.QTQP(a=sp.csc_matrix(w.a), b=w.b, c=w.c, z=w.z, p=sp.csc_matrix(w.p)).solve(
+        linear_solver=ls, verbose=False)
   for key, label, qp in (
       ("single_small_dense", "n=200 m=300 z=50 dense QP (seed 0)",
        problems.random_dense_qp(200, 300, 50, seed=0)),
@@ -382,6 +389,9 @@ def main():
   ap.add_argument("--no-plain", action="store_true")
   ap.add_argument("--single-only", action="store_true")
   ap.add_argument("--oracle-sample", type=int, default=64)
+  ap.add_argument("--profile-single", choices=("mixed", "fp64"),
+                  help="only solve the n=4000 m=6000 z=1000 dense QP once through QTQP.solve "
+                       "(for kernel profiles of the device-resident loop)")
   args = ap.parse_args()
   if args.gpus != 1:
     print(json.dumps({"error": "this build measures one GPU only; the batched engine shards "
@@ -389,6 +399,21 @@ def main():
     sys.exit(2)
   if args.calibrate_copy:
     calibrate_copy()
+    return
+  if args.profile_single:
+    import scipy.sparse as sp
+    import qtqp_amd
+    from qtqp_amd import problems
+    qp = problems.random_dense_qp(4000, 6000, 1000, seed=0)
+    ls = {"mixed": qtqp_amd.LinearSolver.HIP_DENSE,
+          "fp64": qtqp_amd.LinearSolver.HIP_DENSE_FP64}[args.profile_single]
+    q = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p))
+    t0 = time.perf_counter()
+    sol = q.solve(linear_solver=ls, verbose=False, collect_stats=True)
+    print(json.dumps({"workload": "n=4000 m=6000 z=1000 dense QP (seed 0)",
+                      "linear_solver": ls.name, "status": sol.status.value,
+                      "iterations": len(sol.stats), "solve_s": time.perf_counter() - t0}),
+          flush=True)
     return
 
   from qtqp_amd import batch, problems
