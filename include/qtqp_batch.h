@@ -44,8 +44,11 @@ typedef struct qtqp_batch_opts {
   double linear_solver_atol, linear_solver_rtol;
   int32_t max_iter, max_iterative_refinement_steps;
   int32_t precision;
-  int32_t use_graph;    /* replay the per-iteration launches from a hipGraph */
+  int32_t use_graph;    /* launch from a hipGraph (the solve, or each iteration) */
   int32_t equilibrate;  /* Ruiz scaling on the device (__init__.py:616-663) */
+  int32_t per_iteration;/* 0: one persistent launch for the whole solve, each
+                           workgroup iterating a QP to completion; 1: one launch
+                           per IPM iteration of every unfinished QP */
 } qtqp_batch_opts;
 
 /* Engine counters of one qtqp_batch_solve call. */
@@ -88,9 +91,16 @@ int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* t
                        int32_t* status, int32_t* iters, double* pres, double* dres,
                        double* gap);
 
-/* Device time of each IPM-iteration launch of the last solve (HIP events on
- * the launch stream).  Writes up to `cap` values; returns how many exist. */
+/* Device time of each IPM launch of the last solve (HIP events on the launch
+ * stream): one value for the persistent schedule, one per iteration with
+ * per_iteration.  Writes up to `cap` values; returns how many exist. */
 int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap);
+
+/* Completion time of each QP in the last solve, in creation order: ms of the
+ * device wall clock from the start of the first IPM launch to the end of the
+ * QP's last iteration (NaN if it did not run).  Writes up to `cap` values;
+ * returns how many exist.  No reference counterpart (batch diagnostics). */
+int qtqp_batch_instance_ms(qtqp_batch* h, double* ms, int32_t cap);
 
 /* Per-iteration trace of QP q for the last solve: 16 doubles per IPM
  * iteration (mu, sigma, alpha, tau, tau_predictor, tau_corrector,

@@ -30,7 +30,7 @@ class BatchOpts(ctypes.Structure):
               ("linear_solver_atol", ctypes.c_double), ("linear_solver_rtol", ctypes.c_double),
               ("max_iter", ctypes.c_int32), ("max_iterative_refinement_steps", ctypes.c_int32),
               ("precision", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-              ("equilibrate", ctypes.c_int32)]
+              ("equilibrate", ctypes.c_int32), ("per_iteration", ctypes.c_int32)]
 
 
 class BatchCounters(ctypes.Structure):
@@ -49,6 +49,7 @@ BATCH_SYMBOLS = {
     "qtqp_batch_results": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _i32p, _i32p,
                                           _dp, _dp, _dp]),
     "qtqp_batch_launch_times": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int32]),
+    "qtqp_batch_instance_ms": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int32]),
     "qtqp_batch_work": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "qtqp_batch_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, _dp, ctypes.c_int32]),
     "qtqp_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts),
@@ -235,14 +236,14 @@ class BatchQP:
             step_size_scale=0.99, min_static_regularization=1e-8,
             max_iterative_refinement_steps=20, linear_solver_atol=1e-12,
             linear_solver_rtol=1e-12, equilibrate=True, precision="mixed",
-            use_graph=True) -> BatchOpts:
+            use_graph=True, per_iteration=False) -> BatchOpts:
     assert atol >= 0 and rtol >= 0 and atol_infeas >= 0 and rtol_infeas >= 0
     assert max_iter > 0 and 0 < step_size_scale < 1 and max_iterative_refinement_steps >= 1
     prec = {"mixed": _native.PREC_MIXED, "fp64": _native.PREC_FP64}[precision]
     return BatchOpts(atol, rtol, atol_infeas, rtol_infeas, step_size_scale,
                      min_static_regularization, linear_solver_atol, linear_solver_rtol, max_iter,
                      max_iterative_refinement_steps, prec, int(bool(use_graph)),
-                     int(bool(equilibrate)))
+                     int(bool(equilibrate)), int(bool(per_iteration)))
 
   # Gram tile shapes and autotuner size buckets (include/qtqp_batch.h).
   TILE_SHAPES = ("64x64", "128x128")
@@ -273,8 +274,9 @@ class BatchQP:
 
   def solve_device(self, **opts) -> dict:
     """Runs the device IPM only (results stay on the GPU); returns counters.
-    Keyword options as QTQP.solve, plus precision ("mixed" | "fp64") and
-    use_graph."""
+    Keyword options as QTQP.solve, plus precision ("mixed" | "fp64"),
+    use_graph, and per_iteration (one launch per IPM iteration instead of one
+    persistent launch for the whole solve)."""
     o = self._opts(**opts)
     max_iter = o.max_iter
     cnt = BatchCounters()
@@ -349,8 +351,18 @@ class BatchQP:
     y, s = restore(y, s, 0.0, drop_s)
     return x, y, s
 
+  def instance_ms(self) -> np.ndarray:
+    """Completion time of each QP in the last solve: device wall-clock ms from
+    the start of the first IPM launch to the end of its last iteration."""
+    out = np.full(self.count, np.nan)
+    k = self._lib.qtqp_batch_instance_ms(self._h, _ptr(out, _dp), self.count)
+    if k < 0:
+      raise RuntimeError("qtqp_batch_instance_ms failed")
+    return out
+
   def launch_times_ms(self) -> np.ndarray:
-    """HIP-event durations of each IPM-iteration launch of the last solve."""
+    """HIP-event durations of the IPM launches of the last solve (one for the
+    persistent schedule, one per iteration with per_iteration)."""
     k = self._lib.qtqp_batch_launch_times(self._h, None, 0)
     out = np.zeros(max(k, 0))
     if k > 0:

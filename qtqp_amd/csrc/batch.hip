@@ -76,6 +76,7 @@ struct Opts {
 struct Counters {
   unsigned long long backend_solves, inner_steps, fallbacks, ir_bad;
   int done;
+  int max_iters;  // most IPM iterations any QP took
 };
 
 struct Smem {
@@ -344,11 +345,14 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
   return r;
 }
 
-// L v = v in place with stored diagonal-block inverses: per 64-block a
-// parallel 64x64 product with L_bb^{-1}, then the rows below are updated.
-// red: >= 4*64 doubles of LDS.
+// L v = v in place (L column-major n x n, lower) with the stored inverses of
+// its 64 x 64 diagonal blocks.  Per block: v_J = Li_J v_J (the four waves
+// split the 64 columns, LDS reduce), then v_i -= L[i][J] v_J for the rows
+// below, each wave taking 16 of the block's columns for 64 rows at a time
+// (loads along a column, contiguous in i) and the four partials meeting in
+// `part` (>= 4 * n doubles of LDS).
 template <typename T>
-__device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* red) {
+__device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* red, double* part) {
   const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int nb = min(NB, n - j0);
@@ -360,17 +364,32 @@ __device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* re
     __syncthreads();
     if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
     __syncthreads();
-    for (int i = j0 + nb + tid; i < n; i += WG) {
-      const T* col = L + (int64_t)j0 * n + i;
+    const int i0 = j0 + nb, rows = n - i0;
+    if (rows <= 0) break;
+    const int c0 = 16 * p;
+    for (int ib = 0; ib < rows; ib += 64) {
+      const int i = i0 + ib + lane;
       double a = 0.0;
-      for (int c = 0; c < nb; ++c) a += (double)ldg(col + (int64_t)c * n) * (double)v[j0 + c];
-      v[i] = (T)((double)v[i] - a);
+      if (i < n) {
+        const T* col = L + (int64_t)(j0 + c0) * n + i;
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc)
+          if (c0 + cc < nb) a += (double)ldg(col + (int64_t)cc * n) * (double)v[j0 + c0 + cc];
+      }
+      part[p * n + ib + lane] = a;
     }
+    __syncthreads();
+    for (int r = tid; r < rows; r += WG)
+      v[i0 + r] = (T)((double)v[i0 + r] -
+                      ((part[r] + part[n + r]) + (part[2 * n + r] + part[3 * n + r])));
     __syncthreads();
   }
 }
 
-// L' v = v in place with stored diagonal-block inverses.  rb: 64 doubles.
+// L' v = v in place with stored diagonal-block inverses.  Per block (last
+// first): rb = v_J - L[below][J]' v_below, read down the block's columns (wave
+// p takes columns 16p .. 16p+15, lanes consecutive rows, completed across
+// the lanes by a transpose-reduce), then v_J = Li_J' rb.  rb: 64 doubles.
 template <typename T>
 __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* red, double* rb) {
   const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;
@@ -378,20 +397,26 @@ __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* re
   for (int b = nblk - 1; b >= 0; --b) {
     const int j0 = b * NB, nb = min(NB, n - j0);
     const T* Li = LiB + (int64_t)b * 2 * NB * NB + NB * NB;  // Li[k*64 + c] (row-major)
-    double acc = 0.0;
-    if (lane < nb) {
-      const T* col = L + (int64_t)(j0 + lane) * n;
-      for (int i = j0 + nb + p; i < n; i += 4) acc += (double)ldg(col + i) * (double)v[i];
+    const int c0 = 16 * p;
+    double acc[16];
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
+    for (int i = j0 + nb + lane; i < n; i += 64) {
+      const double vi = (double)v[i];
+      const T* row = L + (int64_t)(j0 + c0) * n + i;
+#pragma unroll
+      for (int cc = 0; cc < 16; ++cc)
+        if (c0 + cc < nb) acc[cc] += (double)ldg(row + (int64_t)cc * n) * vi;
     }
-    red[p * NB + lane] = acc;
+    reduce_rows<16>(acc);  // lane l: column c0 + (l >> 2)
+    if ((lane & 3) == 0) red[c0 + (lane >> 2)] = acc[0];
     __syncthreads();
-    if (tid < nb)
-      rb[tid] = (double)v[j0 + tid] - ((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
+    if (tid < nb) rb[tid] = (double)v[j0 + tid] - red[tid];
     __syncthreads();
-    acc = 0.0;
+    double a2 = 0.0;
     if (lane < nb)
-      for (int k = p; k < nb; k += 4) acc += (double)ldg(Li + k * NB + lane) * rb[k];
-    red[p * NB + lane] = acc;
+      for (int k = p; k < nb; k += 4) a2 += (double)ldg(Li + k * NB + lane) * rb[k];
+    red[p * NB + lane] = a2;
     __syncthreads();
     if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
     __syncthreads();
@@ -693,7 +718,7 @@ __device__ __noinline__ void factor_solve(const Inst& I, const T* L, const T* Li
   T* v = reinterpret_cast<T*>(sm.v);
   for (int j = threadIdx.x; j < n; j += WG) v[j] = (T)g[j];
   __syncthreads();
-  trsv_lower_inv(L, n, LiB, v, sm.red);
+  trsv_lower_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
   trsv_upper_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
   for (int j = threadIdx.x; j < n; j += WG) xc[j] = (double)v[j];
   __syncthreads();
@@ -1292,10 +1317,52 @@ __global__ void k_reset_queue(int* queue) {
   if (threadIdx.x == 0) *queue = 0;
 }
 
+// clk[count] = the earliest workgroup start of the solve (set to the maximum
+// here, lowered with atomicMin); clk[q] = wall clock when QP q finished.
+__global__ void k_reset_clock(unsigned long long* clk, int count) {
+  const int i = blockIdx.x * WG + threadIdx.x;
+  if (i < count) clk[i] = 0;
+  else if (i == count) clk[count] = ~0ull;
+}
+
+// The whole solve in one persistent launch: each workgroup takes the next QP
+// in longest-first order and runs its IPM iterations to completion (a final
+// status or max_iter), then takes the next.  No workgroup waits on another,
+// and with the largest QPs first the launch ends on small ones.
+__global__ __launch_bounds__(WG, 2) void k_ipm_solve(const Inst* insts, const int* order, int count,
+                                                  int* queue, Opts o, Counters* cnt,
+                                                  unsigned long long* clk) {
+  Smem sm = lds_smem();
+  __shared__ int s_q;
+  if (threadIdx.x == 0) atomicMin(&clk[count], (unsigned long long)wall_clock64());
+  for (;;) {
+    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+    __syncthreads();
+    const int q = s_q;
+    __syncthreads();
+    if (q >= count) break;
+    const int id = order[q];
+    const Inst& I = insts[id];
+    bool ran = false;
+    while (I.si[I_STATUS] == QTQP_BATCH_UNFINISHED && I.si[I_ITERS] < o.max_iter) {
+      ipm_iteration(I, o, sm, cnt);
+      ran = true;
+    }
+    if (threadIdx.x == 0 && ran) {
+      atomicAdd(&cnt->done, 1);
+      atomicMax(&cnt->max_iters, I.si[I_ITERS]);
+      clk[id] = wall_clock64();
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int* order, int count,
-                                                 int* queue, Opts o, int nmax, Counters* cnt) {
+                                                 int* queue, Opts o, int nmax, Counters* cnt,
+                                                 unsigned long long* clk) {
   Smem sm = lds_smem();
   (void)nmax;
+  if (clk && threadIdx.x == 0) atomicMin(&clk[count], (unsigned long long)wall_clock64());
   __shared__ int s_q;
   for (;;) {
     if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
@@ -1307,8 +1374,11 @@ __global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int
     if (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter) continue;
     ipm_iteration(I, o, sm, cnt);
     if (threadIdx.x == 0 &&
-        (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter))
+        (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter)) {
       atomicAdd(&cnt->done, 1);
+      atomicMax(&cnt->max_iters, I.si[I_ITERS]);
+      if (clk) clk[order[q]] = wall_clock64();
+    }
     __syncthreads();
   }
 }
@@ -1339,6 +1409,8 @@ struct qtqp_batch {
   std::vector<hipEvent_t> iter_ev;  // per-iteration launch timing
   std::vector<int> order;           // longest-first queue order
   std::vector<double> iter_ms;
+  unsigned long long* dclk = nullptr;  // per-QP completion clock, [count] = solve start
+  std::vector<double> inst_ms;        // per-QP completion time of the last solve
 };
 
 namespace {
@@ -1543,25 +1615,27 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
       hipMalloc(&h->d_py, sizeof(double) * std::max<int64_t>(tot_m, 1)) != hipSuccess ||
       hipMalloc(&h->d_ps, sizeof(double) * std::max<int64_t>(tot_m, 1)) != hipSuccess ||
       hipMalloc(&h->d_scal, sizeof(double) * S_NSCAL * count) != hipSuccess ||
-      hipMalloc(&h->d_ints, sizeof(int) * I_NINT * count) != hipSuccess)
+      hipMalloc(&h->d_ints, sizeof(int) * I_NINT * count) != hipSuccess ||
+      hipMalloc(&h->dclk, sizeof(unsigned long long) * (count + 1)) != hipSuccess)
     return cleanup(fail(QTQP_ERR_HIP, "hipMalloc tables"));
   for (void* ptr : {(void*)h->dinst, (void*)h->dorder, (void*)h->dqueue, (void*)h->dcnt,
                     (void*)h->d_noff, (void*)h->d_moff, (void*)h->d_px, (void*)h->d_py,
-                    (void*)h->d_ps, (void*)h->d_scal, (void*)h->d_ints})
+                    (void*)h->d_ps, (void*)h->d_scal, (void*)h->d_ints, (void*)h->dclk})
     h->allocs.push_back(ptr);
   if (hipMemcpyAsync(h->d_noff, n_off, sizeof(int64_t) * count, hipMemcpyHostToDevice,
                      h->stream) != hipSuccess ||
       hipMemcpyAsync(h->d_moff, m_off, sizeof(int64_t) * count, hipMemcpyHostToDevice,
                      h->stream) != hipSuccess)
     return cleanup(fail(QTQP_ERR_HIP, "offset upload"));
-  // longest-processing-time order: descending m*n^2 (+ n^3/3)
+  // longest-processing-time order: descending estimated iteration cost, the
+  // Gram product and factor (m n^2 + n^3/3) plus the ~20 passes over A a
+  // bandwidth-bound iteration makes (weighted as 64 m n)
   std::vector<int> order(count);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int u, int v) {
-    const double cu = (double)m[u] * n[u] * n[u] + (double)n[u] * n[u] * n[u] / 3.0;
-    const double cv = (double)m[v] * n[v] * n[v] + (double)n[v] * n[v] * n[v] / 3.0;
-    return cu > cv;
-  });
+  auto cost = [&](int u) {
+    return (double)m[u] * n[u] * (n[u] + 64.0) + (double)n[u] * n[u] * n[u] / 3.0;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return cost(u) > cost(v); });
   h->order = order;
   if (hipMemcpyAsync(h->dorder, order.data(), sizeof(int) * count, hipMemcpyHostToDevice,
                      h->stream) != hipSuccess ||
@@ -1608,44 +1682,71 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   HIPC(hipGetDevice(&dev));
   HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int grid = std::min(h->count, std::max(1, cus * 2));
-  // graph of one iteration: reset queue, persistent iteration kernel
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
-  if (opts->use_graph) {
-    HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
-    hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
-                       h->dorder, h->count, h->dqueue, o, h->nmax, h->dcnt);
-    HIPC(hipStreamEndCapture(h->stream, &graph));
-    HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  int it = 0;
-  int check_every = 2;
+  hipLaunchKernelGGL(k_reset_clock, dim3(std::max(1, (h->count + 1 + WG - 1) / WG)), dim3(WG), 0,
+                     h->stream, h->dclk, h->count);
+  HIPC(hipGetLastError());
   while ((int)h->iter_ev.size() < o.max_iter + 1) {
     hipEvent_t ev;
     HIPC(hipEventCreate(&ev));
     h->iter_ev.push_back(ev);
   }
-  HIPC(hipEventRecord(h->iter_ev[0], h->stream));
-  for (it = 0; it < o.max_iter; ++it) {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int it = 0, launched = 0;
+  if (!opts->per_iteration) {
+    // the whole solve in one persistent launch (k_ipm_solve)
+    if (opts->use_graph) {
+      HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+      hipLaunchKernelGGL(k_ipm_solve, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
+                         h->dorder, h->count, h->dqueue, o, h->dcnt, h->dclk);
+      HIPC(hipStreamEndCapture(h->stream, &graph));
+      HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    HIPC(hipEventRecord(h->iter_ev[0], h->stream));
     if (exec) {
       HIPC(hipGraphLaunch(exec, h->stream));
     } else {
       hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
-      hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
-                         h->dorder, h->count, h->dqueue, o, h->nmax, h->dcnt);
+      hipLaunchKernelGGL(k_ipm_solve, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
+                         h->dorder, h->count, h->dqueue, o, h->dcnt, h->dclk);
       HIPC(hipGetLastError());
     }
-    HIPC(hipEventRecord(h->iter_ev[it + 1], h->stream));
-    if ((it + 1) % check_every == 0 || it + 1 == o.max_iter) {
-      Counters hc{};
-      HIPC(hipMemcpyAsync(&hc, h->dcnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
-      HIPC(hipStreamSynchronize(h->stream));
-      if (hc.done >= h->count) {
-        ++it;
-        break;
+    HIPC(hipEventRecord(h->iter_ev[1], h->stream));
+    launched = 1;
+  } else {
+    // one launch per IPM iteration, replayed from a graph {reset queue, k_ipm_iter}
+    if (opts->use_graph) {
+      HIPC(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+      hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
+                         h->dorder, h->count, h->dqueue, o, h->nmax, h->dcnt, h->dclk);
+      HIPC(hipStreamEndCapture(h->stream, &graph));
+      HIPC(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    }
+    const int check_every = 2;
+    HIPC(hipEventRecord(h->iter_ev[0], h->stream));
+    for (it = 0; it < o.max_iter; ++it) {
+      if (exec) {
+        HIPC(hipGraphLaunch(exec, h->stream));
+      } else {
+        hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+        hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, h->dinst,
+                           h->dorder, h->count, h->dqueue, o, h->nmax, h->dcnt, h->dclk);
+        HIPC(hipGetLastError());
+      }
+      HIPC(hipEventRecord(h->iter_ev[it + 1], h->stream));
+      if ((it + 1) % check_every == 0 || it + 1 == o.max_iter) {
+        Counters hc{};
+        HIPC(hipMemcpyAsync(&hc, h->dcnt, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+        HIPC(hipStreamSynchronize(h->stream));
+        if (hc.done >= h->count) {
+          ++it;
+          break;
+        }
       }
     }
+    launched = std::min(it, o.max_iter);
   }
   HIPC(hipEventRecord(h->ev1, h->stream));
   HIPC(hipEventSynchronize(h->ev1));
@@ -1653,12 +1754,24 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   HIPC(hipEventElapsedTime(&ms, h->ev0, h->ev1));
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
-  const int launched = std::min(it, o.max_iter);
   h->iter_ms.assign(launched, 0.0);
   for (int k = 0; k < launched; ++k) {
     float t = 0.f;
     HIPC(hipEventElapsedTime(&t, h->iter_ev[k], h->iter_ev[k + 1]));
     h->iter_ms[k] = t;
+  }
+  // per-QP completion times from the device wall clock
+  {
+    std::vector<unsigned long long> clk((size_t)h->count + 1);
+    HIPC(hipMemcpy(clk.data(), h->dclk, sizeof(unsigned long long) * (h->count + 1),
+                   hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIPC(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    h->inst_ms.assign(h->count, NAN);
+    const unsigned long long t0 = clk[h->count];
+    for (int q = 0; q < h->count; ++q)
+      if (clk[q] != 0 && t0 != ~0ull && clk[q] >= t0 && khz > 0)
+        h->inst_ms[q] = (double)(clk[q] - t0) / (double)khz;
   }
   if (counters) {
     Counters hc{};
@@ -1667,7 +1780,7 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
     counters->inner_steps = (int64_t)hc.inner_steps;
     counters->fp64_fallbacks = (int64_t)hc.fallbacks;
     counters->ir_not_converged = (int64_t)hc.ir_bad;
-    counters->iterations = std::min(it, o.max_iter);
+    counters->iterations = hc.max_iters;
     counters->device_ms = ms;
   }
   h->have_start = false;
@@ -1721,7 +1834,7 @@ int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t samp
       for (int it = 0; it < iters; ++it) {
         hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
         hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, tmp,
-                           tmp_order, k, h->dqueue, o, h->nmax, h->dcnt);
+                           tmp_order, k, h->dqueue, o, h->nmax, h->dcnt, nullptr);
       }
       if ((e = hipGetLastError()) != hipSuccess) break;
       if ((e = hipEventRecord(h->ev1, h->stream)) != hipSuccess) break;
@@ -1784,6 +1897,14 @@ int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* t
     if (iters) iters[q] = iv[I_ITERS];
   }
   return QTQP_OK;
+}
+
+int qtqp_batch_instance_ms(qtqp_batch* h, double* ms, int32_t cap) {
+  if (!h) return -1;
+  const int k = (int)h->inst_ms.size();
+  if (ms)
+    for (int i = 0; i < std::min(k, (int)cap); ++i) ms[i] = h->inst_ms[i];
+  return k;
 }
 
 int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap) {

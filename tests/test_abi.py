@@ -8,7 +8,7 @@ import subprocess
 
 import pytest
 
-from qtqp_amd import _native
+from qtqp_amd import _native, batch
 
 INCLUDE = pathlib.Path(__file__).resolve().parents[1] / "include"
 HEADER_TO_LIB = {"qtqp_kkt.h": "libqtqp_kkt.so", "qtqp_ipm.h": "libqtqp_kkt.so",
@@ -49,13 +49,16 @@ def _c_layout(tmp_path, header, struct, fields):
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
-@pytest.mark.parametrize("cls,struct", [(_native.IpmOpts, "qtqp_ipm_opts"),
-                                        (_native.IpmRecord, "qtqp_ipm_record"),
-                                        (_native.IrStats, "qtqp_ir_stats"),
-                                        (_native.KktInfo, "qtqp_kkt_info")])
-def test_ctypes_structs_match_c_layout(tmp_path, cls, struct):
+@pytest.mark.parametrize("cls,struct,header", [
+    (_native.IpmOpts, "qtqp_ipm_opts", "qtqp_ipm.h"),
+    (_native.IpmRecord, "qtqp_ipm_record", "qtqp_ipm.h"),
+    (_native.IrStats, "qtqp_ir_stats", "qtqp_kkt.h"),
+    (_native.KktInfo, "qtqp_kkt_info", "qtqp_kkt.h"),
+    (batch.BatchOpts, "qtqp_batch_opts", "qtqp_batch.h"),
+    (batch.BatchCounters, "qtqp_batch_counters", "qtqp_batch.h")])
+def test_ctypes_structs_match_c_layout(tmp_path, cls, struct, header):
   names = [f for f, _ in cls._fields_]
-  c = _c_layout(tmp_path, "qtqp_ipm.h", struct, names)
+  c = _c_layout(tmp_path, header, struct, names)
   assert c[0] == ctypes.sizeof(cls)
   assert c[1:] == [getattr(cls, f).offset for f in names]
 

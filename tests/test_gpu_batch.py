@@ -171,3 +171,27 @@ def test_full_size_uneven_mpc_batch(require_gpu):
     q = qps[i]
     _compare(res.solutions[i], O.solve_qp(q.a, q.b, q.c, q.z, q.p), 1e-6)
   bq.free()
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_persistent_and_per_iteration_schedules_agree_bitwise(require_gpu, use_graph):
+  """One persistent launch per solve (each workgroup iterates a QP to
+  completion) and one launch per IPM iteration run the same per-QP code on the
+  same data, so every result is bitwise identical; completion times are finite
+  and within the solve."""
+  qps = problems.uneven_mpc_batch(300, base_seed=5000)
+  bq = batch.BatchQP(qps)
+  outs, cnts, inst = [], [], []
+  for per_iteration in (False, True):
+    cnts.append(bq.solve_device(precision="mixed", per_iteration=per_iteration,
+                                use_graph=use_graph))
+    outs.append(bq.raw_iterates())
+    inst.append(bq.instance_ms())
+  for k in ("x", "y", "s", "tau", "status", "iters"):
+    assert np.array_equal(outs[0][k], outs[1][k]), k
+  assert cnts[0]["iterations"] == cnts[1]["iterations"] == int(outs[0]["iters"].max())
+  # the host checks for completion every second launch
+  assert cnts[1]["iterations"] <= len(bq.launch_times_ms()) <= cnts[1]["iterations"] + 1
+  for c, t in zip(cnts, inst):
+    assert np.all(np.isfinite(t)) and np.all(t >= 0) and np.max(t) <= c["device_ms"]
+  bq.free()
