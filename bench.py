@@ -183,6 +183,8 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
           "statuses_identical": bool(np.array_equal(warm["status"], cold["status"])),
           "fallback_rate": float(warm["fallback"].mean()),
           "cold_qps_per_s": n_eval / t_cold, "warm_qps_per_s_incl_inference": n_eval / t_warm,
+          "cold_device_ms": float(cold["device_ms"]), "warm_device_ms": float(warm["device_ms"]),
+          "device_time_cut": float(1.0 - warm["device_ms"] / cold["device_ms"]),
           "oracle_sample": len(qps), "oracle_status_match": bool(ok),
           "max_rel_kkt_residual": float(max(res)), "max_rel_x_diff_vs_oracle": float(max(dx))}
 
