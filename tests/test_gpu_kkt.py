@@ -286,3 +286,43 @@ def test_device_driver_without_equality_rows_and_p(require_gpu):
   assert sol.status.value == ref["status"]
   if ref["status"] == "solved":
     assert abs(c @ sol.x - c @ ref["x"]) <= 1e-6 * max(1.0, abs(c @ ref["x"]))
+
+
+@pytest.mark.parametrize("linear_solver", ENUMS, ids=lambda e: e.name)
+def test_device_driver_hit_max_iter_matches_host_loop(require_gpu, monkeypatch, linear_solver):
+  """max_iter reached on the device loop: HIT_MAX_ITER in the status and the last
+  stats entry, and the tau-scaled iterate of the host loop after the same
+  iterations (tests/test_qtqp.py hit-max-iter behaviour)."""
+  a, b, c, p = G.gen_feasible(60, 40, 5, 1)
+  sols = {}
+  for dev in (True, False):
+    monkeypatch.setattr(direct._HipDenseBase, "device_driver", dev)
+    sols[dev] = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=5, p=sp.csc_matrix(p)).solve(
+        linear_solver=linear_solver, max_iter=3, verbose=False, collect_stats=True)
+  for sol in sols.values():
+    assert sol.status == qtqp_amd.SolutionStatus.HIT_MAX_ITER
+    assert len(sol.stats) == 3 and sol.stats[-1]["status"] == sol.status
+  for k in "xys":
+    assert _rel(getattr(sols[True], k), getattr(sols[False], k)) < 1e-8, k
+
+
+def test_device_driver_presolve(require_gpu):
+  """Rows with b >= 1e20 are dropped before the device loop and restored after
+  it: the infeasibility certificate and the solved case's slack
+  (tests/test_qtqp.py:561-596)."""
+  rng = np.random.default_rng(1842)
+  a, b, c, p = G.gen_infeasible(30, 20, 4, 1842)
+  a_full = np.vstack([a, rng.normal(size=(3, 20))])
+  b_full = np.concatenate([b, np.full(3, 1e21)])
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a_full), b=b_full, c=c, z=4, p=sp.csc_matrix(p)).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE, verbose=False)
+  assert sol.status == qtqp_amd.SolutionStatus.INFEASIBLE
+  G.check_infeasible(sol.x, sol.y, sol.s, a_full, b_full, 4)
+  a, b, c, p = G.gen_feasible(30, 20, 4, 5)
+  extra = rng.normal(size=(2, 20))
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(np.vstack([a, extra])), b=np.concatenate([b, [1e20, np.inf]]),
+                      c=c, z=4, p=sp.csc_matrix(p)).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_DENSE, verbose=False)
+  assert sol.status == qtqp_amd.SolutionStatus.SOLVED
+  np.testing.assert_allclose(sol.y[-2:], 0.0)
+  np.testing.assert_allclose(sol.s[-2:][0], 1e20 - extra[0] @ sol.x)
