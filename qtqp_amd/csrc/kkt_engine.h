@@ -270,8 +270,8 @@ __global__ void k_rowfin(const double* __restrict__ rpart, int nb, int rows,
 
 // K_reg x (or a residual):  kx = [P_off x_x + d_x*x_x + A'x_y ; A x_x + d_y*x_y]
 // from the row partials of P_off x_x (rpp) and A x_x (rpa, ncb tiles each)
-// and aty = A'x_y.  If rhs: out = rhs - kx (+ corr*x if corr), and *norm gets
-// max |out|.
+// and aty = A'x_y.  If rhs: out = rhs - kx (+ corr*x if corr).  With norm:
+// norm[0] gets max |out| and norm[1] becomes nonzero if x has a NaN.
 __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* __restrict__ x,
                                                    const double* __restrict__ diag,
                                                    const double* __restrict__ rpp,
@@ -300,6 +300,7 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
   if (norm) {
     double a = live ? fabs(v) : 0.0;
     if (live && v != v) a = INFINITY;  // fmax drops NaN; report it as inf
+    if (live && x[idx] != x[idx]) atomicOr(&norm[1], 1ull);
     a = block_max(a, red);
     if (threadIdx.x == 0) atomic_max_nonneg(norm, a);
   }
@@ -509,11 +510,6 @@ __global__ void k_axpy(int64_t count, double alpha, const double* __restrict__ x
   if (i < count) y[i] += alpha * x[i];
 }
 
-__global__ void k_nan_count(int64_t count, const double* __restrict__ x, int* flag) {
-  const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
-  if (i < count && x[i] != x[i]) atomicOr(flag, 1);
-}
-
 // krhs = [r_x; -r_y] (direct.py:245-248) and ||krhs||_inf into *norm (a NaN
 // entry is reported as inf, since fmax drops NaN).
 __global__ __launch_bounds__(WG) void k_negate_y(int64_t count, int64_t n,
@@ -603,7 +599,6 @@ struct qtqp_kkt {
   unsigned* sync = nullptr;  // tickets and block flags of the triangular sweeps
   unsigned trsv_epoch = 0;   // completed sweep launches
   unsigned long long* dnorm = nullptr;
-  int* dflag = nullptr;
   // state
   bool have_diag = false, have_factor = false, force64 = false;
   int fprec = 64;
@@ -660,27 +655,30 @@ int tile_pass(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double*
 inline unsigned colfin_blocks(int64_t ncol) { return (unsigned)((ncol + 63) / 64); }
 
 // out = K_reg x, or out = rhs - K_reg x + corr*x with ||out||_inf into *norm_host.
-//  With rhs_norm_host, also reads back dnorm[1] (set by k_negate_y).
+//  With norm_host the device words dnorm = {||out||_inf, NaN-in-x flag,
+//  ||rhs||_inf (set by k_negate_y)} are read back: the flag into *nan_host and
+//  the rhs norm into *rhs_norm_host when given.
 int kmatvec(qtqp_kkt* h, const double* x, const double* rhs, const double* corr, double* out,
-            double* norm_host, double* rhs_norm_host = nullptr) {
+            double* norm_host, double* rhs_norm_host = nullptr, int* nan_host = nullptr) {
   const int64_t n = h->n, m = h->m, N = n + m;
   int rc;
   if ((rc = tile_pass<double>(h, h->A64, m, n, x, h->rpa, x + n, nullptr))) return rc;
   if ((rc = tile_pass<double>(h, h->Poff64, n, n, x, h->rpp, nullptr, nullptr))) return rc;
   hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
                      tiles_r(m), (int)n, nullptr, h->aty);
-  if (norm_host) HIPC(hipMemsetAsync(h->dnorm, 0, sizeof(unsigned long long), h->stream));
+  if (norm_host) HIPC(hipMemsetAsync(h->dnorm, 0, 2 * sizeof(unsigned long long), h->stream));
   hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
                      h->diag, h->rpp, h->rpa, tiles_c(n), h->aty, rhs, corr, out,
                      norm_host ? h->dnorm : nullptr);
   HIPC(hipGetLastError());
   if (norm_host) {
-    unsigned long long bits[2] = {0, 0};
-    const int cnt = rhs_norm_host ? 2 : 1;
+    unsigned long long bits[3] = {0, 0, 0};
+    const int cnt = rhs_norm_host ? 3 : 2;
     HIPC(hipMemcpyAsync(bits, h->dnorm, sizeof(bits[0]) * cnt, hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
     std::memcpy(norm_host, &bits[0], sizeof(double));
-    if (rhs_norm_host) std::memcpy(rhs_norm_host, &bits[1], sizeof(double));
+    if (nan_host) *nan_host = bits[1] != 0;
+    if (rhs_norm_host) std::memcpy(rhs_norm_host, &bits[2], sizeof(double));
   }
   return QTQP_OK;
 }
@@ -772,7 +770,9 @@ int solve_current(qtqp_kkt* h, const double* r, double* out, int accumulate) {
 // abs_target > 0 the FP32 refinement may stop once ||r - K_reg out||_inf is
 // below it: a caller refining against a looser tolerance gains nothing from
 // the last digits.
-int backend_solve(qtqp_kkt* h, const double* r, double* out, double abs_target = 0.0) {
+//  r0_known >= 0 is ||r||_inf when the caller has it (saves a readback).
+int backend_solve(qtqp_kkt* h, const double* r, double* out, double abs_target = 0.0,
+                  double r0_known = -1.0) {
   const int64_t N = h->n + h->m;
   if (h->fprec == 64) return solve_current(h, r, out, 0);
   // FP32 factor: refine in FP64 against the exact K_reg.
@@ -786,7 +786,11 @@ int backend_solve(qtqp_kkt* h, const double* r, double* out, double abs_target =
   int rc;
   double r0 = 0.0, prev = 0.0, cur = 0.0;
   // the residual of x = 0 is r itself: rres = r, r0 = ||r||_inf, racc = 0
-  {
+  if (r0_known >= 0.0) {
+    r0 = r0_known;
+    HIPC(hipMemsetAsync(h->racc, 0, sizeof(double) * N, h->stream));
+    HIPC(hipMemcpyAsync(h->rres, r, sizeof(double) * N, hipMemcpyDeviceToDevice, h->stream));
+  } else {
     HIPC(hipMemsetAsync(h->dnorm, 0, sizeof(unsigned long long), h->stream));
     HIPC(hipMemsetAsync(h->racc, 0, sizeof(double) * N, h->stream));
     hipLaunchKernelGGL(k_copy_norm, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, r, h->rres,
@@ -834,7 +838,7 @@ T* dalloc(int64_t count, hipError_t* err) {
 void release(qtqp_kkt* h) {
   void* bufs[] = {h->A64, h->Poff64, h->A32, h->Poff32, h->diag, h->corr, h->G64, h->G32,
                   h->info, h->vin, h->vout, h->vsol, h->vres, h->rin, h->racc, h->rres,
-                  h->rtmp, h->tx, h->tg, h->part, h->dnorm, h->dflag,
+                  h->rtmp, h->tx, h->tg, h->part, h->dnorm,
                   h->tv, h->linv, h->sync, h->rpa, h->rpp, h->aty};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -920,8 +924,7 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
   ALLOC(rpa, double, m * tiles_c(n));
   ALLOC(rpp, double, n * tiles_c(n));
   ALLOC(aty, double, n);
-  ALLOC(dnorm, unsigned long long, 2);
-  ALLOC(dflag, int, 1);
+  ALLOC(dnorm, unsigned long long, 3);
   if (precision == QTQP_PREC_MIXED) {
     ALLOC(A32, float, m * n);
     ALLOC(Poff32, float, n * n);
@@ -1005,12 +1008,13 @@ int engine_factorize(qtqp_kkt* h) {
 int ir_refine(qtqp_kkt* h, const double* rhs, double* sol, int max_steps, double atol,
               double rtol, qtqp_ir_stats* st) {
   const int64_t n = h->n, N = h->n + h->m;
-  HIPC(hipMemsetAsync(h->dnorm + 1, 0, sizeof(unsigned long long), h->stream));
+  HIPC(hipMemsetAsync(h->dnorm + 2, 0, sizeof(unsigned long long), h->stream));
   hipLaunchKernelGGL(k_negate_y, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, n, rhs, h->vin,
-                     h->dnorm + 1);
+                     h->dnorm + 2);
   HIPC(hipGetLastError());
   double res = 0.0, old = 0.0, rhs_norm = 0.0;
-  int rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res, &rhs_norm);
+  int nanflag = 0;
+  int rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res, &rhs_norm, &nanflag);
   if (rc) return rc;
   const double tol = atol + rtol * rhs_norm;
   int status = 1, solves = 0;
@@ -1018,10 +1022,10 @@ int ir_refine(qtqp_kkt* h, const double* rhs, double* sol, int max_steps, double
     old = res;
     // An inner residual a tenth of the tolerance leaves the next outer
     // residual below it (up to the correction term).
-    if ((rc = backend_solve(h, h->vres, h->vout, 0.1 * tol))) return rc;
+    if ((rc = backend_solve(h, h->vres, h->vout, 0.1 * tol, res))) return rc;
     hipLaunchKernelGGL(k_axpy, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, 1.0, h->vout, sol);
     HIPC(hipGetLastError());
-    if ((rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res))) return rc;
+    if ((rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res, nullptr, &nanflag))) return rc;
     if (res < tol) {
       status = 0;
       break;
@@ -1032,12 +1036,7 @@ int ir_refine(qtqp_kkt* h, const double* rhs, double* sol, int max_steps, double
     }
   }
   if (solves > max_steps) solves = max_steps;
-  HIPC(hipMemsetAsync(h->dflag, 0, sizeof(int), h->stream));
-  hipLaunchKernelGGL(k_nan_count, dim3(blocks_for(N)), dim3(WG), 0, h->stream, N, sol, h->dflag);
-  HIPC(hipGetLastError());
-  int nanflag = 0;
-  HIPC(hipMemcpyAsync(&nanflag, h->dflag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
-  HIPC(hipStreamSynchronize(h->stream));
+  // nanflag comes from the last residual, which was formed from the final sol
   if (st) {
     st->solves = solves;
     st->status = status;
