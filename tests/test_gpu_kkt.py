@@ -326,3 +326,22 @@ def test_device_driver_presolve(require_gpu):
   assert sol.status == qtqp_amd.SolutionStatus.SOLVED
   np.testing.assert_allclose(sol.y[-2:], 0.0)
   np.testing.assert_allclose(sol.s[-2:][0], 1e20 - extra[0] @ sol.x)
+
+
+def test_device_ipm_argument_errors(require_gpu):
+  """The device loop's C ABI rejects bad arguments with QTQP_ERR_ARG (ValueError)."""
+  lib = direct.HipDenseSolver()._lib
+  a, b, c = np.ones((4, 3)), np.ones(4), np.ones(3)
+  with pytest.raises(ValueError):
+    direct.DeviceIpm(lib, 0, a, None, False, b, c, 4, True)   # z must be < m
+  with pytest.raises(ValueError):
+    direct.DeviceIpm(lib, 7, a, None, False, b, c, 1, True)   # unknown precision
+  ipm = direct.DeviceIpm(lib, 0, a, np.eye(3), True, b, c, 1, True)
+  opts = qtqp_amd._native.IpmOpts(atol=1e-7, rtol=1e-8, atol_infeas=1e-8, rtol_infeas=1e-9,
+                                  step_size_scale=0.99, min_static_regularization=1e-8,
+                                  linear_solver_atol=1e-12, linear_solver_rtol=1e-12,
+                                  max_iterative_refinement_steps=0, collect_stats=0)
+  with pytest.raises(ValueError):
+    ipm.iterate(opts)                                            # refinement steps >= 1
+  ipm.free()
+  ipm.free()  # idempotent
