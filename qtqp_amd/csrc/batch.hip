@@ -311,7 +311,7 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
     const int nb = min(NB, n - k0);
     load_diag_block(G, n, k0, nb, S);
     __syncthreads();
-    potrf_inv_lds(S, Li, sinv, nb, &info, k0);
+    potrf_inv_blk(S, Li, sinv, nb, &info, k0);
     store_diag_block(G, n, k0, nb, S);
     // keep L_bb^{-1} for the triangular solves: column-major, then row-major
     T* dst = LiB + (int64_t)(k0 / NB) * 2 * NB * NB;

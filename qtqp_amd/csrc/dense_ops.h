@@ -342,122 +342,148 @@ __device__ inline void load_diag_block(const T* G, int64_t ld, int k0, int nb,
   }
 }
 
-// Reciprocal from the hardware estimate refined by Newton steps (within an
-// ulp or two of 1/x; used where a correctly rounded quotient is not needed).
-__device__ inline double approx_rcp(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = r * (2.0 - x * r);
-  return r * (2.0 - x * r);
-}
-__device__ inline float approx_rcp(float x) {
-  float r = __builtin_amdgcn_rcpf(x);
-  return r * (2.0f - x * r);
-}
-
-// In-LDS Cholesky of the diagonal block fused with the inverse of its factor
-// (all WG threads, one barrier per column).  On entry S holds the block
-// (lower part; padding = identity from load_diag_block).  On return S holds
-// L (lower, L[j][j] = sqrt(pivot), zeros above), X = L^{-1} (lower, zeros
-// above) and sinv[j] = 1 / L[j][j].
-//
-// Thread (i = lane, w = wave) keeps its 16 columns of row i in registers,
-// R[t] = S[i][16w+t] and Y[t] = X[i][16w+t].  Both sweeps run right-looking
-// on unscaled values:
-//   Cholesky step p:  S[i][k] -= S[i][p] S[k][p] / S[p][p]   (p < k <= i)
-//   inverse step j:   Y[i][c] -= S[i][j] / S[j][j] * Y[j][c] (c <= j < i)
-// with Y starting as the identity; row j of L^{-1} is Y[j][:] / L[j][j].
-// Phase p runs Cholesky step p and inverse step p-1: the owners publish
-// column p of S and row p-1 of Y into double-buffered LDS vectors (kept in
-// the first rows of X, which is written only at the end), one barrier, then
-// every thread updates its registers from broadcast reads.  A non-positive
-// or NaN pivot records `kglob + j + 1` in *info (the first such column) and
-// is replaced by 1 so the sweep finishes without NaNs.
+// In-LDS Cholesky of the 64 x 64 diagonal block fused with the inverse of its
+// factor (all workgroup threads).  On entry S holds the block (lower part;
+// padding = identity from load_diag_block).  On return S holds L (lower,
+// L[j][j] = sqrt(pivot), zeros above), X = L^{-1} (lower, zeros above) and
+// sinv[j] = 1 / L[j][j].  Blocked into four 16-column panels, so that it
+// needs about 19 barriers instead of one per column:
+//   1. every wave factors the 16 x 16 diagonal block D of the panel and
+//      inverts its factor in registers (lane = row r, columns 4cg..4cg+3,
+//      cg = lane / 16; column broadcasts by cross-lane shuffles);
+//   2. the panel below, L_p = A_p L_d^{-T}, one 16-row strip per wave on MFMA
+//      16x16x4 (K = 16) with L_d^{-1} read back from X's diagonal block;
+//   3. the trailing tiles, S_ij -= L_i L_j^T, on MFMA.
+// Then the off-diagonal 16 x 16 blocks of X = L^{-1}, block row by block row:
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj.  A non-positive or NaN pivot records
+// `kglob + j + 1` in *info (the first such column) and is replaced by 1.
 template <typename T>
-__device__ inline void potrf_inv_lds(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, int nb,
+__device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, int nb,
                                      int* info, int kglob) {
-  const int tid = threadIdx.x, i = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, kept scalar
-  T R[16], Y[16];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    R[t] = S[i][16 * w + t];
-    Y[t] = T((16 * w + t) == i);
-  }
-  T* col = &X[0][0];     // [2][64]: column p of S below the diagonal, zeros above
-  T* xrow = col + 128;   // [2][64]: row p-1 of Y
-  T* pivs = col + 256;   // [2]: S[p][p]
-  T fprev = T(0);
-  for (int p = 0; p <= NB; ++p) {
-    const int b = (p & 1) * NB;
-    if (p < NB && w == (p >> 4)) {
-      const T v = R[p & 15];
-      col[b + i] = i > p ? v : T(0);
-      if (i == p) pivs[p & 1] = v;
-    }
-    if (p >= 1 && i == p - 1) {
-#pragma unroll
-      for (int t = 0; t < 16; ++t) xrow[b + 16 * w + t] = Y[t];
-    }
-    __syncthreads();
-    // All updates are unconditional: the published column is zero in rows
-    // <= p, so rows i <= p get f = 0 and columns k <= p get cv = 0; row p-1
-    // of Y is zero right of the diagonal, and rows i <= p-1 get fprev = 0.
-    // Entries of R above the diagonal pick up values that are never
-    // published (zeros are) and are dropped at the end.
-    T cv[16], xv[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      cv[t] = col[b + 16 * w + t];
-      xv[t] = xrow[b + 16 * w + t];
-    }
-    const T piv = pivs[p & 1];
-    const T fcur = col[b + i] * approx_rcp(piv > T(0) ? piv : T(1));
-    if (p < NB) {
-#pragma unroll
-      for (int t = 0; t < 16; ++t) R[t] -= fcur * cv[t];
-    }
-    if (p >= 1) {
-#pragma unroll
-      for (int t = 0; t < 16; ++t) Y[t] -= fprev * xv[t];
-    }
-    fprev = fcur;
-  }
-  // Pivots are the final diagonal entries of R (raw): 1/sqrt, breakdowns.
+  using acc_t = typename Mfma<T>::acc_t;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = lane & 15, cg = lane >> 4, kq = lane >> 4;
   __shared__ int first_bad;
   if (tid == 0) first_bad = NB;
-  T pv = T(1);
-  bool own = false;
+  // X above its diagonal 16 x 16 blocks is zero (the diagonal blocks are
+  // written whole below)
+  for (int idx = tid; idx < NB * NB; idx += WG) {
+    const int row = idx >> 6, col = idx & (NB - 1);
+    if ((col >> 4) > (row >> 4)) X[row][col] = T(0);
+  }
+  for (int pb = 0; pb < 4; ++pb) {
+    const int c0 = 16 * pb;
+    T d[4], x[4];
 #pragma unroll
-  for (int t = 0; t < 16; ++t)
-    if (16 * w + t == i) {
-      pv = R[t];
-      own = true;
+    for (int t = 0; t < 4; ++t) {
+      const int k = 4 * cg + t;
+      d[t] = k <= r ? S[c0 + r][c0 + k] : T(0);
+      x[t] = T(k == r);
     }
-  __syncthreads();  // broadcast buffers (inside X) were read for the last time above
-  if (own) {
-    const bool ok = pv > T(0);
-    sinv[i] = ok ? (T)(1.0 / sqrt((double)pv)) : T(1);
-    if (!ok && i < nb) atomicMin(&first_bad, i);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int src = 16 * (j >> 2);  // lanes holding column j (slot j & 3)
+      T piv = __shfl(d[j & 3], j + src, 64);
+      const bool ok = piv > T(0);
+      if (!ok) {
+        piv = T(1);
+        if (w == 0 && lane == 0 && c0 + j < nb) atomicMin(&first_bad, c0 + j);
+      }
+      const T dj = sqrt(piv);
+      const T rj = T(1) / dj;
+      if (cg == (j >> 2)) d[j & 3] = r > j ? d[j & 3] * rj : (r == j ? dj : T(0));
+      const T lrj = __shfl(d[j & 3], r + src, 64);  // L[r][j]
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int k = 4 * cg + t;
+        const T lkj = __shfl(d[j & 3], k + src, 64);  // L[k][j]
+        if (k > j && k <= r) d[t] -= lrj * lkj;
+      }
+      // L^{-1} by forward elimination on the identity: finish row j, then
+      // remove it from the rows below
+      if (r == j) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x[t] *= rj;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const T xjt = __shfl(x[t], j + 16 * cg, 64);  // X[j][4cg+t]
+        if (r > j) x[t] -= lrj * xjt;
+      }
+    }
+    if (w == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        X[c0 + r][c0 + 4 * cg + t] = x[t];
+        if (4 * cg + t == r) sinv[c0 + r] = T(1) / d[t];
+      }
+    }
+    __syncthreads();  // every wave has read D; L_d^{-1} is in X
+    if (w == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) S[c0 + r][c0 + 4 * cg + t] = d[t];
+    }
+    const int nstrip = 3 - pb;
+    if (w < nstrip) {
+      const int rs = c0 + 16 + 16 * w;
+      acc_t acc;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = T(0);
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = Mfma<T>::mac(S[rs + r][c0 + k0 + kq], X[c0 + r][c0 + k0 + kq], acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) S[rs + Mfma<T>::row(lane, q)][c0 + r] = acc[q];
+    }
+    __syncthreads();  // panel strips in S
+    const int ntile = nstrip * (nstrip + 1) / 2;
+    for (int tix = w; tix < ntile; tix += 4) {
+      int si, sj;
+      lower_tile(tix, &si, &sj);
+      const int ri = c0 + 16 + 16 * si, rj = c0 + 16 + 16 * sj;
+      acc_t acc;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = T(0);
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = Mfma<T>::mac(S[ri + r][c0 + k0 + kq], S[rj + r][c0 + k0 + kq], acc);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) S[ri + Mfma<T>::row(lane, q)][rj + r] -= acc[q];
+    }
+    __syncthreads();  // trailing tiles updated
+  }
+  // off-diagonal blocks of X = L^{-1}, block row bi: wave w takes block column w
+  for (int bi = 1; bi < 4; ++bi) {
+    acc_t acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = T(0);
+    if (w < bi) {
+      for (int bk = w; bk < bi; ++bk) {
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4)
+          acc = Mfma<T>::mac(S[16 * bi + r][16 * bk + k0 + kq], X[16 * bk + k0 + kq][16 * w + r],
+                             acc);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[16 * bi + Mfma<T>::row(lane, q)][16 * w + r] = acc[q];
+    }
+    __syncthreads();  // the sums T_ij are in X
+    if (w < bi) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q] = T(0);
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = Mfma<T>::mac(X[16 * bi + r][16 * bi + k0 + kq], X[16 * bi + k0 + kq][16 * w + r], acc);
+    }
+    __syncthreads();  // every read of T_ij is done
+    if (w < bi) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[16 * bi + Mfma<T>::row(lane, q)][16 * w + r] = -acc[q];
+    }
   }
   __syncthreads();
   if (tid == 0 && info && first_bad < nb) atomicCAS(info, 0, kglob + first_bad + 1);
-#pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const int k = 16 * w + t;
-    T sv, xv;
-    if (k < i) {
-      sv = R[t] * sinv[k];
-      xv = Y[t] * sinv[i];
-    } else if (k == i) {
-      sv = R[t] > T(0) ? (T)sqrt((double)R[t]) : T(1);
-      xv = sinv[i];
-    } else {
-      sv = T(0);
-      xv = T(0);
-    }
-    S[i][k] = sv;
-    X[i][k] = xv;
-  }
   __syncthreads();
 }
 

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int 
   const int nb = min(NB, n - k0);
   load_diag_block(G, ld, k0, nb, S);
   __syncthreads();
-  potrf_inv_lds(S, Li, sinv, nb, info, k0);
+  potrf_inv_blk(S, Li, sinv, nb, info, k0);
   store_diag_block(G, ld, k0, nb, S);
   T* dst = linv + (int64_t)(k0 / NB) * 2 * NB * NB;
   for (int idx = threadIdx.x; idx < NB * NB; idx += WG) {
