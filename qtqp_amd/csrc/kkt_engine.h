@@ -52,22 +52,42 @@ __global__ void k_cast(const double* __restrict__ src, T* __restrict__ dst, int6
     dst[i] = (T)src[i];
 }
 
-// G (lower tiles) = P_off + diag(d_x) + A' W A.
-template <typename T>
+// Lower 64 x 64 tile handled by this workgroup.  Workgroups are dispatched to
+// the eight XCDs in turn (blockIdx mod 8), each with its own L2; the grid is
+// padded to a multiple of 8 and XCD x takes the contiguous tile range
+// [x * per, (x+1) * per), whose tiles share rows of the operand.  Returns
+// false for padding workgroups.
+__device__ inline bool xcd_tile(int ntiles, int* ti, int* tj) {
+  const int per = gridDim.x >> 3;
+  const int t = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (t >= ntiles) return false;
+  lower_tile(t, ti, tj);
+  return true;
+}
+inline unsigned xcd_grid(int ntiles) { return (unsigned)(8 * ((ntiles + 7) / 8)); }
+
+// Rows of the operand staged per LDS round by the tile kernels.
+template <typename T> constexpr int kch() { return RankKTile<T>::KCH; }
+
+// G (lower tiles) = P_off + diag(d_x) + A' W A.  VEC: rows of A are 16-byte
+// aligned vectors (n a multiple of 16 / sizeof(T)), fetched as such.
+template <typename T, bool VEC>
 __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
                                                   const T* __restrict__ Poff,
                                                   const double* __restrict__ diag,
                                                   int n, int m, T* __restrict__ G,
-                                                  int64_t ld) {
-  __shared__ T si[KC][NB + 1];
-  __shared__ T sj[KC][NB + 1];
+                                                  int64_t ld, int ntiles) {
+  __shared__ T si[kch<T>()][NB + 1];
+  __shared__ T sj[kch<T>()][NB + 1];
   int ti, tj;
-  lower_tile(blockIdx.x, &ti, &tj);
+  if (!xcd_tile(ntiles, &ti, &tj)) return;
   const int i0 = ti * NB, j0 = tj * NB;
   RankKTile<T> t;
   t.zero();
   const double* dy = diag + n;
-  t.run(A, n, 0, m, i0, j0, n, n, [dy](int k) { return (T)(-1.0 / dy[k]); }, si, sj);
+  auto w = [dy](int k) { return (T)(-1.0 / dy[k]); };
+  if (VEC) t.run_vec(A, n, 0, m, i0, j0, w, &si[0][0], &sj[0][0]);
+  else t.run(A, n, 0, m, i0, j0, n, n, w, si, sj);
   t.for_each(i0, j0, [&](int i, int j, T v) {
     if (i < n && j < n && i >= j) {
       T g = v + Poff[(int64_t)j * n + i];
@@ -122,19 +142,22 @@ __global__ __launch_bounds__(WG) void k_trsm_panel(T* G, int64_t ld, int n, int 
   trsm_mfma_rows<T, 1>(G, ld, n, k0, min(NB, n - k0), k0 + NB + blockIdx.x * NB, Li);
 }
 
-// Trailing update G[t0:, t0:] -= L[t0:, k0:t0] L[t0:, k0:t0]', t0 = k0 + NB.
-template <typename T>
-__global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int k0) {
-  __shared__ T si[KC][NB + 1];
-  __shared__ T sj[KC][NB + 1];
+// Trailing update G[t0:, t0:] -= L[t0:, k0:t0] L[t0:, k0:t0]', t0 = k0 + NB
+// (VEC as for k_gram_syrk, with ld = n).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int k0, int ntiles) {
+  __shared__ T si[kch<T>()][NB + 1];
+  __shared__ T sj[kch<T>()][NB + 1];
   int ti, tj;
-  lower_tile(blockIdx.x, &ti, &tj);
+  if (!xcd_tile(ntiles, &ti, &tj)) return;
   const int t0 = k0 + NB;
   const int i0 = t0 + ti * NB, j0 = t0 + tj * NB;
   RankKTile<T> t;
   t.zero();
   // X[k*ldx + i] = L[i][k0+k] (column k0+k of the column-major factor).
-  t.run(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, n, n, [](int) { return T(1); }, si, sj);
+  auto one = [](int) { return T(1); };
+  if (VEC) t.run_vec(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, one, &si[0][0], &sj[0][0]);
+  else t.run(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, n, n, one, si, sj);
   t.for_each(i0, j0, [&](int i, int j, T v) {
     if (i < n && j < n && i >= j) G[(int64_t)j * ld + i] -= v;
   });
@@ -690,8 +713,15 @@ int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
   const int n = (int)h->n, m = (int)h->m;
   const int nt = (n + NB - 1) / NB;
   HIPC(hipMemsetAsync(h->info, 0, sizeof(int), h->stream));
-  hipLaunchKernelGGL(k_gram_syrk<T>, dim3(nt * (nt + 1) / 2), dim3(WG), 0, h->stream, A, Poff,
-                     h->diag, n, m, G, (int64_t)n);
+  // 16-byte vector fetches need rows of n elements to stay 16-byte aligned
+  const bool vec = (n * (int)sizeof(T)) % 16 == 0;
+  const int ng = nt * (nt + 1) / 2;
+  if (vec)
+    hipLaunchKernelGGL((k_gram_syrk<T, true>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A, Poff,
+                       h->diag, n, m, G, (int64_t)n, ng);
+  else
+    hipLaunchKernelGGL((k_gram_syrk<T, false>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A,
+                       Poff, h->diag, n, m, G, (int64_t)n, ng);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n);
   HIPC(hipGetLastError());
@@ -705,8 +735,13 @@ int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
                          (int64_t)n, n, k0, reinterpret_cast<const T*>(h->linv));
       HIPC(hipGetLastError());
       const int tt = (below + NB - 1) / NB;
-      hipLaunchKernelGGL(k_syrk_trail<T>, dim3(tt * (tt + 1) / 2), dim3(WG), 0, h->stream, G,
-                         (int64_t)n, n, k0);
+      const int nst = tt * (tt + 1) / 2;
+      if (vec)
+        hipLaunchKernelGGL((k_syrk_trail<T, true>), dim3(xcd_grid(nst)), dim3(WG), 0, h->stream,
+                           G, (int64_t)n, n, k0, nst);
+      else
+        hipLaunchKernelGGL((k_syrk_trail<T, false>), dim3(xcd_grid(nst)), dim3(WG), 0, h->stream,
+                           G, (int64_t)n, n, k0, nst);
       HIPC(hipGetLastError());
     }
   }
