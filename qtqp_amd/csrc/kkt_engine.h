@@ -86,7 +86,7 @@ __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
   t.zero();
   const double* dy = diag + n;
   auto w = [dy](int k) { return (T)(-1.0 / dy[k]); };
-  if (VEC) t.run_vec(A, n, 0, m, i0, j0, w, &si[0][0], &sj[0][0]);
+  if constexpr (VEC) t.run_vec(A, n, 0, m, i0, j0, w, &si[0][0], &sj[0][0]);
   else t.run(A, n, 0, m, i0, j0, n, n, w, si, sj);
   t.for_each(i0, j0, [&](int i, int j, T v) {
     if (i < n && j < n && i >= j) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int 
   t.zero();
   // X[k*ldx + i] = L[i][k0+k] (column k0+k of the column-major factor).
   auto one = [](int) { return T(1); };
-  if (VEC) t.run_vec(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, one, &si[0][0], &sj[0][0]);
+  if constexpr (VEC) t.run_vec(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, one, &si[0][0], &sj[0][0]);
   else t.run(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, n, n, one, si, sj);
   t.for_each(i0, j0, [&](int i, int j, T v) {
     if (i < n && j < n && i >= j) G[(int64_t)j * ld + i] -= v;
