@@ -266,10 +266,10 @@ __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, c
     Tile acc;
     acc.zero();
     // rows of A are 16-byte aligned and zero-padded to ld (a multiple of 4)
-    acc.run_vec(A, ld, 0, m, i0, j0, [w](int k) { return (T)w[k]; }, si, sj);
+    acc.run_vec(A, ld, 0, m, i0, j0, w, [](double v) { return (T)v; }, si, sj);
     acc.for_each(i0, j0, [&](int i, int j, T v) {
       if (i < n && j < n && i >= j)
-        G[(int64_t)j * n + i] = v + ((i == j) ? (T)I.dg[i] : P[(int64_t)j * ld + i]);
+        stg(G + (int64_t)j * n + i, v + ((i == j) ? (T)ldg(I.dg + i) : ldg(P + (int64_t)j * ld + i)));
     });
   }
 }
@@ -345,6 +345,31 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
   return r;
 }
 
+// Loads of the triangular sweeps below.  None of them depends on v, so
+// they are issued ahead of use: a block's inverse while the previous block
+// is solved, each 64-row chunk of a block's column panel while the chunk
+// before it is consumed.  Every load is unconditional (indices clamped into
+// the block storage or into L, out-of-range terms skipped where they are
+// used), which lets the waits count them in issue order.
+//
+// Li16: the 16 entries X[(p + 4q) * NB + lane] (q < 16) of a 64 x 64 block X.
+template <typename T>
+__device__ inline void load_blk16(const T* X, T (&d)[16]) {
+  const int lane = threadIdx.x & 63, p = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) d[q] = ldg(X + (p + 4 * q) * NB + lane);
+}
+
+// Panel16: L[col][i] for the columns col = j0 + 16p + cc (cc < 16) of block
+// j0 (clamped below j0 + nb) and the row i (clamped below n) of column-major L.
+template <typename T>
+__device__ inline void load_panel16(const T* L, int n, int j0, int nb, int i, T (&d)[16]) {
+  const int p = threadIdx.x >> 6;
+  const T* col = L + min(i, n - 1);
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) d[cc] = ldg(col + (int64_t)min(j0 + 16 * p + cc, j0 + nb - 1) * n);
+}
+
 // L v = v in place (L column-major n x n, lower) with the stored inverses of
 // its 64 x 64 diagonal blocks.  Per block: v_J = Li_J v_J (the four waves
 // split the 64 columns, LDS reduce), then v_i -= L[i][J] v_J for the rows
@@ -354,29 +379,51 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
 template <typename T>
 __device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* red, double* part) {
   const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int nb = min(NB, n - j0);
-    const T* LiT = LiB + (int64_t)(j0 / NB) * 2 * NB * NB;  // LiT[k*64 + i] = Li[i][k]
+  L = wave_uniform(L);
+  LiB = wave_uniform(LiB);
+  n = wave_uniform(n);
+  const int nblk = (n + NB - 1) / NB;
+  wait_mem();
+  T li[16], pa[16], pb[16];
+  load_blk16(LiB, li);  // LiT[k*64 + i] = Li[i][k] of block 0
+  for (int b = 0; b < nblk; ++b) {
+    const int j0 = b * NB, nb = min(NB, n - j0);
+    const int i0 = j0 + nb, rows = n - i0;
+    load_panel16(L, n, j0, nb, i0 + lane, pa);
+    __builtin_amdgcn_sched_barrier(0);
     double acc = 0.0;
-    if (lane < nb)
-      for (int k = p; k < nb; k += 4) acc += (double)ldg(LiT + k * NB + lane) * (double)v[j0 + k];
+    if (lane < nb) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (p + 4 * q < nb) acc += (double)li[q] * (double)v[j0 + p + 4 * q];
+    }
     red[p * NB + lane] = acc;
     __syncthreads();
     if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
     __syncthreads();
-    const int i0 = j0 + nb, rows = n - i0;
+    load_blk16(LiB + (int64_t)min(b + 1, nblk - 1) * 2 * NB * NB, li);
+    __builtin_amdgcn_sched_barrier(0);
     if (rows <= 0) break;
     const int c0 = 16 * p;
-    for (int ib = 0; ib < rows; ib += 64) {
+    auto chunk = [&](int ib, const T (&pv)[16]) {
       const int i = i0 + ib + lane;
       double a = 0.0;
       if (i < n) {
-        const T* col = L + (int64_t)(j0 + c0) * n + i;
 #pragma unroll
         for (int cc = 0; cc < 16; ++cc)
-          if (c0 + cc < nb) a += (double)ldg(col + (int64_t)cc * n) * (double)v[j0 + c0 + cc];
+          if (c0 + cc < nb) a += (double)pv[cc] * (double)v[j0 + c0 + cc];
       }
       part[p * n + ib + lane] = a;
+    };
+    // Whole pairs of chunks, the second skipped (not its refill) past the
+    // end, so the loop head is reached with the same loads in flight.
+    for (int ib = 0; ib < rows; ib += 128) {
+      load_panel16(L, n, j0, nb, i0 + ib + 64 + lane, pb);
+      __builtin_amdgcn_sched_barrier(0);
+      chunk(ib, pa);
+      load_panel16(L, n, j0, nb, i0 + ib + 128 + lane, pa);
+      __builtin_amdgcn_sched_barrier(0);
+      if (ib + 64 < rows) chunk(ib + 64, pb);
     }
     __syncthreads();
     for (int r = tid; r < rows; r += WG)
@@ -393,20 +440,37 @@ __device__ void trsv_lower_inv(const T* L, int n, const T* LiB, T* v, double* re
 template <typename T>
 __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* red, double* rb) {
   const int tid = threadIdx.x, lane = tid & 63, p = tid >> 6;
+  L = wave_uniform(L);
+  LiB = wave_uniform(LiB);
+  n = wave_uniform(n);
   const int nblk = (n + NB - 1) / NB;
+  wait_mem();
+  T li[16], pa[16], pb[16];
+  // Li[k*64 + c] (row-major) of the last block
+  load_blk16(LiB + (int64_t)(nblk - 1) * 2 * NB * NB + NB * NB, li);
   for (int b = nblk - 1; b >= 0; --b) {
     const int j0 = b * NB, nb = min(NB, n - j0);
-    const T* Li = LiB + (int64_t)b * 2 * NB * NB + NB * NB;  // Li[k*64 + c] (row-major)
     const int c0 = 16 * p;
     double acc[16];
 #pragma unroll
     for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
-    for (int i = j0 + nb + lane; i < n; i += 64) {
-      const double vi = (double)v[i];
-      const T* row = L + (int64_t)(j0 + c0) * n + i;
+    auto chunk = [&](int i, const T (&pv)[16]) {
+      if (i < n) {
+        const double vi = (double)v[i];
 #pragma unroll
-      for (int cc = 0; cc < 16; ++cc)
-        if (c0 + cc < nb) acc[cc] += (double)ldg(row + (int64_t)cc * n) * vi;
+        for (int cc = 0; cc < 16; ++cc)
+          if (c0 + cc < nb) acc[cc] += (double)pv[cc] * vi;
+      }
+    };
+    load_panel16(L, n, j0, nb, j0 + nb + lane, pa);
+    __builtin_amdgcn_sched_barrier(0);
+    for (int i = j0 + nb; i < n; i += 128) {  // whole pairs of chunks, as in the forward sweep
+      load_panel16(L, n, j0, nb, i + 64 + lane, pb);
+      __builtin_amdgcn_sched_barrier(0);
+      chunk(i + lane, pa);
+      load_panel16(L, n, j0, nb, i + 128 + lane, pa);
+      __builtin_amdgcn_sched_barrier(0);
+      chunk(i + 64 + lane, pb);
     }
     reduce_rows<16>(acc);  // lane l: column c0 + (l >> 2)
     if ((lane & 3) == 0) red[c0 + (lane >> 2)] = acc[0];
@@ -414,8 +478,13 @@ __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* re
     if (tid < nb) rb[tid] = (double)v[j0 + tid] - red[tid];
     __syncthreads();
     double a2 = 0.0;
-    if (lane < nb)
-      for (int k = p; k < nb; k += 4) a2 += (double)ldg(Li + k * NB + lane) * rb[k];
+    if (lane < nb) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q)
+        if (p + 4 * q < nb) a2 += (double)li[q] * rb[p + 4 * q];
+    }
+    load_blk16(LiB + (int64_t)max(b - 1, 0) * 2 * NB * NB + NB * NB, li);
+    __builtin_amdgcn_sched_barrier(0);
     red[p * NB + lane] = a2;
     __syncthreads();
     if (tid < nb) v[j0 + tid] = (T)((red[tid] + red[NB + tid]) + (red[2 * NB + tid] + red[3 * NB + tid]));
@@ -520,62 +589,109 @@ __device__ inline void rowpass_absmax_v(const double* __restrict__ M, int nrow, 
   __syncthreads();
 }
 
+// Slack allocated after each instance's FP64 A (kPadRows rows of ld plus
+// kPadCols values), so that the pipelined row pass below can issue its
+// loads unconditionally.
+constexpr int kPadRows = 136;
+constexpr int kPadCols = 1024;
+__host__ __device__ constexpr int64_t a_pad(int ld) { return (int64_t)kPadRows * ld + kPadCols; }
+
+// Rows past the last row of M from which rowpass2<NS, R, D> may start a
+// read: the last rotation starts below nrow, its D + 1 stages load up to
+// 2D stages (of 4R rows) ahead of that, R rows each.  Each read then spans
+// the columns up to 128 NS <= kPadCols from its row start.
+template <int NS, int R, int D>
+constexpr int row_overrun() { return 2 * D * 4 * R + R - 1; }
+
+// Rows a pipelined row pass takes between two flushes of its staged row
+// outputs: a multiple of every stage rotation (D + 1) * 4R used below, and
+// small enough for three doubles per row in the LDS scratch.
+constexpr int kPassChunk = 2016;
+
 // One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
 // zero-padded to even; ld <= 128 NS): for every row i,
-//   (c1_i, c2_i) = rowepi(i, M[i,:] . xs, writer, &c1, &c2)
-// and then colepi(j, sum_i M[i][j] c1_i, sum_i M[i][j] c2_i).  A wavefront
-// takes R rows at a time and keeps them in registers: it forms the dot
-// products, completes them with reduce_rows, evaluates the epilogue (the
-// lanes holding a row's total compute it; `writer` is true for one of
-// them), broadcasts the coefficients and accumulates the column sums from
-// the same registers.  The next R rows are fetched while the current ones
-// are processed.  scratch: >= 8 * ncol doubles of LDS.
-template <int NS, int R, typename RL, typename RE, typename CE>
+//   rowepi(i, M[i,:] . xs, in_i, &c1_i, &c2_i, out_i)   with in_i = rowload(i)
+// then rowflush(i, out_i) and finally
+//   colepi(j, sum_i M[i][j] c1_i, sum_i M[i][j] c2_i).
+// A wavefront takes R rows at a time and keeps them in registers: it forms
+// the dot products, completes them with reduce_rows, evaluates the epilogue
+// (every lane holding a row's total computes it), broadcasts the
+// coefficients and accumulates the column sums from the same registers.
+// Loads are software-pipelined D stages deep: a stage is a wavefront's R
+// rows together with the epilogue inputs of those rows, and the loads of
+// stage k + D are issued before stage k is processed.  The compiler waits
+// for a stage only as long as it can count the memory operations issued
+// after it, so the loop body has none that it cannot count:
+//  - every load is unconditional: rows past nrow and columns past ld are
+//    read from the slack that follows M (a_pad) and masked to zero where
+//    they are used, and the epilogue inputs are read at a clamped row;
+//  - nothing is stored to global memory (a pending store makes every later
+//    wait drain the whole queue): the NOUT row outputs go to LDS and are
+//    written out by rowflush once per kPassChunk rows.
+// scratch: >= max(8 ncol, NOUT kPassChunk) doubles of LDS.
+template <int NS, int R, int D, int NOUT, typename RL, typename RE, typename RF, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
-                                const double* __restrict__ xs, RL rowload, RE rowepi, CE colepi,
-                                double* scratch) {
+                                const double* __restrict__ xs, RL rowload, RE rowepi, RF rowflush,
+                                CE colepi, double* scratch) {
   using V = VecT<double, 2>::type;
+  using In = decltype(rowload(0));
   constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int STEP = 4 * R;  // rows a workgroup takes per stage
+  static_assert(R == 2 || R == 4 || R == 8, "rows per wavefront");
+  static_assert(kPassChunk % ((D + 1) * STEP) == 0, "chunks hold whole stage rotations");
+  M = wave_uniform(M);
+  nrow = wave_uniform(nrow);
+  ld = wave_uniform(ld);
+  const int lane = threadIdx.x & 63, wave = wave_uniform((int)(threadIdx.x >> 6));
+  const int lrow = lane >> (6 - LR);  // row of the stage whose total this lane holds
   double xr[NS][2], c1a[NS][2], c2a[NS][2];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int col = s * 128 + lane * 2 + e;
-      xr[s][e] = col < ncol ? xs[col] : 0.0;
+      // (a flat load still pending in the loop would make every wait drain)
+      xr[s][e] = col < ncol ? ldg(xs + col) : 0.0;
       c1a[s][e] = 0.0;
       c2a[s][e] = 0.0;
     }
-  V cur[R][NS], nxt[R][NS];
-  auto fetch = [&](int i0, V (*dst)[NS]) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int col0 = s * 128 + lane * 2;
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        dst[r][s] = (col0 < ld && i0 + r < nrow)
-                        ? ldg(reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0))
-                        : V(0);
-    }
+  struct Stage {
+    V a[R][NS];
+    In in;
   };
-  fetch(wave * R, cur);
-  for (int i0 = wave * R; i0 < nrow; i0 += 4 * R) {
-    // The epilogue's per-row inputs are loaded before the next rows are
-    // requested, so waiting for them does not wait for the prefetch.
-    const int row = i0 + (lane >> (6 - LR));
-    const auto in = rowload(min(row, nrow - 1));
-    if (i0 + 4 * R < nrow) fetch(i0 + 4 * R, nxt);
+  auto load = [&](int i0, Stage& st) {
+    const double* base = M + (int64_t)i0 * ld + lane * 2;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        st.a[r][s] = ldg(reinterpret_cast<const V*>(base + (int64_t)r * ld + s * 128));
+    st.in = rowload(min(i0 + lrow, nrow - 1));
+  };
+  auto process = [&](int i0, int c0, const Stage& st) {
+    V a[R][NS];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        a[r][s] = (i0 + r < nrow && s * 128 + lane * 2 < ld) ? st.a[r][s] : V(0);
     double d[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       d[r] = 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) d[r] += cur[r][s][0] * xr[s][0] + cur[r][s][1] * xr[s][1];
+      for (int s = 0; s < NS; ++s) d[r] += a[r][s][0] * xr[s][0] + a[r][s][1] * xr[s][1];
     }
     reduce_rows<R>(d);
+    const int row = i0 + lrow;
     double c1 = 0.0, c2 = 0.0;
-    if (row < nrow) rowepi(row, d[0], (lane & ((64 >> LR) - 1)) == 0, in, &c1, &c2);
+    if (row < nrow) {
+      double out[NOUT];
+      rowepi(row, d[0], st.in, &c1, &c2, out);
+      if ((lane & ((64 >> LR) - 1)) == 0)
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) scratch[k * kPassChunk + row - c0] = out[k];
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double k1 = __shfl(c1, r << (6 - LR), 64);
@@ -584,14 +700,41 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          c1a[s][e] += cur[r][s][e] * k1;
-          c2a[s][e] += cur[r][s][e] * k2;
+          c1a[s][e] += a[r][s][e] * k1;
+          c2a[s][e] += a[r][s][e] * k2;
         }
     }
+  };
+  if (nrow > 0) {
+    wait_mem();
+    Stage st[D + 1];
+    const int first = wave * R;
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int k = 0; k < D; ++k) load(first + k * STEP, st[k]);
+    for (int c0 = 0; c0 < nrow; c0 += kPassChunk) {
+      const int cend = min(nrow, c0 + kPassChunk);
+      // A full chunk holds whole rotations, so the next one starts at st[0].
+      // Every stage of a rotation issues its loads, also past cend, so the
+      // loads in flight at the loop head are the same whichever way it was
+      // reached and the wait there covers one stage.
+      for (int i0 = c0 + first; i0 < cend; i0 += (D + 1) * STEP) {
 #pragma unroll
-      for (int s = 0; s < NS; ++s) cur[r][s] = nxt[r][s];
+        for (int u = 0; u <= D; ++u) {
+          const int i = i0 + u * STEP;
+          load(i + D * STEP, st[(u + D) % (D + 1)]);
+          if (i < cend) process(i, c0, st[u]);
+        }
+      }
+      __syncthreads();
+      for (int j = threadIdx.x; j < cend - c0; j += WG) {
+        double out[NOUT];
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) out[k] = scratch[k * kPassChunk + j];
+        rowflush(c0 + j, out);
+      }
+      wait_vmem();
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -623,33 +766,43 @@ struct IrState {
 // whose x-part is xc:  y_c = W (A x_c - r0_y), sol1 = sol0 + (x_c, y_c),
 // with the products updated incrementally.  One pass over A (both A x and
 // A' y, A' u in it), one over P.  Returns ||r1||_inf (NaN reported as inf).
-template <int NS, int R>
+template <int NS, int R, int D>
 __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const IrState S1,
                                        const double* xc, bool init, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
-  const double* dy = I.dg + n;
-  const double* ky = I.krhs + n;
-  const double* cy = I.corr + n;
+  const double* dy = wave_uniform(I.dg + n);
+  const double* ky = wave_uniform(I.krhs + n);
+  const double* cy = wave_uniform(I.corr + n);
+  const double* r0y = wave_uniform(S0.r + n);
+  const double* ax0 = wave_uniform(S0.ax);
+  double* s1y = wave_uniform(S1.sol + n);
+  double* s1ax = wave_uniform(S1.ax);
+  double* s1ry = wave_uniform(S1.r + n);
   double mx = 0.0;
   struct RowIn {
     double sy, r0, ax0, dy, ky, cy;
   };
-  rowpass2<NS, R>(
+  // Row inputs are loaded whether or not `init` uses them (S0 holds valid
+  // arrays in both modes), so no load depends on a branch.
+  const double* sy0 = wave_uniform((init ? S1.sol : S0.sol) + n);
+  static_assert(row_overrun<NS, R, D>() <= kPadRows && 128 * NS <= kPadCols, "slack after A");
+  static_assert(3 * kPassChunk <= kLdsU && 8 * 1024 <= kLdsU, "LDS scratch of the row pass");
+  rowpass2<NS, R, D, 3>(
       I.A, m, n, I.ld, init ? S1.sol : xc,
       [&](int i) {
         RowIn in;
-        in.sy = ldg((init ? S1.sol : S0.sol) + n + i);
-        in.r0 = init ? 0.0 : ldg(S0.r + n + i);
-        in.ax0 = init ? 0.0 : ldg(S0.ax + i);
+        in.sy = ldg(sy0 + i);
+        in.r0 = ldg(r0y + i);
+        in.ax0 = ldg(ax0 + i);
         in.dy = ldg(dy + i);
         in.ky = ldg(ky + i);
         in.cy = ldg(cy + i);
         return in;
       },
-      [&](int i, double t, bool writer, const RowIn& in, double* c1, double* c2) {
+      [&](int i, double t, const RowIn& in, double* c1, double* c2, double* out) {
         double yv, axv, yc = 0.0;
         if (init) {
           yv = in.sy;
@@ -662,13 +815,16 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
         const double ry = in.ky - (axv + in.dy * yv) + in.cy * yv;
         *c1 = init ? yv : yc;
         *c2 = -ry / in.dy;
-        if (writer) {
-          if (!init) stg(S1.sol + n + i, yv);
-          stg(S1.ax + i, axv);
-          stg(S1.r + n + i, ry);
-          const double a = fabs(ry);
-          mx = fmax(mx, (a == a) ? a : INFINITY);
-        }
+        out[0] = yv;
+        out[1] = axv;
+        out[2] = ry;
+      },
+      [&](int i, const double* out) {
+        if (!init) stg(s1y + i, out[0]);
+        stg(s1ax + i, out[1]);
+        stg(s1ry + i, out[2]);
+        const double a = fabs(out[2]);
+        mx = fmax(mx, (a == a) ? a : INFINITY);
       },
       [&](int j, double s1, double s2) {
         S1.aty[j] = init ? s1 : S0.aty[j] + s1;
@@ -701,11 +857,11 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
 
 __device__ inline double ir_pass_any(const Inst& I, const IrState& S0, const IrState& S1,
                                      const double* xc, bool init, Smem& sm) {
-  // rows in flight per wavefront x 128-column slots kept within 16 vectors
-  if (I.ld <= 128) return ir_pass<1, 8>(I, S0, S1, xc, init, sm);
-  if (I.ld <= 256) return ir_pass<2, 8>(I, S0, S1, xc, init, sm);
-  if (I.ld <= 512) return ir_pass<4, 4>(I, S0, S1, xc, init, sm);
-  return ir_pass<8, 2>(I, S0, S1, xc, init, sm);
+  // R rows x NS 128-column slots = 8 vectors per stage; D stages in flight
+  if (I.ld <= 128) return ir_pass<1, 8, 2>(I, S0, S1, xc, init, sm);
+  if (I.ld <= 256) return ir_pass<2, 4, 2>(I, S0, S1, xc, init, sm);
+  if (I.ld <= 512) return ir_pass<4, 2, 2>(I, S0, S1, xc, init, sm);
+  return ir_pass<8, 2, 1>(I, S0, S1, xc, init, sm);
 }
 
 // xc = G^{-1} g with the factor in precision T.
@@ -1469,7 +1625,8 @@ size_t inst_bytes(int n, int m) {
   const int64_t N = n + m, ld = (n + 3) & ~3;
   const int64_t mn = (int64_t)m * ld, nn = (int64_t)n * ld;
   size_t b = 0;
-  b += 2 * align256(8 * mn);       // raw A, A (FP64)
+  b += align256(8 * mn);           // raw A
+  b += align256(8 * (mn + a_pad((int)ld)));  // A (FP64) and its read slack
   b += align256(4 * mn);           // A32
   b += 2 * align256(8 * nn);       // raw P, P
   b += align256(4 * nn);           // P32
@@ -1558,7 +1715,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
     double* araw = ar.take<double>(mn);
     I.Araw = araw;
-    I.A = ar.take<double>(mld);
+    I.A = ar.take<double>(mld + a_pad(I.ld));
     I.A32 = ar.take<float>(mld);
     double* praw = ar.take<double>(nn);
     I.Praw = praw;

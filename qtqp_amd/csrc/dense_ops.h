@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace qk {
 
 constexpr int NB = 64;   // panel width of the blocked factorisation, tile edge
@@ -77,9 +79,17 @@ __device__ inline double block_max(double v, double* red) {
 // (R a power of two up to 32): a transpose-reduce that halves the rows a lane carries
 // at each of log2(R) exchanges, then combines the rest.  Lane l ends with
 // the result of row l >> (6 - log2 R) in d[0].
+// The rows live in a vector value while they are exchanged: on a plain
+// array, the compiler may merge the lane-dependent choice between two
+// elements into one load at a lane-dependent index, which sends the array
+// to scratch memory.
 template <int R, bool MAX = false>
 __device__ inline void reduce_rows(double* d) {
+  typedef double Rows __attribute__((ext_vector_type(R < 2 ? 2 : R)));
   const int lane = threadIdx.x & 63;
+  Rows v;
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = d[q];
   int cnt = R;
 #pragma unroll
   for (int bit = 32; bit >= 64 / R; bit >>= 1) {
@@ -88,20 +98,51 @@ __device__ inline void reduce_rows(double* d) {
 #pragma unroll
     for (int q = 0; q < R / 2; ++q) {
       if (q < half) {
-        const double send = hi ? d[q] : d[q + half];
-        const double keep = hi ? d[q + half] : d[q];
+        const double lo_v = v[q], hi_v = v[q + half];
+        const double send = hi ? lo_v : hi_v;
+        const double keep = hi ? hi_v : lo_v;
         const double got = __shfl_xor(send, bit, 64);
-        d[q] = MAX ? fmax(keep, got) : keep + got;
+        v[q] = MAX ? fmax(keep, got) : keep + got;
       }
     }
     cnt = half;
   }
+  double r0 = v[0];
 #pragma unroll
   for (int bit = 32 / R; bit >= 1; bit >>= 1) {
-    const double got = __shfl_xor(d[0], bit, 64);
-    d[0] = MAX ? fmax(d[0], got) : d[0] + got;
+    const double got = __shfl_xor(r0, bit, 64);
+    r0 = MAX ? fmax(r0, got) : r0 + got;
   }
+  d[0] = r0;
 }
+
+// A value the whole wavefront shares, moved to scalar registers.  The
+// compiler cannot see that the arguments of a non-inlined function, or
+// threadIdx.x / 64, are the same in every lane; once it can, loops over them
+// branch on scalars and addresses keep their uniform part in SGPRs.
+__device__ inline int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <typename T>
+__device__ inline T* wave_uniform(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// Waits until every vector memory operation of this wavefront has completed
+// (s_waitcnt vmcnt(0), the other counters left alone: gfx9 encoding, vmcnt
+// in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).  After stores, this
+// tells the compiler's wait insertion that none are pending any more:
+// loads and stores share the counter without a fixed order between them, so
+// while a store may be pending every load is waited for with vmcnt(0).
+__device__ inline void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Waits for every vector memory and LDS/scalar/flat operation (vmcnt(0)
+// lgkmcnt(0)).  Placed before a software-pipelined loop, it retires what
+// the code before it left in flight (stores, and flat accesses, which count
+// on both counters), so the waits inside the loop count its own loads only.
+__device__ inline void wait_mem() { __builtin_amdgcn_s_waitcnt(0x0070); }
 
 // Loads through a pointer known to address global memory (device buffers
 // reached through tables in memory are otherwise accessed with flat
@@ -145,6 +186,11 @@ __device__ inline void atomic_max_nonneg(unsigned long long* slot, double v) {
 // of work wasted on partial tiles; the batched engine picks MI x MJ per
 // size bucket with its autotuner.
 // ---------------------------------------------------------------------------
+// Weight functor of RankKTileG::run_vec for unweighted rank-k updates.
+struct UnitWeight {
+  __device__ double operator()(double) const { return 1.0; }
+};
+
 template <typename T, int MI = 1, int MJ = 1>
 struct RankKTileG {
   using acc_t = typename Mfma<T>::acc_t;
@@ -219,48 +265,76 @@ struct RankKTileG {
   // run() for a row-major X whose rows are 16-byte aligned and zero-padded
   // to ldx (a multiple of 16 / sizeof(T) elements; columns >= ilim / jlim
   // must read as zero, which the padding provides): every thread fetches
-  // 16-byte vectors, KCH rows per round, one round ahead of the MFMAs.
+  // 16-byte vectors, KCH rows per round, DEPTH rounds ahead of the MFMAs.
+  // Row k of the j operand is scaled by wt(wsrc[k]) (by 1 for UnitWeight).
+  // The loads are unconditional (rows clamped to kend - 1, columns to
+  // ldx - VW, the values masked to zero when they are staged), so the wait
+  // for one round leaves the rounds behind it in flight.
   // si: KCH rows of (TI + 1), sj: KCH rows of (TJ + 1) elements of LDS.
-  template <typename WFun>
+  template <typename WT>
   __device__ inline void run_vec(const T* __restrict__ X, int64_t ldx, int kbeg, int kend, int i0,
-                                 int j0, WFun wfun, T* si, T* sj) {
+                                 int j0, const double* __restrict__ wsrc, WT wt, T* si, T* sj) {
     constexpr int VW = 16 / sizeof(T);
     typedef T VT __attribute__((ext_vector_type(VW)));
     constexpr int VI = TI / VW, VJ = TJ / VW;  // vectors per staged row
     constexpr int RI = WG / VI, RJ = WG / VJ;  // rows per pass of the workgroup
     constexpr int PI = KCH / RI, PJ = KCH / RJ;
+    constexpr bool kUnit = std::is_same<WT, UnitWeight>::value;
+    constexpr int DEPTH = MI * MJ == 1 ? 2 : 1;  // larger tiles keep one round in registers
+    X = wave_uniform(X);
+    kbeg = wave_uniform(kbeg);
+    kend = wave_uniform(kend);
+    i0 = wave_uniform(i0);
+    j0 = wave_uniform(j0);
+    ldx = ((int64_t)wave_uniform((int)(ldx >> 32)) << 32) | (uint32_t)wave_uniform((int)ldx);
+    if constexpr (!kUnit) wsrc = wave_uniform(wsrc);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int jb = 32 * MJ * (wave >> 1), ib = 32 * MI * (wave & 1);
     const int vi = tid % VI, ri = tid / VI, vj = tid % VJ, rj = tid / VJ;
     const bool ci = i0 + vi * VW < ldx, cj = j0 + vj * VW < ldx;
-    VT pi[PI], pj[PJ];
-    auto fetch = [&](int kc) {
+    const int64_t coli = min<int64_t>(i0 + vi * VW, ldx - VW);
+    const int64_t colj = min<int64_t>(j0 + vj * VW, ldx - VW);
+    struct Round {
+      VT a[PI], b[PJ];
+      double w[kUnit ? 1 : PJ];
+    };
+    auto fetch = [&](int kc, Round& rd) {
 #pragma unroll
       for (int q = 0; q < PI; ++q) {
-        const int k = kc + ri + RI * q;
-        pi[q] = (k < kend && ci) ? ldg(reinterpret_cast<const VT*>(X + (int64_t)k * ldx + i0 + vi * VW))
-                                 : VT(0);
+        const int k = min(kc + ri + RI * q, kend - 1);
+        rd.a[q] = ldg(reinterpret_cast<const VT*>(X + (int64_t)k * ldx + coli));
       }
 #pragma unroll
       for (int q = 0; q < PJ; ++q) {
-        const int k = kc + rj + RJ * q;
-        pj[q] = (k < kend && cj)
-                    ? ldg(reinterpret_cast<const VT*>(X + (int64_t)k * ldx + j0 + vj * VW)) * wfun(k)
-                    : VT(0);
+        const int k = min(kc + rj + RJ * q, kend - 1);
+        rd.b[q] = ldg(reinterpret_cast<const VT*>(X + (int64_t)k * ldx + colj));
+        if constexpr (!kUnit) rd.w[q] = ldg(wsrc + k);
       }
     };
-    if (kbeg < kend) fetch(kbeg);
-    for (int kc = kbeg; kc < kend; kc += KCH) {
+    // One round: stage it (when it holds rows below kend), refill its
+    // registers DEPTH rounds ahead, run its MFMAs.  The refill is issued on
+    // every path, so the loads in flight at the loop head are the same
+    // whichever way it was reached and the wait there covers one round.
+    auto step = [&](int kc, Round& rd, bool active) {
+      if (active) {
 #pragma unroll
-      for (int q = 0; q < PI; ++q)
+      for (int q = 0; q < PI; ++q) {
+        const VT v = (kc + ri + RI * q < kend && ci) ? rd.a[q] : VT(0);
 #pragma unroll
-        for (int e = 0; e < VW; ++e) si[(ri + RI * q) * (TI + 1) + vi * VW + e] = pi[q][e];
+        for (int e = 0; e < VW; ++e) si[(ri + RI * q) * (TI + 1) + vi * VW + e] = v[e];
+      }
 #pragma unroll
-      for (int q = 0; q < PJ; ++q)
+      for (int q = 0; q < PJ; ++q) {
+        VT v = rd.b[q];
+        if constexpr (!kUnit) v = v * wt(rd.w[q]);
+        v = (kc + rj + RJ * q < kend && cj) ? v : VT(0);
 #pragma unroll
-        for (int e = 0; e < VW; ++e) sj[(rj + RJ * q) * (TJ + 1) + vj * VW + e] = pj[q][e];
+        for (int e = 0; e < VW; ++e) sj[(rj + RJ * q) * (TJ + 1) + vj * VW + e] = v[e];
+      }
+      }
       __syncthreads();
-      if (kc + KCH < kend) fetch(kc + KCH);
+      fetch(kc + DEPTH * KCH, rd);
+      if (active) {
 #pragma unroll
       for (int kk = 0; kk < KCH; kk += 4) {
         const int kq = kk + (lane >> 4), cl = lane & 15;
@@ -274,7 +348,23 @@ struct RankKTileG {
 #pragma unroll
           for (int b = 0; b < BI; ++b) acc[a][b] = Mfma<T>::mac(av[a], bv[b], acc[a][b]);
       }
+      }
       __syncthreads();
+    };
+    if (kbeg >= kend) return;
+    wait_mem();
+    Round rd[DEPTH];
+    // Rounds are consumed in the order they are fetched; the scheduling
+    // barriers keep their loads in that order, so a wait for the older one
+    // can leave the younger in flight.
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) {
+      fetch(kbeg + d * KCH, rd[d]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int kc = kbeg; kc < kend; kc += DEPTH * KCH) {
+#pragma unroll
+      for (int d = 0; d < DEPTH; ++d) step(kc + d * KCH, rd[d], kc + d * KCH < kend);
     }
   }
 

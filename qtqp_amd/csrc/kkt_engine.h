@@ -85,9 +85,11 @@ __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
   RankKTile<T> t;
   t.zero();
   const double* dy = diag + n;
-  auto w = [dy](int k) { return (T)(-1.0 / dy[k]); };
-  if constexpr (VEC) t.run_vec(A, n, 0, m, i0, j0, w, &si[0][0], &sj[0][0]);
-  else t.run(A, n, 0, m, i0, j0, n, n, w, si, sj);
+  if constexpr (VEC) {
+    t.run_vec(A, n, 0, m, i0, j0, dy, [](double d) { return (T)(-1.0 / d); }, &si[0][0], &sj[0][0]);
+  } else {
+    t.run(A, n, 0, m, i0, j0, n, n, [dy](int k) { return (T)(-1.0 / dy[k]); }, si, sj);
+  }
   t.for_each(i0, j0, [&](int i, int j, T v) {
     if (i < n && j < n && i >= j) {
       T g = v + Poff[(int64_t)j * n + i];
@@ -155,9 +157,11 @@ __global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int 
   RankKTile<T> t;
   t.zero();
   // X[k*ldx + i] = L[i][k0+k] (column k0+k of the column-major factor).
-  auto one = [](int) { return T(1); };
-  if constexpr (VEC) t.run_vec(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, one, &si[0][0], &sj[0][0]);
-  else t.run(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, n, n, one, si, sj);
+  if constexpr (VEC) {
+    t.run_vec(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, nullptr, UnitWeight(), &si[0][0], &sj[0][0]);
+  } else {
+    t.run(G + (int64_t)k0 * ld, ld, 0, NB, i0, j0, n, n, [](int) { return T(1); }, si, sj);
+  }
   t.for_each(i0, j0, [&](int i, int j, T v) {
     if (i < n && j < n && i >= j) G[(int64_t)j * ld + i] -= v;
   });
