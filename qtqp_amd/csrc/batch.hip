@@ -125,111 +125,10 @@ __device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
 // with a barrier so results are visible to the whole workgroup).
 // -------------------------------------------------------------------------
 
-constexpr int RPW = 8;     // rows in flight per wavefront in a row pass
 
 template <typename T, int W> struct VecT;
 template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
-
-// Completes 8 per-lane partial sums across the wavefront with 10 exchanges
-// (a transpose-reduce: each step halves the rows a lane still carries).  On
-// return every lane holds in d[0] the total of row
-// 4*((l>>5)&1) + 2*((l>>4)&1) + ((l>>3)&1).
-__device__ inline void reduce8(double* d, int lane) {
-  const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const double send = h32 ? d[q] : d[q + 4];
-    const double keep = h32 ? d[q + 4] : d[q];
-    d[q] = keep + __shfl_xor(send, 32, 64);
-  }
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const double send = h16 ? d[q] : d[q + 2];
-    const double keep = h16 ? d[q + 2] : d[q];
-    d[q] = keep + __shfl_xor(send, 16, 64);
-  }
-  {
-    const double send = h8 ? d[0] : d[1];
-    const double keep = h8 ? d[1] : d[0];
-    d[0] = keep + __shfl_xor(send, 8, 64);
-  }
-  d[0] += __shfl_xor(d[0], 4, 64);
-  d[0] += __shfl_xor(d[0], 2, 64);
-  d[0] += __shfl_xor(d[0], 1, 64);
-}
-
-// One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
-// a multiple of W with zero padding; ld <= 1024), reading M once with
-// W-wide vector loads:
-//   HX: rowepi(i, sum_j M[i][j] xs[j])   (one lane per row)
-//   HU: colepi(j, sum_i M[i][j] u[i])    (one thread per j)
-// Wavefront w takes RPW = 8 consecutive rows at a time (rows w*8 + 32k),
-// lane l the columns s*64W + lW .. +W; column partials of the four
-// wavefronts meet in `scratch` (>= 4*ncol doubles of LDS).  Deterministic.
-template <typename T, int W, bool HX, bool HU, typename RE, typename CE>
-__device__ inline void rowpass(const T* __restrict__ M, int nrow, int ncol, int ld,
-                               const double* __restrict__ xs, const double* __restrict__ u,
-                               RE rowepi, CE colepi, double* scratch) {
-  using V = typename VecT<T, W>::type;
-  constexpr int NS = 1024 / (64 * W);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nslot = (ld + 64 * W - 1) / (64 * W);
-  double xr[NS][W], ca[NS][W];
-#pragma unroll
-  for (int s = 0; s < NS; ++s)
-#pragma unroll
-    for (int e = 0; e < W; ++e) {
-      const int col = s * 64 * W + lane * W + e;
-      xr[s][e] = (HX && s < nslot && col < ncol) ? xs[col] : 0.0;
-      ca[s][e] = 0.0;
-    }
-  for (int i0 = wave * RPW; i0 < nrow; i0 += 4 * RPW) {
-    double d[RPW], uu[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-      d[r] = 0.0;
-      uu[r] = (HU && i0 + r < nrow) ? u[i0 + r] : 0.0;
-    }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int col0 = s * 64 * W + lane * W;
-      if (s < nslot && col0 < ld) {
-        V a[RPW];
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-          a[r] = (i0 + r < nrow) ? ldg(reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0))
-                                 : V(0);
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-#pragma unroll
-          for (int e = 0; e < W; ++e) {
-            const double av = (double)a[r][e];
-            if (HX) d[r] += av * xr[s][e];
-            if (HU) ca[s][e] += av * uu[r];
-          }
-      }
-    }
-    if (HX) {
-      reduce8(d, lane);
-      const int row = i0 + 4 * ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 3) & 1);
-      if ((lane & 7) == 0 && row < nrow) rowepi(row, d[0]);
-    }
-  }
-  if (HU) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-#pragma unroll
-      for (int e = 0; e < W; ++e) {
-        const int col = s * 64 * W + lane * W + e;
-        if (s < nslot && col < ncol) scratch[wave * ncol + col] = ca[s][e];
-      }
-    __syncthreads();
-    for (int j = threadIdx.x; j < ncol; j += WG)
-      colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]));
-  }
-  __syncthreads();
-}
 
 template <typename T> struct VW;
 template <> struct VW<double> { static constexpr int value = 2; };
@@ -589,19 +488,19 @@ __device__ inline void rowpass_absmax_v(const double* __restrict__ M, int nrow, 
   __syncthreads();
 }
 
-// Slack allocated after each instance's FP64 A (kPadRows rows of ld plus
-// kPadCols values), so that the pipelined row pass below can issue its
+// Slack allocated after each instance's FP64 A and P (kPadRows rows of ld
+// plus kPadCols values), so that the pipelined row pass below can issue its
 // loads unconditionally.
-constexpr int kPadRows = 136;
+constexpr int kPadRows = 16;
 constexpr int kPadCols = 1024;
 __host__ __device__ constexpr int64_t a_pad(int ld) { return (int64_t)kPadRows * ld + kPadCols; }
 
 // Rows past the last row of M from which rowpass2<NS, R, D> may start a
-// read: the last rotation starts below nrow, its D + 1 stages load up to
-// 2D stages (of 4R rows) ahead of that, R rows each.  Each read then spans
-// the columns up to 128 NS <= kPadCols from its row start.
+// read: a stage starts at row nrow - 1 at the latest and reads R rows.
+// Each read then spans the columns up to 128 NS <= kPadCols from its row
+// start.
 template <int NS, int R, int D>
-constexpr int row_overrun() { return 2 * D * 4 * R + R - 1; }
+constexpr int row_overrun() { return R - 1; }
 
 // Rows a pipelined row pass takes between two flushes of its staged row
 // outputs: a multiple of every stage rotation (D + 1) * 4R used below, and
@@ -622,14 +521,18 @@ constexpr int kPassChunk = 2016;
 // stage k + D are issued before stage k is processed.  The compiler waits
 // for a stage only as long as it can count the memory operations issued
 // after it, so the loop body has none that it cannot count:
-//  - every load is unconditional: rows past nrow and columns past ld are
-//    read from the slack that follows M (a_pad) and masked to zero where
-//    they are used, and the epilogue inputs are read at a clamped row;
+//  - every load is unconditional: a stage starts at row min(i0, nrow - 1),
+//    rows past nrow and columns past ld are read from the slack that
+//    follows M (a_pad) and masked to zero where they are used, and the
+//    epilogue inputs are read at a clamped row;
 //  - nothing is stored to global memory (a pending store makes every later
 //    wait drain the whole queue): the NOUT row outputs go to LDS and are
 //    written out by rowflush once per kPassChunk rows.
+// COLS (0, 1 or 2) column sums are formed; with COLS = 1, c2 is ignored and
+// colepi receives 0 for it, with COLS = 0 colepi is not called.
 // scratch: >= max(8 ncol, NOUT kPassChunk) doubles of LDS.
-template <int NS, int R, int D, int NOUT, typename RL, typename RE, typename RF, typename CE>
+template <int NS, int R, int D, int NOUT, int COLS, typename RL, typename RE, typename RF,
+          typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
                                 const double* __restrict__ xs, RL rowload, RE rowepi, RF rowflush,
                                 CE colepi, double* scratch) {
@@ -639,6 +542,7 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
   constexpr int STEP = 4 * R;  // rows a workgroup takes per stage
   static_assert(R == 2 || R == 4 || R == 8, "rows per wavefront");
   static_assert(kPassChunk % ((D + 1) * STEP) == 0, "chunks hold whole stage rotations");
+  static_assert(row_overrun<NS, R, D>() <= kPadRows && 128 * NS <= kPadCols, "slack after M");
   M = wave_uniform(M);
   nrow = wave_uniform(nrow);
   ld = wave_uniform(ld);
@@ -660,7 +564,9 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
     In in;
   };
   auto load = [&](int i0, Stage& st) {
-    const double* base = M + (int64_t)i0 * ld + lane * 2;
+    // A stage past the last row re-reads the rows from nrow - 1 on: they are
+    // in cache, where fresh slack would cost HBM traffic.
+    const double* base = M + (int64_t)min(i0, nrow - 1) * ld + lane * 2;
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -692,17 +598,19 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
 #pragma unroll
         for (int k = 0; k < NOUT; ++k) scratch[k * kPassChunk + row - c0] = out[k];
     }
+    if constexpr (COLS > 0) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double k1 = __shfl(c1, r << (6 - LR), 64);
-      const double k2 = __shfl(c2, r << (6 - LR), 64);
+      for (int r = 0; r < R; ++r) {
+        const double k1 = __shfl(c1, r << (6 - LR), 64);
+        const double k2 = COLS > 1 ? __shfl(c2, r << (6 - LR), 64) : 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
+        for (int s = 0; s < NS; ++s)
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          c1a[s][e] += a[r][s][e] * k1;
-          c2a[s][e] += a[r][s][e] * k2;
-        }
+          for (int e = 0; e < 2; ++e) {
+            c1a[s][e] += a[r][s][e] * k1;
+            if (COLS > 1) c2a[s][e] += a[r][s][e] * k2;
+          }
+      }
     }
   };
   if (nrow > 0) {
@@ -736,22 +644,41 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       __syncthreads();
     }
   }
+  if constexpr (COLS > 0) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int col = s * 128 + lane * 2 + e;
-      if (col < ncol) {
-        scratch[wave * ncol + col] = c1a[s][e];
-        scratch[(4 + wave) * ncol + col] = c2a[s][e];
+      for (int e = 0; e < 2; ++e) {
+        const int col = s * 128 + lane * 2 + e;
+        if (col < ncol) {
+          scratch[wave * ncol + col] = c1a[s][e];
+          if (COLS > 1) scratch[(4 + wave) * ncol + col] = c2a[s][e];
+        }
       }
-    }
-  __syncthreads();
-  for (int j = threadIdx.x; j < ncol; j += WG)
-    colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]),
-           (scratch[4 * ncol + j] + scratch[5 * ncol + j]) +
-               (scratch[6 * ncol + j] + scratch[7 * ncol + j]));
-  __syncthreads();
+    __syncthreads();
+    for (int j = threadIdx.x; j < ncol; j += WG)
+      colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]),
+             COLS > 1 ? (scratch[4 * ncol + j] + scratch[5 * ncol + j]) +
+                            (scratch[6 * ncol + j] + scratch[7 * ncol + j])
+                      : 0.0);
+    __syncthreads();
+  }
+}
+
+// Row-pass shapes by leading dimension: R rows x NS 128-column slots = 8
+// vectors per stage, D stages in flight (one for the widest rows, whose
+// column sums take the registers of the second).
+template <int NS_, int R_, int D_>
+struct PassShape {
+  static constexpr int NS = NS_, R = R_, D = D_;
+};
+
+template <typename F>
+__device__ inline auto with_pass_shape(int ld, F f) {
+  if (ld <= 128) return f(PassShape<1, 8, 2>());
+  if (ld <= 256) return f(PassShape<2, 4, 2>());
+  if (ld <= 512) return f(PassShape<4, 2, 2>());
+  return f(PassShape<8, 2, 1>());
 }
 
 // State of a refined solve for the current iterate sol: the products
@@ -788,9 +715,8 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   // Row inputs are loaded whether or not `init` uses them (S0 holds valid
   // arrays in both modes), so no load depends on a branch.
   const double* sy0 = wave_uniform((init ? S1.sol : S0.sol) + n);
-  static_assert(row_overrun<NS, R, D>() <= kPadRows && 128 * NS <= kPadCols, "slack after A");
   static_assert(3 * kPassChunk <= kLdsU && 8 * 1024 <= kLdsU, "LDS scratch of the row pass");
-  rowpass2<NS, R, D, 3>(
+  rowpass2<NS, R, D, 3, 2>(
       I.A, m, n, I.ld, init ? S1.sol : xc,
       [&](int i) {
         RowIn in;
@@ -832,13 +758,30 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
       },
       scratch);
   if (I.has_p) {
-    rowpass<double, 2, true, false>(
-        I.P, n, n, I.ld, init ? S1.sol : xc, nullptr,
-        [&](int j, double t) {
-          const double v = t - I.P[(int64_t)j * I.ld + j] * (init ? S1.sol[j] : xc[j]);
-          S1.px[j] = init ? v : S0.px[j] + v;
+    // P_off u with u = sol_x (init) or x_c, added to P_off sol0_x otherwise.
+    struct PIn {
+      double u, px0, pjj;
+    };
+    const double* u = wave_uniform(init ? S1.sol : xc);
+    const double* px0 = wave_uniform(S0.px);
+    const double* P = wave_uniform(I.P);
+    const int ld = I.ld;
+    double* s1px = wave_uniform(S1.px);
+    rowpass2<NS, R, D, 1, 0>(
+        P, n, n, ld, u,
+        [&](int j) {
+          PIn in;
+          in.u = ldg(u + j);
+          in.px0 = ldg(px0 + j);
+          in.pjj = ldg(P + (int64_t)j * ld + j);
+          return in;
         },
-        [](int, double) {}, scratch);
+        [&](int j, double t, const PIn& in, double* c1, double* c2, double* out) {
+          const double v = t - in.pjj * in.u;
+          out[0] = init ? v : in.px0 + v;
+        },
+        [&](int j, const double* out) { stg(s1px + j, out[0]); },
+        [](int, double, double) {}, scratch);
   } else {
     for (int j = threadIdx.x; j < n; j += WG) S1.px[j] = 0.0;
     __syncthreads();
@@ -1000,9 +943,15 @@ __device__ double max_step(const double* v, const double* dv_a, const double* dv
 // P u for the n-vector u into out (has_p only).
 __device__ inline void pmul(const Inst& I, const double* u, double* out, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
-  rowpass<double, 2, true, false>(I.P, I.n, I.n, I.ld, u, nullptr,
-                                  [&](int j, double t) { out[j] = t; }, [](int, double) {},
-                                  reinterpret_cast<double*>(sm.u));
+  out = wave_uniform(out);
+  with_pass_shape(I.ld, [&](auto sh) {
+    using S = decltype(sh);
+    rowpass2<S::NS, S::R, S::D, 1, 0>(
+        I.P, I.n, I.n, I.ld, u, [](int) { return 0; },
+        [](int, double t, int, double*, double*, double* o) { o[0] = t; },
+        [&](int j, const double* o) { stg(out + j, o[0]); }, [](int, double, double) {},
+        reinterpret_cast<double*>(sm.u));
+  });
 }
 
 // Quadratic tau (__init__.py:759-817); returns false where it raises.
@@ -1235,15 +1184,33 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   const double inv_tau = 1.0 / fmax(tau, kEps);
   double* ax = I.um;  // m
   double* aty = I.un; // n
-  rowpass<double, 2, true, true>(I.A, m, n, I.ld, I.x, I.y,
-                                 [&](int i, double t) { ax[i] = t / I.d[i]; },
-                                 [&](int j, double t) { aty[j] = t / I.e[j]; },
-                                 reinterpret_cast<double*>(sm.u));
+  {
+    struct TIn {
+      double y, d;
+    };
+    const double* yv = wave_uniform(I.y);
+    const double* dv = wave_uniform(I.d);
+    double* axw = wave_uniform(ax);
+    with_pass_shape(I.ld, [&](auto sh) {
+      using S = decltype(sh);
+      rowpass2<S::NS, S::R, S::D, 1, 1>(
+          I.A, m, n, I.ld, I.x,
+          [&](int i) {
+            TIn in;
+            in.y = ldg(yv + i);
+            in.d = ldg(dv + i);
+            return in;
+          },
+          [](int, double t, const TIn& in, double* c1, double*, double* o) {
+            *c1 = in.y;
+            o[0] = t / in.d;
+          },
+          [&](int i, const double* o) { stg(axw + i, o[0]); },
+          [&](int j, double s1, double) { aty[j] = s1 / I.e[j]; }, reinterpret_cast<double*>(sm.u));
+    });
+  }
   double* px = I.iacc;  // n
-  if (I.has_p)
-    rowpass<double, 2, true, false>(I.P, n, n, I.ld, I.x, nullptr,
-                                    [&](int j, double t) { px[j] = t; }, [](int, double) {},
-                                    reinterpret_cast<double*>(sm.u));
+  if (I.has_p) pmul(I, I.x, px, sm);
   double l_xpx = 0.0, l_ctx = 0.0, l_px = 0.0, l_aty = 0.0, l_dres = 0.0, l_nx = 0.0;
   for (int j = tid; j < n; j += WG) {
     const double pxj = I.has_p ? px[j] : 0.0;
@@ -1628,7 +1595,7 @@ size_t inst_bytes(int n, int m) {
   b += align256(8 * mn);           // raw A
   b += align256(8 * (mn + a_pad((int)ld)));  // A (FP64) and its read slack
   b += align256(4 * mn);           // A32
-  b += 2 * align256(8 * nn);       // raw P, P
+  b += align256(8 * nn) + align256(8 * (nn + a_pad((int)ld)));  // raw P, P and its read slack
   b += align256(4 * nn);           // P32
   b += align256(8 * nn);           // G
   b += align256(8 * 2 * 64 * 64 * ((n + 63) / 64));  // diagonal-block inverses
@@ -1719,7 +1686,7 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.A32 = ar.take<float>(mld);
     double* praw = ar.take<double>(nn);
     I.Praw = praw;
-    I.P = ar.take<double>(nld);
+    I.P = ar.take<double>(nld + a_pad(I.ld));
     I.P32 = ar.take<float>(nld);
     I.G = ar.take<double>(nn);
     I.LiB = ar.take<double>((int64_t)2 * NB * NB * ((nq + NB - 1) / NB));
