@@ -863,6 +863,7 @@ struct IrResult {
 // reference rule exactly.  rhs may alias I.r (it is consumed into krhs
 // first).
 constexpr double kPoorContraction = 3e-2;
+constexpr int kF32StepBudget = 3;  // refinement steps per IPM iteration kept on an FP32 factor
 
 __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm,
                                           double* sol, const Opts o, Smem& sm_in, Counters* cnt) {
@@ -1177,6 +1178,15 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     I.s[i] = sv;
   }
   tau = fmax(tau + step * (tau_c - tau), 1e-30);
+  // An FP32 factor whose three solves took more than kF32StepBudget
+  // refinement steps costs more than an FP64 one: every extra step streams A
+  // in FP64 once more, more than the FP64 Gram product adds.  Such a QP
+  // factors in FP64 from its next iteration on.
+  if (tid == 0 && !I.si[I_FORCE64] && irq.solves + irp.solves + irc.solves > kF32StepBudget) {
+    I.si[I_FORCE64] = 1;
+    I.si[I_FALLBACK] += 1;
+    atomicAdd(&cnt->fallbacks, 1ull);
+  }
   __syncthreads();
 
   // termination on the unscaled problem (__init__.py:888-965):
