@@ -629,6 +629,8 @@ struct qtqp_kkt {
   // state
   bool have_diag = false, have_factor = false, force64 = false;
   int fprec = 64;
+  // backend solves and their FP32 refinement steps on the current FP32 factor
+  int64_t f32_solves = 0, f32_steps = 0;
   qtqp_kkt_info info_acc{};
 };
 
@@ -845,9 +847,13 @@ int backend_solve(qtqp_kkt* h, const double* r, double* out, double abs_target =
     return QTQP_OK;
   }
   prev = r0;
+  h->f32_solves++;
   for (int step = 0; step < kMaxInner; ++step) {
     if ((rc = solve_current(h, h->rres, h->racc, 1))) return rc;
-    if (h->fprec == 32) h->info_acc.inner_steps++;
+    if (h->fprec == 32) {
+      h->info_acc.inner_steps++;
+      h->f32_steps++;
+    }
     if ((rc = kmatvec(h, h->racc, r, nullptr, h->rres, &cur))) return rc;
     if (!(cur == cur)) break;
     if (cur <= kTarget * r0 || cur <= abs_target) break;
@@ -1014,13 +1020,26 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
   return QTQP_OK;
 }
 
+// Mean FP32 refinement steps per backend solve above which a mixed handle
+// factors in FP64 from the next factorisation on: an FP64 factor solves in
+// one pair of triangular sweeps, and the extra sweeps and FP64 residuals of
+// the FP32 refinement then cost more than the FP32 factor saves.
+constexpr int64_t kF32StepsPerSolve = 2;
+
 // LinearSolver.factorize (solvers_dense.py:99-116) on the diagonal in
 // h->diag: FP32 factor for a mixed handle (FP64 refactor on breakdown, and
-// FP64 from then on), FP64 otherwise.
+// FP64 from then on, as after a factor whose solves refined too slowly),
+// FP64 otherwise.
 int engine_factorize(qtqp_kkt* h) {
   HIPC(hipEventRecord(h->ev0, h->stream));
   int rc = QTQP_OK;
   h->info_acc.factorizations++;
+  if (h->precision == QTQP_PREC_MIXED && !h->force64 && h->fprec == 32 &&
+      h->f32_steps > kF32StepsPerSolve * h->f32_solves) {
+    h->info_acc.fp64_fallbacks++;
+    h->force64 = true;
+  }
+  h->f32_solves = h->f32_steps = 0;
   if (h->precision == QTQP_PREC_MIXED && !h->force64) {
     int brk = 0;
     rc = factor_prec<float>(h, h->A32, h->Poff32, h->G32, &brk);
