@@ -130,6 +130,26 @@ def test_tile_shapes_give_identical_results_and_autotune(require_gpu):
   bq.free()
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_row_pass_chunks_and_slot_boundaries(require_gpu, precision):
+  """Row counts past the 2016-row staging chunk (one, two and three chunks, a
+  partial last stage) and leading dimensions on both sides of the 128/256/512
+  column-slot boundaries of the pipelined row pass, against the oracle."""
+  qps = [problems.random_dense_qp(128, 2017, 10, seed=21),   # ld 128, 2 chunks
+         problems.random_dense_qp(129, 2100, 20, seed=22),   # ld 132 (2 slots)
+         problems.random_dense_qp(256, 4100, 50, seed=23),   # ld 256, 3 chunks
+         problems.random_dense_qp(257, 300, 40, seed=24),    # ld 260 (4 slots)
+         problems.random_dense_qp(513, 700, 30, seed=25),    # ld 516 (8 slots)
+         problems.random_dense_qp(5, 33, 3, seed=26)]        # ld 8, 33 rows
+  bq = batch.BatchQP(qps)
+  res = bq.solve(precision=precision)
+  bq.free()
+  for qp, sol in zip(qps, res.solutions):
+    ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+    _compare(sol, ref, 1e-6)
+    assert abs(sol.iters - ref["iters"]) <= 1
+
+
 def test_maximum_size_instance_in_batch(require_gpu):
   """n near the engine's limit (1024) beside small instances: the widest row
   passes (8 slots of 128 columns), 16 diagonal blocks, both Gram tile shapes."""
