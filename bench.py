@@ -536,12 +536,15 @@ def main():
   line = {
       "metric": METRIC, "value": value, "unit": "QPs/s", "n_gpus": 1, "steps": args.steps,
       "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-      "scaling": "weak", "vs_baseline": None, "dtype": "f32 factor + f64 refinement",
+      "scaling": "weak", "vs_baseline": None,
+      "dtype": "f64 refinement; f32 factor while it needs <= 3 refinement steps per "
+               "IPM iteration, f64 factor after",
       "data": "synthetic (seeded condensed MPC)",
       "config": {"workload": f"batched uneven condensed MPC, {args.count} QPs, T~U{{8..64}}, "
                              f"nx~U{{4..16}}, nu~U{{1..6}}, seeds {args.base_seed}..",
                  "count": args.count, "sum_mn": int(np.sum(m * n)),
                  "mean_N": float(np.mean(n + m)), "precision": "mixed",
+                 "factorizations": {"fp32": int(work[:, 0].sum()), "fp64": int(work[:, 1].sum())},
                  "tolerances": "atol 1e-7, rtol 1e-8"},
       "roofline": roof,
       "schedule": {"used": "one persistent launch per solve (k_ipm_solve)",
