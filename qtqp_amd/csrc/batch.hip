@@ -416,78 +416,6 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
 // KKT solves with iterative refinement, one pass over A per step.
 // -------------------------------------------------------------------------
 
-// Row maxima max_j |M[i][j]| wc[j] and column maxima max_i |M[i][j]| wr[i]
-// of row-major M (leading dimension ld, zero-padded, 16-byte aligned rows;
-// ld <= 128 NS) in one pass: R rows per wavefront in registers, the next R
-// fetched while the current ones are reduced.  scratch: >= 4 * ncol
-// doubles.
-template <int NS, int R, typename RE, typename CE>
-__device__ inline void rowpass_absmax_v(const double* __restrict__ M, int nrow, int ncol, int ld,
-                                        const double* __restrict__ wc,
-                                        const double* __restrict__ wr, RE rowepi, CE colepi,
-                                        double* scratch) {
-  using V = VecT<double, 2>::type;
-  constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double wcr[NS][2], cm[NS][2];
-#pragma unroll
-  for (int s = 0; s < NS; ++s)
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int col = s * 128 + lane * 2 + e;
-      wcr[s][e] = col < ncol ? wc[col] : 0.0;
-      cm[s][e] = 0.0;
-    }
-  V cur[R][NS], nxt[R][NS];
-  auto fetch = [&](int i0, V (*dst)[NS]) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int col0 = s * 128 + lane * 2;
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        dst[r][s] = (col0 < ld && i0 + r < nrow)
-                        ? ldg(reinterpret_cast<const V*>(M + (int64_t)(i0 + r) * ld + col0))
-                        : V(0);
-    }
-  };
-  fetch(wave * R, cur);
-  for (int i0 = wave * R; i0 < nrow; i0 += 4 * R) {
-    if (i0 + 4 * R < nrow) fetch(i0 + 4 * R, nxt);
-    double d[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double w = i0 + r < nrow ? wr[i0 + r] : 0.0;
-      d[r] = 0.0;
-#pragma unroll
-      for (int s = 0; s < NS; ++s)
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const double a = fabs(cur[r][s][e]);
-          d[r] = fmax(d[r], a * wcr[s][e]);
-          cm[s][e] = fmax(cm[s][e], a * w);
-        }
-    }
-    reduce_rows<R, true>(d);
-    const int row = i0 + (lane >> (6 - LR));
-    if (row < nrow && (lane & ((64 >> LR) - 1)) == 0) rowepi(row, d[0]);
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int s = 0; s < NS; ++s) cur[r][s] = nxt[r][s];
-  }
-#pragma unroll
-  for (int s = 0; s < NS; ++s)
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int col = s * 128 + lane * 2 + e;
-      if (col < ncol) scratch[wave * ncol + col] = cm[s][e];
-    }
-  __syncthreads();
-  for (int j = threadIdx.x; j < ncol; j += WG)
-    colepi(j, fmax(fmax(scratch[j], scratch[ncol + j]), fmax(scratch[2 * ncol + j], scratch[3 * ncol + j])));
-  __syncthreads();
-}
-
 // Slack allocated after each instance's FP64 A and P (kPadRows rows of ld
 // plus kPadCols values), so that the pipelined row pass below can issue its
 // loads unconditionally.
@@ -529,10 +457,13 @@ constexpr int kPassChunk = 2016;
 //    wait drain the whole queue): the NOUT row outputs go to LDS and are
 //    written out by rowflush once per kPassChunk rows.
 // COLS (0, 1 or 2) column sums are formed; with COLS = 1, c2 is ignored and
-// colepi receives 0 for it, with COLS = 0 colepi is not called.
+// colepi receives 0 for it, with COLS = 0 colepi is not called.  MAXM
+// replaces the sums by maxima of absolute values: the row value is
+// max_j |M[i][j]| xs[j] and the column value max_i |M[i][j]| c1_i (for
+// non-negative xs and c1; equilibration).  NOUT may be 0.
 // scratch: >= max(8 ncol, NOUT kPassChunk) doubles of LDS.
-template <int NS, int R, int D, int NOUT, int COLS, typename RL, typename RE, typename RF,
-          typename CE>
+template <int NS, int R, int D, int NOUT, int COLS, bool MAXM = false, typename RL, typename RE,
+          typename RF, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
                                 const double* __restrict__ xs, RL rowload, RE rowepi, RF rowflush,
                                 CE colepi, double* scratch) {
@@ -586,13 +517,18 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
     for (int r = 0; r < R; ++r) {
       d[r] = 0.0;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) d[r] += a[r][s][0] * xr[s][0] + a[r][s][1] * xr[s][1];
+      for (int s = 0; s < NS; ++s) {
+        if constexpr (MAXM)
+          d[r] = fmax(d[r], fmax(fabs(a[r][s][0]) * xr[s][0], fabs(a[r][s][1]) * xr[s][1]));
+        else
+          d[r] += a[r][s][0] * xr[s][0] + a[r][s][1] * xr[s][1];
+      }
     }
-    reduce_rows<R>(d);
+    reduce_rows<R, MAXM>(d);
     const int row = i0 + lrow;
     double c1 = 0.0, c2 = 0.0;
     if (row < nrow) {
-      double out[NOUT];
+      double out[NOUT > 0 ? NOUT : 1];
       rowepi(row, d[0], st.in, &c1, &c2, out);
       if ((lane & ((64 >> LR) - 1)) == 0)
 #pragma unroll
@@ -607,8 +543,12 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
         for (int s = 0; s < NS; ++s)
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
-            c1a[s][e] += a[r][s][e] * k1;
-            if (COLS > 1) c2a[s][e] += a[r][s][e] * k2;
+            if constexpr (MAXM) {
+              c1a[s][e] = fmax(c1a[s][e], fabs(a[r][s][e]) * k1);
+            } else {
+              c1a[s][e] += a[r][s][e] * k1;
+              if (COLS > 1) c2a[s][e] += a[r][s][e] * k2;
+            }
           }
       }
     }
@@ -633,15 +573,17 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
           if (i < cend) process(i, c0, st[u]);
         }
       }
-      __syncthreads();
-      for (int j = threadIdx.x; j < cend - c0; j += WG) {
-        double out[NOUT];
+      if constexpr (NOUT > 0) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < cend - c0; j += WG) {
+          double out[NOUT];
 #pragma unroll
-        for (int k = 0; k < NOUT; ++k) out[k] = scratch[k * kPassChunk + j];
-        rowflush(c0 + j, out);
+          for (int k = 0; k < NOUT; ++k) out[k] = scratch[k * kPassChunk + j];
+          rowflush(c0 + j, out);
+        }
+        wait_vmem();
+        __syncthreads();
       }
-      wait_vmem();
-      __syncthreads();
     }
   }
   if constexpr (COLS > 0) {
@@ -656,11 +598,16 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
         }
       }
     __syncthreads();
-    for (int j = threadIdx.x; j < ncol; j += WG)
-      colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]),
-             COLS > 1 ? (scratch[4 * ncol + j] + scratch[5 * ncol + j]) +
-                            (scratch[6 * ncol + j] + scratch[7 * ncol + j])
-                      : 0.0);
+    for (int j = threadIdx.x; j < ncol; j += WG) {
+      if constexpr (MAXM)
+        colepi(j, fmax(fmax(scratch[j], scratch[ncol + j]),
+                       fmax(scratch[2 * ncol + j], scratch[3 * ncol + j])), 0.0);
+      else
+        colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]),
+               COLS > 1 ? (scratch[4 * ncol + j] + scratch[5 * ncol + j]) +
+                              (scratch[6 * ncol + j] + scratch[7 * ncol + j])
+                        : 0.0);
+    }
     __syncthreads();
   }
 }
@@ -1307,21 +1254,102 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 // and column inf-norms of D A E and E P E (one pass over each) and scaling
 // d by 1/sqrt(row norm), e by 1/sqrt(max(column norms)), clipped to
 // [1e-3, 1e3].  Reads the unscaled padded copies I.A and I.P.
-template <int NS, int R>
+// Ruiz equilibration, 10 passes (__init__.py:129-137 via the reference's
+// equilibrate): row maxima of |D A E| give D, column maxima of |D A E| and
+// |E P E| give E, each factor clamped to [1e-3, 1e3].  Each pass streams A
+// and P once with the pipelined row pass in its max-abs form.
+template <typename S>
 __device__ inline void ruiz(const Inst& I, double* scratch) {
   const int n = I.n, m = I.m, ld = I.ld, tid = threadIdx.x;
+  const double* dv = wave_uniform(I.d);
+  const double* ev = wave_uniform(I.e);
+  double* um = wave_uniform(I.um);
   for (int pass = 0; pass < 10; ++pass) {
-    rowpass_absmax_v<NS, R>(I.A, m, n, ld, I.e, I.d, [&](int i, double v) {
-      const double r = v * I.d[i];
-      I.um[i] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
-    }, [&](int j, double v) { I.un[j] = v * I.e[j]; }, scratch);
-    rowpass_absmax_v<NS, R>(I.P, n, n, ld, I.e, I.e, [](int, double) {}, [&](int j, double v) {
-      const double cn = fmax(I.un[j], v * I.e[j]);
-      I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
-    }, scratch);
+    rowpass2<S::NS, S::R, S::D, 1, 1, true>(
+        I.A, m, n, ld, ev, [&](int i) { return ldg(dv + i); },
+        [](int, double v, double di, double* c1, double*, double* o) {
+          *c1 = di;
+          const double r = v * di;
+          o[0] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
+        },
+        [&](int i, const double* o) { stg(um + i, o[0]); },
+        [&](int j, double v, double) { I.un[j] = v * I.e[j]; }, scratch);
+    rowpass2<S::NS, S::R, S::D, 0, 1, true>(
+        I.P, n, n, ld, ev, [&](int i) { return ldg(ev + i); },
+        [](int, double, double ei, double* c1, double*, double*) { *c1 = ei; },
+        [](int, const double*) {},
+        [&](int j, double v, double) {
+          const double cn = fmax(I.un[j], v * I.e[j]);
+          I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
+        },
+        scratch);
     for (int i = tid; i < m; i += WG) I.d[i] *= I.um[i];
     for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
     __syncthreads();
+  }
+}
+
+// rows x n row-major src into dst with leading dimension ld (zeros in the
+// pad): each wave copies 4 rows per round, each lane two columns of each
+// per 128 columns, so 8 loads of a round are in flight together.
+__device__ inline void copy_padded(const double* __restrict__ src, double* __restrict__ dst,
+                                   int rows, int n, int ld) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i0 = 4 * wave; i0 < rows; i0 += 16)
+    for (int j0 = 0; j0 < ld; j0 += 128) {
+      double v[4][2];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = i0 + r, j = j0 + lane + 64 * h;
+          v[r][h] = (i < rows && j < n) ? ldg(src + (int64_t)i * n + j) : 0.0;
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = i0 + r, j = j0 + lane + 64 * h;
+          if (i < rows && j < ld) stg(dst + (int64_t)i * ld + j, v[r][h]);
+        }
+    }
+}
+
+// M[i][j] *= sl[i] * sr[j] in place (rows x ld, zero pad kept) and its FP32
+// copy: 16-byte vectors, 4 rows per wave per round.
+__device__ inline void scale_rows_cols(double* __restrict__ M, float* __restrict__ M32,
+                                       const double* __restrict__ sl,
+                                       const double* __restrict__ sr, int rows, int n, int ld) {
+  using V = VecT<double, 2>::type;
+  typedef float F2 __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < ld; j0 += 128) {
+    const int j = j0 + 2 * lane;
+    const double e0 = j < n ? ldg(sr + j) : 0.0, e1 = j + 1 < n ? ldg(sr + j + 1) : 0.0;
+    for (int i0 = 4 * wave; i0 < rows; i0 += 16) {
+      V a[4];
+      double di[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = min(i0 + r, rows - 1);
+        a[r] = j < ld ? ldg(reinterpret_cast<const V*>(M + (int64_t)i * ld + j)) : V(0);
+        di[r] = ldg(sl + i);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r;
+        if (i < rows && j < ld) {
+          V v;
+          v[0] = j < n ? a[r][0] * (di[r] * e0) : 0.0;
+          v[1] = j + 1 < n ? a[r][1] * (di[r] * e1) : 0.0;
+          stg(reinterpret_cast<V*>(M + (int64_t)i * ld + j), v);
+          F2 f;
+          f[0] = (float)v[0];
+          f[1] = (float)v[1];
+          stg(reinterpret_cast<F2*>(M32 + (int64_t)i * ld + j), f);
+        }
+      }
+    }
   }
 }
 
@@ -1337,43 +1365,15 @@ __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equi
     for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
     for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
     // unscaled copies with the padded leading dimension (zeros in the pad)
-    const int ld = I.ld, lane = tid & 63, wave = tid >> 6;
-    for (int i = wave; i < m; i += 4)
-      for (int j = lane; j < ld; j += 64)
-        I.A[(int64_t)i * ld + j] = j < n ? ldg(I.Araw + (int64_t)i * n + j) : 0.0;
-    for (int i = wave; i < n; i += 4)
-      for (int j = lane; j < ld; j += 64)
-        I.P[(int64_t)i * ld + j] = j < n ? ldg(I.Praw + (int64_t)i * n + j) : 0.0;
+    const int ld = I.ld;
+    copy_padded(I.Araw, I.A, m, n, ld);
+    copy_padded(I.Praw, I.P, n, n, ld);
     __syncthreads();
-    if (equilibrate) {
-      if (ld <= 128)
-        ruiz<1, 8>(I, scratch);
-      else if (ld <= 256)
-        ruiz<2, 8>(I, scratch);
-      else if (ld <= 512)
-        ruiz<4, 4>(I, scratch);
-      else
-        ruiz<8, 2>(I, scratch);
-    }
+    if (equilibrate) with_pass_shape(ld, [&](auto sh) { ruiz<decltype(sh)>(I, scratch); });
+    __syncthreads();
     // D A E and E P E in place, and their FP32 copies
-    for (int i = wave; i < m; i += 4) {
-      const double di = I.d[i];
-      for (int j = lane; j < ld; j += 64) {
-        const int64_t k = (int64_t)i * ld + j;
-        const double v = j < n ? I.A[k] * (di * I.e[j]) : 0.0;
-        I.A[k] = v;
-        I.A32[k] = (float)v;
-      }
-    }
-    for (int i = wave; i < n; i += 4) {
-      const double ei = I.e[i];
-      for (int j = lane; j < ld; j += 64) {
-        const int64_t k = (int64_t)i * ld + j;
-        const double v = j < n ? I.P[k] * (ei * I.e[j]) : 0.0;
-        I.P[k] = v;
-        I.P32[k] = (float)v;
-      }
-    }
+    scale_rows_cols(I.A, I.A32, I.d, I.e, m, n, ld);
+    scale_rows_cols(I.P, I.P32, I.e, I.e, n, n, ld);
     double lb = 0.0, lc = 0.0;
     for (int i = tid; i < m; i += WG) {
       I.b[i] = I.braw[i] * I.d[i];
@@ -1807,7 +1807,7 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   const Opts o = to_opts(opts);
   HIPC(hipEventRecord(h->ev0, h->stream));
   HIPC(hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream));
-  const size_t smem_prep = sizeof(double) * (WG + 8 + 4 * h->nmax);
+  const size_t smem_prep = sizeof(double) * (WG + 8 + std::max(4 * h->nmax, kPassChunk));
   hipLaunchKernelGGL(k_prepare, dim3(std::min(h->count, 2048)), dim3(WG), smem_prep, h->stream,
                      h->dinst, h->count, opts->equilibrate ? 1 : 0, h->have_start ? 0 : 1,
                      o.precision == QTQP_PREC_FP64 ? 1 : 0);
