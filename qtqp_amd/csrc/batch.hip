@@ -1353,13 +1353,25 @@ __device__ inline void scale_rows_cols(double* __restrict__ M, float* __restrict
   }
 }
 
-__global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equilibrate,
-                                                int default_start, int force64) {
-  // dynamic LDS: (WG + 8 + 4 * nmax) doubles
+// Equilibration, scaled copies and start of every QP: a persistent grid
+// whose workgroups take QPs from an atomic queue in `order` (longest first;
+// null: index order), as k_ipm_solve does, so one workgroup does not end up
+// with several of the largest.
+__global__ __launch_bounds__(WG) void k_prepare(Inst* insts, const int* order, int count,
+                                                int* queue, int equilibrate, int default_start,
+                                                int force64) {
+  // dynamic LDS: prepare_lds_bytes(nmax)
   extern __shared__ double smem[];
-  double* red = smem;           // WG + 8
-  double* scratch = smem + WG + 8;  // 4 * nmax
-  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+  double* red = smem;               // WG + 8
+  double* scratch = smem + WG + 8;  // max(4 nmax, kPassChunk)
+  __shared__ int s_q;
+  for (;;) {
+    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+    __syncthreads();
+    const int t = s_q;
+    __syncthreads();
+    if (t >= count) break;
+    const int q = order ? order[t] : t;
     const Inst& I = insts[q];
     const int n = I.n, m = I.m, tid = threadIdx.x;
     for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
@@ -1416,6 +1428,12 @@ __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, int count, int equi
 }
 
 // Packed copies between the per-QP iterates and caller layouts.
+// Dynamic LDS of k_prepare: reduction slots and the row pass's scratch
+// (column maxima of nmax columns per wavefront, or the staged row outputs).
+inline size_t prepare_lds_bytes(int nmax) {
+  return sizeof(double) * (WG + 8 + std::max(4 * nmax, kPassChunk));
+}
+
 __global__ __launch_bounds__(WG) void k_gather(const Inst* insts, int count, const int64_t* n_off,
                                                const int64_t* m_off, double* x, double* y,
                                                double* s, double* scal, int* ints) {
@@ -1807,14 +1825,15 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   const Opts o = to_opts(opts);
   HIPC(hipEventRecord(h->ev0, h->stream));
   HIPC(hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream));
-  const size_t smem_prep = sizeof(double) * (WG + 8 + std::max(4 * h->nmax, kPassChunk));
-  hipLaunchKernelGGL(k_prepare, dim3(std::min(h->count, 2048)), dim3(WG), smem_prep, h->stream,
-                     h->dinst, h->count, opts->equilibrate ? 1 : 0, h->have_start ? 0 : 1,
-                     o.precision == QTQP_PREC_FP64 ? 1 : 0);
-  HIPC(hipGetLastError());
   int dev = 0, cus = 0;
   HIPC(hipGetDevice(&dev));
   HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+  hipLaunchKernelGGL(k_prepare, dim3(std::min(h->count, std::max(1, cus * 2))), dim3(WG),
+                     prepare_lds_bytes(h->nmax), h->stream, h->dinst, h->dorder, h->count, h->dqueue,
+                     opts->equilibrate ? 1 : 0, h->have_start ? 0 : 1,
+                     o.precision == QTQP_PREC_FP64 ? 1 : 0);
+  HIPC(hipGetLastError());
   const int grid = std::min(h->count, std::max(1, cus * 2));
   hipLaunchKernelGGL(k_reset_clock, dim3(std::max(1, (h->count + 1 + WG - 1) / WG)), dim3(WG), 0,
                      h->stream, h->dclk, h->count);
@@ -1946,7 +1965,6 @@ int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t samp
   int dev = 0, cus = 0;
   if (e == hipSuccess) e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const size_t smem_prep = sizeof(double) * (WG + 8 + 4 * h->nmax);
   for (int b = 0; b < QTQP_BATCH_NBUCKETS && e == hipSuccess; ++b) {
     const std::vector<int>& mem = members[b];
     if (mem.empty()) continue;
@@ -1961,7 +1979,9 @@ int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t samp
                               h->stream)) != hipSuccess)
         break;
       if ((e = hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream)) != hipSuccess) break;
-      hipLaunchKernelGGL(k_prepare, dim3(k), dim3(WG), smem_prep, h->stream, tmp, k,
+      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+      hipLaunchKernelGGL(k_prepare, dim3(std::min(k, std::max(1, cus * 2))), dim3(WG),
+                         prepare_lds_bytes(h->nmax), h->stream, tmp, nullptr, k, h->dqueue,
                          opts->equilibrate ? 1 : 0, 1, o.precision == QTQP_PREC_FP64 ? 1 : 0);
       const int grid = std::min(k, std::max(1, cus * 2));
       if ((e = hipEventRecord(h->ev0, h->stream)) != hipSuccess) break;
