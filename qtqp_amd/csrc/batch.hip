@@ -2,20 +2,22 @@
 //
 // Layout in HBM (one allocation per array kind, instances packed back to
 // back in input order, every per-instance array 256-byte aligned):
-//   raw A (m x n row-major, as uploaded), the equilibrated A in FP64 and in
-//   FP32 (for the FP32 factor path), raw and equilibrated P (n x n) in FP64
-//   and FP32, vectors b, c, d, e, the Gram/factor workspace G (n x n FP64,
-//   whose front half holds the FP32 factor), iterates and scratch vectors.
-// Every pass over A is one coalesced sweep of its rows that yields both A x
-// (row dots, wavefront reductions) and A' u (column partials per lane,
-// combined across the four wavefronts in LDS).
+//   raw A (m x n row-major, as uploaded), the equilibrated A in FP64 (with
+//   read slack after it) and in FP32 (for the FP32 factor path), raw and
+//   equilibrated P (n x n) in FP64 (with slack) and FP32, vectors b, c, d,
+//   e, the Gram/factor workspace G (n x n FP64, whose front half holds the
+//   FP32 factor), iterates and scratch vectors.
+// Every pass over A is one coalesced, software-pipelined sweep of its rows
+// that yields both A x (row dots, wavefront reductions) and A' u (column
+// partials per lane, combined across the four wavefronts in LDS).
 //
-// Execution: k_prepare (one workgroup per QP: Ruiz scaling, FP32 copies,
-// default start) once per solve, then per IPM iteration the pair
-// {k_reset_queue, k_ipm_iter} replayed from a HIP graph.  k_ipm_iter is a
-// persistent grid: each workgroup takes the next QP from an atomic queue
-// (QPs sorted by descending m*n^2, so large ones start first) and runs one
-// whole IPM iteration of it; every workgroup exits when the queue is empty.
+// Execution: k_prepare (Ruiz scaling, FP32 copies, start) once per solve,
+// then either one persistent k_ipm_solve launch that runs every QP to
+// completion, or per IPM iteration the pair {k_reset_queue, k_ipm_iter}
+// replayed from a HIP graph.  Both kernels, and k_prepare, are persistent
+// grids: each workgroup takes the next QP from an atomic queue (QPs sorted
+// by descending m*n^2, so large ones start first) and every workgroup exits
+// when the queue is empty.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
