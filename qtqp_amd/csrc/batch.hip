@@ -909,18 +909,39 @@ __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, doub
                               double r_tau, double* tau_out, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
-  double kqmax = 0.0;
-  for (int k = threadIdx.x; k < n + m; k += WG) kqmax = fmax(kqmax, fabs(kr[k]));
-  const double dot_dbg_kq = bmax(kqmax, sm.red);
-  double t_a = mu + (dot(I.kq, I.c, n, sm.red) + dot(I.kq + n, I.b, m, sm.red));
-  double t_b = -r_tau - (dot(kr, I.c, n, sm.red) + dot(kr + n, I.b, m, sm.red));
-  double t_c = -mu_t;
   if (I.has_p) {
     pmul(I, kr, I.un, sm);   // P kr_x
     pmul(I, I.kq, I.um, sm); // P kq_x (um has >= n slots: m >= 1 ... sized max(n,m))
-    t_a -= dot(I.kq, I.um, n, sm.red);
-    t_b += dot(kr, I.um, n, sm.red) + dot(I.kq, I.un, n, sm.red);
-    t_c -= dot(kr, I.un, n, sm.red);
+  }
+  // the dot products of the tau quadratic and max|kr|, reduced together:
+  // slots kq.c, kq_y.b, kr.c, kr_y.b, kq.Pkq, kr.Pkq, kq.Pkr, kr.Pkr, max|kr|
+  double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < n + m; k += WG) v[8] = fmax(v[8], fabs(kr[k]));
+  for (int j = threadIdx.x; j < n; j += WG) {
+    v[0] += I.kq[j] * I.c[j];
+    v[2] += kr[j] * I.c[j];
+  }
+  for (int i = threadIdx.x; i < m; i += WG) {
+    v[1] += I.kq[n + i] * I.b[i];
+    v[3] += kr[n + i] * I.b[i];
+  }
+  if (I.has_p) {
+    for (int j = threadIdx.x; j < n; j += WG) {
+      v[4] += I.kq[j] * I.um[j];
+      v[5] += kr[j] * I.um[j];
+      v[6] += I.kq[j] * I.un[j];
+      v[7] += kr[j] * I.un[j];
+    }
+  }
+  block_reduce_n<9, 0x100u>(v, sm.red);
+  const double dot_dbg_kq = v[8];
+  double t_a = mu + (v[0] + v[1]);
+  double t_b = -r_tau - (v[2] + v[3]);
+  double t_c = -mu_t;
+  if (I.has_p) {
+    t_a -= v[4];
+    t_b += v[5] + v[6];
+    t_c -= v[7];
   }
   if (threadIdx.x == 0 && I.trace) {
     const int row = I.si[I_ITERS];
@@ -1095,8 +1116,10 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     sys += ya * sa;
   }
   const double ta = tau + alpha_p * (tau_p - tau);
-  const double nrm2 = bsum(sx, red) + bsum(sy, red) + ta * ta;
-  const double mu_aff = (double)(m - z + 1) / fmax(kEps * kEps, nrm2) * bsum(sys, red) / (m - z);
+  double sums[3] = {sx, sy, sys};
+  block_reduce_n<3>(sums, red);
+  const double nrm2 = sums[0] + sums[1] + ta * ta;
+  const double mu_aff = (double)(m - z + 1) / fmax(kEps * kEps, nrm2) * sums[2] / (m - z);
   const double base = mu_aff / fmax(kEps, mu);
   const double sigma = fmin(fmax(base * base * base, 0.0), 1.0);
 
@@ -1191,12 +1214,16 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     l_axs = fmax(l_axs, fabs(ax[i] + so));
     l_ny = fmax(l_ny, fabs(I.d[i] * I.y[i]));
   }
-  const double xpx = I.has_p ? bsum(l_xpx, red) : 0.0;
-  const double ctx = bsum(l_ctx, red), bty = bsum(l_bty, red);
-  const double norm_px = bmax(l_px, red), norm_aty = bmax(l_aty, red);
-  const double dres = bmax(l_dres, red), pres = bmax(l_pres, red);
-  const double norm_ax = bmax(l_ax, red), norm_s = bmax(l_s, red), norm_axs = bmax(l_axs, red);
-  const double norm_x = bmax(l_nx, red), norm_y = bmax(l_ny, red);
+  // three sums (slots 0-2) and eleven maxima, reduced together
+  double v[14] = {l_xpx, l_ctx, l_bty, l_px, l_aty, l_dres, l_pres, l_ax,
+                  l_s,   l_axs, l_nx,  l_ny, 0.0,  0.0};
+  block_reduce_n<14, 0x3FF8u>(v, red);
+  const double xpx = I.has_p ? v[0] : 0.0;
+  const double ctx = v[1], bty = v[2];
+  const double norm_px = v[3], norm_aty = v[4];
+  const double dres = v[5], pres = v[6];
+  const double norm_ax = v[7], norm_s = v[8], norm_axs = v[9];
+  const double norm_x = v[10], norm_y = v[11];
   const double pcost = (ctx + 0.5 * xpx * inv_tau) * inv_tau;
   const double dcost = (-bty - 0.5 * xpx * inv_tau) * inv_tau;
   const double gap = fabs((ctx + bty + xpx * inv_tau) * inv_tau);

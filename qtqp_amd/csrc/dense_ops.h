@@ -75,6 +75,27 @@ __device__ inline double block_max(double v, double* red) {
   return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// K block-wide reductions with one pair of barriers: slot k is a sum, or a
+// maximum where bit k of MAXMASK is set.  Each slot is combined exactly as
+// block_sum / block_max combine a single value.  red: >= 4 K doubles of LDS.
+template <int K, unsigned MAXMASK = 0u>
+__device__ inline void block_reduce_n(double (&v)[K], double* red) {
+  static_assert(K <= 32, "slots");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = ((MAXMASK >> k) & 1u) ? wave_max(v[k]) : wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[wave * K + k] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    v[k] = ((MAXMASK >> k) & 1u) ? fmax(fmax(red[k], red[K + k]), fmax(red[2 * K + k], red[3 * K + k]))
+                                 : red[k] + red[K + k] + red[2 * K + k] + red[3 * K + k];
+}
+
 // Completes R per-lane partial sums (or maxima, MAX) across the wavefront
 // (R a power of two up to 32): a transpose-reduce that halves the rows a lane carries
 // at each of log2(R) exchanges, then combines the rest.  Lane l ends with
