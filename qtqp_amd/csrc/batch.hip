@@ -149,29 +149,203 @@ __device__ inline double dot(const double* a, const double* b, int len, double* 
 // -------------------------------------------------------------------------
 // Factorisation of G = P_off + diag(d_x) + A' W A (one workgroup).
 // -------------------------------------------------------------------------
-// Lower tiles of the Gram block G = P_off + diag(d_x) + A' W A as
-// (64 MT) x (64 MT) rank-m MFMA tiles.
+// NJ 64 x 64 Gram tiles of one tile row, G[i0 block, j0[t] block] +=
+// A[:, i0 block]' W A[:, j0[t] block], streamed together so the row strip
+// of A is read once for all of them.  DIAG: the last tile is the diagonal
+// one (j0[NJ-1] == i0), whose second strip is the row strip itself, so it
+// costs no loads of its own.  Rounds of KCH rows are staged in LDS and kept
+// two rounds ahead in registers, with unconditional (clamped) loads, as in
+// RankKTileG::run_vec; each tile accumulates exactly as a RankKTileG<T, 1, 1>
+// tile would.  lds: (1 + NJ) KCH x 65 elements of T.
+template <typename T, int NJ, bool DIAG>
+struct GramRow {
+  using acc_t = typename Mfma<T>::acc_t;
+  static constexpr int KCH = RankKTileG<T, 1, 1>::KCH;
+  static constexpr int VW = 16 / sizeof(T);
+  static constexpr int VI = NB / VW;    // vectors per staged row of a strip
+  static constexpr int RI = WG / VI;    // rows per pass of the workgroup
+  static constexpr int PI = KCH / RI;   // vectors per thread per strip and round
+  static constexpr int NLD = NJ - (DIAG ? 1 : 0);  // strips loaded besides the row strip
+  typedef T VT __attribute__((ext_vector_type(VW)));
+  acc_t acc[NJ][2][2];
+
+  __device__ inline void run(const T* __restrict__ X, int ldx, int kend, int i0,
+                             const int (&j0)[NJ], const double* __restrict__ w, T* lds) {
+    X = wave_uniform(X);
+    w = wave_uniform(w);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int jb = 32 * (wave >> 1), ib = 32 * (wave & 1);
+    const int vi = tid % VI, ri = tid / VI;
+#pragma unroll
+    for (int t = 0; t < NJ; ++t)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[t][a][b][r] = T(0);
+    // columns past ldx read as the last vector and are masked when staged
+    const int coli = min(i0 + vi * VW, ldx - VW);
+    const bool ci = i0 + vi * VW < ldx;
+    int colj[NLD > 0 ? NLD : 1];
+    bool cj[NLD > 0 ? NLD : 1];
+#pragma unroll
+    for (int t = 0; t < NLD; ++t) {
+      colj[t] = min(j0[t] + vi * VW, ldx - VW);
+      cj[t] = j0[t] + vi * VW < ldx;
+    }
+    struct Round {
+      VT a[PI], b[NLD > 0 ? NLD : 1][PI];
+      double w[PI];
+    };
+    auto fetch = [&](int kc, Round& rd) {
+#pragma unroll
+      for (int q = 0; q < PI; ++q) {
+        const int k = min(kc + ri + RI * q, kend - 1);
+        const T* row = X + (int64_t)k * ldx;
+        rd.a[q] = ldg(reinterpret_cast<const VT*>(row + coli));
+#pragma unroll
+        for (int t = 0; t < NLD; ++t) rd.b[t][q] = ldg(reinterpret_cast<const VT*>(row + colj[t]));
+        rd.w[q] = ldg(w + k);
+      }
+    };
+    T* si = lds;
+    auto sj = [&](int t) { return lds + (1 + t) * KCH * (NB + 1); };
+    auto step = [&](int kc, Round& rd, bool active) {
+      if (active) {
+#pragma unroll
+        for (int q = 0; q < PI; ++q) {
+          const bool kin = kc + ri + RI * q < kend;
+          const VT va = (kin && ci) ? rd.a[q] : VT(0);
+          const T wk = (T)rd.w[q];
+#pragma unroll
+          for (int e = 0; e < VW; ++e) si[(ri + RI * q) * (NB + 1) + vi * VW + e] = va[e];
+#pragma unroll
+          for (int t = 0; t < NJ; ++t) {
+            VT vb;
+            if (DIAG && t == NJ - 1)
+              vb = (kin && ci) ? rd.a[q] * wk : VT(0);
+            else
+              vb = (kin && cj[t]) ? rd.b[t][q] * wk : VT(0);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) sj(t)[(ri + RI * q) * (NB + 1) + vi * VW + e] = vb[e];
+          }
+        }
+      }
+      __syncthreads();
+      fetch(kc + 2 * KCH, rd);
+      if (active) {
+#pragma unroll
+        for (int kk = 0; kk < KCH; kk += 4) {
+          const int kq = kk + (lane >> 4), cl = lane & 15;
+          T bv[2];
+#pragma unroll
+          for (int b = 0; b < 2; ++b) bv[b] = si[kq * (NB + 1) + ib + 16 * b + cl];
+#pragma unroll
+          for (int t = 0; t < NJ; ++t) {
+            T av[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = sj(t)[kq * (NB + 1) + jb + 16 * a + cl];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int b = 0; b < 2; ++b) acc[t][a][b] = Mfma<T>::mac(av[a], bv[b], acc[t][a][b]);
+          }
+        }
+      }
+      __syncthreads();
+    };
+    if (kend <= 0) return;
+    wait_mem();
+    Round rd[2];
+    fetch(0, rd[0]);
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(KCH, rd[1]);
+    __builtin_amdgcn_sched_barrier(0);
+    for (int kc = 0; kc < kend; kc += 2 * KCH) {
+      step(kc, rd[0], true);
+      step(kc + KCH, rd[1], kc + KCH < kend);
+    }
+  }
+
+  // f(i, j, value) for every accumulated element of tile t of this thread
+  template <typename F>
+  __device__ inline void for_each(int t, int i0, int j0, F f) const {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int jb = j0 + 32 * (wave >> 1), ib = i0 + 32 * (wave & 1);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          f(ib + 16 * b + (lane & 15), jb + 16 * a + Mfma<T>::row(lane, r), acc[t][a][b][r]);
+  }
+};
+
+// G tiles (i0, j0[t]) = A' W A + (P_off or diag(d_x)) for one GramRow pass
+// (a function of its own, so each pass shape gets its own registers).
+template <typename T, int NJ, bool DIAG>
+__device__ __noinline__ void gram_pass(const Inst& I, const T* A, const T* P, T* G,
+                                       const double* w, int i0, int j00, T* lds_in) {
+  T* lds = in_lds(lds_in);
+  const int n = I.n, m = I.m, ld = I.ld;
+  int j0[NJ];
+  j0[0] = j00;
+  if (NJ > 1) j0[NJ - 1] = DIAG ? i0 : j00 + NB;
+  GramRow<T, NJ, DIAG> g;
+  g.run(A, ld, m, i0, j0, w, lds);
+  auto store = [&](int i, int j, T v) {
+    if (i < n && j < n && i >= j)
+      stg(G + (int64_t)j * n + i, v + ((i == j) ? (T)ldg(I.dg + i) : ldg(P + (int64_t)j * ld + i)));
+  };
+#pragma unroll
+  for (int t = 0; t < NJ; ++t) g.for_each(t, i0, j0[t], store);
+}
+
+// Lower tiles of the Gram block G = P_off + diag(d_x) + A' W A.  MT = 2:
+// 128 x 128 rank-m MFMA tiles (RankKTileG).  MT = 1: 64 x 64 tiles a tile
+// row at a time, two per pass on a shared row strip (GramRow), the
+// diagonal tile reusing the row strip: a tile row of r + 1 tiles reads
+// about 1.5 (r + 1) strips of A instead of 2 (r + 1).
 template <typename T, int MT>
 __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, const double* w,
                                   Smem& sm) {
   const int n = I.n, m = I.m, ld = I.ld;
-  constexpr int TS = 64 * MT;
-  const int nt = (n + TS - 1) / TS;
-  using Tile = RankKTileG<T, MT, MT>;
-  T* si = reinterpret_cast<T*>(sm.u);
-  T* sj = si + Tile::KCH * (TS + 1);
-  for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
-    int ti, tj;
-    lower_tile(t, &ti, &tj);
-    const int i0 = ti * TS, j0 = tj * TS;
-    Tile acc;
-    acc.zero();
-    // rows of A are 16-byte aligned and zero-padded to ld (a multiple of 4)
-    acc.run_vec(A, ld, 0, m, i0, j0, w, [](double v) { return (T)v; }, si, sj);
-    acc.for_each(i0, j0, [&](int i, int j, T v) {
-      if (i < n && j < n && i >= j)
-        stg(G + (int64_t)j * n + i, v + ((i == j) ? (T)ldg(I.dg + i) : ldg(P + (int64_t)j * ld + i)));
-    });
+  if constexpr (MT == 1) {
+    static_assert(3 * GramRow<T, 2, false>::KCH * (NB + 1) * sizeof(T) <= kLdsU * sizeof(double),
+                  "GramRow staging fits the LDS union");
+    T* lds = reinterpret_cast<T*>(sm.u);
+    const int nt = (n + NB - 1) / NB;
+    for (int ti = 0; ti < nt; ++ti) {
+      const int i0 = ti * NB;
+      int tj = 0;
+      for (; tj + 1 < ti; tj += 2) gram_pass<T, 2, false>(I, A, P, G, w, i0, tj * NB, lds);
+      if (tj + 1 == ti)
+        gram_pass<T, 2, true>(I, A, P, G, w, i0, tj * NB, lds);
+      else
+        gram_pass<T, 1, true>(I, A, P, G, w, i0, i0, lds);
+    }
+  } else {
+    constexpr int TS = 64 * MT;
+    const int nt = (n + TS - 1) / TS;
+    using Tile = RankKTileG<T, MT, MT>;
+    T* si = reinterpret_cast<T*>(sm.u);
+    T* sj = si + Tile::KCH * (TS + 1);
+    for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
+      int ti, tj;
+      lower_tile(t, &ti, &tj);
+      const int i0 = ti * TS, j0 = tj * TS;
+      Tile acc;
+      acc.zero();
+      // rows of A are 16-byte aligned and zero-padded to ld (a multiple of 4)
+      acc.run_vec(A, ld, 0, m, i0, j0, w, [](double v) { return (T)v; }, si, sj);
+      acc.for_each(i0, j0, [&](int i, int j, T v) {
+        if (i < n && j < n && i >= j)
+          stg(G + (int64_t)j * n + i,
+              v + ((i == j) ? (T)ldg(I.dg + i) : ldg(P + (int64_t)j * ld + i)));
+      });
+    }
   }
 }
 
