@@ -166,6 +166,7 @@ struct GramRow {
   static constexpr int RI = WG / VI;    // rows per pass of the workgroup
   static constexpr int PI = KCH / RI;   // vectors per thread per strip and round
   static constexpr int NLD = NJ - (DIAG ? 1 : 0);  // strips loaded besides the row strip
+  static constexpr int DEPTH = NLD > 1 ? 1 : 2;     // rounds in flight (registers)
   typedef T VT __attribute__((ext_vector_type(VW)));
   acc_t acc[NJ][2][2];
 
@@ -233,7 +234,7 @@ struct GramRow {
         }
       }
       __syncthreads();
-      fetch(kc + 2 * KCH, rd);
+      fetch(kc + DEPTH * KCH, rd);
       if (active) {
 #pragma unroll
         for (int kk = 0; kk < KCH; kk += 4) {
@@ -257,14 +258,15 @@ struct GramRow {
     };
     if (kend <= 0) return;
     wait_mem();
-    Round rd[2];
-    fetch(0, rd[0]);
-    __builtin_amdgcn_sched_barrier(0);
-    fetch(KCH, rd[1]);
-    __builtin_amdgcn_sched_barrier(0);
-    for (int kc = 0; kc < kend; kc += 2 * KCH) {
-      step(kc, rd[0], true);
-      step(kc + KCH, rd[1], kc + KCH < kend);
+    Round rd[DEPTH];
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) {
+      fetch(d * KCH, rd[d]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int kc = 0; kc < kend; kc += DEPTH * KCH) {
+#pragma unroll
+      for (int d = 0; d < DEPTH; ++d) step(kc + d * KCH, rd[d], kc + d * KCH < kend);
     }
   }
 
