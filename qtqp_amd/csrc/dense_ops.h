@@ -621,16 +621,24 @@ __device__ inline void trsm_mfma_rows(T* G, int64_t ld, int n, int k0, int nb,
   for (int s = 0; s < STRIPS; ++s) {
     const int r0 = rbeg + 16 * STRIPS * wave + 16 * s;
     if (r0 >= n) break;
+    // the strip's whole operand is loaded before its first MFMA (rows
+    // clamped below n, columns below nb, masked where used): one memory
+    // latency per strip rather than one per group of 4 columns
+    T bv[16];
+    const int rowc = min(r0 + cl, n - 1);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      bv[q] = ldg(G + (int64_t)(k0 + min(4 * q + kq, nb - 1)) * ld + rowc);
     acc_t acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[ct][r] = T(0);
     const int row = r0 + cl;
-#pragma unroll 4
-    for (int kk = 0; kk < NB; kk += 4) {
-      const int k = kk + kq;
-      const T b = (row < n && k < nb) ? ldg(G + (int64_t)(k0 + k) * ld + row) : T(0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = 4 * q + kq;
+      const T b = (row < n && k < nb) ? bv[q] : T(0);
 #pragma unroll
       for (int ct = 0; ct < 4; ++ct) acc[ct] = Mfma<T>::mac(Li[16 * ct + cl][k], b, acc[ct]);
     }
