@@ -45,6 +45,7 @@ class LinearSolver(enum.Enum):
   AUTO = "auto"
   HIP_DENSE = direct.HipDenseSolver
   HIP_DENSE_FP64 = direct.HipDenseFp64Solver
+  HIP_SPARSE = direct.HipSparseSolver
 
 
 _AUTO_SOLVER_CACHE: dict[str, LinearSolver] = {}
@@ -58,7 +59,7 @@ def _instantiate_linear_solver(linear_solver: LinearSolver) -> direct.LinearSolv
 
 
 def _auto_linear_solver_order() -> list[LinearSolver]:
-  return [LinearSolver.HIP_DENSE, LinearSolver.HIP_DENSE_FP64]
+  return [LinearSolver.HIP_DENSE, LinearSolver.HIP_DENSE_FP64, LinearSolver.HIP_SPARSE]
 
 
 class _ProductMat:

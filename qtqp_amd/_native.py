@@ -99,6 +99,38 @@ IPM_SYMBOLS = {
 }
 
 
+class SparseInfo(ctypes.Structure):
+  """qtqp_sparse_info (include/qtqp_sparse.h)."""
+  _fields_ = [("n", ctypes.c_int64), ("nnz_kkt", ctypes.c_int64), ("nnz_l", ctypes.c_int64),
+              ("supernodes", ctypes.c_int64), ("levels", ctypes.c_int64),
+              ("max_rows", ctypes.c_int64), ("update_tiles", ctypes.c_int64),
+              ("factorizations", ctypes.c_int64), ("solves", ctypes.c_int64),
+              ("factor_flops", ctypes.c_double), ("analyze_ms", ctypes.c_double),
+              ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+
+
+_c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+SPARSE_CSC = 0
+SPARSE_CSR = 1
+
+# Exported symbol -> (restype, argtypes), matching include/qtqp_sparse.h.
+SPARSE_SYMBOLS = {
+    "qtqp_sparse_analyze": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_int64_p, _c_int64_p,
+                                           _c_int64_p, ctypes.POINTER(SparseInfo)]),
+    "qtqp_sparse_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_int64_p, _c_int64_p,
+                                          _c_double_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_sparse_update_diag": (ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    "qtqp_sparse_factorize": (ctypes.c_int, [ctypes.c_void_p]),
+    "qtqp_sparse_solve": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    "qtqp_sparse_matvec": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
+    "qtqp_sparse_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SparseInfo)]),
+    "qtqp_sparse_free": (None, [ctypes.c_void_p]),
+    "qtqp_sparse_last_error": (ctypes.c_char_p, []),
+    "qtqp_sparse_device_count": (ctypes.c_int, []),
+}
+
+
 def lib_path(name: str) -> pathlib.Path:
   return LIBDIR / name
 
@@ -125,8 +157,22 @@ def kkt_lib() -> ctypes.CDLL:
   return load("libqtqp_kkt.so", {**KKT_SYMBOLS, **IPM_SYMBOLS})
 
 
+def sparse_lib() -> ctypes.CDLL:
+  return load("libqtqp_sparse.so", SPARSE_SYMBOLS)
+
+
+def iptr(a: np.ndarray):
+  return a.ctypes.data_as(_c_int64_p)
+
+
+def as_i64(a) -> np.ndarray:
+  return np.ascontiguousarray(a, dtype=np.int64)
+
+
 def require_device(lib: ctypes.CDLL) -> None:
-  if lib.qtqp_device_count() < 1:
+  count = lib.qtqp_sparse_device_count if hasattr(lib, "qtqp_sparse_device_count") else \
+      lib.qtqp_device_count
+  if count() < 1:
     raise OSError("no HIP device is visible; the qtqp_amd HIP backends need an MI355X GPU")
 
 
@@ -142,7 +188,9 @@ def check(lib: ctypes.CDLL, rc: int) -> None:
   """Maps a C status to the exception the reference backends raise."""
   if rc == QTQP_OK:
     return
-  msg = (lib.qtqp_last_error() or b"").decode(errors="replace")
+  last = lib.qtqp_sparse_last_error if hasattr(lib, "qtqp_sparse_last_error") else \
+      lib.qtqp_last_error
+  msg = (last() or b"").decode(errors="replace")
   if rc == QTQP_ERR_FACTOR:
     raise np.linalg.LinAlgError(msg)
   if rc in (QTQP_ERR_ARG, QTQP_ERR_NAN):

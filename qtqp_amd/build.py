@@ -24,6 +24,7 @@ LIBRARIES = {
     "libqtqp_kkt.so": ("kkt.hip", "ipm.hip"),
     "libqtqp_batch.so": ("batch.hip",),
     "libqtqp_mlp.so": ("mlp.hip",),
+    "libqtqp_sparse.so": ("sparse.hip",),
 }
 
 HIPCC_FLAGS = [

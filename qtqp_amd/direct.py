@@ -256,6 +256,97 @@ class HipDenseFp64Solver(_HipDenseBase):
   _precision = _native.PREC_FP64
 
 
+class HipSparseSolver(LinearSolver):
+  """Sparse KKT backend on an MI355X: supernodal LDL' through the qtqp_amd
+  sparse C ABI (include/qtqp_sparse.h).
+
+  The counterpart of the reference's sparse direct backends, CuDssSolver
+  (solvers_gpu.py:27-115) and QdldlSolver (solvers_sparse.py:74-93): it takes
+  the upper-triangular KKT scaffold in CSC, analyses its pattern once
+  (ordering, elimination tree, supernodes) and keeps pattern, values and
+  factor on the device.  ``solve`` is one application of the FP64 factor;
+  DirectKktSolver refines around it on the host, as it does for cuDSS.
+  """
+
+  def __init__(self):
+    self._lib = _native.sparse_lib()
+    _native.require_device(self._lib)
+    self._h = None
+
+  def set_kkt(self, kkt: sp.spmatrix) -> None:
+    super().set_kkt(kkt)
+    self._release()
+    csc = sp.csc_matrix(kkt)
+    csc.sort_indices()
+    ptr, ind = _native.as_i64(csc.indptr), _native.as_i64(csc.indices)
+    val = _native.as_f64(csc.data)
+    h = _native.ctypes.c_void_p()
+    rc = self._lib.qtqp_sparse_create(csc.shape[0], _native.SPARSE_CSC, _native.iptr(ptr),
+                                      _native.iptr(ind), _native.dptr(val),
+                                      _native.ctypes.byref(h))
+    _native.check(self._lib, rc)
+    self._h = h
+
+  def update_diag(self, diag: np.ndarray) -> None:
+    d = _native.as_f64(diag)
+    _native.check(self._lib, self._lib.qtqp_sparse_update_diag(self._h, _native.dptr(d)))
+    np.copyto(self._kkt_diag, d)
+
+  def factorize(self) -> None:
+    _native.check(self._lib, self._lib.qtqp_sparse_factorize(self._h))
+
+  def solve(self, rhs: np.ndarray) -> np.ndarray:
+    r = _native.as_f64(rhs)
+    out = np.empty_like(r)
+    _native.check(self._lib, self._lib.qtqp_sparse_solve(self._h, _native.dptr(r),
+                                                         _native.dptr(out)))
+    return out
+
+  def __matmul__(self, x: np.ndarray) -> np.ndarray:
+    v = _native.as_f64(x)
+    out = np.empty_like(v)
+    _native.check(self._lib, self._lib.qtqp_sparse_matvec(self._h, _native.dptr(v),
+                                                          _native.dptr(out)))
+    return out
+
+  def format(self) -> Literal["csc"]:
+    return "csc"
+
+  def info(self) -> dict[str, Any]:
+    inf = _native.SparseInfo()
+    _native.check(self._lib, self._lib.qtqp_sparse_get_info(self._h, _native.ctypes.byref(inf)))
+    return {k: getattr(inf, k) for k, _ in inf._fields_}
+
+  def free(self) -> None:
+    self._release()
+
+  def _release(self) -> None:
+    if getattr(self, "_h", None) is not None:
+      self._lib.qtqp_sparse_free(self._h)
+      self._h = None
+
+  def __del__(self):
+    try:
+      self._release()
+    except Exception:  # pylint: disable=broad-except
+      pass
+
+
+def sparse_analyze(kkt: sp.spmatrix) -> tuple[np.ndarray, dict[str, Any]]:
+  """Symbolic analysis of an upper-triangular KKT pattern on the host only
+  (qtqp_sparse_analyze): the elimination order and the factor's statistics."""
+  lib = _native.sparse_lib()
+  csc = sp.csc_matrix(kkt)
+  csc.sort_indices()
+  ptr, ind = _native.as_i64(csc.indptr), _native.as_i64(csc.indices)
+  perm = np.empty(csc.shape[0], dtype=np.int64)
+  inf = _native.SparseInfo()
+  rc = lib.qtqp_sparse_analyze(csc.shape[0], _native.SPARSE_CSC, _native.iptr(ptr),
+                               _native.iptr(ind), _native.iptr(perm), _native.ctypes.byref(inf))
+  _native.check(lib, rc)
+  return perm, {k: getattr(inf, k) for k, _ in inf._fields_}
+
+
 class DirectKktSolver:
   """Regularised quasi-definite KKT solves with iterative refinement.
 

@@ -12,7 +12,8 @@ from qtqp_amd import _native, batch
 
 INCLUDE = pathlib.Path(__file__).resolve().parents[1] / "include"
 HEADER_TO_LIB = {"qtqp_kkt.h": "libqtqp_kkt.so", "qtqp_ipm.h": "libqtqp_kkt.so",
-                 "qtqp_batch.h": "libqtqp_batch.so", "qtqp_mlp.h": "libqtqp_mlp.so"}
+                 "qtqp_batch.h": "libqtqp_batch.so", "qtqp_mlp.h": "libqtqp_mlp.so",
+                 "qtqp_sparse.h": "libqtqp_sparse.so"}
 
 
 def declared(header: pathlib.Path):
@@ -34,6 +35,7 @@ def test_library_exports_every_declared_symbol(header):
 def test_kkt_bindings_cover_header():
   assert set(declared(INCLUDE / "qtqp_kkt.h")) == set(_native.KKT_SYMBOLS)
   assert set(declared(INCLUDE / "qtqp_ipm.h")) == set(_native.IPM_SYMBOLS)
+  assert set(declared(INCLUDE / "qtqp_sparse.h")) == set(_native.SPARSE_SYMBOLS)
 
 
 def _c_layout(tmp_path, header, struct, fields):
@@ -54,6 +56,7 @@ def _c_layout(tmp_path, header, struct, fields):
     (_native.IpmRecord, "qtqp_ipm_record", "qtqp_ipm.h"),
     (_native.IrStats, "qtqp_ir_stats", "qtqp_kkt.h"),
     (_native.KktInfo, "qtqp_kkt_info", "qtqp_kkt.h"),
+    (_native.SparseInfo, "qtqp_sparse_info", "qtqp_sparse.h"),
     (batch.BatchOpts, "qtqp_batch_opts", "qtqp_batch.h"),
     (batch.BatchCounters, "qtqp_batch_counters", "qtqp_batch.h")])
 def test_ctypes_structs_match_c_layout(tmp_path, cls, struct, header):

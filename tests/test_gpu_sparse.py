@@ -1,0 +1,216 @@
+"""GPU parity of the sparse KKT engine (supernodal LDL' in HIP) against the
+CPU oracle.
+
+Bars: one backend solve matches numpy.linalg.solve of the assembled K_reg to
+1e-10 relative and the product matches K_reg @ x to 1e-12 (the reference's
+backend contract, tests/test_qtqp.py:740-819); refined KKT solves satisfy the
+block equations to 1e-10; end-to-end QTQP solves with HIP_SPARSE reach the
+oracle's status with x/y/s within 1e-6 relative (1e-4 where the optimum is not
+unique), the same bars as the dense backends.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import qtqp_amd
+from qtqp_amd import direct
+from oracle import qtqp_oracle as O
+import golden_cases
+import qp_generators as G
+
+pytestmark = pytest.mark.gpu
+
+SPARSE = qtqp_amd.LinearSolver.HIP_SPARSE
+TRAJ = golden_cases.load_trajectories()
+
+
+def _rel(u, v):
+  return float(np.max(np.abs(u - v)) / max(1.0, np.max(np.abs(v))))
+
+
+def _scaffold(a, p):
+  m, n = a.shape
+  return sp.bmat([[sp.triu(p, format="csc") + sp.eye(n, format="csc"), sp.csc_matrix(a).T],
+                  [None, sp.eye(m, format="csc")]], format="csc")
+
+
+def _reg_diag(p, m, z, rng, mu=0.3):
+  n = p.shape[0]
+  s, y = rng.uniform(0.01, 2.0, m), rng.uniform(0.01, 2.0, m)
+  d = np.concatenate([np.asarray(p.diagonal()) + mu, np.full(z, mu), s[z:] / y[z:] + mu])
+  d = np.maximum(d, 1e-8)
+  d[n:] *= -1.0
+  return d
+
+
+def _full(kkt, diag):
+  u = sp.triu(kkt, k=1)
+  return (u + u.T).toarray() + np.diag(diag)
+
+
+@pytest.mark.parametrize("mnz,density,seed", [((6, 4, 1), 0.5, 0), ((150, 100, 10), 0.1, 1),
+                                              ((400, 300, 20), 0.05, 2),
+                                              ((700, 330, 40), 0.1, 3)])
+def test_backend_solve_and_matvec_match_numpy(require_gpu, mnz, density, seed):
+  m, n, z = mnz
+  a, b, c, p = G.gen_feasible(m, n, z, seed, density=density)
+  kkt = _scaffold(a, sp.csc_matrix(p))
+  rng = np.random.default_rng(seed)
+  be = direct.HipSparseSolver()
+  be.set_dims(n, m, z)
+  be.set_kkt(kkt.copy())
+  for rep in range(2):  # a second factorisation of new values on the same pattern
+    diag = _reg_diag(sp.csc_matrix(p), m, z, rng, mu=0.3 / (rep + 1))
+    be.update_diag(diag)
+    be.factorize()
+    k = _full(kkt, diag)
+    rhs = rng.normal(size=n + m)
+    got = be.solve(rhs)
+    want = np.linalg.solve(k, rhs)
+    assert _rel(got, want) < 1e-10
+    x = rng.normal(size=n + m)
+    assert _rel(be @ x, k @ x) < 1e-12
+  info = be.info()
+  assert info["factorizations"] == 2 and info["solves"] == 2
+  assert info["nnz_l"] >= kkt.nnz
+  be.free()
+  be.free()  # idempotent
+
+
+@pytest.mark.parametrize("pattern", ["tridiagonal", "arrow", "dense", "block_diagonal"])
+def test_structured_patterns(require_gpu, pattern):
+  """Long chains (one supernode per level), a hub, a dense matrix split into
+  64-column supernodes, and independent blocks (many supernodes per level,
+  several updating one ancestor only through the hub-free blocks)."""
+  rng = np.random.default_rng(7)
+  n = {"tridiagonal": 3000, "arrow": 500, "dense": 300, "block_diagonal": 1200}[pattern]
+  if pattern == "tridiagonal":
+    up = sp.diags([rng.normal(size=n - 1)], [1])
+  elif pattern == "arrow":
+    up = sp.csc_matrix((rng.normal(size=n - 1), (np.zeros(n - 1, int), np.arange(1, n))),
+                       shape=(n, n))
+  elif pattern == "dense":
+    up = sp.csc_matrix(np.triu(rng.normal(size=(n, n)), 1))
+  else:
+    blocks = [sp.csc_matrix(np.triu(rng.normal(size=(b, b)), 1)) for b in (5, 17, 64, 65, 129)
+              for _ in range(4)]
+    up = sp.block_diag(blocks, format="csc")
+    n = up.shape[0]
+  # quasi-definite: positive first half, negative second half, dominant enough
+  sign = np.where(np.arange(n) < n // 2, 1.0, -1.0)
+  diag = sign * (np.abs(up).sum(axis=0).A1 + np.abs(up).sum(axis=1).A1 + 1.0)
+  kkt = sp.csc_matrix(up + sp.eye(n))
+  be = direct.HipSparseSolver()
+  be.set_kkt(kkt)
+  be.update_diag(diag)
+  be.factorize()
+  k = _full(kkt, diag)
+  rhs = rng.normal(size=n)
+  assert _rel(be.solve(rhs), np.linalg.solve(k, rhs)) < 1e-10
+  x = rng.normal(size=n)
+  assert _rel(be @ x, k @ x) < 1e-12
+  be.free()
+
+
+@pytest.mark.parametrize("mnz", [(6, 4, 1), (150, 100, 10), (700, 330, 40)])
+def test_kkt_block_equations(require_gpu, mnz):
+  m, n, z = mnz
+  a, b, c, p = G.gen_feasible(m, n, z, 11, density=0.1)
+  a, p = sp.csc_matrix(a), sp.csc_matrix(p)
+  rng = np.random.default_rng(5)
+  mu = 0.4
+  s, y = rng.uniform(size=m), rng.uniform(size=m)
+  s[:z] = 0.0
+  kkt = direct.DirectKktSolver(a=a, p=p, z=z, min_static_regularization=1e-8,
+                               max_iterative_refinement_steps=50, atol=1e-12, rtol=1e-12,
+                               solver=direct.HipSparseSolver())
+  kkt.update(mu, s, y)
+  rhs = rng.normal(size=n + m)
+  sol, stats = kkt.solve(rhs, np.zeros(n + m))
+  x, yy = sol[:n], sol[n:]
+  d = np.concatenate([np.zeros(z), s[z:] / y[z:]])
+  assert np.max(np.abs(p @ x + mu * x + a.T @ yy - rhs[:n])) < 1e-10
+  assert np.max(np.abs(-(a @ x) + (d + mu) * yy - rhs[n:])) < 1e-10
+  assert stats["status"] == "converged"
+  kkt.free()
+
+
+@pytest.mark.parametrize("key", sorted(TRAJ))
+def test_end_to_end_matches_reference_fixtures(require_gpu, key):
+  case = TRAJ[key]
+  a, b, c, z, p = golden_cases.inputs(case["spec"])
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+      linear_solver=SPARSE, equilibrate=case["equilibrate"], verbose=False, collect_stats=True)
+  assert sol.status.value == case["status"]
+  assert abs(len(sol.stats) - case["iters"]) <= 1
+  ref = O.solve_qp(a, b, c, z, p, equilibrate=case["equilibrate"])
+  unique = case["spec"]["kind"] in ("readme", "dense", "mpc") or case["status"] != "solved"
+  if not unique:
+    # A singular P leaves x free along a null-space direction, where roundoff
+    # in another factor order settles elsewhere: the bar is optimality (the
+    # solve tolerances' KKT conditions) and the oracle's objective.
+    G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, z)
+    obj = lambda x: c @ x + 0.5 * x @ p @ x
+    assert abs(obj(sol.x) - obj(ref["x"])) <= 1e-6 * max(1.0, abs(obj(ref["x"])))
+    return
+  for k in "xys":
+    got, want = getattr(sol, k), ref[k]
+    assert np.array_equal(np.isnan(got), np.isnan(want)), k
+    ok = ~np.isnan(want)
+    if ok.any():
+      assert _rel(got[ok], want[ok]) < 1e-6, k
+
+
+@pytest.mark.parametrize("kind", ["feasible", "infeasible", "unbounded"])
+@pytest.mark.parametrize("seed", [42, 43])
+def test_reference_problem_families(require_gpu, kind, seed):
+  m, n, z = 150, 100, 10
+  gen = {"feasible": G.gen_feasible, "infeasible": G.gen_infeasible,
+         "unbounded": G.gen_unbounded}[kind]
+  a, b, c, p = gen(m, n, z, seed)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+      linear_solver=SPARSE, verbose=False)
+  if kind == "feasible":
+    G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, z)
+  elif kind == "infeasible":
+    assert sol.status == qtqp_amd.SolutionStatus.INFEASIBLE
+    G.check_infeasible(sol.x, sol.y, sol.s, a, b, z)
+  else:
+    assert sol.status == qtqp_amd.SolutionStatus.UNBOUNDED
+    G.check_unbounded(sol.x, sol.y, sol.s, a, c, p, z)
+
+
+def test_larger_sparse_qp_matches_oracle(require_gpu):
+  """A sparse QP of the reference's family at m=3000, n=2000 (A density 0.01)."""
+  m, n, z = 3000, 2000, 100
+  a, b, c, p = G.gen_feasible(m, n, z, 9, density=0.01)
+  sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+      linear_solver=SPARSE, verbose=False)
+  assert sol.status == qtqp_amd.SolutionStatus.SOLVED
+  G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, z)
+
+
+def test_zero_pivot_raises_linalg_error(require_gpu):
+  be = direct.HipSparseSolver()
+  be.set_kkt(sp.csc_matrix(sp.eye(3)))
+  be.update_diag(np.array([1.0, 0.0, -1.0]))
+  with pytest.raises(np.linalg.LinAlgError):
+    be.factorize()
+  with pytest.raises(RuntimeError):
+    be.solve(np.ones(3))  # no valid factor
+  be.update_diag(np.array([1.0, 2.0, -1.0]))
+  be.factorize()
+  assert np.allclose(be.solve(np.ones(3)), [1.0, 0.5, -1.0], rtol=0, atol=1e-15)
+  be.free()
+
+
+def test_protocol_errors(require_gpu):
+  be = direct.HipSparseSolver()
+  with pytest.raises(ValueError, match="upper triangle"):
+    be.set_kkt(sp.csc_matrix(np.ones((3, 3))))
+  be.set_kkt(sp.csc_matrix(np.triu(np.ones((3, 3)))))
+  with pytest.raises(RuntimeError):
+    be.solve(np.ones(3))  # before factorize
+  with pytest.raises(ValueError):
+    be.update_diag(np.array([1.0, np.nan, -1.0]))
+  be.free()
