@@ -325,6 +325,86 @@ def single_legs(problems, qtqp_oracle):
   return out
 
 
+def sparse_leg(m=6000, n=4000, z=1000, density=0.002, seed=0, reps=10, cpu=True):
+  """The sparse KKT backend (HIP_SPARSE, supernodal LDL') on the reference's sparse
+  QP family (tests/test_qtqp.py:211-235) at n=4000, m=6000, z=1000.
+
+  Times, with HIP events on the backend's stream, the numeric factorisation and
+  one solve of the KKT matrix K_reg of a mid-solve IPM iterate (mu = 0.3, s/y
+  drawn in [0.005, 200]), then one whole QTQP.solve with HIP_SPARSE.  The CPU
+  comparison is scipy.sparse.linalg.factorized on the full symmetric K_reg, which
+  is what the reference's SCIPY backend calls (solvers_sparse.py:96-121)."""
+  import scipy.sparse as sp
+  import scipy.sparse.linalg as spla
+  import qtqp_amd
+  from qtqp_amd import direct, problems
+  qp = problems.random_sparse_qp(m, n, z, seed=seed, density=density)
+  kkt = sp.bmat([[sp.triu(qp.p, format="csc") + sp.eye(n, format="csc"), qp.a.T],
+                 [None, sp.eye(m, format="csc")]], format="csc")
+  rng = np.random.default_rng(seed + 1)
+  mu = 0.3
+  s_over_y = rng.uniform(0.1, 2.0, m) / rng.uniform(0.01, 2.0, m)
+  diag = np.concatenate([np.asarray(qp.p.diagonal()) + mu, np.full(z, mu), s_over_y[z:] + mu])
+  diag[n:] *= -1.0
+  log(f"sparse KKT: analysis of N={n + m}")
+  t0 = time.perf_counter()
+  be = direct.HipSparseSolver()
+  be.set_kkt(kkt)
+  create_s = time.perf_counter() - t0
+  be.update_diag(diag)
+  be.factorize()  # first use (code objects) outside the timings
+  rhs = rng.normal(size=n + m)
+  sol = be.solve(rhs)
+  info0 = be.info()
+  for _ in range(reps):
+    be.factorize()
+  for _ in range(reps):
+    sol = be.solve(rhs)
+  info = be.info()
+  fac_ms = (info["factor_ms"] - info0["factor_ms"]) / reps
+  sol_ms = (info["solve_ms"] - info0["solve_ms"]) / reps
+  u = sp.triu(kkt, k=1)
+  full = sp.csc_matrix(u + u.T + sp.diags(diag))
+  res = float(np.max(np.abs(full @ sol - rhs)) / np.max(np.abs(rhs)))
+  be.free()
+  flops = info["factor_flops"]
+  # L's algorithmic bytes for one solve: the factor read twice (forward, backward)
+  solve_bytes = 2.0 * 8.0 * info["nnz_l"]
+  out = {"workload": f"reference sparse QP family, n={n} m={m} z={z}, A density {density}, "
+                     f"seed {seed}",
+         "N": n + m, "nnz_kkt_upper": info["nnz_kkt"], "nnz_l": info["nnz_l"],
+         "supernodes": info["supernodes"], "levels": info["levels"],
+         "update_tiles": info["update_tiles"], "analysis_ms": info["analyze_ms"],
+         "create_s": create_s,
+         "factor_ms": fac_ms, "factor_flops": flops,
+         "factor_tflops": flops / (fac_ms / 1e3) / 1e12,
+         "factor_frac_of_fp64_peak": flops / (fac_ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+         "solve_ms": sol_ms, "solve_gbs": solve_bytes / (sol_ms / 1e3) / 1e9,
+         "solve_frac_of_hbm_peak": solve_bytes / (sol_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+         "solve_rel_residual": res}
+  log("sparse KKT: QTQP.solve with HIP_SPARSE")
+  t0 = time.perf_counter()
+  qs = qtqp_amd.QTQP(a=qp.a, b=qp.b, c=qp.c, z=z, p=qp.p).solve(
+      linear_solver=qtqp_amd.LinearSolver.HIP_SPARSE, verbose=False)
+  out["qtqp_solve"] = {"status": qs.status.value, "iterations": len(qs.stats) if qs.stats else None,
+                       "wall_s": time.perf_counter() - t0,
+                       "rel_kkt_residual": kkt_rel_residual(qs.x, qs.y, qs.s, qp.a, qp.b, qp.c,
+                                                            qp.p, z)}
+  if cpu:
+    log("sparse KKT: scipy.sparse.linalg.factorized on the CPU")
+    t0 = time.perf_counter()
+    f = spla.factorized(full)
+    t_fac = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xc = f(rhs)
+    t_sol = time.perf_counter() - t0
+    out["cpu_scipy_factorized"] = {
+        "factor_ms": t_fac * 1e3, "solve_ms": t_sol * 1e3, "cores": 1,
+        "max_rel_diff_vs_gpu": float(np.max(np.abs(xc - sol)) / np.max(np.abs(xc))),
+        "factor_speedup_gpu": t_fac * 1e3 / fac_ms}
+  return out
+
+
 def dropin_fp64(qps, reps=1):
   """The drop-in path: qtqp_amd.QTQP.solve with the HIP_DENSE_FP64 backend, one QP per
   call (the device-resident iteration loop)."""
@@ -390,6 +470,10 @@ def main():
   ap.add_argument("--calibrate-copy", action="store_true")
   ap.add_argument("--no-plain", action="store_true")
   ap.add_argument("--single-only", action="store_true")
+  ap.add_argument("--no-sparse", action="store_true")
+  ap.add_argument("--no-sparse-cpu", action="store_true")
+  ap.add_argument("--sparse-only", action="store_true",
+                  help="only the sparse KKT backend leg (for its kernel profiles)")
   ap.add_argument("--oracle-sample", type=int, default=64)
   ap.add_argument("--profile-single", choices=("mixed", "fp64"),
                   help="only solve the n=4000 m=6000 z=1000 dense QP once through QTQP.solve "
@@ -418,6 +502,9 @@ def main():
           flush=True)
     return
 
+  if args.sparse_only:
+    print(json.dumps(sparse_leg(cpu=not args.no_sparse_cpu)), flush=True)
+    return
   from qtqp_amd import batch, problems
   from oracle import qtqp_oracle
   if args.single_only:
@@ -565,6 +652,8 @@ def main():
   bq.free()
   if not args.no_single:
     line.update(single_legs(problems, qtqp_oracle))
+  if not args.no_sparse:
+    line["sparse_kkt"] = sparse_leg(cpu=not args.no_sparse_cpu)
   if not args.no_ws:
     log("warm start leg")
     line["warm_start"] = warm_start_leg(args.ws_train, args.ws_eval, args.ws_steps)

@@ -93,6 +93,7 @@ struct Symbolic {
   std::vector<int64_t> tile_off;          // L+1
   std::vector<int64_t> dblk_off;          // S+1: pivot blocks, c*c each
   std::vector<int32_t> pjobs, sjobs;      // (supernode, a0) per panel / solve job
+  std::vector<int32_t> contig;            // S: rows are consecutive indices
   std::vector<int64_t> pjob_off, sjob_off;  // L+1
 };
 
@@ -338,7 +339,7 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
           S.tiles.insert(S.tiles.end(), {s, (int32_t)(bi * kTile), (int32_t)(bj * kTile)});
       for (int64_t a0 = 0; a0 == 0 || a0 < below; a0 += kPanelRows)
         S.pjobs.insert(S.pjobs.end(), {s, (int32_t)a0});
-      for (int64_t a0 = 0; a0 < below; a0 += kSolveRows)
+      for (int64_t a0 = 0; a0 == 0 || a0 < below; a0 += kSolveRows)
         S.sjobs.insert(S.sjobs.end(), {s, (int32_t)a0});
     }
     S.level_off.push_back((int64_t)S.level_sn.size());
@@ -346,6 +347,10 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
     S.pjob_off.push_back((int64_t)S.pjobs.size() / 2);
     S.sjob_off.push_back((int64_t)S.sjobs.size() / 2);
   }
+  S.contig.assign(ns, 0);
+  for (int64_t s = 0; s < ns; ++s)
+    S.contig[s] = S.rows[S.sn_rows[s + 1] - 1] - S.rows[S.sn_rows[s]] ==
+                  S.sn_rows[s + 1] - S.sn_rows[s] - 1;
   S.dblk_off.assign(ns + 1, 0);
   for (int64_t s = 0; s < ns; ++s) {
     const int64_t c = S.sn_first[s + 1] - S.sn_first[s];
@@ -395,6 +400,7 @@ struct Tables {
   const int32_t* tiles;    // (supernode, a0, b0) per Schur-update tile
   const int32_t* pjobs;    // (supernode, a0) per panel job: 256 rows below the block
   const int32_t* sjobs;    // (supernode, a0) per solve job: 64 rows below the block
+  const int32_t* contig;   // S: 1 when a supernode's rows are consecutive indices
 };
 
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
@@ -412,21 +418,32 @@ __global__ void k_load_diag(double* __restrict__ lx, const int64_t* __restrict__
 
 // One row of L below a supernode's pivot block, left-looking along the row:
 // w_k = a_ik - sum_{j<k} w_j l_kj, l_ik = w_k / d_k (w_j = l_ij d_j stays in
-// registers; a_ik is loaded only at step k).  Dl holds the factored pivot
-// block (l_kj below its diagonal) and rd the reciprocal pivots.
+// registers).  The row is loaded 16 columns at a time with unconditional,
+// clamped loads, so one memory round trip serves 16 steps.  Dl holds the
+// factored pivot block (l_kj below its diagonal) and rd the reciprocal pivots.
 template <int CM>
 __device__ __forceinline__ void panel_row(const double (*Dl)[kMaxCols + 1], const double* rd,
                                           double* __restrict__ L, int r, int c, int i) {
+  constexpr int B = CM < 16 ? CM : 16;
   double w[CM];
   double* p = L + i;
 #pragma unroll
-  for (int k = 0; k < CM; ++k) {
-    if (k < c) {  // uniform across the workgroup
-      double v = p[(int64_t)k * r];
+  for (int kb = 0; kb < CM; kb += B) {
+    double a[B];
 #pragma unroll
-      for (int j = 0; j < k; ++j) v -= w[j] * Dl[k][j];
-      w[k] = v;
-      p[(int64_t)k * r] = v * rd[k];
+    for (int q = 0; q < B; ++q) a[q] = p[(int64_t)min(kb + q, c - 1) * r];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int k = kb + q;
+      if (k < c) {  // uniform across the workgroup
+        double v = a[q], v2 = 0.0;  // two partial sums halve the FMA chain
+#pragma unroll
+        for (int j = 0; j + 1 < k; j += 2) v -= w[j] * Dl[k][j], v2 += w[j + 1] * Dl[k][j + 1];
+        if (k % 2) v -= w[k - 1] * Dl[k][k - 1];
+        v -= v2;
+        w[k] = v;
+        p[(int64_t)k * r] = v * rd[k];
+      }
     }
   }
 }
@@ -457,7 +474,8 @@ __global__ void __launch_bounds__(kThreads) k_panel(Tables T, double* __restrict
   }
   if (tid < kMaxCols) rd[tid] = 0.0;
   __syncthreads();
-  // Right-looking LDL' of the block: thread i owns row i.
+  // Right-looking LDL' of the block; each step's rank-1 update is spread over
+  // all (i, j) pairs of the trailing triangle.
   for (int k = 0; k < c; ++k) {
     const double d = Dl[k][k];
     if (!(d != 0.0) || !isfinite(d)) {  // uniform across the workgroup
@@ -465,13 +483,22 @@ __global__ void __launch_bounds__(kThreads) k_panel(Tables T, double* __restrict
       return;
     }
     const double rk = 1.0 / d;
-    double v = 0.0;
-    if (tid > k && tid < c) {
-      v = Dl[tid][k];
-      for (int j = k + 1; j <= tid; ++j) Dl[tid][j] -= v * Dl[j][k] * rk;
+    // 16 x 16 threads over the trailing triangle, 4 x 4 entries each
+    const int ti = tid / 16, tj = tid % 16;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = k + 1 + ti + 16 * u;
+      if (i < c) {
+        const double vi = Dl[i][k] * rk;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int j = k + 1 + tj + 16 * v;
+          if (j <= i) Dl[i][j] -= vi * Dl[j][k];
+        }
+      }
     }
     __syncthreads();
-    if (tid > k && tid < c) Dl[tid][k] = v * rk;
+    if (tid > k && tid < c) Dl[tid][k] *= rk;
     if (tid == k) rd[k] = rk;
     __syncthreads();
   }
@@ -506,7 +533,8 @@ __device__ __forceinline__ int find_row(const int32_t* __restrict__ rows, int lo
 // entries (rows[c+a], rows[c+b]).  Each thread owns a 4x4 block of the tile.
 __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restrict__ lx,
                                                      const double* __restrict__ dblk,
-                                                     int64_t tile_begin, int* __restrict__ bad) {
+                                                     int64_t tile_begin, int exclusive,
+                                                     int* __restrict__ bad) {
   constexpr int kChunk = 32;  // pivot columns staged at a time (33 KB of LDS)
   __shared__ double As[kChunk][kTile + 1];
   __shared__ double Bs[kChunk][kTile + 1];
@@ -566,41 +594,76 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
       const int a = a0 + ty + 16 * u;
       if (a >= nb || a < b) continue;
       const int32_t row = R[c + a];
-      if (pos_t[u] != t) pos[u] = find_row(tR, lc, tr, row), pos_t[u] = t;
-      if (pos[u] < tr && tR[pos[u]] == row) unsafeAtomicAdd(dst + pos[u], -acc[u][v]);
-      else atomicCAS(bad, 0, -1);  // the symbolic structure misses a fill entry
+      if (pos_t[u] != t) {
+        pos[u] = T.contig[t] ? row - tR[0] : find_row(tR, lc, tr, row);
+        pos_t[u] = t;
+      }
+      if (pos[u] >= lc && pos[u] < tr && tR[pos[u]] == row) {
+        // One supernode in the level: each target entry has one writer.
+        if (exclusive) dst[pos[u]] -= acc[u][v];
+        else unsafeAtomicAdd(dst + pos[u], -acc[u][v]);
+      } else {
+        atomicCAS(bad, 0, -1);  // the symbolic structure misses a fill entry
+      }
     }
   }
 }
 
-// Forward substitution, pivot blocks of one level (one 64-thread workgroup
-// per supernode): y = L11^{-1} b in place in x, and z = D^{-1} y into xo.
-__global__ void __launch_bounds__(64) k_fwd_diag(Tables T, const double* __restrict__ dblk,
-                                                 int64_t lvl_begin, double* __restrict__ x,
-                                                 double* __restrict__ xo) {
-  __shared__ double y[kMaxCols];
-  const int32_t s = T.level_sn[lvl_begin + blockIdx.x];
-  const int32_t f = T.first[s];
-  const int c = T.first[s + 1] - f;
-  const double* D = dblk + T.dblk_off[s];
-  const int tid = threadIdx.x;
-  if (tid < c) y[tid] = x[f + tid];
-  __syncthreads();
-  for (int k = 0; k < c - 1; ++k) {
-    if (tid > k && tid < c) y[tid] -= D[k * c + tid] * y[k];
-    __syncthreads();
-  }
-  if (tid < c) {
-    x[f + tid] = y[tid];
-    xo[f + tid] = y[tid] / D[tid * (c + 1)];
-  }
+__device__ __forceinline__ double bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
 }
 
-// Forward substitution, rows below the pivot blocks: 64 rows per job, four
-// partial sums per row over the pivot columns, then b[rows] -= L21 y with
-// atomics (several supernodes of a level can update one ancestor row).
-__global__ void __launch_bounds__(kThreads) k_fwd_off(Tables T, const double* __restrict__ lx,
-                                                      int64_t job_begin, double* __restrict__ x) {
+// L11 y = b by one wavefront: lane i holds b_i and row i of the unit lower
+// pivot block (D column-major, c x c); each solved y_k is broadcast with
+// readlane.  Returns y_i (0 for lanes >= c).
+__device__ __forceinline__ double wave_lower_solve(const double* __restrict__ D, int c, int lane,
+                                                   double b) {
+  double row[kMaxCols];
+#pragma unroll
+  for (int k = 0; k < kMaxCols; ++k)
+    row[k] = (k < c && lane < c) ? D[k * c + lane] : 0.0;
+  double y = lane < c ? b : 0.0;
+#pragma unroll
+  for (int k = 0; k < kMaxCols - 1; ++k) {
+    if (k + 1 < c) {  // uniform
+      const double yk = bcast(y, k);
+      if (lane > k) y -= row[k] * yk;
+    }
+  }
+  return y;
+}
+
+// L11' x = v by one wavefront: lane j holds v_j and column j of the pivot
+// block; x_i is broadcast from the last row up.  Returns x_j.
+__device__ __forceinline__ double wave_upper_solve(const double* __restrict__ D, int c, int lane,
+                                                   double v) {
+  double col[kMaxCols];
+#pragma unroll
+  for (int k = 0; k < kMaxCols; ++k)
+    col[k] = (k < c && lane < c) ? D[lane * c + k] : 0.0;
+  double x = lane < c ? v : 0.0;
+#pragma unroll
+  for (int i = kMaxCols - 1; i > 0; --i) {
+    if (i < c) {  // uniform
+      const double xi = bcast(x, i);
+      if (lane < i) x -= col[i] * xi;
+    }
+  }
+  return x;
+}
+
+// Forward substitution for one level, one job per 64 rows below a pivot
+// block (a single job when there are none): every job solves L11 y = b for
+// its supernode in LDS (b unchanged in x during the launch), the job with
+// a0 == 0 stores z = D^{-1} y in xo, and the job's rows update
+// b[rows] -= L21 y with atomics (supernodes of one level can share an
+// ancestor row).
+__global__ void __launch_bounds__(kThreads) k_fwd(Tables T, const double* __restrict__ lx,
+                                                  const double* __restrict__ dblk,
+                                                  int64_t job_begin, double* __restrict__ x,
+                                                  double* __restrict__ xo) {
   __shared__ double y[kMaxCols];
   __shared__ double part[4][kSolveRows];
   const int32_t* job = T.sjobs + 2 * (job_begin + blockIdx.x);
@@ -611,8 +674,15 @@ __global__ void __launch_bounds__(kThreads) k_fwd_off(Tables T, const double* __
   const int64_t roff = T.rows_off[s];
   const int r = (int)(T.rows_off[s + 1] - roff);
   const double* L = lx + T.lx_off[s];
+  const double* D = dblk + T.dblk_off[s];
   const int tid = threadIdx.x, lane = tid % kSolveRows, g = tid / kSolveRows;
-  if (tid < c) y[tid] = x[f + tid];
+  if (g == 0) {
+    const double yi = wave_lower_solve(D, c, lane, lane < c ? x[f + lane] : 0.0);
+    if (lane < c) {
+      y[lane] = yi;
+      if (a0 == 0) xo[f + lane] = yi / D[lane * (c + 1)];
+    }
+  }
   __syncthreads();
   const int a = a0 + lane;
   double t = 0.0;
@@ -660,23 +730,18 @@ __global__ void __launch_bounds__(kThreads) k_bwd_off(Tables T, const double* __
   if (tid < c) unsafeAtomicAdd(tb + f + tid, part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
 }
 
-// Backward substitution, pivot blocks of one level: x = L11^{-T} (z - t).
+// Backward substitution, pivot blocks of one level: x = L11^{-T} (z - t),
+// one wavefront per supernode.
 __global__ void __launch_bounds__(64) k_bwd_diag(Tables T, const double* __restrict__ dblk,
                                                  int64_t lvl_begin, const double* __restrict__ tb,
                                                  double* __restrict__ xo) {
-  __shared__ double xs[kMaxCols];
   const int32_t s = T.level_sn[lvl_begin + blockIdx.x];
   const int32_t f = T.first[s];
   const int c = T.first[s + 1] - f;
-  const double* D = dblk + T.dblk_off[s];
-  const int tid = threadIdx.x;
-  if (tid < c) xs[tid] = xo[f + tid] - tb[f + tid];
-  __syncthreads();
-  for (int i = c - 1; i > 0; --i) {
-    if (tid < i) xs[tid] -= D[tid * c + i] * xs[i];
-    __syncthreads();
-  }
-  if (tid < c) xo[f + tid] = xs[tid];
+  const int lane = threadIdx.x;
+  const double v = lane < c ? xo[f + lane] - tb[f + lane] : 0.0;
+  const double xj = wave_upper_solve(dblk + T.dblk_off[s], c, lane, v);
+  if (lane < c) xo[f + lane] = xj;
 }
 
 __global__ void k_zero(double* __restrict__ x, int64_t n) {
@@ -807,7 +872,7 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&h->ev0));
   HIP_TRY(hipEventCreate(&h->ev1));
-  int32_t *first, *rows, *col_sn, *level_sn, *tiles, *pjobs, *sjobs;
+  int32_t *first, *rows, *col_sn, *level_sn, *tiles, *pjobs, *sjobs, *contig;
   int64_t *rows_off, *lx_off, *dblk_off;
   if (int rc = h->upload(&first, S.sn_first)) return rc;
   if (int rc = h->upload(&rows_off, S.sn_rows)) return rc;
@@ -819,7 +884,9 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   if (int rc = h->upload(&dblk_off, S.dblk_off)) return rc;
   if (int rc = h->upload(&pjobs, S.pjobs)) return rc;
   if (int rc = h->upload(&sjobs, S.sjobs)) return rc;
-  h->T = Tables{first, rows_off, lx_off, dblk_off, rows, col_sn, level_sn, tiles, pjobs, sjobs};
+  if (int rc = h->upload(&contig, S.contig)) return rc;
+  h->T = Tables{first,    rows_off, lx_off, dblk_off, rows,  col_sn,
+                level_sn, tiles,    pjobs,  sjobs,    contig};
   if (int rc = h->alloc(&h->dblk, S.dblk_off.back())) return rc;
   if (int rc = h->alloc(&h->tb, n)) return rc;
   if (int rc = h->upload(&h->lx0, lx0)) return rc;
@@ -909,8 +976,9 @@ int qtqp_sparse_factorize(qtqp_sparse* h) {
                                                       h->bad, h->perm);
     const int64_t nt = S.tile_off[l + 1] - S.tile_off[l];
     if (nt > 0)
-      k_update<<<(unsigned)nt, kThreads, 0, h->stream>>>(h->T, h->lx, h->dblk, S.tile_off[l],
-                                                         h->bad);
+      k_update<<<(unsigned)nt, kThreads, 0, h->stream>>>(
+          h->T, h->lx, h->dblk, S.tile_off[l],
+          (int)(S.level_off[l + 1] - S.level_off[l] == 1), h->bad);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
@@ -936,18 +1004,12 @@ int qtqp_sparse_solve(qtqp_sparse* h, const double* rhs, double* out) {
   k_gather<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xp, h->xin, h->perm, n);
   k_zero<<<blocks_for(n), kThreads, 0, h->stream>>>(h->tb, n);
   const int64_t nlev = (int64_t)S.level_off.size() - 1;
-  for (int64_t l = 0; l < nlev; ++l) {
-    k_fwd_diag<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), 64, 0, h->stream>>>(
-        h->T, h->dblk, S.level_off[l], h->xp, h->xo);
-    const int64_t nj = S.sjob_off[l + 1] - S.sjob_off[l];
-    if (nj > 0)
-      k_fwd_off<<<(unsigned)nj, kThreads, 0, h->stream>>>(h->T, h->lx, S.sjob_off[l], h->xp);
-  }
+  for (int64_t l = 0; l < nlev; ++l)
+    k_fwd<<<(unsigned)(S.sjob_off[l + 1] - S.sjob_off[l]), kThreads, 0, h->stream>>>(
+        h->T, h->lx, h->dblk, S.sjob_off[l], h->xp, h->xo);
   for (int64_t l = nlev - 1; l >= 0; --l) {
-    const int64_t nj = S.sjob_off[l + 1] - S.sjob_off[l];
-    if (nj > 0)
-      k_bwd_off<<<(unsigned)nj, kThreads, 0, h->stream>>>(h->T, h->lx, S.sjob_off[l], h->xo,
-                                                          h->tb);
+    k_bwd_off<<<(unsigned)(S.sjob_off[l + 1] - S.sjob_off[l]), kThreads, 0, h->stream>>>(
+        h->T, h->lx, S.sjob_off[l], h->xo, h->tb);
     k_bwd_diag<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), 64, 0, h->stream>>>(
         h->T, h->dblk, S.level_off[l], h->tb, h->xo);
   }

@@ -146,3 +146,44 @@ def parametric_mpc_family(family_seed: int = 7, horizon: int = 32, nx: int = 8,
 
 def parametric_mpc_x0(count: int, seed: int, nx: int = 8, std: float = 0.5) -> np.ndarray:
   return std * np.random.default_rng(seed).standard_normal((count, nx))
+
+
+@dataclasses.dataclass
+class SparseQP:
+  """A QP with scipy CSC data (``a`` m x n, ``p`` n x n) and a planted KKT point."""
+  a: "scipy.sparse.csc_matrix"
+  b: np.ndarray
+  c: np.ndarray
+  z: int
+  p: "scipy.sparse.csc_matrix"
+  x0: np.ndarray
+  y0: np.ndarray
+  s0: np.ndarray
+
+  @property
+  def n(self) -> int:
+    return self.a.shape[1]
+
+  @property
+  def m(self) -> int:
+    return self.a.shape[0]
+
+
+def random_sparse_qp(m: int, n: int, z: int, seed: int = 0, density: float = 0.1) -> SparseQP:
+  """The reference test suite's feasible sparse family (tests/test_qtqp.py:211-235),
+  kept sparse: A ~ N(0,1) entries at ``density``, P = R'R / 100 with R ~ N(0,1) at
+  density 0.01, and b, c built from a random primal-dual pair (x, y, s) that is
+  complementary on the inequality rows."""
+  import scipy.sparse as sp
+  rng = np.random.default_rng(seed)
+  w = rng.normal(size=m)
+  x = rng.normal(size=n)
+  y = w.copy()
+  y[z:] = 0.5 * (w[z:] + np.abs(w[z:]))
+  s = y - w
+  a = sp.random(m, n, density=density, format="csc", random_state=rng,
+                data_rvs=lambda k: rng.normal(size=k))
+  r = sp.random(n, n, density=0.01, format="csc", random_state=rng,
+                data_rvs=lambda k: rng.normal(size=k))
+  p = sp.csc_matrix(r.T @ r * 0.01)
+  return SparseQP(sp.csc_matrix(a), a @ x + s, -(a.T @ y), z, p, x, y, s)
