@@ -385,8 +385,8 @@ def sparse_leg(m=6000, n=4000, z=1000, density=0.002, seed=0, reps=10, cpu=True)
   log("sparse KKT: QTQP.solve with HIP_SPARSE")
   t0 = time.perf_counter()
   qs = qtqp_amd.QTQP(a=qp.a, b=qp.b, c=qp.c, z=z, p=qp.p).solve(
-      linear_solver=qtqp_amd.LinearSolver.HIP_SPARSE, verbose=False)
-  out["qtqp_solve"] = {"status": qs.status.value, "iterations": len(qs.stats) if qs.stats else None,
+      linear_solver=qtqp_amd.LinearSolver.HIP_SPARSE, verbose=False, collect_stats=True)
+  out["qtqp_solve"] = {"status": qs.status.value, "iterations": len(qs.stats),
                        "wall_s": time.perf_counter() - t0,
                        "rel_kkt_residual": kkt_rel_residual(qs.x, qs.y, qs.s, qp.a, qp.b, qp.c,
                                                             qp.p, z)}
