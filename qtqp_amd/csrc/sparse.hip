@@ -139,6 +139,7 @@ std::vector<int32_t> min_degree(int64_t n, const std::vector<std::vector<int32_t
   std::vector<uint64_t> bits((size_t)(n * w), 0);
   std::vector<int64_t> deg(n);
   std::vector<char> done(n, 0);
+  std::vector<int64_t> clique;
   for (int64_t i = 0; i < n; ++i) {
     for (int32_t j : adj[i]) bits[i * w + j / 64] |= 1ull << (j % 64);
     deg[i] = (int64_t)adj[i].size();
@@ -173,6 +174,26 @@ std::vector<int32_t> min_degree(int64_t n, const std::vector<std::vector<int32_t
         deg[u] = d;
       }
     }
+    // v's neighbours now form a clique C.  A neighbour whose adjacency is
+    // exactly C minus itself is simplicial: eliminating it adds no fill, so
+    // it goes next (in index order), at the cost of clearing one bit in the
+    // rest of C instead of a bitset merge.
+    clique.clear();
+    for (int64_t q = 0; q < w; ++q)
+      for (uint64_t word = bv[q]; word; word &= word - 1) clique.push_back(q * 64 + __builtin_ctzll(word));
+    int64_t csize = (int64_t)clique.size();
+    for (int64_t u : clique) {
+      if (deg[u] != csize - 1) continue;
+      done[u] = 1;
+      order.push_back((int32_t)u);
+      --left;
+      --csize;
+      for (int64_t x : clique) {
+        if (done[x]) continue;
+        bits[x * w + u / 64] &= ~(1ull << (u % 64));
+        --deg[x];
+      }
+    }
   }
   return order;
 }
@@ -202,7 +223,6 @@ std::vector<std::vector<int32_t>> permuted_adjacency(const std::vector<std::vect
   for (int64_t k = 0; k < n; ++k) {
     padj[k].reserve(adj[perm[k]].size());
     for (int32_t j : adj[perm[k]]) padj[k].push_back(iperm[j]);
-    std::sort(padj[k].begin(), padj[k].end());
   }
   return padj;
 }
@@ -215,12 +235,13 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
   if (int rc = adjacency(n, format, ptr, ind, adj, S.nnz_kkt)) return rc;
   S.n = n;
 
-  // 1-2. ordering, then a postorder of its elimination tree.
+  // 1-2. ordering, then a postorder of its elimination tree.  A postorder is
+  // an equivalent elimination order: same fill, same tree, relabelled.
   std::vector<int32_t> perm = min_degree(n, adj), iperm(n);
   for (int64_t k = 0; k < n; ++k) iperm[perm[k]] = (int32_t)k;
+  std::vector<std::vector<int32_t>> padj = permuted_adjacency(adj, perm, iperm);
+  std::vector<int32_t> parent = etree(n, padj);
   {
-    const auto padj = permuted_adjacency(adj, perm, iperm);
-    const auto parent = etree(n, padj);
     std::vector<int32_t> head(n, -1), next(n, -1), post;
     post.reserve(n);
     for (int64_t k = n - 1; k >= 0; --k)  // children lists in increasing order
@@ -241,13 +262,21 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
         }
       }
     }
-    std::vector<int32_t> p2(n);
-    for (int64_t k = 0; k < n; ++k) p2[k] = perm[post[k]];
+    std::vector<int32_t> ipost(n), p2(n), parent2(n);
+    for (int64_t k = 0; k < n; ++k) ipost[post[k]] = (int32_t)k;
+    std::vector<std::vector<int32_t>> padj2(n);
+    for (int64_t k = 0; k < n; ++k) {
+      const int32_t old = post[k];
+      p2[k] = perm[old];
+      parent2[k] = parent[old] == -1 ? -1 : ipost[parent[old]];
+      padj2[k].swap(padj[old]);
+      for (int32_t& i : padj2[k]) i = ipost[i];
+    }
     perm.swap(p2);
+    parent.swap(parent2);
+    padj.swap(padj2);
     for (int64_t k = 0; k < n; ++k) iperm[perm[k]] = (int32_t)k;
   }
-  const auto padj = permuted_adjacency(adj, perm, iperm);
-  const auto parent = etree(n, padj);
 
   // 3-4. column structures and supernodes.
   std::vector<int32_t> nchild(n, 0), head(n, -1), next(n, -1);
@@ -271,11 +300,11 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
         if (mark[i] != k) mark[i] = (int32_t)k, s.push_back(i);
       std::vector<int32_t>().swap(live[c]);
     }
-    std::sort(s.begin(), s.end());
     count[k] = (int64_t)s.size() + 1;
     const bool cont = k > 0 && parent[k - 1] == k && count[k - 1] == count[k] + 1 &&
                       k - start < kMaxCols;
-    if (!cont) {
+    if (!cont) {  // only a supernode's first structure needs its rows in order
+      std::sort(s.begin(), s.end());
       start = k;
       if (!S.sn_first.empty()) S.sn_rows.push_back((int64_t)S.rows.size());
       S.sn_first.push_back((int32_t)k);
