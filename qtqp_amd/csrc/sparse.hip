@@ -10,33 +10,34 @@
 // CuDssSolver, solvers_gpu.py:27-115).
 //
 // Host (once per pattern, qtqp_sparse_analyze / create):
-//   1. minimum-degree ordering on a bitset elimination graph; once the
-//      remaining graph is a clique the rest is taken in index order;
-//   2. elimination tree (path-compressed ancestor walk) and a postorder, so
-//      every subtree is a contiguous column range;
+//   1. minimum-degree ordering on a bitset elimination graph; neighbours left
+//      simplicial by an elimination follow it at once, and once the remaining
+//      graph is a clique the rest is taken in index order;
+//   2. elimination tree (path-compressed ancestor walk), relabelled by a
+//      postorder so that every subtree is a contiguous column range;
 //   3. column structures of L by merging each child's structure into its
 //      parent (each structure is freed once its parent has consumed it);
 //   4. supernodes: column j joins j-1's supernode when parent(j-1) = j and
 //      |struct(j-1)| = |struct(j)| + 1 (then struct(j-1) = {j} u struct(j)),
 //      at most kMaxCols columns each;
 //   5. levels of the supernodal tree (a supernode's level is one above its
-//      highest child) and, per level, the list of 64x64 Schur-update tiles.
+//      highest child) and, per level, the job lists of the kernels below.
 //
-// Device (per factorize / solve), one launch per level and kernel:
-//   * k_panel: one workgroup per supernode factors its dense r x c column
-//     panel (rows = the supernode's row structure) in place: right-looking
-//     LDL' over the c pivot columns, L scaled by D, D on the diagonal;
-//   * k_update: one workgroup per 64x64 tile of the supernode's Schur update
-//     U = L21 D L21' (L21 the panel's rows below the pivot block), scattered
-//     with FP64 atomics into the ancestor supernodes that own those entries
-//     (several supernodes of one level may update one ancestor);
-//   * k_forward / k_backward: level-ordered triangular solves, one
-//     workgroup per supernode; the forward pass scatters its below-block
-//     products into the right-hand side with atomics, the backward pass
-//     gathers the solved ancestor values.
+// Device, one launch per level and kernel (levels bottom-up to factor and
+// solve forward, top-down to solve backward):
+//   * k_panel: jobs of 256 rows below a pivot block; each factors the
+//     supernode's pivot block in LDS, then its rows left-looking in registers;
+//   * k_update: 64x64 tiles of each supernode's Schur update U = L21 D L21',
+//     subtracted from the ancestor panels that own the entries (FP64 atomics
+//     when several supernodes share the level);
+//   * k_fwd: pivot-block solve by one wavefront, then 64-row jobs that
+//     update the right-hand side of the ancestor rows (atomics);
+//   * k_bwd_off / k_bwd_diag: 64-row jobs of partial sums over the solved
+//     ancestor values, then the pivot-block solve by one wavefront.
 // Storage in HBM: each supernode's panel column-major (ld = its row count),
 // panels back to back; an untouched copy holds K's off-diagonal values in
-// the same layout, copied over the panel storage before each factorisation.
+// the same layout, copied over the panel storage before each factorisation;
+// the factored pivot blocks in a separate array.
 
 #include <hip/hip_runtime.h>
 
