@@ -49,6 +49,7 @@
 #include <string>
 #include <vector>
 
+#include "dense_ops.h"
 #include "qtqp_kkt.h"
 #include "qtqp_sparse.h"
 
@@ -446,6 +447,38 @@ __global__ void k_load_diag(double* __restrict__ lx, const int64_t* __restrict__
   if (k < n) lx[dpos[k]] = diag[perm[k]];
 }
 
+__device__ __forceinline__ double bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Right-looking LDL' of a pivot block by one wavefront, lane i holding row
+// i in registers, padded to CM x CM with identity rows (pivot 1, zero
+// coupling), so every step runs without a branch: step k reads the pivot
+// and column k of the other rows with readlane, and no step touches LDS or
+// waits at a barrier.  On return row[j] = l_ij for j < i and row[i] = d_i.
+// Returns false on a zero or non-finite pivot (the same in every lane).
+template <int CM>
+__device__ __forceinline__ bool wave_ldl(double (&row)[kMaxCols], int lane, int& bad_k) {
+#pragma unroll
+  for (int k = 0; k < CM; ++k) {
+    const double d = bcast(row[k], k);
+    if (!(d != 0.0) || !isfinite(d)) {  // uniform
+      bad_k = k;
+      return false;
+    }
+    const double vik = row[k] * (1.0 / d);
+#pragma unroll
+    for (int j = k + 1; j < CM; ++j) {
+      const double ajk = bcast(row[k], j);
+      if (lane >= j) row[j] -= vik * ajk;
+    }
+    if (lane > k) row[k] = vik;
+  }
+  return true;
+}
+
 // One row of L below a supernode's pivot block, left-looking along the row:
 // w_k = a_ik - sum_{j<k} w_j l_kj, l_ik = w_k / d_k (w_j = l_ij d_j stays in
 // registers).  The row is loaded 16 columns at a time with unconditional,
@@ -498,45 +531,43 @@ __global__ void __launch_bounds__(kThreads) k_panel(Tables T, double* __restrict
   const int r = (int)(T.rows_off[s + 1] - T.rows_off[s]);
   double* L = lx + T.lx_off[s];
   const int tid = threadIdx.x;
+  __shared__ int failed;
+  // The block's lower triangle, staged by the whole workgroup (coalesced).
   for (int e = tid; e < kMaxCols * kMaxCols; e += kThreads) {
     const int j = e / kMaxCols, i = e % kMaxCols;
     Dl[i][j] = (i < c && j < c && i >= j) ? L[(int64_t)j * r + i] : 0.0;
   }
-  if (tid < kMaxCols) rd[tid] = 0.0;
   __syncthreads();
-  // Right-looking LDL' of the block; each step's rank-1 update is spread over
-  // all (i, j) pairs of the trailing triangle.
-  for (int k = 0; k < c; ++k) {
-    const double d = Dl[k][k];
-    if (!(d != 0.0) || !isfinite(d)) {  // uniform across the workgroup
-      if (tid == 0) atomicCAS(bad, 0, perm[f + k] + 1);
-      return;
-    }
-    const double rk = 1.0 / d;
-    // 16 x 16 threads over the trailing triangle, 4 x 4 entries each
-    const int ti = tid / 16, tj = tid % 16;
+  if (tid < kMaxCols) {  // wave 0 factors the pivot block in registers
+    double row[kMaxCols];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = k + 1 + ti + 16 * u;
-      if (i < c) {
-        const double vi = Dl[i][k] * rk;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int j = k + 1 + tj + 16 * v;
-          if (j <= i) Dl[i][j] -= vi * Dl[j][k];
-        }
+    for (int j = 0; j < kMaxCols; ++j) row[j] = tid < c ? Dl[tid][j] : (j == tid ? 1.0 : 0.0);
+    int bad_k = -1;
+    const bool ok = c <= 8    ? wave_ldl<8>(row, tid, bad_k)
+                    : c <= 16 ? wave_ldl<16>(row, tid, bad_k)
+                    : c <= 32 ? wave_ldl<32>(row, tid, bad_k)
+                              : wave_ldl<64>(row, tid, bad_k);
+    if (!ok) {
+      if (tid == 0) {
+        failed = 1;
+        atomicCAS(bad, 0, perm[f + bad_k] + 1);
       }
+    } else {
+      if (tid == 0) failed = 0;
+#pragma unroll
+      for (int j = 0; j < kMaxCols; ++j) Dl[tid][j] = (j <= tid && tid < c) ? row[j] : 0.0;
+      // (row[] is indexed by constants only, so it stays in registers; the
+      // pivot comes back from LDS)
+      rd[tid] = tid < c ? 1.0 / Dl[tid][tid] : 0.0;
     }
-    __syncthreads();
-    if (tid > k && tid < c) Dl[tid][k] *= rk;
-    if (tid == k) rd[k] = rk;
-    __syncthreads();
   }
+  __syncthreads();
+  if (failed) return;  // uniform: one value in LDS
   if (a0 == 0) {
     double* D = dblk + T.dblk_off[s];
     for (int e = tid; e < c * c; e += kThreads) {
       const int j = e / c, i = e % c;
-      D[e] = i >= j ? Dl[i][j] : 0.0;
+      D[e] = i >= j ? Dl[i][j] : 0.0;  // l_ij below the diagonal, d_i on it
     }
   }
   const int a = a0 + tid;
@@ -560,14 +591,16 @@ __device__ __forceinline__ int find_row(const int32_t* __restrict__ rows, int lo
 
 // One 64x64 tile of a supernode's Schur update U = L21 D L21' (rows a, b of
 // the panel below its pivot block, a >= b), subtracted from the ancestor
-// entries (rows[c+a], rows[c+b]).  Each thread owns a 4x4 block of the tile.
+// entries (rows[c+a], rows[c+b]).  The panel read as a row-major matrix with
+// ld = r has the pivot columns as its rows, so U is the weighted rank-c
+// product of the batched Gram engine: qk::RankKTile on MFMA 16x16x4 f64 with
+// the pivots as weights.
 __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restrict__ lx,
                                                      const double* __restrict__ dblk,
                                                      int64_t tile_begin, int exclusive,
                                                      int* __restrict__ bad) {
-  constexpr int kChunk = 32;  // pivot columns staged at a time (33 KB of LDS)
-  __shared__ double As[kChunk][kTile + 1];
-  __shared__ double Bs[kChunk][kTile + 1];
+  __shared__ double si[qk::KC][qk::NB + 1];
+  __shared__ double sj[qk::KC][qk::NB + 1];
   const int32_t* tl = T.tiles + 3 * (tile_begin + blockIdx.x);
   const int32_t s = tl[0];
   const int a0 = tl[1], b0 = tl[2];
@@ -575,74 +608,31 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
   const int c = T.first[s + 1] - f;
   const int64_t roff = T.rows_off[s];
   const int r = (int)(T.rows_off[s + 1] - roff);
-  const int nb = r - c;  // rows below the pivot block
   const double* L = lx + T.lx_off[s];
   const double* D = dblk + T.dblk_off[s];
-  const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
-  double acc[4][4] = {};
-  for (int k0 = 0; k0 < c; k0 += kChunk) {
-    const int kc = min(kChunk, c - k0);
-    __syncthreads();  // the previous chunk has been consumed
-    // Stage L21 rows a0.. and (scaled by D) rows b0.. of these pivot columns.
-    for (int e = tid; e < kc * kTile; e += kThreads) {
-      const int kk = e / kTile, t = e % kTile;
-      const double* Lk = L + (int64_t)(k0 + kk) * r;
-      const int a = a0 + t, b = b0 + t;
-      As[kk][t] = a < nb ? Lk[c + a] : 0.0;
-      Bs[kk][t] = b < nb ? Lk[c + b] * D[(int64_t)(k0 + kk) * (c + 1)] : 0.0;
-    }
-    __syncthreads();
-    for (int kk = 0; kk < kc; ++kk) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) av[u] = As[kk][ty + 16 * u], bv[u] = Bs[kk][tx + 16 * u];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
-    }
-  }
+  qk::RankKTile<double> tile;
+  tile.zero();
+  tile.run(L, r, 0, c, c + a0, c + b0, r, r, [=](int k) { return D[(int64_t)k * (c + 1)]; }, si,
+           sj);
   const int32_t* R = T.rows + roff;
-  // A row's position in the target supernode's structure does not depend on
-  // the target column, so consecutive columns in one supernode reuse it.
-  int32_t pos_t[4] = {-1, -1, -1, -1};
-  int pos[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int b = b0 + tx + 16 * v;
-    if (b >= nb) continue;
-    const int32_t col = R[c + b];
+  tile.for_each(c + a0, c + b0, [&](int i, int j, double v) {
+    if (i >= r || j >= r || i < j) return;  // outside the panel or above the diagonal
+    const int32_t row = R[i], col = R[j];
     const int32_t t = T.col_sn[col];
     const int64_t troff = T.rows_off[t];
     const int tr = (int)(T.rows_off[t + 1] - troff);
     const int lc = col - T.first[t];
     const int32_t* tR = T.rows + troff;
-    double* dst = lx + T.lx_off[t] + (int64_t)lc * tr;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int a = a0 + ty + 16 * u;
-      if (a >= nb || a < b) continue;
-      const int32_t row = R[c + a];
-      if (pos_t[u] != t) {
-        pos[u] = T.contig[t] ? row - tR[0] : find_row(tR, lc, tr, row);
-        pos_t[u] = t;
-      }
-      if (pos[u] >= lc && pos[u] < tr && tR[pos[u]] == row) {
-        // One supernode in the level: each target entry has one writer.
-        if (exclusive) dst[pos[u]] -= acc[u][v];
-        else unsafeAtomicAdd(dst + pos[u], -acc[u][v]);
-      } else {
-        atomicCAS(bad, 0, -1);  // the symbolic structure misses a fill entry
-      }
+    const int pos = T.contig[t] ? row - tR[0] : find_row(tR, lc, tr, row);
+    if (pos >= lc && pos < tr && tR[pos] == row) {
+      double* dst = lx + T.lx_off[t] + (int64_t)lc * tr + pos;
+      // One supernode in the level: each target entry has one writer.
+      if (exclusive) *dst -= v;
+      else unsafeAtomicAdd(dst, -v);
+    } else {
+      atomicCAS(bad, 0, -1);  // the symbolic structure misses a fill entry
     }
-  }
-}
-
-__device__ __forceinline__ double bcast(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
+  });
 }
 
 // L11 y = b by one wavefront: lane i holds b_i and row i of the unit lower
