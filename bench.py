@@ -325,7 +325,7 @@ def single_legs(problems, qtqp_oracle):
   return out
 
 
-def sparse_leg(m=6000, n=4000, z=1000, density=0.002, seed=0, reps=10, cpu=True):
+def sparse_leg(m=6000, n=4000, z=1000, density=0.002, seed=0, reps=10, cpu=True, qtqp=True):
   """The sparse KKT backend (HIP_SPARSE, supernodal LDL') on the reference's sparse
   QP family (tests/test_qtqp.py:211-235) at n=4000, m=6000, z=1000.
 
@@ -382,14 +382,15 @@ def sparse_leg(m=6000, n=4000, z=1000, density=0.002, seed=0, reps=10, cpu=True)
          "solve_ms": sol_ms, "solve_gbs": solve_bytes / (sol_ms / 1e3) / 1e9,
          "solve_frac_of_hbm_peak": solve_bytes / (sol_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
          "solve_rel_residual": res}
-  log("sparse KKT: QTQP.solve with HIP_SPARSE")
-  t0 = time.perf_counter()
-  qs = qtqp_amd.QTQP(a=qp.a, b=qp.b, c=qp.c, z=z, p=qp.p).solve(
-      linear_solver=qtqp_amd.LinearSolver.HIP_SPARSE, verbose=False, collect_stats=True)
-  out["qtqp_solve"] = {"status": qs.status.value, "iterations": len(qs.stats),
-                       "wall_s": time.perf_counter() - t0,
-                       "rel_kkt_residual": kkt_rel_residual(qs.x, qs.y, qs.s, qp.a, qp.b, qp.c,
-                                                            qp.p, z)}
+  if qtqp:
+    log("sparse KKT: QTQP.solve with HIP_SPARSE")
+    t0 = time.perf_counter()
+    qs = qtqp_amd.QTQP(a=qp.a, b=qp.b, c=qp.c, z=z, p=qp.p).solve(
+        linear_solver=qtqp_amd.LinearSolver.HIP_SPARSE, verbose=False, collect_stats=True)
+    out["qtqp_solve"] = {"status": qs.status.value, "iterations": len(qs.stats),
+                         "wall_s": time.perf_counter() - t0,
+                         "rel_kkt_residual": kkt_rel_residual(qs.x, qs.y, qs.s, qp.a, qp.b, qp.c,
+                                                              qp.p, z)}
   if cpu:
     log("sparse KKT: scipy.sparse.linalg.factorized on the CPU")
     t0 = time.perf_counter()
@@ -474,6 +475,10 @@ def main():
   ap.add_argument("--no-sparse-cpu", action="store_true")
   ap.add_argument("--sparse-only", action="store_true",
                   help="only the sparse KKT backend leg (for its kernel profiles)")
+  ap.add_argument("--sparse-reps", type=int, default=10,
+                  help="timed factorisations and solves of the sparse leg")
+  ap.add_argument("--sparse-no-qtqp", action="store_true",
+                  help="sparse leg without its whole QTQP.solve (for counter runs)")
   ap.add_argument("--oracle-sample", type=int, default=64)
   ap.add_argument("--profile-single", choices=("mixed", "fp64"),
                   help="only solve the n=4000 m=6000 z=1000 dense QP once through QTQP.solve "
@@ -503,7 +508,8 @@ def main():
     return
 
   if args.sparse_only:
-    print(json.dumps(sparse_leg(cpu=not args.no_sparse_cpu)), flush=True)
+    print(json.dumps(sparse_leg(reps=args.sparse_reps, cpu=not args.no_sparse_cpu,
+                                qtqp=not args.sparse_no_qtqp)), flush=True)
     return
   from qtqp_amd import batch, problems
   from oracle import qtqp_oracle
