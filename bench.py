@@ -475,6 +475,9 @@ def main():
   ap.add_argument("--no-sparse-cpu", action="store_true")
   ap.add_argument("--sparse-only", action="store_true",
                   help="only the sparse KKT backend leg (for its kernel profiles)")
+  ap.add_argument("--sparse-sweep", action="store_true",
+                  help="the sparse leg at several sizes of the reference's sparse family, "
+                       "each with the scipy.sparse.linalg.factorized comparison")
   ap.add_argument("--sparse-reps", type=int, default=10,
                   help="timed factorisations and solves of the sparse leg")
   ap.add_argument("--sparse-no-qtqp", action="store_true",
@@ -507,6 +510,16 @@ def main():
           flush=True)
     return
 
+  if args.sparse_sweep:
+    sweep = []
+    for m_, n_, z_, dens in ((150, 100, 10, 0.1), (1500, 1000, 100, 0.01),
+                             (3000, 2000, 100, 0.01), (6000, 4000, 1000, 0.002)):
+      leg = sparse_leg(m=m_, n=n_, z=z_, density=dens, reps=args.sparse_reps,
+                       cpu=not args.no_sparse_cpu)
+      log(f"sparse sweep N={m_ + n_}: factor {leg['factor_ms']:.2f} ms")
+      sweep.append(leg)
+    print(json.dumps({"sparse_sweep": sweep}), flush=True)
+    return
   if args.sparse_only:
     print(json.dumps(sparse_leg(reps=args.sparse_reps, cpu=not args.no_sparse_cpu,
                                 qtqp=not args.sparse_no_qtqp)), flush=True)
