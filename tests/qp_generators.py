@@ -1,8 +1,11 @@
 """Restatements of the reference test-suite generators and KKT assertion helpers.
 
 Follows qtqp tests/test_qtqp.py: _gen_feasible 211-235, _gen_infeasible
-238-255, _gen_unbounded 258-280, _assert_solution 283-309, _assert_infeasible
-312-325, _assert_unbounded 328-346.  Dense outputs (a, b, c, p).
+238-255, _gen_unbounded 258-280, _append_dropped_inequalities 490-496,
+_assert_solution 283-309, _assert_infeasible 312-325, _assert_unbounded
+328-346.  Dense outputs (a, b, c, p).  `seed` is an int or a
+numpy Generator: as with the reference's random_state=rng, a Generator is used
+as it is, so one stream can feed several generators in turn.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -47,15 +50,25 @@ def gen_unbounded(m, n, z, seed):
   a = rng.normal(size=(m, n))
   p = rng.normal(size=(n, n))
   p = p.T @ p * 0.01
-  e, v = np.linalg.eigh(p)
-  e[0] = 0.0
-  x = v[:, 0]
+  # general eigensolver and the last pair it returns, as the reference does;
+  # P = V diag(e) V' is then symmetric only to rounding (within the 1e-12
+  # asymmetry that QTQP accepts)
+  e, v = np.linalg.eig(p)
+  e, v = np.real(e), np.real(v)
+  e[-1] = 0.0
+  x = v[:, -1]
   p = v @ np.diag(e) @ v.T
-  p = 0.5 * (p + p.T)
   a = a - np.outer(s + a @ x, x) / np.linalg.norm(x) ** 2
   c = -c / (c @ x)
   b = rng.normal(size=m)
   return a, b, c, p
+
+
+def append_dropped_inequalities(a, b, n_extra, seed, rhs_value):
+  """Rows that presolve drops (b >= 1e20): dense N(0,1) rows with rhs_value."""
+  rng = np.random.default_rng(seed)
+  a_extra = rng.normal(size=(n_extra, a.shape[1]))
+  return np.vstack([a, a_extra]), np.concatenate([b, np.full(n_extra, rhs_value)])
 
 
 def check_solution(x, y, s, a, b, c, p, z, atol=1e-7, rtol=1e-8):
