@@ -59,6 +59,7 @@ typedef struct qtqp_batch_counters {
   int64_t ir_not_converged;/* KKT solves that ended stalled / non-converged */
   int32_t iterations;      /* IPM iterations launched (max over QPs) */
   double device_ms;        /* device time of the whole solve (HIP events) */
+  int64_t jitter_retries;  /* FP64 factors redone with a larger diagonal shift */
 } qtqp_batch_counters;
 
 /* Packs `count` QPs onto the device (no compute).  QP i has sizes n[i],
@@ -102,12 +103,13 @@ int qtqp_batch_launch_times(qtqp_batch* h, double* ms, int32_t cap);
  * returns how many exist.  No reference counterpart (batch diagnostics). */
 int qtqp_batch_instance_ms(qtqp_batch* h, double* ms, int32_t cap);
 
-/* Per-iteration trace of QP q for the last solve: 16 doubles per IPM
+/* Per-iteration trace of QP q for the last solve: 24 doubles per IPM
  * iteration (mu, sigma, alpha, tau, tau_predictor, tau_corrector,
  * alpha_predictor, then solves and final residual of the q / predictor /
  * corrector KKT solves (solves + 1000 when stalled), pres, dres, gap) --
- * the scalars of the reference's per-iteration stats (__init__.py:967-990).
- * Copies min(rows, 128) rows. */
+ * the scalars of the reference's per-iteration stats (__init__.py:967-990)
+ * -- then, for the predictor and the corrector, the coefficients a, b, c of
+ * the tau quadratic and max |K^{-1} r|.  Copies min(rows, 128) rows. */
 int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows);
 
 /* Executed work of the last solve per QP, 5 counts each: FP32 and FP64

@@ -54,6 +54,7 @@ typedef struct qtqp_kkt_info {
   int64_t inner_steps;        /* FP32 refinement steps inside backend solves */
   double factor_ms;           /* device time in factorisations (HIP events) */
   double solve_ms;            /* device time in solves and matvecs */
+  int64_t jitter_retries;     /* FP64 factors redone with a larger diagonal shift */
 } qtqp_kkt_info;
 
 /* Creates a handle for an (n+m)x(n+m) KKT system.  Replaces

@@ -39,7 +39,8 @@ class KktInfo(ctypes.Structure):
   _fields_ = [("precision", ctypes.c_int32), ("factor_precision", ctypes.c_int32),
               ("factorizations", ctypes.c_int64), ("fp64_fallbacks", ctypes.c_int64),
               ("backend_solves", ctypes.c_int64), ("inner_steps", ctypes.c_int64),
-              ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+              ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+              ("jitter_retries", ctypes.c_int64)]
 
 
 class IpmOpts(ctypes.Structure):

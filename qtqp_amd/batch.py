@@ -36,7 +36,8 @@ class BatchOpts(ctypes.Structure):
 class BatchCounters(ctypes.Structure):
   _fields_ = [("backend_solves", ctypes.c_int64), ("inner_steps", ctypes.c_int64),
               ("fp64_fallbacks", ctypes.c_int64), ("ir_not_converged", ctypes.c_int64),
-              ("iterations", ctypes.c_int32), ("device_ms", ctypes.c_double)]
+              ("iterations", ctypes.c_int32), ("device_ms", ctypes.c_double),
+              ("jitter_retries", ctypes.c_int64)]
 
 
 BATCH_SYMBOLS = {
@@ -371,9 +372,8 @@ class BatchQP:
 
   TRACE_FIELDS = ("mu", "sigma", "alpha", "tau", "tau_p", "tau_c", "alpha_p", "q_solves",
                   "q_res", "p_solves", "p_res", "c_solves", "c_res", "pres", "dres", "gap",
-                  "dbg_p_ta", "dbg_p_tb", "dbg_p_tc", "dbg_p_krmax", "dbg_c_ta", "dbg_c_tb",
-                  "dbg_c_tc", "dbg_c_krmax", "snap_sd_tau", "snap_tau_norm", "snap_scale",
-                  "snap_tau_after_factor", "snap_tau_before_pred", "s5", "s6", "s7")
+                  "p_tau_a", "p_tau_b", "p_tau_c", "p_kr_max", "c_tau_a", "c_tau_b",
+                  "c_tau_c", "c_kr_max")
 
   def trace(self, q: int, iters: int) -> list:
     """Per-iteration scalars of QP q in the last solve (list of dicts)."""

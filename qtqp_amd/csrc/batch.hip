@@ -76,7 +76,7 @@ struct Opts {
 };
 
 struct Counters {
-  unsigned long long backend_solves, inner_steps, fallbacks, ir_bad;
+  unsigned long long backend_solves, inner_steps, fallbacks, ir_bad, jitter_retries;
   int done;
   int max_iters;  // most IPM iterations any QP took
 };
@@ -352,7 +352,8 @@ __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, c
 }
 
 template <typename T>
-__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm_in) {
+__device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm_in,
+                                       double jit_rel = 1e-14) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
   for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -1.0 / I.dg[n + i];
@@ -364,10 +365,11 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
   else
     gram_tiles<T, 1>(I, A, P, G, I.um, sm);
   __syncthreads();
-  // diag(G) += 1e-14 * max(diag(G))  (solvers_dense.py:107-112)
+  // diag(G) += jit_rel * max(diag(G)), jit_rel = 1e-14 (solvers_dense.py:107-112)
+  // or larger on a retry after a breakdown
   double mx = -INFINITY;
   for (int i = threadIdx.x; i < n; i += WG) mx = fmax(mx, (double)G[(int64_t)i * n + i]);
-  const double jit = 1e-14 * bmax(mx, sm.red);
+  const double jit = jit_rel * bmax(mx, sm.red);
   for (int i = threadIdx.x; i < n; i += WG)
     G[(int64_t)i * n + i] = (T)((double)G[(int64_t)i * n + i] + jit);
   __syncthreads();
@@ -569,6 +571,24 @@ __device__ void trsv_upper_inv(const T* L, int n, const T* LiB, T* v, double* re
   }
 }
 
+// FP64 factor with the reference's diagonal shift of 1e-14 max(diag(G))
+// (solvers_dense.py:107-112), redone with 1e-12 and 1e-10 after a breakdown.
+// Near the end of an IPM on an unbounded or infeasible problem G's condition
+// number reaches 1e18 and the sign of its last pivots depends on rounding
+// (the reference's LAPACK Gram and Cholesky can land on either side too); a
+// larger shift keeps the factor usable and the refinement against the exact
+// K removes its effect.  Returns false if all three break down.
+__device__ bool factor64_retry(const Inst& I, Smem& sm_in, Counters* cnt) {
+  Smem sm = lds_view(sm_in);
+  const double rel[3] = {1e-14, 1e-12, 1e-10};
+  for (int t = 0; t < 3; ++t) {
+    gram_syrk<double>(I, I.A, I.P, I.G, sm, rel[t]);
+    if (chol<double>(I.G, I.n, I.LiB, sm) == 0) return true;
+    if (t < 2 && threadIdx.x == 0) atomicAdd(&cnt->jitter_retries, 1ull);
+  }
+  return false;
+}
+
 // Factor in the instance's current precision; FP32 breakdown falls back to
 // FP64.  Returns false on an FP64 breakdown.
 __device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
@@ -586,8 +606,7 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
     __syncthreads();
   }
   if (threadIdx.x == 0) I.si[I_F64] += 1;
-  gram_syrk<double>(I, I.A, I.P, I.G, sm);
-  return chol<double>(I.G, I.n, I.LiB, sm) == 0;
+  return factor64_retry(I, sm, cnt);
 }
 
 // -------------------------------------------------------------------------
@@ -967,8 +986,7 @@ __device__ inline bool fallback_fp64(const Inst& I, Smem& sm_in, Counters* cnt) 
     atomicAdd(&cnt->fallbacks, 1ull);
   }
   __syncthreads();
-  gram_syrk<double>(I, I.A, I.P, I.G, sm);
-  return chol<double>(I.G, I.n, I.LiB, sm) == 0;
+  return factor64_retry(I, sm, cnt);
 }
 
 struct IrResult {
@@ -2138,6 +2156,7 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
     counters->inner_steps = (int64_t)hc.inner_steps;
     counters->fp64_fallbacks = (int64_t)hc.fallbacks;
     counters->ir_not_converged = (int64_t)hc.ir_bad;
+    counters->jitter_retries = (int64_t)hc.jitter_retries;
     counters->iterations = hc.max_iters;
     counters->device_ms = ms;
   }

@@ -99,14 +99,15 @@ __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
   });
 }
 
-// diag(G) += 1e-14 * max(diag(G))  (solvers_dense.py:107-112).  One workgroup.
+// diag(G) += rel * max(diag(G)); rel = 1e-14 as solvers_dense.py:107-112 (and
+// larger on a retry after a breakdown).  One workgroup.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n) {
+__global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n, double rel) {
   __shared__ double red[4];
   double mx = -INFINITY;
   for (int i = threadIdx.x; i < n; i += WG) mx = fmax(mx, (double)G[(int64_t)i * ld + i]);
   mx = block_max(mx, red);
-  const double jit = 1e-14 * mx;
+  const double jit = rel * mx;
   for (int i = threadIdx.x; i < n; i += WG) G[(int64_t)i * ld + i] = (T)((double)G[(int64_t)i * ld + i] + jit);
 }
 
@@ -297,7 +298,8 @@ __global__ void k_rowfin(const double* __restrict__ rpart, int nb, int rows,
 
 // K_reg x (or a residual):  kx = [P_off x_x + d_x*x_x + A'x_y ; A x_x + d_y*x_y]
 // from the row partials of P_off x_x (rpp) and A x_x (rpa, ncb tiles each)
-// and aty = A'x_y.  If rhs: out = rhs - kx (+ corr*x if corr).  With norm:
+// and aty = A'x_y (rpp NULL: P_off = 0).  If rhs: out = rhs - kx (+ corr*x if
+// corr).  With norm:
 // norm[0] gets max |out| and norm[1] becomes nonzero if x has a NaN.
 __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* __restrict__ x,
                                                    const double* __restrict__ diag,
@@ -314,7 +316,7 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
   bool live = idx < n + m;
   if (live) {
     double kx;
-    if (idx < n) kx = row_total(rpp, ncb, n, idx) + diag[idx] * x[idx] + aty[idx];
+    if (idx < n) kx = (rpp ? row_total(rpp, ncb, n, idx) : 0.0) + diag[idx] * x[idx] + aty[idx];
     else kx = row_total(rpa, ncb, m, idx - n) + diag[idx] * x[idx];
     if (rhs) {
       v = rhs[idx] - kx;
@@ -650,35 +652,41 @@ int fail(int code, const std::string& msg) {
 
 int nblk_rows(int64_t rows) { return (int)((rows + RB - 1) / RB); }
 
-// One tile pass over M (k_tile_pass): row partials into rpart (if x) and
-// column partials into h->part (if u).
+// One tile pass over M (k_tile_pass) on stream st: row partials into rpart
+// (if x) and column partials into cpart (if u).
 template <typename T, bool VEC>
-void launch_tile_pass(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* x,
-                      double* rpart, const double* u, const double* w) {
+void launch_tile_pass(hipStream_t st, const T* M, int64_t rows, int64_t ncol, const double* x,
+                      double* rpart, const double* u, const double* w, double* cpart) {
   const dim3 grid(tiles_r(rows), tiles_c(ncol));
   if (x && u)
-    hipLaunchKernelGGL((k_tile_pass<T, true, true, VEC>), grid, dim3(WG), 0, h->stream, M,
-                       (int)rows, (int)ncol, x, rpart, u, w, h->part);
+    hipLaunchKernelGGL((k_tile_pass<T, true, true, VEC>), grid, dim3(WG), 0, st, M,
+                       (int)rows, (int)ncol, x, rpart, u, w, cpart);
   else if (x)
-    hipLaunchKernelGGL((k_tile_pass<T, true, false, VEC>), grid, dim3(WG), 0, h->stream, M,
-                       (int)rows, (int)ncol, x, rpart, u, w, h->part);
+    hipLaunchKernelGGL((k_tile_pass<T, true, false, VEC>), grid, dim3(WG), 0, st, M,
+                       (int)rows, (int)ncol, x, rpart, u, w, cpart);
   else
-    hipLaunchKernelGGL((k_tile_pass<T, false, true, VEC>), grid, dim3(WG), 0, h->stream, M,
-                       (int)rows, (int)ncol, x, rpart, u, w, h->part);
+    hipLaunchKernelGGL((k_tile_pass<T, false, true, VEC>), grid, dim3(WG), 0, st, M,
+                       (int)rows, (int)ncol, x, rpart, u, w, cpart);
+}
+
+template <typename T>
+int tile_pass_on(hipStream_t st, double* cpart, const T* M, int64_t rows, int64_t ncol,
+                 const double* x, double* rpart, const double* u, const double* w) {
+  if (rows == 0) return QTQP_OK;
+  // rows start on 16-byte boundaries when ncol * sizeof(T) is a multiple of 16
+  // and M itself is 16-byte aligned
+  const bool vec = (ncol * (int64_t)sizeof(T)) % 16 == 0 &&
+                   (reinterpret_cast<uintptr_t>(M) & 15) == 0;
+  if (vec) launch_tile_pass<T, true>(st, M, rows, ncol, x, rpart, u, w, cpart);
+  else launch_tile_pass<T, false>(st, M, rows, ncol, x, rpart, u, w, cpart);
+  HIPC(hipGetLastError());
+  return QTQP_OK;
 }
 
 template <typename T>
 int tile_pass(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* x,
               double* rpart, const double* u, const double* w) {
-  if (rows == 0) return QTQP_OK;
-  // rows start on 16-byte boundaries when ncol * sizeof(T) is a multiple of 16
-  // (device allocations are 256-byte aligned)
-  const bool vec = (ncol * (int64_t)sizeof(T)) % 16 == 0 &&
-                   (reinterpret_cast<uintptr_t>(M) & 15) == 0;
-  if (vec) launch_tile_pass<T, true>(h, M, rows, ncol, x, rpart, u, w);
-  else launch_tile_pass<T, false>(h, M, rows, ncol, x, rpart, u, w);
-  HIPC(hipGetLastError());
-  return QTQP_OK;
+  return tile_pass_on<T>(h->stream, h->part, M, rows, ncol, x, rpart, u, w);
 }
 
 inline unsigned colfin_blocks(int64_t ncol) { return (unsigned)((ncol + 63) / 64); }
@@ -715,7 +723,7 @@ int kmatvec(qtqp_kkt* h, const double* x, const double* rhs, const double* corr,
 // Factor G in precision T: Gram SYRK, jitter, blocked LDL'. Returns the
 // breakdown column (1-based) or 0 in *brk.
 template <typename T>
-int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
+int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk, double jit_rel = 1e-14) {
   const int n = (int)h->n, m = (int)h->m;
   const int nt = (n + NB - 1) / NB;
   HIPC(hipMemsetAsync(h->info, 0, sizeof(int), h->stream));
@@ -729,7 +737,7 @@ int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
     hipLaunchKernelGGL((k_gram_syrk<T, false>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A,
                        Poff, h->diag, n, m, G, (int64_t)n, ng);
   HIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n);
+  hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, jit_rel);
   HIPC(hipGetLastError());
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int below = std::max(0, n - k0 - NB);
@@ -758,9 +766,23 @@ int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk) {
   return QTQP_OK;
 }
 
+// Diagonal shifts of the FP64 factor, relative to max(diag(G)): the
+// reference's 1e-14 (solvers_dense.py:107-112), then two retries after a
+// breakdown.  Near the end of an IPM on an unbounded or infeasible problem
+// G's condition number reaches 1e18 and whether its last pivots come out
+// positive depends on rounding (the reference's LAPACK Gram and Cholesky can
+// land on either side too); a larger shift keeps the factor usable, and the
+// refinement against the exact K, which never sees the shift, removes its
+// effect, as the reference relies on for its own shift.
+constexpr double kJitterRel[3] = {1e-14, 1e-12, 1e-10};
+
 int factor64(qtqp_kkt* h) {
   int brk = 0, rc;
-  if ((rc = factor_prec<double>(h, h->A64, h->Poff64, h->G64, &brk))) return rc;
+  for (int t = 0; t < 3; ++t) {
+    if ((rc = factor_prec<double>(h, h->A64, h->Poff64, h->G64, &brk, kJitterRel[t]))) return rc;
+    if (brk == 0) break;
+    if (t < 2) h->info_acc.jitter_retries++;
+  }
   if (brk != 0) {
     h->have_factor = false;
     char buf[128];

@@ -64,6 +64,30 @@ def test_uneven_mpc_batch_matches_oracle(require_gpu, precision):
   bq.free()
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_trace_reports_the_refinement_results(require_gpu, precision):
+  """The per-iteration trace carries each KKT solve's own result: solve counts
+  within max_iterative_refinement_steps, non-negative residual norms, and a
+  residual below the refinement tolerance wherever a solve converged.  (A
+  miscompiled copy of the residual once returned an element of the solution
+  vector instead, for solves with an odd number of steps.)"""
+  keys = sorted(TRAJ)
+  inputs = [golden_cases.inputs(TRAJ[k]["spec"]) for k in keys]
+  bq = batch.BatchQP(inputs)
+  bq.solve_device(precision=precision)
+  raw = bq.raw_iterates()
+  for q in range(len(keys)):
+    for row in bq.trace(q, int(raw["iters"][q])):
+      for kind in ("q", "p", "c"):
+        solves, res = row[f"{kind}_solves"], row[f"{kind}_res"]
+        stalled = solves >= 1000
+        steps = solves - 1000 if stalled else solves
+        assert 1 <= steps <= 20 and res >= 0.0, (keys[q], kind, row)
+        if not stalled and steps < 20:
+          assert res < 1e-6, (keys[q], kind, row)
+  bq.free()
+
+
 def test_large_instances_in_batch(require_gpu):
   # multi-panel factor (n > 64), long rows (m > 1000) and uneven neighbours
   qps = [problems.mpc_qp(40, 12, 4, seed=3), problems.random_dense_qp(150, 220, 30, seed=4),
