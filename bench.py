@@ -164,14 +164,20 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
   t0 = time.perf_counter()
   warm = warmstart.solve_with_warm_start(ws, x_ev, cap=cap)
   t_warm = time.perf_counter() - t0
-  qps = fam.qps(x_ev[:32])
-  res, dx = [], []
+  idx = np.linspace(0, n_eval - 1, 32).astype(int)
+  qps = fam.qps(x_ev[idx])
+  names = {0: "unfinished", 1: "solved", 2: "infeasible", 3: "unbounded"}
+  res, dxys = [], {k: [] for k in "xys"}
   ok = True
-  for i, (a, b, c, z, p) in enumerate(qps):
+  for i, (a, b, c, z, p) in zip(idx, qps):
     ref = qtqp_oracle.solve_qp(a, b, c, z, p)
-    ok &= ref["status"] == ("solved" if warm["status"][i] == 1 else "other")
+    ok &= ref["status"] == names.get(int(warm["status"][i]), "other")
+    if ref["status"] != "solved":
+      continue
     res.append(kkt_rel_residual(warm["x"][i], warm["y"][i], warm["s"][i], a, b, c, p, z))
-    dx.append(float(np.max(np.abs(warm["x"][i] - ref["x"])) / max(1.0, np.max(np.abs(ref["x"])))))
+    for k in "xys":
+      got, want = warm[k][i], ref[k]
+      dxys[k].append(float(np.max(np.abs(got - want)) / max(1.0, np.max(np.abs(want)))))
   ws.mlp.free()
   return {"family": "condensed MPC T=32 nx=8 nu=3, x0 ~ N(0, 0.5^2 I)", "k_star": k_star,
           "train_qps": n_train, "eval_qps": n_eval, "mlp": "8-256-256-1506 GELU, BF16 MFMA, Adam",
@@ -186,7 +192,8 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
           "cold_device_ms": float(cold["device_ms"]), "warm_device_ms": float(warm["device_ms"]),
           "device_time_cut": float(1.0 - warm["device_ms"] / cold["device_ms"]),
           "oracle_sample": len(qps), "oracle_status_match": bool(ok),
-          "max_rel_kkt_residual": float(max(res)), "max_rel_x_diff_vs_oracle": float(max(dx))}
+          "max_rel_kkt_residual": float(max(res)),
+          "max_rel_err_vs_oracle": {k: float(max(v)) for k, v in dxys.items()}}
 
 
 def single_instance_leg(name, qp, oracle_ref=None):
@@ -572,6 +579,44 @@ def main():
               "qps_per_s": args.count / (time.perf_counter() - t0)}
   per_iter["statuses_equal"] = [s_.status.value for s_ in bq.results()] == statuses
 
+  def timed_solves(reps, **kw):
+    ts, cn = [], None
+    for _ in range(reps):
+      t0 = time.perf_counter()
+      cn = bq.solve_device(**kw)
+      ts.append(time.perf_counter() - t0)
+    return 1e3 * float(np.mean(ts)), cn
+
+  # the same 4096 QPs with the plain FP64 factor (the precision pattern's effect)
+  log("batched FP64 (same batch, FP64 factor)")
+  bq.solve_device(precision="fp64")
+  ms64, cnt64 = timed_solves(args.steps, precision="fp64")
+  sols64 = bq.results()
+  work64 = bq.work_counts()
+  batched_fp64 = {
+      "value": args.count / (ms64 / 1e3), "unit": "QPs/s", "ms_per_solve": ms64,
+      "statuses_equal": [s_.status.value for s_ in sols64] == statuses,
+      "iterations_mean": float(np.mean([s_.iters for s_ in sols64])),
+      "factorizations": {"fp32": int(work64[:, 0].sum()), "fp64": int(work64[:, 1].sum())},
+      "refinement_passes": int(work64[:, 2].sum()),
+      "mixed_over_fp64": value / (args.count / (ms64 / 1e3))}
+  # tighter stopping tolerances, where every solved QP is meant to reach a
+  # relative KKT residual of 1e-8 (the metric's accuracy) on the unscaled data
+  tight = {"atol": 1e-10, "rtol": 1e-9}
+  log(f"tight tolerances {tight}")
+  bq.solve_device(precision="mixed", **tight)
+  ms_t, cnt_t = timed_solves(args.steps, precision="mixed", **tight)
+  sols_t = bq.results()
+  kkt_t = np.array([kkt_rel_residual(s_.x, s_.y, s_.s, q.a, q.b, q.c, q.p, q.z)
+                    for s_, q in zip(sols_t, qps) if s_.status.value == "solved"])
+  st_t = [s_.status.value for s_ in sols_t]
+  tight_leg = {"value": args.count / (ms_t / 1e3), "unit": "QPs/s", "ms_per_solve": ms_t,
+               "tolerances": tight, "statuses": {k: st_t.count(k) for k in set(st_t)},
+               "iterations_mean": float(np.mean([s_.iters for s_ in sols_t])),
+               "rel_kkt_residual_solved": {"max": float(kkt_t.max()),
+                                           "median": float(np.median(kkt_t)),
+                                           "frac_le_1e-8": float(np.mean(kkt_t <= 1e-8))}}
+
   # roofline of the dominant kernel (the per-iteration IPM kernel)
   flops32, flops64, bytes_ = algorithmic_work(n, m, work, iters)
   iter_ms = float(np.sum(launch_ms))
@@ -597,8 +642,9 @@ def main():
   if args.no_plain:
     plain = None
   else:
-    sample = qps[: args.plain_sample]
-    log(f"plain FP64: {len(sample)} QPs one per call")
+    pidx = np.linspace(0, args.count - 1, args.plain_sample).astype(int)
+    sample = [qps[i] for i in pidx]
+    log(f"plain FP64: {len(sample)} QPs (evenly spaced over the batch) one per call")
     singles = [batch.BatchQP([q]) for q in sample]
     for s_ in singles[:2]:
       s_.solve_device(precision="fp64", use_graph=False)
@@ -607,17 +653,19 @@ def main():
       s_.solve_device(precision="fp64", use_graph=False)
     plain_qps = len(sample) / (time.perf_counter() - t0)
     plain_status_match = all(s_.results()[0].status.value == statuses[i]
-                             for i, s_ in enumerate(singles))
+                             for i, s_ in zip(pidx, singles))
     for s_ in singles:
       s_.free()
-    drop_qps, drop_st = dropin_fp64(sample[:8])
-    drop_match = all(st == statuses[i] for i, st in enumerate(drop_st))
+    didx = pidx[np.linspace(0, len(pidx) - 1, 8).astype(int)]
+    drop_qps, drop_st = dropin_fp64([qps[i] for i in didx])
+    drop_match = all(st == statuses[i] for i, st in zip(didx, drop_st))
     denom = max(plain_qps, drop_qps)
     plain = {"value": denom, "unit": "QPs/s",
              "device_1qp_per_call": {"value": plain_qps, "sample": len(sample),
                                      "status_match": plain_status_match},
-             "dropin_driver_1qp_per_call": {"value": drop_qps, "sample": len(sample[:8]),
+             "dropin_driver_1qp_per_call": {"value": drop_qps, "sample": len(didx),
                                             "status_match": drop_match},
+             "sample_indices": "evenly spaced over the batch",
              "speedup_of_batched_mixed": value / denom}
 
   # accuracy against the FP64 oracle on a sample
@@ -661,7 +709,10 @@ def main():
       "instance_completion_ms": {"p50": float(np.percentile(done_ms, 50)),
                                  "p99": float(np.percentile(done_ms, 99)),
                                  "prepare_ms": prep_ms},
-      "rel_kkt_residual_solved": {"max": float(np.max(kkt)), "median": float(np.median(kkt))},
+      "rel_kkt_residual_solved": {"max": float(np.max(kkt)), "median": float(np.median(kkt)),
+                                  "frac_le_1e-8": float(np.mean(np.array(kkt) <= 1e-8))},
+      "batched_fp64": batched_fp64,
+      "tight_tolerance": tight_leg,
       "statuses": {s: statuses.count(s) for s in set(statuses)},
       "oracle_check": {"sample": len(errs) // 3, "status_match": bool(status_ok),
                        "max_rel_err_xys": max(errs) if errs else None},
