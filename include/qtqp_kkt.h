@@ -83,6 +83,23 @@ int qtqp_kkt_solve(qtqp_kkt* h, const double* rhs, double* out);
  * solvers_dense.py:118-128). */
 int qtqp_kkt_matvec(qtqp_kkt* h, const double* x, double* out);
 
+/* out = K_reg x, as qtqp_kkt_matvec, on caller-owned device memory and with
+ * no allocation of any kind.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc).  a: m x n row-major A, poff: n x n
+ * row-major P with a zero diagonal (NULL when P = 0), diag: the N = n+m KKT
+ * diagonal (y-block negated), x and out: N values; all device pointers.  The
+ * kernels are enqueued on `stream` (a hipStream_t; NULL is the null stream)
+ * and the call returns without waiting: out is ready when that stream's work
+ * is.  Their partial sums live in a fixed 16 MiB scratch array per GPU, held
+ * until the process ends; a size whose partials need more,
+ *   (n + m) * ceil(n / 256) + ceil(m / 64) * n + n  >  2^21 doubles,
+ * is rejected with QTQP_ERR_ARG (n = 4000, m = 6000 needs 0.54e6).  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (two streams,
+ * two threads) race and must be serialised by the caller. */
+int qtqp_kkt_matvec_noalloc(int64_t n, int64_t m, const double* a, const double* poff,
+                            const double* diag, const double* x, double* out, void* stream);
+
 /* Stores diag_correction = reg - true (N values), used by solve_ir.
  * Replaces the host-side buffer of DirectKktSolver.update (direct.py:216). */
 int qtqp_kkt_set_correction(qtqp_kkt* h, const double* corr);

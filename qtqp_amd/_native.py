@@ -75,6 +75,9 @@ KKT_SYMBOLS = {
     "qtqp_kkt_solve": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
     "qtqp_kkt_matvec": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p]),
     "qtqp_kkt_set_correction": (ctypes.c_int, [ctypes.c_void_p, _c_double_p]),
+    "qtqp_kkt_matvec_noalloc": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
     "qtqp_kkt_solve_ir": (ctypes.c_int, [ctypes.c_void_p, _c_double_p, _c_double_p, ctypes.c_int32,
                                          ctypes.c_double, ctypes.c_double, _c_double_p,
                                          ctypes.POINTER(IrStats)]),

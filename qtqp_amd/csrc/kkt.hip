@@ -1,10 +1,55 @@
 // LinearSolver backend of the single-instance dense KKT engine: the C ABI of
 // include/qtqp_kkt.h over the kernels and helpers of kkt_engine.h.
+#include <mutex>
+
 #include "kkt_engine.h"
 
 using namespace qk;
 
 thread_local std::string g_qtqp_last_error;
+
+namespace {
+
+// Scratch of qtqp_kkt_matvec_noalloc: the row partials of A x and P_off x and
+// the column partials and total of A'y.  A file-scope device array, so the
+// entry point allocates nothing; every GPU holds its own copy until the
+// process ends, and its address is taken once per device with
+// hipGetSymbolAddress (the array's name is not a device address on the host).
+constexpr int64_t kNoAllocScratchDoubles = (int64_t)1 << 21;  // 16 MiB
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array.
+int scratch_ptr(double** out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(QTQP_ERR_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return QTQP_OK;
+}
+
+// Doubles of scratch a product with K of this size takes: row partials of
+// A x (m per 256-column tile) and of P_off x (n per tile), column partials of
+// A'y (n per 64-row tile), and A'y itself.
+int64_t noalloc_scratch_doubles(int64_t n, int64_t m) {
+  return (m + n) * tiles_c(n) + (int64_t)tiles_r(m) * n + n;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -75,6 +120,39 @@ int qtqp_kkt_matvec(qtqp_kkt* h, const double* x, double* out) {
   if (rc) return rc;
   HIPC(hipMemcpyAsync(out, h->vout, sizeof(double) * N, hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
+  return QTQP_OK;
+}
+
+int qtqp_kkt_matvec_noalloc(int64_t n, int64_t m, const double* a, const double* poff,
+                            const double* diag, const double* x, double* out, void* stream) {
+  if (!a || !diag || !x || !out) return fail(QTQP_ERR_ARG, "NULL argument");
+  if (n < 1 || m < 1 || n > (int64_t)1 << 30 || m > (int64_t)1 << 30)
+    return fail(QTQP_ERR_ARG, "n and m must be in [1, 2^30]");
+  const int64_t need = noalloc_scratch_doubles(n, m);
+  if (need > kNoAllocScratchDoubles) {
+    char buf[200];
+    snprintf(buf, sizeof(buf),
+             "problem too large for the fixed scratch: needs %lld doubles, the maximum is %lld "
+             "(16 MiB)", (long long)need, (long long)kNoAllocScratchDoubles);
+    return fail(QTQP_ERR_ARG, buf);
+  }
+  double* sc = nullptr;
+  int rc = scratch_ptr(&sc);
+  if (rc) return rc;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  double* rpa = sc;                             // m * tiles_c(n)
+  double* rpp = rpa + m * tiles_c(n);           // n * tiles_c(n)
+  double* cpart = rpp + n * tiles_c(n);         // tiles_r(m) * n
+  double* aty = cpart + (int64_t)tiles_r(m) * n;  // n
+  if ((rc = tile_pass_on<double>(st, cpart, a, m, n, x, rpa, x + n, nullptr))) return rc;
+  if (poff && (rc = tile_pass_on<double>(st, cpart, poff, n, n, x, rpp, nullptr, nullptr)))
+    return rc;
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, st, cpart, tiles_r(m), (int)n,
+                     nullptr, aty);
+  hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(n + m)), dim3(WG), 0, st, (int)n, (int)m, x,
+                     diag, poff ? rpp : nullptr, rpa, tiles_c(n), aty, nullptr, nullptr, out,
+                     nullptr);
+  HIPC(hipGetLastError());
   return QTQP_OK;
 }
 
