@@ -44,6 +44,11 @@ typedef struct qtqp_ipm_opts {
   double linear_solver_atol, linear_solver_rtol;  /* refinement tolerances */
   int32_t max_iterative_refinement_steps;
   int32_t collect_stats;             /* nonzero: fill the collect_stats fields */
+  int32_t use_graph;    /* nonzero: replay the iteration from a HIP graph with the
+                           refinement and precision decisions made on the device
+                           (0: host-driven launches, one readback per refinement step) */
+  int32_t graph_steps;  /* refinement steps per refined solve held in the graph (0: 3);
+                           an iteration that needs more is redone host-driven */
 } qtqp_ipm_opts;
 
 /* Status after an iteration (SolutionStatus, __init__.py:129-137). */
@@ -80,7 +85,9 @@ int qtqp_ipm_create(int64_t n, int64_t m, int64_t z, const double* a, const doub
                     int32_t precision, int32_t equilibrate, qtqp_ipm** out);
 
 /* Runs one IPM iteration (__init__.py:473-566) and its termination check
- * (888-1008); fills *rec.  Returns QTQP_ERR_FACTOR on a breakdown of the FP64
+ * (888-1008); fills *rec.  With opts->use_graph the iteration is one launch
+ * of a captured HIP graph and one readback of the record (the graph is
+ * captured on the first call and again when the options change).  Returns QTQP_ERR_FACTOR on a breakdown of the FP64
  * factor and QTQP_ERR_NAN when a refined solve has NaNs, as the reference's
  * backends raise LinAlgError and ValueError. */
 int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* opts, qtqp_ipm_record* rec);

@@ -55,6 +55,7 @@ typedef struct qtqp_kkt_info {
   double factor_ms;           /* device time in factorisations (HIP events) */
   double solve_ms;            /* device time in solves and matvecs */
   int64_t jitter_retries;     /* FP64 factors redone with a larger diagonal shift */
+  int64_t graph_redos;        /* device-loop iterations redone host-driven (qtqp_ipm) */
 } qtqp_kkt_info;
 
 /* Creates a handle for an (n+m)x(n+m) KKT system.  Replaces

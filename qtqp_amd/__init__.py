@@ -267,7 +267,9 @@ class QTQP:
               min_static_regularization=min_static_regularization,
               linear_solver_atol=linear_solver_atol, linear_solver_rtol=linear_solver_rtol,
               max_iterative_refinement_steps=max_iterative_refinement_steps,
-              collect_stats=int(bool(collect_stats))))
+              collect_stats=int(bool(collect_stats)),
+              use_graph=int(bool(getattr(backend, "device_graph", True))),
+              graph_steps=int(getattr(backend, "device_graph_steps", 0))))
     if equilibrate:
       a, p, b, c, self.d, self.e = self._equilibrate()
     else:

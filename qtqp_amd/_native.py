@@ -40,7 +40,7 @@ class KktInfo(ctypes.Structure):
               ("factorizations", ctypes.c_int64), ("fp64_fallbacks", ctypes.c_int64),
               ("backend_solves", ctypes.c_int64), ("inner_steps", ctypes.c_int64),
               ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
-              ("jitter_retries", ctypes.c_int64)]
+              ("jitter_retries", ctypes.c_int64), ("graph_redos", ctypes.c_int64)]
 
 
 class IpmOpts(ctypes.Structure):
@@ -50,7 +50,8 @@ class IpmOpts(ctypes.Structure):
               ("step_size_scale", ctypes.c_double), ("min_static_regularization", ctypes.c_double),
               ("linear_solver_atol", ctypes.c_double), ("linear_solver_rtol", ctypes.c_double),
               ("max_iterative_refinement_steps", ctypes.c_int32),
-              ("collect_stats", ctypes.c_int32)]
+              ("collect_stats", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+              ("graph_steps", ctypes.c_int32)]
 
 
 IPM_RECORD_DOUBLES = ("ctx", "bty", "pcost", "dcost", "pres", "dres", "gap", "pinfeas", "dinfeas",

@@ -3,9 +3,27 @@
 // qtqp's iteration loop (src/qtqp/__init__.py:473-578) with the KKT engine of
 // kkt_engine.h underneath: the iterate (x, y, tau, s), K^{-1}q, the Newton
 // directions and every scalar of an iteration (mu, tau+, alpha, sigma, the
-// termination quantities) live in HBM.  The host launches one iteration's
-// kernels and reads back one record; the only other host synchronisations are
-// the factor's breakdown flag and the refinement loop's residual norms.
+// termination quantities) live in HBM.
+//
+// An iteration is one fixed sequence of kernel launches, captured once in a
+// HIP graph and replayed: one graph launch and one record read back per IPM
+// iteration.  What the host loop decided between launches is decided on the
+// device instead, in the control block `Ctl`:
+//  - the factor precision (an FP32 chain and an FP64 chain are both in the
+//    sequence; a gate word lets one of them run, and an FP32 breakdown opens
+//    the FP64 one);
+//  - each refinement loop of DirectKktSolver.solve (direct.py:243-291): the
+//    sequence holds a fixed number of unrolled steps, each switched on by the
+//    loop's gate while the loop is still running, and a one-thread kernel
+//    after every step applies the reference's stopping rules to the FP64
+//    residual norm the step left in device memory;
+//  - the tau method (__init__.py:741-752) from the refinement result.
+// When an iteration needs what the sequence does not hold (more refinement
+// steps than unrolled, an FP64 refactor after a slowly contracting FP32
+// step, or a retry of an FP64 factor that broke down), the control block
+// says so; the host restores the iteration's saved start (x, y, s, tau,
+// K^{-1}q) and runs that iteration again with the host-driven launches of
+// the same kernels, which loop as long as the reference does.
 //
 // Each vector phase is a grid-stride kernel whose workgroups write partial
 // sums / maxima / minima to `vpart` (kSlots per workgroup), followed by a
@@ -16,6 +34,8 @@
 // Termination uses the equilibrated data: with A_eq = D A E, P_eq = E P E and
 // the equilibrated iterate, A x_u = (A_eq x)/d, A' y_u = (A_eq' y)/e and
 // P x_u = (P_eq x)/e, while b and c are the unscaled vectors.
+#include <cstddef>
+
 #include "kkt_engine.h"
 #include "qtqp_ipm.h"
 
@@ -355,14 +375,18 @@ __global__ __launch_bounds__(WG) void k_tau_part(int n, int m, int has_p, const 
 // (_solve_for_tau, __init__.py:759-817) when `quadratic` is allowed and the
 // root is valid, else the linearised step with its [0.1, 10] trust region
 // (_solve_for_tau_linearized_fallback, __init__.py:819-857).
+// quad_res: when given, the refined solve's (status, residual) on the device,
+// which decide `quadratic` as the host would (__init__.py:741-752).
 __global__ __launch_bounds__(WG) void k_tau_fin(int corrector, int quadratic, int has_p,
                                                 const double* part, int nb, Scal* S,
-                                                qtqp_ipm_record* rec) {
+                                                qtqp_ipm_record* rec, const int* quad_status,
+                                                const double* quad_res) {
   __shared__ double red[4];
   double v[11];
 #pragma unroll
   for (int t = 0; t < 11; ++t) v[t] = fin_sum(part, nb, t, red);
   if (threadIdx.x != 0) return;
+  if (quad_status) quadratic = *quad_status == 0 || *quad_res < 1e-7;
   const double mu = S->mu, tau = S->tau;
   const double mu_t = corrector ? S->sigma * mu : 0.0;
   const double tau_anchor = corrector ? S->tau_p : tau;
@@ -721,6 +745,185 @@ __global__ __launch_bounds__(WG) void k_std_fin(int m, int z, const double* part
   rec->std_s_over_y = sqrt(b / (m - z));
 }
 
+
+// ---------------------------------------------------------------------------
+// Device control of an iteration replayed from a graph
+// ---------------------------------------------------------------------------
+// An FP32 step contracting the residual by less than this asks for FP64 (the
+// same bound as the host-driven refinement's); kF32StepsPerSolve
+// (kkt_engine.h) bounds the mean FP32 steps per refined solve.
+constexpr double kPoorContraction = 0.25;
+
+// One refinement loop (DirectKktSolver.solve, direct.py:243-307).
+struct IrCtl {
+  unsigned long long norm[2];  // ||res||_inf bits and NaN flag of the last residual
+  unsigned long long rhs_bits; // ||krhs||_inf bits
+  double res, old, tol, rhs_norm;
+  int status, solves, nan, pad;
+  Gate gate;  // active while the loop runs; prec = precision of the factor
+};
+
+struct Ctl {
+  IrCtl ir[3];  // kinv_q, predictor, corrector
+  Gate fgate;   // prec selects the factor chain that runs
+  int info32, info64;
+  // state across iterations (the host sets it after an iteration it ran itself)
+  int force64, fprec, prev_steps, prev_fprec;
+  // this iteration
+  int redo;      // 1: refinement needs more steps than unrolled, 2: FP32 contracted
+                 // too slowly (FP64 from now on), 3: FP64 factor broke down
+  int fact_col;  // breakdown column of the FP64 factor
+  int steps, f32_steps;
+  // cumulative counters
+  long long factorizations, fp64_fallbacks, backend_solves, inner_steps;
+  unsigned long long t0, factor_ticks;
+};
+
+__device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// Saves the iteration's start (xy, s, K^{-1}q, scalars) and resets the
+// control block; decides the factor precision of this iteration.
+__global__ void k_iter_begin(Ctl* c, int mixed, int64_t N, int64_t m, const double* __restrict__ xy,
+                             const double* __restrict__ s, const double* __restrict__ kq,
+                             const Scal* S, double* __restrict__ xy_sv, double* __restrict__ s_sv,
+                             double* __restrict__ kq_sv, Scal* S_sv) {
+  GRID_STRIDE(k, N) {
+    xy_sv[k] = xy[k];
+    kq_sv[k] = kq[k];
+    if (k < m) s_sv[k] = s[k];
+  }
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  *S_sv = *S;
+  for (int r = 0; r < 3; ++r) {
+    IrCtl& ir = c->ir[r];
+    ir.norm[0] = ir.norm[1] = 0ull;
+    ir.rhs_bits = 0ull;
+    ir.status = 1;
+    ir.solves = 0;
+    ir.nan = 0;
+    ir.gate.active = 0;
+  }
+  c->info32 = c->info64 = 0;
+  c->redo = 0;
+  c->fact_col = 0;
+  // an FP32 factor whose three refined solves of the last iteration took more
+  // than kF32StepsPerSolve steps each on average costs more than an FP64 one
+  int use32 = mixed && !c->force64;
+  if (use32 && c->prev_fprec == 32 && c->prev_steps > 3 * kF32StepsPerSolve) {
+    c->force64 = 1;
+    c->fp64_fallbacks++;
+    use32 = 0;
+  }
+  c->steps = c->f32_steps = 0;
+  c->fgate.active = 1;
+  c->fgate.prec = use32 ? 32 : 64;
+  c->factorizations++;
+  c->t0 = wall_clock64();
+}
+
+// After the FP32 chain: a breakdown opens the FP64 chain (and FP64 stays).
+__global__ void k_fact_mid(Ctl* c) {
+  if (c->fgate.prec == 32 && c->info32 != 0) {
+    c->fgate.prec = 64;
+    c->force64 = 1;
+    c->fp64_fallbacks++;
+  }
+}
+
+// After the FP64 chain: a breakdown asks the host for the retried factor.
+__global__ void k_fact_end(Ctl* c) {
+  if (c->fgate.prec == 64 && c->info64 != 0) {
+    c->redo = 3;
+    c->fact_col = c->info64;
+  }
+  c->fprec = c->fgate.prec;
+  c->factor_ticks += wall_clock64() - c->t0;
+}
+
+// Refinement control after the residual of step `step` (0: the initial
+// residual of the warm start).  Stops as DirectKktSolver.solve does
+// (direct.py:259-283): converged below tol, stalled when the residual stops
+// decreasing, or max_steps solves; an FP32 step contracting by less than
+// kPoorContraction, or a loop that needs more than the `unrolled` steps in
+// the sequence, hands the iteration back to the host.
+__global__ void k_ir_ctl(Ctl* c, int r, int step, int max_steps, int unrolled, double atol,
+                         double rtol) {
+  IrCtl& ir = c->ir[r];
+  const double nv = bits_to_double(ir.norm[0]);
+  const int nan = ir.norm[1] != 0ull;
+  if (step == 0) {
+    ir.rhs_norm = bits_to_double(ir.rhs_bits);
+    ir.tol = fma(rtol, ir.rhs_norm, atol);  // one rounding, as the host path computes it
+    ir.res = nv;
+    ir.nan = nan;
+    ir.norm[0] = ir.norm[1] = 0ull;
+    ir.gate.prec = c->fprec;
+    ir.gate.active = 1;
+    return;
+  }
+  if (!ir.gate.active) return;
+  ir.norm[0] = ir.norm[1] = 0ull;
+  ir.old = ir.res;
+  ir.res = nv;
+  ir.nan = nan;
+  ir.solves = step;
+  c->steps++;
+  c->backend_solves++;
+  if (ir.gate.prec == 32) {
+    c->f32_steps++;
+    c->inner_steps++;
+  }
+  if (ir.res < ir.tol) {
+    ir.status = 0;
+    ir.gate.active = 0;
+  } else if (ir.res >= ir.old) {
+    ir.status = 2;
+    ir.gate.active = 0;
+  } else if (ir.gate.prec == 32 && ir.res > kPoorContraction * ir.old) {
+    c->redo = 2;
+    c->force64 = 1;
+    c->fp64_fallbacks++;
+    ir.gate.active = 0;
+  } else if (step >= max_steps) {
+    ir.gate.active = 0;
+  } else if (step >= unrolled) {
+    c->redo = 1;
+    ir.gate.active = 0;
+  }
+}
+
+// The refined solves' statistics into the record; the state the next
+// iteration's precision rule reads.
+__global__ void k_iter_end(Ctl* c, qtqp_ipm_record* rec) {
+  for (int r = 0; r < 3; ++r) {
+    const IrCtl& ir = c->ir[r];
+    rec->lin[r].solves = ir.solves < 1 ? 1 : ir.solves;
+    rec->lin[r].status = ir.status;
+    rec->lin[r].final_residual_norm = ir.res;
+    rec->lin[r].rhs_norm = ir.rhs_norm;
+    rec->lin[r].tolerance = ir.tol;
+  }
+  c->prev_steps = c->steps;
+  c->prev_fprec = c->fprec;
+}
+
+__global__ void k_copy(int64_t count, const double* __restrict__ src, double* __restrict__ dst) {
+  GRID_STRIDE(k, count) dst[k] = src[k];
+}
+
+// Puts back the start saved by k_iter_begin.
+__global__ void k_iter_restore(int64_t N, int64_t m, double* __restrict__ xy, double* __restrict__ s,
+                               double* __restrict__ kq, Scal* S, const double* __restrict__ xy_sv,
+                               const double* __restrict__ s_sv, const double* __restrict__ kq_sv,
+                               const Scal* S_sv) {
+  GRID_STRIDE(k, N) {
+    xy[k] = xy_sv[k];
+    kq[k] = kq_sv[k];
+    if (k < m) s[k] = s_sv[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *S = *S_sv;
+}
+
 }  // namespace
 
 struct qtqp_ipm {
@@ -735,6 +938,16 @@ struct qtqp_ipm {
   double *vpart = nullptr, *cpartp = nullptr, *rowmax = nullptr, *di = nullptr, *ei = nullptr;
   Scal* S = nullptr;
   qtqp_ipm_record* drec = nullptr;
+  // graph path: control block, the saved start of the iteration, graphs
+  Ctl* ctl = nullptr;
+  double *xy_sv = nullptr, *s_sv = nullptr, *kq_sv = nullptr;
+  Scal* S_sv = nullptr;
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    qtqp_ipm_opts opts{};
+  } graphs[2];  // [0]: FP64 factor only, [1]: FP32 and FP64 chains
+  int wall_khz = 100000;  // device wall clock rate
+  int64_t redos = 0;      // iterations handed back to the host-driven launches
 };
 
 namespace {
@@ -743,9 +956,11 @@ void ipm_release(qtqp_ipm* h) {
   void* bufs[] = {h->pdiag, h->b,  h->c,  h->d,  h->e,   h->q,      h->xy,     h->s,
                   h->kq,    h->xp, h->kr, h->r,  h->ds,  h->cm,     h->px,     h->pq,
                   h->pr,    h->ax, h->aty, h->pxe, h->vpart, h->cpartp, h->rowmax, h->di,
-                  h->ei,    h->S,  h->drec};
+                  h->ei,    h->S,  h->drec, h->ctl, h->xy_sv, h->s_sv, h->kq_sv, h->S_sv};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
+  for (auto& g : h->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   if (h->k) {
     release(h->k);
     delete h->k;
@@ -841,7 +1056,18 @@ int qtqp_ipm_create(int64_t n, int64_t m, int64_t z, const double* a, const doub
   ALLOC(ei, double, n);
   ALLOC(S, Scal, 1);
   ALLOC(drec, qtqp_ipm_record, 1);
+  ALLOC(ctl, Ctl, 1);
+  ALLOC(xy_sv, double, N);
+  ALLOC(s_sv, double, m);
+  ALLOC(kq_sv, double, N);
+  ALLOC(S_sv, Scal, 1);
 #undef ALLOC
+  {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      h->wall_khz = khz;
+  }
   const hipStream_t st = k->stream;
   std::vector<double> pd((size_t)n, 0.0);
   if (p)
@@ -857,6 +1083,7 @@ int qtqp_ipm_create(int64_t n, int64_t m, int64_t z, const double* a, const doub
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n)), dim3(WG), 0, st, h->e, n, 1.0);
   if ((e = hipMemsetAsync(h->S, 0, sizeof(Scal), st))) return bad("scalars");
   if ((e = hipMemsetAsync(h->drec, 0, sizeof(qtqp_ipm_record), st))) return bad("record");
+  if ((e = hipMemsetAsync(h->ctl, 0, sizeof(Ctl), st))) return bad("control");
   if ((e = hipGetLastError())) return bad("fill");
   if (equilibrate && (rc = ruiz(h))) {
     ipm_release(h);
@@ -876,10 +1103,14 @@ int qtqp_ipm_create(int64_t n, int64_t m, int64_t z, const double* a, const doub
   return QTQP_OK;
 }
 
-int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
-  if (!h || !o || !rec) return fail(QTQP_ERR_ARG, "NULL argument");
-  if (o->max_iterative_refinement_steps < 1)
-    return fail(QTQP_ERR_ARG, "max_iterative_refinement_steps must be >= 1");
+}  // extern "C"
+
+namespace {
+
+// One iteration with host-driven launches: the host reads the factor's
+// breakdown flag and every refinement residual back and loops as the
+// reference does.
+int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
   qtqp_kkt* k = h->k;
   const hipStream_t st = k->stream;
   const int n = (int)h->n, m = (int)h->m, z = (int)h->z;
@@ -931,7 +1162,7 @@ int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) 
     hipLaunchKernelGGL(k_tau_part, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
                        h->pr, h->px, part);
     hipLaunchKernelGGL(k_tau_fin, g1, gb, 0, st, corrector, quadratic, h->has_p, part, nv, h->S,
-                       h->drec);
+                       h->drec, nullptr, nullptr);
     hipLaunchKernelGGL(k_newton_fin, dim3(gN), gb, 0, st, N, corrector, h->kr, h->kq, h->xp, h->S);
     HIPC(hipGetLastError());
   }
@@ -969,6 +1200,230 @@ int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) 
   return QTQP_OK;
 }
 
+constexpr int kDefaultGraphSteps = 3;
+
+// One refinement loop of the fixed sequence: krhs = [rhs_x; -rhs_y], the
+// residual of the warm start in sol, then `steps` gated steps sol +=
+// K_reg^{-1} res with the factor in its current precision (both precisions'
+// kernels are in the sequence when `both`), each followed by its control.
+int refine_seq(qtqp_ipm* h, int r, const double* rhs, double* sol, bool both, int steps,
+               const qtqp_ipm_opts* o) {
+  qtqp_kkt* k = h->k;
+  const hipStream_t st = k->stream;
+  const int64_t n = h->n, N = h->n + h->m;
+  IrCtl* ic = &h->ctl->ir[r];
+  const Gate* g = &ic->gate;
+  const int max_steps = o->max_iterative_refinement_steps;
+  const double atol = o->linear_solver_atol, rtol = o->linear_solver_rtol;
+  int rc;
+  hipLaunchKernelGGL(k_negate_y, dim3(blocks_for(N)), dim3(WG), 0, st, N, n, rhs, k->vin,
+                     &ic->rhs_bits);
+  if ((rc = kmatvec_dev(k, sol, k->vin, k->corr, k->vres, ic->norm))) return rc;
+  hipLaunchKernelGGL(k_ir_ctl, dim3(1), dim3(1), 0, st, h->ctl, r, 0, max_steps, steps, atol, rtol);
+  for (int step = 1; step <= std::min(steps, max_steps); ++step) {
+    if (both && (rc = solve_with<float>(k, k->A32, k->G32, k->vres, k->vout, 0, g, 32))) return rc;
+    if ((rc = solve_with<double>(k, k->A64, k->G64, k->vres, k->vout, 0, g, 64))) return rc;
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(N)), dim3(WG), 0, st, N, 1.0, k->vout, sol, g);
+    if ((rc = kmatvec_dev(k, sol, k->vin, k->corr, k->vres, ic->norm, g))) return rc;
+    hipLaunchKernelGGL(k_ir_ctl, dim3(1), dim3(1), 0, st, h->ctl, r, step, max_steps, steps, atol,
+                       rtol);
+  }
+  HIPC(hipGetLastError());
+  return QTQP_OK;
+}
+
+// The launches of one whole iteration with no host synchronisation: the
+// sequence a graph captures.  `both`: the FP32 factor chain and solves are
+// in it (a mixed handle that has not fallen back to FP64).
+int launch_iteration(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, int steps) {
+  qtqp_kkt* k = h->k;
+  const hipStream_t st = k->stream;
+  const int n = (int)h->n, m = (int)h->m, z = (int)h->z;
+  const int64_t N = h->n + h->m;
+  const int nv = h->nv;
+  const dim3 gv(nv), gb(WG), g1(1);
+  const unsigned gN = blocks_for(N);
+  double* part = h->vpart;
+  Ctl* c = h->ctl;
+  int rc;
+  hipLaunchKernelGGL(k_iter_begin, dim3(gN), gb, 0, st, c, both ? 1 : 0, N, (int64_t)m, h->xy, h->s,
+                     h->kq, h->S, h->xy_sv, h->s_sv, h->kq_sv, h->S_sv);
+  // normalise, mu, regularised diagonal (__init__.py:474-478, direct.py:183-216)
+  hipLaunchKernelGGL(k_norm_part, gv, gb, 0, st, n, m, h->xy, h->s, part);
+  hipLaunchKernelGGL(k_norm_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+  hipLaunchKernelGGL(k_scale_diag, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->pdiag,
+                     o->min_static_regularization, h->S, k->diag, k->corr);
+  HIPC(hipGetLastError());
+  // factor: the FP32 chain, then the FP64 chain if FP64 or after an FP32 breakdown
+  if (both && (rc = factor_launch<float>(k, k->A32, k->Poff32, k->G32, &c->info32, kJitterRel[0],
+                                         &c->fgate, 32)))
+    return rc;
+  hipLaunchKernelGGL(k_fact_mid, g1, g1, 0, st, c);
+  if ((rc = factor_launch<double>(k, k->A64, k->Poff64, k->G64, &c->info64, kJitterRel[0],
+                                  &c->fgate, 64)))
+    return rc;
+  hipLaunchKernelGGL(k_fact_end, g1, g1, 0, st, c);
+  // kinv_q = K^{-1} q, warm-started from the previous one (__init__.py:479-481)
+  if ((rc = refine_seq(h, 0, h->q, h->kq, both, steps, o))) return rc;
+  if (h->has_p)
+    hipLaunchKernelGGL(k_pmul<2>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
+                       h->px, h->kq, h->pq);
+  for (int corrector = 0; corrector < 2; ++corrector) {
+    if (corrector) {
+      hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 0, h->xy, h->s, h->xp, h->cm, h->ds,
+                         h->S, part);
+      hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 0, o->step_size_scale, part, nv, h->S);
+      hipLaunchKernelGGL(k_sigma_part, gv, gb, 0, st, n, m, h->xy, h->s, h->xp, h->ds, h->S, part);
+      hipLaunchKernelGGL(k_sigma_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+    }
+    hipLaunchKernelGGL(k_newton_rhs, dim3(gN), gb, 0, st, n, m, z, corrector, h->xy, h->s, h->ds,
+                       h->xp, h->cm, h->r, h->S);
+    hipLaunchKernelGGL(k_copy, dim3(gN), gb, 0, st, N, h->xp, h->kr);
+    if ((rc = refine_seq(h, 1 + corrector, h->r, h->kr, both, steps, o))) return rc;
+    if (h->has_p)
+      hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->kr,
+                         h->pr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_tau_part, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
+                       h->pr, h->px, part);
+    const IrCtl* ic = &c->ir[1 + corrector];
+    hipLaunchKernelGGL(k_tau_fin, g1, gb, 0, st, corrector, 0, h->has_p, part, nv, h->S, h->drec,
+                       &ic->status, &ic->res);
+    hipLaunchKernelGGL(k_newton_fin, dim3(gN), gb, 0, st, N, corrector, h->kr, h->kq, h->xp, h->S);
+    HIPC(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 1, h->xy, h->s, h->xp, h->cm, h->ds, h->S,
+                     part);
+  hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 1, o->step_size_scale, part, nv, h->S);
+  hipLaunchKernelGGL(k_update, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->xp, h->ds, h->S);
+  HIPC(hipGetLastError());
+  // termination (__init__.py:888-1008)
+  if ((rc = tile_pass<double>(k, k->A64, m, n, h->xy, k->rpa, h->xy + n, nullptr))) return rc;
+  hipLaunchKernelGGL(k_rowfin, dim3(blocks_for(m)), gb, 0, st, k->rpa, tiles_c(n), m, h->ax);
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), gb, 0, st, k->part, tiles_r(m), n, nullptr,
+                     h->aty, nullptr, 0);
+  if (h->has_p)
+    hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
+                       h->pxe, nullptr, nullptr);
+  const int collect = o->collect_stats ? 1 : 0;
+  hipLaunchKernelGGL(k_term_part, gv, gb, 0, st, n, m, z, h->has_p, collect, h->xy, h->s, h->d,
+                     h->e, h->ax, h->aty, h->pxe, h->b, h->c, h->S, part);
+  const Tol tl{o->atol, o->rtol, o->atol_infeas, o->rtol_infeas};
+  hipLaunchKernelGGL(k_term_fin, g1, gb, 0, st, m, z, h->has_p, collect, tl, part, nv, h->S,
+                     h->drec);
+  if (collect) {
+    hipLaunchKernelGGL(k_std_part, gv, gb, 0, st, n, m, z, h->xy, h->s, h->d, h->S, part);
+    hipLaunchKernelGGL(k_std_fin, g1, gb, 0, st, m, z, part, nv, h->drec);
+  }
+  hipLaunchKernelGGL(k_iter_end, g1, g1, 0, st, c, h->drec);
+  HIPC(hipGetLastError());
+  return QTQP_OK;
+}
+
+// Options that change the captured sequence.
+bool same_graph_opts(const qtqp_ipm_opts& a, const qtqp_ipm_opts& b) {
+  return a.atol == b.atol && a.rtol == b.rtol && a.atol_infeas == b.atol_infeas &&
+         a.rtol_infeas == b.rtol_infeas && a.step_size_scale == b.step_size_scale &&
+         a.min_static_regularization == b.min_static_regularization &&
+         a.linear_solver_atol == b.linear_solver_atol &&
+         a.linear_solver_rtol == b.linear_solver_rtol &&
+         a.max_iterative_refinement_steps == b.max_iterative_refinement_steps &&
+         a.collect_stats == b.collect_stats && a.graph_steps == b.graph_steps;
+}
+
+// The graph of one iteration for these options (captured on first use).
+int iteration_graph(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, hipGraphExec_t* out) {
+  auto& slot = h->graphs[both ? 1 : 0];
+  if (slot.exec && same_graph_opts(slot.opts, *o)) {
+    *out = slot.exec;
+    return QTQP_OK;
+  }
+  if (slot.exec) {
+    HIPC(hipGraphExecDestroy(slot.exec));
+    slot.exec = nullptr;
+  }
+  const hipStream_t st = h->k->stream;
+  const int steps = o->graph_steps > 0 ? o->graph_steps : kDefaultGraphSteps;
+  HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = launch_iteration(h, o, both, steps);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc != QTQP_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != QTQP_OK) return rc;
+    return fail(QTQP_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess)
+    return fail(QTQP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+  slot.exec = exec;
+  slot.opts = *o;
+  *out = exec;
+  return QTQP_OK;
+}
+
+// The state of the control block the next iteration's precision rule reads,
+// after an iteration the host ran itself.
+struct CtlCarry {
+  int force64, fprec, prev_steps, prev_fprec;
+};
+
+int iterate_graph(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
+  qtqp_kkt* k = h->k;
+  const hipStream_t st = k->stream;
+  const bool both = k->precision == QTQP_PREC_MIXED && !k->force64;
+  hipGraphExec_t exec = nullptr;
+  int rc = iteration_graph(h, o, both, &exec);
+  if (rc) return rc;
+  HIPC(hipGraphLaunch(exec, st));
+  qtqp_ipm_record dev{};
+  Ctl c{};
+  HIPC(hipMemcpyAsync(&dev, h->drec, sizeof(dev), hipMemcpyDeviceToHost, st));
+  HIPC(hipMemcpyAsync(&c, h->ctl, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  k->force64 = c.force64 != 0;
+  if (c.redo) {
+    // Back to the iteration's start, then the host-driven launches, which
+    // loop as long as the reference does (and retry a broken-down factor).
+    h->redos++;
+    const int64_t N = h->n + h->m;
+    hipLaunchKernelGGL(k_iter_restore, dim3(blocks_for(N)), dim3(WG), 0, st, N, h->m, h->xy, h->s,
+                       h->kq, h->S, h->xy_sv, h->s_sv, h->kq_sv, h->S_sv);
+    HIPC(hipGetLastError());
+    k->f32_solves = k->f32_steps = 0;
+    k->fprec = c.fprec;
+    rc = iterate_eager(h, o, rec);
+    if (rc != QTQP_OK) return rc;
+    int steps = 0;
+    for (int t = 0; t < 3; ++t) steps += rec->lin[t].solves;
+    const CtlCarry carry{k->force64 ? 1 : 0, k->fprec, steps, k->fprec};
+    HIPC(hipMemcpyAsync(reinterpret_cast<char*>(h->ctl) + offsetof(Ctl, force64), &carry,
+                        sizeof(carry), hipMemcpyHostToDevice, st));
+    HIPC(hipStreamSynchronize(st));
+    return QTQP_OK;
+  }
+  for (int t = 0; t < 3; ++t)
+    if (c.ir[t].nan) return fail(QTQP_ERR_NAN, "Linear solver returned NaNs.");
+  k->fprec = c.fprec;
+  k->have_diag = true;
+  k->have_factor = true;
+  *rec = dev;
+  return QTQP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qtqp_ipm_iterate(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
+  if (!h || !o || !rec) return fail(QTQP_ERR_ARG, "NULL argument");
+  if (o->max_iterative_refinement_steps < 1)
+    return fail(QTQP_ERR_ARG, "max_iterative_refinement_steps must be >= 1");
+  if (o->graph_steps < 0) return fail(QTQP_ERR_ARG, "graph_steps must be >= 0");
+  return o->use_graph ? iterate_graph(h, o, rec) : iterate_eager(h, o, rec);
+}
+
 int qtqp_ipm_get_iterates(qtqp_ipm* h, double* x, double* y, double* s, double* tau) {
   if (!h || !x || !y || !s || !tau) return fail(QTQP_ERR_ARG, "NULL argument");
   const int64_t n = h->n, m = h->m;
@@ -993,8 +1448,17 @@ int qtqp_ipm_get_iterates(qtqp_ipm* h, double* x, double* y, double* s, double* 
 
 int qtqp_ipm_get_info(qtqp_ipm* h, qtqp_kkt_info* info) {
   if (!h || !info) return fail(QTQP_ERR_ARG, "NULL argument");
+  Ctl c{};
+  HIPC(hipMemcpy(&c, h->ctl, sizeof(c), hipMemcpyDeviceToHost));
   *info = h->k->info_acc;
   info->factor_precision = h->k->fprec;
+  // the graph path's counters (the host-driven path counts in info_acc)
+  info->factorizations += c.factorizations;
+  info->fp64_fallbacks += c.fp64_fallbacks;
+  info->backend_solves += c.backend_solves;
+  info->inner_steps += c.inner_steps;
+  info->factor_ms += (double)c.factor_ticks / h->wall_khz;
+  info->graph_redos = h->redos;
   return QTQP_OK;
 }
 

@@ -42,6 +42,18 @@ namespace {
 
 constexpr int RB = 16;  // rows per workgroup in the row/column passes over A
 
+// Switch of a kernel in a fixed launch sequence (one IPM iteration replayed
+// from a HIP graph): the kernel returns at once unless `active` is set and,
+// for want != 0, the precision `prec` of the current factor is `want`.  The
+// protocol entry points pass no gate (always run).
+struct Gate {
+  int active;
+  int prec;  // 32 or 64
+};
+__device__ inline bool gated_off(const Gate* g, int want) {
+  return g != nullptr && (g->active == 0 || (want != 0 && g->prec != want));
+}
+
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
@@ -76,9 +88,11 @@ __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
                                                   const T* __restrict__ Poff,
                                                   const double* __restrict__ diag,
                                                   int n, int m, T* __restrict__ G,
-                                                  int64_t ld, int ntiles) {
+                                                  int64_t ld, int ntiles, const Gate* gate,
+                                                  int want) {
   __shared__ T si[kch<T>()][NB + 1];
   __shared__ T sj[kch<T>()][NB + 1];
+  if (gated_off(gate, want)) return;
   int ti, tj;
   if (!xcd_tile(ntiles, &ti, &tj)) return;
   const int i0 = ti * NB, j0 = tj * NB;
@@ -102,8 +116,10 @@ __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
 // diag(G) += rel * max(diag(G)); rel = 1e-14 as solvers_dense.py:107-112 (and
 // larger on a retry after a breakdown).  One workgroup.
 template <typename T>
-__global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n, double rel) {
+__global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n, double rel,
+                                               const Gate* gate, int want) {
   __shared__ double red[4];
+  if (gated_off(gate, want)) return;
   double mx = -INFINITY;
   for (int i = threadIdx.x; i < n; i += WG) mx = fmax(mx, (double)G[(int64_t)i * ld + i]);
   mx = block_max(mx, red);
@@ -117,10 +133,11 @@ __global__ __launch_bounds__(WG) void k_jitter(T* G, int64_t ld, int n, double r
 // sweep).  One workgroup.
 template <typename T>
 __global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int k0, int* info,
-                                                  T* linv) {
+                                                  T* linv, const Gate* gate, int want) {
   __shared__ T S[NB][NB + 1];
   __shared__ T Li[NB][NB + 1];
   __shared__ T sinv[NB];
+  if (gated_off(gate, want)) return;
   const int nb = min(NB, n - k0);
   load_diag_block(G, ld, k0, nb, S);
   __syncthreads();
@@ -137,8 +154,9 @@ __global__ __launch_bounds__(WG) void k_potrf_diag(T* G, int64_t ld, int n, int 
 // workgroup b handling rows k0+NB+b*64 .. +64 (one 16-row strip per wave).
 template <typename T>
 __global__ __launch_bounds__(WG) void k_trsm_panel(T* G, int64_t ld, int n, int k0,
-                                                  const T* linv) {
+                                                  const T* linv, const Gate* gate, int want) {
   __shared__ T Li[NB][NB + 1];
+  if (gated_off(gate, want)) return;
   const T* src = linv + (int64_t)(k0 / NB) * 2 * NB * NB + NB * NB;
   for (int idx = threadIdx.x; idx < NB * NB; idx += WG) Li[idx >> 6][idx & (NB - 1)] = src[idx];
   __syncthreads();
@@ -148,9 +166,11 @@ __global__ __launch_bounds__(WG) void k_trsm_panel(T* G, int64_t ld, int n, int 
 // Trailing update G[t0:, t0:] -= L[t0:, k0:t0] L[t0:, k0:t0]', t0 = k0 + NB
 // (VEC as for k_gram_syrk, with ld = n).
 template <typename T, bool VEC>
-__global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int k0, int ntiles) {
+__global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int k0, int ntiles,
+                                                  const Gate* gate, int want) {
   __shared__ T si[kch<T>()][NB + 1];
   __shared__ T sj[kch<T>()][NB + 1];
+  if (gated_off(gate, want)) return;
   int ti, tj;
   if (!xcd_tile(ntiles, &ti, &tj)) return;
   const int t0 = k0 + NB;
@@ -207,9 +227,11 @@ __global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int r
                                                   double* __restrict__ rpart,
                                                   const double* __restrict__ u,
                                                   const double* __restrict__ w,
-                                                  double* __restrict__ cpart) {
+                                                  double* __restrict__ cpart, const Gate* gate,
+                                                  int want) {
   constexpr int RPW = 16;  // rows per wave
   __shared__ double cs[4][TC];
+  if (gated_off(gate, want)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r0 = blockIdx.x * TR + wave * RPW;
   const int j0 = blockIdx.y * TC + 4 * lane;  // this lane's first column
@@ -268,8 +290,10 @@ __global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int r
 // takes 64 columns; its four waves add every fourth partial, then meet in LDS.
 __global__ __launch_bounds__(WG) void k_colfin(const double* __restrict__ cpart, int nb, int ncol,
                                                const double* __restrict__ base,
-                                               double* __restrict__ out) {
+                                               double* __restrict__ out, const Gate* gate = nullptr,
+                                               int want = 0) {
   __shared__ double red[4][64];
+  if (gated_off(gate, want)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + lane;
   double s = 0.0;
@@ -309,8 +333,10 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
                                                    const double* __restrict__ rhs,
                                                    const double* __restrict__ corr,
                                                    double* __restrict__ out,
-                                                   unsigned long long* norm) {
+                                                   unsigned long long* norm,
+                                                   const Gate* gate = nullptr, int want = 0) {
   __shared__ double red[4];
+  if (gated_off(gate, want)) return;
   const int idx = blockIdx.x * WG + threadIdx.x;
   double v = 0.0;
   bool live = idx < n + m;
@@ -343,18 +369,40 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
 // (forward) or I = nt-1-t (backward).  It streams the off-diagonal blocks
 // it needs, waiting per block on the flag of the block that produced the
 // matching piece of the solution, then applies the stored inverse of its
-// diagonal block and publishes its piece.  Flags hold the launch epoch so
-// no reset is needed between solves: sync[0], sync[1] are the forward and
-// backward ticket counters, sync[2..2+nt) and sync[2+nt..2+2nt) the block
-// flags.  A wait is bounded: past the limit the block writes NaN (which the
-// callers' NaN checks report) and still publishes, so every workgroup ends.
+// diagonal block and publishes its piece.
+// The sweep state lives on the device, so a launch needs no host-side
+// sequence number and replays from a graph: per direction a ticket counter,
+// a done counter and a generation g.  A launch publishes its blocks' flags
+// as g + 1; the last workgroup to finish resets both counters and advances g,
+// so the next launch waits for g + 2 and never mistakes an old flag for a new
+// one.  Layout of `sync`: [ticket, done, generation] forward, the same
+// backward, then the nt forward and nt backward flags.  A wait is bounded:
+// past the limit the block writes NaN (which the callers' NaN checks report)
+// and still publishes, so every workgroup ends.
 // ---------------------------------------------------------------------------
 constexpr unsigned kSpinLimit = 1u << 21;
+constexpr int kSyncHdr = 6;
 
-__device__ inline unsigned take_ticket(unsigned* ctr, unsigned base, unsigned* sh) {
-  if (threadIdx.x == 0) *sh = atomicAdd(ctr, 1u) - base;
+// Takes this workgroup's ticket and the launch's flag value (generation + 1).
+__device__ inline unsigned take_ticket(unsigned* st, unsigned* sh, unsigned* epoch) {
+  if (threadIdx.x == 0) {
+    sh[1] = __hip_atomic_load(&st[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    sh[0] = atomicAdd(&st[0], 1u);
+  }
   __syncthreads();
-  return *sh;
+  *epoch = sh[1];
+  return sh[0];
+}
+
+// After the workgroup's publish: the last of the nt workgroups resets the
+// counters and advances the generation for the next launch.
+__device__ inline void end_sweep(unsigned* st, unsigned nt) {
+  if (threadIdx.x == 0 && atomicAdd(&st[1], 1u) == nt - 1) {
+    __hip_atomic_store(&st[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned g = __hip_atomic_load(&st[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st[2], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Lane 0 of the calling wave spins until *flag == want; returns false on
@@ -412,14 +460,16 @@ template <typename T>
 __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_t ld, int n,
                                                  const T* __restrict__ LiB,
                                                  const double* __restrict__ g, T* v,
-                                                 unsigned* sync, unsigned base, unsigned epoch) {
-  __shared__ unsigned sh_ticket;
+                                                 unsigned* sync, const Gate* gate, int want) {
+  __shared__ unsigned sh_ticket[2];
   __shared__ double red[4 * NB];
   __shared__ double w[NB];
   __shared__ T sv[4][NB];
+  if (gated_off(gate, want)) return;
   const int nt = gridDim.x;
-  const int I = (int)take_ticket(&sync[0], base, &sh_ticket);
-  unsigned* flags = sync + 2;
+  unsigned epoch;
+  const int I = (int)take_ticket(sync, sh_ticket, &epoch);
+  unsigned* flags = sync + kSyncHdr;
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int i0 = I * NB, rows = min(NB, n - i0);
   const bool live = r < rows;
@@ -453,7 +503,7 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
   const double vi = block_gemv(inv, w, red);
   if (q == 0 && live) v[i0 + r] = ok ? (T)vi : T(NAN);
   publish(&flags[I], epoch);
-  (void)nt;
+  end_sweep(sync, (unsigned)nt);
 }
 
 // Backward sweep L' x = v.  Thread (rr = lane, q = wave) accumulates, for
@@ -462,14 +512,16 @@ template <typename T>
 __global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_t ld, int n,
                                                  const T* __restrict__ LiB,
                                                  const T* __restrict__ v, double* x,
-                                                 unsigned* sync, unsigned base, unsigned epoch) {
-  __shared__ unsigned sh_ticket;
+                                                 unsigned* sync, const Gate* gate, int want) {
+  __shared__ unsigned sh_ticket[2];
   __shared__ double red[4 * NB];
   __shared__ double part[NB][NB + 1];
   __shared__ double w[NB];
+  if (gated_off(gate, want)) return;
   const int nt = gridDim.x;
-  const int I = nt - 1 - (int)take_ticket(&sync[1], base, &sh_ticket);
-  unsigned* flags = sync + 2 + nt;
+  unsigned epoch;
+  const int I = nt - 1 - (int)take_ticket(sync + 3, sh_ticket, &epoch);
+  unsigned* flags = sync + kSyncHdr + nt;
   const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int i0 = I * NB, rows = min(NB, n - i0);
   bool ok = true;
@@ -517,6 +569,7 @@ __global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_
   const double xi = block_gemv(inv, w, red);
   if (q == 0 && rr < rows) x[i0 + rr] = ok ? xi : (double)NAN;
   publish(&flags[I], epoch);
+  end_sweep(sync + 3, (unsigned)nt);
 }
 
 // y = W (Ax - r_y) with W = -1/d_y, A x from its row partials (rpa, ncb
@@ -525,7 +578,9 @@ __global__ __launch_bounds__(WG) void k_solve_finish(int n, int m, const double*
                                                      const double* __restrict__ rpa, int ncb,
                                                      const double* __restrict__ r,
                                                      const double* __restrict__ diag,
-                                                     double* __restrict__ out, int accumulate) {
+                                                     double* __restrict__ out, int accumulate,
+                                                     const Gate* gate, int want) {
+  if (gated_off(gate, want)) return;
   const int idx = blockIdx.x * WG + threadIdx.x;
   if (idx >= n + m) return;
   double v;
@@ -534,7 +589,9 @@ __global__ __launch_bounds__(WG) void k_solve_finish(int n, int m, const double*
   out[idx] = accumulate ? out[idx] + v : v;
 }
 
-__global__ void k_axpy(int64_t count, double alpha, const double* __restrict__ x, double* __restrict__ y) {
+__global__ void k_axpy(int64_t count, double alpha, const double* __restrict__ x,
+                       double* __restrict__ y, const Gate* gate = nullptr) {
+  if (gated_off(gate, 0)) return;
   const int64_t i = blockIdx.x * (int64_t)WG + threadIdx.x;
   if (i < count) y[i] += alpha * x[i];
 }
@@ -625,8 +682,7 @@ struct qtqp_kkt {
   double *rpa = nullptr, *rpp = nullptr, *aty = nullptr;  // row partials of A, P_off; A'v
   double* tv = nullptr;    // forward-sweep result of the triangular solves
   double* linv = nullptr;  // inverses of the 64x64 diagonal blocks of the factor
-  unsigned* sync = nullptr;  // tickets and block flags of the triangular sweeps
-  unsigned trsv_epoch = 0;   // completed sweep launches
+  unsigned* sync = nullptr;  // counters and block flags of the triangular sweeps
   unsigned long long* dnorm = nullptr;
   // state
   bool have_diag = false, have_factor = false, force64 = false;
@@ -656,40 +712,60 @@ int nblk_rows(int64_t rows) { return (int)((rows + RB - 1) / RB); }
 // (if x) and column partials into cpart (if u).
 template <typename T, bool VEC>
 void launch_tile_pass(hipStream_t st, const T* M, int64_t rows, int64_t ncol, const double* x,
-                      double* rpart, const double* u, const double* w, double* cpart) {
+                      double* rpart, const double* u, const double* w, double* cpart,
+                      const Gate* g, int want) {
   const dim3 grid(tiles_r(rows), tiles_c(ncol));
   if (x && u)
     hipLaunchKernelGGL((k_tile_pass<T, true, true, VEC>), grid, dim3(WG), 0, st, M,
-                       (int)rows, (int)ncol, x, rpart, u, w, cpart);
+                       (int)rows, (int)ncol, x, rpart, u, w, cpart, g, want);
   else if (x)
     hipLaunchKernelGGL((k_tile_pass<T, true, false, VEC>), grid, dim3(WG), 0, st, M,
-                       (int)rows, (int)ncol, x, rpart, u, w, cpart);
+                       (int)rows, (int)ncol, x, rpart, u, w, cpart, g, want);
   else
     hipLaunchKernelGGL((k_tile_pass<T, false, true, VEC>), grid, dim3(WG), 0, st, M,
-                       (int)rows, (int)ncol, x, rpart, u, w, cpart);
+                       (int)rows, (int)ncol, x, rpart, u, w, cpart, g, want);
 }
 
 template <typename T>
 int tile_pass_on(hipStream_t st, double* cpart, const T* M, int64_t rows, int64_t ncol,
-                 const double* x, double* rpart, const double* u, const double* w) {
+                 const double* x, double* rpart, const double* u, const double* w,
+                 const Gate* g = nullptr, int want = 0) {
   if (rows == 0) return QTQP_OK;
   // rows start on 16-byte boundaries when ncol * sizeof(T) is a multiple of 16
   // and M itself is 16-byte aligned
   const bool vec = (ncol * (int64_t)sizeof(T)) % 16 == 0 &&
                    (reinterpret_cast<uintptr_t>(M) & 15) == 0;
-  if (vec) launch_tile_pass<T, true>(st, M, rows, ncol, x, rpart, u, w, cpart);
-  else launch_tile_pass<T, false>(st, M, rows, ncol, x, rpart, u, w, cpart);
+  if (vec) launch_tile_pass<T, true>(st, M, rows, ncol, x, rpart, u, w, cpart, g, want);
+  else launch_tile_pass<T, false>(st, M, rows, ncol, x, rpart, u, w, cpart, g, want);
   HIPC(hipGetLastError());
   return QTQP_OK;
 }
 
 template <typename T>
 int tile_pass(qtqp_kkt* h, const T* M, int64_t rows, int64_t ncol, const double* x,
-              double* rpart, const double* u, const double* w) {
-  return tile_pass_on<T>(h->stream, h->part, M, rows, ncol, x, rpart, u, w);
+              double* rpart, const double* u, const double* w, const Gate* g = nullptr,
+              int want = 0) {
+  return tile_pass_on<T>(h->stream, h->part, M, rows, ncol, x, rpart, u, w, g, want);
 }
 
 inline unsigned colfin_blocks(int64_t ncol) { return (unsigned)((ncol + 63) / 64); }
+
+// Launches out = K_reg x, or out = rhs - K_reg x + corr*x.  With norm (two
+// device words, zero on entry): norm[0] gets ||out||_inf and norm[1] becomes
+// nonzero if x has a NaN.  Nothing is read back.
+int kmatvec_dev(qtqp_kkt* h, const double* x, const double* rhs, const double* corr, double* out,
+                unsigned long long* norm, const Gate* g = nullptr) {
+  const int64_t n = h->n, m = h->m, N = n + m;
+  int rc;
+  if ((rc = tile_pass<double>(h, h->A64, m, n, x, h->rpa, x + n, nullptr, g))) return rc;
+  if ((rc = tile_pass<double>(h, h->Poff64, n, n, x, h->rpp, nullptr, nullptr, g))) return rc;
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
+                     tiles_r(m), (int)n, nullptr, h->aty, g, 0);
+  hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
+                     h->diag, h->rpp, h->rpa, tiles_c(n), h->aty, rhs, corr, out, norm, g, 0);
+  HIPC(hipGetLastError());
+  return QTQP_OK;
+}
 
 // out = K_reg x, or out = rhs - K_reg x + corr*x with ||out||_inf into *norm_host.
 //  With norm_host the device words dnorm = {||out||_inf, NaN-in-x flag,
@@ -697,17 +773,9 @@ inline unsigned colfin_blocks(int64_t ncol) { return (unsigned)((ncol + 63) / 64
 //  the rhs norm into *rhs_norm_host when given.
 int kmatvec(qtqp_kkt* h, const double* x, const double* rhs, const double* corr, double* out,
             double* norm_host, double* rhs_norm_host = nullptr, int* nan_host = nullptr) {
-  const int64_t n = h->n, m = h->m, N = n + m;
-  int rc;
-  if ((rc = tile_pass<double>(h, h->A64, m, n, x, h->rpa, x + n, nullptr))) return rc;
-  if ((rc = tile_pass<double>(h, h->Poff64, n, n, x, h->rpp, nullptr, nullptr))) return rc;
-  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
-                     tiles_r(m), (int)n, nullptr, h->aty);
   if (norm_host) HIPC(hipMemsetAsync(h->dnorm, 0, 2 * sizeof(unsigned long long), h->stream));
-  hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
-                     h->diag, h->rpp, h->rpa, tiles_c(n), h->aty, rhs, corr, out,
-                     norm_host ? h->dnorm : nullptr);
-  HIPC(hipGetLastError());
+  int rc = kmatvec_dev(h, x, rhs, corr, out, norm_host ? h->dnorm : nullptr);
+  if (rc) return rc;
   if (norm_host) {
     unsigned long long bits[3] = {0, 0, 0};
     const int cnt = rhs_norm_host ? 3 : 2;
@@ -720,45 +788,58 @@ int kmatvec(qtqp_kkt* h, const double* x, const double* rhs, const double* corr,
   return QTQP_OK;
 }
 
-// Factor G in precision T: Gram SYRK, jitter, blocked LDL'. Returns the
-// breakdown column (1-based) or 0 in *brk.
+// Launches the factorisation of G in precision T: Gram SYRK, diagonal shift
+// jit_rel * max(diag(G)), blocked LDL'.  A breakdown stores its column
+// (1-based) in *info, which must be zero on entry.  Nothing is read back.
 template <typename T>
-int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk, double jit_rel = 1e-14) {
+int factor_launch(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* info, double jit_rel,
+                  const Gate* g = nullptr, int want = 0) {
   const int n = (int)h->n, m = (int)h->m;
   const int nt = (n + NB - 1) / NB;
-  HIPC(hipMemsetAsync(h->info, 0, sizeof(int), h->stream));
   // 16-byte vector fetches need rows of n elements to stay 16-byte aligned
   const bool vec = (n * (int)sizeof(T)) % 16 == 0;
   const int ng = nt * (nt + 1) / 2;
   if (vec)
     hipLaunchKernelGGL((k_gram_syrk<T, true>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A, Poff,
-                       h->diag, n, m, G, (int64_t)n, ng);
+                       h->diag, n, m, G, (int64_t)n, ng, g, want);
   else
     hipLaunchKernelGGL((k_gram_syrk<T, false>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A,
-                       Poff, h->diag, n, m, G, (int64_t)n, ng);
+                       Poff, h->diag, n, m, G, (int64_t)n, ng, g, want);
   HIPC(hipGetLastError());
-  hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, jit_rel);
+  hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, jit_rel, g,
+                     want);
   HIPC(hipGetLastError());
+  T* linv = reinterpret_cast<T*>(h->linv);
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int below = std::max(0, n - k0 - NB);
     hipLaunchKernelGGL(k_potrf_diag<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, k0,
-                       h->info, reinterpret_cast<T*>(h->linv));
+                       info, linv, g, want);
     HIPC(hipGetLastError());
     if (below > 0) {
       hipLaunchKernelGGL(k_trsm_panel<T>, dim3((below + NB - 1) / NB), dim3(WG), 0, h->stream, G,
-                         (int64_t)n, n, k0, reinterpret_cast<const T*>(h->linv));
+                         (int64_t)n, n, k0, (const T*)linv, g, want);
       HIPC(hipGetLastError());
       const int tt = (below + NB - 1) / NB;
       const int nst = tt * (tt + 1) / 2;
       if (vec)
         hipLaunchKernelGGL((k_syrk_trail<T, true>), dim3(xcd_grid(nst)), dim3(WG), 0, h->stream,
-                           G, (int64_t)n, n, k0, nst);
+                           G, (int64_t)n, n, k0, nst, g, want);
       else
         hipLaunchKernelGGL((k_syrk_trail<T, false>), dim3(xcd_grid(nst)), dim3(WG), 0, h->stream,
-                           G, (int64_t)n, n, k0, nst);
+                           G, (int64_t)n, n, k0, nst, g, want);
       HIPC(hipGetLastError());
     }
   }
+  return QTQP_OK;
+}
+
+// Factor G in precision T and read the breakdown column (1-based, or 0)
+// back into *brk.
+template <typename T>
+int factor_prec(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* brk, double jit_rel = 1e-14) {
+  HIPC(hipMemsetAsync(h->info, 0, sizeof(int), h->stream));
+  int rc = factor_launch<T>(h, A, Poff, G, h->info, jit_rel);
+  if (rc) return rc;
   int hinfo = 0;
   HIPC(hipMemcpyAsync(&hinfo, h->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPC(hipStreamSynchronize(h->stream));
@@ -794,33 +875,31 @@ int factor64(qtqp_kkt* h) {
   return QTQP_OK;
 }
 
-// d = G^{-1}-based K_reg solve of r with the current factor: [x; y] into out.
+// K_reg solve of r with the factor in precision T: [x; y] into out (or added
+// to it): g = r_x + A'(W r_y), two triangular sweeps for x, y = W(A x - r_y).
 template <typename T>
-int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out, int accumulate) {
+int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out, int accumulate,
+               const Gate* g = nullptr, int want = 0) {
   const int64_t n = h->n, m = h->m, N = n + m;
   int rc;
-  // g = r_x + A'(W r_y)
-  if ((rc = tile_pass<T>(h, A, m, n, nullptr, nullptr, r + n, h->diag + n))) return rc;
+  if ((rc = tile_pass<T>(h, A, m, n, nullptr, nullptr, r + n, h->diag + n, g, want))) return rc;
   hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
-                     tiles_r(m), (int)n, r, h->tg);
+                     tiles_r(m), (int)n, r, h->tg, g, want);
   HIPC(hipGetLastError());
   const int nt = (int)((n + NB - 1) / NB);
-  const unsigned base = h->trsv_epoch * (unsigned)nt;
-  const unsigned epoch = h->trsv_epoch + 1u;
   const T* LiB = reinterpret_cast<const T*>(h->linv);
   T* tv = reinterpret_cast<T*>(h->tv);
   hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
-                     h->tg, tv, h->sync, base, epoch);
+                     h->tg, tv, h->sync, g, want);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
-                     tv, h->tx, h->sync, base, epoch);
+                     tv, h->tx, h->sync, g, want);
   HIPC(hipGetLastError());
-  h->trsv_epoch++;
-  if ((rc = tile_pass<T>(h, A, m, n, h->tx, h->rpa, nullptr, nullptr))) return rc;
+  if ((rc = tile_pass<T>(h, A, m, n, h->tx, h->rpa, nullptr, nullptr, g, want))) return rc;
   hipLaunchKernelGGL(k_solve_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m,
-                     h->tx, h->rpa, tiles_c(n), r, h->diag, out, accumulate);
+                     h->tx, h->rpa, tiles_c(n), r, h->diag, out, accumulate, g, want);
   HIPC(hipGetLastError());
-  h->info_acc.backend_solves++;
+  if (!g) h->info_acc.backend_solves++;
   return QTQP_OK;
 }
 
@@ -986,7 +1065,7 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
   ALLOC(tv, double, n);
   const int64_t nt = (n + NB - 1) / NB;
   ALLOC(linv, double, nt * 2 * NB * NB);
-  ALLOC(sync, unsigned, 2 + 2 * nt);
+  ALLOC(sync, unsigned, kSyncHdr + 2 * nt);
   ALLOC(part, double, (int64_t)nblk_rows(std::max(m, n)) * n);
   ALLOC(rpa, double, m * tiles_c(n));
   ALLOC(rpp, double, n * tiles_c(n));
@@ -1031,7 +1110,8 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
     return bad("P");
   }
   if ((e = hipMemsetAsync(h->corr, 0, sizeof(double) * N, h->stream)) != hipSuccess) return bad("corr");
-  if ((e = hipMemsetAsync(h->sync, 0, sizeof(unsigned) * (2 + 2 * nt), h->stream)) != hipSuccess)
+  if ((e = hipMemsetAsync(h->sync, 0, sizeof(unsigned) * (kSyncHdr + 2 * nt), h->stream)) !=
+      hipSuccess)
     return bad("sync");
   if (cast && cast_fp32(h) != QTQP_OK) {
     e = hipGetLastError();
@@ -1096,7 +1176,8 @@ int ir_refine(qtqp_kkt* h, const double* rhs, double* sol, int max_steps, double
   int nanflag = 0;
   int rc = kmatvec(h, sol, h->vin, h->corr, h->vres, &res, &rhs_norm, &nanflag);
   if (rc) return rc;
-  const double tol = atol + rtol * rhs_norm;
+  // one rounding, as the device-side control computes it
+  const double tol = std::fma(rtol, rhs_norm, atol);
   int status = 1, solves = 0;
   for (solves = 1; solves <= max_steps; ++solves) {
     old = res;

@@ -86,6 +86,11 @@ class _HipDenseBase(LinearSolver):
   # QTQP.solve runs the whole iteration loop on the device (DeviceIpm) for
   # backends that set this; the protocol methods below stay available.
   device_driver = True
+  # Each device-loop iteration is one HIP graph launch with the refinement and
+  # precision decisions taken on the device (False: host-driven launches);
+  # device_graph_steps refinement steps per solve are in the graph (0: 3).
+  device_graph = True
+  device_graph_steps = 0
 
   def __init__(self):
     self._lib = _native.kkt_lib()

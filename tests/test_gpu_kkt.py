@@ -345,3 +345,57 @@ def test_device_ipm_argument_errors(require_gpu):
     ipm.iterate(opts)                                            # refinement steps >= 1
   ipm.free()
   ipm.free()  # idempotent
+
+
+def _solve_with_graph(monkeypatch, linear_solver, qp, use_graph, steps=0, **kw):
+  backend = linear_solver.value
+  monkeypatch.setattr(backend, "device_graph", use_graph)
+  monkeypatch.setattr(backend, "device_graph_steps", steps)
+  return qtqp_amd.QTQP(a=sp.csc_matrix(qp[0]), b=qp[1], c=qp[2], z=qp[3],
+                       p=sp.csc_matrix(qp[4])).solve(linear_solver=linear_solver, verbose=False,
+                                                     collect_stats=True, **kw)
+
+
+@pytest.mark.parametrize("steps", [0, 1])
+@pytest.mark.parametrize("kind", ["feasible", "infeasible", "mpc"])
+def test_graph_iteration_matches_host_driven_fp64(require_gpu, monkeypatch, kind, steps):
+  """With the FP64 factor, the device-loop iteration replayed from a HIP graph
+  (refinement and stopping decided on the device) runs the same kernels on the
+  same data as the host-driven launches, so every iterate, statistic and
+  status is bitwise equal.  steps=1 holds a single refinement step per solve
+  in the graph, so most iterations are handed back and redone host-driven
+  from the saved start, which must not change anything either."""
+  if kind == "mpc":
+    q = problems.mpc_qp(16, 6, 2, seed=4)
+    qp = (q.a, q.b, q.c, q.z, q.p)
+  else:
+    gen = G.gen_feasible if kind == "feasible" else G.gen_infeasible
+    a, b, c, p = gen(150, 100, 10, 42)
+    qp = (a, b, c, 10, p)
+  ls = qtqp_amd.LinearSolver.HIP_DENSE_FP64
+  eager = _solve_with_graph(monkeypatch, ls, qp, False)
+  graph = _solve_with_graph(monkeypatch, ls, qp, True, steps)
+  assert graph.status == eager.status
+  assert len(graph.stats) == len(eager.stats)
+  for k in "xys":
+    np.testing.assert_array_equal(getattr(graph, k), getattr(eager, k))
+  for sg, se in zip(graph.stats, eager.stats):
+    for key in ("mu", "sigma", "alpha", "tau", "pres", "dres", "gap"):
+      assert sg[key] == se[key], (sg["iter"], key)
+    for lin in ("q_lin_sys_stats", "predictor_lin_sys_stats", "corrector_lin_sys_stats"):
+      for key in ("solves", "status", "final_residual_norm", "tolerance"):
+        assert sg[lin][key] == se[lin][key], (sg["iter"], lin, key)
+
+
+def test_graph_iteration_mixed_precision_matches_oracle(require_gpu, monkeypatch):
+  """The mixed handle in the graph (FP32 chain and solves gated on the device,
+  FP64 chain opened on a breakdown or when FP32 refinement contracts slowly):
+  oracle status and x/y/s, on a QP where FP32 holds and on one where it falls
+  back, with the handle's counters telling which ran."""
+  for qp in (problems.mpc_qp(32, 8, 3, seed=11), problems.random_dense_qp(200, 300, 50, seed=0)):
+    ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+    sol = _solve_with_graph(monkeypatch, qtqp_amd.LinearSolver.HIP_DENSE,
+                            (qp.a, qp.b, qp.c, qp.z, qp.p), True)
+    assert sol.status.value == ref["status"]
+    for k in "xys":
+      assert _rel(getattr(sol, k), ref[k]) < 1e-6, k
