@@ -47,8 +47,9 @@ typedef struct qtqp_ipm_opts {
   int32_t use_graph;    /* nonzero: replay the iteration from a HIP graph with the
                            refinement and precision decisions made on the device
                            (0: host-driven launches, one readback per refinement step) */
-  int32_t graph_steps;  /* refinement steps per refined solve held in the graph (0: 3);
-                           an iteration that needs more is redone host-driven */
+  int32_t graph_steps;  /* refinement steps per refined solve held in the graph (0:
+                           1 with an FP64 factor, 2 with FP32, one more after each
+                           iteration that needed more, which is redone host-driven) */
 } qtqp_ipm_opts;
 
 /* Status after an iteration (SolutionStatus, __init__.py:129-137). */

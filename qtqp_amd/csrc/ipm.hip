@@ -81,11 +81,57 @@ __device__ inline double fin_min(const double* part, int nb, int k, double* red)
   for (int b = threadIdx.x; b < nb; b += WG) v = fmin(v, part[b * kSlots + k]);
   return -block_max(-v, red);
 }
+// K partial results of this workgroup into slots k0.. of part, combined across
+// the workgroup exactly as put_sum / put_max would, with one pair of barriers;
+// bit k of MAXMASK marks a maximum.  red: >= 4 K doubles of LDS.
+template <int K, unsigned MAXMASK = 0u>
+__device__ inline void put_n(double (&v)[K], double* part, int k0, double* red) {
+  block_reduce_n<K, MAXMASK>(v, red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[blockIdx.x * kSlots + k0 + k] = v[k];
+  }
+}
+// The K slots k0.. over nb workgroups, as fin_sum / fin_max would combine them.
+template <int K, unsigned MAXMASK = 0u>
+__device__ inline void fin_n(const double* part, int nb, int k0, double (&v)[K], double* red) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = ((MAXMASK >> k) & 1u) ? -INFINITY : 0.0;
+  for (int b = threadIdx.x; b < nb; b += WG) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double x = part[b * kSlots + k0 + k];
+      v[k] = ((MAXMASK >> k) & 1u) ? fmax(v[k], x) : v[k] + x;
+    }
+  }
+  block_reduce_n<K, MAXMASK>(v, red);
+}
+
 // |v| for an inf-norm; NaN is reported as inf (fmax drops NaN).
 __device__ inline double absn(double v) { return v != v ? INFINITY : fabs(v); }
 
 #define GRID_STRIDE(k, count) \
   for (int64_t k = blockIdx.x * (int64_t)WG + threadIdx.x; k < (count); k += (int64_t)gridDim.x * WG)
+
+// True in every thread of the last workgroup of the grid to get here, which
+// then sees the partial results every other workgroup wrote before it; that
+// workgroup resets *ctr for the next launch.  Lets one launch both write
+// per-workgroup partials and combine them (the "part" and "fin" halves of a
+// reduction below).
+__device__ inline bool last_block(unsigned* ctr) {
+  __shared__ unsigned sh_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    sh_last = atomicAdd(ctr, 1u) == gridDim.x - 1 ? 1u : 0u;
+    if (sh_last) {
+      *ctr = 0u;
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  return sh_last != 0u;
+}
 
 // ---------------------------------------------------------------------------
 // Equilibration (__init__.py:616-663) and start (__init__.py:299-307)
@@ -205,7 +251,7 @@ __global__ void k_start(int n, int m, int z, const double* __restrict__ b,
 // ---------------------------------------------------------------------------
 
 // ||x||^2, ||y||^2, y's  (__init__.py:859-880, 477).
-__global__ __launch_bounds__(WG) void k_norm_part(int n, int m, const double* __restrict__ xy,
+__device__ inline void norm_part(int n, int m, const double* __restrict__ xy,
                                                   const double* __restrict__ s, double* part) {
   __shared__ double red[4];
   double xx = 0.0, yy = 0.0, ys = 0.0;
@@ -225,7 +271,7 @@ __global__ __launch_bounds__(WG) void k_norm_part(int n, int m, const double* __
 
 // scale = sqrt(m - z + 1) / max(eps, ||(x, y, tau)||), tau *= scale,
 // mu = (scale y)'(scale s) / (m - z).
-__global__ __launch_bounds__(WG) void k_norm_fin(int m, int z, const double* part, int nb, Scal* S) {
+__device__ inline void norm_fin(int m, int z, const double* part, int nb, Scal* S) {
   __shared__ double red[4];
   const double xx = fin_sum(part, nb, 0, red);
   const double yy = fin_sum(part, nb, 1, red);
@@ -299,10 +345,11 @@ __global__ __launch_bounds__(WG) void k_pmul(const double* __restrict__ Poff,
 // Predictor (corrector = 0): mu_t = 0, anchor = [x; y], copied into xp.
 // Corrector: mu_t = sigma mu, anchor = xp = [x_p; y_p], and the Mehrotra term
 // corr = -d_s (y_p - y) / y (__init__.py:528) is formed into cm.
+// kr (optional) gets a copy of the anchor: the refined solve's warm start.
 __global__ void k_newton_rhs(int n, int m, int z, int corrector, const double* __restrict__ xy,
                              const double* __restrict__ s, const double* __restrict__ ds,
                              double* __restrict__ xp, double* __restrict__ cm,
-                             double* __restrict__ r, const Scal* S) {
+                             double* __restrict__ r, const Scal* S, double* __restrict__ kr) {
   const double mu = S->mu;
   const double mu_t = corrector ? S->sigma * mu : 0.0;
   GRID_STRIDE(k, (int64_t)n + m) {
@@ -313,6 +360,7 @@ __global__ void k_newton_rhs(int n, int m, int z, int corrector, const double* _
       a = xy[k];
       xp[k] = a;
     }
+    if (kr) kr[k] = a;
     double v = (mu - mu_t) * a;
     if (k >= n) {
       const int i = (int)(k - n);
@@ -337,14 +385,14 @@ __global__ void k_newton_rhs(int n, int m, int z, int corrector, const double* _
 //   0 kq.q  1 kr.q  2 kq_x.pq  3 kr_x.pq  4 kq_x.pr  5 kr_x.pr
 //   6 q_x.x  7 q_y.y  8 x.px  9 px.kq_x  10 px.kr_x
 // with pq = P kq_x, pr = P kr_x, px = P x (slots 2-5, 8-10 only with P).
-__global__ __launch_bounds__(WG) void k_tau_part(int n, int m, int has_p, const double* __restrict__ q,
+__device__ inline void tau_part(int n, int m, int has_p, const double* __restrict__ q,
                                                  const double* __restrict__ kq,
                                                  const double* __restrict__ kr,
                                                  const double* __restrict__ xy,
                                                  const double* __restrict__ pq,
                                                  const double* __restrict__ pr,
                                                  const double* __restrict__ px, double* part) {
-  __shared__ double red[4];
+  __shared__ double red[4 * 11];
   double v[11];
 #pragma unroll
   for (int t = 0; t < 11; ++t) v[t] = 0.0;
@@ -367,8 +415,7 @@ __global__ __launch_bounds__(WG) void k_tau_part(int n, int m, int has_p, const 
       v[7] += qk * xk;
     }
   }
-#pragma unroll
-  for (int t = 0; t < 11; ++t) put_sum(v[t], part, t, red);
+  put_n<11>(v, part, 0, red);
 }
 
 // tau+ of a Newton step: the stable root of t_a tau^2 + t_b tau + t_c = 0
@@ -377,14 +424,13 @@ __global__ __launch_bounds__(WG) void k_tau_part(int n, int m, int has_p, const 
 // (_solve_for_tau_linearized_fallback, __init__.py:819-857).
 // quad_res: when given, the refined solve's (status, residual) on the device,
 // which decide `quadratic` as the host would (__init__.py:741-752).
-__global__ __launch_bounds__(WG) void k_tau_fin(int corrector, int quadratic, int has_p,
+__device__ inline void tau_fin(int corrector, int quadratic, int has_p,
                                                 const double* part, int nb, Scal* S,
                                                 qtqp_ipm_record* rec, const int* quad_status,
                                                 const double* quad_res) {
-  __shared__ double red[4];
+  __shared__ double red[4 * 11];
   double v[11];
-#pragma unroll
-  for (int t = 0; t < 11; ++t) v[t] = fin_sum(part, nb, t, red);
+  fin_n<11>(part, nb, 0, v, red);
   if (threadIdx.x != 0) return;
   if (quad_status) quadratic = *quad_status == 0 || *quad_res < 1e-7;
   const double mu = S->mu, tau = S->tau;
@@ -451,7 +497,7 @@ __global__ void k_newton_fin(int64_t count, int corrector, const double* __restr
 // (__init__.py:507-510, 546-547, 671-680):
 //   predictor: d_s = -y_p s / y;  corrector: d_s = sigma mu / y + corr - y_c s / y;
 //   slot 0: min -s/d_s over d_s < -eps;  slot 1: min -y/d_y over d_y < -eps.
-__global__ __launch_bounds__(WG) void k_step_part(int n, int m, int z, int corrector,
+__device__ inline void step_part(int n, int m, int z, int corrector,
                                                   const double* __restrict__ xy,
                                                   const double* __restrict__ s,
                                                   const double* __restrict__ xp,
@@ -478,7 +524,7 @@ __global__ __launch_bounds__(WG) void k_step_part(int n, int m, int z, int corre
 
 // alpha = min(max step on s, max step on y) (__init__.py:882-886); for the
 // corrector also step = scale * alpha and tau = max(tau + step (tau_c - tau), 1e-30).
-__global__ __launch_bounds__(WG) void k_alpha_fin(int corrector, double step_scale,
+__device__ inline void alpha_fin(int corrector, double step_scale,
                                                   const double* part, int nb, Scal* S) {
   __shared__ double red[4];
   const double ms = fin_min(part, nb, 0, red), my = fin_min(part, nb, 1, red);
@@ -497,7 +543,7 @@ __global__ __launch_bounds__(WG) void k_alpha_fin(int corrector, double step_sca
 }
 
 // Affine trial point norms for sigma (__init__.py:682-706).
-__global__ __launch_bounds__(WG) void k_sigma_part(int n, int m, const double* __restrict__ xy,
+__device__ inline void sigma_part(int n, int m, const double* __restrict__ xy,
                                                    const double* __restrict__ s,
                                                    const double* __restrict__ xp,
                                                    const double* __restrict__ ds, const Scal* S,
@@ -522,7 +568,7 @@ __global__ __launch_bounds__(WG) void k_sigma_part(int n, int m, const double* _
   put_sum(sys, part, 2, red);
 }
 
-__global__ __launch_bounds__(WG) void k_sigma_fin(int m, int z, const double* part, int nb, Scal* S) {
+__device__ inline void sigma_fin(int m, int z, const double* part, int nb, Scal* S) {
   __shared__ double red[4];
   const double sx = fin_sum(part, nb, 0, red), sy = fin_sum(part, nb, 1, red);
   const double sys = fin_sum(part, nb, 2, red);
@@ -558,13 +604,16 @@ __global__ void k_update(int n, int m, int z, double* __restrict__ xy, double* _
   }
 }
 
+// Slots of the termination reduction that are maxima (the rest are sums).
+constexpr unsigned kTermMax = (((1u << 15) - 1u) & ~0xFu) | (1u << 16) | (1u << 17);
+
 // Termination quantities on the unscaled problem (__init__.py:888-965).
 // ax = A_eq x (m), aty = A_eq' y (n), pxe = P_eq x (n).
 //   sums: 0 x'Px  1 c'x  2 b'y  3 y's     maxima: 4 |Px| 5 |A'y| 6 dres
 //   7 |x| 8 |Ax| 9 |s| 10 pres 11 |Ax+s| 12 |y|
 //   collect_stats, rows z..m of sy = s*y, soy = s/max(eps, y):
 //   13 max sy 14 min sy 15 sum sy 16 max soy 17 min soy 18 sum soy
-__global__ __launch_bounds__(WG) void k_term_part(int n, int m, int z, int has_p, int collect,
+__device__ inline void term_part(int n, int m, int z, int has_p, int collect,
                                                   const double* __restrict__ xy,
                                                   const double* __restrict__ s,
                                                   const double* __restrict__ d,
@@ -575,7 +624,7 @@ __global__ __launch_bounds__(WG) void k_term_part(int n, int m, int z, int has_p
                                                   const double* __restrict__ b,
                                                   const double* __restrict__ c, const Scal* S,
                                                   double* part) {
-  __shared__ double red[4];
+  __shared__ double red[4 * 19];
   const double inv_tau = 1.0 / fmax(S->tau, kEps);
   double xpx = 0.0, ctx = 0.0, bty = 0.0, ys = 0.0;
   double n_px = 0.0, n_aty = 0.0, dres = 0.0, n_x = 0.0;
@@ -614,27 +663,10 @@ __global__ __launch_bounds__(WG) void k_term_part(int n, int m, int z, int has_p
       }
     }
   }
-  put_sum(xpx, part, 0, red);
-  put_sum(ctx, part, 1, red);
-  put_sum(bty, part, 2, red);
-  put_sum(ys, part, 3, red);
-  put_max(n_px, part, 4, red);
-  put_max(n_aty, part, 5, red);
-  put_max(dres, part, 6, red);
-  put_max(n_x, part, 7, red);
-  put_max(n_ax, part, 8, red);
-  put_max(n_s, part, 9, red);
-  put_max(pres, part, 10, red);
-  put_max(n_axs, part, 11, red);
-  put_max(n_y, part, 12, red);
-  if (collect) {
-    put_max(sy_max, part, 13, red);
-    put_min(sy_min, part, 14, red);
-    put_sum(sy_sum, part, 15, red);
-    put_max(so_max, part, 16, red);
-    put_min(so_min, part, 17, red);
-    put_sum(so_sum, part, 18, red);
-  }
+  // minima (slots 14, 17) travel negated as maxima
+  double v[19] = {xpx,   ctx,   bty,  ys,      n_px,    n_aty,   dres,   n_x,     n_ax, n_s,
+                  pres,  n_axs, n_y,  sy_max,  -sy_min, sy_sum,  so_max, -so_min, so_sum};
+  put_n<19, kTermMax>(v, part, 0, red);
 }
 
 struct Tol {
@@ -642,23 +674,14 @@ struct Tol {
 };
 
 // Costs, residuals, certificates and the status (__init__.py:905-1008).
-__global__ __launch_bounds__(WG) void k_term_fin(int m, int z, int has_p, int collect, Tol tl,
+__device__ inline void term_fin(int m, int z, int has_p, int collect, Tol tl,
                                                  const double* part, int nb, Scal* S,
                                                  qtqp_ipm_record* rec) {
-  __shared__ double red[4];
+  __shared__ double red[4 * 19];
   double v[19];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) v[t] = fin_sum(part, nb, t, red);
-#pragma unroll
-  for (int t = 4; t < 13; ++t) v[t] = fin_max(part, nb, t, red);
-  if (collect) {
-    v[13] = fin_max(part, nb, 13, red);
-    v[14] = fin_min(part, nb, 14, red);
-    v[15] = fin_sum(part, nb, 15, red);
-    v[16] = fin_max(part, nb, 16, red);
-    v[17] = fin_min(part, nb, 17, red);
-    v[18] = fin_sum(part, nb, 18, red);
-  }
+  fin_n<19, kTermMax>(part, nb, 0, v, red);
+  v[14] = -v[14];
+  v[17] = -v[17];
   if (threadIdx.x != 0) return;
   const double tau = S->tau;
   const double inv_tau = 1.0 / fmax(tau, kEps);
@@ -717,7 +740,7 @@ __global__ __launch_bounds__(WG) void k_term_fin(int m, int z, int has_p, int co
 }
 
 // Second pass of the standard deviations (np.std: sqrt(mean(|v - mean|^2))).
-__global__ __launch_bounds__(WG) void k_std_part(int n, int m, int z, const double* __restrict__ xy,
+__device__ inline void std_part(int n, int m, int z, const double* __restrict__ xy,
                                                  const double* __restrict__ s,
                                                  const double* __restrict__ d, const Scal* S,
                                                  double* part) {
@@ -736,7 +759,7 @@ __global__ __launch_bounds__(WG) void k_std_part(int n, int m, int z, const doub
   put_sum(b, part, 1, red);
 }
 
-__global__ __launch_bounds__(WG) void k_std_fin(int m, int z, const double* part, int nb,
+__device__ inline void std_fin(int m, int z, const double* part, int nb,
                                                 qtqp_ipm_record* rec) {
   __shared__ double red[4];
   const double a = fin_sum(part, nb, 0, red), b = fin_sum(part, nb, 1, red);
@@ -745,6 +768,74 @@ __global__ __launch_bounds__(WG) void k_std_fin(int m, int z, const double* part
   rec->std_s_over_y = sqrt(b / (m - z));
 }
 
+
+// Each reduction in one launch: the per-workgroup partials, then (in the last
+// workgroup) the combination and the scalar logic, in the same order as two
+// launches would.
+__global__ __launch_bounds__(WG) void k_norm(int n, int m, int z, const double* __restrict__ xy,
+                                             const double* __restrict__ s, double* part, Scal* S,
+                                             unsigned* ctr) {
+  norm_part(n, m, xy, s, part);
+  if (last_block(ctr)) norm_fin(m, z, part, gridDim.x, S);
+}
+
+__global__ __launch_bounds__(WG) void k_tau(int n, int m, int has_p, const double* __restrict__ q,
+                                            const double* __restrict__ kq,
+                                            const double* __restrict__ kr,
+                                            const double* __restrict__ xy,
+                                            const double* __restrict__ pq,
+                                            const double* __restrict__ pr,
+                                            const double* __restrict__ px, double* part,
+                                            int corrector, int quadratic, Scal* S,
+                                            qtqp_ipm_record* rec, const int* quad_status,
+                                            const double* quad_res, unsigned* ctr) {
+  tau_part(n, m, has_p, q, kq, kr, xy, pq, pr, px, part);
+  if (last_block(ctr))
+    tau_fin(corrector, quadratic, has_p, part, gridDim.x, S, rec, quad_status, quad_res);
+}
+
+__global__ __launch_bounds__(WG) void k_step_alpha(int n, int m, int z, int corrector,
+                                                   const double* __restrict__ xy,
+                                                   const double* __restrict__ s,
+                                                   const double* __restrict__ xp,
+                                                   const double* __restrict__ cm,
+                                                   double* __restrict__ ds, Scal* S, double* part,
+                                                   double step_scale, unsigned* ctr) {
+  step_part(n, m, z, corrector, xy, s, xp, cm, ds, S, part);
+  if (last_block(ctr)) alpha_fin(corrector, step_scale, part, gridDim.x, S);
+}
+
+__global__ __launch_bounds__(WG) void k_sigma(int n, int m, int z, const double* __restrict__ xy,
+                                              const double* __restrict__ s,
+                                              const double* __restrict__ xp,
+                                              const double* __restrict__ ds, Scal* S, double* part,
+                                              unsigned* ctr) {
+  sigma_part(n, m, xy, s, xp, ds, S, part);
+  if (last_block(ctr)) sigma_fin(m, z, part, gridDim.x, S);
+}
+
+__global__ __launch_bounds__(WG) void k_term(int n, int m, int z, int has_p, int collect,
+                                             const double* __restrict__ xy,
+                                             const double* __restrict__ s,
+                                             const double* __restrict__ d,
+                                             const double* __restrict__ e,
+                                             const double* __restrict__ ax,
+                                             const double* __restrict__ aty,
+                                             const double* __restrict__ pxe,
+                                             const double* __restrict__ b,
+                                             const double* __restrict__ c, Scal* S, double* part,
+                                             Tol tl, qtqp_ipm_record* rec, unsigned* ctr) {
+  term_part(n, m, z, has_p, collect, xy, s, d, e, ax, aty, pxe, b, c, S, part);
+  if (last_block(ctr)) term_fin(m, z, has_p, collect, tl, part, gridDim.x, S, rec);
+}
+
+__global__ __launch_bounds__(WG) void k_std(int n, int m, int z, const double* __restrict__ xy,
+                                            const double* __restrict__ s,
+                                            const double* __restrict__ d, Scal* S, double* part,
+                                            qtqp_ipm_record* rec, unsigned* ctr) {
+  std_part(n, m, z, xy, s, d, S, part);
+  if (last_block(ctr)) std_fin(m, z, part, gridDim.x, rec);
+}
 
 // ---------------------------------------------------------------------------
 // Device control of an iteration replayed from a graph
@@ -766,9 +857,15 @@ struct IrCtl {
 struct Ctl {
   IrCtl ir[3];  // kinv_q, predictor, corrector
   Gate fgate;   // prec selects the factor chain that runs
+  unsigned fin_ctr, pad2;  // workgroups done in a one-launch reduction (last_block)
   int info32, info64;
   // state across iterations (the host sets it after an iteration it ran itself)
   int force64, fprec, prev_steps, prev_fprec;
+  // precision probe of a mixed handle: 0 before its first FP32 iteration, 1
+  // after it (the next iteration runs FP64); then 2 if the FP64 iteration was
+  // faster (FP64 from then on), 3 if the FP32 one was (FP32 kept)
+  int probe, pad1;
+  unsigned long long t_iter0, t32, t64;
   // this iteration
   int redo;      // 1: refinement needs more steps than unrolled, 2: FP32 contracted
                  // too slowly (FP64 from now on), 3: FP64 factor broke down
@@ -806,19 +903,23 @@ __global__ void k_iter_begin(Ctl* c, int mixed, int64_t N, int64_t m, const doub
   c->info32 = c->info64 = 0;
   c->redo = 0;
   c->fact_col = 0;
-  // an FP32 factor whose three refined solves of the last iteration took more
-  // than kF32StepsPerSolve steps each on average costs more than an FP64 one
+  // A mixed handle times one iteration with the FP32 factor (its first) and
+  // the next with the FP64 factor, and keeps the faster: the FP32 factor
+  // saves MFMA time but its solves need more refinement steps, and which
+  // wins depends on the size and the conditioning.
   int use32 = mixed && !c->force64;
-  if (use32 && c->prev_fprec == 32 && c->prev_steps > 3 * kF32StepsPerSolve) {
+  if (use32 && c->probe == 1) use32 = 0;
+  if (use32 && c->probe == 2) {
+    use32 = 0;  // FP64 was faster (else the probe would not have ended here)
     c->force64 = 1;
     c->fp64_fallbacks++;
-    use32 = 0;
   }
   c->steps = c->f32_steps = 0;
   c->fgate.active = 1;
   c->fgate.prec = use32 ? 32 : 64;
   c->factorizations++;
   c->t0 = wall_clock64();
+  c->t_iter0 = c->t0;
 }
 
 // After the FP32 chain: a breakdown opens the FP64 chain (and FP64 stays).
@@ -846,13 +947,17 @@ __global__ void k_fact_end(Ctl* c) {
 // decreasing, or max_steps solves; an FP32 step contracting by less than
 // kPoorContraction, or a loop that needs more than the `unrolled` steps in
 // the sequence, hands the iteration back to the host.
-__global__ void k_ir_ctl(Ctl* c, int r, int step, int max_steps, int unrolled, double atol,
-                         double rtol) {
+__device__ inline unsigned long long load_bits(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void ir_control(Ctl* c, int r, int step, int max_steps, int unrolled,
+                                  double atol, double rtol) {
   IrCtl& ir = c->ir[r];
-  const double nv = bits_to_double(ir.norm[0]);
-  const int nan = ir.norm[1] != 0ull;
+  const double nv = bits_to_double(load_bits(&ir.norm[0]));
+  const int nan = load_bits(&ir.norm[1]) != 0ull;
   if (step == 0) {
-    ir.rhs_norm = bits_to_double(ir.rhs_bits);
+    ir.rhs_norm = bits_to_double(load_bits(&ir.rhs_bits));
     ir.tol = fma(rtol, ir.rhs_norm, atol);  // one rounding, as the host path computes it
     ir.res = nv;
     ir.nan = nan;
@@ -887,8 +992,36 @@ __global__ void k_ir_ctl(Ctl* c, int r, int step, int max_steps, int unrolled, d
   } else if (step >= max_steps) {
     ir.gate.active = 0;
   } else if (step >= unrolled) {
-    c->redo = 1;
+    // more steps than the sequence holds; with an FP32 factor that is more
+    // than an FP64 factor would need, so FP64 from now on
+    c->redo = ir.gate.prec == 32 ? 2 : 1;
+    if (ir.gate.prec == 32) {
+      c->force64 = 1;
+      c->fp64_fallbacks++;
+    }
     ir.gate.active = 0;
+  }
+}
+
+// The residual of a refinement step (k_kmv_finish) and, in the last
+// workgroup to finish, the loop's control for that step.
+__global__ __launch_bounds__(WG) void k_kmv_finish_ctl(
+    int n, int m, const double* __restrict__ x, const double* __restrict__ diag,
+    const double* __restrict__ rpp, const double* __restrict__ rpa, int ncb,
+    const double* __restrict__ cpart, int nbr, const double* __restrict__ rhs,
+    double* __restrict__ krhs, const double* __restrict__ corr, double* __restrict__ out, Ctl* c,
+    int r, int step, int max_steps, int unrolled, double atol, double rtol) {
+  IrCtl& ir = c->ir[r];
+  if (step > 0 && gated_off(&ir.gate, 0)) return;
+  kmv_finish(n, m, x, diag, rpp, rpa, ncb, cpart, nbr, rhs, krhs, krhs ? &ir.rhs_bits : nullptr,
+             corr, out, ir.norm);
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&c->fin_ctr, 1u) == gridDim.x - 1) {
+      c->fin_ctr = 0u;
+      __threadfence();
+      ir_control(c, r, step, max_steps, unrolled, atol, rtol);
+    }
   }
 }
 
@@ -905,6 +1038,16 @@ __global__ void k_iter_end(Ctl* c, qtqp_ipm_record* rec) {
   }
   c->prev_steps = c->steps;
   c->prev_fprec = c->fprec;
+  const unsigned long long dt = wall_clock64() - c->t_iter0;
+  if (c->probe == 0 && c->fprec == 32) {
+    c->t32 = dt;
+    c->probe = 1;
+  } else if (c->probe == 1 && c->fprec == 64) {
+    c->t64 = dt;
+    c->probe = 2;
+    // FP32 faster: keep it (probe 3 = decided, FP32)
+    if (c->t32 < c->t64) c->probe = 3;
+  }
 }
 
 __global__ void k_copy(int64_t count, const double* __restrict__ src, double* __restrict__ dst) {
@@ -945,7 +1088,9 @@ struct qtqp_ipm {
   struct Graph {
     hipGraphExec_t exec = nullptr;
     qtqp_ipm_opts opts{};
+    int steps = 0;
   } graphs[2];  // [0]: FP64 factor only, [1]: FP32 and FP64 chains
+  int steps_cur = 0;  // refinement steps per solve in the graphs (grows after a redo)
   int wall_khz = 100000;  // device wall clock rate
   int64_t redos = 0;      // iterations handed back to the host-driven launches
 };
@@ -1119,12 +1264,12 @@ int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
   const dim3 gv(nv), gb(WG), g1(1);
   const unsigned gN = blocks_for(N);
   double* part = h->vpart;
+  unsigned* fin = &h->ctl->fin_ctr;
   int rc;
   qtqp_ipm_record out{};
 
   // normalise, mu, regularised diagonal (__init__.py:474-478, direct.py:183-216)
-  hipLaunchKernelGGL(k_norm_part, gv, gb, 0, st, n, m, h->xy, h->s, part);
-  hipLaunchKernelGGL(k_norm_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+  hipLaunchKernelGGL(k_norm, gv, gb, 0, st, n, m, z, h->xy, h->s, part, h->S, fin);
   hipLaunchKernelGGL(k_scale_diag, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->pdiag,
                      o->min_static_regularization, h->S, k->diag, k->corr);
   HIPC(hipGetLastError());
@@ -1143,15 +1288,13 @@ int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
   for (int corrector = 0; corrector < 2; ++corrector) {
     if (corrector) {
       // sigma from the affine trial point (__init__.py:503-520)
-      hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 0, h->xy, h->s, h->xp, h->cm, h->ds,
-                         h->S, part);
-      hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 0, o->step_size_scale, part, nv, h->S);
-      hipLaunchKernelGGL(k_sigma_part, gv, gb, 0, st, n, m, h->xy, h->s, h->xp, h->ds, h->S, part);
-      hipLaunchKernelGGL(k_sigma_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+      hipLaunchKernelGGL(k_step_alpha, gv, gb, 0, st, n, m, z, 0, h->xy, h->s, h->xp, h->cm, h->ds,
+                         h->S, part, o->step_size_scale, fin);
+      hipLaunchKernelGGL(k_sigma, gv, gb, 0, st, n, m, z, h->xy, h->s, h->xp, h->ds, h->S, part, fin);
     }
     // Newton step (__init__.py:708-757)
     hipLaunchKernelGGL(k_newton_rhs, dim3(gN), gb, 0, st, n, m, z, corrector, h->xy, h->s, h->ds,
-                       h->xp, h->cm, h->r, h->S);
+                       h->xp, h->cm, h->r, h->S, nullptr);
     HIPC(hipMemcpyAsync(h->kr, h->xp, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     qtqp_ir_stats& ls = out.lin[1 + corrector];
     if ((rc = ir_refine(k, h->r, h->kr, steps, latol, lrtol, &ls))) return rc;
@@ -1159,17 +1302,14 @@ int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
     if (h->has_p)
       hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->kr,
                          h->pr, nullptr, nullptr);
-    hipLaunchKernelGGL(k_tau_part, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
-                       h->pr, h->px, part);
-    hipLaunchKernelGGL(k_tau_fin, g1, gb, 0, st, corrector, quadratic, h->has_p, part, nv, h->S,
-                       h->drec, nullptr, nullptr);
+    hipLaunchKernelGGL(k_tau, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
+                       h->pr, h->px, part, corrector, quadratic, h->S, h->drec, nullptr, nullptr, fin);
     hipLaunchKernelGGL(k_newton_fin, dim3(gN), gb, 0, st, N, corrector, h->kr, h->kq, h->xp, h->S);
     HIPC(hipGetLastError());
   }
   // step and update (__init__.py:546-565)
-  hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 1, h->xy, h->s, h->xp, h->cm, h->ds, h->S,
-                     part);
-  hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 1, o->step_size_scale, part, nv, h->S);
+  hipLaunchKernelGGL(k_step_alpha, gv, gb, 0, st, n, m, z, 1, h->xy, h->s, h->xp, h->cm, h->ds, h->S,
+                     part, o->step_size_scale, fin);
   hipLaunchKernelGGL(k_update, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->xp, h->ds, h->S);
   HIPC(hipGetLastError());
 
@@ -1182,15 +1322,11 @@ int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
     hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
                        h->pxe, nullptr, nullptr);
   const int collect = o->collect_stats ? 1 : 0;
-  hipLaunchKernelGGL(k_term_part, gv, gb, 0, st, n, m, z, h->has_p, collect, h->xy, h->s, h->d,
-                     h->e, h->ax, h->aty, h->pxe, h->b, h->c, h->S, part);
   const Tol tl{o->atol, o->rtol, o->atol_infeas, o->rtol_infeas};
-  hipLaunchKernelGGL(k_term_fin, g1, gb, 0, st, m, z, h->has_p, collect, tl, part, nv, h->S,
-                     h->drec);
-  if (collect) {
-    hipLaunchKernelGGL(k_std_part, gv, gb, 0, st, n, m, z, h->xy, h->s, h->d, h->S, part);
-    hipLaunchKernelGGL(k_std_fin, g1, gb, 0, st, m, z, part, nv, h->drec);
-  }
+  hipLaunchKernelGGL(k_term, gv, gb, 0, st, n, m, z, h->has_p, collect, h->xy, h->s, h->d, h->e,
+                     h->ax, h->aty, h->pxe, h->b, h->c, h->S, part, tl, h->drec, fin);
+  if (collect)
+    hipLaunchKernelGGL(k_std, gv, gb, 0, st, n, m, z, h->xy, h->s, h->d, h->S, part, h->drec, fin);
   HIPC(hipGetLastError());
   qtqp_ipm_record dev{};
   HIPC(hipMemcpyAsync(&dev, h->drec, sizeof(dev), hipMemcpyDeviceToHost, st));
@@ -1200,7 +1336,6 @@ int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
   return QTQP_OK;
 }
 
-constexpr int kDefaultGraphSteps = 3;
 
 // One refinement loop of the fixed sequence: krhs = [rhs_x; -rhs_y], the
 // residual of the warm start in sol, then `steps` gated steps sol +=
@@ -1210,23 +1345,34 @@ int refine_seq(qtqp_ipm* h, int r, const double* rhs, double* sol, bool both, in
                const qtqp_ipm_opts* o) {
   qtqp_kkt* k = h->k;
   const hipStream_t st = k->stream;
-  const int64_t n = h->n, N = h->n + h->m;
   IrCtl* ic = &h->ctl->ir[r];
   const Gate* g = &ic->gate;
   const int max_steps = o->max_iterative_refinement_steps;
   const double atol = o->linear_solver_atol, rtol = o->linear_solver_rtol;
   int rc;
-  hipLaunchKernelGGL(k_negate_y, dim3(blocks_for(N)), dim3(WG), 0, st, N, n, rhs, k->vin,
-                     &ic->rhs_bits);
-  if ((rc = kmatvec_dev(k, sol, k->vin, k->corr, k->vres, ic->norm))) return rc;
-  hipLaunchKernelGGL(k_ir_ctl, dim3(1), dim3(1), 0, st, h->ctl, r, 0, max_steps, steps, atol, rtol);
+  const int64_t n = h->n, m = h->m, N = n + m;
+  // res = krhs - K_reg sol + corr * sol and the loop's control after it, with
+  // the passes over A and P_off in one launch
+  auto residual = [&](int step, const double* rhs_in, double* krhs_out) {
+    const dim3 grid(tiles_r(m) + tiles_r(n), tiles_c(n));
+    if (n % 2 == 0)
+      hipLaunchKernelGGL(k_tile_pass_kx<true>, grid, dim3(WG), 0, st, k->A64, k->Poff64, (int)m,
+                         (int)n, sol, k->rpa, k->rpp, k->part, step > 0 ? g : nullptr);
+    else
+      hipLaunchKernelGGL(k_tile_pass_kx<false>, grid, dim3(WG), 0, st, k->A64, k->Poff64, (int)m,
+                         (int)n, sol, k->rpa, k->rpp, k->part, step > 0 ? g : nullptr);
+    hipLaunchKernelGGL(k_kmv_finish_ctl, dim3(blocks_for(N)), dim3(WG), 0, st, (int)n, (int)m, sol,
+                       k->diag, k->rpp, k->rpa, tiles_c(n), k->part, tiles_r(m), rhs_in, krhs_out,
+                       k->corr, k->vres, h->ctl, r, step, max_steps, steps, atol, rtol);
+  };
+  // the warm start's residual against krhs = [rhs_x; -rhs_y] (written to vin)
+  residual(0, rhs, k->vin);
   for (int step = 1; step <= std::min(steps, max_steps); ++step) {
-    if (both && (rc = solve_with<float>(k, k->A32, k->G32, k->vres, k->vout, 0, g, 32))) return rc;
-    if ((rc = solve_with<double>(k, k->A64, k->G64, k->vres, k->vout, 0, g, 64))) return rc;
-    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(N)), dim3(WG), 0, st, N, 1.0, k->vout, sol, g);
-    if ((rc = kmatvec_dev(k, sol, k->vin, k->corr, k->vres, ic->norm, g))) return rc;
-    hipLaunchKernelGGL(k_ir_ctl, dim3(1), dim3(1), 0, st, h->ctl, r, step, max_steps, steps, atol,
-                       rtol);
+    // sol += K_reg^{-1} res
+    if (both && (rc = solve_with<float>(k, k->A32, k->G32, k->vres, nullptr, 0, g, 32, sol)))
+      return rc;
+    if ((rc = solve_with<double>(k, k->A64, k->G64, k->vres, nullptr, 0, g, 64, sol))) return rc;
+    residual(step, k->vin, nullptr);
   }
   HIPC(hipGetLastError());
   return QTQP_OK;
@@ -1245,12 +1391,12 @@ int launch_iteration(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, int steps) 
   const unsigned gN = blocks_for(N);
   double* part = h->vpart;
   Ctl* c = h->ctl;
+  unsigned* fin = &c->fin_ctr;
   int rc;
   hipLaunchKernelGGL(k_iter_begin, dim3(gN), gb, 0, st, c, both ? 1 : 0, N, (int64_t)m, h->xy, h->s,
                      h->kq, h->S, h->xy_sv, h->s_sv, h->kq_sv, h->S_sv);
   // normalise, mu, regularised diagonal (__init__.py:474-478, direct.py:183-216)
-  hipLaunchKernelGGL(k_norm_part, gv, gb, 0, st, n, m, h->xy, h->s, part);
-  hipLaunchKernelGGL(k_norm_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+  hipLaunchKernelGGL(k_norm, gv, gb, 0, st, n, m, z, h->xy, h->s, part, h->S, fin);
   hipLaunchKernelGGL(k_scale_diag, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->pdiag,
                      o->min_static_regularization, h->S, k->diag, k->corr);
   HIPC(hipGetLastError());
@@ -1270,30 +1416,24 @@ int launch_iteration(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, int steps) 
                        h->px, h->kq, h->pq);
   for (int corrector = 0; corrector < 2; ++corrector) {
     if (corrector) {
-      hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 0, h->xy, h->s, h->xp, h->cm, h->ds,
-                         h->S, part);
-      hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 0, o->step_size_scale, part, nv, h->S);
-      hipLaunchKernelGGL(k_sigma_part, gv, gb, 0, st, n, m, h->xy, h->s, h->xp, h->ds, h->S, part);
-      hipLaunchKernelGGL(k_sigma_fin, g1, gb, 0, st, m, z, part, nv, h->S);
+      hipLaunchKernelGGL(k_step_alpha, gv, gb, 0, st, n, m, z, 0, h->xy, h->s, h->xp, h->cm, h->ds,
+                         h->S, part, o->step_size_scale, fin);
+      hipLaunchKernelGGL(k_sigma, gv, gb, 0, st, n, m, z, h->xy, h->s, h->xp, h->ds, h->S, part, fin);
     }
     hipLaunchKernelGGL(k_newton_rhs, dim3(gN), gb, 0, st, n, m, z, corrector, h->xy, h->s, h->ds,
-                       h->xp, h->cm, h->r, h->S);
-    hipLaunchKernelGGL(k_copy, dim3(gN), gb, 0, st, N, h->xp, h->kr);
+                       h->xp, h->cm, h->r, h->S, h->kr);
     if ((rc = refine_seq(h, 1 + corrector, h->r, h->kr, both, steps, o))) return rc;
     if (h->has_p)
       hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->kr,
                          h->pr, nullptr, nullptr);
-    hipLaunchKernelGGL(k_tau_part, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
-                       h->pr, h->px, part);
     const IrCtl* ic = &c->ir[1 + corrector];
-    hipLaunchKernelGGL(k_tau_fin, g1, gb, 0, st, corrector, 0, h->has_p, part, nv, h->S, h->drec,
-                       &ic->status, &ic->res);
+    hipLaunchKernelGGL(k_tau, gv, gb, 0, st, n, m, h->has_p, h->q, h->kq, h->kr, h->xy, h->pq,
+                       h->pr, h->px, part, corrector, 0, h->S, h->drec, &ic->status, &ic->res, fin);
     hipLaunchKernelGGL(k_newton_fin, dim3(gN), gb, 0, st, N, corrector, h->kr, h->kq, h->xp, h->S);
     HIPC(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_step_part, gv, gb, 0, st, n, m, z, 1, h->xy, h->s, h->xp, h->cm, h->ds, h->S,
-                     part);
-  hipLaunchKernelGGL(k_alpha_fin, g1, gb, 0, st, 1, o->step_size_scale, part, nv, h->S);
+  hipLaunchKernelGGL(k_step_alpha, gv, gb, 0, st, n, m, z, 1, h->xy, h->s, h->xp, h->cm, h->ds, h->S,
+                     part, o->step_size_scale, fin);
   hipLaunchKernelGGL(k_update, dim3(gN), gb, 0, st, n, m, z, h->xy, h->s, h->xp, h->ds, h->S);
   HIPC(hipGetLastError());
   // termination (__init__.py:888-1008)
@@ -1305,15 +1445,11 @@ int launch_iteration(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, int steps) 
     hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
                        h->pxe, nullptr, nullptr);
   const int collect = o->collect_stats ? 1 : 0;
-  hipLaunchKernelGGL(k_term_part, gv, gb, 0, st, n, m, z, h->has_p, collect, h->xy, h->s, h->d,
-                     h->e, h->ax, h->aty, h->pxe, h->b, h->c, h->S, part);
   const Tol tl{o->atol, o->rtol, o->atol_infeas, o->rtol_infeas};
-  hipLaunchKernelGGL(k_term_fin, g1, gb, 0, st, m, z, h->has_p, collect, tl, part, nv, h->S,
-                     h->drec);
-  if (collect) {
-    hipLaunchKernelGGL(k_std_part, gv, gb, 0, st, n, m, z, h->xy, h->s, h->d, h->S, part);
-    hipLaunchKernelGGL(k_std_fin, g1, gb, 0, st, m, z, part, nv, h->drec);
-  }
+  hipLaunchKernelGGL(k_term, gv, gb, 0, st, n, m, z, h->has_p, collect, h->xy, h->s, h->d, h->e,
+                     h->ax, h->aty, h->pxe, h->b, h->c, h->S, part, tl, h->drec, fin);
+  if (collect)
+    hipLaunchKernelGGL(k_std, gv, gb, 0, st, n, m, z, h->xy, h->s, h->d, h->S, part, h->drec, fin);
   hipLaunchKernelGGL(k_iter_end, g1, g1, 0, st, c, h->drec);
   HIPC(hipGetLastError());
   return QTQP_OK;
@@ -1333,7 +1469,14 @@ bool same_graph_opts(const qtqp_ipm_opts& a, const qtqp_ipm_opts& b) {
 // The graph of one iteration for these options (captured on first use).
 int iteration_graph(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, hipGraphExec_t* out) {
   auto& slot = h->graphs[both ? 1 : 0];
-  if (slot.exec && same_graph_opts(slot.opts, *o)) {
+  // An FP32 factor needs two steps where an FP64 one needs one.  A step held
+  // in the graph but not needed costs a few empty launches; a refinement that
+  // needs more steps than the graph holds costs the whole iteration again, so
+  // an expensive iteration (factor work above 2e9 flops) starts with two.
+  const double nn = (double)h->n, work = (double)h->m * nn * nn + nn * nn * nn / 3.0;
+  const int base_steps = (both || work > 2e9) ? 2 : 1;
+  const int steps = o->graph_steps > 0 ? o->graph_steps : std::max(base_steps, h->steps_cur);
+  if (slot.exec && same_graph_opts(slot.opts, *o) && slot.steps == steps) {
     *out = slot.exec;
     return QTQP_OK;
   }
@@ -1342,7 +1485,6 @@ int iteration_graph(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, hipGraphExec
     slot.exec = nullptr;
   }
   const hipStream_t st = h->k->stream;
-  const int steps = o->graph_steps > 0 ? o->graph_steps : kDefaultGraphSteps;
   HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   const int rc = launch_iteration(h, o, both, steps);
   hipGraph_t graph = nullptr;
@@ -1359,6 +1501,7 @@ int iteration_graph(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, hipGraphExec
     return fail(QTQP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
   slot.exec = exec;
   slot.opts = *o;
+  slot.steps = steps;
   *out = exec;
   return QTQP_OK;
 }
@@ -1386,7 +1529,13 @@ int iterate_graph(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
   if (c.redo) {
     // Back to the iteration's start, then the host-driven launches, which
     // loop as long as the reference does (and retry a broken-down factor).
+    // A refinement that needed more steps than the graph holds makes the next
+    // graphs hold one more (up to 8).
     h->redos++;
+    if (c.redo == 1 && o->graph_steps == 0) {
+      const int used = h->graphs[both ? 1 : 0].steps;
+      h->steps_cur = std::min(std::min(used + 1, 8), o->max_iterative_refinement_steps);
+    }
     const int64_t N = h->n + h->m;
     hipLaunchKernelGGL(k_iter_restore, dim3(blocks_for(N)), dim3(WG), 0, st, N, h->m, h->xy, h->s,
                        h->kq, h->S, h->xy_sv, h->s_sv, h->kq_sv, h->S_sv);

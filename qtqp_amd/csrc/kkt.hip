@@ -143,15 +143,12 @@ int qtqp_kkt_matvec_noalloc(int64_t n, int64_t m, const double* a, const double*
   double* rpa = sc;                             // m * tiles_c(n)
   double* rpp = rpa + m * tiles_c(n);           // n * tiles_c(n)
   double* cpart = rpp + n * tiles_c(n);         // tiles_r(m) * n
-  double* aty = cpart + (int64_t)tiles_r(m) * n;  // n
   if ((rc = tile_pass_on<double>(st, cpart, a, m, n, x, rpa, x + n, nullptr))) return rc;
   if (poff && (rc = tile_pass_on<double>(st, cpart, poff, n, n, x, rpp, nullptr, nullptr)))
     return rc;
-  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, st, cpart, tiles_r(m), (int)n,
-                     nullptr, aty);
   hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(n + m)), dim3(WG), 0, st, (int)n, (int)m, x,
-                     diag, poff ? rpp : nullptr, rpa, tiles_c(n), aty, nullptr, nullptr, out,
-                     nullptr);
+                     diag, poff ? rpp : nullptr, rpa, tiles_c(n), cpart, tiles_r(m), nullptr,
+                     nullptr, nullptr, nullptr, out, nullptr, nullptr, 0);
   HIPC(hipGetLastError());
   return QTQP_OK;
 }

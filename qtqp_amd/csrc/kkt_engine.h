@@ -113,6 +113,60 @@ __global__ __launch_bounds__(WG) void k_gram_syrk(const T* __restrict__ A,
   });
 }
 
+// The Gram product split over the rows of A when G has few tiles (a small n
+// with many constraints): workgroup (t, c) forms tile t's sum over rows
+// [c * chunk, (c + 1) * chunk) into Gp, and k_gram_reduce adds the chunks in
+// order, P_off and diag(d_x).  Enough workgroups then stream A at once.
+template <typename T, bool VEC>
+__global__ __launch_bounds__(WG) void k_gram_part(const T* __restrict__ A,
+                                                  const double* __restrict__ diag, int n, int m,
+                                                  int nchunk, int chunk, T* __restrict__ Gp,
+                                                  const Gate* gate, int want) {
+  __shared__ T si[kch<T>()][NB + 1];
+  __shared__ T sj[kch<T>()][NB + 1];
+  if (gated_off(gate, want)) return;
+  const int t = blockIdx.x / nchunk, c = blockIdx.x % nchunk;
+  int ti, tj;
+  lower_tile(t, &ti, &tj);
+  const int i0 = ti * NB, j0 = tj * NB;
+  const int k0 = c * chunk, k1 = min(m, k0 + chunk);
+  RankKTile<T> acc;
+  acc.zero();
+  const double* dy = diag + n;
+  if constexpr (VEC) {
+    acc.run_vec(A, n, k0, k1, i0, j0, dy, [](double d) { return (T)(-1.0 / d); }, &si[0][0],
+                &sj[0][0]);
+  } else {
+    acc.run(A, n, k0, k1, i0, j0, n, n, [dy](int k) { return (T)(-1.0 / dy[k]); }, si, sj);
+  }
+  T* dst = Gp + ((int64_t)t * nchunk + c) * NB * NB;
+  acc.for_each(i0, j0, [&](int i, int j, T v) { dst[(j - j0) * NB + (i - i0)] = v; });
+}
+
+//  Workgroup b takes 256 entries of tile b / 16, one per thread.
+template <typename T>
+__global__ __launch_bounds__(WG) void k_gram_reduce(const T* __restrict__ Gp,
+                                                    const T* __restrict__ Poff,
+                                                    const double* __restrict__ diag, int n,
+                                                    int nchunk, T* __restrict__ G, int64_t ld,
+                                                    const Gate* gate, int want) {
+  if (gated_off(gate, want)) return;
+  constexpr int kParts = NB * NB / WG;
+  const int t = blockIdx.x / kParts;
+  int ti, tj;
+  lower_tile(t, &ti, &tj);
+  const int i0 = ti * NB, j0 = tj * NB;
+  const int e = (blockIdx.x % kParts) * WG + threadIdx.x;
+  const int i = i0 + (e & (NB - 1)), j = j0 + (e >> 6);
+  if (i >= n || j >= n || i < j) return;
+  const T* src = Gp + (int64_t)t * nchunk * NB * NB + e;
+  T v = T(0);
+  for (int c = 0; c < nchunk; ++c) v += src[(int64_t)c * NB * NB];
+  T g = v + Poff[(int64_t)j * n + i];
+  if (i == j) g += (T)diag[i];
+  G[(int64_t)j * ld + i] = g;
+}
+
 // diag(G) += rel * max(diag(G)); rel = 1e-14 as solvers_dense.py:107-112 (and
 // larger on a retry after a breakdown).  One workgroup.
 template <typename T>
@@ -203,6 +257,7 @@ constexpr int TR = 64;
 constexpr int TC = 256;
 inline int tiles_r(int64_t rows) { return (int)((rows + TR - 1) / TR); }
 inline int tiles_c(int64_t ncol) { return (int)((ncol + TC - 1) / TC); }
+__device__ inline int tiles_r_dev(int rows) { return (rows + TR - 1) / TR; }
 
 template <typename T> struct Vec4;
 template <> struct Vec4<float> {
@@ -221,20 +276,16 @@ template <> struct Vec4<double> {
   }
 };
 
+// Tile (bx, by) of the pass; cs: 4 x TC doubles of LDS.
 template <typename T, bool HX, bool HU, bool VEC>
-__global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int rows, int ncol,
-                                                  const double* __restrict__ x,
-                                                  double* __restrict__ rpart,
-                                                  const double* __restrict__ u,
-                                                  const double* __restrict__ w,
-                                                  double* __restrict__ cpart, const Gate* gate,
-                                                  int want) {
+__device__ inline void tile_body(const T* __restrict__ M, int rows, int ncol,
+                                 const double* __restrict__ x, double* __restrict__ rpart,
+                                 const double* __restrict__ u, const double* __restrict__ w,
+                                 double* __restrict__ cpart, int bx, int by, double (*cs)[TC]) {
   constexpr int RPW = 16;  // rows per wave
-  __shared__ double cs[4][TC];
-  if (gated_off(gate, want)) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r0 = blockIdx.x * TR + wave * RPW;
-  const int j0 = blockIdx.y * TC + 4 * lane;  // this lane's first column
+  const int r0 = bx * TR + wave * RPW;
+  const int j0 = by * TC + 4 * lane;  // this lane's first column
   const bool full = j0 + 3 < ncol;
   double a[RPW][4];
 #pragma unroll
@@ -261,7 +312,7 @@ __global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int r
     }
     reduce_rows<RPW>(rd);  // lane l: row l >> 2
     const int q = lane >> 2;
-    if ((lane & 3) == 0 && r0 + q < rows) rpart[(int64_t)blockIdx.y * rows + r0 + q] = rd[0];
+    if ((lane & 3) == 0 && r0 + q < rows) rpart[(int64_t)by * rows + r0 + q] = rd[0];
   }
   if (HU) {
     double cacc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -279,11 +330,45 @@ __global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int r
 #pragma unroll
     for (int c = 0; c < 4; ++c) cs[wave][4 * lane + c] = cacc[c];
     __syncthreads();
-    const int j = blockIdx.y * TC + threadIdx.x;
+    const int j = by * TC + threadIdx.x;
     if (j < ncol)
-      cpart[(int64_t)blockIdx.x * ncol + j] =
+      cpart[(int64_t)bx * ncol + j] =
           (cs[0][threadIdx.x] + cs[1][threadIdx.x]) + (cs[2][threadIdx.x] + cs[3][threadIdx.x]);
   }
+}
+
+template <typename T, bool HX, bool HU, bool VEC>
+__global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int rows, int ncol,
+                                                  const double* __restrict__ x,
+                                                  double* __restrict__ rpart,
+                                                  const double* __restrict__ u,
+                                                  const double* __restrict__ w,
+                                                  double* __restrict__ cpart, const Gate* gate,
+                                                  int want) {
+  __shared__ double cs[4][TC];
+  if (gated_off(gate, want)) return;
+  tile_body<T, HX, HU, VEC>(M, rows, ncol, x, rpart, u, w, cpart, blockIdx.x, blockIdx.y, cs);
+}
+
+// The passes of a K product in one launch: A x_x and A' x_y over the first
+// tiles_r(m) row tiles, P_off x_x over the next tiles_r(n) (P == NULL: none).
+template <bool VEC>
+__global__ __launch_bounds__(WG) void k_tile_pass_kx(const double* __restrict__ A,
+                                                     const double* __restrict__ P, int m, int n,
+                                                     const double* __restrict__ x,
+                                                     double* __restrict__ rpa,
+                                                     double* __restrict__ rpp,
+                                                     double* __restrict__ cpart,
+                                                     const Gate* gate) {
+  __shared__ double cs[4][TC];
+  if (gated_off(gate, 0)) return;
+  const int ta = tiles_r_dev(m);
+  if ((int)blockIdx.x < ta)
+    tile_body<double, true, true, VEC>(A, m, n, x, rpa, x + n, nullptr, cpart, blockIdx.x,
+                                       blockIdx.y, cs);
+  else
+    tile_body<double, true, false, VEC>(P, n, n, x, rpp, nullptr, nullptr, nullptr,
+                                        blockIdx.x - ta, blockIdx.y, cs);
 }
 
 // out[j] = (base ? base[j] : 0) + sum_{b < nb} cpart[b * ncol + j].  A workgroup
@@ -320,37 +405,63 @@ __global__ void k_rowfin(const double* __restrict__ rpart, int nb, int rows,
   if (r < rows) out[r] = row_total(rpart, nb, rows, r);
 }
 
+// A'x_y at column j from the column partials of nbr row tiles (the order of
+// k_colfin: four interleaved partial sums, then pairs).
+__device__ inline double col_total(const double* __restrict__ cpart, int nbr, int ncol, int j) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int b = 0;
+  for (; b + 8 <= nbr; b += 8) {  // eight loads in flight at a time
+    double v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = cpart[(int64_t)(b + t) * ncol + j];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) s[t & 3] += v[t];
+  }
+  for (; b < nbr; ++b) s[b & 3] += cpart[(int64_t)b * ncol + j];
+  return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 // K_reg x (or a residual):  kx = [P_off x_x + d_x*x_x + A'x_y ; A x_x + d_y*x_y]
 // from the row partials of P_off x_x (rpp) and A x_x (rpa, ncb tiles each)
-// and aty = A'x_y (rpp NULL: P_off = 0).  If rhs: out = rhs - kx (+ corr*x if
-// corr).  With norm:
-// norm[0] gets max |out| and norm[1] becomes nonzero if x has a NaN.
-__global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* __restrict__ x,
-                                                   const double* __restrict__ diag,
-                                                   const double* __restrict__ rpp,
-                                                   const double* __restrict__ rpa, int ncb,
-                                                   const double* __restrict__ aty,
-                                                   const double* __restrict__ rhs,
-                                                   const double* __restrict__ corr,
-                                                   double* __restrict__ out,
-                                                   unsigned long long* norm,
-                                                   const Gate* gate = nullptr, int want = 0) {
+// and the column partials of A'x_y (cpart, nbr row tiles; rpp NULL: P_off =
+// 0).  If rhs: out = rhs - kx (+ corr*x if corr); with krhs the rhs is taken
+// as [rhs_x; -rhs_y] (the quasi-definite form, direct.py:245-248), written to
+// krhs, and its inf-norm goes into *rhs_norm.  With norm: norm[0] gets
+// max |out| and norm[1] becomes nonzero if x has a NaN.
+__device__ inline void kmv_finish(int n, int m, const double* __restrict__ x,
+                                  const double* __restrict__ diag, const double* __restrict__ rpp,
+                                  const double* __restrict__ rpa, int ncb,
+                                  const double* __restrict__ cpart, int nbr,
+                                  const double* __restrict__ rhs, double* __restrict__ krhs,
+                                  unsigned long long* rhs_norm, const double* __restrict__ corr,
+                                  double* __restrict__ out, unsigned long long* norm) {
   __shared__ double red[4];
-  if (gated_off(gate, want)) return;
   const int idx = blockIdx.x * WG + threadIdx.x;
-  double v = 0.0;
+  double v = 0.0, rv = 0.0;
   bool live = idx < n + m;
   if (live) {
     double kx;
-    if (idx < n) kx = (rpp ? row_total(rpp, ncb, n, idx) : 0.0) + diag[idx] * x[idx] + aty[idx];
-    else kx = row_total(rpa, ncb, m, idx - n) + diag[idx] * x[idx];
+    if (idx < n)
+      kx = (rpp ? row_total(rpp, ncb, n, idx) : 0.0) + diag[idx] * x[idx] +
+           col_total(cpart, nbr, n, idx);
+    else
+      kx = row_total(rpa, ncb, m, idx - n) + diag[idx] * x[idx];
     if (rhs) {
-      v = rhs[idx] - kx;
+      rv = rhs[idx];
+      if (krhs) {
+        if (idx >= n) rv = -rv;
+        krhs[idx] = rv;
+      }
+      v = rv - kx;
       if (corr) v += corr[idx] * x[idx];
     } else {
       v = kx;
     }
     out[idx] = v;
+  }
+  if (rhs_norm) {
+    const double a = block_max(live ? (rv != rv ? INFINITY : fabs(rv)) : 0.0, red);
+    if (threadIdx.x == 0) atomic_max_nonneg(rhs_norm, a);
   }
   if (norm) {
     double a = live ? fabs(v) : 0.0;
@@ -359,6 +470,22 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
     a = block_max(a, red);
     if (threadIdx.x == 0) atomic_max_nonneg(norm, a);
   }
+}
+
+__global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* __restrict__ x,
+                                                   const double* __restrict__ diag,
+                                                   const double* __restrict__ rpp,
+                                                   const double* __restrict__ rpa, int ncb,
+                                                   const double* __restrict__ cpart, int nbr,
+                                                   const double* __restrict__ rhs,
+                                                   double* __restrict__ krhs,
+                                                   unsigned long long* rhs_norm,
+                                                   const double* __restrict__ corr,
+                                                   double* __restrict__ out,
+                                                   unsigned long long* norm,
+                                                   const Gate* gate = nullptr, int want = 0) {
+  if (gated_off(gate, want)) return;
+  kmv_finish(n, m, x, diag, rpp, rpa, ncb, cpart, nbr, rhs, krhs, rhs_norm, corr, out, norm);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,12 +581,15 @@ __device__ inline double block_gemv(const T* mv, const double* w, double* red) {
   return v;
 }
 
-// Forward sweep L v = g.  Thread (r = lane, q = wave) accumulates
+// Forward sweep L v = g with g = gbase + A'u from the column partials of nbr
+// row tiles (the right-hand side g = r_x + A'(W r_y) of a K solve, summed as
+// k_colfin sums).  Thread (r = lane, q = wave) accumulates
 // sum_{J<I} sum_{c in q's 16 columns} L[I*64+r][J*64+c] v[J*64+c].
 template <typename T>
 __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_t ld, int n,
                                                  const T* __restrict__ LiB,
-                                                 const double* __restrict__ g, T* v,
+                                                 const double* __restrict__ gbase,
+                                                 const double* __restrict__ cpart, int nbr, T* v,
                                                  unsigned* sync, const Gate* gate, int want) {
   __shared__ unsigned sh_ticket[2];
   __shared__ double red[4 * NB];
@@ -476,8 +606,22 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
   bool ok = true;
   double acc = 0.0;
   T cur[16], nxt[16], inv[16];
-  // the block inverse is needed only after the last wait: fetch it now
+  // the block inverse and this block's g are needed only after the last
+  // wait: fetch them now
   load_block_part(LiB + (int64_t)I * 2 * NB * NB, inv);
+  // g for this block's rows: wave q sums the partials b = q, q + 4, ... (the
+  // order of k_colfin), wave 0 combines them
+  {
+    double sq = 0.0;
+    if (live)
+      for (int b = q; b < nbr; b += 4) sq += cpart[(int64_t)b * n + i0 + r];
+    red[q * NB + r] = sq;
+  }
+  __syncthreads();
+  const double gv = (q == 0 && live)
+                        ? gbase[i0 + r] + ((red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]))
+                        : 0.0;
+  __syncthreads();
   auto load = [&](int J, T* dst) {
     const T* col = L + (int64_t)(J * NB + q * 16) * ld + i0 + r;
 #pragma unroll
@@ -497,7 +641,7 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
   __syncthreads();
   if (q == 0) {
     const double s = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
-    w[r] = live ? g[i0 + r] - s : 0.0;
+    w[r] = live ? gv - s : 0.0;
   }
   __syncthreads();
   const double vi = block_gemv(inv, w, red);
@@ -587,6 +731,22 @@ __global__ __launch_bounds__(WG) void k_solve_finish(int n, int m, const double*
   if (idx < n) v = x[idx];
   else v = (-1.0 / diag[idx]) * (row_total(rpa, ncb, m, idx - n) - r[idx]);
   out[idx] = accumulate ? out[idx] + v : v;
+}
+
+// sol += [x; W (A x - r_y)] (the solve's finish added to the iterate).
+__global__ __launch_bounds__(WG) void k_solve_add(int n, int m, const double* __restrict__ x,
+                                                  const double* __restrict__ rpa, int ncb,
+                                                  const double* __restrict__ r,
+                                                  const double* __restrict__ diag,
+                                                  double* __restrict__ sol, const Gate* gate,
+                                                  int want) {
+  if (gated_off(gate, want)) return;
+  const int idx = blockIdx.x * WG + threadIdx.x;
+  if (idx >= n + m) return;
+  double v;
+  if (idx < n) v = x[idx];
+  else v = (-1.0 / diag[idx]) * (row_total(rpa, ncb, m, idx - n) - r[idx]);
+  sol[idx] += v;
 }
 
 __global__ void k_axpy(int64_t count, double alpha, const double* __restrict__ x,
@@ -682,6 +842,9 @@ struct qtqp_kkt {
   double *rpa = nullptr, *rpp = nullptr, *aty = nullptr;  // row partials of A, P_off; A'v
   double* tv = nullptr;    // forward-sweep result of the triangular solves
   double* linv = nullptr;  // inverses of the 64x64 diagonal blocks of the factor
+  // row split of the Gram product (gsplit chunks of gchunk rows; 1: none)
+  int gsplit = 1, gchunk = 0;
+  double* gpart = nullptr;
   unsigned* sync = nullptr;  // counters and block flags of the triangular sweeps
   unsigned long long* dnorm = nullptr;
   // state
@@ -753,16 +916,24 @@ inline unsigned colfin_blocks(int64_t ncol) { return (unsigned)((ncol + 63) / 64
 // Launches out = K_reg x, or out = rhs - K_reg x + corr*x.  With norm (two
 // device words, zero on entry): norm[0] gets ||out||_inf and norm[1] becomes
 // nonzero if x has a NaN.  Nothing is read back.
+//  With krhs, rhs is taken as [rhs_x; -rhs_y], written to krhs, and its
+//  inf-norm goes into *rhs_norm (zero on entry).
 int kmatvec_dev(qtqp_kkt* h, const double* x, const double* rhs, const double* corr, double* out,
-                unsigned long long* norm, const Gate* g = nullptr) {
+                unsigned long long* norm, const Gate* g = nullptr, double* krhs = nullptr,
+                unsigned long long* rhs_norm = nullptr) {
   const int64_t n = h->n, m = h->m, N = n + m;
-  int rc;
-  if ((rc = tile_pass<double>(h, h->A64, m, n, x, h->rpa, x + n, nullptr, g))) return rc;
-  if ((rc = tile_pass<double>(h, h->Poff64, n, n, x, h->rpp, nullptr, nullptr, g))) return rc;
-  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
-                     tiles_r(m), (int)n, nullptr, h->aty, g, 0);
+  // A x_x, A' x_y and P_off x_x in one launch: rows of A and P_off start on
+  // 16-byte boundaries when n is even (allocations are 256-byte aligned)
+  const dim3 grid(tiles_r(m) + tiles_r(n), tiles_c(n));
+  if (n % 2 == 0)
+    hipLaunchKernelGGL(k_tile_pass_kx<true>, grid, dim3(WG), 0, h->stream, h->A64, h->Poff64,
+                       (int)m, (int)n, x, h->rpa, h->rpp, h->part, g);
+  else
+    hipLaunchKernelGGL(k_tile_pass_kx<false>, grid, dim3(WG), 0, h->stream, h->A64, h->Poff64,
+                       (int)m, (int)n, x, h->rpa, h->rpp, h->part, g);
   hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
-                     h->diag, h->rpp, h->rpa, tiles_c(n), h->aty, rhs, corr, out, norm, g, 0);
+                     h->diag, h->rpp, h->rpa, tiles_c(n), h->part, tiles_r(m), rhs, krhs, rhs_norm,
+                     corr, out, norm, g, 0);
   HIPC(hipGetLastError());
   return QTQP_OK;
 }
@@ -799,12 +970,25 @@ int factor_launch(qtqp_kkt* h, const T* A, const T* Poff, T* G, int* info, doubl
   // 16-byte vector fetches need rows of n elements to stay 16-byte aligned
   const bool vec = (n * (int)sizeof(T)) % 16 == 0;
   const int ng = nt * (nt + 1) / 2;
-  if (vec)
+  if (h->gsplit > 1) {
+    T* gp = reinterpret_cast<T*>(h->gpart);
+    const unsigned parts = (unsigned)(ng * h->gsplit);
+    if (vec)
+      hipLaunchKernelGGL((k_gram_part<T, true>), dim3(parts), dim3(WG), 0, h->stream, A, h->diag, n,
+                         m, h->gsplit, h->gchunk, gp, g, want);
+    else
+      hipLaunchKernelGGL((k_gram_part<T, false>), dim3(parts), dim3(WG), 0, h->stream, A, h->diag,
+                         n, m, h->gsplit, h->gchunk, gp, g, want);
+    hipLaunchKernelGGL(k_gram_reduce<T>, dim3(ng * (NB * NB / WG)), dim3(WG), 0, h->stream,
+                       (const T*)gp, Poff,
+                       h->diag, n, h->gsplit, G, (int64_t)n, g, want);
+  } else if (vec) {
     hipLaunchKernelGGL((k_gram_syrk<T, true>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A, Poff,
                        h->diag, n, m, G, (int64_t)n, ng, g, want);
-  else
+  } else {
     hipLaunchKernelGGL((k_gram_syrk<T, false>), dim3(xcd_grid(ng)), dim3(WG), 0, h->stream, A,
                        Poff, h->diag, n, m, G, (int64_t)n, ng, g, want);
+  }
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_jitter<T>, dim3(1), dim3(WG), 0, h->stream, G, (int64_t)n, n, jit_rel, g,
                      want);
@@ -877,27 +1061,29 @@ int factor64(qtqp_kkt* h) {
 
 // K_reg solve of r with the factor in precision T: [x; y] into out (or added
 // to it): g = r_x + A'(W r_y), two triangular sweeps for x, y = W(A x - r_y).
+//  With add_to, [x; y] is added to add_to instead (out is not used).
 template <typename T>
 int solve_with(qtqp_kkt* h, const T* A, const T* L, const double* r, double* out, int accumulate,
-               const Gate* g = nullptr, int want = 0) {
+               const Gate* g = nullptr, int want = 0, double* add_to = nullptr) {
   const int64_t n = h->n, m = h->m, N = n + m;
   int rc;
   if ((rc = tile_pass<T>(h, A, m, n, nullptr, nullptr, r + n, h->diag + n, g, want))) return rc;
-  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), dim3(WG), 0, h->stream, h->part,
-                     tiles_r(m), (int)n, r, h->tg, g, want);
-  HIPC(hipGetLastError());
   const int nt = (int)((n + NB - 1) / NB);
   const T* LiB = reinterpret_cast<const T*>(h->linv);
   T* tv = reinterpret_cast<T*>(h->tv);
   hipLaunchKernelGGL(k_trsv_fwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
-                     h->tg, tv, h->sync, g, want);
+                     r, h->part, tiles_r(m), tv, h->sync, g, want);
   HIPC(hipGetLastError());
   hipLaunchKernelGGL(k_trsv_bwd<T>, dim3(nt), dim3(WG), 0, h->stream, L, (int64_t)n, (int)n, LiB,
                      tv, h->tx, h->sync, g, want);
   HIPC(hipGetLastError());
   if ((rc = tile_pass<T>(h, A, m, n, h->tx, h->rpa, nullptr, nullptr, g, want))) return rc;
-  hipLaunchKernelGGL(k_solve_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m,
-                     h->tx, h->rpa, tiles_c(n), r, h->diag, out, accumulate, g, want);
+  if (add_to)
+    hipLaunchKernelGGL(k_solve_add, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m,
+                       h->tx, h->rpa, tiles_c(n), r, h->diag, add_to, g, want);
+  else
+    hipLaunchKernelGGL(k_solve_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n,
+                       (int)m, h->tx, h->rpa, tiles_c(n), r, h->diag, out, accumulate, g, want);
   HIPC(hipGetLastError());
   if (!g) h->info_acc.backend_solves++;
   return QTQP_OK;
@@ -985,7 +1171,7 @@ void release(qtqp_kkt* h) {
   void* bufs[] = {h->A64, h->Poff64, h->A32, h->Poff32, h->diag, h->corr, h->G64, h->G32,
                   h->info, h->vin, h->vout, h->vsol, h->vres, h->rin, h->racc, h->rres,
                   h->rtmp, h->tx, h->tg, h->part, h->dnorm,
-                  h->tv, h->linv, h->sync, h->rpa, h->rpp, h->aty};
+                  h->tv, h->linv, h->sync, h->rpa, h->rpp, h->aty, h->gpart};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -1066,6 +1252,18 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
   const int64_t nt = (n + NB - 1) / NB;
   ALLOC(linv, double, nt * 2 * NB * NB);
   ALLOC(sync, unsigned, kSyncHdr + 2 * nt);
+  {
+    // Split the Gram product over the rows of A when its lower tiles alone
+    // would leave most of the chip idle: about 512 workgroups, chunks of at
+    // least 128 rows.
+    const int64_t ng = nt * (nt + 1) / 2;
+    const int64_t want_chunks = std::min<int64_t>((512 + ng - 1) / ng, (m + 127) / 128);
+    if (want_chunks > 1) {
+      h->gchunk = (int)((m + want_chunks - 1) / want_chunks);
+      h->gsplit = (int)((m + h->gchunk - 1) / h->gchunk);
+    }
+  }
+  if (h->gsplit > 1) ALLOC(gpart, double, (nt * (nt + 1) / 2) * h->gsplit * NB * NB);
   ALLOC(part, double, (int64_t)nblk_rows(std::max(m, n)) * n);
   ALLOC(rpa, double, m * tiles_c(n));
   ALLOC(rpp, double, n * tiles_c(n));

@@ -88,7 +88,7 @@ class _HipDenseBase(LinearSolver):
   device_driver = True
   # Each device-loop iteration is one HIP graph launch with the refinement and
   # precision decisions taken on the device (False: host-driven launches);
-  # device_graph_steps refinement steps per solve are in the graph (0: 3).
+  # device_graph_steps refinement steps per solve are in the graph (0: adaptive).
   device_graph = True
   device_graph_steps = 0
 
