@@ -111,7 +111,9 @@ def _cpu_model():
 def algorithmic_work(n, m, counts, iters):
   """Algorithmic FLOPs and HBM bytes of the executed work of every QP.
 
-  Per QP, from the engine's own counters: each factorisation costs
+  Per QP, from the engine's own counters (m is the rows of A the engine
+  stores: a row that is the exact negation of another is stored once and
+  serves both): each factorisation costs
   m n^2 + n^3/3 flops (Gram product + LDL') and reads A once (FP32 or
   FP64); each refinement pass (the rest of a correction, the FP64 residual
   and the next right-hand side, fused) reads A and P once in FP64; each
@@ -618,7 +620,8 @@ def main():
                                            "frac_le_1e-8": float(np.mean(kkt_t <= 1e-8))}}
 
   # roofline of the dominant kernel (the per-iteration IPM kernel)
-  flops32, flops64, bytes_ = algorithmic_work(n, m, work, iters)
+  mc = bq.stored_rows()
+  flops32, flops64, bytes_ = algorithmic_work(n, mc, work, iters)
   iter_ms = float(np.sum(launch_ms))
   t_mem = bytes_ / (HBM_PEAK_GBS * 1e9) * 1e3
   t_mfma = flops32 / (FP32_PEAK_TFLOPS * 1e12) * 1e3 + flops64 / (FP64_PEAK_TFLOPS * 1e12) * 1e3
@@ -635,6 +638,7 @@ def main():
   roof.update({"kernel": "k_ipm_solve", "launches": nl, "avg_launch_ms": iter_ms / nl,
                "algorithmic_bytes_per_launch": bytes_ / nl,
                "algorithmic_flops_per_launch": (flops32 + flops64) / nl,
+               "stored_rows_of_a": float(mc.sum() / m.sum()),
                "combined_frac": (t_mem + t_mfma) / iter_ms})
 
   # plain FP64 GPU paths, one QP per call, FP64 factor/solves, no batching/graph:

@@ -72,6 +72,24 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
                       const int64_t* m_off, const double* a, const double* p,
                       const double* b, const double* c, qtqp_batch** out);
 
+/* qtqp_batch_create with flags (QTQP_BATCH_NO_ROW_PAIRS or 0).  By default
+ * rows that are exact negations of another row (A[q] = -A[p], as in the
+ * two-sided bounds lo <= G x <= hi written as two inequality blocks) are
+ * stored once: every pass over A reads the stored rows only and forms both
+ * constraints' products from them (A[q] x = -A[p] x; the pair's terms of
+ * A'y and A'WA are summed as (y_p - y_q) A[p] and (w_p + w_q) A[p]'A[p]), so
+ * results agree with the unpaired layout to rounding, not bit for bit.
+ * No reference counterpart: the reference keeps A as given. */
+#define QTQP_BATCH_NO_ROW_PAIRS 1
+int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                         const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                         const int64_t* m_off, const double* a, const double* p,
+                         const double* b, const double* c, int32_t flags, qtqp_batch** out);
+
+/* Stored rows of A per QP (m[i] less the number of row pairs), in creation
+ * order.  No reference counterpart (batch diagnostics). */
+int qtqp_batch_stored_rows(qtqp_batch* h, int32_t* mc);
+
 /* Sets the starting point of the next solve for every QP: x, y, s in the
  * unscaled coordinates (equilibrated on the device like _init_variables,
  * __init__.py:299-307 and 668-669) and tau.  Without this call the default

@@ -44,6 +44,10 @@ BATCH_SYMBOLS = {
     "qtqp_batch_create": (ctypes.c_int, [ctypes.c_int32, _i32p, _i32p, _i32p, _i64p, _i64p, _i64p,
                                          _i64p, _dp, _dp, _dp, _dp,
                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_batch_create_ex": (ctypes.c_int, [ctypes.c_int32, _i32p, _i32p, _i32p, _i64p, _i64p,
+                                            _i64p, _i64p, _dp, _dp, _dp, _dp, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_batch_stored_rows": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "qtqp_batch_set_start": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "qtqp_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts),
                                         ctypes.POINTER(BatchCounters)]),
@@ -165,9 +169,13 @@ class BatchQP:
 
   Each QP follows qtqp's conventions: ``min 1/2 x'Px + c'x  s.t.
   Ax + s = b, s[:z] = 0, s[z:] >= 0``; ``p`` may be None (LP).
+  With ``pair_rows`` (default) rows of A that are exact negations of another
+  row are stored once on the device (``stored_rows`` gives the count).
   """
 
-  def __init__(self, qps: Sequence):
+  NO_ROW_PAIRS = 1
+
+  def __init__(self, qps: Sequence, pair_rows: bool = True):
     self._lib = batch_lib()
     self._h = None
     ns, ms, zs, a_parts, p_parts, b_parts, c_parts = [], [], [], [], [], [], []
@@ -214,13 +222,19 @@ class BatchQP:
     b_all = np.concatenate(b_parts)
     c_all = np.concatenate(c_parts)
     h = ctypes.c_void_p()
-    rc = self._lib.qtqp_batch_create(self.count, _ptr(self.n, _i32p), _ptr(self.m, _i32p),
-                                     _ptr(self.z, _i32p), _ptr(self.a_off, _i64p),
-                                     _ptr(self.p_off, _i64p), _ptr(self.n_off, _i64p),
-                                     _ptr(self.m_off, _i64p), _ptr(a_all, _dp), _ptr(p_all, _dp),
-                                     _ptr(b_all, _dp), _ptr(c_all, _dp), ctypes.byref(h))
+    rc = self._lib.qtqp_batch_create_ex(
+        self.count, _ptr(self.n, _i32p), _ptr(self.m, _i32p), _ptr(self.z, _i32p),
+        _ptr(self.a_off, _i64p), _ptr(self.p_off, _i64p), _ptr(self.n_off, _i64p),
+        _ptr(self.m_off, _i64p), _ptr(a_all, _dp), _ptr(p_all, _dp), _ptr(b_all, _dp),
+        _ptr(c_all, _dp), 0 if pair_rows else self.NO_ROW_PAIRS, ctypes.byref(h))
     _check(self._lib, rc)
     self._h = h
+
+  def stored_rows(self) -> np.ndarray:
+    """Rows of A each QP keeps on the device (m less its row pairs)."""
+    out = np.zeros(self.count, dtype=np.int32)
+    _check(self._lib, self._lib.qtqp_batch_stored_rows(self._h, _ptr(out, _i32p)))
+    return out
 
   def set_start(self, x, y, s, tau) -> None:
     """Starting point of the next solve (unscaled coordinates), packed or lists."""

@@ -21,10 +21,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dense_ops.h"
@@ -49,8 +51,25 @@ constexpr int kTraceRows = 128;
 enum { I_STATUS = 0, I_ITERS, I_FORCE64, I_FALLBACK, I_F32, I_F64, I_KMV, I_FS32, I_FS64,
        I_NINT = 12 };
 
+// Rows of A that are exact negations of each other (a two-sided constraint
+// l <= a'x <= u posed as a'x <= u and -a'x <= -l, or a bound pair) are
+// stored once: stored row r stands for row p(r) and, if it has one, its
+// partner q(r) with A[q] = -A[p].  Every pass over A then reads the stored
+// rows only.  The map is a few segments of consecutive rows: in segment k
+// (stored rows [start[k], start[k+1])) p = r + dp[k] and q = r + dq[k], or
+// no partner when dq[k] == kNoPartner.
+constexpr int kMaxSeg = 4;
+constexpr int kNoPartner = -(1 << 30);
+struct PairMap {
+  int nseg;
+  int start[kMaxSeg + 1];
+  int dp[kMaxSeg], dq[kMaxSeg];
+};
+
 struct Inst {
   int n, m, z, has_p;
+  int mc;       // stored rows of A (m when no row has a partner)
+  PairMap pm;
   int ld;    // leading dimension of A, A32, P, P32: n rounded up to 4
   int tile;  // Gram tile shape (QTQP_BATCH_TILE_*), chosen per size bucket
   // raw data
@@ -116,6 +135,33 @@ __device__ inline Smem lds_smem() {
 }
 
 __device__ inline Smem lds_view(const Smem&) { return lds_smem(); }
+
+// The segment table of a QP's row pairs, in scalar registers.
+struct PairRegs {
+  int nseg, s1, s2, s3;
+  int dp[kMaxSeg], dq[kMaxSeg];
+  __device__ inline explicit PairRegs(const PairMap& pm) {
+    nseg = wave_uniform(pm.nseg);
+    s1 = wave_uniform(pm.start[1]);
+    s2 = wave_uniform(pm.start[2]);
+    s3 = wave_uniform(pm.start[3]);
+#pragma unroll
+    for (int k = 0; k < kMaxSeg; ++k) {
+      dp[k] = wave_uniform(pm.dp[k]);
+      dq[k] = wave_uniform(pm.dq[k]);
+    }
+  }
+  // row p(r) and its partner q(r) (-1: none); segments past nseg are empty
+  __device__ inline void rows(int r, int* p, int* q) const {
+    const int k = (r >= s1) + (r >= s2) + (r >= s3);
+    int a = dp[0], b = dq[0];
+    if (k == 1) { a = dp[1]; b = dq[1]; }
+    if (k == 2) { a = dp[2]; b = dq[2]; }
+    if (k == 3) { a = dp[3]; b = dq[3]; }
+    *p = r + a;
+    *q = b == kNoPartner ? -1 : r + b;
+  }
+};
 
 template <typename T>
 __device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
@@ -291,7 +337,7 @@ template <typename T, int NJ, bool DIAG>
 __device__ __noinline__ void gram_pass(const Inst& I, const T* A, const T* P, T* G,
                                        const double* w, int i0, int j00, T* lds_in) {
   T* lds = in_lds(lds_in);
-  const int n = I.n, m = I.m, ld = I.ld;
+  const int n = I.n, m = I.mc, ld = I.ld;  // the stored rows of A
   int j0[NJ];
   j0[0] = j00;
   if (NJ > 1) j0[NJ - 1] = DIAG ? i0 : j00 + NB;
@@ -313,7 +359,7 @@ __device__ __noinline__ void gram_pass(const Inst& I, const T* A, const T* P, T*
 template <typename T, int MT>
 __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, const double* w,
                                   Smem& sm) {
-  const int n = I.n, m = I.m, ld = I.ld;
+  const int n = I.n, m = I.mc, ld = I.ld;  // the stored rows of A
   if constexpr (MT == 1) {
     static_assert(3 * GramRow<T, 2, false>::KCH * (NB + 1) * sizeof(T) <= kLdsU * sizeof(double),
                   "GramRow staging fits the LDS union");
@@ -355,8 +401,19 @@ template <typename T>
 __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T* G, Smem& sm_in,
                                        double jit_rel = 1e-14) {
   Smem sm = lds_view(sm_in);
-  const int n = I.n, m = I.m;
-  for (int i = threadIdx.x; i < m; i += WG) I.um[i] = -1.0 / I.dg[n + i];
+  const int n = I.n, mc = I.mc;
+  // weights of the stored rows: W = -1/d_y of the row plus its partner's
+  // (A[q] = -A[p], so both add w a a' with the same a)
+  {
+    const PairRegs pr(I.pm);
+    for (int r = threadIdx.x; r < mc; r += WG) {
+      int p, q;
+      pr.rows(r, &p, &q);
+      double w = -1.0 / I.dg[n + p];
+      if (q >= 0) w += -1.0 / I.dg[n + q];
+      I.um[r] = w;
+    }
+  }
   __syncthreads();
   // 128 x 128 tiles halve the re-reads of A for large n; they are built for
   // the FP32 factor only (the FP64 accumulators would not fit in registers).
@@ -631,6 +688,8 @@ constexpr int row_overrun() { return R - 1; }
 // outputs: a multiple of every stage rotation (D + 1) * 4R used below, and
 // small enough for three doubles per row in the LDS scratch.
 constexpr int kPassChunk = 2016;
+// The same for passes that stage five outputs per row (a row pair's).
+constexpr int kPairChunk = 1344;
 
 // One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
 // zero-padded to even; ld <= 128 NS): for every row i,
@@ -659,8 +718,8 @@ constexpr int kPassChunk = 2016;
 // max_j |M[i][j]| xs[j] and the column value max_i |M[i][j]| c1_i (for
 // non-negative xs and c1; equilibration).  NOUT may be 0.
 // scratch: >= max(8 ncol, NOUT kPassChunk) doubles of LDS.
-template <int NS, int R, int D, int NOUT, int COLS, bool MAXM = false, typename RL, typename RE,
-          typename RF, typename CE>
+template <int NS, int R, int D, int NOUT, int COLS, bool MAXM = false, int CHUNK = kPassChunk,
+          typename RL, typename RE, typename RF, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
                                 const double* __restrict__ xs, RL rowload, RE rowepi, RF rowflush,
                                 CE colepi, double* scratch) {
@@ -669,7 +728,7 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
   constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
   constexpr int STEP = 4 * R;  // rows a workgroup takes per stage
   static_assert(R == 2 || R == 4 || R == 8, "rows per wavefront");
-  static_assert(kPassChunk % ((D + 1) * STEP) == 0, "chunks hold whole stage rotations");
+  static_assert(CHUNK % ((D + 1) * STEP) == 0, "chunks hold whole stage rotations");
   static_assert(row_overrun<NS, R, D>() <= kPadRows && 128 * NS <= kPadCols, "slack after M");
   M = wave_uniform(M);
   nrow = wave_uniform(nrow);
@@ -729,7 +788,7 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       rowepi(row, d[0], st.in, &c1, &c2, out);
       if ((lane & ((64 >> LR) - 1)) == 0)
 #pragma unroll
-        for (int k = 0; k < NOUT; ++k) scratch[k * kPassChunk + row - c0] = out[k];
+        for (int k = 0; k < NOUT; ++k) scratch[k * CHUNK + row - c0] = out[k];
     }
     if constexpr (COLS > 0) {
 #pragma unroll
@@ -756,8 +815,8 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
     const int first = wave * R;
 #pragma unroll
     for (int k = 0; k < D; ++k) load(first + k * STEP, st[k]);
-    for (int c0 = 0; c0 < nrow; c0 += kPassChunk) {
-      const int cend = min(nrow, c0 + kPassChunk);
+    for (int c0 = 0; c0 < nrow; c0 += CHUNK) {
+      const int cend = min(nrow, c0 + CHUNK);
       // A full chunk holds whole rotations, so the next one starts at st[0].
       // Every stage of a rotation issues its loads, also past cend, so the
       // loads in flight at the loop head are the same whichever way it was
@@ -775,7 +834,7 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
         for (int j = threadIdx.x; j < cend - c0; j += WG) {
           double out[NOUT];
 #pragma unroll
-          for (int k = 0; k < NOUT; ++k) out[k] = scratch[k * kPassChunk + j];
+          for (int k = 0; k < NOUT; ++k) out[k] = scratch[k * CHUNK + j];
           rowflush(c0 + j, out);
         }
         wait_vmem();
@@ -841,7 +900,8 @@ template <int NS, int R, int D>
 __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const IrState S1,
                                        const double* xc, bool init, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
-  const int n = I.n, m = I.m;
+  const int n = I.n, mc = I.mc;
+  const PairRegs pr(I.pm);
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
   const double* dy = wave_uniform(I.dg + n);
@@ -853,48 +913,80 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   double* s1ax = wave_uniform(S1.ax);
   double* s1ry = wave_uniform(S1.r + n);
   double mx = 0.0;
+  // Inputs of a stored row r: its row p and its partner q (A[q] = -A[p];
+  // without one the partner slots repeat p and are ignored).  The product
+  // with A[q] is the negated product with A[p], so one dot product serves
+  // both, and the column coefficients are c(p) - c(q).  ax (A x of the
+  // refinement iterate) is kept per stored row.
   struct RowIn {
-    double sy, r0, ax0, dy, ky, cy;
+    double sy[2], r0[2], dy[2], ky[2], cy[2], ax0;
+    int has_q;
   };
   // Row inputs are loaded whether or not `init` uses them (S0 holds valid
   // arrays in both modes), so no load depends on a branch.
   const double* sy0 = wave_uniform((init ? S1.sol : S0.sol) + n);
-  static_assert(3 * kPassChunk <= kLdsU && 8 * 1024 <= kLdsU, "LDS scratch of the row pass");
-  rowpass2<NS, R, D, 3, 2>(
-      I.A, m, n, I.ld, init ? S1.sol : xc,
-      [&](int i) {
+  static_assert(5 * kPairChunk <= kLdsU && 8 * 1024 <= kLdsU, "LDS scratch of the row pass");
+  rowpass2<NS, R, D, 5, 2, false, kPairChunk>(
+      I.A, mc, n, I.ld, init ? S1.sol : xc,
+      [&](int r) {
+        int p, q;
+        pr.rows(r, &p, &q);
         RowIn in;
-        in.sy = ldg(sy0 + i);
-        in.r0 = ldg(r0y + i);
-        in.ax0 = ldg(ax0 + i);
-        in.dy = ldg(dy + i);
-        in.ky = ldg(ky + i);
-        in.cy = ldg(cy + i);
+        in.has_q = q >= 0;
+        const int qq = q >= 0 ? q : p;
+        in.sy[0] = ldg(sy0 + p);
+        in.sy[1] = ldg(sy0 + qq);
+        in.r0[0] = ldg(r0y + p);
+        in.r0[1] = ldg(r0y + qq);
+        in.dy[0] = ldg(dy + p);
+        in.dy[1] = ldg(dy + qq);
+        in.ky[0] = ldg(ky + p);
+        in.ky[1] = ldg(ky + qq);
+        in.cy[0] = ldg(cy + p);
+        in.cy[1] = ldg(cy + qq);
+        in.ax0 = ldg(ax0 + r);
         return in;
       },
-      [&](int i, double t, const RowIn& in, double* c1, double* c2, double* out) {
-        double yv, axv, yc = 0.0;
-        if (init) {
-          yv = in.sy;
-          axv = t;
-        } else {
-          yc = -(t - in.r0) / in.dy;
-          yv = in.sy + yc;
-          axv = in.ax0 + t;
+      [&](int r, double t, const RowIn& in, double* c1, double* c2, double* out) {
+        const double axv = init ? t : in.ax0 + t;
+        double cc1 = 0.0, cc2 = 0.0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          // row p (h = 0): A x = axv, A x_c = t; its partner: the negatives
+          const double sg = h == 0 ? 1.0 : -1.0;
+          double yv, yc = 0.0;
+          if (init) {
+            yv = in.sy[h];
+          } else {
+            yc = -(sg * t - in.r0[h]) / in.dy[h];
+            yv = in.sy[h] + yc;
+          }
+          const double ry = in.ky[h] - (sg * axv + in.dy[h] * yv) + in.cy[h] * yv;
+          if (h == 0 || in.has_q) {
+            cc1 += sg * (init ? yv : yc);
+            cc2 += sg * (-ry / in.dy[h]);
+          }
+          out[h == 0 ? 0 : 3] = yv;
+          out[h == 0 ? 2 : 4] = ry;
         }
-        const double ry = in.ky - (axv + in.dy * yv) + in.cy * yv;
-        *c1 = init ? yv : yc;
-        *c2 = -ry / in.dy;
-        out[0] = yv;
+        *c1 = cc1;
+        *c2 = cc2;
         out[1] = axv;
-        out[2] = ry;
       },
-      [&](int i, const double* out) {
-        if (!init) stg(s1y + i, out[0]);
-        stg(s1ax + i, out[1]);
-        stg(s1ry + i, out[2]);
-        const double a = fabs(out[2]);
+      [&](int r, const double* out) {
+        int p, q;
+        pr.rows(r, &p, &q);
+        if (!init) stg(s1y + p, out[0]);
+        stg(s1ax + r, out[1]);
+        stg(s1ry + p, out[2]);
+        double a = fabs(out[2]);
         mx = fmax(mx, (a == a) ? a : INFINITY);
+        if (q >= 0) {
+          if (!init) stg(s1y + q, out[3]);
+          stg(s1ry + q, out[4]);
+          a = fabs(out[4]);
+          mx = fmax(mx, (a == a) ? a : INFINITY);
+        }
       },
       [&](int j, double s1, double s2) {
         S1.aty[j] = init ? s1 : S0.aty[j] + s1;
@@ -1361,27 +1453,39 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   double* ax = I.um;  // m
   double* aty = I.un; // n
   {
+    // over the stored rows: row p gets (A x)/d, its partner the negative
+    // (the partner's d equals p's), and the column coefficient is y_p - y_q
     struct TIn {
-      double y, d;
+      double y, yq, d;
     };
+    const PairRegs pr(I.pm);
     const double* yv = wave_uniform(I.y);
     const double* dv = wave_uniform(I.d);
     double* axw = wave_uniform(ax);
     with_pass_shape(I.ld, [&](auto sh) {
       using S = decltype(sh);
       rowpass2<S::NS, S::R, S::D, 1, 1>(
-          I.A, m, n, I.ld, I.x,
-          [&](int i) {
+          I.A, I.mc, n, I.ld, I.x,
+          [&](int r) {
+            int p, q;
+            pr.rows(r, &p, &q);
             TIn in;
-            in.y = ldg(yv + i);
-            in.d = ldg(dv + i);
+            in.y = ldg(yv + p);
+            const double yq = ldg(yv + (q >= 0 ? q : p));  // unconditional load
+            in.yq = q >= 0 ? yq : 0.0;
+            in.d = ldg(dv + p);
             return in;
           },
           [](int, double t, const TIn& in, double* c1, double*, double* o) {
-            *c1 = in.y;
+            *c1 = in.y - in.yq;
             o[0] = t / in.d;
           },
-          [&](int i, const double* o) { stg(axw + i, o[0]); },
+          [&](int r, const double* o) {
+            int p, q;
+            pr.rows(r, &p, &q);
+            stg(axw + p, o[0]);
+            if (q >= 0) stg(axw + q, -o[0]);
+          },
           [&](int j, double s1, double) { aty[j] = s1 / I.e[j]; }, reinterpret_cast<double*>(sm.u));
     });
   }
@@ -1481,15 +1585,23 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 // equilibrate): row maxima of |D A E| give D, column maxima of |D A E| and
 // |E P E| give E, each factor clamped to [1e-3, 1e3].  Each pass streams A
 // and P once with the pipelined row pass in its max-abs form.
+//  The passes run over the stored rows of A; a row and its negated partner
+//  have equal norms, so they share every factor.
 template <typename S>
 __device__ inline void ruiz(const Inst& I, double* scratch) {
-  const int n = I.n, m = I.m, ld = I.ld, tid = threadIdx.x;
+  const int n = I.n, mc = I.mc, ld = I.ld, tid = threadIdx.x;
+  const PairRegs pr(I.pm);
   const double* dv = wave_uniform(I.d);
   const double* ev = wave_uniform(I.e);
   double* um = wave_uniform(I.um);
   for (int pass = 0; pass < 10; ++pass) {
     rowpass2<S::NS, S::R, S::D, 1, 1, true>(
-        I.A, m, n, ld, ev, [&](int i) { return ldg(dv + i); },
+        I.A, mc, n, ld, ev,
+        [&](int r) {
+          int p, q;
+          pr.rows(r, &p, &q);
+          return ldg(dv + p);
+        },
         [](int, double v, double di, double* c1, double*, double* o) {
           *c1 = di;
           const double r = v * di;
@@ -1506,7 +1618,13 @@ __device__ inline void ruiz(const Inst& I, double* scratch) {
           I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
         },
         scratch);
-    for (int i = tid; i < m; i += WG) I.d[i] *= I.um[i];
+    for (int r = tid; r < mc; r += WG) {
+      int p, q;
+      pr.rows(r, &p, &q);
+      const double f = I.um[r];
+      I.d[p] *= f;
+      if (q >= 0) I.d[q] *= f;
+    }
     for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
     __syncthreads();
   }
@@ -1515,19 +1633,23 @@ __device__ inline void ruiz(const Inst& I, double* scratch) {
 // rows x n row-major src into dst with leading dimension ld (zeros in the
 // pad): each wave copies 4 rows per round, each lane two columns of each
 // per 128 columns, so 8 loads of a round are in flight together.
+//  With pr, dst row r is src row p(r) (the stored rows of A).
 __device__ inline void copy_padded(const double* __restrict__ src, double* __restrict__ dst,
-                                   int rows, int n, int ld) {
+                                   int rows, int n, int ld, const PairRegs* pr = nullptr) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i0 = 4 * wave; i0 < rows; i0 += 16)
     for (int j0 = 0; j0 < ld; j0 += 128) {
       double v[4][2];
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < 4; ++r) {
+        int si = i0 + r, q;
+        if (pr && si < rows) pr->rows(si, &si, &q);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int i = i0 + r, j = j0 + lane + 64 * h;
-          v[r][h] = (i < rows && j < n) ? ldg(src + (int64_t)i * n + j) : 0.0;
+          v[r][h] = (i < rows && j < n) ? ldg(src + (int64_t)si * n + j) : 0.0;
         }
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -1600,14 +1722,21 @@ __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, const int* order, i
     for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
     for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
     // unscaled copies with the padded leading dimension (zeros in the pad)
-    const int ld = I.ld;
-    copy_padded(I.Araw, I.A, m, n, ld);
+    const int ld = I.ld, mc = I.mc;
+    const PairRegs pr(I.pm);
+    copy_padded(I.Araw, I.A, mc, n, ld, &pr);
     copy_padded(I.Praw, I.P, n, n, ld);
     __syncthreads();
     if (equilibrate) with_pass_shape(ld, [&](auto sh) { ruiz<decltype(sh)>(I, scratch); });
     __syncthreads();
-    // D A E and E P E in place, and their FP32 copies
-    scale_rows_cols(I.A, I.A32, I.d, I.e, m, n, ld);
+    // D A E and E P E in place, and their FP32 copies (um: d of the stored rows)
+    for (int r = tid; r < mc; r += WG) {
+      int p, q;
+      pr.rows(r, &p, &q);
+      I.um[r] = I.d[p];
+    }
+    __syncthreads();
+    scale_rows_cols(I.A, I.A32, I.um, I.e, mc, n, ld);
     scale_rows_cols(I.P, I.P32, I.e, I.e, n, n, ld);
     double lb = 0.0, lc = 0.0;
     for (int i = tid; i < m; i += WG) {
@@ -1775,6 +1904,7 @@ struct qtqp_batch {
   bool have_start = false;
   std::vector<int64_t> n_off, m_off;
   std::vector<int> n, m;
+  std::vector<int> mc;  // stored rows of A per QP (row pairs kept once)
   int64_t tot_n = 0, tot_m = 0;
   int64_t *d_noff = nullptr, *d_moff = nullptr;
   double *d_px = nullptr, *d_py = nullptr, *d_ps = nullptr, *d_scal = nullptr;
@@ -1827,6 +1957,96 @@ int size_bucket(int n) {
   return QTQP_BATCH_NBUCKETS - 1;
 }
 
+// The row pairs of one QP's A (m x n row-major): rows that are exact
+// negations of each other, found by hashing each row with its sign fixed by
+// its first nonzero and comparing candidates exactly.  The stored rows are the
+// rows without a partner and the first of each pair, in row order; pm maps
+// them back as a few segments of consecutive rows.  Returns the number of
+// stored rows (m, with the identity map, when pairing is off or the pairs do
+// not fit in kMaxSeg segments).
+int find_row_pairs(const double* a, int m, int n, bool enable, PairMap* pm) {
+  auto identity = [&]() {
+    pm->nseg = 1;
+    for (int k = 0; k <= kMaxSeg; ++k) pm->start[k] = k == 0 ? 0 : m;
+    for (int k = 0; k < kMaxSeg; ++k) {
+      pm->dp[k] = 0;
+      pm->dq[k] = kNoPartner;
+    }
+    return m;
+  };
+  if (!enable) return identity();
+  struct Key {
+    uint64_t h;
+    int neg, row;
+  };
+  std::vector<Key> keys(m);
+  for (int i = 0; i < m; ++i) {
+    const double* r = a + (int64_t)i * n;
+    int f = 0;
+    while (f < n && r[f] == 0.0) ++f;
+    const double sg = (f < n && r[f] < 0.0) ? -1.0 : 1.0;
+    uint64_t h = 1469598103934665603ull;
+    for (int j = f; j < n; ++j) {
+      const double v = sg * r[j] + 0.0;  // -0 -> +0
+      uint64_t bits;
+      std::memcpy(&bits, &v, 8);
+      h = (h ^ bits) * 1099511628211ull;
+      h ^= h >> 29;
+    }
+    keys[i] = {h, sg < 0.0 ? 1 : 0, i};
+  }
+  std::sort(keys.begin(), keys.end(), [](const Key& u, const Key& v) {
+    return u.h != v.h ? u.h < v.h : u.row < v.row;
+  });
+  std::vector<int> partner(m, -1);
+  for (size_t b0 = 0; b0 < keys.size();) {
+    size_t b1 = b0;
+    while (b1 < keys.size() && keys[b1].h == keys[b0].h) ++b1;
+    for (size_t u = b0; u < b1; ++u) {
+      if (keys[u].neg || partner[keys[u].row] >= 0) continue;
+      const double* ru = a + (int64_t)keys[u].row * n;
+      for (size_t v = b0; v < b1; ++v) {
+        if (!keys[v].neg || partner[keys[v].row] >= 0) continue;
+        const double* rv = a + (int64_t)keys[v].row * n;
+        bool neg = true;
+        for (int j = 0; j < n && neg; ++j) neg = ru[j] == -rv[j];
+        if (neg) {
+          partner[keys[u].row] = keys[v].row;
+          partner[keys[v].row] = keys[u].row;
+          break;
+        }
+      }
+    }
+    b0 = b1;
+  }
+  // stored rows and their segments
+  int nseg = 0, r = 0, pdp = 0, pdq = 0;
+  PairMap out{};
+  for (int i = 0; i < m; ++i) {
+    const int q = partner[i];
+    if (q >= 0 && q < i) continue;  // the second row of a pair
+    const int dp = i - r, dq = q >= 0 ? q - r : kNoPartner;
+    if (nseg == 0 || dp != pdp || dq != pdq) {
+      if (nseg == kMaxSeg) return identity();
+      out.start[nseg] = r;
+      out.dp[nseg] = dp;
+      out.dq[nseg] = dq;
+      ++nseg;
+      pdp = dp;
+      pdq = dq;
+    }
+    ++r;
+  }
+  out.nseg = nseg;
+  for (int k = nseg; k <= kMaxSeg; ++k) out.start[k] = r;
+  for (int k = nseg; k < kMaxSeg; ++k) {
+    out.dp[k] = 0;
+    out.dq[k] = kNoPartner;
+  }
+  *pm = out;
+  return r;
+}
+
 // Bump allocator over one device allocation.
 struct Arena {
   unsigned char* base = nullptr;
@@ -1839,13 +2059,13 @@ struct Arena {
   }
 };
 
-size_t inst_bytes(int n, int m) {
+size_t inst_bytes(int n, int m, int mc) {
   const int64_t N = n + m, ld = (n + 3) & ~3;
-  const int64_t mn = (int64_t)m * ld, nn = (int64_t)n * ld;
+  const int64_t mn = (int64_t)m * ld, mcn = (int64_t)mc * ld, nn = (int64_t)n * ld;
   size_t b = 0;
   b += align256(8 * mn);           // raw A
-  b += align256(8 * (mn + a_pad((int)ld)));  // A (FP64) and its read slack
-  b += align256(4 * mn);           // A32
+  b += align256(8 * (mcn + a_pad((int)ld)));  // stored rows of A (FP64) and read slack
+  b += align256(4 * mcn);          // A32
   b += align256(8 * nn) + align256(8 * (nn + a_pad((int)ld)));  // raw P, P and its read slack
   b += align256(4 * nn);           // P32
   b += align256(8 * nn);           // G
@@ -1871,6 +2091,13 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
                       const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
                       const int64_t* m_off, const double* a, const double* p, const double* b,
                       const double* c, qtqp_batch** out) {
+  return qtqp_batch_create_ex(count, n, m, z, a_off, p_off, n_off, m_off, a, p, b, c, 0, out);
+}
+
+int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                         const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                         const int64_t* m_off, const double* a, const double* p,
+                         const double* b, const double* c, int32_t flags, qtqp_batch** out) {
   if (!out) return fail(QTQP_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (count < 1) return fail(QTQP_ERR_ARG, "count must be >= 1");
@@ -1902,8 +2129,25 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
   if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(QTQP_ERR_HIP, "stream"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return cleanup(fail(QTQP_ERR_HIP, "event"));
+  // row pairs of each QP, found on host threads
+  std::vector<PairMap> pmap(count);
+  std::vector<int> mcs(count);
+  {
+    const bool pairs = (flags & QTQP_BATCH_NO_ROW_PAIRS) == 0;
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int q; (q = next.fetch_add(1)) < count;)
+        mcs[q] = find_row_pairs(a + a_off[q], m[q], n[q], pairs, &pmap[q]);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < std::min(nt, count); ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+  }
+  h->mc = mcs;
   size_t total = 0;
-  for (int q = 0; q < count; ++q) total += inst_bytes(n[q], m[q]);
+  for (int q = 0; q < count; ++q) total += inst_bytes(n[q], m[q], mcs[q]);
   void* mem = nullptr;
   hipError_t err = hipMalloc(&mem, total);
   if (err != hipSuccess)
@@ -1926,9 +2170,11 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
     I.n = nq;
     I.m = mq;
     I.z = z[q];
+    I.mc = mcs[q];
+    I.pm = pmap[q];
     I.ld = (nq + 3) & ~3;
     I.tile = QTQP_BATCH_TILE_64;
-    const int64_t mld = (int64_t)mq * I.ld, nld = (int64_t)nq * I.ld;
+    const int64_t mld = (int64_t)I.mc * I.ld, nld = (int64_t)nq * I.ld;
     I.has_p = 0;
     for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
     double* araw = ar.take<double>(mn);
@@ -2296,6 +2542,12 @@ int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows) {
   if (!h || !out || q < 0 || q >= h->count || rows < 0) return fail(QTQP_ERR_ARG, "bad argument");
   const int r = std::min<int>(rows, kTraceRows);
   HIPC(hipMemcpy(out, h->hinst[q].trace, sizeof(double) * kTraceFields * r, hipMemcpyDeviceToHost));
+  return QTQP_OK;
+}
+
+int qtqp_batch_stored_rows(qtqp_batch* h, int32_t* mc) {
+  if (!h || !mc) return fail(QTQP_ERR_ARG, "NULL argument");
+  for (int q = 0; q < h->count; ++q) mc[q] = h->mc[q];
   return QTQP_OK;
 }
 

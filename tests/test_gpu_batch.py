@@ -239,3 +239,55 @@ def test_persistent_and_per_iteration_schedules_agree_bitwise(require_gpu, use_g
   for c, t in zip(cnts, inst):
     assert np.all(np.isfinite(t)) and np.all(t >= 0) and np.max(t) <= c["device_ms"]
   bq.free()
+
+
+def _paired_qp(seed):
+  """A feasible QP whose A has row pairs in four segments: equality rows
+  E (the first two negated again at the end, as inequalities), inequality
+  rows G and -G, then unpaired rows H."""
+  rng = np.random.default_rng(seed)
+  n, z, k, j = 24, 5, 30, 17
+  e, g, hh = (rng.standard_normal((r, n)) for r in (z, k, j))
+  a = np.concatenate([e, g, -g, hh, -e[:2]])
+  x0 = rng.standard_normal(n)
+  slack = rng.uniform(0.1, 1.0, a.shape[0] - z)
+  b = a @ x0
+  b[z:] += slack
+  r = rng.standard_normal((n, n))
+  p = r.T @ r / n + 0.1 * np.eye(n)
+  return problems.DenseQP(a, b, rng.standard_normal(n), z, p)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_row_pairs_stored_once(require_gpu, precision):
+  """Rows that negate another row are stored once: MPC's [I; -I; G; -G] keeps
+  half its rows, a QP with pairs in four segments keeps m less its pairs, and
+  one whose pairs need more segments (adjacent pairs) keeps every row.  Each
+  matches the oracle, and the paired and unpaired layouts agree closely."""
+  mpc = problems.uneven_mpc_batch(12, base_seed=700, t_range=(4, 20))
+  seg = _paired_qp(3)
+  rng = np.random.default_rng(4)
+  g = rng.standard_normal((12, 10))
+  adj = np.empty((24, 10))
+  adj[0::2], adj[1::2] = g, -g  # pairs (2i, 2i+1): one segment per pair
+  adj_qp = problems.DenseQP(adj, np.abs(rng.standard_normal(24)) + 0.5,
+                            rng.standard_normal(10), 0, np.eye(10))
+  qps = mpc + [seg, adj_qp]
+  paired = batch.BatchQP(qps)
+  plain = batch.BatchQP(qps, pair_rows=False)
+  mc = paired.stored_rows()
+  np.testing.assert_array_equal(mc[:12], [q.m // 2 for q in mpc])
+  assert mc[12] == seg.a.shape[0] - 30 - 2
+  assert mc[13] == 24
+  np.testing.assert_array_equal(plain.stored_rows(), [q.a.shape[0] for q in qps])
+  rp = paired.solve(precision=precision)
+  ru = plain.solve(precision=precision)
+  paired.free()
+  plain.free()
+  for q, sp, su in zip(qps, rp.solutions, ru.solutions):
+    ref = O.solve_qp(q.a, q.b, q.c, q.z, q.p)
+    _compare(sp, ref, 1e-6)
+    assert abs(sp.iters - ref["iters"]) <= 1
+    assert sp.status == su.status
+    for k in "xys":
+      assert _rel(getattr(sp, k), getattr(su, k)) < 1e-7, k
