@@ -81,6 +81,11 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
  * results agree with the unpaired layout to rounding, not bit for bit.
  * No reference counterpart: the reference keeps A as given. */
 #define QTQP_BATCH_NO_ROW_PAIRS 1
+/* By default each stored row's passes read only its nonzero column extent
+ * (the zero columns before the first and after the last nonzero read as
+ * zero without memory traffic: the bound rows of [I; -I] read one 16-byte
+ * piece).  Results are those of the full rows.  This flag reads full rows. */
+#define QTQP_BATCH_NO_ROW_EXTENTS 2
 int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
                          const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
                          const int64_t* m_off, const double* a, const double* p,

@@ -170,12 +170,15 @@ class BatchQP:
   Each QP follows qtqp's conventions: ``min 1/2 x'Px + c'x  s.t.
   Ax + s = b, s[:z] = 0, s[z:] >= 0``; ``p`` may be None (LP).
   With ``pair_rows`` (default) rows of A that are exact negations of another
-  row are stored once on the device (``stored_rows`` gives the count).
+  row are stored once on the device (``stored_rows`` gives the count); with
+  ``row_extents`` (default) the passes over A read only each row's nonzero
+  column extent (same results as full rows).
   """
 
   NO_ROW_PAIRS = 1
+  NO_ROW_EXTENTS = 2
 
-  def __init__(self, qps: Sequence, pair_rows: bool = True):
+  def __init__(self, qps: Sequence, pair_rows: bool = True, row_extents: bool = True):
     self._lib = batch_lib()
     self._h = None
     ns, ms, zs, a_parts, p_parts, b_parts, c_parts = [], [], [], [], [], [], []
@@ -226,7 +229,8 @@ class BatchQP:
         self.count, _ptr(self.n, _i32p), _ptr(self.m, _i32p), _ptr(self.z, _i32p),
         _ptr(self.a_off, _i64p), _ptr(self.p_off, _i64p), _ptr(self.n_off, _i64p),
         _ptr(self.m_off, _i64p), _ptr(a_all, _dp), _ptr(p_all, _dp), _ptr(b_all, _dp),
-        _ptr(c_all, _dp), 0 if pair_rows else self.NO_ROW_PAIRS, ctypes.byref(h))
+        _ptr(c_all, _dp), (0 if pair_rows else self.NO_ROW_PAIRS)
+        | (0 if row_extents else self.NO_ROW_EXTENTS), ctypes.byref(h))
     _check(self._lib, rc)
     self._h = h
 

@@ -70,6 +70,7 @@ struct Inst {
   int n, m, z, has_p;
   int mc;       // stored rows of A (m when no row has a partner)
   PairMap pm;
+  const uint32_t* ext;  // nonzero column extent lo | hi << 16 of each stored row
   int ld;    // leading dimension of A, A32, P, P32: n rounded up to 4
   int tile;  // Gram tile shape (QTQP_BATCH_TILE_*), chosen per size bucket
   // raw data
@@ -136,28 +137,37 @@ __device__ inline Smem lds_smem() {
 
 __device__ inline Smem lds_view(const Smem&) { return lds_smem(); }
 
-// The segment table of a QP's row pairs, in scalar registers.
+// The segment table of a QP's row pairs, in scalar registers.  A row's
+// offsets are formed by adding the steps between consecutive segments
+// selected by comparisons: choosing among the fields themselves lets the
+// compiler turn the choice into a load at a chosen address, which keeps the
+// table in scratch memory, and each such load waits for every load in
+// flight.
 struct PairRegs {
-  int nseg, s1, s2, s3;
-  int dp[kMaxSeg], dq[kMaxSeg];
+  int s1, s2, s3;
+  int dp0, dp1, dp2, dp3, dq0, dq1, dq2, dq3;
   __device__ inline explicit PairRegs(const PairMap& pm) {
-    nseg = wave_uniform(pm.nseg);
+    static_assert(kMaxSeg == 4, "four segments");
     s1 = wave_uniform(pm.start[1]);
     s2 = wave_uniform(pm.start[2]);
     s3 = wave_uniform(pm.start[3]);
-#pragma unroll
-    for (int k = 0; k < kMaxSeg; ++k) {
-      dp[k] = wave_uniform(pm.dp[k]);
-      dq[k] = wave_uniform(pm.dq[k]);
-    }
+    dp0 = wave_uniform(pm.dp[0]);
+    dp1 = wave_uniform(pm.dp[1]);
+    dp2 = wave_uniform(pm.dp[2]);
+    dp3 = wave_uniform(pm.dp[3]);
+    dq0 = wave_uniform(pm.dq[0]);
+    dq1 = wave_uniform(pm.dq[1]);
+    dq2 = wave_uniform(pm.dq[2]);
+    dq3 = wave_uniform(pm.dq[3]);
   }
   // row p(r) and its partner q(r) (-1: none); segments past nseg are empty
+  // (their start is the stored row count), so r picks the last one that
+  // starts at or before it
   __device__ inline void rows(int r, int* p, int* q) const {
-    const int k = (r >= s1) + (r >= s2) + (r >= s3);
-    int a = dp[0], b = dq[0];
-    if (k == 1) { a = dp[1]; b = dq[1]; }
-    if (k == 2) { a = dp[2]; b = dq[2]; }
-    if (k == 3) { a = dp[3]; b = dq[3]; }
+    const bool g1 = r >= s1, g2 = r >= s2, g3 = r >= s3;
+    // (steps between kNoPartner and an offset stay within int: |offset| < 2^29)
+    const int a = dp0 + (g1 ? dp1 - dp0 : 0) + (g2 ? dp2 - dp1 : 0) + (g3 ? dp3 - dp2 : 0);
+    const int b = dq0 + (g1 ? dq1 - dq0 : 0) + (g2 ? dq2 - dq1 : 0) + (g3 ? dq3 - dq2 : 0);
     *p = r + a;
     *q = b == kNoPartner ? -1 : r + b;
   }
@@ -177,6 +187,8 @@ __device__ inline T (*tile_ptr(unsigned char* p, int row_off))[NB + 1] {
 template <typename T, int W> struct VecT;
 template <> struct VecT<double, 2> { typedef double type __attribute__((ext_vector_type(2))); };
 template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct VecT<uint32_t, 2> { typedef uint32_t type __attribute__((ext_vector_type(2))); };
+template <> struct VecT<uint32_t, 4> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
 
 template <typename T> struct VW;
 template <> struct VW<double> { static constexpr int value = 2; };
@@ -691,6 +703,43 @@ constexpr int kPassChunk = 2016;
 // The same for passes that stage five outputs per row (a row pair's).
 constexpr int kPairChunk = 1344;
 
+// Row extents of one stage of R rows (R packed words) and the word of row r.
+template <int R>
+struct ExtVec;
+template <>
+struct ExtVec<2> {
+  using type = VecT<uint32_t, 2>::type;
+};
+template <>
+struct ExtVec<4> {
+  using type = VecT<uint32_t, 4>::type;
+};
+struct Ext8 {
+  VecT<uint32_t, 4>::type a, b;
+};
+template <>
+struct ExtVec<8> {
+  using type = Ext8;
+};
+template <int R>
+__device__ inline typename ExtVec<R>::type load_ext(const uint32_t* p) {
+  using U4 = VecT<uint32_t, 4>::type;
+  if constexpr (R == 8)
+    return Ext8{ldg(reinterpret_cast<const U4*>(p)), ldg(reinterpret_cast<const U4*>(p + 4))};
+  else
+    return ldg(reinterpret_cast<const typename ExtVec<R>::type*>(p));
+}
+template <int R>
+__device__ inline uint32_t ext_word(const typename ExtVec<R>::type& e, int r) {
+  if constexpr (R == 8)
+    return r < 4 ? e.a[r] : e.b[r - 4];
+  else
+    return e[r];
+}
+// Extent words past the last stored row (read by the stages that prefetch
+// beyond it): covers (2 D + 2) stages of 4 R rows for every pass shape.
+constexpr int kExtPad = 256;
+
 // One pass over the rows of row-major M (nrow x ncol, leading dimension ld,
 // zero-padded to even; ld <= 128 NS): for every row i,
 //   rowepi(i, M[i,:] . xs, in_i, &c1_i, &c2_i, out_i)   with in_i = rowload(i)
@@ -717,12 +766,21 @@ constexpr int kPairChunk = 1344;
 // replaces the sums by maxima of absolute values: the row value is
 // max_j |M[i][j]| xs[j] and the column value max_i |M[i][j]| c1_i (for
 // non-negative xs and c1; equilibration).  NOUT may be 0.
+// With EXT, ext[i] = lo | hi << 16 bounds the nonzero columns [lo, hi) of
+// row i (0 past the last row): the row is read through a buffer resource
+// and the 16-byte pieces outside its extent are given an out-of-range
+// offset, so they read as zero without a memory request.  The values left
+// out are exact zeros, so every dot product and column sum is the one the
+// full row gives.  A stage's extents are loaded D stages ahead of its rows,
+// before the loads of the stage then issued, so waiting for them never waits
+// for rows not yet needed.
 // scratch: >= max(8 ncol, NOUT kPassChunk) doubles of LDS.
 template <int NS, int R, int D, int NOUT, int COLS, bool MAXM = false, int CHUNK = kPassChunk,
-          typename RL, typename RE, typename RF, typename CE>
+          bool EXT = false, typename RL, typename RE, typename RF, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
                                 const double* __restrict__ xs, RL rowload, RE rowepi, RF rowflush,
-                                CE colepi, double* scratch) {
+                                CE colepi, double* scratch,
+                                const uint32_t* __restrict__ ext = nullptr) {
   using V = VecT<double, 2>::type;
   using In = decltype(rowload(0));
   constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
@@ -730,6 +788,7 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
   static_assert(R == 2 || R == 4 || R == 8, "rows per wavefront");
   static_assert(CHUNK % ((D + 1) * STEP) == 0, "chunks hold whole stage rotations");
   static_assert(row_overrun<NS, R, D>() <= kPadRows && 128 * NS <= kPadCols, "slack after M");
+  static_assert(3 * D * STEP + R <= kExtPad, "extent words read past the last row");
   M = wave_uniform(M);
   nrow = wave_uniform(nrow);
   ld = wave_uniform(ld);
@@ -746,19 +805,44 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       c1a[s][e] = 0.0;
       c2a[s][e] = 0.0;
     }
+  using E = typename ExtVec<R>::type;
   struct Stage {
     V a[R][NS];
     In in;
+    E e;  // extents of the rows of the stage D stages after this one (EXT)
   };
-  auto load = [&](int i0, Stage& st) {
-    // A stage past the last row re-reads the rows from nrow - 1 on: they are
-    // in cache, where fresh slack would cost HBM traffic.
-    const double* base = M + (int64_t)min(i0, nrow - 1) * ld + lane * 2;
+  // (unused without EXT)
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(M), 0, nrow * ld * 8, 0x00020000);
+  auto load = [&](int i0, Stage& st, Stage& ahead) {
+    if constexpr (EXT) {
+      // the extents of rows i0 .. i0 + R - 1 arrived in st.e; those of the
+      // stage D ahead go into its slot before this stage's rows are read
+      const E e = st.e;
+      ahead.e = load_ext<R>(ext + i0 + D * STEP);
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+      for (int r = 0; r < R; ++r) {
+        const uint32_t er = ext_word<R>(e, r);
+        const int lo = er & 0xffff, hi = er >> 16;
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
-        st.a[r][s] = ldg(reinterpret_cast<const V*>(base + (int64_t)r * ld + s * 128));
+        for (int s = 0; s < NS; ++s) {
+          const int c = s * 128 + lane * 2;
+          const uint32_t off = (c + 2 > lo && c < hi)
+                                   ? (uint32_t)(((i0 + r) * ld + c) * 8)
+                                   : 0x80000000u;
+          st.a[r][s] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+        }
+      }
+    } else {
+      // A stage past the last row re-reads the rows from nrow - 1 on: they
+      // are in cache, where fresh slack would cost HBM traffic.
+      const double* base = M + (int64_t)min(i0, nrow - 1) * ld + lane * 2;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          st.a[r][s] = ldg(reinterpret_cast<const V*>(base + (int64_t)r * ld + s * 128));
+    }
     st.in = rowload(min(i0 + lrow, nrow - 1));
   };
   auto process = [&](int i0, int c0, const Stage& st) {
@@ -813,8 +897,12 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
     wait_mem();
     Stage st[D + 1];
     const int first = wave * R;
+    if constexpr (EXT) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) load(first + k * STEP, st[k]);
+      for (int k = 0; k < D; ++k) st[k].e = load_ext<R>(ext + first + k * STEP);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) load(first + k * STEP, st[k], st[(k + D) % (D + 1)]);
     for (int c0 = 0; c0 < nrow; c0 += CHUNK) {
       const int cend = min(nrow, c0 + CHUNK);
       // A full chunk holds whole rotations, so the next one starts at st[0].
@@ -825,7 +913,7 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
 #pragma unroll
         for (int u = 0; u <= D; ++u) {
           const int i = i0 + u * STEP;
-          load(i + D * STEP, st[(u + D) % (D + 1)]);
+          load(i + D * STEP, st[(u + D) % (D + 1)], st[(u + 2 * D) % (D + 1)]);
           if (i < cend) process(i, c0, st[u]);
         }
       }
@@ -926,7 +1014,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   // arrays in both modes), so no load depends on a branch.
   const double* sy0 = wave_uniform((init ? S1.sol : S0.sol) + n);
   static_assert(5 * kPairChunk <= kLdsU && 8 * 1024 <= kLdsU, "LDS scratch of the row pass");
-  rowpass2<NS, R, D, 5, 2, false, kPairChunk>(
+  rowpass2<NS, R, D, 5, 2, false, kPairChunk, true>(
       I.A, mc, n, I.ld, init ? S1.sol : xc,
       [&](int r) {
         int p, q;
@@ -992,7 +1080,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
         S1.aty[j] = init ? s1 : S0.aty[j] + s1;
         I.um[j] = s2;  // A'(W r1_y)
       },
-      scratch);
+      scratch, I.ext);
   if (I.has_p) {
     // P_off u with u = sol_x (init) or x_c, added to P_off sol0_x otherwise.
     struct PIn {
@@ -1464,7 +1552,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     double* axw = wave_uniform(ax);
     with_pass_shape(I.ld, [&](auto sh) {
       using S = decltype(sh);
-      rowpass2<S::NS, S::R, S::D, 1, 1>(
+      rowpass2<S::NS, S::R, S::D, 1, 1, false, kPassChunk, true>(
           I.A, I.mc, n, I.ld, I.x,
           [&](int r) {
             int p, q;
@@ -1486,7 +1574,8 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
             stg(axw + p, o[0]);
             if (q >= 0) stg(axw + q, -o[0]);
           },
-          [&](int j, double s1, double) { aty[j] = s1 / I.e[j]; }, reinterpret_cast<double*>(sm.u));
+          [&](int j, double s1, double) { aty[j] = s1 / I.e[j]; }, reinterpret_cast<double*>(sm.u),
+          I.ext);
     });
   }
   double* px = I.iacc;  // n
@@ -1595,7 +1684,7 @@ __device__ inline void ruiz(const Inst& I, double* scratch) {
   const double* ev = wave_uniform(I.e);
   double* um = wave_uniform(I.um);
   for (int pass = 0; pass < 10; ++pass) {
-    rowpass2<S::NS, S::R, S::D, 1, 1, true>(
+    rowpass2<S::NS, S::R, S::D, 1, 1, true, kPassChunk, true>(
         I.A, mc, n, ld, ev,
         [&](int r) {
           int p, q;
@@ -1608,7 +1697,7 @@ __device__ inline void ruiz(const Inst& I, double* scratch) {
           o[0] = fmin(fmax(1.0 / sqrt(r == 0.0 ? 1.0 : r), 1e-3), 1e3);
         },
         [&](int i, const double* o) { stg(um + i, o[0]); },
-        [&](int j, double v, double) { I.un[j] = v * I.e[j]; }, scratch);
+        [&](int j, double v, double) { I.un[j] = v * I.e[j]; }, scratch, I.ext);
     rowpass2<S::NS, S::R, S::D, 0, 1, true>(
         I.P, n, n, ld, ev, [&](int i) { return ldg(ev + i); },
         [](int, double, double ei, double* c1, double*, double*) { *c1 = ei; },
@@ -2047,6 +2136,25 @@ int find_row_pairs(const double* a, int m, int n, bool enable, PairMap* pm) {
   return r;
 }
 
+// The nonzero column extent [lo, hi) of each stored row (of row p(r) of the
+// m x n row-major a), packed as lo | hi << 16 (0 for a zero row), then
+// kExtPad zero words.  Without `enable` every stored row gets [0, n).
+void row_extents(const double* a, int n, int mc, const PairMap& pm, bool enable,
+                 std::vector<uint32_t>* out) {
+  out->assign((size_t)mc + kExtPad, 0u);
+  for (int r = 0; r < mc; ++r) {
+    const int k = (r >= pm.start[1]) + (r >= pm.start[2]) + (r >= pm.start[3]);
+    const double* row = a + (int64_t)(r + pm.dp[k]) * n;
+    int lo = 0, hi = n;
+    if (enable) {
+      while (lo < n && row[lo] == 0.0) ++lo;
+      while (hi > lo && row[hi - 1] == 0.0) --hi;
+      if (lo == hi) lo = hi = 0;
+    }
+    (*out)[r] = (uint32_t)lo | (uint32_t)hi << 16;
+  }
+}
+
 // Bump allocator over one device allocation.
 struct Arena {
   unsigned char* base = nullptr;
@@ -2066,6 +2174,7 @@ size_t inst_bytes(int n, int m, int mc) {
   b += align256(8 * mn);           // raw A
   b += align256(8 * (mcn + a_pad((int)ld)));  // stored rows of A (FP64) and read slack
   b += align256(4 * mcn);          // A32
+  b += align256(4 * ((int64_t)mc + kExtPad));  // row extents
   b += align256(8 * nn) + align256(8 * (nn + a_pad((int)ld)));  // raw P, P and its read slack
   b += align256(4 * nn);           // P32
   b += align256(8 * nn);           // G
@@ -2129,16 +2238,20 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
   if (hipStreamCreate(&h->stream) != hipSuccess) return cleanup(fail(QTQP_ERR_HIP, "stream"));
   if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
     return cleanup(fail(QTQP_ERR_HIP, "event"));
-  // row pairs of each QP, found on host threads
+  // row pairs and row extents of each QP, found on host threads
   std::vector<PairMap> pmap(count);
   std::vector<int> mcs(count);
+  std::vector<std::vector<uint32_t>> exts(count);
   {
     const bool pairs = (flags & QTQP_BATCH_NO_ROW_PAIRS) == 0;
+    const bool extents = (flags & QTQP_BATCH_NO_ROW_EXTENTS) == 0;
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::atomic<int> next{0};
     auto work = [&]() {
-      for (int q; (q = next.fetch_add(1)) < count;)
+      for (int q; (q = next.fetch_add(1)) < count;) {
         mcs[q] = find_row_pairs(a + a_off[q], m[q], n[q], pairs, &pmap[q]);
+        row_extents(a + a_off[q], n[q], mcs[q], pmap[q], extents, &exts[q]);
+      }
     };
     std::vector<std::thread> th;
     for (int t = 1; t < std::min(nt, count); ++t) th.emplace_back(work);
@@ -2146,6 +2259,10 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     for (auto& t : th) t.join();
   }
   h->mc = mcs;
+  for (int q = 0; q < count; ++q)
+    if ((int64_t)mcs[q] * ((n[q] + 3) & ~3) * 8 >= (int64_t)1 << 31)
+      return cleanup(fail(QTQP_ERR_ARG, "instance " + std::to_string(q) +
+                                            ": A exceeds 2 GiB (batched engine limit)"));
   size_t total = 0;
   for (int q = 0; q < count; ++q) total += inst_bytes(n[q], m[q], mcs[q]);
   void* mem = nullptr;
@@ -2181,6 +2298,8 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     I.Araw = araw;
     I.A = ar.take<double>(mld + a_pad(I.ld));
     I.A32 = ar.take<float>(mld);
+    uint32_t* ext = ar.take<uint32_t>((int64_t)I.mc + kExtPad);
+    I.ext = ext;
     double* praw = ar.take<double>(nn);
     I.Praw = praw;
     I.P = ar.take<double>(nld + a_pad(I.ld));
@@ -2220,7 +2339,9 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     if (hipMemcpyAsync(araw, a + a_off[q], 8 * mn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipMemcpyAsync(praw, p + p_off[q], 8 * nn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipMemcpyAsync(braw, b + m_off[q], 8 * mq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(craw, c + n_off[q], 8 * nq, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        hipMemcpyAsync(craw, c + n_off[q], 8 * nq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipMemcpyAsync(ext, exts[q].data(), 4 * exts[q].size(), hipMemcpyHostToDevice,
+                       h->stream) != hipSuccess)
       return cleanup(fail(QTQP_ERR_HIP, "upload failed"));
   }
   if (ar.used > total) return cleanup(fail(QTQP_ERR_HIP, "arena overflow"));
