@@ -71,6 +71,7 @@ struct Inst {
   int mc;       // stored rows of A (m when no row has a partner)
   PairMap pm;
   const uint32_t* ext;  // nonzero column extent lo | hi << 16 of each stored row
+  const uint32_t* cext; // the same for each 32-row chunk of stored rows (their union)
   int ld;    // leading dimension of A, A32, P, P32: n rounded up to 4
   int tile;  // Gram tile shape (QTQP_BATCH_TILE_*), chosen per size bucket
   // raw data
@@ -190,6 +191,24 @@ template <> struct VecT<float, 4> { typedef float type __attribute__((ext_vector
 template <> struct VecT<uint32_t, 2> { typedef uint32_t type __attribute__((ext_vector_type(2))); };
 template <> struct VecT<uint32_t, 4> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
 
+// Buffer resource over `bytes` bytes at p (raw, unswizzled).  A buffer access
+// at an offset past the end reads zero or writes nothing, without a memory
+// request.
+__device__ inline __amdgpu_buffer_rsrc_t buffer_of(const void* p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+}
+// Stores v at byte offset off of r when ok, else nothing: the store is
+// issued either way, so no branch forms around it (whose join would make
+// later waits drain every load in flight).
+template <typename T>
+__device__ inline void store_if(__amdgpu_buffer_rsrc_t r, bool ok, uint32_t off, T v) {
+  const uint32_t o = ok ? off : 0x80000000u;
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(VecT<uint32_t, 2>::type, v), r, o, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, o, 0, 0);
+}
+
 template <typename T> struct VW;
 template <> struct VW<double> { static constexpr int value = 2; };
 template <> struct VW<float> { static constexpr int value = 4; };
@@ -215,6 +234,11 @@ __device__ inline double dot(const double* a, const double* b, int len, double* 
 // two rounds ahead in registers, with unconditional (clamped) loads, as in
 // RankKTileG::run_vec; each tile accumulates exactly as a RankKTileG<T, 1, 1>
 // tile would.  lds: (1 + NJ) KCH x 65 elements of T.
+//  With cext (the nonzero column extent of each 32-row chunk of X, packed
+// lo | hi << 16, 0 for a zero chunk) only the rounds whose rows reach into
+// both the row strip and the tiles' column strips are read: their indices
+// are listed in `list` (LDS, `cap` ints) first.  The rounds left out add
+// only exact zeros, so every tile gets the sums it would get from all rows.
 template <typename T, int NJ, bool DIAG>
 struct GramRow {
   using acc_t = typename Mfma<T>::acc_t;
@@ -229,10 +253,58 @@ struct GramRow {
   acc_t acc[NJ][2][2];
 
   __device__ inline void run(const T* __restrict__ X, int ldx, int kend, int i0,
-                             const int (&j0)[NJ], const double* __restrict__ w, T* lds) {
+                             const int (&j0)[NJ], const double* __restrict__ w, T* lds,
+                             const uint32_t* __restrict__ cext = nullptr, int* list = nullptr,
+                             int cap = 0) {
     X = wave_uniform(X);
     w = wave_uniform(w);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // rounds to read: all (cnt = nround, list unused) or the listed ones
+    const int nround = (kend + KCH - 1) / KCH;
+    int cnt = nround;
+    const bool use_list = cext != nullptr && nround <= cap;
+    if (use_list) {
+      constexpr int SUB = KCH / 32;  // 32-row chunks per round
+      const int jlo = j0[0], jhi = j0[0] + NB * NJ;
+      if (wave == 0) {
+        int base = 0;
+        for (int c0 = 0; c0 < nround; c0 += 64) {
+          const int c = c0 + lane;
+          bool on = false;
+          if (c < nround) {
+#pragma unroll
+            for (int u = 0; u < SUB; ++u) {
+              const uint32_t e = ldg(cext + SUB * c + u);
+              const int lo = e & 0xffff, hi = e >> 16;
+              on = on || (lo < i0 + NB && hi > i0 && lo < jhi && hi > jlo);
+            }
+          }
+          const uint64_t mask = __ballot(on);
+          if (on) list[base + __popcll(mask & ((1ull << lane) - 1))] = c;
+          base += __popcll(mask);
+        }
+        if (lane == 0) list[cap] = base;
+      }
+      __syncthreads();
+      cnt = list[cap];
+      __syncthreads();
+      if (cnt == 0) {
+#pragma unroll
+        for (int t = 0; t < NJ; ++t)
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+              for (int r = 0; r < 4; ++r) acc[t][a][b][r] = T(0);
+        return;
+      }
+    }
+    // first row of the round at list position p (clamped to the last one)
+    auto round_row = [&](int p) {
+      p = min(p, cnt - 1);
+      return (use_list ? list[p] : p) * KCH;
+    };
     const int jb = 32 * (wave >> 1), ib = 32 * (wave & 1);
     const int vi = tid % VI, ri = tid / VI;
 #pragma unroll
@@ -257,7 +329,8 @@ struct GramRow {
       VT a[PI], b[NLD > 0 ? NLD : 1][PI];
       double w[PI];
     };
-    auto fetch = [&](int kc, Round& rd) {
+    auto fetch = [&](int pos, Round& rd) {
+      const int kc = round_row(pos);
 #pragma unroll
       for (int q = 0; q < PI; ++q) {
         const int k = min(kc + ri + RI * q, kend - 1);
@@ -270,7 +343,8 @@ struct GramRow {
     };
     T* si = lds;
     auto sj = [&](int t) { return lds + (1 + t) * KCH * (NB + 1); };
-    auto step = [&](int kc, Round& rd, bool active) {
+    auto step = [&](int pos, Round& rd, bool active) {
+      const int kc = round_row(pos);
       if (active) {
 #pragma unroll
         for (int q = 0; q < PI; ++q) {
@@ -292,7 +366,7 @@ struct GramRow {
         }
       }
       __syncthreads();
-      fetch(kc + DEPTH * KCH, rd);
+      fetch(pos + DEPTH, rd);
       if (active) {
 #pragma unroll
         for (int kk = 0; kk < KCH; kk += 4) {
@@ -319,12 +393,12 @@ struct GramRow {
     Round rd[DEPTH];
 #pragma unroll
     for (int d = 0; d < DEPTH; ++d) {
-      fetch(d * KCH, rd[d]);
+      fetch(d, rd[d]);
       __builtin_amdgcn_sched_barrier(0);
     }
-    for (int kc = 0; kc < kend; kc += DEPTH * KCH) {
+    for (int p0 = 0; p0 < cnt; p0 += DEPTH) {
 #pragma unroll
-      for (int d = 0; d < DEPTH; ++d) step(kc + d * KCH, rd[d], kc + d * KCH < kend);
+      for (int d = 0; d < DEPTH; ++d) step(p0 + d, rd[d], p0 + d < cnt);
     }
   }
 
@@ -353,14 +427,36 @@ __device__ __noinline__ void gram_pass(const Inst& I, const T* A, const T* P, T*
   int j0[NJ];
   j0[0] = j00;
   if (NJ > 1) j0[NJ - 1] = DIAG ? i0 : j00 + NB;
-  GramRow<T, NJ, DIAG> g;
-  g.run(A, ld, m, i0, j0, w, lds);
-  auto store = [&](int i, int j, T v) {
-    if (i < n && j < n && i >= j)
-      stg(G + (int64_t)j * n + i, v + ((i == j) ? (T)ldg(I.dg + i) : ldg(P + (int64_t)j * ld + i)));
-  };
+  using G_ = GramRow<T, NJ, DIAG>;
+  G_ g;
+  // the round list goes after the staging rows in the LDS union
+  constexpr int kUsed = (1 + NJ) * G_::KCH * (NB + 1) * (int)sizeof(T);
+  constexpr int kCap = (kLdsU * (int)sizeof(double) - kUsed) / (int)sizeof(int) - 1;
+  static_assert(kCap >= 256, "room for the round list");
+  g.run(A, ld, m, i0, j0, w, lds, I.cext,
+        in_lds(reinterpret_cast<int*>(reinterpret_cast<char*>(lds) + kUsed)), kCap);
+  // P_off and the diagonal d_x of every element, loaded before any is used
+  // (clamped addresses, unconditional loads: one memory latency in all)
+  T add[NJ][16];
 #pragma unroll
-  for (int t = 0; t < NJ; ++t) g.for_each(t, i0, j0[t], store);
+  for (int t = 0; t < NJ; ++t) {
+    int e = 0;
+    g.for_each(t, i0, j0[t], [&](int i, int j, T) {
+      const int ic = min(i, n - 1), jc = min(j, n - 1);
+      const T pv = ldg(P + (int64_t)jc * ld + ic);
+      const T dv = (T)ldg(I.dg + ic);
+      add[t][e++] = ic == jc ? dv : pv;
+    });
+  }
+  const __amdgpu_buffer_rsrc_t gr = buffer_of(G, n * n * (int)sizeof(T));
+#pragma unroll
+  for (int t = 0; t < NJ; ++t) {
+    int e = 0;
+    g.for_each(t, i0, j0[t], [&](int i, int j, T v) {
+      const T a = add[t][e++];
+      store_if(gr, i < n && j < n && i >= j, (uint32_t)((j * n + i) * (int)sizeof(T)), v + a);
+    });
+  }
 }
 
 // Lower tiles of the Gram block G = P_off + diag(d_x) + A' W A.  MT = 2:
@@ -481,8 +577,18 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
         RankKTile<T> acc;
         acc.zero();
         acc.run(G + (int64_t)k0 * n, n, 0, NB, i0, j0, n, n, [](int) { return T(1); }, si, sj);
+        // the tile's G values, loaded before any is updated (clamped,
+        // unconditional: one memory latency per tile)
+        T gv[16];
+        int e = 0;
+        acc.for_each(i0, j0, [&](int i, int j, T) {
+          gv[e++] = ldg(G + (int64_t)min(j, n - 1) * n + min(i, n - 1));
+        });
+        const __amdgpu_buffer_rsrc_t gr = buffer_of(G, n * n * (int)sizeof(T));
+        e = 0;
         acc.for_each(i0, j0, [&](int i, int j, T v) {
-          if (i < n && j < n && i >= j) G[(int64_t)j * n + i] -= v;
+          const T gij = gv[e++];
+          store_if(gr, i < n && j < n && i >= j, (uint32_t)((j * n + i) * (int)sizeof(T)), gij - v);
         });
       }
     }
@@ -2155,6 +2261,25 @@ void row_extents(const double* a, int n, int mc, const PairMap& pm, bool enable,
   }
 }
 
+// The union of the row extents of each 32-row chunk of the mc stored rows
+// (zero rows left out; 0 for a chunk of zero rows), then zero words up to
+// mc / 32 + 4 entries.
+std::vector<uint32_t> chunk_extents(const std::vector<uint32_t>& ext, int mc) {
+  std::vector<uint32_t> out((size_t)mc / 32 + 4, 0u);
+  for (int c = 0; c * 32 < mc; ++c) {
+    int lo = 1 << 16, hi = 0;
+    for (int r = 32 * c; r < std::min(mc, 32 * c + 32); ++r) {
+      const int l = ext[r] & 0xffff, h = ext[r] >> 16;
+      if (h > l) {
+        lo = std::min(lo, l);
+        hi = std::max(hi, h);
+      }
+    }
+    out[c] = hi > 0 ? ((uint32_t)lo | (uint32_t)hi << 16) : 0u;
+  }
+  return out;
+}
+
 // Bump allocator over one device allocation.
 struct Arena {
   unsigned char* base = nullptr;
@@ -2175,6 +2300,7 @@ size_t inst_bytes(int n, int m, int mc) {
   b += align256(8 * (mcn + a_pad((int)ld)));  // stored rows of A (FP64) and read slack
   b += align256(4 * mcn);          // A32
   b += align256(4 * ((int64_t)mc + kExtPad));  // row extents
+  b += align256(4 * ((int64_t)mc / 32 + 4));     // chunk extents
   b += align256(8 * nn) + align256(8 * (nn + a_pad((int)ld)));  // raw P, P and its read slack
   b += align256(4 * nn);           // P32
   b += align256(8 * nn);           // G
@@ -2300,6 +2426,9 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     I.A32 = ar.take<float>(mld);
     uint32_t* ext = ar.take<uint32_t>((int64_t)I.mc + kExtPad);
     I.ext = ext;
+    const std::vector<uint32_t> cx = chunk_extents(exts[q], I.mc);
+    uint32_t* cext = ar.take<uint32_t>((int64_t)cx.size());
+    I.cext = cext;
     double* praw = ar.take<double>(nn);
     I.Praw = praw;
     I.P = ar.take<double>(nld + a_pad(I.ld));
@@ -2341,7 +2470,9 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
         hipMemcpyAsync(braw, b + m_off[q], 8 * mq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipMemcpyAsync(craw, c + n_off[q], 8 * nq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipMemcpyAsync(ext, exts[q].data(), 4 * exts[q].size(), hipMemcpyHostToDevice,
-                       h->stream) != hipSuccess)
+                       h->stream) != hipSuccess ||
+        hipMemcpyAsync(cext, cx.data(), 4 * cx.size(), hipMemcpyHostToDevice, h->stream) !=
+            hipSuccess)
       return cleanup(fail(QTQP_ERR_HIP, "upload failed"));
   }
   if (ar.used > total) return cleanup(fail(QTQP_ERR_HIP, "arena overflow"));
