@@ -983,8 +983,9 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
     if constexpr (COLS > 0) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double k1 = __shfl(c1, r << (6 - LR), 64);
-        const double k2 = COLS > 1 ? __shfl(c2, r << (6 - LR), 64) : 0.0;
+        // row r's coefficients, from the first lane of its group
+        const double k1 = lane_value(c1, r << (6 - LR));
+        const double k2 = COLS > 1 ? lane_value(c2, r << (6 - LR)) : 0.0;
 #pragma unroll
         for (int s = 0; s < NS; ++s)
 #pragma unroll

@@ -96,24 +96,67 @@ __device__ inline void block_reduce_n(double (&v)[K], double* red) {
                                  : red[k] + red[K + k] + red[2 * K + k] + red[3 * K + k];
 }
 
+// The value v of the lane paired with this one across bit BIT of the lane
+// index, moved without LDS: lane ^ 32 and lane ^ 16 by the gfx950 lane
+// swaps (v_permlane32_swap / v_permlane16_swap: of the pair they return, the
+// first holds the lower half's values in both halves, the second the upper
+// half's), lane ^ 8 by DPP row_ror:8, lane ^ 2 and lane ^ 1 by DPP quad
+// permutes, and for BIT = 4 lane 7 - (lane & 7) of the same 8-lane group
+// (DPP row_half_mirror; it also crosses bit 2, which is what a pairwise
+// exchange across bit 2 needs).
+template <int BIT>
+__device__ inline uint32_t lane_partner32(uint32_t v) {
+  const int lane = __lane_id();
+  if constexpr (BIT == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return lane < 32 ? r[1] : r[0];
+  } else if constexpr (BIT == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? r[0] : r[1];
+  } else {
+    constexpr int ctrl = BIT == 8 ? 0x128 : BIT == 4 ? 0x141 : BIT == 2 ? 0x4E : 0xB1;
+    static_assert(BIT == 8 || BIT == 4 || BIT == 2 || BIT == 1, "lane bit");
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, ctrl, 0xf, 0xf, false);
+  }
+}
+template <int BIT>
+__device__ inline double lane_partner(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = lane_partner32<BIT>((uint32_t)u), hi = lane_partner32<BIT>((uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
+}
+
+// v of lane `src` (a constant) in every lane, through scalar registers.
+__device__ inline double lane_value(double v, int src) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), src);
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
+}
+
 // Completes R per-lane partial sums (or maxima, MAX) across the wavefront
 // (R a power of two up to 32): a transpose-reduce that halves the rows a lane carries
 // at each of log2(R) exchanges, then combines the rest.  Lane l ends with
-// the result of row l >> (6 - log2 R) in d[0].
+// the result of row l >> (6 - log2 R) in d[0].  The exchanges use
+// lane_partner (no LDS round trips); the partner across bit 4 is the
+// mirror lane, which pairs lanes with opposite bit 4 as the exchange needs.
 // The rows live in a vector value while they are exchanged: on a plain
 // array, the compiler may merge the lane-dependent choice between two
 // elements into one load at a lane-dependent index, which sends the array
 // to scratch memory.
+template <int BIT>
+__device__ inline double partner_of(double v) {
+  return lane_partner<BIT>(v);
+}
 template <int R, bool MAX = false>
 __device__ inline void reduce_rows(double* d) {
   typedef double Rows __attribute__((ext_vector_type(R < 2 ? 2 : R)));
-  const int lane = threadIdx.x & 63;
+  const int lane = __lane_id();
   Rows v;
 #pragma unroll
   for (int q = 0; q < R; ++q) v[q] = d[q];
-  int cnt = R;
-#pragma unroll
-  for (int bit = 32; bit >= 64 / R; bit >>= 1) {
+  auto exchange = [&](auto bitc, int cnt) {
+    constexpr int bit = decltype(bitc)::value;
     const int half = cnt / 2;
     const bool hi = lane & bit;
 #pragma unroll
@@ -122,18 +165,28 @@ __device__ inline void reduce_rows(double* d) {
         const double lo_v = v[q], hi_v = v[q + half];
         const double send = hi ? lo_v : hi_v;
         const double keep = hi ? hi_v : lo_v;
-        const double got = __shfl_xor(send, bit, 64);
+        const double got = partner_of<bit>(send);
         v[q] = MAX ? fmax(keep, got) : keep + got;
       }
     }
-    cnt = half;
-  }
+  };
+  // exchanges across bits 32, 16, ... 64 / R, each halving the rows
+  if constexpr (R >= 2) exchange(std::integral_constant<int, 32>(), R);
+  if constexpr (R >= 4) exchange(std::integral_constant<int, 16>(), R / 2);
+  if constexpr (R >= 8) exchange(std::integral_constant<int, 8>(), R / 4);
+  if constexpr (R >= 16) exchange(std::integral_constant<int, 4>(), R / 8);
+  if constexpr (R >= 32) exchange(std::integral_constant<int, 2>(), R / 16);
   double r0 = v[0];
-#pragma unroll
-  for (int bit = 32 / R; bit >= 1; bit >>= 1) {
-    const double got = __shfl_xor(r0, bit, 64);
+  auto combine = [&](auto bitc) {
+    const double got = partner_of<decltype(bitc)::value>(r0);
     r0 = MAX ? fmax(r0, got) : r0 + got;
-  }
+  };
+  // the rest across bits 32 / R ... 1 (all lanes of a row's group end equal)
+  if constexpr (32 / R >= 16) combine(std::integral_constant<int, 16>());
+  if constexpr (32 / R >= 8) combine(std::integral_constant<int, 8>());
+  if constexpr (32 / R >= 4) combine(std::integral_constant<int, 4>());
+  if constexpr (32 / R >= 2) combine(std::integral_constant<int, 2>());
+  if constexpr (32 / R >= 1) combine(std::integral_constant<int, 1>());
   d[0] = r0;
 }
 
