@@ -127,6 +127,9 @@ __device__ inline double lane_partner(double v) {
 }
 
 // v of lane `src` (a constant) in every lane, through scalar registers.
+__device__ inline float lane_value(float v, int src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(uint32_t, v), src));
+}
 __device__ inline double lane_value(double v, int src) {
   const uint64_t u = __builtin_bit_cast(uint64_t, v);
   const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, src);
@@ -527,7 +530,7 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
   using acc_t = typename Mfma<T>::acc_t;
   const int tid = threadIdx.x, lane = tid & 63;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int r = lane & 15, cg = lane >> 4, kq = lane >> 4;
+  const int r = lane & 15, kq = lane >> 4;
   __shared__ int first_bad;
   if (tid == 0) first_bad = NB;
   // X above its diagonal 16 x 16 blocks is zero (the diagonal blocks are
@@ -538,17 +541,18 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
   }
   for (int pb = 0; pb < 4; ++pb) {
     const int c0 = 16 * pb;
-    T d[4], x[4];
+    // lane r (mod 16) holds row r of D and of X = L_d^{-1}; the values of
+    // other rows come from constant lanes by v_readlane (no LDS)
+    T d[16], x[16];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int k = 4 * cg + t;
-      d[t] = k <= r ? S[c0 + r][c0 + k] : T(0);
-      x[t] = T(k == r);
+    for (int k = 0; k < 16; ++k) {
+      d[k] = k <= r ? S[c0 + r][c0 + k] : T(0);
+      x[k] = T(k == r);
     }
+    T drr = T(1);  // L[r][r]
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      const int src = 16 * (j >> 2);  // lanes holding column j (slot j & 3)
-      T piv = __shfl(d[j & 3], j + src, 64);
+      T piv = lane_value(d[j], j);
       const bool ok = piv > T(0);
       if (!ok) {
         piv = T(1);
@@ -556,37 +560,34 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
       }
       const T dj = sqrt(piv);
       const T rj = T(1) / dj;
-      if (cg == (j >> 2)) d[j & 3] = r > j ? d[j & 3] * rj : (r == j ? dj : T(0));
-      const T lrj = __shfl(d[j & 3], r + src, 64);  // L[r][j]
+      const T lrj = r > j ? d[j] * rj : (r == j ? dj : T(0));  // L[r][j]
+      // rows below j lose L[r][j] L[k][j] (entries above row r's diagonal
+      // are never read back)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int k = 4 * cg + t;
-        const T lkj = __shfl(d[j & 3], k + src, 64);  // L[k][j]
-        if (k > j && k <= r) d[t] -= lrj * lkj;
-      }
+      for (int k = j + 1; k < 16; ++k) d[k] -= lrj * (lane_value(d[j], k) * rj);
+      d[j] = lrj;
+      if (r == j) drr = dj;
       // L^{-1} by forward elimination on the identity: finish row j, then
-      // remove it from the rows below
+      // remove it from the rows below (row j of X is zero past column j)
       if (r == j) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) x[t] *= rj;
+        for (int c = 0; c <= j; ++c) x[c] *= rj;
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const T xjt = __shfl(x[t], j + 16 * cg, 64);  // X[j][4cg+t]
-        if (r > j) x[t] -= lrj * xjt;
+      for (int c = 0; c <= j; ++c) {
+        const T xjc = lane_value(x[c], j);
+        if (r > j) x[c] -= lrj * xjc;
       }
     }
-    if (w == 0) {
+    if (w == 0 && lane < 16) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        X[c0 + r][c0 + 4 * cg + t] = x[t];
-        if (4 * cg + t == r) sinv[c0 + r] = T(1) / d[t];
-      }
+      for (int k = 0; k < 16; ++k) X[c0 + r][c0 + k] = x[k];
+      sinv[c0 + r] = T(1) / drr;
     }
     __syncthreads();  // every wave has read D; L_d^{-1} is in X
-    if (w == 0) {
+    if (w == 0 && lane < 16) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) S[c0 + r][c0 + 4 * cg + t] = d[t];
+      for (int k = 0; k < 16; ++k) S[c0 + r][c0 + k] = k <= r ? d[k] : T(0);
     }
     const int nstrip = 3 - pb;
     if (w < nstrip) {
