@@ -85,6 +85,7 @@ struct Inst {
   double* LiB;  // per 64-block of the factor: L_bb^{-1} column-major then row-major
   double *x, *y, *s, *kq, *dg, *corr, *krhs, *sol, *res, *iacc, *ires, *r, *xp;
   double *um, *un, *ds, *cm;
+  double* pkq;  // P kq_x of this iteration's kinv_q
   double* irb;  // refinement state, two copies: A x (m), A'y, P_off x, g (n each)
   double* sd;
   int* si;
@@ -1386,14 +1387,12 @@ __device__ inline void pmul(const Inst& I, const double* u, double* out, Smem& s
 }
 
 // Quadratic tau (__init__.py:759-817); returns false where it raises.
+// P kq_x is the iteration's pkq.
 __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, double mu, double mu_t,
                               double r_tau, double* tau_out, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m;
-  if (I.has_p) {
-    pmul(I, kr, I.un, sm);   // P kr_x
-    pmul(I, I.kq, I.um, sm); // P kq_x (um has >= n slots: m >= 1 ... sized max(n,m))
-  }
+  if (I.has_p) pmul(I, kr, I.un, sm);  // P kr_x
   // the dot products of the tau quadratic and max|kr|, reduced together:
   // slots kq.c, kq_y.b, kr.c, kr_y.b, kq.Pkq, kr.Pkq, kq.Pkr, kr.Pkr, max|kr|
   double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1408,8 +1407,8 @@ __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, doub
   }
   if (I.has_p) {
     for (int j = threadIdx.x; j < n; j += WG) {
-      v[4] += I.kq[j] * I.um[j];
-      v[5] += kr[j] * I.um[j];
+      v[4] += I.kq[j] * I.pkq[j];
+      v[5] += kr[j] * I.pkq[j];
       v[6] += I.kq[j] * I.un[j];
       v[7] += kr[j] * I.un[j];
     }
@@ -1569,6 +1568,8 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     }
     for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
     __syncthreads();
+    // P kq_x, which both tau quadratics of the iteration use
+    if (I.has_p) pmul(I, I.kq, I.pkq, sm);
   }
 
   // predictor (mu_target = 0) anchored at [x; y]
@@ -2311,6 +2312,7 @@ size_t inst_bytes(int n, int m, int mc) {
   b += 3 * align256(8 * std::max<int64_t>(m, n));  // x/y/s slack (x n, y m, s m)
   b += 10 * align256(8 * N);                       // kq dg corr krhs sol res iacc ires r xp
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
+  b += align256(8 * n);                                // pkq
   b += align256(8 * (2 * (int64_t)m + 6 * (int64_t)n));  // refinement state
   b += align256(8 * S_NSCAL) + align256(4 * I_NINT);
   b += align256(8 * kTraceFields * kTraceRows);
@@ -2461,6 +2463,7 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     I.un = ar.take<double>(mx);
     I.ds = ar.take<double>(mx);
     I.cm = ar.take<double>(mx);
+    I.pkq = ar.take<double>(nq);
     I.irb = ar.take<double>(2 * (int64_t)mq + 6 * (int64_t)nq);
     I.sd = ar.take<double>(S_NSCAL);
     I.si = ar.take<int>(I_NINT);
