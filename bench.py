@@ -108,7 +108,11 @@ def _cpu_model():
   return "unknown"
 
 
-def algorithmic_work(n, m, counts, iters):
+VEC_PASSES_PER_IR = 4
+VEC_PASSES_PER_ITER = 25
+
+
+def algorithmic_work(n, m, counts, iters, a_bytes=None, m_full=None):
   """Algorithmic FLOPs and HBM bytes of the executed work of every QP.
 
   Per QP, from the engine's own counters (m is the rows of A the engine
@@ -119,16 +123,27 @@ def algorithmic_work(n, m, counts, iters):
   and the next right-hand side, fused) reads A and P once in FP64; each
   factor solve reads the triangular factor twice (n^2/2 entries each way)
   in the factor's precision; each IPM iteration's termination test reads A
-  and P once in FP64.
+  and P once in FP64.  A read of A costs a_bytes (FP64; half that in FP32)
+  when given -- the bytes inside the stored rows' nonzero column extents,
+  which is what the engine's passes read -- else 8 m n.  Pointwise work is
+  counted in vector passes of 8 (n + 2 m_full) bytes (m_full: all rows):
+  four per refinement pass (its row inputs and outputs, P-pass vectors and
+  the closing vector update) and VEC_PASSES_PER_ITER per IPM iteration (the
+  iterate's normalisation, regularised diagonal, right-hand sides, step
+  lengths, update and termination vectors).
   """
   f32, f64, kmv, fs32, fs64 = (counts[:, k].astype(np.float64) for k in range(5))
   it = iters.astype(np.float64)
   mn, nn = m.astype(np.float64) * n, n.astype(np.float64) ** 2
+  vec = 8.0 * (n.astype(np.float64) + 2.0 * m_full) if m_full is not None else 0.0
+  if a_bytes is not None:
+    mn = np.asarray(a_bytes, dtype=np.float64) / 8.0
   fac_flops = m.astype(np.float64) * nn + nn * n / 3.0
   flops32 = float(np.sum(f32 * fac_flops))
   flops64 = float(np.sum(f64 * fac_flops))
   bytes_ = np.sum(f32 * 4 * mn + f64 * 8 * mn + (kmv + it) * (8 * mn + 8 * nn)
-                  + fs32 * 4 * nn + fs64 * 8 * nn)
+                  + fs32 * 4 * nn + fs64 * 8 * nn
+                  + (kmv * VEC_PASSES_PER_IR + it * VEC_PASSES_PER_ITER) * vec)
   return flops32, flops64, float(bytes_)
 
 
@@ -621,7 +636,9 @@ def main():
 
   # roofline of the dominant kernel (the per-iteration IPM kernel)
   mc = bq.stored_rows()
-  flops32, flops64, bytes_ = algorithmic_work(n, mc, work, iters)
+  a_bytes = bq.row_bytes()
+  flops32, flops64, bytes_ = algorithmic_work(n, mc, work, iters, a_bytes, m)
+  _, _, bytes_dense = algorithmic_work(n, mc, work, iters, None, m)
   iter_ms = float(np.sum(launch_ms))
   t_mem = bytes_ / (HBM_PEAK_GBS * 1e9) * 1e3
   t_mfma = flops32 / (FP32_PEAK_TFLOPS * 1e12) * 1e3 + flops64 / (FP64_PEAK_TFLOPS * 1e12) * 1e3
@@ -639,6 +656,8 @@ def main():
                "algorithmic_bytes_per_launch": bytes_ / nl,
                "algorithmic_flops_per_launch": (flops32 + flops64) / nl,
                "stored_rows_of_a": float(mc.sum() / m.sum()),
+               "a_read_bytes_over_dense_stored_rows": float(a_bytes.sum() / (8.0 * (mc * n).sum())),
+               "algorithmic_bytes_per_launch_dense_stored_rows": bytes_dense / nl,
                "combined_frac": (t_mem + t_mfma) / iter_ms})
 
   # plain FP64 GPU paths, one QP per call, FP64 factor/solves, no batching/graph:

@@ -95,6 +95,11 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
  * order.  No reference counterpart (batch diagnostics). */
 int qtqp_batch_stored_rows(qtqp_batch* h, int32_t* mc);
 
+/* FP64 bytes one pass over each QP's stored rows reads: the 16-byte pieces
+ * inside each row's nonzero column extent, in creation order.  No reference
+ * counterpart (batch diagnostics: the executed-byte count of a pass). */
+int qtqp_batch_row_bytes(qtqp_batch* h, int64_t* bytes);
+
 /* Sets the starting point of the next solve for every QP: x, y, s in the
  * unscaled coordinates (equilibrated on the device like _init_variables,
  * __init__.py:299-307 and 668-669) and tau.  Without this call the default

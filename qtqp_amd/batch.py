@@ -48,6 +48,7 @@ BATCH_SYMBOLS = {
                                             _i64p, _i64p, _dp, _dp, _dp, _dp, ctypes.c_int32,
                                             ctypes.POINTER(ctypes.c_void_p)]),
     "qtqp_batch_stored_rows": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
+    "qtqp_batch_row_bytes": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
     "qtqp_batch_set_start": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
     "qtqp_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts),
                                         ctypes.POINTER(BatchCounters)]),
@@ -233,6 +234,12 @@ class BatchQP:
         | (0 if row_extents else self.NO_ROW_EXTENTS), ctypes.byref(h))
     _check(self._lib, rc)
     self._h = h
+
+  def row_bytes(self) -> np.ndarray:
+    """FP64 bytes one pass over each QP's stored row extents reads."""
+    out = np.zeros(self.count, dtype=np.int64)
+    _check(self._lib, self._lib.qtqp_batch_row_bytes(self._h, _ptr(out, _i64p)))
+    return out
 
   def stored_rows(self) -> np.ndarray:
     """Rows of A each QP keeps on the device (m less its row pairs)."""

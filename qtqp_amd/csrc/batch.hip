@@ -2102,6 +2102,7 @@ struct qtqp_batch {
   std::vector<int64_t> n_off, m_off;
   std::vector<int> n, m;
   std::vector<int> mc;  // stored rows of A per QP (row pairs kept once)
+  std::vector<int64_t> row_bytes;  // FP64 bytes a pass over the stored rows' extents reads
   int64_t tot_n = 0, tot_m = 0;
   int64_t *d_noff = nullptr, *d_moff = nullptr;
   double *d_px = nullptr, *d_py = nullptr, *d_ps = nullptr, *d_scal = nullptr;
@@ -2388,6 +2389,12 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     for (auto& t : th) t.join();
   }
   h->mc = mcs;
+  h->row_bytes.assign(count, 0);
+  for (int q = 0; q < count; ++q)
+    for (int r = 0; r < mcs[q]; ++r) {
+      const int lo = exts[q][r] & 0xffff, hi = exts[q][r] >> 16;
+      if (hi > lo) h->row_bytes[q] += 16 * ((hi + 1) / 2 - lo / 2);
+    }
   for (int q = 0; q < count; ++q)
     if ((int64_t)mcs[q] * ((n[q] + 3) & ~3) * 8 >= (int64_t)1 << 31)
       return cleanup(fail(QTQP_ERR_ARG, "instance " + std::to_string(q) +
@@ -2798,6 +2805,12 @@ int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows) {
   if (!h || !out || q < 0 || q >= h->count || rows < 0) return fail(QTQP_ERR_ARG, "bad argument");
   const int r = std::min<int>(rows, kTraceRows);
   HIPC(hipMemcpy(out, h->hinst[q].trace, sizeof(double) * kTraceFields * r, hipMemcpyDeviceToHost));
+  return QTQP_OK;
+}
+
+int qtqp_batch_row_bytes(qtqp_batch* h, int64_t* bytes) {
+  if (!h || !bytes) return fail(QTQP_ERR_ARG, "NULL argument");
+  for (int q = 0; q < h->count; ++q) bytes[q] = h->row_bytes[q];
   return QTQP_OK;
 }
 
