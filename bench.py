@@ -507,6 +507,8 @@ def main():
   ap.add_argument("--sparse-no-qtqp", action="store_true",
                   help="sparse leg without its whole QTQP.solve (for counter runs)")
   ap.add_argument("--oracle-sample", type=int, default=64)
+  ap.add_argument("--profile-qp", choices=("dense4000", "mpc64"), default="dense4000",
+                  help="the QP --profile-single solves")
   ap.add_argument("--profile-single", choices=("mixed", "fp64"),
                   help="only solve the n=4000 m=6000 z=1000 dense QP once through QTQP.solve "
                        "(for kernel profiles of the device-resident loop)")
@@ -522,16 +524,21 @@ def main():
     import scipy.sparse as sp
     import qtqp_amd
     from qtqp_amd import problems
-    qp = problems.random_dense_qp(4000, 6000, 1000, seed=0)
+    if args.profile_qp == "mpc64":
+      qp, label = problems.mpc_qp(64, 12, 4, seed=0), "condensed MPC T=64 nx=12 nu=4 (seed 0)"
+    else:
+      qp, label = problems.random_dense_qp(4000, 6000, 1000, seed=0), \
+          "n=4000 m=6000 z=1000 dense QP (seed 0)"
     ls = {"mixed": qtqp_amd.LinearSolver.HIP_DENSE,
           "fp64": qtqp_amd.LinearSolver.HIP_DENSE_FP64}[args.profile_single]
     q = qtqp_amd.QTQP(a=sp.csc_matrix(qp.a), b=qp.b, c=qp.c, z=qp.z, p=sp.csc_matrix(qp.p))
-    t0 = time.perf_counter()
-    sol = q.solve(linear_solver=ls, verbose=False, collect_stats=True)
-    print(json.dumps({"workload": "n=4000 m=6000 z=1000 dense QP (seed 0)",
-                      "linear_solver": ls.name, "status": sol.status.value,
-                      "iterations": len(sol.stats), "solve_s": time.perf_counter() - t0}),
-          flush=True)
+    times = []
+    for _ in range(2):  # the first call also loads code objects and captures graphs
+      t0 = time.perf_counter()
+      sol = q.solve(linear_solver=ls, verbose=False, collect_stats=True)
+      times.append(time.perf_counter() - t0)
+    print(json.dumps({"workload": label, "linear_solver": ls.name, "status": sol.status.value,
+                      "iterations": len(sol.stats), "solve_s": times}), flush=True)
     return
 
   if args.sparse_sweep:
