@@ -40,7 +40,9 @@ _inf_norm = lambda v: float(np.linalg.norm(v, np.inf)) if np.size(v) else 0.0
 
 
 class LinearSolver(enum.Enum):
-  """KKT backends.  AUTO picks the first that initialises on this machine."""
+  """KKT backends.  AUTO picks by problem: the sparse backend for large sparse
+  problems and for sizes past the dense engine's limits, the dense mixed-precision
+  backend otherwise; the first of its candidates that initialises on this machine."""
 
   AUTO = "auto"
   HIP_DENSE = direct.HipDenseSolver
@@ -48,8 +50,15 @@ class LinearSolver(enum.Enum):
   HIP_SPARSE = direct.HipSparseSolver
 
 
-_AUTO_SOLVER_CACHE: dict[str, LinearSolver] = {}
+# (platform, backend name) -> why the backend could not initialise here; only
+# availability is cached, the choice itself depends on each problem
+_AUTO_SOLVER_CACHE: dict[tuple, str] = {}
 _AUTO_UNAVAILABLE_ERRORS = (ImportError, OSError)
+
+_DENSE_N_LIMIT = 46340          # largest n of the dense KKT engine (n*n entries of G in int32)
+_DENSE_BYTES_LIMIT = 4 << 30    # A, P and G in FP64 past this go to the sparse backend
+_SPARSE_MIN_CELLS = 1 << 22     # below this many cells of A and P the dense path is kept
+_SPARSE_MAX_FILL = 0.05         # stored fraction of A and P below which a larger problem is sparse
 
 
 def _instantiate_linear_solver(linear_solver: LinearSolver) -> direct.LinearSolver:
@@ -58,8 +67,20 @@ def _instantiate_linear_solver(linear_solver: LinearSolver) -> direct.LinearSolv
   return linear_solver.value()
 
 
-def _auto_linear_solver_order() -> list[LinearSolver]:
-  return [LinearSolver.HIP_DENSE, LinearSolver.HIP_DENSE_FP64, LinearSolver.HIP_SPARSE]
+def _auto_linear_solver_order(m: int = 0, n: int = 0, nnz: int = 0) -> list[LinearSolver]:
+  """AUTO's candidates, in order, for A of m x n with nnz stored entries in A and P.
+
+  The sparse backend comes first when n is past the dense engine's limit, when the
+  dense A, P and G would take more than _DENSE_BYTES_LIMIT, or when a problem of
+  more than _SPARSE_MIN_CELLS cells stores less than _SPARSE_MAX_FILL of them (the
+  reference's AUTO is sparse-first; the dense MFMA path is faster for the small and
+  dense QPs it keeps)."""
+  cells = m * n + n * n
+  dense_bytes = 8 * (m * n + 2 * n * n)
+  sparse_first = (n > _DENSE_N_LIMIT or dense_bytes > _DENSE_BYTES_LIMIT
+                  or (cells > _SPARSE_MIN_CELLS and nnz < _SPARSE_MAX_FILL * cells))
+  dense = [LinearSolver.HIP_DENSE, LinearSolver.HIP_DENSE_FP64]
+  return [LinearSolver.HIP_SPARSE] + dense if sparse_first else dense + [LinearSolver.HIP_SPARSE]
 
 
 class _ProductMat:
@@ -87,22 +108,24 @@ def _segment_max(vals: np.ndarray, indptr: np.ndarray) -> np.ndarray:
   return out
 
 
-def _resolve_linear_solver(linear_solver: LinearSolver):
-  """(enum, backend instance); mirrors __init__.py:107-126."""
+def _resolve_linear_solver(linear_solver: LinearSolver, shape: tuple = (0, 0, 0)):
+  """(enum, backend instance); mirrors __init__.py:107-126.  shape = (m, n, nnz of
+  A and P) of the problem AUTO chooses for."""
   if linear_solver is not LinearSolver.AUTO:
     return linear_solver, _instantiate_linear_solver(linear_solver)
-  cached = _AUTO_SOLVER_CACHE.get(sys.platform)
-  if cached is not None:
-    return cached, _instantiate_linear_solver(cached)
   errors = []
-  for cand in _auto_linear_solver_order():
+  for cand in _auto_linear_solver_order(*shape):
+    known = _AUTO_SOLVER_CACHE.get((sys.platform, cand.name))
+    if known is not None:
+      errors.append(f"{cand.name}: {known}")
+      continue
     try:
       backend = _instantiate_linear_solver(cand)
     except _AUTO_UNAVAILABLE_ERRORS as e:
+      _AUTO_SOLVER_CACHE[(sys.platform, cand.name)] = str(e)
       errors.append(f"{cand.name}: {e}")
       logging.debug("AUTO skipped %s: %s", cand.name, e)
       continue
-    _AUTO_SOLVER_CACHE[sys.platform] = cand
     return cand, backend
   raise RuntimeError("AUTO could not initialize any linear solver backend: " + "; ".join(errors))
 
@@ -249,7 +272,8 @@ class QTQP:
     assert max_iterative_refinement_steps >= 1
     assert linear_solver_atol >= 0 and linear_solver_rtol >= 0
 
-    resolved, backend = _resolve_linear_solver(linear_solver)
+    resolved, backend = _resolve_linear_solver(
+        linear_solver, (self.m, self.n, int(self.a.nnz) + int(self.p.nnz)))
     self.start_time = timeit.default_timer()
     self.atol, self.rtol = atol, rtol
     self.atol_infeas, self.rtol_infeas = atol_infeas, rtol_infeas

@@ -48,7 +48,7 @@ class OracleBackend(direct.LinearSolver):
 @pytest.fixture
 def oracle_backend(monkeypatch):
   monkeypatch.setattr(qtqp_amd, "_resolve_linear_solver",
-                      lambda ls: (qtqp_amd.LinearSolver.HIP_DENSE, OracleBackend()))
+                      lambda ls, *shape: (qtqp_amd.LinearSolver.HIP_DENSE, OracleBackend()))
 
 
 def _solve(a, b, c, z, p, **kw):
@@ -174,7 +174,7 @@ def test_backend_freed_on_exception(monkeypatch):
       self.freed = True
 
   be = Failing()
-  monkeypatch.setattr(qtqp_amd, "_resolve_linear_solver", lambda ls: (ls, be))
+  monkeypatch.setattr(qtqp_amd, "_resolve_linear_solver", lambda ls, *shape: (ls, be))
   a, b, c, p = G.gen_feasible(20, 10, 3, 842)
   solver = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=3, p=sp.csc_matrix(p))
   with pytest.raises(RuntimeError, match="forced factorization failure"):
@@ -251,3 +251,43 @@ def test_auto_reports_unavailable_backends(monkeypatch):
   monkeypatch.setattr(qtqp_amd, "_instantiate_linear_solver", boom)
   with pytest.raises(RuntimeError, match="AUTO could not initialize"):
     qtqp_amd._resolve_linear_solver(qtqp_amd.LinearSolver.AUTO)  # pylint: disable=protected-access
+
+
+@pytest.mark.parametrize("m, n, nnz, first", [
+    (300, 200, 300 * 200 + 200 * 200, "HIP_DENSE"),         # small dense QP (cfg1)
+    (2048, 256, 2048 * 256 // 2, "HIP_DENSE"),              # condensed MPC (cfg3)
+    (6000, 4000, 6000 * 4000 + 4000 * 4000, "HIP_DENSE"),   # large dense QP (cfg2)
+    (6000, 4000, 60_000, "HIP_SPARSE"),                     # reference's sparse family
+    (60_000, 50_000, 200_000, "HIP_SPARSE"),                # past the dense n limit
+    (40_000, 20_000, 40_000 * 20_000, "HIP_SPARSE"),        # dense, but over 4 GiB
+    (1000, 500, 2000, "HIP_DENSE"),                         # sparse but small
+])
+def test_auto_chooses_by_problem(monkeypatch, m, n, nnz, first):
+  """AUTO's choice depends on each problem, and a backend that failed to
+  initialise is skipped without fixing the choice for later problems."""
+  monkeypatch.setattr(qtqp_amd, "_AUTO_SOLVER_CACHE", {})
+  made = []
+
+  def make(ls):
+    made.append(ls.name)
+    return object()
+  monkeypatch.setattr(qtqp_amd, "_instantiate_linear_solver", make)
+  order = qtqp_amd._auto_linear_solver_order(m, n, nnz)  # pylint: disable=protected-access
+  assert order[0].name == first and len(set(order)) == 3
+  ls, _ = qtqp_amd._resolve_linear_solver(qtqp_amd.LinearSolver.AUTO, (m, n, nnz))  # pylint: disable=protected-access
+  assert ls.name == first and made == [first]
+
+
+def test_auto_skips_unavailable_backends_per_problem(monkeypatch):
+  monkeypatch.setattr(qtqp_amd, "_AUTO_SOLVER_CACHE", {})
+
+  def make(ls):
+    if ls is qtqp_amd.LinearSolver.HIP_SPARSE:
+      raise OSError("no sparse library")
+    return object()
+  monkeypatch.setattr(qtqp_amd, "_instantiate_linear_solver", make)
+  sparse_shape = (6000, 4000, 60_000)
+  ls, _ = qtqp_amd._resolve_linear_solver(qtqp_amd.LinearSolver.AUTO, sparse_shape)  # pylint: disable=protected-access
+  assert ls is qtqp_amd.LinearSolver.HIP_DENSE
+  ls, _ = qtqp_amd._resolve_linear_solver(qtqp_amd.LinearSolver.AUTO, (300, 200, 100_000))  # pylint: disable=protected-access
+  assert ls is qtqp_amd.LinearSolver.HIP_DENSE
