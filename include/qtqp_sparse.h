@@ -56,6 +56,16 @@ typedef struct qtqp_sparse_info {
 int qtqp_sparse_analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind,
                         int64_t* perm_out, qtqp_sparse_info* info);
 
+/* qtqp_sparse_analyze with a choice of fill-reducing ordering: 0 (what
+ * qtqp_sparse_analyze and create use) exact minimum degree up to N = 40000 and
+ * approximate minimum degree on a quotient graph above; 1 exact minimum degree
+ * (approximate above N = 40000, where the explicit elimination graph would not
+ * fit); 2 approximate minimum degree at every size.  A factor whose panels
+ * would exceed 64 GiB is refused with QTQP_ERR_ARG.  No reference counterpart
+ * (the reference's sparse backends choose their orderings internally). */
+int qtqp_sparse_analyze_ex(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind,
+                           int32_t ordering, int64_t* perm_out, qtqp_sparse_info* info);
+
 /* Creates a handle for the N x N KKT matrix given by its upper triangle
  * (ptr[N+1], ind[nnz], val[nnz] in `format`).  Runs the symbolic analysis
  * and keeps the pattern, the off-diagonal values and the stored diagonal on

@@ -123,6 +123,9 @@ SPARSE_CSR = 1
 SPARSE_SYMBOLS = {
     "qtqp_sparse_analyze": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_int64_p, _c_int64_p,
                                            _c_int64_p, ctypes.POINTER(SparseInfo)]),
+    "qtqp_sparse_analyze_ex": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_int64_p,
+                                              _c_int64_p, ctypes.c_int32, _c_int64_p,
+                                              ctypes.POINTER(SparseInfo)]),
     "qtqp_sparse_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _c_int64_p, _c_int64_p,
                                           _c_double_p, ctypes.POINTER(ctypes.c_void_p)]),
     "qtqp_sparse_update_diag": (ctypes.c_int, [ctypes.c_void_p, _c_double_p]),

@@ -60,7 +60,8 @@ constexpr int kTile = 64;         // Schur-update tile edge
 constexpr int kThreads = 256;     // workgroup size of the row kernels
 constexpr int kPanelRows = 256;   // rows below the pivot block per panel job
 constexpr int kSolveRows = 64;    // rows below the pivot block per solve job
-constexpr int64_t kMaxOrderN = 40000;  // larger systems keep their given order
+constexpr int64_t kMaxOrderN = 40000;  // exact minimum degree up to here, AMD above
+constexpr int64_t kMaxPanelEntries = int64_t(8) << 30;  // 64 GiB of FP64 panels
 
 thread_local std::string g_err;
 
@@ -127,16 +128,140 @@ int adjacency(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind,
   return QTQP_OK;
 }
 
-// Minimum degree on the explicit elimination graph, held as one bitset row
-// per vertex: eliminating v joins its neighbours into a clique.  Ties go to
-// the lowest index, so the order is deterministic.
-std::vector<int32_t> min_degree(int64_t n, const std::vector<std::vector<int32_t>>& adj) {
+// Approximate minimum degree on a quotient graph (the method of Amestoy,
+// Davis and Duff, SIAM J. Matrix Anal. Appl. 17(4), 1996), for systems too
+// large for the explicit elimination graph below.  Eliminated vertices
+// become elements: an element e keeps the list L_e of live variables it
+// connects, and a variable keeps its remaining variable neighbours A_i and
+// its adjacent elements E_i, so memory stays near the size of the pattern.
+// Eliminating the pivot p forms L_p from A_p and the lists of p's elements,
+// which are absorbed into p.  Each variable i in L_p then gets the degree
+// bound
+//     d_i = min(left - 1, d_i + |L_p| - 1,
+//               |A_i| + |L_p| - 1 + sum over e in E_i, e != p, of |L_e \ L_p|),
+// with |L_e \ L_p| counted by decrementing |L_e| once per variable of
+// L_p in L_e; an element with |L_e \ L_p| = 0 is absorbed into p as well.
+// Variables with more than max(16, 10 sqrt(n)) neighbours (dense rows of A)
+// are ordered last.  Pivots are taken from degree buckets, lowest degree
+// first, most recently inserted first, so the order is deterministic.
+std::vector<int32_t> amd_order(int64_t n, const std::vector<std::vector<int32_t>>& adj) {
   std::vector<int32_t> order;
   order.reserve(n);
-  if (n > kMaxOrderN) {
-    for (int64_t i = 0; i < n; ++i) order.push_back((int32_t)i);
-    return order;
+  const int64_t dense = std::max<int64_t>(16, (int64_t)(10.0 * std::sqrt((double)n)));
+  std::vector<std::vector<int32_t>> A(n), E(n), L(n);
+  std::vector<int64_t> deg(n, 0), w(n, 0);
+  std::vector<int64_t> stamp(n, -1), wstamp(n, -1);
+  // 0 live variable, 1 eliminated (live element), 2 absorbed element, 3 dense
+  std::vector<char> state(n, 0);
+  std::vector<int32_t> dense_list;
+  for (int64_t i = 0; i < n; ++i) {
+    if ((int64_t)adj[i].size() > dense) {
+      state[i] = 3;
+      dense_list.push_back((int32_t)i);
+    }
   }
+  for (int64_t i = 0; i < n; ++i) {
+    if (state[i] == 3) continue;
+    for (int32_t j : adj[i])
+      if (j != i && state[j] != 3) A[i].push_back(j);
+    deg[i] = (int64_t)A[i].size();
+  }
+  // degree buckets: doubly linked lists
+  std::vector<int32_t> head(n + 1, -1), nxt(n, -1), prv(n, -1);
+  auto insert = [&](int32_t i) {
+    const int64_t d = deg[i];
+    nxt[i] = head[d];
+    prv[i] = -1;
+    if (head[d] != -1) prv[head[d]] = i;
+    head[d] = i;
+  };
+  auto remove = [&](int32_t i) {
+    if (prv[i] != -1) nxt[prv[i]] = nxt[i];
+    else head[deg[i]] = nxt[i];
+    if (nxt[i] != -1) prv[nxt[i]] = prv[i];
+    nxt[i] = prv[i] = -1;
+  };
+  int64_t left = 0;
+  for (int64_t i = n - 1; i >= 0; --i)
+    if (state[i] == 0) insert((int32_t)i), ++left;
+  int64_t mindeg = 0, tag = 0;
+  std::vector<int32_t> lp;
+  while (left > 0) {
+    while (head[mindeg] == -1) ++mindeg;
+    const int32_t p = head[mindeg];
+    remove(p);
+    state[p] = 1;
+    order.push_back(p);
+    --left;
+    // L_p: live variables of A_p and of p's elements (which are absorbed)
+    ++tag;
+    stamp[p] = tag;
+    lp.clear();
+    for (int32_t j : A[p])
+      if (state[j] == 0 && stamp[j] != tag) stamp[j] = tag, lp.push_back(j);
+    for (int32_t e : E[p]) {
+      if (state[e] != 1) continue;
+      for (int32_t j : L[e])
+        if (state[j] == 0 && stamp[j] != tag) stamp[j] = tag, lp.push_back(j);
+      state[e] = 2;
+      std::vector<int32_t>().swap(L[e]);
+    }
+    std::vector<int32_t>().swap(A[p]);
+    std::vector<int32_t>().swap(E[p]);
+    const int64_t np = (int64_t)lp.size();
+    // |L_e \ L_p| for the elements next to L_p
+    for (int32_t i : lp) {
+      remove(i);
+      for (int32_t e : E[i]) {
+        if (state[e] != 1) continue;
+        if (wstamp[e] != tag) wstamp[e] = tag, w[e] = (int64_t)L[e].size();
+        --w[e];
+      }
+    }
+    for (int32_t i : lp) {
+      // elements: drop absorbed ones and those inside L_p, then add p
+      int64_t ext = 0;
+      size_t k = 0;
+      for (int32_t e : E[i]) {
+        if (state[e] != 1) continue;
+        if (w[e] <= 0) {  // L_e within L_p: absorbed into p
+          state[e] = 2;
+          std::vector<int32_t>().swap(L[e]);
+          continue;
+        }
+        ext += w[e];
+        E[i][k++] = e;
+      }
+      E[i].resize(k);
+      E[i].push_back(p);
+      // variables: those in L_p are now reached through p
+      k = 0;
+      for (int32_t j : A[i])
+        if (state[j] == 0 && stamp[j] != tag) A[i][k++] = j;
+      A[i].resize(k);
+      int64_t d = std::min<int64_t>(left - 1, deg[i] + np - 1);
+      d = std::min<int64_t>(d, (int64_t)A[i].size() + np - 1 + ext);
+      deg[i] = std::max<int64_t>(d, 0);
+      insert(i);
+      mindeg = std::min(mindeg, deg[i]);
+    }
+    L[p].swap(lp);
+    lp.clear();
+  }
+  for (int32_t i : dense_list) order.push_back(i);
+  return order;
+}
+
+// Minimum degree on the explicit elimination graph, held as one bitset row
+// per vertex: eliminating v joins its neighbours into a clique.  Ties go to
+// the lowest index, so the order is deterministic.  ordering: 0 this one up
+// to kMaxOrderN and amd_order above, 1 this one, 2 amd_order.
+std::vector<int32_t> min_degree(int64_t n, const std::vector<std::vector<int32_t>>& adj,
+                                int ordering = 0) {
+  if (ordering == 2 || (ordering == 0 && n > kMaxOrderN) || (ordering == 1 && n > kMaxOrderN))
+    return amd_order(n, adj);
+  std::vector<int32_t> order;
+  order.reserve(n);
   const int64_t w = (n + 63) / 64;
   std::vector<uint64_t> bits((size_t)(n * w), 0);
   std::vector<int64_t> deg(n);
@@ -229,7 +354,8 @@ std::vector<std::vector<int32_t>> permuted_adjacency(const std::vector<std::vect
   return padj;
 }
 
-int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, Symbolic& S) {
+int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, Symbolic& S,
+            int32_t ordering = 0) {
   if (n < 1) return fail(QTQP_ERR_ARG, "n must be positive");
   if (n >= INT32_MAX) return fail(QTQP_ERR_ARG, "n too large");
   if (!ptr || !ind) return fail(QTQP_ERR_ARG, "null pattern");
@@ -239,7 +365,8 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
 
   // 1-2. ordering, then a postorder of its elimination tree.  A postorder is
   // an equivalent elimination order: same fill, same tree, relabelled.
-  std::vector<int32_t> perm = min_degree(n, adj), iperm(n);
+  if (ordering < 0 || ordering > 2) return fail(QTQP_ERR_ARG, "ordering must be 0, 1 or 2");
+  std::vector<int32_t> perm = min_degree(n, adj, ordering), iperm(n);
   for (int64_t k = 0; k < n; ++k) iperm[perm[k]] = (int32_t)k;
   std::vector<std::vector<int32_t>> padj = permuted_adjacency(adj, perm, iperm);
   std::vector<int32_t> parent = etree(n, padj);
@@ -337,6 +464,11 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
       S.flops += below + below * (below + 1.0);  // scale + symmetric rank-1 update
     }
   }
+  if (S.sn_lx[ns] > kMaxPanelEntries)
+    return fail(QTQP_ERR_ARG, "the factor's panels would hold " + std::to_string(S.sn_lx[ns]) +
+                                  " values (" + std::to_string(S.sn_lx[ns] * 8 >> 30) +
+                                  " GiB), over the limit of " +
+                                  std::to_string(kMaxPanelEntries * 8 >> 30) + " GiB");
 
   // 5. levels and update tiles.
   std::vector<int32_t> level(ns, 0);
@@ -935,9 +1067,14 @@ extern "C" {
 
 int qtqp_sparse_analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind,
                         int64_t* perm_out, qtqp_sparse_info* info) {
+  return qtqp_sparse_analyze_ex(n, format, ptr, ind, 0, perm_out, info);
+}
+
+int qtqp_sparse_analyze_ex(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind,
+                           int32_t ordering, int64_t* perm_out, qtqp_sparse_info* info) {
   Symbolic S;
   const auto t0 = std::chrono::steady_clock::now();
-  if (int rc = analyze(n, format, ptr, ind, S)) return rc;
+  if (int rc = analyze(n, format, ptr, ind, S, ordering)) return rc;
   const double ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (perm_out)

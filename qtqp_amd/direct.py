@@ -337,17 +337,20 @@ class HipSparseSolver(LinearSolver):
       pass
 
 
-def sparse_analyze(kkt: sp.spmatrix) -> tuple[np.ndarray, dict[str, Any]]:
+def sparse_analyze(kkt: sp.spmatrix, ordering: int = 0) -> tuple[np.ndarray, dict[str, Any]]:
   """Symbolic analysis of an upper-triangular KKT pattern on the host only
-  (qtqp_sparse_analyze): the elimination order and the factor's statistics."""
+  (qtqp_sparse_analyze_ex): the elimination order and the factor's statistics.
+  ordering: 0 exact minimum degree up to N = 40000 and approximate minimum
+  degree above (what the backend uses), 1 exact, 2 approximate."""
   lib = _native.sparse_lib()
   csc = sp.csc_matrix(kkt)
   csc.sort_indices()
   ptr, ind = _native.as_i64(csc.indptr), _native.as_i64(csc.indices)
   perm = np.empty(csc.shape[0], dtype=np.int64)
   inf = _native.SparseInfo()
-  rc = lib.qtqp_sparse_analyze(csc.shape[0], _native.SPARSE_CSC, _native.iptr(ptr),
-                               _native.iptr(ind), _native.iptr(perm), _native.ctypes.byref(inf))
+  rc = lib.qtqp_sparse_analyze_ex(csc.shape[0], _native.SPARSE_CSC, _native.iptr(ptr),
+                                  _native.iptr(ind), int(ordering), _native.iptr(perm),
+                                  _native.ctypes.byref(inf))
   _native.check(lib, rc)
   return perm, {k: getattr(inf, k) for k, _ in inf._fields_}
 

@@ -120,3 +120,64 @@ def test_analysis_rejects_lower_entries():
   kkt = sp.csc_matrix(np.ones((3, 3)))
   with pytest.raises(ValueError, match="upper triangle"):
     direct.sparse_analyze(kkt)
+
+
+# -- approximate minimum degree (the ordering above N = 40000) ---------------
+
+@pytest.mark.parametrize("mnz,density", [((15, 10, 2), 0.3), ((60, 40, 5), 0.1),
+                                         ((150, 100, 10), 0.1), ((90, 120, 0), 0.05)])
+def test_amd_analysis_matches_the_oracle(mnz, density):
+  """With the approximate minimum degree order the analysis still agrees with
+  the elimination game on fill, flops and supernode tables."""
+  m, n, z = mnz
+  kkt = _kkt_scaffold(m, n, z, 3, density)
+  perm, info = direct.sparse_analyze(kkt, ordering=2)
+  assert sorted(perm.tolist()) == list(range(kkt.shape[0]))
+  counts = SO.elimination_counts(SO.adjacency_from_upper(kkt), perm)
+  assert info["nnz_l"] == counts.sum()
+  below = counts - 1.0
+  assert info["factor_flops"] == pytest.approx(float(np.sum(below + below * (below + 1))))
+
+
+@pytest.mark.parametrize("mnz,density", [((1500, 1000, 100), 0.01), ((3000, 2000, 100), 0.01)])
+def test_amd_fill_is_close_to_exact_minimum_degree(mnz, density):
+  m, n, z = mnz
+  kkt = _kkt_scaffold(m, n, z, 3, density)
+  _, exact = direct.sparse_analyze(kkt, ordering=1)
+  _, approx = direct.sparse_analyze(kkt, ordering=2)
+  assert approx["nnz_l"] <= 1.05 * exact["nnz_l"]
+  _, auto = direct.sparse_analyze(kkt)
+  assert auto["nnz_l"] == exact["nnz_l"]  # exact below N = 40000
+
+
+def _random_rows_kkt(n, seed=7):
+  """KKT scaffold of n variables and 3n constraints with 3 random entries each,
+  P tridiagonal (P^-1 is dense, so eliminating x first fills the y block)."""
+  rng = np.random.default_rng(seed)
+  m = 3 * n
+  rows = np.repeat(np.arange(m), 3)
+  cols = rng.integers(0, n, 3 * m)
+  a = sp.csc_matrix((rng.standard_normal(3 * m), (rows, cols)), shape=(m, n))
+  p = sp.diags([np.ones(n - 1), 4.0 * np.ones(n)], [1, 0], format="csc")
+  return sp.bmat([[p + sp.eye(n, format="csc"), a.T], [None, sp.eye(m, format="csc")]],
+                 format="csc")
+
+
+def test_amd_fill_on_random_rows_is_close_to_exact():
+  kkt = _random_rows_kkt(4000)  # N = 16000
+  _, exact = direct.sparse_analyze(kkt, ordering=1)
+  _, approx = direct.sparse_analyze(kkt, ordering=2)
+  assert approx["nnz_l"] <= 1.05 * exact["nnz_l"]
+
+
+def test_analysis_above_40000_orders_for_fill():
+  """N = 48000 (12000 variables, 36000 constraints): the default analysis orders
+  by approximate minimum degree.  The natural [x; y] order would fill the whole
+  36000 x 36000 constraint block (6.5e8 entries); the ordering keeps nnz(L)
+  under a tenth of that, and the order is a permutation."""
+  n, m = 12000, 36000
+  kkt = _random_rows_kkt(n)
+  perm, info = direct.sparse_analyze(kkt)
+  assert kkt.shape[0] == 48000
+  assert np.array_equal(np.sort(perm), np.arange(48000))
+  assert info["nnz_l"] < 0.1 * m * m / 2, info["nnz_l"]
