@@ -45,6 +45,9 @@ typedef struct qtqp_sparse_info {
   double analyze_ms;        /* host time of the symbolic analysis */
   double factor_ms;         /* device time in factorisations (HIP events) */
   double solve_ms;          /* device time in solves and matvecs */
+  int64_t phases;           /* launches of Schur updates: levels split into classes of
+                               supernodes that share no ancestor row (deterministic
+                               assembly without atomics) */
 } qtqp_sparse_info;
 
 /* Symbolic analysis only, on the host (no device is touched): fill-reducing

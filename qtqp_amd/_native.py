@@ -111,7 +111,8 @@ class SparseInfo(ctypes.Structure):
               ("max_rows", ctypes.c_int64), ("update_tiles", ctypes.c_int64),
               ("factorizations", ctypes.c_int64), ("solves", ctypes.c_int64),
               ("factor_flops", ctypes.c_double), ("analyze_ms", ctypes.c_double),
-              ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double)]
+              ("factor_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+              ("phases", ctypes.c_int64)]
 
 
 _c_int64_p = ctypes.POINTER(ctypes.c_int64)

@@ -98,6 +98,14 @@ struct Symbolic {
   std::vector<int32_t> pjobs, sjobs;      // (supernode, a0) per panel / solve job
   std::vector<int32_t> contig;            // S: rows are consecutive indices
   std::vector<int64_t> pjob_off, sjob_off;  // L+1
+  // Phases: the supernodes of a level in colour classes that share no row
+  // below their pivot blocks, so that each class writes disjoint entries of
+  // the ancestors; classes run in a fixed order.  tiles and sjobs are in
+  // phase order (phases of a level contiguous, levels in order).
+  std::vector<int64_t> phase_tile_off, phase_sjob_off;  // P+1
+  std::vector<int64_t> level_phase_off;                  // L+1: phases of each level
+  std::vector<int64_t> sn_sjob;                          // S+1 by supernode: its solve jobs
+  std::vector<int32_t> sjob_sorted;                      // solve-job indices by supernode
 };
 
 // Symmetric adjacency (no diagonal) of the upper triangle given as
@@ -491,24 +499,77 @@ int analyze(int64_t n, int32_t format, const int64_t* ptr, const int64_t* ind, S
   S.pjob_off.assign(1, 0);
   S.sjobs.clear();
   S.sjob_off.assign(1, 0);
+  S.phase_tile_off.assign(1, 0);
+  S.phase_sjob_off.assign(1, 0);
+  S.level_phase_off.assign(1, 0);
+  // colours of the current level's rows below pivot blocks (greedy colouring:
+  // a supernode takes the lowest colour none of its rows carries yet)
+  std::vector<std::vector<int32_t>> row_colours(n);
+  std::vector<int32_t> touched;
+  std::vector<char> forbid;
   for (int32_t l = 0; l < nlev; ++l) {
+    std::vector<int32_t> colour(by_level[l].size());
+    int32_t ncol = 0;
+    for (size_t k = 0; k < by_level[l].size(); ++k) {
+      const int32_t s = by_level[l][k];
+      const int64_t c = S.sn_first[s + 1] - S.sn_first[s];
+      forbid.assign((size_t)ncol + 1, 0);
+      for (int64_t q = S.sn_rows[s] + c; q < S.sn_rows[s + 1]; ++q)
+        for (int32_t col : row_colours[S.rows[q]]) forbid[col] = 1;
+      int32_t col = 0;
+      while (forbid[col]) ++col;
+      colour[k] = col;
+      ncol = std::max(ncol, col + 1);
+      for (int64_t q = S.sn_rows[s] + c; q < S.sn_rows[s + 1]; ++q) {
+        if (row_colours[S.rows[q]].empty()) touched.push_back(S.rows[q]);
+        row_colours[S.rows[q]].push_back(col);
+      }
+    }
+    for (int32_t r : touched) row_colours[r].clear();
+    touched.clear();
     for (int32_t s : by_level[l]) {
       S.level_sn.push_back(s);
       const int64_t c = S.sn_first[s + 1] - S.sn_first[s];
       const int64_t below = S.sn_rows[s + 1] - S.sn_rows[s] - c;
-      const int64_t nb = (below + kTile - 1) / kTile;
-      for (int64_t bj = 0; bj < nb; ++bj)
-        for (int64_t bi = bj; bi < nb; ++bi)
-          S.tiles.insert(S.tiles.end(), {s, (int32_t)(bi * kTile), (int32_t)(bj * kTile)});
       for (int64_t a0 = 0; a0 == 0 || a0 < below; a0 += kPanelRows)
         S.pjobs.insert(S.pjobs.end(), {s, (int32_t)a0});
-      for (int64_t a0 = 0; a0 == 0 || a0 < below; a0 += kSolveRows)
-        S.sjobs.insert(S.sjobs.end(), {s, (int32_t)a0});
     }
+    for (int32_t col = 0; col < ncol; ++col) {
+      for (size_t k = 0; k < by_level[l].size(); ++k) {
+        if (colour[k] != col) continue;
+        const int32_t s = by_level[l][k];
+        const int64_t c = S.sn_first[s + 1] - S.sn_first[s];
+        const int64_t below = S.sn_rows[s + 1] - S.sn_rows[s] - c;
+        const int64_t nb = (below + kTile - 1) / kTile;
+        for (int64_t bj = 0; bj < nb; ++bj)
+          for (int64_t bi = bj; bi < nb; ++bi)
+            S.tiles.insert(S.tiles.end(), {s, (int32_t)(bi * kTile), (int32_t)(bj * kTile)});
+        for (int64_t a0 = 0; a0 == 0 || a0 < below; a0 += kSolveRows)
+          S.sjobs.insert(S.sjobs.end(), {s, (int32_t)a0});
+      }
+      S.phase_tile_off.push_back((int64_t)S.tiles.size() / 3);
+      S.phase_sjob_off.push_back((int64_t)S.sjobs.size() / 2);
+    }
+    S.level_phase_off.push_back((int64_t)S.phase_tile_off.size() - 1);
     S.level_off.push_back((int64_t)S.level_sn.size());
     S.tile_off.push_back((int64_t)S.tiles.size() / 3);
     S.pjob_off.push_back((int64_t)S.pjobs.size() / 2);
     S.sjob_off.push_back((int64_t)S.sjobs.size() / 2);
+  }
+  // each supernode's solve jobs, in increasing a0 (they are contiguous)
+  S.sn_sjob.assign(ns + 1, 0);
+  {
+    std::vector<int64_t> cnt(ns, 0), first(ns, -1);
+    const int64_t nj = (int64_t)S.sjobs.size() / 2;
+    for (int64_t j = 0; j < nj; ++j) {
+      const int32_t s = S.sjobs[2 * j];
+      if (first[s] < 0) first[s] = j;
+      ++cnt[s];
+    }
+    S.sjob_sorted.assign(nj, 0);
+    for (int64_t s = 0; s < ns; ++s) S.sn_sjob[s + 1] = S.sn_sjob[s] + cnt[s];
+    for (int64_t s = 0; s < ns; ++s)
+      for (int64_t k = 0; k < cnt[s]; ++k) S.sjob_sorted[S.sn_sjob[s] + k] = (int32_t)(first[s] + k);
   }
   S.contig.assign(ns, 0);
   for (int64_t s = 0; s < ns; ++s)
@@ -545,11 +606,35 @@ void fill_info(const Symbolic& S, qtqp_sparse_info* info) {
   info->max_rows = S.max_rows;
   info->update_tiles = (int64_t)S.tiles.size() / 3;
   info->factor_flops = S.flops;
+  info->phases = (int64_t)S.phase_tile_off.size() - 1;
 }
 
 // ---------------------------------------------------------------------------
 // Device kernels
 // ---------------------------------------------------------------------------
+
+// Records the launches `enqueue` makes on `stream` into a graph and
+// instantiates it (no memset nodes: the captured kernels clear what they need).
+template <class F>
+int capture(hipStream_t stream, F enqueue, hipGraphExec_t* exec) {
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  enqueue();
+  const hipError_t launch = hipGetLastError();
+  const hipError_t end = hipStreamEndCapture(stream, &graph);
+  if (launch != hipSuccess || end != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(QTQP_ERR_HIP, std::string("graph capture: ") +
+                                  hipGetErrorString(launch != hipSuccess ? launch : end));
+  }
+  const hipError_t inst = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (inst != hipSuccess) {
+    *exec = nullptr;
+    return fail(QTQP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(inst));
+  }
+  return QTQP_OK;
+}
 
 // Supernode tables, device pointers.
 struct Tables {
@@ -564,6 +649,8 @@ struct Tables {
   const int32_t* pjobs;    // (supernode, a0) per panel job: 256 rows below the block
   const int32_t* sjobs;    // (supernode, a0) per solve job: 64 rows below the block
   const int32_t* contig;   // S: 1 when a supernode's rows are consecutive indices
+  const int64_t* sn_sjob;  // S+1: each supernode's solve jobs in sjob_sorted
+  const int32_t* sjob_sorted;
 };
 
 __global__ void k_copy(double* __restrict__ dst, const double* __restrict__ src, int64_t n) {
@@ -729,8 +816,7 @@ __device__ __forceinline__ int find_row(const int32_t* __restrict__ rows, int lo
 // the pivots as weights.
 __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restrict__ lx,
                                                      const double* __restrict__ dblk,
-                                                     int64_t tile_begin, int exclusive,
-                                                     int* __restrict__ bad) {
+                                                     int64_t tile_begin, int* __restrict__ bad) {
   __shared__ double si[qk::KC][qk::NB + 1];
   __shared__ double sj[qk::KC][qk::NB + 1];
   const int32_t* tl = T.tiles + 3 * (tile_begin + blockIdx.x);
@@ -757,10 +843,9 @@ __global__ void __launch_bounds__(kThreads) k_update(Tables T, double* __restric
     const int32_t* tR = T.rows + troff;
     const int pos = T.contig[t] ? row - tR[0] : find_row(tR, lc, tr, row);
     if (pos >= lc && pos < tr && tR[pos] == row) {
-      double* dst = lx + T.lx_off[t] + (int64_t)lc * tr + pos;
-      // One supernode in the level: each target entry has one writer.
-      if (exclusive) *dst -= v;
-      else unsafeAtomicAdd(dst, -v);
+      // The supernodes of one phase share no row below their pivot blocks,
+      // so each target entry has one writer in the launch.
+      lx[T.lx_off[t] + (int64_t)lc * tr + pos] -= v;
     } else {
       atomicCAS(bad, 0, -1);  // the symbolic structure misses a fill entry
     }
@@ -806,12 +891,12 @@ __device__ __forceinline__ double wave_upper_solve(const double* __restrict__ D,
   return x;
 }
 
-// Forward substitution for one level, one job per 64 rows below a pivot
-// block (a single job when there are none): every job solves L11 y = b for
-// its supernode in LDS (b unchanged in x during the launch), the job with
-// a0 == 0 stores z = D^{-1} y in xo, and the job's rows update
-// b[rows] -= L21 y with atomics (supernodes of one level can share an
-// ancestor row).
+// Forward substitution for one phase of a level, one job per 64 rows below
+// a pivot block (a single job when there are none): every job solves
+// L11 y = b for its supernode in LDS (b unchanged in x during the launch),
+// the job with a0 == 0 stores z = D^{-1} y in xo, and the job's rows update
+// b[rows] -= L21 y (the supernodes of a phase share no ancestor row, and the
+// jobs of one supernode take disjoint rows, so each row has one writer).
 __global__ void __launch_bounds__(kThreads) k_fwd(Tables T, const double* __restrict__ lx,
                                                   const double* __restrict__ dblk,
                                                   int64_t job_begin, double* __restrict__ x,
@@ -843,13 +928,13 @@ __global__ void __launch_bounds__(kThreads) k_fwd(Tables T, const double* __rest
   part[g][lane] = t;
   __syncthreads();
   if (g == 0 && a < r - c)
-    unsafeAtomicAdd(x + T.rows[roff + c + a],
-                    -(part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]));
+    x[T.rows[roff + c + a]] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
-// Backward substitution, rows below the pivot blocks: partial sums
-// t_k += sum_a L21[a][k] x[rows[a]] over 64 rows per job into tb (zeroed
-// before the solve), staged through LDS so the loads stay coalesced.
+// Backward substitution, rows below the pivot blocks: each job's partial sums
+// sum_a L21[a][k] x[rows[a]] over its 64 rows into its own slot of tb
+// (kMaxCols per job), which k_bwd_diag adds up in job order; staged through
+// LDS so the loads stay coalesced.
 __global__ void __launch_bounds__(kThreads) k_bwd_off(Tables T, const double* __restrict__ lx,
                                                       int64_t job_begin,
                                                       const double* __restrict__ xo,
@@ -879,7 +964,8 @@ __global__ void __launch_bounds__(kThreads) k_bwd_off(Tables T, const double* __
     for (int q = 16 * g; q < 16 * g + 16; ++q) t += Ls[k][q] * xa[q];
   part[g][k] = t;
   __syncthreads();
-  if (tid < c) unsafeAtomicAdd(tb + f + tid, part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
+  if (tid < c)
+    tb[(job_begin + blockIdx.x) * kMaxCols + tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
 }
 
 // Backward substitution, pivot blocks of one level: x = L11^{-T} (z - t),
@@ -891,9 +977,25 @@ __global__ void __launch_bounds__(64) k_bwd_diag(Tables T, const double* __restr
   const int32_t f = T.first[s];
   const int c = T.first[s + 1] - f;
   const int lane = threadIdx.x;
-  const double v = lane < c ? xo[f + lane] - tb[f + lane] : 0.0;
+  // the jobs' partial sums, four running sums over the jobs in increasing row
+  // order, combined in a fixed order
+  double t4[4] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t qb = T.sn_sjob[s], qe = T.sn_sjob[s + 1];
+  if (lane < c) {
+    int64_t q = qb;
+    for (; q + 4 <= qe; q += 4)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t4[u] += tb[(int64_t)T.sjob_sorted[q + u] * kMaxCols + lane];
+    for (; q < qe; ++q) t4[0] += tb[(int64_t)T.sjob_sorted[q] * kMaxCols + lane];
+  }
+  const double t = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+  const double v = lane < c ? xo[f + lane] - t : 0.0;
   const double xj = wave_upper_solve(dblk + T.dblk_off[s], c, lane, v);
   if (lane < c) xo[f + lane] = xj;
+}
+
+__global__ void k_zero_int(int* __restrict__ x) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *x = 0;
 }
 
 __global__ void k_zero(double* __restrict__ x, int64_t n) {
@@ -959,6 +1061,9 @@ struct qtqp_sparse {
   double* xp = nullptr;        // N-vector work (permuted order): b, then y
   double* xo = nullptr;        // N-vector work (permuted order): z, then x
   int* bad = nullptr;
+  // the factorisation's and the solve's launches, captured once per handle
+  // (the pattern and the buffers are fixed) and replayed
+  hipGraphExec_t gfac = nullptr, gsol = nullptr;
   bool factored = false;
   int64_t factorizations = 0, solves = 0;
   double analyze_ms = 0, factor_ms = 0, solve_ms = 0;
@@ -978,6 +1083,8 @@ struct qtqp_sparse {
     return QTQP_OK;
   }
   ~qtqp_sparse() {
+    if (gfac) (void)hipGraphExecDestroy(gfac);
+    if (gsol) (void)hipGraphExecDestroy(gsol);
     for (void* p : allocs) (void)hipFree(p);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -1024,8 +1131,8 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&h->ev0));
   HIP_TRY(hipEventCreate(&h->ev1));
-  int32_t *first, *rows, *col_sn, *level_sn, *tiles, *pjobs, *sjobs, *contig;
-  int64_t *rows_off, *lx_off, *dblk_off;
+  int32_t *first, *rows, *col_sn, *level_sn, *tiles, *pjobs, *sjobs, *contig, *sjob_sorted;
+  int64_t *rows_off, *lx_off, *dblk_off, *sn_sjob;
   if (int rc = h->upload(&first, S.sn_first)) return rc;
   if (int rc = h->upload(&rows_off, S.sn_rows)) return rc;
   if (int rc = h->upload(&lx_off, S.sn_lx)) return rc;
@@ -1037,10 +1144,13 @@ int build_handle(qtqp_sparse* h, int32_t format, const int64_t* ptr, const int64
   if (int rc = h->upload(&pjobs, S.pjobs)) return rc;
   if (int rc = h->upload(&sjobs, S.sjobs)) return rc;
   if (int rc = h->upload(&contig, S.contig)) return rc;
-  h->T = Tables{first,    rows_off, lx_off, dblk_off, rows,  col_sn,
-                level_sn, tiles,    pjobs,  sjobs,    contig};
+  if (int rc = h->upload(&sn_sjob, S.sn_sjob)) return rc;
+  if (int rc = h->upload(&sjob_sorted, S.sjob_sorted)) return rc;
+  h->T = Tables{first,    rows_off, lx_off, dblk_off, rows,   col_sn,  level_sn,
+                tiles,    pjobs,    sjobs,  contig,   sn_sjob, sjob_sorted};
   if (int rc = h->alloc(&h->dblk, S.dblk_off.back())) return rc;
-  if (int rc = h->alloc(&h->tb, n)) return rc;
+  if (int rc = h->alloc(&h->tb, std::max<int64_t>(1, (int64_t)S.sjobs.size() / 2 * kMaxCols)))
+    return rc;
   if (int rc = h->upload(&h->lx0, lx0)) return rc;
   if (int rc = h->alloc(&h->lx, (int64_t)lx0.size())) return rc;
   if (int rc = h->upload(&h->dpos, dpos)) return rc;
@@ -1120,24 +1230,31 @@ int qtqp_sparse_factorize(qtqp_sparse* h) {
   if (!h) return fail(QTQP_ERR_ARG, "null handle");
   const Symbolic& S = h->S;
   h->factored = false;
-  HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  HIP_TRY(hipMemsetAsync(h->bad, 0, sizeof(int), h->stream));
-  const int64_t nlx = S.sn_lx.back();
-  k_copy<<<(unsigned)std::min<int64_t>(4096, blocks_for(nlx)), kThreads, 0, h->stream>>>(
-      h->lx, h->lx0, nlx);
-  k_load_diag<<<blocks_for(S.n), kThreads, 0, h->stream>>>(h->lx, h->dpos, h->diag, h->perm, S.n);
-  const int64_t nlev = (int64_t)S.level_off.size() - 1;
-  for (int64_t l = 0; l < nlev; ++l) {
-    const int64_t np = S.pjob_off[l + 1] - S.pjob_off[l];
-    k_panel<<<(unsigned)np, kThreads, 0, h->stream>>>(h->T, h->lx, h->dblk, S.pjob_off[l],
-                                                      h->bad, h->perm);
-    const int64_t nt = S.tile_off[l + 1] - S.tile_off[l];
-    if (nt > 0)
-      k_update<<<(unsigned)nt, kThreads, 0, h->stream>>>(
-          h->T, h->lx, h->dblk, S.tile_off[l],
-          (int)(S.level_off[l + 1] - S.level_off[l] == 1), h->bad);
+  if (!h->gfac) {
+    auto enqueue = [&]() {
+      k_zero_int<<<1, 64, 0, h->stream>>>(h->bad);
+      const int64_t nlx = S.sn_lx.back();
+      k_copy<<<(unsigned)std::min<int64_t>(4096, blocks_for(nlx)), kThreads, 0, h->stream>>>(
+          h->lx, h->lx0, nlx);
+      k_load_diag<<<blocks_for(S.n), kThreads, 0, h->stream>>>(h->lx, h->dpos, h->diag, h->perm,
+                                                                S.n);
+      const int64_t nlev = (int64_t)S.level_off.size() - 1;
+      for (int64_t l = 0; l < nlev; ++l) {
+        const int64_t np = S.pjob_off[l + 1] - S.pjob_off[l];
+        k_panel<<<(unsigned)np, kThreads, 0, h->stream>>>(h->T, h->lx, h->dblk, S.pjob_off[l],
+                                                          h->bad, h->perm);
+        for (int64_t ph = S.level_phase_off[l]; ph < S.level_phase_off[l + 1]; ++ph) {
+          const int64_t nt = S.phase_tile_off[ph + 1] - S.phase_tile_off[ph];
+          if (nt > 0)
+            k_update<<<(unsigned)nt, kThreads, 0, h->stream>>>(h->T, h->lx, h->dblk,
+                                                               S.phase_tile_off[ph], h->bad);
+        }
+      }
+    };
+    if (int rc = capture(h->stream, enqueue, &h->gfac)) return rc;
   }
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(hipGraphLaunch(h->gfac, h->stream));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   int bad = 0;
   HIP_TRY(hipMemcpyAsync(&bad, h->bad, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1157,21 +1274,26 @@ int qtqp_sparse_solve(qtqp_sparse* h, const double* rhs, double* out) {
   const Symbolic& S = h->S;
   const int64_t n = S.n;
   HIP_TRY(hipMemcpyAsync(h->xin, rhs, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
-  HIP_TRY(hipEventRecord(h->ev0, h->stream));
-  k_gather<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xp, h->xin, h->perm, n);
-  k_zero<<<blocks_for(n), kThreads, 0, h->stream>>>(h->tb, n);
-  const int64_t nlev = (int64_t)S.level_off.size() - 1;
-  for (int64_t l = 0; l < nlev; ++l)
-    k_fwd<<<(unsigned)(S.sjob_off[l + 1] - S.sjob_off[l]), kThreads, 0, h->stream>>>(
-        h->T, h->lx, h->dblk, S.sjob_off[l], h->xp, h->xo);
-  for (int64_t l = nlev - 1; l >= 0; --l) {
-    k_bwd_off<<<(unsigned)(S.sjob_off[l + 1] - S.sjob_off[l]), kThreads, 0, h->stream>>>(
-        h->T, h->lx, S.sjob_off[l], h->xo, h->tb);
-    k_bwd_diag<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), 64, 0, h->stream>>>(
-        h->T, h->dblk, S.level_off[l], h->tb, h->xo);
+  if (!h->gsol) {
+    auto enqueue = [&]() {
+      k_gather<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xp, h->xin, h->perm, n);
+      const int64_t nph = (int64_t)S.phase_sjob_off.size() - 1;
+      for (int64_t ph = 0; ph < nph; ++ph)
+        k_fwd<<<(unsigned)(S.phase_sjob_off[ph + 1] - S.phase_sjob_off[ph]), kThreads, 0,
+                h->stream>>>(h->T, h->lx, h->dblk, S.phase_sjob_off[ph], h->xp, h->xo);
+      const int64_t nlev = (int64_t)S.level_off.size() - 1;
+      for (int64_t l = nlev - 1; l >= 0; --l) {
+        k_bwd_off<<<(unsigned)(S.sjob_off[l + 1] - S.sjob_off[l]), kThreads, 0, h->stream>>>(
+            h->T, h->lx, S.sjob_off[l], h->xo, h->tb);
+        k_bwd_diag<<<(unsigned)(S.level_off[l + 1] - S.level_off[l]), 64, 0, h->stream>>>(
+            h->T, h->dblk, S.level_off[l], h->tb, h->xo);
+      }
+      k_scatter<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xin, h->xo, h->perm, n);
+    };
+    if (int rc = capture(h->stream, enqueue, &h->gsol)) return rc;
   }
-  k_scatter<<<blocks_for(n), kThreads, 0, h->stream>>>(h->xin, h->xo, h->perm, n);
-  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(h->ev0, h->stream));
+  HIP_TRY(hipGraphLaunch(h->gsol, h->stream));
   HIP_TRY(hipEventRecord(h->ev1, h->stream));
   HIP_TRY(hipMemcpyAsync(out, h->xin, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));

@@ -13,7 +13,7 @@ import pytest
 import scipy.sparse as sp
 
 import qtqp_amd
-from qtqp_amd import direct
+from qtqp_amd import direct, problems
 from oracle import qtqp_oracle as O
 import golden_cases
 import qp_generators as G
@@ -145,12 +145,16 @@ def test_end_to_end_matches_reference_fixtures(require_gpu, key):
   if case["status"] == "solved":
     assert abs(len(sol.stats) - case["iters"]) <= 1
   else:
-    # The Schur updates of one level reach shared ancestors through FP64
-    # atomics, so the factor's last bits depend on their order.  On a
-    # certificate instance the last iterations run on stalled refinement
-    # (residuals of order one) and the count moves with those bits (12-16
-    # observed on one fixture); the status and the certificate do not.
-    assert case["iters"] - 1 <= len(sol.stats) <= case["iters"] + 6
+    # A certificate instance ends on stalled refinement (residuals of order
+    # one), where the count follows the factor's rounding, and the supernodal
+    # pivot order rounds differently from the reference's factor: one fixture
+    # takes 4 iterations more.  The assembly is deterministic, so the count
+    # and the certificate repeat exactly from run to run.
+    assert abs(len(sol.stats) - case["iters"]) <= 4
+    again = qtqp_amd.QTQP(a=sp.csc_matrix(a), b=b, c=c, z=z, p=sp.csc_matrix(p)).solve(
+        linear_solver=SPARSE, equilibrate=case["equilibrate"], verbose=False, collect_stats=True)
+    assert len(again.stats) == len(sol.stats)
+    np.testing.assert_array_equal(again.x, sol.x)
     if case["status"] == "infeasible":
       G.check_infeasible(sol.x, sol.y, sol.s, a, b, z)
     else:
@@ -310,3 +314,32 @@ def test_denser_qp_matches_oracle_objective(require_gpu):
   G.check_solution(sol.x, sol.y, sol.s, a, b, c, p, z)
   obj = lambda x: c @ x + 0.5 * x @ p @ x
   assert abs(obj(sol.x) - obj(ref["x"])) <= 1e-6 * max(1.0, abs(obj(ref["x"])))
+
+
+def test_factor_and_solve_are_bitwise_reproducible(require_gpu):
+  """The Schur updates and the triangular solves assemble without atomics
+  (supernodes of a level in phases that share no ancestor row; backward
+  partial sums added in job order), so two handles built from the same KKT
+  matrix give the same bits, and so does a second solve."""
+  qp = problems.random_sparse_qp(1500, 1000, 100, seed=4, density=0.01)
+  m, n, z = 1500, 1000, 100
+  kkt = sp.bmat([[sp.triu(qp.p, format="csc") + sp.eye(n, format="csc"), qp.a.T],
+                 [None, sp.eye(m, format="csc")]], format="csc")
+  rng = np.random.default_rng(2)
+  diag = np.concatenate([np.asarray(qp.p.diagonal()) + 0.3, np.full(z, 0.3),
+                         rng.uniform(0.05, 20.0, m - z) + 0.3])
+  diag[n:] *= -1.0
+  rhs = rng.normal(size=n + m)
+  outs = []
+  for _ in range(2):
+    be = direct.HipSparseSolver()
+    be.set_kkt(kkt)
+    be.update_diag(diag)
+    be.factorize()
+    first = be.solve(rhs)
+    outs.append((first, be.solve(rhs)))
+    info = be.info()
+    be.free()
+  assert info["phases"] >= info["levels"]
+  np.testing.assert_array_equal(outs[0][0], outs[1][0])
+  np.testing.assert_array_equal(outs[0][0], outs[0][1])
