@@ -159,16 +159,31 @@ def kkt_rel_residual(x, y, s, a, b, c, p, z):
 
 
 def warm_start_leg(n_train, n_eval, steps, k_star=7):
-  """Learned warm start on the parametric MPC family (T=32, nx=8, nu=3)."""
+  """Learned warm start on the parametric MPC family (T=32, nx=8, nu=3).
+
+  Cold and warm solves run on one resident family batch (A and P uploaded
+  once; each solve uploads b and c): cold from the default start, warm from
+  the network's proposal decoded on the device.  The warm wall time includes
+  the inference.  Parity: a sample of the warm-started QPs is re-solved by
+  the FP64 oracle from the same proposed start (same status, x/y/s error),
+  and compared with the oracle's default-start solution.
+  """
   from qtqp_amd import problems, warmstart
   from oracle import qtqp_oracle
   fam = warmstart.Family()
   x_tr = problems.parametric_mpc_x0(n_train, seed=11)
   x_ev = problems.parametric_mpc_x0(n_eval, seed=22)
+  solver = warmstart.FamilySolver(fam, chunk=n_eval)
   t0 = time.perf_counter()
-  cold = warmstart._solve(fam, x_ev)
-  t_cold = time.perf_counter() - t0
-  log(f"warm start: cold solve of {n_eval} QPs: {cold['iters'].mean():.2f} iterations mean")
+  solver.solve(x_ev, max_iter=1)  # creates the resident batch
+  t_create = time.perf_counter() - t0
+  t_cold = []
+  for _ in range(3):
+    t0 = time.perf_counter()
+    cold = solver.solve(x_ev)
+    t_cold.append(time.perf_counter() - t0)
+  log(f"warm start: cold solve of {n_eval} QPs: {cold['iters'].mean():.2f} iterations mean, "
+      f"{min(t_cold):.3f}s")
   ws = warmstart.WarmStart(fam, k_star=k_star)
   t0 = time.perf_counter()
   t, _ = ws.targets(x_tr)
@@ -178,23 +193,36 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
   losses = ws.fit(x_tr, t, steps=steps)
   t_train = time.perf_counter() - t0
   cap = int(np.percentile(cold["iters"], 99))
-  t0 = time.perf_counter()
-  warm = warmstart.solve_with_warm_start(ws, x_ev, cap=cap)
-  t_warm = time.perf_counter() - t0
+  t_warm = []
+  for _ in range(3):
+    t0 = time.perf_counter()
+    warm = warmstart.solve_with_warm_start(ws, x_ev, cap=cap, solver=solver)
+    t_warm.append(time.perf_counter() - t0)
+  log(f"warm start: warm solve {warm['iters'].mean():.2f} iterations mean, {min(t_warm):.3f}s")
+  solver.free()
   idx = np.linspace(0, n_eval - 1, 32).astype(int)
   qps = fam.qps(x_ev[idx])
+  px, py, ps, pt, _, valid = ws.propose(x_ev[idx])
   names = {0: "unfinished", 1: "solved", 2: "infeasible", 3: "unbounded"}
-  res, dxys = [], {k: [] for k in "xys"}
-  ok = True
-  for i, (a, b, c, z, p) in zip(idx, qps):
+  res, d_same, d_default = [], {k: [] for k in "xys"}, {k: [] for k in "xys"}
+  ok_same = ok_default = True
+  rel = lambda u, v: float(np.max(np.abs(u - v)) / max(1.0, np.max(np.abs(v))))
+  for j, (i, (a, b, c, z, p)) in enumerate(zip(idx, qps)):
+    got = names.get(int(warm["status"][i]), "other")
     ref = qtqp_oracle.solve_qp(a, b, c, z, p)
-    ok &= ref["status"] == names.get(int(warm["status"][i]), "other")
+    ok_default &= ref["status"] == got
+    same = None
+    if valid[j] and not warm["fallback"][i]:
+      same = qtqp_oracle.solve_qp(a, b, c, z, p, max_iter=cap,
+                                  start=(px[j], py[j], ps[j], pt[j]))
+      ok_same &= same["status"] == got
     if ref["status"] != "solved":
       continue
     res.append(kkt_rel_residual(warm["x"][i], warm["y"][i], warm["s"][i], a, b, c, p, z))
     for k in "xys":
-      got, want = warm[k][i], ref[k]
-      dxys[k].append(float(np.max(np.abs(got - want)) / max(1.0, np.max(np.abs(want)))))
+      d_default[k].append(rel(warm[k][i], ref[k]))
+      if same is not None:
+        d_same[k].append(rel(warm[k][i], same[k]))
   ws.mlp.free()
   return {"family": "condensed MPC T=32 nx=8 nu=3, x0 ~ N(0, 0.5^2 I)", "k_star": k_star,
           "train_qps": n_train, "eval_qps": n_eval, "mlp": "8-256-256-1506 GELU, BF16 MFMA, Adam",
@@ -205,12 +233,19 @@ def warm_start_leg(n_train, n_eval, steps, k_star=7):
           "iteration_cut": float(1.0 - warm["iters"].mean() / cold["iters"].mean()),
           "statuses_identical": bool(np.array_equal(warm["status"], cold["status"])),
           "fallback_rate": float(warm["fallback"].mean()),
-          "cold_qps_per_s": n_eval / t_cold, "warm_qps_per_s_incl_inference": n_eval / t_warm,
+          "resident_batch": f"one family batch of {n_eval} QPs, A and P uploaded once "
+                            f"(created in {t_create:.3f}s, outside the timings); each solve "
+                            "uploads b and c and downloads the solutions",
+          "cold_qps_per_s": n_eval / min(t_cold), "warm_qps_per_s_incl_inference": n_eval / min(t_warm),
+          "warm_over_cold_qps": min(t_cold) / min(t_warm),
           "cold_device_ms": float(cold["device_ms"]), "warm_device_ms": float(warm["device_ms"]),
           "device_time_cut": float(1.0 - warm["device_ms"] / cold["device_ms"]),
-          "oracle_sample": len(qps), "oracle_status_match": bool(ok),
-          "max_rel_kkt_residual": float(max(res)),
-          "max_rel_err_vs_oracle": {k: float(max(v)) for k, v in dxys.items()}}
+          "oracle_sample": len(qps),
+          "oracle_same_start": {"status_match": bool(ok_same),
+                                "max_rel_err": {k: float(max(v)) if v else None for k, v in d_same.items()}},
+          "oracle_default_start": {"status_match": bool(ok_default),
+                                   "max_rel_err": {k: float(max(v)) for k, v in d_default.items()}},
+          "max_rel_kkt_residual": float(max(res))}
 
 
 def single_instance_leg(name, qp, oracle_ref=None):

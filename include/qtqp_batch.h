@@ -91,6 +91,26 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
                          const int64_t* m_off, const double* a, const double* p,
                          const double* b, const double* c, int32_t flags, qtqp_batch** out);
 
+/* A batch of `count` QPs of one parametric family: every QP has the same
+ * n, m, z, A (m x n row-major) and P (n x n), and its own b (row q of the
+ * count x m row-major b) and c (row q of the count x n c).  A and P are
+ * uploaded, paired and equilibrated once (Ruiz scaling depends on A and P
+ * only, __init__.py:616-663) and every QP reads that one device copy;
+ * results are those of qtqp_batch_create_ex on `count` copies of A and P.
+ * Packed x/y/s follow n_off[q] = q n, m_off[q] = q m.  Serves the same
+ * QTQP(...).solve() per QP as the reference (__init__.py:309-614); the
+ * sharing itself has no reference counterpart. */
+int qtqp_batch_create_family(int32_t count, int32_t n, int32_t m, int32_t z, const double* a,
+                             const double* p, const double* b, const double* c, int32_t flags,
+                             qtqp_batch** out);
+
+/* Replaces b and c of every QP (packed by m_off / n_off, as at creation) for
+ * the next solve: a re-solve of the same A and P with new right-hand sides,
+ * as QTQP(a, b', c', z, p).solve() would (the caller keeps the presolve's
+ * dropped rows the same).  No reference counterpart (the reference builds a
+ * new QTQP per problem). */
+int qtqp_batch_set_bc(qtqp_batch* h, const double* b, const double* c);
+
 /* Stored rows of A per QP (m[i] less the number of row pairs), in creation
  * order.  No reference counterpart (batch diagnostics). */
 int qtqp_batch_stored_rows(qtqp_batch* h, int32_t* mc);
@@ -107,6 +127,18 @@ int qtqp_batch_row_bytes(qtqp_batch* h, int64_t* bytes);
 int qtqp_batch_set_start(qtqp_batch* h, const double* x, const double* y, const double* s,
                          const double* tau);
 
+/* qtqp_batch_set_start from device memory of the handle's device: x, y, s
+ * and tau packed as for qtqp_batch_set_start (for example filled by
+ * qtqp_mlp_propose).  Same meaning, same __init__.py lines. */
+int qtqp_batch_set_start_device(qtqp_batch* h, const double* x, const double* y,
+                                const double* s, const double* tau);
+
+/* The handle's own packed device buffers of x (sum of n), y and s (sum of m)
+ * and tau (count): a producer on the device may fill them and pass them to
+ * qtqp_batch_set_start_device.  qtqp_batch_results and qtqp_batch_solution
+ * overwrite them.  No reference counterpart. */
+int qtqp_batch_device_buffers(qtqp_batch* h, double** x, double** y, double** s, double** tau);
+
 /* Equilibrates on the device and runs the IPM on every QP until each has a
  * status or max_iter iterations.  Inputs stay on the device, so a handle can
  * be solved repeatedly. */
@@ -119,6 +151,15 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
 int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* tau,
                        int32_t* status, int32_t* iters, double* pres, double* dres,
                        double* gap);
+
+/* The solution of every QP as QTQP.solve returns it before the presolve's
+ * dropped rows are restored (__init__.py:580-614): x, y, s divided by tau
+ * (solved, max_iter, error), or the infeasibility certificate (x = s = NaN,
+ * y / |b'y|), or the unboundedness certificate (y = NaN, x and s divided by
+ * |c'x|), formed on the device; plus status and iteration count.  Packed as
+ * qtqp_batch_results; any output pointer may be NULL. */
+int qtqp_batch_solution(qtqp_batch* h, double* x, double* y, double* s, int32_t* status,
+                        int32_t* iters);
 
 /* Device time of each IPM launch of the last solve (HIP events on the launch
  * stream): one value for the persistent schedule, one per iteration with

@@ -32,6 +32,21 @@ int qtqp_mlp_get_params(qtqp_mlp* h, float* params);
 /* y (batch x n_out) = network(x (batch x n_in)); any batch size. */
 int qtqp_mlp_forward(qtqp_mlp* h, const float* x, int64_t batch, float* y);
 
+/* Starting points of the batched interior-point engine from the network, on
+ * the device.  x: batch x n_in standardised inputs (host); tm, ts: the
+ * n_out = n + 2 m + 2 target means and scales (host, FP64).  Each row's
+ * outputs are un-standardised in FP64 (raw = out ts + tm) and read as
+ * (x, log y, log s, log tau, log kappa): x_start (batch x n), y_start and
+ * s_start (batch x m: exp of the clipped logs on the m - z inequality rows,
+ * y = raw and s = 0 on the z equality rows) and tau_start (batch) are
+ * written to DEVICE memory, packed as qtqp_batch_set_start_device takes
+ * them.  A row with a non-finite output gets the default start of
+ * src/qtqp/__init__.py:299-307 and valid[row] = 0 (valid is host memory
+ * and may be NULL).  No reference counterpart (the reference has only the
+ * default start). */
+int qtqp_mlp_propose(qtqp_mlp* h, const float* x, int64_t batch, const double* tm,
+                     const double* ts, int32_t n, int32_t m, int32_t z, double* x_start,
+                     double* y_start, double* s_start, double* tau_start, int32_t* valid);
 /* Mean-squared-error loss of one batch (batch <= max_batch) and its gradient
  * with respect to the packed parameters (no update). */
 int qtqp_mlp_loss_grad(qtqp_mlp* h, const float* x, const float* t, int32_t batch, float* grad,

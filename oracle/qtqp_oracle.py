@@ -276,11 +276,16 @@ def solve_qp(a, b, c, z, p=None, *, atol=1e-7, rtol=1e-8, atol_infeas=1e-8,
              min_static_regularization=1e-8, max_iterative_refinement_steps=20,
              linear_solver_atol=1e-12, linear_solver_rtol=1e-12,
              equilibrate=True, backend_factory=DenseGramBackend,
-             force_linearized_tau=False) -> dict[str, Any]:
+             force_linearized_tau=False, start=None) -> dict[str, Any]:
   """Solve one QP exactly as qtqp's IPM does, with the dense Gram backend.
 
   Returns a dict with x, y, s, status (string), stats (per-iteration dicts,
   always collected) and iters.  Accepts dense arrays or scipy sparse matrices.
+  start: optional (x, y, s, tau) in unscaled coordinates of the presolved QP,
+  replacing the default start of _init_variables (__init__.py:299-307); it is
+  equilibrated as that hook equilibrates the default one (668-669).  This is
+  the batched engine's warm-start hook; the reference itself always starts
+  from the default point.
   """
   a_full = a.toarray() if sp.issparse(a) else np.array(a, dtype=np.float64)
   m_full, n = a_full.shape
@@ -318,13 +323,19 @@ def solve_qp(a, b, c, z, p=None, *, atol=1e-7, rtol=1e-8, atol_infeas=1e-8,
   kkt = KktIR(a_eq, p_eq, z, min_static_regularization,
               max_iterative_refinement_steps, linear_solver_atol,
               linear_solver_rtol, backend_factory)
-  x = np.zeros(n)
-  y = np.zeros(m)
-  s = np.zeros(m)
-  y[z:] = 1.0
-  s[z:] = 1.0
+  if start is None:
+    x = np.zeros(n)
+    y = np.zeros(m)
+    s = np.zeros(m)
+    y[z:] = 1.0
+    s[z:] = 1.0
+    tau = 1.0
+  else:
+    x, y, s = (np.array(v, dtype=np.float64) for v in start[:3])
+    tau = float(start[3])
+    if x.shape != (n,) or y.shape != (m,) or s.shape != (m,):
+      raise ValueError("start has the wrong sizes")
   x, y, s = x / e, y / d, s * d
-  tau = 1.0
   kinv_q = np.zeros(n + m)
   stats = []
   status = UNFINISHED

@@ -47,9 +47,19 @@ BATCH_SYMBOLS = {
     "qtqp_batch_create_ex": (ctypes.c_int, [ctypes.c_int32, _i32p, _i32p, _i32p, _i64p, _i64p,
                                             _i64p, _i64p, _dp, _dp, _dp, _dp, ctypes.c_int32,
                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_batch_create_family": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_int32, _dp, _dp, _dp, _dp, ctypes.c_int32,
+                                                ctypes.POINTER(ctypes.c_void_p)]),
+    "qtqp_batch_set_bc": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "qtqp_batch_stored_rows": (ctypes.c_int, [ctypes.c_void_p, _i32p]),
     "qtqp_batch_row_bytes": (ctypes.c_int, [ctypes.c_void_p, _i64p]),
     "qtqp_batch_set_start": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp]),
+    "qtqp_batch_set_start_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_void_p, ctypes.c_void_p,
+                                                   ctypes.c_void_p]),
+    "qtqp_batch_device_buffers": (ctypes.c_int, [ctypes.c_void_p] +
+                                  [ctypes.POINTER(ctypes.c_void_p)] * 4),
+    "qtqp_batch_solution": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _i32p, _i32p]),
     "qtqp_batch_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(BatchOpts),
                                         ctypes.POINTER(BatchCounters)]),
     "qtqp_batch_results": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _dp, _i32p, _i32p,
@@ -235,6 +245,109 @@ class BatchQP:
     _check(self._lib, rc)
     self._h = h
 
+  @classmethod
+  def family(cls, a, p, z: int, b, c, pair_rows: bool = True,
+             row_extents: bool = True) -> "BatchQP":
+    """count QPs of one parametric family: the same A (m x n), P (n x n or
+    None) and z, with b of shape (count, m) and c of shape (count, n).  A and
+    P are held once on the device and every QP reads them (results as for
+    ``BatchQP`` over ``count`` copies).  Rows with b >= 1e20 are dropped by
+    the presolve as in ``BatchQP``; they must be the same rows in every QP."""
+    self = cls.__new__(cls)
+    self._lib = batch_lib()
+    self._h = None
+    a = np.asarray(a, dtype=np.float64)
+    m, n = a.shape
+    p = np.zeros((n, n)) if p is None else np.asarray(p, dtype=np.float64)
+    if not 0 <= z <= m or p.shape != (n, n):
+      raise ValueError("inconsistent QP shapes")
+    self._a_family = a
+    self.count = count = int(np.shape(b)[0]) if np.ndim(b) == 2 else 1
+    self.z = np.full(count, z, dtype=np.int32)
+    self.n = np.full(count, n, dtype=np.int32)
+    bk, ck, keep, self._b_drop = self._family_vectors(b, c, m)
+    a_k = np.ascontiguousarray(a[keep])
+    if z == a_k.shape[0]:
+      raise ValueError("effective z == m after presolve; some rows may have been dropped")
+    mk = a_k.shape[0]
+    self._keep = keep
+    self.m = np.full(count, mk, dtype=np.int32)
+    self.n_off = np.arange(count, dtype=np.int64) * n
+    self.m_off = np.arange(count, dtype=np.int64) * mk
+    h = ctypes.c_void_p()
+    rc = self._lib.qtqp_batch_create_family(
+        count, n, mk, z, _ptr(a_k, _dp), _ptr(p, _dp), _ptr(bk, _dp), _ptr(ck, _dp),
+        (0 if pair_rows else cls.NO_ROW_PAIRS) | (0 if row_extents else cls.NO_ROW_EXTENTS),
+        ctypes.byref(h))
+    _check(self._lib, rc)
+    self._h = h
+    return self
+
+  @staticmethod
+  def _family_keep(b, z):
+    """Rows every QP keeps after the presolve (the same in all, or an error)."""
+    if z and not np.all(np.isfinite(b[:, :z])):
+      raise ValueError("Equality RHS entries in 'b' must be finite.")
+    # per-row minimum and maximum over the QPs: NaN and -inf show in them,
+    # and a row is dropped in every QP or in none iff both sides agree
+    lo, hi = b[:, z:].min(axis=0), b[:, z:].max(axis=0)
+    if np.any(np.isnan(lo) | np.isnan(hi) | np.isneginf(lo)):
+      raise ValueError("Inequality RHS entries in 'b' must be finite, +inf, or >= inf_bound.")
+    if np.any((lo >= 1e20) != (hi >= 1e20)):
+      raise ValueError("a family batch needs the same rows with b >= 1e20 in every QP")
+    keep = np.ones(b.shape[1], dtype=bool)
+    keep[z:] = ~(lo >= 1e20)
+    return keep
+
+  def _family_vectors(self, b, c, m):
+    """Checks a family batch's b (count, m) and c (count, n) and returns the
+    contiguous presolved b and c to upload, the kept rows and the b of the
+    dropped rows (None if none), the only part of b kept on the host (for
+    the dropped rows' slacks)."""
+    count, n, z = self.count, int(self.n[0]), int(self.z[0])
+    b = np.asarray(b, dtype=np.float64)
+    c = np.asarray(c, dtype=np.float64)
+    if b.shape != (count, m) or c.shape != (count, n):
+      raise ValueError("inconsistent QP shapes")
+    keep = self._family_keep(b, z)
+    if keep.all():
+      return np.ascontiguousarray(b), np.ascontiguousarray(c), keep, None
+    return np.ascontiguousarray(b[:, keep]), np.ascontiguousarray(c), keep, b[:, ~keep].copy()
+
+  def set_bc(self, b, c) -> None:
+    """New right-hand sides for every QP (the next solve re-solves with them):
+    per-QP lists, or (count, m) / (count, n) arrays for a family batch.  The
+    presolve must drop the same rows as at creation."""
+    if getattr(self, "_a_family", None) is not None:
+      bk, ck, keep, b_drop = self._family_vectors(b, c, self._keep.size)
+      if not np.array_equal(keep, self._keep):
+        raise ValueError("the presolve would drop other rows than at creation")
+      self._b_drop = b_drop
+    else:
+      if len(b) != self.count or len(c) != self.count:
+        raise ValueError("one b and one c per QP")
+      bk, ck = [], []
+      for q in range(self.count):
+        bq = np.array(b[q], dtype=np.float64)
+        cq = np.asarray(c[q], dtype=np.float64)
+        ps = self._presolve[q]
+        keep = np.ones(int(self.m[q]), dtype=bool) if ps is None else ps[0]
+        zq = int(self.z[q])
+        if (bq.shape != keep.shape or cq.shape != (int(self.n[q]),)
+            or not np.all(np.isfinite(bq[:zq]))):
+          raise ValueError(f"QP {q}: inconsistent b or c")
+        newkeep = np.ones(bq.size, dtype=bool)
+        newkeep[zq:] = bq[zq:] < 1e20
+        if not np.array_equal(newkeep, keep):
+          raise ValueError(f"QP {q}: the presolve would drop other rows than at creation")
+        if ps is not None:
+          self._presolve[q] = (keep, ps[1], bq[~keep])
+        self._b[q], self._c[q] = bq[keep], cq
+        bk.append(bq[keep])
+        ck.append(cq)
+      bk, ck = np.concatenate(bk), np.concatenate(ck)
+    _check(self._lib, self._lib.qtqp_batch_set_bc(self._h, _ptr(bk, _dp), _ptr(ck, _dp)))
+
   def row_bytes(self) -> np.ndarray:
     """FP64 bytes one pass over each QP's stored row extents reads."""
     out = np.zeros(self.count, dtype=np.int64)
@@ -256,6 +369,17 @@ class BatchQP:
       raise ValueError("start has the wrong packed sizes")
     _check(self._lib, self._lib.qtqp_batch_set_start(self._h, _ptr(x, _dp), _ptr(y, _dp),
                                                      _ptr(s, _dp), _ptr(t, _dp)))
+
+  def device_buffers(self) -> tuple:
+    """Device addresses of the handle's packed x, y, s and tau buffers (a
+    producer on the device may fill them for ``set_start_device``)."""
+    ptrs = [ctypes.c_void_p() for _ in range(4)]
+    _check(self._lib, self._lib.qtqp_batch_device_buffers(self._h, *map(ctypes.byref, ptrs)))
+    return tuple(p.value for p in ptrs)
+
+  def set_start_device(self, x, y, s, tau) -> None:
+    """``set_start`` from device addresses (packed as ``set_start`` takes them)."""
+    _check(self._lib, self._lib.qtqp_batch_set_start_device(self._h, x, y, s, tau))
 
   @staticmethod
   def _opts(*, atol=1e-7, rtol=1e-8, atol_infeas=1e-8, rtol_infeas=1e-9, max_iter=100,
@@ -325,6 +449,17 @@ class BatchQP:
     return {"x": x, "y": y, "s": s, "tau": tau, "status": st, "iters": its}
 
   def results(self) -> list:
+    if getattr(self, "_a_family", None) is not None:
+      ra = self.results_arrays()
+      k = self.count
+      tau, pres, dres, gap = (np.empty(k) for _ in range(4))
+      _check(self._lib, self._lib.qtqp_batch_results(
+          self._h, None, None, None, _ptr(tau, _dp), None, None, _ptr(pres, _dp),
+          _ptr(dres, _dp), _ptr(gap, _dp)))
+      return [BatchSolution(ra["x"][q], ra["y"][q], ra["s"][q],
+                            _STATUS.get(int(ra["status"][q]), SolutionStatus.HIT_MAX_ITER),
+                            int(ra["iters"][q]), float(pres[q]), float(dres[q]), float(gap[q]))
+              for q in range(k)]
     tn, tm = int(self.n.sum()), int(self.m.sum())
     x, y, s = np.empty(tn), np.empty(tm), np.empty(tm)
     tau, pres, dres, gap = (np.empty(self.count) for _ in range(4))
@@ -347,6 +482,47 @@ class BatchQP:
       out.append(BatchSolution(xq, yq, sq, status, int(its[q]), float(pres[q]), float(dres[q]),
                                float(gap[q])))
     return out
+
+  def results_arrays(self) -> dict:
+    """The solutions of ``results()`` for a batch whose QPs all have the same
+    n and m (a family batch), as stacked arrays: x (count, n), y and s
+    (count, m), status codes (QTQP_BATCH_*, 0 for max_iter) and iterations.
+    The status-dependent scaling runs on the device; the presolve's dropped
+    rows are restored here for all QPs at once."""
+    n, m = int(self.n[0]), int(self.m[0])
+    if np.any(self.n != n) or np.any(self.m != m):
+      raise ValueError("results_arrays needs QPs of equal n and m")
+    if getattr(self, "_a_family", None) is None and any(ps is not None for ps in self._presolve):
+      # per-QP presolves may drop different rows: finish QP by QP
+      sols = self.results()
+      out = {k: np.stack([getattr(q, k) for q in sols]) for k in ("x", "y", "s")}
+      inv = {v: k for k, v in _STATUS.items()}
+      out["status"] = np.array([inv.get(q.status, 0) for q in sols], dtype=np.int32)
+      out["iters"] = np.array([q.iters for q in sols], dtype=np.int32)
+      return out
+    k = self.count
+    x, y, s = np.empty((k, n)), np.empty((k, m)), np.empty((k, m))
+    st = np.empty(k, dtype=np.int32)
+    its = np.empty(k, dtype=np.int32)
+    # the status-dependent scaling runs on the device (qtqp_batch_solution)
+    _check(self._lib, self._lib.qtqp_batch_solution(
+        self._h, _ptr(x, _dp), _ptr(y, _dp), _ptr(s, _dp), _ptr(st, _i32p), _ptr(its, _i32p)))
+    if np.any(st == 5):
+      raise np.linalg.LinAlgError(f"QP {int(np.argmax(st == 5))}: FP64 factorisation broke down")
+    inf = st == 2
+    unb = st == 3
+    fin = ~(inf | unb)
+    if getattr(self, "_b_drop", None) is not None:
+      keep = self._keep
+      a_drop = self._a_family[~keep]
+      b_drop = self._b_drop
+      yf = np.empty((k, keep.size))
+      sf = np.empty((k, keep.size))
+      yf[:, keep], sf[:, keep] = y, s
+      yf[:, ~keep] = np.where(unb[:, None], np.nan, 0.0)
+      sf[:, ~keep] = np.where(fin[:, None], b_drop - x @ a_drop.T, np.nan)
+      y, s = yf, sf
+    return {"x": x, "y": y, "s": s, "status": st, "iters": its}
 
   def _finish(self, q, status, x, y, s, tau):
     """Status-dependent scaling and presolve restore (__init__.py:580-614, 272-297)."""

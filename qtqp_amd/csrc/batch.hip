@@ -1898,10 +1898,15 @@ __device__ inline void scale_rows_cols(double* __restrict__ M, float* __restrict
 // Equilibration, scaled copies and start of every QP: a persistent grid
 // whose workgroups take QPs from an atomic queue in `order` (longest first;
 // null: index order), as k_ipm_solve does, so one workgroup does not end up
-// with several of the largest.
+// with several of the largest.  `phase` selects the matrix work (PREP_MATRIX:
+// d, e, the padded, equilibrated and FP32 copies of A and P), the vector work
+// (PREP_VECTORS: scaled b and c, their norms, counters, start) or both.  A
+// family batch, whose QPs share one A and P, runs the matrix work once for
+// QP 0 and then the vector work for every QP.
+enum { PREP_MATRIX = 1, PREP_VECTORS = 2, PREP_ALL = 3 };
 __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, const int* order, int count,
                                                 int* queue, int equilibrate, int default_start,
-                                                int force64) {
+                                                int force64, int phase) {
   // dynamic LDS: prepare_lds_bytes(nmax)
   extern __shared__ double smem[];
   double* red = smem;               // WG + 8
@@ -1916,25 +1921,29 @@ __global__ __launch_bounds__(WG) void k_prepare(Inst* insts, const int* order, i
     const int q = order ? order[t] : t;
     const Inst& I = insts[q];
     const int n = I.n, m = I.m, tid = threadIdx.x;
-    for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
-    for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
-    // unscaled copies with the padded leading dimension (zeros in the pad)
-    const int ld = I.ld, mc = I.mc;
-    const PairRegs pr(I.pm);
-    copy_padded(I.Araw, I.A, mc, n, ld, &pr);
-    copy_padded(I.Praw, I.P, n, n, ld);
-    __syncthreads();
-    if (equilibrate) with_pass_shape(ld, [&](auto sh) { ruiz<decltype(sh)>(I, scratch); });
-    __syncthreads();
-    // D A E and E P E in place, and their FP32 copies (um: d of the stored rows)
-    for (int r = tid; r < mc; r += WG) {
-      int p, q;
-      pr.rows(r, &p, &q);
-      I.um[r] = I.d[p];
+    if (phase & PREP_MATRIX) {
+      for (int i = tid; i < m; i += WG) I.d[i] = 1.0;
+      for (int j = tid; j < n; j += WG) I.e[j] = 1.0;
+      // unscaled copies with the padded leading dimension (zeros in the pad)
+      const int ld = I.ld, mc = I.mc;
+      const PairRegs pr(I.pm);
+      copy_padded(I.Araw, I.A, mc, n, ld, &pr);
+      copy_padded(I.Praw, I.P, n, n, ld);
+      __syncthreads();
+      if (equilibrate) with_pass_shape(ld, [&](auto sh) { ruiz<decltype(sh)>(I, scratch); });
+      __syncthreads();
+      // D A E and E P E in place, and their FP32 copies (um: d of the stored rows)
+      for (int r = tid; r < mc; r += WG) {
+        int p, q;
+        pr.rows(r, &p, &q);
+        I.um[r] = I.d[p];
+      }
+      __syncthreads();
+      scale_rows_cols(I.A, I.A32, I.um, I.e, mc, n, ld);
+      scale_rows_cols(I.P, I.P32, I.e, I.e, n, n, ld);
+      __syncthreads();
     }
-    __syncthreads();
-    scale_rows_cols(I.A, I.A32, I.um, I.e, mc, n, ld);
-    scale_rows_cols(I.P, I.P32, I.e, I.e, n, n, ld);
+    if (!(phase & PREP_VECTORS)) continue;
     double lb = 0.0, lc = 0.0;
     for (int i = tid; i < m; i += WG) {
       I.b[i] = I.braw[i] * I.d[i];
@@ -2010,6 +2019,53 @@ __global__ __launch_bounds__(WG) void k_scatter_start(const Inst* insts, int cou
       I.s[i] = s[m_off[q] + i];
     }
     if (threadIdx.x == 0) I.sd[S_TAU] = tau[q];
+  }
+}
+
+// The solution of each QP as the reference returns it (__init__.py:580-614),
+// from the packed unscaled iterates k_gather wrote (x = e x_eq, y = d y_eq,
+// s = s_eq / d, not yet divided by tau): infeasible: x = s = NaN and y
+// divided by |b'y|; unbounded: y = NaN, x and s divided by |c'x|; otherwise
+// (solved, max_iter, error) x, y, s divided by tau.  b and c are the raw
+// ones of the presolved QP.
+__global__ __launch_bounds__(WG) void k_finish(const Inst* insts, int count, const int64_t* n_off,
+                                               const int64_t* m_off, double* x, double* y,
+                                               double* s, const double* scal, const int* ints) {
+  __shared__ double red[8];
+  for (int q = blockIdx.x; q < count; q += gridDim.x) {
+    const Inst& I = insts[q];
+    const int st = ints[(int64_t)q * I_NINT + I_STATUS];
+    double* xq = x + n_off[q];
+    double* yq = y + m_off[q];
+    double* sq = s + m_off[q];
+    const int n = I.n, m = I.m, tid = threadIdx.x;
+    if (st == QTQP_BATCH_INFEASIBLE) {
+      double acc = 0.0;
+      for (int i = tid; i < m; i += WG) acc += I.braw[i] * yq[i];
+      const double k = fabs(block_sum(acc, red));
+      for (int j = tid; j < n; j += WG) xq[j] = NAN;
+      for (int i = tid; i < m; i += WG) {
+        yq[i] = yq[i] / k;
+        sq[i] = NAN;
+      }
+    } else if (st == QTQP_BATCH_UNBOUNDED) {
+      double acc = 0.0;
+      for (int j = tid; j < n; j += WG) acc += I.craw[j] * xq[j];
+      const double k = fabs(block_sum(acc, red));
+      for (int j = tid; j < n; j += WG) xq[j] = xq[j] / k;
+      for (int i = tid; i < m; i += WG) {
+        yq[i] = NAN;
+        sq[i] = sq[i] / k;
+      }
+    } else {
+      const double tau = scal[(int64_t)q * S_NSCAL + S_TAU];
+      for (int j = tid; j < n; j += WG) xq[j] = xq[j] / tau;
+      for (int i = tid; i < m; i += WG) {
+        yq[i] = yq[i] / tau;
+        sq[i] = sq[i] / tau;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -2106,6 +2162,7 @@ struct qtqp_batch {
   int64_t tot_n = 0, tot_m = 0;
   int64_t *d_noff = nullptr, *d_moff = nullptr;
   double *d_px = nullptr, *d_py = nullptr, *d_ps = nullptr, *d_scal = nullptr;
+  double* d_tau = nullptr;  // packed tau of a start (count values)
   int* d_ints = nullptr;
   size_t smem_iter = 0;
   std::vector<hipEvent_t> iter_ev;  // per-iteration launch timing
@@ -2113,6 +2170,8 @@ struct qtqp_batch {
   std::vector<double> iter_ms;
   unsigned long long* dclk = nullptr;  // per-QP completion clock, [count] = solve start
   std::vector<double> inst_ms;        // per-QP completion time of the last solve
+  bool family = false;  // every QP shares QP 0's A and P (qtqp_batch_create_family)
+  double *d_braw = nullptr, *d_craw = nullptr;  // raw b and c of all QPs, packed by m_off / n_off
 };
 
 namespace {
@@ -2295,8 +2354,10 @@ struct Arena {
   }
 };
 
-size_t inst_bytes(int n, int m, int mc) {
-  const int64_t N = n + m, ld = (n + 3) & ~3;
+// Device bytes of one QP's matrix data (A, P, their copies, extents, d, e),
+// held once per family batch, and of its own vectors and workspace.
+size_t matrix_bytes(int n, int m, int mc) {
+  const int64_t ld = (n + 3) & ~3;
   const int64_t mn = (int64_t)m * ld, mcn = (int64_t)mc * ld, nn = (int64_t)n * ld;
   size_t b = 0;
   b += align256(8 * mn);           // raw A
@@ -2306,10 +2367,16 @@ size_t inst_bytes(int n, int m, int mc) {
   b += align256(4 * ((int64_t)mc / 32 + 4));     // chunk extents
   b += align256(8 * nn) + align256(8 * (nn + a_pad((int)ld)));  // raw P, P and its read slack
   b += align256(4 * nn);           // P32
+  b += align256(8 * m) + align256(8 * n);  // d, e
+  return b;
+}
+
+size_t vector_bytes(int n, int m) {
+  const int64_t N = n + m, nn = (int64_t)n * n;
+  size_t b = 0;
   b += align256(8 * nn);           // G
   b += align256(8 * 2 * 64 * 64 * ((n + 63) / 64));  // diagonal-block inverses
-  b += 2 * align256(8 * m) + 2 * align256(8 * n);  // raw b, c ; (b, c below)
-  b += 2 * align256(8 * m) + 2 * align256(8 * n);  // b, c, d, e
+  b += align256(8 * m) + align256(8 * n);  // b, c
   b += 3 * align256(8 * std::max<int64_t>(m, n));  // x/y/s slack (x n, y m, s m)
   b += 10 * align256(8 * N);                       // kq dg corr krhs sol res iacc ires r xp
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
@@ -2317,26 +2384,34 @@ size_t inst_bytes(int n, int m, int mc) {
   b += align256(8 * (2 * (int64_t)m + 6 * (int64_t)n));  // refinement state
   b += align256(8 * S_NSCAL) + align256(4 * I_NINT);
   b += align256(8 * kTraceFields * kTraceRows);
-  return b + 4096;
+  return b;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* qtqp_batch_last_error(void) { return g_err.c_str(); }
-
-int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
-                      const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
-                      const int64_t* m_off, const double* a, const double* p, const double* b,
-                      const double* c, qtqp_batch** out) {
-  return qtqp_batch_create_ex(count, n, m, z, a_off, p_off, n_off, m_off, a, p, b, c, 0, out);
+// k_prepare over `count` QPs of `insts` (device table) in `order`: the matrix
+// and vector work of each QP, or, for a family batch, the shared matrix work
+// once (QP 0 of the handle's table) and then the vector work of each QP.
+void launch_prepare(const qtqp_batch* h, Inst* insts, const int* order, int count,
+                    int equilibrate, int default_start, int force64, int cus) {
+  const size_t lds = prepare_lds_bytes(h->nmax);
+  const int grid = std::min(count, std::max(1, cus * 2));
+  if (h->family) {
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(WG), lds, h->stream, h->dinst, nullptr, 1,
+                       h->dqueue, equilibrate, default_start, force64, (int)PREP_MATRIX);
+  }
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+  hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(WG), lds, h->stream, insts, order, count,
+                     h->dqueue, equilibrate, default_start, force64,
+                     (int)(h->family ? PREP_VECTORS : PREP_ALL));
 }
 
-int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
-                         const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
-                         const int64_t* m_off, const double* a, const double* p,
-                         const double* b, const double* c, int32_t flags, qtqp_batch** out) {
+// Packs the QPs onto the device.  With `family`, every QP has QP 0's A and
+// P (a_off = p_off = 0): they are uploaded, paired and equilibrated once and
+// every QP's table entry points at that one copy.
+int create_impl(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                const int64_t* m_off, const double* a, const double* p, const double* b,
+                const double* c, int32_t flags, bool family, qtqp_batch** out) {
   if (!out) return fail(QTQP_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (count < 1) return fail(QTQP_ERR_ARG, "count must be >= 1");
@@ -2377,37 +2452,36 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     const bool extents = (flags & QTQP_BATCH_NO_ROW_EXTENTS) == 0;
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::atomic<int> next{0};
+    const int distinct = family ? 1 : count;
     auto work = [&]() {
-      for (int q; (q = next.fetch_add(1)) < count;) {
+      for (int q; (q = next.fetch_add(1)) < distinct;) {
         mcs[q] = find_row_pairs(a + a_off[q], m[q], n[q], pairs, &pmap[q]);
         row_extents(a + a_off[q], n[q], mcs[q], pmap[q], extents, &exts[q]);
       }
     };
     std::vector<std::thread> th;
-    for (int t = 1; t < std::min(nt, count); ++t) th.emplace_back(work);
+    for (int t = 1; t < std::min(nt, distinct); ++t) th.emplace_back(work);
     work();
     for (auto& t : th) t.join();
+    for (int q = distinct; q < count; ++q) {
+      mcs[q] = mcs[0];
+      pmap[q] = pmap[0];
+    }
   }
   h->mc = mcs;
+  h->family = family;
   h->row_bytes.assign(count, 0);
-  for (int q = 0; q < count; ++q)
+  for (int q = 0; q < count; ++q) {
+    const std::vector<uint32_t>& ex = exts[family ? 0 : q];
     for (int r = 0; r < mcs[q]; ++r) {
-      const int lo = exts[q][r] & 0xffff, hi = exts[q][r] >> 16;
+      const int lo = ex[r] & 0xffff, hi = ex[r] >> 16;
       if (hi > lo) h->row_bytes[q] += 16 * ((hi + 1) / 2 - lo / 2);
     }
+  }
   for (int q = 0; q < count; ++q)
     if ((int64_t)mcs[q] * ((n[q] + 3) & ~3) * 8 >= (int64_t)1 << 31)
       return cleanup(fail(QTQP_ERR_ARG, "instance " + std::to_string(q) +
                                             ": A exceeds 2 GiB (batched engine limit)"));
-  size_t total = 0;
-  for (int q = 0; q < count; ++q) total += inst_bytes(n[q], m[q], mcs[q]);
-  void* mem = nullptr;
-  hipError_t err = hipMalloc(&mem, total);
-  if (err != hipSuccess)
-    return cleanup(fail(QTQP_ERR_HIP, std::string("hipMalloc batch arena: ") + hipGetErrorString(err)));
-  h->allocs.push_back(mem);
-  Arena ar{static_cast<unsigned char*>(mem), 0};
-  h->hinst.resize(count);
   int64_t tot_mn = 0, tot_nn = 0, tot_n = 0, tot_m = 0;
   for (int q = 0; q < count; ++q) {
     tot_mn = std::max(tot_mn, a_off[q] + (int64_t)m[q] * n[q]);
@@ -2415,6 +2489,23 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     tot_n = std::max(tot_n, n_off[q] + n[q]);
     tot_m = std::max(tot_m, m_off[q] + m[q]);
   }
+  size_t total = align256(8 * (size_t)std::max<int64_t>(tot_m, 1)) +
+                 align256(8 * (size_t)std::max<int64_t>(tot_n, 1)) + 4096;
+  for (int q = 0; q < count; ++q)
+    total += (family && q > 0 ? 0 : matrix_bytes(n[q], m[q], mcs[q])) + vector_bytes(n[q], m[q]);
+  void* mem = nullptr;
+  hipError_t err = hipMalloc(&mem, total);
+  if (err != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, std::string("hipMalloc batch arena: ") + hipGetErrorString(err)));
+  h->allocs.push_back(mem);
+  Arena ar{static_cast<unsigned char*>(mem), 0};
+  h->hinst.resize(count);
+  // raw b and c of every QP, packed as the caller packs them
+  h->d_braw = ar.take<double>(std::max<int64_t>(tot_m, 1));
+  h->d_craw = ar.take<double>(std::max<int64_t>(tot_n, 1));
+  if (hipMemcpyAsync(h->d_braw, b, 8 * tot_m, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+      hipMemcpyAsync(h->d_craw, c, 8 * tot_n, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+    return cleanup(fail(QTQP_ERR_HIP, "upload failed"));
   for (int q = 0; q < count; ++q) {
     Inst& I = h->hinst[q];
     const int nq = n[q], mq = m[q];
@@ -2427,32 +2518,55 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     I.pm = pmap[q];
     I.ld = (nq + 3) & ~3;
     I.tile = QTQP_BATCH_TILE_64;
-    const int64_t mld = (int64_t)I.mc * I.ld, nld = (int64_t)nq * I.ld;
-    I.has_p = 0;
-    for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
-    double* araw = ar.take<double>(mn);
-    I.Araw = araw;
-    I.A = ar.take<double>(mld + a_pad(I.ld));
-    I.A32 = ar.take<float>(mld);
-    uint32_t* ext = ar.take<uint32_t>((int64_t)I.mc + kExtPad);
-    I.ext = ext;
-    const std::vector<uint32_t> cx = chunk_extents(exts[q], I.mc);
-    uint32_t* cext = ar.take<uint32_t>((int64_t)cx.size());
-    I.cext = cext;
-    double* praw = ar.take<double>(nn);
-    I.Praw = praw;
-    I.P = ar.take<double>(nld + a_pad(I.ld));
-    I.P32 = ar.take<float>(nld);
+    if (family && q > 0) {
+      // QP 0's matrix data
+      const Inst& T = h->hinst[0];
+      I.has_p = T.has_p;
+      I.Araw = T.Araw;
+      I.A = T.A;
+      I.A32 = T.A32;
+      I.ext = T.ext;
+      I.cext = T.cext;
+      I.Praw = T.Praw;
+      I.P = T.P;
+      I.P32 = T.P32;
+      I.d = T.d;
+      I.e = T.e;
+    } else {
+      const int64_t mld = (int64_t)I.mc * I.ld, nld = (int64_t)nq * I.ld;
+      I.has_p = 0;
+      for (int64_t k = 0; k < nn && !I.has_p; ++k) I.has_p = p[p_off[q] + k] != 0.0;
+      double* araw = ar.take<double>(mn);
+      I.Araw = araw;
+      I.A = ar.take<double>(mld + a_pad(I.ld));
+      I.A32 = ar.take<float>(mld);
+      uint32_t* ext = ar.take<uint32_t>((int64_t)I.mc + kExtPad);
+      I.ext = ext;
+      const std::vector<uint32_t> cx = chunk_extents(exts[q], I.mc);
+      uint32_t* cext = ar.take<uint32_t>((int64_t)cx.size());
+      I.cext = cext;
+      double* praw = ar.take<double>(nn);
+      I.Praw = praw;
+      I.P = ar.take<double>(nld + a_pad(I.ld));
+      I.P32 = ar.take<float>(nld);
+      I.d = ar.take<double>(mq);
+      I.e = ar.take<double>(nq);
+      if (hipMemcpyAsync(araw, a + a_off[q], 8 * mn, hipMemcpyHostToDevice, h->stream) !=
+              hipSuccess ||
+          hipMemcpyAsync(praw, p + p_off[q], 8 * nn, hipMemcpyHostToDevice, h->stream) !=
+              hipSuccess ||
+          hipMemcpyAsync(ext, exts[q].data(), 4 * exts[q].size(), hipMemcpyHostToDevice,
+                         h->stream) != hipSuccess ||
+          hipMemcpyAsync(cext, cx.data(), 4 * cx.size(), hipMemcpyHostToDevice, h->stream) !=
+              hipSuccess)
+        return cleanup(fail(QTQP_ERR_HIP, "upload failed"));
+    }
     I.G = ar.take<double>(nn);
     I.LiB = ar.take<double>((int64_t)2 * NB * NB * ((nq + NB - 1) / NB));
-    double* braw = ar.take<double>(mq);
-    double* craw = ar.take<double>(nq);
-    I.braw = braw;
-    I.craw = craw;
+    I.braw = h->d_braw + m_off[q];
+    I.craw = h->d_craw + n_off[q];
     I.b = ar.take<double>(mq);
     I.c = ar.take<double>(nq);
-    I.d = ar.take<double>(mq);
-    I.e = ar.take<double>(nq);
     I.x = ar.take<double>(mx);
     I.y = ar.take<double>(mx);
     I.s = ar.take<double>(mx);
@@ -2476,15 +2590,6 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
     I.si = ar.take<int>(I_NINT);
     I.trace = ar.take<double>(kTraceFields * kTraceRows);
     I.normb = I.normc = 0.0;
-    if (hipMemcpyAsync(araw, a + a_off[q], 8 * mn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(praw, p + p_off[q], 8 * nn, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(braw, b + m_off[q], 8 * mq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(craw, c + n_off[q], 8 * nq, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-        hipMemcpyAsync(ext, exts[q].data(), 4 * exts[q].size(), hipMemcpyHostToDevice,
-                       h->stream) != hipSuccess ||
-        hipMemcpyAsync(cext, cx.data(), 4 * cx.size(), hipMemcpyHostToDevice, h->stream) !=
-            hipSuccess)
-      return cleanup(fail(QTQP_ERR_HIP, "upload failed"));
   }
   if (ar.used > total) return cleanup(fail(QTQP_ERR_HIP, "arena overflow"));
   h->tot_n = tot_n;
@@ -2499,12 +2604,14 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
       hipMalloc(&h->d_py, sizeof(double) * std::max<int64_t>(tot_m, 1)) != hipSuccess ||
       hipMalloc(&h->d_ps, sizeof(double) * std::max<int64_t>(tot_m, 1)) != hipSuccess ||
       hipMalloc(&h->d_scal, sizeof(double) * S_NSCAL * count) != hipSuccess ||
+      hipMalloc(&h->d_tau, sizeof(double) * count) != hipSuccess ||
       hipMalloc(&h->d_ints, sizeof(int) * I_NINT * count) != hipSuccess ||
       hipMalloc(&h->dclk, sizeof(unsigned long long) * (count + 1)) != hipSuccess)
     return cleanup(fail(QTQP_ERR_HIP, "hipMalloc tables"));
   for (void* ptr : {(void*)h->dinst, (void*)h->dorder, (void*)h->dqueue, (void*)h->dcnt,
                     (void*)h->d_noff, (void*)h->d_moff, (void*)h->d_px, (void*)h->d_py,
-                    (void*)h->d_ps, (void*)h->d_scal, (void*)h->d_ints, (void*)h->dclk})
+                    (void*)h->d_ps, (void*)h->d_scal, (void*)h->d_tau, (void*)h->d_ints,
+                    (void*)h->dclk})
     h->allocs.push_back(ptr);
   if (hipMemcpyAsync(h->d_noff, n_off, sizeof(int64_t) * count, hipMemcpyHostToDevice,
                      h->stream) != hipSuccess ||
@@ -2532,21 +2639,78 @@ int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, cons
   return QTQP_OK;
 }
 
+
+}  // namespace
+
+extern "C" {
+
+const char* qtqp_batch_last_error(void) { return g_err.c_str(); }
+
+int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                      const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                      const int64_t* m_off, const double* a, const double* p, const double* b,
+                      const double* c, qtqp_batch** out) {
+  return qtqp_batch_create_ex(count, n, m, z, a_off, p_off, n_off, m_off, a, p, b, c, 0, out);
+}
+
+int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
+                         const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
+                         const int64_t* m_off, const double* a, const double* p,
+                         const double* b, const double* c, int32_t flags, qtqp_batch** out) {
+  return create_impl(count, n, m, z, a_off, p_off, n_off, m_off, a, p, b, c, flags, false, out);
+}
+
+int qtqp_batch_create_family(int32_t count, int32_t n, int32_t m, int32_t z, const double* a,
+                             const double* p, const double* b, const double* c, int32_t flags,
+                             qtqp_batch** out) {
+  if (!out) return fail(QTQP_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (count < 1) return fail(QTQP_ERR_ARG, "count must be >= 1");
+  std::vector<int32_t> nv(count, n), mv(count, m), zv(count, z);
+  std::vector<int64_t> zero(count, 0), no(count), mo(count);
+  for (int q = 0; q < count; ++q) {
+    no[q] = (int64_t)q * n;
+    mo[q] = (int64_t)q * m;
+  }
+  return create_impl(count, nv.data(), mv.data(), zv.data(), zero.data(), zero.data(), no.data(),
+                     mo.data(), a, p, b, c, flags, true, out);
+}
+
+int qtqp_batch_set_bc(qtqp_batch* h, const double* b, const double* c) {
+  if (!h || !b || !c) return fail(QTQP_ERR_ARG, "NULL argument");
+  HIPC(hipMemcpyAsync(h->d_braw, b, 8 * h->tot_m, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipMemcpyAsync(h->d_craw, c, 8 * h->tot_n, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  return QTQP_OK;
+}
+
 int qtqp_batch_set_start(qtqp_batch* h, const double* x, const double* y, const double* s,
                          const double* tau) {
   if (!h || !x || !y || !s || !tau) return fail(QTQP_ERR_ARG, "NULL argument");
-  double* dt = nullptr;
-  HIPC(hipMalloc(&dt, sizeof(double) * h->count));
   HIPC(hipMemcpyAsync(h->d_px, x, 8 * h->tot_n, hipMemcpyHostToDevice, h->stream));
   HIPC(hipMemcpyAsync(h->d_py, y, 8 * h->tot_m, hipMemcpyHostToDevice, h->stream));
   HIPC(hipMemcpyAsync(h->d_ps, s, 8 * h->tot_m, hipMemcpyHostToDevice, h->stream));
-  HIPC(hipMemcpyAsync(dt, tau, 8 * h->count, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipMemcpyAsync(h->d_tau, tau, 8 * h->count, hipMemcpyHostToDevice, h->stream));
+  return qtqp_batch_set_start_device(h, h->d_px, h->d_py, h->d_ps, h->d_tau);
+}
+
+int qtqp_batch_set_start_device(qtqp_batch* h, const double* x, const double* y,
+                                const double* s, const double* tau) {
+  if (!h || !x || !y || !s || !tau) return fail(QTQP_ERR_ARG, "NULL argument");
   hipLaunchKernelGGL(k_scatter_start, dim3(std::min(h->count, 2048)), dim3(WG), 0, h->stream,
-                     h->dinst, h->count, h->d_noff, h->d_moff, h->d_px, h->d_py, h->d_ps, dt);
+                     h->dinst, h->count, h->d_noff, h->d_moff, x, y, s, tau);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(h->stream));
-  HIPC(hipFree(dt));
   h->have_start = true;
+  return QTQP_OK;
+}
+
+int qtqp_batch_device_buffers(qtqp_batch* h, double** x, double** y, double** s, double** tau) {
+  if (!h || !x || !y || !s || !tau) return fail(QTQP_ERR_ARG, "NULL argument");
+  *x = h->d_px;
+  *y = h->d_py;
+  *s = h->d_ps;
+  *tau = h->d_tau;
   return QTQP_OK;
 }
 
@@ -2560,11 +2724,8 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   int dev = 0, cus = 0;
   HIPC(hipGetDevice(&dev));
   HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
-  hipLaunchKernelGGL(k_prepare, dim3(std::min(h->count, std::max(1, cus * 2))), dim3(WG),
-                     prepare_lds_bytes(h->nmax), h->stream, h->dinst, h->dorder, h->count, h->dqueue,
-                     opts->equilibrate ? 1 : 0, h->have_start ? 0 : 1,
-                     o.precision == QTQP_PREC_FP64 ? 1 : 0);
+  launch_prepare(h, h->dinst, h->dorder, h->count, opts->equilibrate ? 1 : 0,
+                 h->have_start ? 0 : 1, o.precision == QTQP_PREC_FP64 ? 1 : 0, cus);
   HIPC(hipGetLastError());
   const int grid = std::min(h->count, std::max(1, cus * 2));
   hipLaunchKernelGGL(k_reset_clock, dim3(std::max(1, (h->count + 1 + WG - 1) / WG)), dim3(WG), 0,
@@ -2712,10 +2873,8 @@ int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t samp
                               h->stream)) != hipSuccess)
         break;
       if ((e = hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream)) != hipSuccess) break;
-      hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
-      hipLaunchKernelGGL(k_prepare, dim3(std::min(k, std::max(1, cus * 2))), dim3(WG),
-                         prepare_lds_bytes(h->nmax), h->stream, tmp, nullptr, k, h->dqueue,
-                         opts->equilibrate ? 1 : 0, 1, o.precision == QTQP_PREC_FP64 ? 1 : 0);
+      launch_prepare(h, tmp, nullptr, k, opts->equilibrate ? 1 : 0, 1,
+                     o.precision == QTQP_PREC_FP64 ? 1 : 0, cus);
       const int grid = std::min(k, std::max(1, cus * 2));
       if ((e = hipEventRecord(h->ev0, h->stream)) != hipSuccess) break;
       for (int it = 0; it < iters; ++it) {
@@ -2782,6 +2941,29 @@ int qtqp_batch_results(qtqp_batch* h, double* x, double* y, double* s, double* t
     if (gap) gap[q] = d[S_GAP];
     if (status) status[q] = iv[I_STATUS];
     if (iters) iters[q] = iv[I_ITERS];
+  }
+  return QTQP_OK;
+}
+
+int qtqp_batch_solution(qtqp_batch* h, double* x, double* y, double* s, int32_t* status,
+                        int32_t* iters) {
+  if (!h) return fail(QTQP_ERR_ARG, "NULL handle");
+  hipLaunchKernelGGL(k_gather, dim3(std::min(h->count, 2048)), dim3(WG), 0, h->stream, h->dinst,
+                     h->count, h->d_noff, h->d_moff, h->d_px, h->d_py, h->d_ps, h->d_scal,
+                     h->d_ints);
+  hipLaunchKernelGGL(k_finish, dim3(std::min(h->count, 2048)), dim3(WG), 0, h->stream, h->dinst,
+                     h->count, h->d_noff, h->d_moff, h->d_px, h->d_py, h->d_ps, h->d_scal,
+                     h->d_ints);
+  HIPC(hipGetLastError());
+  std::vector<int> si((size_t)I_NINT * h->count);
+  if (x) HIPC(hipMemcpyAsync(x, h->d_px, 8 * h->tot_n, hipMemcpyDeviceToHost, h->stream));
+  if (y) HIPC(hipMemcpyAsync(y, h->d_py, 8 * h->tot_m, hipMemcpyDeviceToHost, h->stream));
+  if (s) HIPC(hipMemcpyAsync(s, h->d_ps, 8 * h->tot_m, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipMemcpyAsync(si.data(), h->d_ints, si.size() * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPC(hipStreamSynchronize(h->stream));
+  for (int q = 0; q < h->count; ++q) {
+    if (status) status[q] = si[(size_t)q * I_NINT + I_STATUS];
+    if (iters) iters[q] = si[(size_t)q * I_NINT + I_ITERS];
   }
   return QTQP_OK;
 }

@@ -184,6 +184,48 @@ __global__ void k_gather_rows(const float* __restrict__ X, const float* __restri
   for (int c = threadIdx.x; c < k; c += WG) Tb[(int64_t)r * k + c] = T[src * k + c];
 }
 
+// A starting point of the interior-point method from network outputs O
+// (rows x n_out, n_out = n + 2 m + 2; one workgroup per row): raw = O ts + tm
+// in FP64 (product and sum rounded separately), then x = raw[0:n],
+// y = exp(clip(raw[n:n+m], -700, 700)) and s = exp(clip(raw[n+m:n+2m])) on
+// the inequality rows, y = raw and s = 0 on the z equality rows, and
+// tau = exp(clip(raw[n+2m])); raw[n+2m+1] is log kappa.  A row with any
+// non-finite raw value gets the default start (x = 0, y = s = 1 on the
+// inequality rows, 0 on the equality rows, tau = 1), and valid[r] = 0.
+__global__ __launch_bounds__(WG) void k_decode_start(const float* __restrict__ O, int n_out,
+                                                     const double* __restrict__ tm,
+                                                     const double* __restrict__ ts, int n, int m,
+                                                     int z, double* __restrict__ x,
+                                                     double* __restrict__ y,
+                                                     double* __restrict__ s,
+                                                     double* __restrict__ tau,
+                                                     int* __restrict__ valid) {
+  const int64_t r = blockIdx.x;
+  const float* o = O + r * n_out;
+  auto raw = [&](int k) { return __dadd_rn(__dmul_rn((double)o[k], ts[k]), tm[k]); };
+  int fin = 1;
+  for (int k = threadIdx.x; k < n_out; k += WG) fin &= isfinite(raw(k)) ? 1 : 0;
+  const int ok = __syncthreads_and(fin);
+  auto ex = [](double v) { return exp(fmin(fmax(v, -700.0), 700.0)); };
+  double* xr = x + r * n;
+  double* yr = y + r * m;
+  double* sr = s + r * m;
+  for (int j = threadIdx.x; j < n; j += WG) xr[j] = ok ? raw(j) : 0.0;
+  for (int i = threadIdx.x; i < m; i += WG) {
+    if (i < z) {
+      yr[i] = ok ? raw(n + i) : 0.0;
+      sr[i] = 0.0;
+    } else {
+      yr[i] = ok ? ex(raw(n + i)) : 1.0;
+      sr[i] = ok ? ex(raw(n + m + i)) : 1.0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    tau[r] = ok ? ex(raw(n + 2 * m)) : 1.0;
+    valid[r] = ok;
+  }
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + WG - 1) / WG); }
 
 }  // namespace
@@ -198,6 +240,8 @@ struct qtqp_mlp {
   float *dO = nullptr, *dH = nullptr, *dZ = nullptr;
   double* partial = nullptr;
   float* dloss = nullptr;
+  double *dtm = nullptr, *dts = nullptr;  // output standardisation of qtqp_mlp_propose
+  int* dvalid = nullptr;                  // max_batch flags of qtqp_mlp_propose
   int64_t step = 0;
   // offsets into p/g
   int64_t oW1, ob1, oW2, ob2, oW3, ob3;
@@ -257,7 +301,7 @@ void backward(qtqp_mlp* h, int b) {
 
 void release(qtqp_mlp* h) {
   void* bufs[] = {h->p, h->g, h->m, h->v, h->X, h->T, h->Z1, h->H1, h->Z2, h->H2, h->O, h->dO,
-                  h->dH, h->dZ, h->partial, h->dloss};
+                  h->dH, h->dZ, h->partial, h->dloss, h->dtm, h->dts, h->dvalid};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -309,6 +353,9 @@ int qtqp_mlp_create(int32_t n_in, int32_t hidden, int32_t n_out, int32_t max_bat
   A(&h->dZ, b * hidden);
   if (e == hipSuccess) e = hipMalloc(&h->partial, sizeof(double) * (b * std::max(n_out, hidden) / WG + 2));
   A(&h->dloss, 1);
+  if (e == hipSuccess) e = hipMalloc(&h->dtm, sizeof(double) * n_out);
+  if (e == hipSuccess) e = hipMalloc(&h->dts, sizeof(double) * n_out);
+  if (e == hipSuccess) e = hipMalloc(&h->dvalid, sizeof(int) * b);
   if (e == hipSuccess) e = hipMemset(h->m, 0, sizeof(float) * h->np);
   if (e == hipSuccess) e = hipMemset(h->v, 0, sizeof(float) * h->np);
   if (e != hipSuccess) {
@@ -350,6 +397,32 @@ int qtqp_mlp_forward(qtqp_mlp* h, const float* x, int64_t batch, float* y) {
                         hipMemcpyDeviceToHost, h->stream));
     HIPC(hipStreamSynchronize(h->stream));
   }
+  return 0;
+}
+
+int qtqp_mlp_propose(qtqp_mlp* h, const float* x, int64_t batch, const double* tm,
+                     const double* ts, int32_t n, int32_t m, int32_t z, double* x_start,
+                     double* y_start, double* s_start, double* tau_start, int32_t* valid) {
+  if (!h || !x || !tm || !ts || !x_start || !y_start || !s_start || !tau_start || batch < 0)
+    return fail(1, "bad argument");
+  if (n < 1 || m < 1 || z < 0 || z > m || (int64_t)n + 2 * (int64_t)m + 2 != h->n_out)
+    return fail(1, "n_out must be n + 2 m + 2 with 0 <= z <= m");
+  HIPC(hipMemcpyAsync(h->dtm, tm, sizeof(double) * h->n_out, hipMemcpyHostToDevice, h->stream));
+  HIPC(hipMemcpyAsync(h->dts, ts, sizeof(double) * h->n_out, hipMemcpyHostToDevice, h->stream));
+  for (int64_t r0 = 0; r0 < batch; r0 += h->max_batch) {
+    const int b = (int)std::min<int64_t>(h->max_batch, batch - r0);
+    HIPC(hipMemcpyAsync(h->X, x + r0 * h->n_in, sizeof(float) * b * h->n_in, hipMemcpyHostToDevice,
+                        h->stream));
+    forward(h, b);
+    hipLaunchKernelGGL(k_decode_start, dim3(b), dim3(WG), 0, h->stream, h->O, h->n_out, h->dtm,
+                       h->dts, n, m, z, x_start + r0 * n, y_start + r0 * m, s_start + r0 * m,
+                       tau_start + r0, h->dvalid);
+    HIPC(hipGetLastError());
+    if (valid)
+      HIPC(hipMemcpyAsync(valid + r0, h->dvalid, sizeof(int) * b, hipMemcpyDeviceToHost,
+                          h->stream));
+  }
+  HIPC(hipStreamSynchronize(h->stream));
   return 0;
 }
 

@@ -38,6 +38,10 @@ MLP_SYMBOLS = {
     "qtqp_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
     "qtqp_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, _fp]),
     "qtqp_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int32, _fp, _dp]),
+    "qtqp_mlp_propose": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, _dp, _dp,
+                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "qtqp_mlp_train": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, ctypes.c_int64, ctypes.c_int32,
                                       ctypes.c_int32, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _fp]),
@@ -89,6 +93,19 @@ class MLP:
     self._check(self._lib.qtqp_mlp_forward(self._h, x.ctypes.data_as(_fp), x.shape[0],
                                            y.ctypes.data_as(_fp)))
     return y
+
+  def propose(self, x, tm, ts, n, m, z, out) -> np.ndarray:
+    """Decoded starting points of the IPM for the standardised inputs x,
+    written to the device addresses out = (x, y, s, tau) (packed; see
+    include/qtqp_mlp.h).  Returns the per-row validity flags."""
+    x = _f32(x)
+    tm = np.ascontiguousarray(tm, dtype=np.float64)
+    ts = np.ascontiguousarray(ts, dtype=np.float64)
+    valid = np.empty(x.shape[0], dtype=np.int32)
+    self._check(self._lib.qtqp_mlp_propose(
+        self._h, x.ctypes.data_as(_fp), x.shape[0], tm.ctypes.data_as(_dp),
+        ts.ctypes.data_as(_dp), n, m, z, *out, valid.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return valid.astype(bool)
 
   def loss_grad(self, x, t):
     x, t = _f32(x), _f32(t)
@@ -150,41 +167,98 @@ class Family:
     self.bx = np.stack([q.b - base.b for q in cols], axis=1)
     self.cx = np.stack([q.c - base.c for q in cols], axis=1)
 
+  def bc(self, x0s):
+    """b (count, m) and c (count, n) of the family's QPs at the x0s."""
+    b = x0s @ self.bx.T
+    b += self.b0
+    c = x0s @ self.cx.T
+    c += self.c0
+    return b, c
+
   def qps(self, x0s):
-    b = self.b0[None, :] + x0s @ self.bx.T
-    c = self.c0[None, :] + x0s @ self.cx.T
+    b, c = self.bc(x0s)
     return [(self.a, b[i], c[i], self.z, self.p) for i in range(x0s.shape[0])]
 
 
-def _solve(fam: Family, x0s, *, start=None, max_iter=100, chunk=4096, precision="mixed"):
-  """Batched solves in chunks; returns dict of per-QP arrays (raw iterates)."""
-  n, m = fam.n, fam.m
-  out = {k: [] for k in ("x", "y", "s", "tau", "status", "iters", "sol_x", "sol_y", "sol_s")}
-  t_dev = 0.0
-  for c0 in range(0, x0s.shape[0], chunk):
-    qs = fam.qps(x0s[c0:c0 + chunk])
-    bq = batch.BatchQP(qs)
-    if start is not None:
-      sx, sy, ss, st = (v[c0:c0 + chunk] for v in start)
-      bq.set_start(sx.ravel(), sy.ravel(), ss.ravel(), st)
-    cnt = bq.solve_device(max_iter=max_iter, precision=precision)
-    t_dev += cnt["device_ms"]
-    raw = bq.raw_iterates()
-    sols = bq.results()
-    bq.free()
-    k = len(qs)
-    out["x"].append(raw["x"].reshape(k, n))
-    out["y"].append(raw["y"].reshape(k, m))
-    out["s"].append(raw["s"].reshape(k, m))
-    out["tau"].append(raw["tau"])
-    out["status"].append(raw["status"])
-    out["iters"].append(raw["iters"])
-    out["sol_x"].append(np.stack([s.x for s in sols]))
-    out["sol_y"].append(np.stack([s.y for s in sols]))
-    out["sol_s"].append(np.stack([s.s for s in sols]))
-  res = {k: np.concatenate(v) for k, v in out.items()}
-  res["device_ms"] = t_dev
-  return res
+class FamilySolver:
+  """Batched solves of one Family at many x0: a family batch (A and P held
+  once on the device) of `chunk` QPs stays resident, and each solve uploads
+  only the QPs' b and c (and a start).  A chunk of another size gets a batch
+  of its own."""
+
+  def __init__(self, fam: Family, chunk: int = 16384):
+    self.fam, self.chunk = fam, chunk
+    self._bqs = {}  # resident family batches by QP count
+
+  def _batch(self, x0s):
+    b, c = self.fam.bc(x0s)
+    bq = self._bqs.get(b.shape[0])
+    if bq is not None:
+      bq.set_bc(b, c)
+    else:
+      bq = batch.BatchQP.family(self.fam.a, self.fam.p, self.fam.z, b, c)
+      self._bqs[b.shape[0]] = bq
+    return bq
+
+  def solve(self, x0s, *, start=None, proposer=None, max_iter=100, precision="mixed",
+            raw=False, **opts) -> dict:
+    """Solves the family's QPs at the x0s.  start: host arrays (x, y, s, tau)
+    of every QP, or proposer: a WarmStart whose network writes the start on
+    the device.  Returns per-QP arrays status, iters, sol_x, sol_y, sol_s
+    (the solutions as QTQP.solve returns them), valid (proposals that were
+    used) and, with raw, the iterates x, y, s, tau before the final scaling;
+    plus the summed device_ms."""
+    fam = self.fam
+    keys = ["status", "iters", "sol_x", "sol_y", "sol_s", "valid"]
+    if raw:
+      keys += ["x", "y", "s", "tau"]
+    out = {k: [] for k in keys}
+    t_dev = 0.0
+    for c0 in range(0, x0s.shape[0], self.chunk):
+      xs = x0s[c0:c0 + self.chunk]
+      k = xs.shape[0]
+      bq = self._batch(xs)
+      valid = np.ones(k, dtype=bool)
+      if proposer is not None:
+        valid = proposer.propose_device(xs, bq)
+      elif start is not None:
+        sx, sy, ss, st = (v[c0:c0 + self.chunk] for v in start)
+        bq.set_start(sx.ravel(), sy.ravel(), ss.ravel(), st)
+      cnt = bq.solve_device(max_iter=max_iter, precision=precision, **opts)
+      t_dev += cnt["device_ms"]
+      if raw:
+        r = bq.raw_iterates()
+        out["x"].append(r["x"].reshape(k, fam.n))
+        out["y"].append(r["y"].reshape(k, fam.m))
+        out["s"].append(r["s"].reshape(k, fam.m))
+        out["tau"].append(r["tau"])
+      res = bq.results_arrays()
+      out["status"].append(res["status"])
+      out["iters"].append(res["iters"])
+      out["sol_x"].append(res["x"])
+      out["sol_y"].append(res["y"])
+      out["sol_s"].append(res["s"])
+      out["valid"].append(valid)
+    res = {k: np.concatenate(v) for k, v in out.items()}
+    res["device_ms"] = t_dev
+    return res
+
+  def free(self):
+    for bq in self._bqs.values():
+      bq.free()
+    self._bqs = {}
+
+
+def _solve(fam: Family, x0s, *, start=None, max_iter=100, chunk=16384, precision="mixed",
+           **opts):
+  """One-off batched solves of the family at the x0s (a FamilySolver for this
+  call); returns the raw iterates too."""
+  solver = FamilySolver(fam, chunk)
+  try:
+    return solver.solve(x0s, start=start, max_iter=max_iter, precision=precision, raw=True,
+                        **opts)
+  finally:
+    solver.free()
 
 
 class WarmStart:
@@ -196,7 +270,7 @@ class WarmStart:
     self.mlp = MLP(fam.nx, hidden, self.n_out, max_batch=512, seed=seed)
     self.xm = self.xs = self.tm = self.ts = None
 
-  def targets(self, x0s, chunk=4096):
+  def targets(self, x0s, chunk=16384):
     """Solver iterates after k_star iterations, as (x, log y, log s, log tau, log kappa)."""
     r = _solve(self.fam, x0s, max_iter=self.k_star, chunk=chunk)
     z = self.fam.z
@@ -241,6 +315,18 @@ class WarmStart:
       ws.xm, ws.xs, ws.tm, ws.ts = (d[k].copy() for k in ("xm", "xs", "tm", "ts"))
     return ws
 
+  def propose_device(self, x0s, bq) -> np.ndarray:
+    """Writes the proposed start of each x0 into the family batch bq on the
+    device (qtqp_mlp_propose, then set_start_device): the decoding of
+    ``propose``, with the default start for rows that are not finite.
+    Returns the validity flags."""
+    fam = self.fam
+    bufs = bq.device_buffers()
+    valid = self.mlp.propose((x0s - self.xm) / self.xs, self.tm, self.ts, fam.n, fam.m, fam.z,
+                             bufs)
+    bq.set_start_device(*bufs)
+    return valid
+
   def propose(self, x0s):
     """(x, y, s, tau, kappa, valid) for each x0; unscaled solver coordinates."""
     raw = self.mlp.forward((x0s - self.xm) / self.xs).astype(np.float64) * self.ts + self.tm
@@ -267,28 +353,34 @@ def default_start(fam: Family, count: int):
   return x, y, s, np.ones(count)
 
 
-def solve_with_warm_start(ws: WarmStart, x0s, *, cap: int, chunk=4096):
-  """Warm-started batched solve with the default-start fallback rule."""
+def solve_with_warm_start(ws: WarmStart, x0s, *, cap: int, chunk=16384, solver=None, **opts):
+  """Warm-started batched solve with the default-start fallback rule: the
+  network's proposals are decoded into the batch on the device (invalid ones
+  replaced by the default start), and QPs still unfinished after `cap`
+  iterations are re-solved from the default start.  `solver`: a resident
+  FamilySolver of ws.fam to use (one is made for the call otherwise)."""
   fam = ws.fam
-  x, y, s, tau, _, valid = ws.propose(x0s)
-  dx, dy, ds, dt = default_start(fam, x0s.shape[0])
-  x = np.where(valid[:, None], x, dx)
-  y = np.where(valid[:, None], y, dy)
-  s = np.where(valid[:, None], s, ds)
-  tau = np.where(valid, tau, dt)
+  own = solver is None
+  solver = FamilySolver(fam, chunk) if own else solver
   t0 = time.perf_counter()
-  warm = _solve(fam, x0s, start=(x, y, s, tau), max_iter=cap, chunk=chunk)
-  unfinished = warm["status"] == 0
-  iters = warm["iters"].astype(np.int64).copy()
-  status = warm["status"].copy()
-  sol_x, sol_y, sol_s = warm["sol_x"], warm["sol_y"], warm["sol_s"]
-  device_ms = warm["device_ms"]
-  if unfinished.any():
-    redo = _solve(fam, x0s[unfinished], chunk=chunk)
-    iters[unfinished] += redo["iters"]
-    status[unfinished] = redo["status"]
-    sol_x[unfinished], sol_y[unfinished], sol_s[unfinished] = redo["sol_x"], redo["sol_y"], redo["sol_s"]
-    device_ms += redo["device_ms"]
+  try:
+    warm = solver.solve(x0s, proposer=ws, max_iter=cap, **opts)
+    valid = warm["valid"]
+    unfinished = warm["status"] == 0
+    iters = warm["iters"].astype(np.int64).copy()
+    status = warm["status"].copy()
+    sol_x, sol_y, sol_s = warm["sol_x"], warm["sol_y"], warm["sol_s"]
+    device_ms = warm["device_ms"]
+    if unfinished.any():
+      redo = solver.solve(x0s[unfinished], **opts)
+      iters[unfinished] += redo["iters"]
+      status[unfinished] = redo["status"]
+      sol_x[unfinished], sol_y[unfinished], sol_s[unfinished] = (redo["sol_x"], redo["sol_y"],
+                                                                 redo["sol_s"])
+      device_ms += redo["device_ms"]
+  finally:
+    if own:
+      solver.free()
   return {"iters": iters, "status": status, "fallback": unfinished | ~valid,
           "invalid": ~valid, "x": sol_x, "y": sol_y, "s": sol_s, "device_ms": device_ms,
           "wall_s": time.perf_counter() - t0}

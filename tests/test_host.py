@@ -291,3 +291,19 @@ def test_auto_skips_unavailable_backends_per_problem(monkeypatch):
   assert ls is qtqp_amd.LinearSolver.HIP_DENSE
   ls, _ = qtqp_amd._resolve_linear_solver(qtqp_amd.LinearSolver.AUTO, (300, 200, 100_000))  # pylint: disable=protected-access
   assert ls is qtqp_amd.LinearSolver.HIP_DENSE
+
+
+def test_family_batch_presolve_rows_must_agree():
+  """A family batch drops the rows with b >= 1e20 once for all its QPs: the
+  rows must be the same in every QP, and the reference's input checks apply
+  (__init__.py:248-270)."""
+  from qtqp_amd.batch import BatchQP
+  b = np.array([[1.0, 2.0, np.inf, 3.0], [0.5, 1.0, 1e21, 2.0]])
+  keep = BatchQP._family_keep(b, 1)
+  np.testing.assert_array_equal(keep, [True, True, False, True])
+  with pytest.raises(ValueError, match="same rows"):
+    BatchQP._family_keep(np.array([[1.0, 2.0, np.inf], [1.0, 2.0, 3.0]]), 1)
+  with pytest.raises(ValueError, match="Equality RHS"):
+    BatchQP._family_keep(np.array([[np.inf, 2.0], [1.0, 2.0]]), 1)
+  with pytest.raises(ValueError, match="Inequality RHS"):
+    BatchQP._family_keep(np.array([[1.0, -np.inf], [1.0, 2.0]]), 1)

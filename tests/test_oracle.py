@@ -338,3 +338,35 @@ def test_planted_step_scale_is_caught():
   def bad(good):
     return lambda y, s, dy, ds, z: 0.9 * good(y, s, dy, ds, z)
   assert _fixture_mismatch(O, "step_size", bad)
+
+
+def test_oracle_start_hook():
+  """The oracle's start argument (the warm-start hook of the batched engine):
+  the default point passed explicitly reproduces the default-start solve bit
+  for bit, and a start at the solution (y, s moved off zero by 1e-6, tau = 1)
+  ends, at tight tolerances, in at most half the iterations at the same
+  solution."""
+  qp = problems.random_dense_qp(30, 45, 8, seed=4)
+  ref = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p)
+  m = qp.a.shape[0]
+  y0 = np.zeros(m)
+  s0 = np.zeros(m)
+  y0[qp.z:] = 1.0
+  s0[qp.z:] = 1.0
+  same = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p, start=(np.zeros(qp.a.shape[1]), y0, s0, 1.0))
+  assert same["iters"] == ref["iters"]
+  for k in "xys":
+    np.testing.assert_array_equal(same[k], ref[k])
+  ys = ref["y"].copy()
+  ss = ref["s"].copy()
+  ys[qp.z:] = np.maximum(ys[qp.z:], 1e-6)
+  ss[qp.z:] = np.maximum(ss[qp.z:], 1e-6)
+  tight = dict(atol=1e-10, rtol=1e-10)
+  ref_t = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p, **tight)
+  warm = O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p, start=(ref["x"], ys, ss, 1.0), **tight)
+  assert warm["status"] == "solved"
+  assert warm["iters"] <= ref_t["iters"] // 2, (warm["iters"], ref_t["iters"])
+  for k in "xys":
+    assert _relerr(warm[k], ref_t[k]) < 1e-6, k
+  with pytest.raises(ValueError, match="wrong sizes"):
+    O.solve_qp(qp.a, qp.b, qp.c, qp.z, qp.p, start=(np.zeros(3), y0, s0, 1.0))
