@@ -631,7 +631,8 @@ def main():
   t0 = time.perf_counter()
   tuned = bq.autotune(sample=32, iters=3, precision="mixed")
   tuned["wall_s"] = time.perf_counter() - t0
-  log(f"autotune {tuned['wall_s']:.2f}s: table {tuned['table']}")
+  log(f"autotune {tuned['wall_s']:.2f}s: FP32 {tuned['fp32']['table']}, "
+      f"FP64 {tuned['fp64']['table']}")
   log("warmup")
   for _ in range(args.warmup):
     bq.solve_device(precision="mixed")

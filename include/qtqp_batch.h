@@ -187,28 +187,35 @@ int qtqp_batch_trace(qtqp_batch* h, int32_t q, double* out, int32_t rows);
  * and FP64 triangular factor solves. */
 int qtqp_batch_work(qtqp_batch* h, int32_t* counts);
 
-/* Gram tile shapes of the FP32 factor.  Larger tiles read A fewer times but
- * waste MFMA work on partial tiles; which wins depends on n. */
-#define QTQP_BATCH_TILE_64 0  /* 64 x 64 MFMA output tiles */
-#define QTQP_BATCH_TILE_128 1 /* 128 x 128 MFMA output tiles */
-#define QTQP_BATCH_NTILES 2
+/* Tile shapes of the Gram product G = P + diag + A'WA, the MFMA-heavy part
+ * of each factorisation.  Each tile row of G is computed pass by pass; a
+ * pass reads the tile row's strip of A once for all its tiles.  More tiles
+ * per pass read A fewer times but hold more accumulators in registers;
+ * which wins depends on n and the precision. */
+#define QTQP_BATCH_TILE_64 0   /* 64 x 64 MFMA tiles, two per pass (the default) */
+#define QTQP_BATCH_TILE_128 1  /* 128 x 128 MFMA tiles (FP32 only) */
+#define QTQP_BATCH_TILE_64X4 2 /* 64 x 64 tiles, four per pass */
+#define QTQP_BATCH_TILE_64X1 3 /* 64 x 64 tiles, one per pass */
+#define QTQP_BATCH_NTILES 4
 /* Autotuner size buckets by n: n <= 64, 128, 192, 256, 384, 512, 768, 1024. */
 #define QTQP_BATCH_NBUCKETS 8
 
 /* Tile autotuner (no reference counterpart).  For every size bucket that
- * holds QPs of this batch, times `iters` IPM-iteration launches of up to
- * `sample` of its QPs (spread over the bucket) with each tile shape, keeps
- * the fastest and applies it to every QP of the bucket.  table
- * (QTQP_BATCH_NBUCKETS) receives the choices (-1 for empty buckets), ms
- * (QTQP_BATCH_NBUCKETS x QTQP_BATCH_NTILES) the timings in milliseconds (NaN
- * where not measured); either may be NULL.  Discards a start set with
- * qtqp_batch_set_start.  Tile shapes change no result: each G entry sums the
- * same products in the same order. */
+ * holds QPs of this batch and for each precision (FP32, then FP64), times
+ * `iters` Gram products of up to `sample` of its QPs (spread over the
+ * bucket) in each tile shape, keeps the fastest and applies it to every QP
+ * of the bucket.  table (2 x QTQP_BATCH_NBUCKETS: the FP32 choices, then the
+ * FP64 ones; -1 for empty buckets) and ms (2 x QTQP_BATCH_NBUCKETS x
+ * QTQP_BATCH_NTILES, milliseconds, NaN where not measured) may be NULL.
+ * opts is accepted for the solve options of later calls and not used here.
+ * Discards a start set with qtqp_batch_set_start.  Tile shapes change no
+ * result: each G entry sums the same products in the same order. */
 int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t sample,
                         int32_t iters, int32_t* table, double* ms);
 
-/* Applies a tile table (one QTQP_BATCH_TILE_* per bucket), e.g. a saved
- * autotune result. */
+/* Applies a tile table (2 x QTQP_BATCH_NBUCKETS: one QTQP_BATCH_TILE_* per
+ * bucket for the FP32 Gram product, then for the FP64 one, which may not be
+ * QTQP_BATCH_TILE_128), e.g. a saved autotune result. */
 int qtqp_batch_set_tiles(qtqp_batch* h, const int32_t* table);
 
 void qtqp_batch_free(qtqp_batch* h);

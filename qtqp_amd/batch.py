@@ -396,30 +396,37 @@ class BatchQP:
                      int(bool(equilibrate)), int(bool(per_iteration)))
 
   # Gram tile shapes and autotuner size buckets (include/qtqp_batch.h).
-  TILE_SHAPES = ("64x64", "128x128")
+  TILE_SHAPES = ("64x64 x2", "128x128", "64x64 x4", "64x64 x1")
   BUCKET_EDGES = (64, 128, 192, 256, 384, 512, 768, 1024)
 
   def autotune(self, *, sample: int = 32, iters: int = 3, **opts) -> dict:
-    """Times each Gram tile shape on up to `sample` QPs of every size bucket
-    for `iters` IPM iterations and applies the fastest per bucket.  Returns
-    {"table": shape index per bucket (-1: empty), "ms": bucket x shape
-    timings, "buckets": upper n edges, "shapes": names}.  Discards a start
-    set with set_start."""
+    """Times each Gram tile shape, in FP32 and in FP64, on up to `sample`
+    QPs of every size bucket (`iters` Gram products each) and applies the
+    fastest per bucket and precision.  Returns {"fp32": {"table": shape
+    index per bucket (-1: empty), "ms": bucket x shape timings}, "fp64":
+    {...}, "buckets": upper n edges, "shapes": names}.  Discards a start set
+    with set_start."""
     o = self._opts(**opts)
     nb, nt = len(self.BUCKET_EDGES), len(self.TILE_SHAPES)
-    table = np.zeros(nb, dtype=np.int32)
-    ms = np.zeros(nb * nt)
+    table = np.zeros(2 * nb, dtype=np.int32)
+    ms = np.zeros(2 * nb * nt)
     _check(self._lib, self._lib.qtqp_batch_autotune(self._h, ctypes.byref(o), int(sample),
                                                     int(iters), _ptr(table, _i32p),
                                                     _ptr(ms, _dp)))
-    ms_rows = [[float(v) if np.isfinite(v) else None for v in row] for row in ms.reshape(nb, nt)]
-    return {"table": table.tolist(), "ms": ms_rows,
+    ms = ms.reshape(2, nb, nt)
+    rows = lambda k: [[float(v) if np.isfinite(v) else None for v in row] for row in ms[k]]
+    return {"fp32": {"table": table[:nb].tolist(), "ms": rows(0)},
+            "fp64": {"table": table[nb:].tolist(), "ms": rows(1)},
             "buckets": list(self.BUCKET_EDGES), "shapes": list(self.TILE_SHAPES)}
 
-  def set_tiles(self, table) -> None:
-    """Applies a tile table (one shape index per size bucket)."""
-    t = np.ascontiguousarray(table, dtype=np.int32)
-    assert t.shape == (len(self.BUCKET_EDGES),)
+  def set_tiles(self, table32, table64=None) -> None:
+    """Applies tile tables (one shape index per size bucket) for the FP32 and
+    the FP64 Gram product, e.g. saved autotune results (FP64 default: two
+    64 x 64 tiles per pass)."""
+    nb = len(self.BUCKET_EDGES)
+    t64 = np.zeros(nb, dtype=np.int32) if table64 is None else table64
+    t = np.ascontiguousarray(np.concatenate([table32, t64]), dtype=np.int32)
+    assert t.shape == (2 * nb,)
     _check(self._lib, self._lib.qtqp_batch_set_tiles(self._h, _ptr(t, _i32p)))
 
   def solve_device(self, **opts) -> dict:

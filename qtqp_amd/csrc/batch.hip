@@ -73,7 +73,8 @@ struct Inst {
   const uint32_t* ext;  // nonzero column extent lo | hi << 16 of each stored row
   const uint32_t* cext; // the same for each 32-row chunk of stored rows (their union)
   int ld;    // leading dimension of A, A32, P, P32: n rounded up to 4
-  int tile;  // Gram tile shape (QTQP_BATCH_TILE_*), chosen per size bucket
+  int tile;    // FP32 Gram tile shape (QTQP_BATCH_TILE_*), chosen per size bucket
+  int tile64;  // FP64 Gram tile shape (not QTQP_BATCH_TILE_128)
   // raw data
   const double *Araw, *Praw, *braw, *craw;
   // equilibrated data
@@ -240,10 +241,11 @@ __device__ inline double dot(const double* a, const double* b, int len, double* 
 // both the row strip and the tiles' column strips are read: their indices
 // are listed in `list` (LDS, `cap` ints) first.  The rounds left out add
 // only exact zeros, so every tile gets the sums it would get from all rows.
-template <typename T, int NJ, bool DIAG>
+template <typename T, int NJ, bool DIAG, int KCH_ = RankKTileG<T, 1, 1>::KCH>
 struct GramRow {
   using acc_t = typename Mfma<T>::acc_t;
-  static constexpr int KCH = RankKTileG<T, 1, 1>::KCH;
+  static constexpr int KCH = KCH_;
+  static_assert(KCH == 16 || KCH % 32 == 0, "rounds of 16 rows or whole 32-row chunks");
   static constexpr int VW = 16 / sizeof(T);
   static constexpr int VI = NB / VW;    // vectors per staged row of a strip
   static constexpr int RI = WG / VI;    // rows per pass of the workgroup
@@ -265,7 +267,8 @@ struct GramRow {
     int cnt = nround;
     const bool use_list = cext != nullptr && nround <= cap;
     if (use_list) {
-      constexpr int SUB = KCH / 32;  // 32-row chunks per round
+      // 32-row chunks per round (a 16-row round lies in one chunk)
+      constexpr int SUB = KCH >= 32 ? KCH / 32 : 1;
       const int jlo = j0[0], jhi = j0[0] + NB * NJ;
       if (wave == 0) {
         int base = 0;
@@ -275,7 +278,7 @@ struct GramRow {
           if (c < nround) {
 #pragma unroll
             for (int u = 0; u < SUB; ++u) {
-              const uint32_t e = ldg(cext + SUB * c + u);
+              const uint32_t e = ldg(cext + (KCH >= 32 ? SUB * c + u : c * KCH / 32));
               const int lo = e & 0xffff, hi = e >> 16;
               on = on || (lo < i0 + NB && hi > i0 && lo < jhi && hi > jlo);
             }
@@ -418,17 +421,18 @@ struct GramRow {
   }
 };
 
-// G tiles (i0, j0[t]) = A' W A + (P_off or diag(d_x)) for one GramRow pass
-// (a function of its own, so each pass shape gets its own registers).
-template <typename T, int NJ, bool DIAG>
+// G tiles (i0, j0[t] = j00 + 64 t) = A' W A + (P_off or diag(d_x)) for one
+// GramRow pass (a function of its own, so each pass shape gets its own
+// registers).  DIAG: the last tile is the diagonal one (j00 + 64 (NJ-1) = i0).
+template <typename T, int NJ, bool DIAG, int KCH>
 __device__ __noinline__ void gram_pass(const Inst& I, const T* A, const T* P, T* G,
                                        const double* w, int i0, int j00, T* lds_in) {
   T* lds = in_lds(lds_in);
   const int n = I.n, m = I.mc, ld = I.ld;  // the stored rows of A
   int j0[NJ];
-  j0[0] = j00;
-  if (NJ > 1) j0[NJ - 1] = DIAG ? i0 : j00 + NB;
-  using G_ = GramRow<T, NJ, DIAG>;
+#pragma unroll
+  for (int t = 0; t < NJ; ++t) j0[t] = j00 + t * NB;
+  using G_ = GramRow<T, NJ, DIAG, KCH>;
   G_ g;
   // the round list goes after the staging rows in the LDS union
   constexpr int kUsed = (1 + NJ) * G_::KCH * (NB + 1) * (int)sizeof(T);
@@ -437,57 +441,82 @@ __device__ __noinline__ void gram_pass(const Inst& I, const T* A, const T* P, T*
   g.run(A, ld, m, i0, j0, w, lds, I.cext,
         in_lds(reinterpret_cast<int*>(reinterpret_cast<char*>(lds) + kUsed)), kCap);
   // P_off and the diagonal d_x of every element, loaded before any is used
-  // (clamped addresses, unconditional loads: one memory latency in all)
-  T add[NJ][16];
-#pragma unroll
-  for (int t = 0; t < NJ; ++t) {
-    int e = 0;
-    g.for_each(t, i0, j0[t], [&](int i, int j, T) {
-      const int ic = min(i, n - 1), jc = min(j, n - 1);
-      const T pv = ldg(P + (int64_t)jc * ld + ic);
-      const T dv = (T)ldg(I.dg + ic);
-      add[t][e++] = ic == jc ? dv : pv;
-    });
-  }
+  // (clamped addresses, unconditional loads: one memory latency per group
+  // of EG tiles; EG divides NJ, so every group is whole)
   const __amdgpu_buffer_rsrc_t gr = buffer_of(G, n * n * (int)sizeof(T));
+  constexpr int EG = NJ % 2 == 0 ? 2 : 1;
+  static_assert(NJ % EG == 0, "whole groups of tiles");
 #pragma unroll
-  for (int t = 0; t < NJ; ++t) {
-    int e = 0;
-    g.for_each(t, i0, j0[t], [&](int i, int j, T v) {
-      const T a = add[t][e++];
-      store_if(gr, i < n && j < n && i >= j, (uint32_t)((j * n + i) * (int)sizeof(T)), v + a);
-    });
+  for (int t0 = 0; t0 < NJ; t0 += EG) {
+    T add[EG][16];
+#pragma unroll
+    for (int u = 0; u < EG; ++u) {
+      int e = 0;
+      g.for_each(t0 + u, i0, j0[t0 + u], [&](int i, int j, T) {
+        const int ic = min(i, n - 1), jc = min(j, n - 1);
+        const T pv = ldg(P + (int64_t)jc * ld + ic);
+        const T dv = (T)ldg(I.dg + ic);
+        add[u][e++] = ic == jc ? dv : pv;
+      });
+    }
+#pragma unroll
+    for (int u = 0; u < EG; ++u) {
+      int e = 0;
+      g.for_each(t0 + u, i0, j0[t0 + u], [&](int i, int j, T v) {
+        const T a = add[u][e++];
+        store_if(gr, i < n && j < n && i >= j, (uint32_t)((j * n + i) * (int)sizeof(T)), v + a);
+      });
+    }
   }
 }
 
-// Lower tiles of the Gram block G = P_off + diag(d_x) + A' W A.  MT = 2:
-// 128 x 128 rank-m MFMA tiles (RankKTileG).  MT = 1: 64 x 64 tiles a tile
-// row at a time, two per pass on a shared row strip (GramRow), the
-// diagonal tile reusing the row strip: a tile row of r + 1 tiles reads
-// about 1.5 (r + 1) strips of A instead of 2 (r + 1).
-template <typename T, int MT>
+// The lower tiles of one tile row, NJ off-diagonal tiles per pass and the
+// rest with the diagonal tile in a last pass.
+template <typename T, int NJ, int KCH>
+__device__ inline void gram_rows(const Inst& I, const T* A, const T* P, T* G, const double* w,
+                                 T* lds) {
+  static_assert((1 + NJ) * KCH * (NB + 1) * sizeof(T) <= kLdsU * sizeof(double),
+                "GramRow staging fits the LDS union");
+  const int nt = (I.n + NB - 1) / NB;
+  for (int ti = 0; ti < nt; ++ti) {
+    const int i0 = ti * NB;
+    int tj = 0;
+    for (; tj + NJ <= ti; tj += NJ) gram_pass<T, NJ, false, KCH>(I, A, P, G, w, i0, tj * NB, lds);
+    const int r = ti - tj;  // 0 .. NJ - 1 tiles left before the diagonal
+    if (r == 0) gram_pass<T, 1, true, KCH>(I, A, P, G, w, i0, i0, lds);
+    if constexpr (NJ > 1)
+      if (r == 1) gram_pass<T, 2, true, KCH>(I, A, P, G, w, i0, tj * NB, lds);
+    if constexpr (NJ > 2)
+      if (r == 2) gram_pass<T, 3, true, KCH>(I, A, P, G, w, i0, tj * NB, lds);
+    if constexpr (NJ > 3)
+      if (r == 3) gram_pass<T, 4, true, KCH>(I, A, P, G, w, i0, tj * NB, lds);
+  }
+}
+
+// Lower tiles of the Gram block G = P_off + diag(d_x) + A' W A in the tile
+// shape `tile` (QTQP_BATCH_TILE_*):
+//  - 64 x 64 tiles a tile row at a time (GramRow), one, two or four per pass
+//    on a shared row strip, the diagonal tile reusing the row strip: a tile
+//    row of r + 1 tiles reads about (1 + 1/NJ)(r + 1) strips of A instead of
+//    2 (r + 1), at the cost of NJ tiles of accumulators in registers;
+//  - 128 x 128 rank-m MFMA tiles (RankKTileG; FP32 only, the FP64
+//    accumulators would not fit in registers).
+// Each G entry sums the same products in the same order in every shape.
+template <typename T>
 __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, const double* w,
-                                  Smem& sm) {
+                                  Smem& sm, int tile) {
   const int n = I.n, m = I.mc, ld = I.ld;  // the stored rows of A
-  if constexpr (MT == 1) {
-    static_assert(3 * GramRow<T, 2, false>::KCH * (NB + 1) * sizeof(T) <= kLdsU * sizeof(double),
-                  "GramRow staging fits the LDS union");
-    T* lds = reinterpret_cast<T*>(sm.u);
-    const int nt = (n + NB - 1) / NB;
-    for (int ti = 0; ti < nt; ++ti) {
-      const int i0 = ti * NB;
-      int tj = 0;
-      for (; tj + 1 < ti; tj += 2) gram_pass<T, 2, false>(I, A, P, G, w, i0, tj * NB, lds);
-      if (tj + 1 == ti)
-        gram_pass<T, 2, true>(I, A, P, G, w, i0, tj * NB, lds);
-      else
-        gram_pass<T, 1, true>(I, A, P, G, w, i0, i0, lds);
-    }
-  } else {
-    constexpr int TS = 64 * MT;
+  T* lds = reinterpret_cast<T*>(sm.u);
+  constexpr int K1 = RankKTileG<T, 1, 1>::KCH;  // 32 rows (FP64), 64 (FP32)
+  if (tile == QTQP_BATCH_TILE_64X1) {
+    gram_rows<T, 1, K1>(I, A, P, G, w, lds);
+  } else if (tile == QTQP_BATCH_TILE_64X4) {
+    gram_rows<T, 4, K1 / 2>(I, A, P, G, w, lds);
+  } else if (sizeof(T) == 4 && tile == QTQP_BATCH_TILE_128) {
+    constexpr int MT = 2, TS = 64 * MT;
     const int nt = (n + TS - 1) / TS;
     using Tile = RankKTileG<T, MT, MT>;
-    T* si = reinterpret_cast<T*>(sm.u);
+    T* si = lds;
     T* sj = si + Tile::KCH * (TS + 1);
     for (int t = 0; t < nt * (nt + 1) / 2; ++t) {
       int ti, tj;
@@ -503,6 +532,8 @@ __device__ inline void gram_tiles(const Inst& I, const T* A, const T* P, T* G, c
               v + ((i == j) ? (T)ldg(I.dg + i) : ldg(P + (int64_t)j * ld + i)));
       });
     }
+  } else {
+    gram_rows<T, 2, K1>(I, A, P, G, w, lds);
   }
 }
 
@@ -524,12 +555,7 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
     }
   }
   __syncthreads();
-  // 128 x 128 tiles halve the re-reads of A for large n; they are built for
-  // the FP32 factor only (the FP64 accumulators would not fit in registers).
-  if (sizeof(T) == 4 && I.tile == QTQP_BATCH_TILE_128)
-    gram_tiles<T, 2>(I, A, P, G, I.um, sm);
-  else
-    gram_tiles<T, 1>(I, A, P, G, I.um, sm);
+  gram_tiles<T>(I, A, P, G, I.um, sm, sizeof(T) == 4 ? I.tile : I.tile64);
   __syncthreads();
   // diag(G) += jit_rel * max(diag(G)), jit_rel = 1e-14 (solvers_dense.py:107-112)
   // or larger on a retry after a breakdown
@@ -2139,6 +2165,33 @@ __global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int
   }
 }
 
+// The autotuner's timing kernel: `reps` Gram products of each QP in the
+// queue, in the QP's current tile shape for T.  The KKT diagonal is set to
+// d_x = 1, d_y = -1 first; the next solve rewrites it and G before use.
+template <typename T>
+__global__ __launch_bounds__(WG, 2) void k_gram_bench(const Inst* insts, int count, int* queue,
+                                                      int reps) {
+  Smem sm = lds_smem();
+  __shared__ int s_q;
+  for (;;) {
+    if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
+    __syncthreads();
+    const int q = s_q;
+    __syncthreads();
+    if (q >= count) break;
+    const Inst& I = insts[q];
+    for (int i = threadIdx.x; i < I.n + I.m; i += WG) I.dg[i] = i < I.n ? 1.0 : -1.0;
+    __syncthreads();
+    for (int r = 0; r < reps; ++r) {
+      if (sizeof(T) == 4)
+        gram_syrk<float>(I, I.A32, I.P32, reinterpret_cast<float*>(I.G), sm);
+      else
+        gram_syrk<double>(I, I.A, I.P, I.G, sm);
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -2518,6 +2571,7 @@ int create_impl(int32_t count, const int32_t* n, const int32_t* m, const int32_t
     I.pm = pmap[q];
     I.ld = (nq + 3) & ~3;
     I.tile = QTQP_BATCH_TILE_64;
+    I.tile64 = QTQP_BATCH_TILE_64;
     if (family && q > 0) {
       // QP 0's matrix data
       const Inst& T = h->hinst[0];
@@ -2838,65 +2892,67 @@ int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t samp
                         int32_t iters, int32_t* table, double* ms) {
   if (!h || !opts) return fail(QTQP_ERR_ARG, "NULL argument");
   if (sample < 1 || iters < 1) return fail(QTQP_ERR_ARG, "sample and iters must be >= 1");
-  Opts o = to_opts(opts);
-  o.max_iter = std::max(o.max_iter, iters);
-  std::vector<int32_t> choice(QTQP_BATCH_NBUCKETS, -1);
-  std::vector<double> times((size_t)QTQP_BATCH_NBUCKETS * QTQP_BATCH_NTILES, NAN);
+  constexpr int NBK = QTQP_BATCH_NBUCKETS, NT = QTQP_BATCH_NTILES;
+  std::vector<int32_t> choice(2 * NBK, -1);
+  std::vector<double> times((size_t)2 * NBK * NT, NAN);
   // QPs of each bucket in longest-first order
-  std::vector<std::vector<int>> members(QTQP_BATCH_NBUCKETS);
+  std::vector<std::vector<int>> members(NBK);
   for (int q : h->order) members[size_bucket(h->n[q])].push_back(q);
   Inst* tmp = nullptr;
-  int* tmp_order = nullptr;
   HIPC(hipMalloc(&tmp, sizeof(Inst) * sample));
-  HIPC(hipMalloc(&tmp_order, sizeof(int) * sample));
-  auto release_tmp = [&]() {
-    (void)hipFree(tmp);
-    (void)hipFree(tmp_order);
-  };
-  std::vector<int> ident(sample);
-  std::iota(ident.begin(), ident.end(), 0);
-  hipError_t e = hipMemcpy(tmp_order, ident.data(), sizeof(int) * sample, hipMemcpyHostToDevice);
   int dev = 0, cus = 0;
-  if (e == hipSuccess) e = hipGetDevice(&dev);
+  hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  for (int b = 0; b < QTQP_BATCH_NBUCKETS && e == hipSuccess; ++b) {
+  // the matrix data (equilibrated A and P and their FP32 copies) of every
+  // QP, without touching its iterates
+  if (e == hipSuccess) {
+    launch_prepare(h, h->dinst, h->dorder, h->count, opts->equilibrate ? 1 : 0, 1, 0, cus);
+    e = hipGetLastError();
+  }
+  for (int b = 0; b < NBK && e == hipSuccess; ++b) {
     const std::vector<int>& mem = members[b];
     if (mem.empty()) continue;
     // an evenly spread sample of the bucket (it holds the longest QPs first)
     std::vector<Inst> pick;
     const int k = std::min<int>(sample, (int)mem.size());
     for (int j = 0; j < k; ++j) pick.push_back(h->hinst[mem[(size_t)j * mem.size() / k]]);
-    double best = INFINITY;
-    for (int v = 0; v < QTQP_BATCH_NTILES && e == hipSuccess; ++v) {
-      for (Inst& I : pick) I.tile = v;
-      if ((e = hipMemcpyAsync(tmp, pick.data(), sizeof(Inst) * k, hipMemcpyHostToDevice,
-                              h->stream)) != hipSuccess)
-        break;
-      if ((e = hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream)) != hipSuccess) break;
-      launch_prepare(h, tmp, nullptr, k, opts->equilibrate ? 1 : 0, 1,
-                     o.precision == QTQP_PREC_FP64 ? 1 : 0, cus);
-      const int grid = std::min(k, std::max(1, cus * 2));
-      if ((e = hipEventRecord(h->ev0, h->stream)) != hipSuccess) break;
-      for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
-        hipLaunchKernelGGL(k_ipm_iter, dim3(grid), dim3(WG), h->smem_iter, h->stream, tmp,
-                           tmp_order, k, h->dqueue, o, h->nmax, h->dcnt, nullptr);
-      }
-      if ((e = hipGetLastError()) != hipSuccess) break;
-      if ((e = hipEventRecord(h->ev1, h->stream)) != hipSuccess) break;
-      if ((e = hipEventSynchronize(h->ev1)) != hipSuccess) break;
-      float t = 0.f;
-      if ((e = hipEventElapsedTime(&t, h->ev0, h->ev1)) != hipSuccess) break;
-      times[(size_t)b * QTQP_BATCH_NTILES + v] = t;
-      if (t < best) {
-        best = t;
-        choice[b] = v;
+    const int grid = std::min(k, std::max(1, cus * 2));
+    for (int prec = 0; prec < 2 && e == hipSuccess; ++prec) {
+      double best = INFINITY;
+      for (int v = 0; v < NT && e == hipSuccess; ++v) {
+        if (prec == 1 && v == QTQP_BATCH_TILE_128) continue;  // FP32 only
+        for (Inst& I : pick) (prec == 0 ? I.tile : I.tile64) = v;
+        if ((e = hipMemcpyAsync(tmp, pick.data(), sizeof(Inst) * k, hipMemcpyHostToDevice,
+                                h->stream)) != hipSuccess)
+          break;
+        // one untimed launch (code and data warm), then the timed one
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+          hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(64), 0, h->stream, h->dqueue);
+          if ((e = hipEventRecord(h->ev0, h->stream)) != hipSuccess) break;
+          if (prec == 0)
+            hipLaunchKernelGGL(k_gram_bench<float>, dim3(grid), dim3(WG), 0, h->stream, tmp, k,
+                               h->dqueue, (int)iters);
+          else
+            hipLaunchKernelGGL(k_gram_bench<double>, dim3(grid), dim3(WG), 0, h->stream, tmp, k,
+                               h->dqueue, (int)iters);
+          if ((e = hipGetLastError()) != hipSuccess) break;
+          if ((e = hipEventRecord(h->ev1, h->stream)) != hipSuccess) break;
+          if ((e = hipEventSynchronize(h->ev1)) != hipSuccess) break;
+        }
+        if (e != hipSuccess) break;
+        float t = 0.f;
+        if ((e = hipEventElapsedTime(&t, h->ev0, h->ev1)) != hipSuccess) break;
+        times[((size_t)prec * NBK + b) * NT + v] = t;
+        if (t < best) {
+          best = t;
+          choice[prec * NBK + b] = v;
+        }
       }
     }
   }
-  release_tmp();
+  (void)hipFree(tmp);
   if (e != hipSuccess) return fail(QTQP_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-  // the timing runs re-initialised the iterates of the sampled QPs
+  // the timing runs overwrote the KKT diagonal and G of the sampled QPs
   h->have_start = false;
   std::vector<int32_t> apply(choice);
   for (int32_t& c : apply)
@@ -2905,14 +2961,21 @@ int qtqp_batch_autotune(qtqp_batch* h, const qtqp_batch_opts* opts, int32_t samp
   if (rc) return rc;
   if (table) std::copy(choice.begin(), choice.end(), table);
   if (ms) std::copy(times.begin(), times.end(), ms);
+  (void)opts;
   return QTQP_OK;
 }
 
 int qtqp_batch_set_tiles(qtqp_batch* h, const int32_t* table) {
   if (!h || !table) return fail(QTQP_ERR_ARG, "NULL argument");
-  for (int b = 0; b < QTQP_BATCH_NBUCKETS; ++b)
-    if (table[b] < 0 || table[b] >= QTQP_BATCH_NTILES) return fail(QTQP_ERR_ARG, "unknown tile shape");
-  for (int q = 0; q < h->count; ++q) h->hinst[q].tile = table[size_bucket(h->n[q])];
+  constexpr int NBK = QTQP_BATCH_NBUCKETS;
+  for (int b = 0; b < 2 * NBK; ++b)
+    if (table[b] < 0 || table[b] >= QTQP_BATCH_NTILES ||
+        (b >= NBK && table[b] == QTQP_BATCH_TILE_128))
+      return fail(QTQP_ERR_ARG, "unknown tile shape for its precision");
+  for (int q = 0; q < h->count; ++q) {
+    h->hinst[q].tile = table[size_bucket(h->n[q])];
+    h->hinst[q].tile64 = table[NBK + size_bucket(h->n[q])];
+  }
   HIPC(hipMemcpy(h->dinst, h->hinst.data(), sizeof(Inst) * h->count, hipMemcpyHostToDevice));
   return QTQP_OK;
 }

@@ -124,33 +124,48 @@ def test_batch_input_validation(require_gpu):
     batch.BatchQP([(np.ones((2, 2)), np.ones(3), np.ones(2), 0, None)])
 
 
-def test_tile_shapes_give_identical_results_and_autotune(require_gpu):
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_tile_shapes_give_identical_results_and_autotune(require_gpu, precision):
   """Gram tile shapes change no result (each G entry sums the same products in
-  the same order), and the autotuner fills one choice per occupied bucket."""
+  the same order): every FP32 shape (with the FP64 shape fixed) and every
+  FP64 shape (with the FP32 shape fixed) gives the same iterates bit for bit,
+  and the autotuner fills one choice per occupied bucket and precision."""
   qps = problems.uneven_mpc_batch(40, base_seed=3000)
   bq = batch.BatchQP(qps)
   nb = len(batch.BatchQP.BUCKET_EDGES)
+  fp64_shapes = [0, 2, 3]  # 128x128 (1) is FP32 only
   outs = []
   for shape in range(len(batch.BatchQP.TILE_SHAPES)):
-    bq.set_tiles([shape] * nb)
-    bq.solve_device(precision="mixed")
+    bq.set_tiles([shape] * nb, [0] * nb)
+    bq.solve_device(precision=precision)
     outs.append(bq.raw_iterates())
-  for k in ("x", "y", "s", "tau", "status", "iters"):
-    assert np.array_equal(outs[0][k], outs[1][k]), k
+  for shape in fp64_shapes:
+    bq.set_tiles([0] * nb, [shape] * nb)
+    bq.solve_device(precision=precision)
+    outs.append(bq.raw_iterates())
+  for o in outs[1:]:
+    for k in ("x", "y", "s", "tau", "status", "iters"):
+      assert np.array_equal(outs[0][k], o[k]), k
   tuned = bq.autotune(sample=6, iters=2)
   occupied = {int(np.searchsorted(batch.BatchQP.BUCKET_EDGES, q.n)) for q in qps}
-  for b, choice in enumerate(tuned["table"]):
-    if b in occupied:
-      assert choice in (0, 1)
-      assert all(v is not None and v > 0 for v in tuned["ms"][b])
-    else:
-      assert choice == -1 and tuned["ms"][b] == [None, None]
-  bq.solve_device(precision="mixed")
+  for prec, allowed in (("fp32", (0, 1, 2, 3)), ("fp64", tuple(fp64_shapes))):
+    for b, choice in enumerate(tuned[prec]["table"]):
+      ms = tuned[prec]["ms"][b]
+      if b in occupied:
+        assert choice in allowed
+        assert all(ms[v] is not None and ms[v] > 0 for v in allowed)
+      else:
+        assert choice == -1 and all(v is None for v in ms)
+    if prec == "fp64":
+      assert all(row[1] is None for row in tuned[prec]["ms"])
+  bq.solve_device(precision=precision)
   again = bq.raw_iterates()
   for k in ("x", "y", "s", "status", "iters"):
     assert np.array_equal(outs[0][k], again[k]), k
   with pytest.raises(ValueError):
     bq.set_tiles([5] * nb)
+  with pytest.raises(ValueError):
+    bq.set_tiles([0] * nb, [1] * nb)
   bq.free()
 
 
@@ -176,14 +191,14 @@ def test_row_pass_chunks_and_slot_boundaries(require_gpu, precision):
 
 def test_maximum_size_instance_in_batch(require_gpu):
   """n near the engine's limit (1024) beside small instances: the widest row
-  passes (8 slots of 128 columns), 16 diagonal blocks, both Gram tile shapes."""
+  passes (8 slots of 128 columns), 16 diagonal blocks, every Gram tile shape."""
   qps = [problems.random_dense_qp(1000, 1100, 100, seed=6), problems.mpc_qp(6, 3, 1, seed=7),
          problems.random_dense_qp(70, 90, 10, seed=8)]
   refs = [O.solve_qp(q.a, q.b, q.c, q.z, q.p) for q in qps]
   bq = batch.BatchQP(qps)
   nb = len(batch.BatchQP.BUCKET_EDGES)
-  for shape in (0, 1):
-    bq.set_tiles([shape] * nb)
+  for s32, s64 in ((0, 0), (1, 2), (2, 3), (3, 2)):
+    bq.set_tiles([s32] * nb, [s64] * nb)
     res = bq.solve(precision="mixed")
     for sol, ref in zip(res.solutions, refs):
       _compare(sol, ref, 1e-6)
