@@ -494,21 +494,21 @@ __global__ __launch_bounds__(WG) void k_kmv_finish(int n, int m, const double* _
 // Workgroup "ticket" t (taken with an atomic counter, so every lower ticket
 // belongs to a workgroup that is already running) owns row block I = t
 // (forward) or I = nt-1-t (backward).  It streams the off-diagonal blocks
-// it needs, waiting per block on the flag of the block that produced the
-// matching piece of the solution, then applies the stored inverse of its
-// diagonal block and publishes its piece.
+// it needs, waiting per block for the published piece of the solution that
+// block multiplies, then applies the stored inverse of its diagonal block
+// and publishes its own piece.
 // The sweep state lives on the device, so a launch needs no host-side
 // sequence number and replays from a graph: per direction a ticket counter,
-// a done counter and a generation g.  A launch publishes its blocks' flags
-// as g + 1; the last workgroup to finish resets both counters and advances g,
-// so the next launch waits for g + 2 and never mistakes an old flag for a new
-// one.  Layout of `sync`: [ticket, done, generation] forward, the same
-// backward, then the nt forward and nt backward flags.  A wait is bounded:
-// past the limit the block writes NaN (which the callers' NaN checks report)
-// and still publishes, so every workgroup ends.
+// a done counter and a generation g.  A launch publishes its values with
+// the flag value g + 1; the last workgroup to finish resets both counters
+// and advances g, so the next launch waits for g + 2 and never mistakes an
+// old value for a new one.  Layout of `sync`: [ticket, done, generation]
+// forward, the same backward, two words of padding, then the published
+// values.  A wait is bounded: past the limit the block writes NaN (which the
+// callers' NaN checks report) and still publishes, so every workgroup ends.
 // ---------------------------------------------------------------------------
 constexpr unsigned kSpinLimit = 1u << 21;
-constexpr int kSyncHdr = 6;
+constexpr int kSyncHdr = 8;  // two counter triples, padded to 8-byte words
 
 // Takes this workgroup's ticket and the launch's flag value (generation + 1).
 __device__ inline unsigned take_ticket(unsigned* st, unsigned* sh, unsigned* epoch) {
@@ -532,29 +532,66 @@ __device__ inline void end_sweep(unsigned* st, unsigned nt) {
   }
 }
 
-// Lane 0 of the calling wave spins until *flag == want; returns false on
-// timeout.  The acquire fence makes the producer's stores visible.
-__device__ inline bool wave_wait(const unsigned* flag, unsigned want) {
-  int ok = 1;
-  if ((threadIdx.x & 63) == 0) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
-      if (++spins > kSpinLimit) {
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  ok = __shfl(ok, 0, 64);
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return ok != 0;
+// Published solution values.  Each 32-bit half of a value travels in one
+// 64-bit word beside the launch's flag value (half << 32 | epoch), written
+// and read with single 64-bit atomics: a reader that sees the epoch in a
+// word sees the half written with it, so no fence separates data and flag
+// and one memory round trip moves both.  FP32 values take one word, FP64
+// two.  The words follow the sweep counters in `sync`: 2 n words for the
+// forward sweep's v, then 2 n for the backward sweep's x.
+__host__ __device__ inline int64_t sync_words(int64_t n) {
+  return kSyncHdr + 8 * n;  // unsigned words
+}
+__device__ inline uint64_t* sync_ll(unsigned* sync) {
+  return reinterpret_cast<uint64_t*>(sync + kSyncHdr);
 }
 
-__device__ inline void publish(unsigned* flag, unsigned val) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, val, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ inline void ll_put(uint64_t* w, float v, unsigned ep) {
+  __hip_atomic_store(w, ((uint64_t)__float_as_uint(v) << 32) | ep, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void ll_put(uint64_t* w, double v, unsigned ep) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  __hip_atomic_store(w, (b & 0xffffffff00000000ull) | ep, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w + 1, (b << 32) | ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Reading published values in loops that keep loads in flight: ll_poll
+// issues the reads of a lane's words, ll_ready tests them (lanes without
+// `need` count as ready), ll_value decodes them.  Issuing the first poll of
+// a step before that step's prefetch lets the wait for it leave the
+// prefetch in flight.
+// (ll_poll reads unconditionally: w must be a valid address in every lane.)
+template <typename T>
+__device__ inline void ll_poll(const uint64_t* w, uint64_t (&a)[sizeof(T) / 4]) {
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k)
+    a[k] = __hip_atomic_load(w + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline bool ll_ready(const uint64_t (&a)[sizeof(T) / 4], bool need, unsigned ep) {
+  bool rd = true;
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) rd = rd && (unsigned)a[k] == ep;
+  return rd || !need;
+}
+template <typename T>
+__device__ inline T ll_value(const uint64_t (&a)[sizeof(T) / 4]) {
+  if constexpr (sizeof(T) == 4)
+    return __uint_as_float((unsigned)(a[0] >> 32));
+  else
+    return __longlong_as_double((long long)((a[0] & 0xffffffff00000000ull) | (a[1] >> 32)));
+}
+// Value of lane `lane` (wave-uniform) of v.
+__device__ inline double lane_value(float v, int lane) {
+  return (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Thread (r = lane, q = wave) part of the 64 x 64 block M: M[c*64 + r] for
@@ -581,6 +618,20 @@ __device__ inline double block_gemv(const T* mv, const double* w, double* red) {
   return v;
 }
 
+// block_gemv with the vector held by the lanes of every wave (lane c holds
+// w[c]) instead of in LDS: wave q reads its 16 columns across lanes; the
+// four partial sums meet in red (4 x 64 doubles).  One barrier.
+template <typename T>
+__device__ inline double block_gemv_lanes(const T* mv, double wv, int qs, double* red) {
+  const int r = threadIdx.x & 63;
+  double acc = 0.0;
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) acc += (double)mv[cc] * lane_value(wv, qs * 16 + cc);
+  red[qs * NB + r] = acc;
+  __syncthreads();
+  return (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
+}
+
 // Forward sweep L v = g with g = gbase + A'u from the column partials of nbr
 // row tiles (the right-hand side g = r_x + A'(W r_y) of a K solve, summed as
 // k_colfin sums).  Thread (r = lane, q = wave) accumulates
@@ -592,20 +643,17 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
                                                  const double* __restrict__ cpart, int nbr, T* v,
                                                  unsigned* sync, const Gate* gate, int want) {
   __shared__ unsigned sh_ticket[2];
-  __shared__ double red[4 * NB];
-  __shared__ double w[NB];
-  __shared__ T sv[4][NB];
+  __shared__ double red[4 * NB], red2[4 * NB];
   if (gated_off(gate, want)) return;
   const int nt = gridDim.x;
   unsigned epoch;
   const int I = (int)take_ticket(sync, sh_ticket, &epoch);
-  unsigned* flags = sync + kSyncHdr;
   const int r = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int i0 = I * NB, rows = min(NB, n - i0);
   const bool live = r < rows;
   bool ok = true;
   double acc = 0.0;
-  T cur[16], nxt[16], inv[16];
+  T inv[16];
   // the block inverse and this block's g are needed only after the last
   // wait: fetch them now
   load_block_part(LiB + (int64_t)I * 2 * NB * NB, inv);
@@ -618,35 +666,63 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
     red[q * NB + r] = sq;
   }
   __syncthreads();
-  const double gv = (q == 0 && live)
-                        ? gbase[i0 + r] + ((red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]))
-                        : 0.0;
+  const double gv =
+      live ? gbase[i0 + r] + ((red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r])) : 0.0;
   __syncthreads();
+  // unconditional loads (rows clamped into the block, masked afterwards):
+  // a branch around each load would make every wait drain the prefetch
+  const int rc = min(r, rows - 1);
   auto load = [&](int J, T* dst) {
-    const T* col = L + (int64_t)(J * NB + q * 16) * ld + i0 + r;
+    const T* col = L + (int64_t)(J * NB + q * 16) * ld + i0 + rc;
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) dst[cc] = live ? col[(int64_t)cc * ld] : T(0);
+    for (int cc = 0; cc < 16; ++cc) dst[cc] = col[(int64_t)cc * ld];
   };
-  if (I > 0) load(0, cur);
-  for (int J = 0; J < I; ++J) {
-    if (J + 1 < I) load(J + 1, nxt);
-    ok &= wave_wait(&flags[J], epoch);
-    sv[q][r] = v[J * NB + r];  // J < I: a full block
+  constexpr int K = sizeof(T) / 4;  // published words per value
+  uint64_t* vll = sync_ll(sync);
+  // D blocks of this block row in flight while the chain comes: the wait for
+  // v_J is issued before block J + D's loads (clamped to the last block, so
+  // every step issues the same loads), so it leaves them in flight
+  constexpr int D = 3;
+  T buf[D + 1][16];
+  const int qs = __builtin_amdgcn_readfirstlane(q);
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) acc += (double)cur[cc] * (double)sv[q][q * 16 + cc];
+  for (int d = 0; d < D; ++d)
+    if (I > 0) load(min(d, I - 1), buf[d]);
+  for (int J0 = 0; J0 < I; J0 += D + 1) {
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+    for (int u = 0; u <= D; ++u) {
+      const int J = J0 + u;
+      if (J < I) {
+        uint64_t a[K];
+        ll_poll<T>(vll + K * (J * NB + r), a);  // J < I: a full block
+        load(min(J + D, I - 1), buf[(u + D) % (D + 1)]);
+        if (!__all(ll_ready<T>(a, true, epoch))) {
+          // not published yet: poll again (each retry waits for all loads)
+          unsigned spins = 0;
+          do {
+            if (++spins > kSpinLimit) ok = false;
+            __builtin_amdgcn_s_sleep(1);
+            ll_poll<T>(vll + K * (J * NB + r), a);
+          } while (ok && !__all(ll_ready<T>(a, true, epoch)));
+        }
+        const T vj = ok ? ll_value<T>(a) : T(NAN);
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc)
+          acc += (double)(live ? buf[u][cc] : T(0)) * lane_value(vj, qs * 16 + cc);
+      }
+    }
   }
+  // w = g - (the row sums), formed by every wave; then the diagonal block's
+  // inverse applied with w read across lanes (two barriers in all)
   red[q * NB + r] = acc;
   __syncthreads();
-  if (q == 0) {
-    const double s = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
-    w[r] = live ? gv - s : 0.0;
+  const double sr = (red[r] + red[NB + r]) + (red[2 * NB + r] + red[3 * NB + r]);
+  const double vi = block_gemv_lanes(inv, live ? gv - sr : 0.0, qs, red2);
+  if (q == 0 && live) {
+    const T out = ok ? (T)vi : T(NAN);
+    v[i0 + r] = out;
+    ll_put(vll + K * (i0 + r), out, epoch);
   }
-  __syncthreads();
-  const double vi = block_gemv(inv, w, red);
-  if (q == 0 && live) v[i0 + r] = ok ? (T)vi : T(NAN);
-  publish(&flags[I], epoch);
   end_sweep(sync, (unsigned)nt);
 }
 
@@ -658,61 +734,76 @@ __global__ __launch_bounds__(WG) void k_trsv_bwd(const T* __restrict__ L, int64_
                                                  const T* __restrict__ v, double* x,
                                                  unsigned* sync, const Gate* gate, int want) {
   __shared__ unsigned sh_ticket[2];
-  __shared__ double red[4 * NB];
-  __shared__ double part[NB][NB + 1];
-  __shared__ double w[NB];
+  __shared__ double red[4 * NB], red2[4 * NB];
   if (gated_off(gate, want)) return;
   const int nt = gridDim.x;
   unsigned epoch;
   const int I = nt - 1 - (int)take_ticket(sync + 3, sh_ticket, &epoch);
-  unsigned* flags = sync + kSyncHdr + nt;
   const int rr = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int i0 = I * NB, rows = min(NB, n - i0);
   bool ok = true;
   double acc[16];
 #pragma unroll
   for (int cc = 0; cc < 16; ++cc) acc[cc] = 0.0;
-  T cur[16], nxt[16], inv[16];
+  T inv[16];
   load_block_part(LiB + (int64_t)I * 2 * NB * NB + NB * NB, inv);
+  // this block's right-hand side, fetched now: it is needed only after the
+  // last wait
+  const double vin = (double)v[i0 + min(rr, rows - 1)];
+  // unconditional loads: rows clamped below n, columns clamped into the
+  // block; the terms out of range are masked where they are used
   auto load = [&](int J, T* dst) {
-    const int row = J * NB + rr;
-    const bool in = row < n;
-    const T* base_p = L + (int64_t)(i0 + q * 16) * ld + row;
+    const T* base_p = L + (int64_t)i0 * ld + min(J * NB + rr, n - 1);
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc)
-      dst[cc] = (in && q * 16 + cc < rows) ? base_p[(int64_t)cc * ld] : T(0);
+    for (int cc = 0; cc < 16; ++cc) dst[cc] = base_p[(int64_t)min(q * 16 + cc, rows - 1) * ld];
   };
-  if (I + 1 < nt) load(nt - 1, cur);
-  for (int J = nt - 1; J > I; --J) {
-    if (J - 1 > I) load(J - 1, nxt);
-    ok &= wave_wait(&flags[J], epoch);
-    const int row = J * NB + rr;
-    const double xj = row < n ? x[row] : 0.0;
+  uint64_t* xll = sync_ll(sync) + 2 * (int64_t)n;
+  // blocks J = nt-1, nt-2, ..., I+1 (step s: J = nt-1-s), D of them in
+  // flight, each step's wait issued before its prefetch as in k_trsv_fwd
+  constexpr int D = 3;
+  T buf[D + 1][16];
+  const int nJ = nt - 1 - I;
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) acc[cc] += (double)cur[cc] * xj;
+  for (int d = 0; d < D; ++d)
+    if (nJ > 0) load(nt - 1 - min(d, nJ - 1), buf[d]);
+  for (int s0 = 0; s0 < nJ; s0 += D + 1) {
 #pragma unroll
-    for (int cc = 0; cc < 16; ++cc) cur[cc] = nxt[cc];
+    for (int u = 0; u <= D; ++u) {
+      const int st = s0 + u;
+      if (st < nJ) {
+        const int row = (nt - 1 - st) * NB + rr;
+        const bool need = row < n;
+        const uint64_t* wp = xll + 2 * (int64_t)min(row, n - 1);
+        uint64_t a[2];
+        ll_poll<double>(wp, a);
+        load(nt - 1 - min(st + D, nJ - 1), buf[(u + D) % (D + 1)]);
+        if (!__all(ll_ready<double>(a, need, epoch))) {
+          unsigned spins = 0;
+          do {
+            if (++spins > kSpinLimit) ok = false;
+            __builtin_amdgcn_s_sleep(1);
+            ll_poll<double>(wp, a);
+          } while (ok && !__all(ll_ready<double>(a, need, epoch)));
+        }
+        const double xj = !need ? 0.0 : ok ? ll_value<double>(a) : (double)NAN;
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc)
+          acc[cc] += (double)(q * 16 + cc < rows ? buf[u][cc] : T(0)) * xj;
+      }
+    }
   }
-#pragma unroll
-  for (int cc = 0; cc < 16; ++cc) part[q * 16 + cc][rr] = acc[cc];
+  // column sums over the wave's 64 rows by lane exchanges (no LDS): lane l
+  // ends with column q * 16 + (l >> 2)
+  reduce_rows<16>(acc);
+  if ((rr & 3) == 0) red[q * 16 + (rr >> 2)] = acc[0];
   __syncthreads();
-  {
-    // column c = threadIdx.x / 4 summed by 4 threads over 16 lanes each
-    const int c = threadIdx.x >> 2, k = threadIdx.x & 3;
-    double s = 0.0;
-#pragma unroll
-    for (int t = 0; t < 16; ++t) s += part[c][k * 16 + t];
-    red[k * NB + c] = s;
+  const int qs = __builtin_amdgcn_readfirstlane(q);
+  const double xi = block_gemv_lanes(inv, rr < rows ? vin - red[rr] : 0.0, qs, red2);
+  if (q == 0 && rr < rows) {
+    const double out = ok ? xi : (double)NAN;
+    x[i0 + rr] = out;
+    ll_put(xll + 2 * (int64_t)(i0 + rr), out, epoch);
   }
-  __syncthreads();
-  if (q == 0) {
-    const double s = (red[rr] + red[NB + rr]) + (red[2 * NB + rr] + red[3 * NB + rr]);
-    w[rr] = rr < rows ? (double)v[i0 + rr] - s : 0.0;
-  }
-  __syncthreads();
-  const double xi = block_gemv(inv, w, red);
-  if (q == 0 && rr < rows) x[i0 + rr] = ok ? xi : (double)NAN;
-  publish(&flags[I], epoch);
   end_sweep(sync + 3, (unsigned)nt);
 }
 
@@ -845,7 +936,7 @@ struct qtqp_kkt {
   // row split of the Gram product (gsplit chunks of gchunk rows; 1: none)
   int gsplit = 1, gchunk = 0;
   double* gpart = nullptr;
-  unsigned* sync = nullptr;  // counters and block flags of the triangular sweeps
+  unsigned* sync = nullptr;  // counters and published values of the triangular sweeps
   unsigned long long* dnorm = nullptr;
   // state
   bool have_diag = false, have_factor = false, force64 = false;
@@ -1251,7 +1342,7 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
   ALLOC(tv, double, n);
   const int64_t nt = (n + NB - 1) / NB;
   ALLOC(linv, double, nt * 2 * NB * NB);
-  ALLOC(sync, unsigned, kSyncHdr + 2 * nt);
+  ALLOC(sync, unsigned, sync_words(n));
   {
     // Split the Gram product over the rows of A when its lower tiles alone
     // would leave most of the chip idle: about 512 workgroups, chunks of at
@@ -1308,7 +1399,7 @@ int engine_create(int64_t n, int64_t m, const double* a, const double* p, int32_
     return bad("P");
   }
   if ((e = hipMemsetAsync(h->corr, 0, sizeof(double) * N, h->stream)) != hipSuccess) return bad("corr");
-  if ((e = hipMemsetAsync(h->sync, 0, sizeof(unsigned) * (kSyncHdr + 2 * nt), h->stream)) !=
+  if ((e = hipMemsetAsync(h->sync, 0, sizeof(unsigned) * sync_words(n), h->stream)) !=
       hipSuccess)
     return bad("sync");
   if (cast && cast_fp32(h) != QTQP_OK) {
