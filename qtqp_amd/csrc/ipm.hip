@@ -1316,7 +1316,7 @@ int iterate_eager(qtqp_ipm* h, const qtqp_ipm_opts* o, qtqp_ipm_record* rec) {
   // termination (__init__.py:888-1008)
   if ((rc = tile_pass<double>(k, k->A64, m, n, h->xy, k->rpa, h->xy + n, nullptr))) return rc;
   hipLaunchKernelGGL(k_rowfin, dim3(blocks_for(m)), gb, 0, st, k->rpa, tiles_c(n), m, h->ax);
-  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), gb, 0, st, k->part, tiles_r(m), n, nullptr,
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), gb, 0, st, k->part, tiles_rx(m), n, nullptr,
                      h->aty);
   if (h->has_p)
     hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,
@@ -1354,7 +1354,7 @@ int refine_seq(qtqp_ipm* h, int r, const double* rhs, double* sol, bool both, in
   // res = krhs - K_reg sol + corr * sol and the loop's control after it, with
   // the passes over A and P_off in one launch
   auto residual = [&](int step, const double* rhs_in, double* krhs_out) {
-    const dim3 grid(tiles_r(m) + tiles_r(n), tiles_c(n));
+    const dim3 grid(tiles_rx(m) + tiles_r(n), tiles_c(n));
     if (n % 2 == 0)
       hipLaunchKernelGGL(k_tile_pass_kx<true>, grid, dim3(WG), 0, st, k->A64, k->Poff64, (int)m,
                          (int)n, sol, k->rpa, k->rpp, k->part, step > 0 ? g : nullptr);
@@ -1362,7 +1362,7 @@ int refine_seq(qtqp_ipm* h, int r, const double* rhs, double* sol, bool both, in
       hipLaunchKernelGGL(k_tile_pass_kx<false>, grid, dim3(WG), 0, st, k->A64, k->Poff64, (int)m,
                          (int)n, sol, k->rpa, k->rpp, k->part, step > 0 ? g : nullptr);
     hipLaunchKernelGGL(k_kmv_finish_ctl, dim3(blocks_for(N)), dim3(WG), 0, st, (int)n, (int)m, sol,
-                       k->diag, k->rpp, k->rpa, tiles_c(n), k->part, tiles_r(m), rhs_in, krhs_out,
+                       k->diag, k->rpp, k->rpa, tiles_c(n), k->part, tiles_rx(m), rhs_in, krhs_out,
                        k->corr, k->vres, h->ctl, r, step, max_steps, steps, atol, rtol);
   };
   // the warm start's residual against krhs = [rhs_x; -rhs_y] (written to vin)
@@ -1439,7 +1439,7 @@ int launch_iteration(qtqp_ipm* h, const qtqp_ipm_opts* o, bool both, int steps) 
   // termination (__init__.py:888-1008)
   if ((rc = tile_pass<double>(k, k->A64, m, n, h->xy, k->rpa, h->xy + n, nullptr))) return rc;
   hipLaunchKernelGGL(k_rowfin, dim3(blocks_for(m)), gb, 0, st, k->rpa, tiles_c(n), m, h->ax);
-  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), gb, 0, st, k->part, tiles_r(m), n, nullptr,
+  hipLaunchKernelGGL(k_colfin, dim3(colfin_blocks(n)), gb, 0, st, k->part, tiles_rx(m), n, nullptr,
                      h->aty, nullptr, 0);
   if (h->has_p)
     hipLaunchKernelGGL(k_pmul<1>, dim3(pmul_blocks(n)), gb, 0, st, k->Poff64, h->pdiag, n, h->xy,

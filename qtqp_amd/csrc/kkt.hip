@@ -44,9 +44,9 @@ int scratch_ptr(double** out) {
 
 // Doubles of scratch a product with K of this size takes: row partials of
 // A x (m per 256-column tile) and of P_off x (n per tile), column partials of
-// A'y (n per 64-row tile), and A'y itself.
+// A'y (n per 32-row tile), and A'y itself.
 int64_t noalloc_scratch_doubles(int64_t n, int64_t m) {
-  return (m + n) * tiles_c(n) + (int64_t)tiles_r(m) * n + n;
+  return (m + n) * tiles_c(n) + (int64_t)tiles_rx(m) * n + n;
 }
 
 }  // namespace
@@ -142,12 +142,12 @@ int qtqp_kkt_matvec_noalloc(int64_t n, int64_t m, const double* a, const double*
   const hipStream_t st = static_cast<hipStream_t>(stream);
   double* rpa = sc;                             // m * tiles_c(n)
   double* rpp = rpa + m * tiles_c(n);           // n * tiles_c(n)
-  double* cpart = rpp + n * tiles_c(n);         // tiles_r(m) * n
+  double* cpart = rpp + n * tiles_c(n);         // tiles_rx(m) * n
   if ((rc = tile_pass_on<double>(st, cpart, a, m, n, x, rpa, x + n, nullptr))) return rc;
   if (poff && (rc = tile_pass_on<double>(st, cpart, poff, n, n, x, rpp, nullptr, nullptr)))
     return rc;
   hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(n + m)), dim3(WG), 0, st, (int)n, (int)m, x,
-                     diag, poff ? rpp : nullptr, rpa, tiles_c(n), cpart, tiles_r(m), nullptr,
+                     diag, poff ? rpp : nullptr, rpa, tiles_c(n), cpart, tiles_rx(m), nullptr,
                      nullptr, nullptr, nullptr, out, nullptr, nullptr, 0);
   HIPC(hipGetLastError());
   return QTQP_OK;

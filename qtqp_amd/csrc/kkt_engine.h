@@ -255,9 +255,22 @@ __global__ __launch_bounds__(WG) void k_syrk_trail(T* G, int64_t ld, int n, int 
 // deterministic.
 constexpr int TR = 64;
 constexpr int TC = 256;
+// An FP64 pass that forms both products takes tiles of 32 rows (8 per
+// wave): with 16 rows of four doubles per lane it would hold more than 256
+// registers and run one wave per SIMD.  Its column partials come in
+// tiles_rx(rows) row tiles.
+constexpr int TRX = 32;
 inline int tiles_r(int64_t rows) { return (int)((rows + TR - 1) / TR); }
+inline int tiles_rx(int64_t rows) { return (int)((rows + TRX - 1) / TRX); }
 inline int tiles_c(int64_t ncol) { return (int)((ncol + TC - 1) / TC); }
-__device__ inline int tiles_r_dev(int rows) { return (rows + TR - 1) / TR; }
+__device__ inline int tiles_rx_dev(int rows) { return (rows + TRX - 1) / TRX; }
+template <typename T, bool HX, bool HU>
+constexpr int tile_rows() { return sizeof(T) == 8 && HX && HU ? TRX : TR; }
+// Row tiles of a pass over `rows` rows of an FP64 (dbl) or FP32 matrix
+// forming row products (hx) and/or column products (hu).
+inline int tiles_r_pass(int64_t rows, bool dbl, bool hx, bool hu) {
+  return dbl && hx && hu ? tiles_rx(rows) : tiles_r(rows);
+}
 
 template <typename T> struct Vec4;
 template <> struct Vec4<float> {
@@ -282,42 +295,71 @@ __device__ inline void tile_body(const T* __restrict__ M, int rows, int ncol,
                                  const double* __restrict__ x, double* __restrict__ rpart,
                                  const double* __restrict__ u, const double* __restrict__ w,
                                  double* __restrict__ cpart, int bx, int by, double (*cs)[TC]) {
-  constexpr int RPW = 16;  // rows per wave
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int r0 = bx * TR + wave * RPW;
+  constexpr int RPW = tile_rows<T, HX, HU>() / 4;  // rows per wave
+  // the wave index in a scalar register: the per-row values u[r], w[r] are
+  // then scalar loads, which keeps vector registers for the rows
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r0 = bx * (4 * RPW) + wave * RPW;
   const int j0 = by * TC + 4 * lane;  // this lane's first column
-  const bool full = j0 + 3 < ncol;
   double a[RPW][4];
+  if constexpr (VEC) {
+    // the wave's rows through one buffer resource: rows past the end read
+    // zero without a memory request, and columns past the row end (read
+    // from the next row) are masked, so every load is issued without a
+    // branch around it
+    const int nr = max(0, min(RPW, rows - r0));
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(M + (int64_t)min(r0, rows) * ncol), 0, nr * ncol * (int)sizeof(T),
+        0x00020000);
 #pragma unroll
-  for (int q = 0; q < RPW; ++q) {
-    const int r = r0 + q;
-    const T* row = M + (int64_t)r * ncol + j0;
-    if (VEC && full && r < rows) {
-      Vec4<T>::load(row, a[q]);
-    } else {
+    for (int q = 0; q < RPW; ++q) {
+      const uint32_t off = (uint32_t)((q * ncol + j0) * (int)sizeof(T));
+      double v[4];
+      if constexpr (sizeof(T) == 8) {
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        const D2 v0 = __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        const D2 v1 =
+            __builtin_bit_cast(D2, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
+        v[0] = v0.x;
+        v[1] = v0.y;
+        v[2] = v1.x;
+        v[3] = v1.y;
+      } else {
+        typedef float F4 __attribute__((ext_vector_type(4)));
+        const F4 f = __builtin_bit_cast(F4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        v[0] = f.x;
+        v[1] = f.y;
+        v[2] = f.z;
+        v[3] = f.w;
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a[q][c] = j0 + c < ncol ? v[c] : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+      const int r = r0 + q;
+      const T* row = M + (int64_t)r * ncol + j0;
 #pragma unroll
       for (int c = 0; c < 4; ++c) a[q][c] = (r < rows && j0 + c < ncol) ? (double)row[c] : 0.0;
     }
   }
+  // the row dots and the column sums in one pass over the rows, so the rows
+  // are dead before the row dots are reduced across the lanes (each sum
+  // still adds its terms in row / column order)
+  double xv[4], rd[RPW], cacc[4] = {0.0, 0.0, 0.0, 0.0};
   if (HX) {
-    double xv[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) xv[c] = j0 + c < ncol ? x[j0 + c] : 0.0;
-    double rd[RPW];
+  }
 #pragma unroll
-    for (int q = 0; q < RPW; ++q) {
+  for (int q = 0; q < RPW; ++q) {
+    if (HX) {
       rd[q] = 0.0;
 #pragma unroll
       for (int c = 0; c < 4; ++c) rd[q] += a[q][c] * xv[c];
     }
-    reduce_rows<RPW>(rd);  // lane l: row l >> 2
-    const int q = lane >> 2;
-    if ((lane & 3) == 0 && r0 + q < rows) rpart[(int64_t)by * rows + r0 + q] = rd[0];
-  }
-  if (HU) {
-    double cacc[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < RPW; ++q) {
+    if (HU) {
       const int r = r0 + q;
       double ur = 0.0;
       if (r < rows) {
@@ -327,6 +369,14 @@ __device__ inline void tile_body(const T* __restrict__ M, int rows, int ncol,
 #pragma unroll
       for (int c = 0; c < 4; ++c) cacc[c] += a[q][c] * ur;
     }
+  }
+  if (HX) {
+    reduce_rows<RPW>(rd);  // lane l: row l / (64 / RPW)
+    constexpr int LPR = 64 / RPW;  // lanes per row
+    const int q = lane / LPR;
+    if (lane % LPR == 0 && r0 + q < rows) rpart[(int64_t)by * rows + r0 + q] = rd[0];
+  }
+  if (HU) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) cs[wave][4 * lane + c] = cacc[c];
     __syncthreads();
@@ -351,7 +401,7 @@ __global__ __launch_bounds__(WG) void k_tile_pass(const T* __restrict__ M, int r
 }
 
 // The passes of a K product in one launch: A x_x and A' x_y over the first
-// tiles_r(m) row tiles, P_off x_x over the next tiles_r(n) (P == NULL: none).
+// tiles_rx(m) row tiles, P_off x_x over the next tiles_r(n) (P == NULL: none).
 template <bool VEC>
 __global__ __launch_bounds__(WG) void k_tile_pass_kx(const double* __restrict__ A,
                                                      const double* __restrict__ P, int m, int n,
@@ -362,7 +412,7 @@ __global__ __launch_bounds__(WG) void k_tile_pass_kx(const double* __restrict__ 
                                                      const Gate* gate) {
   __shared__ double cs[4][TC];
   if (gated_off(gate, 0)) return;
-  const int ta = tiles_r_dev(m);
+  const int ta = tiles_rx_dev(m);
   if ((int)blockIdx.x < ta)
     tile_body<double, true, true, VEC>(A, m, n, x, rpa, x + n, nullptr, cpart, blockIdx.x,
                                        blockIdx.y, cs);
@@ -968,7 +1018,7 @@ template <typename T, bool VEC>
 void launch_tile_pass(hipStream_t st, const T* M, int64_t rows, int64_t ncol, const double* x,
                       double* rpart, const double* u, const double* w, double* cpart,
                       const Gate* g, int want) {
-  const dim3 grid(tiles_r(rows), tiles_c(ncol));
+  const dim3 grid(tiles_r_pass(rows, sizeof(T) == 8, x != nullptr, u != nullptr), tiles_c(ncol));
   if (x && u)
     hipLaunchKernelGGL((k_tile_pass<T, true, true, VEC>), grid, dim3(WG), 0, st, M,
                        (int)rows, (int)ncol, x, rpart, u, w, cpart, g, want);
@@ -1015,7 +1065,7 @@ int kmatvec_dev(qtqp_kkt* h, const double* x, const double* rhs, const double* c
   const int64_t n = h->n, m = h->m, N = n + m;
   // A x_x, A' x_y and P_off x_x in one launch: rows of A and P_off start on
   // 16-byte boundaries when n is even (allocations are 256-byte aligned)
-  const dim3 grid(tiles_r(m) + tiles_r(n), tiles_c(n));
+  const dim3 grid(tiles_rx(m) + tiles_r(n), tiles_c(n));
   if (n % 2 == 0)
     hipLaunchKernelGGL(k_tile_pass_kx<true>, grid, dim3(WG), 0, h->stream, h->A64, h->Poff64,
                        (int)m, (int)n, x, h->rpa, h->rpp, h->part, g);
@@ -1023,7 +1073,7 @@ int kmatvec_dev(qtqp_kkt* h, const double* x, const double* rhs, const double* c
     hipLaunchKernelGGL(k_tile_pass_kx<false>, grid, dim3(WG), 0, h->stream, h->A64, h->Poff64,
                        (int)m, (int)n, x, h->rpa, h->rpp, h->part, g);
   hipLaunchKernelGGL(k_kmv_finish, dim3(blocks_for(N)), dim3(WG), 0, h->stream, (int)n, (int)m, x,
-                     h->diag, h->rpp, h->rpa, tiles_c(n), h->part, tiles_r(m), rhs, krhs, rhs_norm,
+                     h->diag, h->rpp, h->rpa, tiles_c(n), h->part, tiles_rx(m), rhs, krhs, rhs_norm,
                      corr, out, norm, g, 0);
   HIPC(hipGetLastError());
   return QTQP_OK;

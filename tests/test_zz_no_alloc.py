@@ -106,7 +106,7 @@ def test_noalloc_matvec_matches_handle_matvec(require_gpu):
 
 def test_noalloc_matvec_rejects_oversized_k(require_gpu):
   lib = _native.kkt_lib()
-  # (n + m) * ceil(n / 256) + ceil(m / 64) * n + n > 2^21 for n = m = 20000
+  # (n + m) * ceil(n / 256) + ceil(m / 32) * n + n > 2^21 for n = m = 20000
   rc = lib.qtqp_kkt_matvec_noalloc(20000, 20000, ctypes.c_void_p(256), None, ctypes.c_void_p(256),
                                    ctypes.c_void_p(256), ctypes.c_void_p(256), None)
   assert rc == _native.QTQP_ERR_ARG
