@@ -633,17 +633,6 @@ __device__ inline T ll_value(const uint64_t (&a)[sizeof(T) / 4]) {
   else
     return __longlong_as_double((long long)((a[0] & 0xffffffff00000000ull) | (a[1] >> 32)));
 }
-// Value of lane `lane` (wave-uniform) of v.
-__device__ inline double lane_value(float v, int lane) {
-  return (double)__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-__device__ inline double lane_value(double v, int lane) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // Thread (r = lane, q = wave) part of the 64 x 64 block M: M[c*64 + r] for
 // its 16 columns c = 16q .. 16q+15.
 template <typename T>
@@ -758,7 +747,7 @@ __global__ __launch_bounds__(WG) void k_trsv_fwd(const T* __restrict__ L, int64_
         const T vj = ok ? ll_value<T>(a) : T(NAN);
 #pragma unroll
         for (int cc = 0; cc < 16; ++cc)
-          acc += (double)(live ? buf[u][cc] : T(0)) * lane_value(vj, qs * 16 + cc);
+          acc += (double)(live ? buf[u][cc] : T(0)) * (double)lane_value(vj, qs * 16 + cc);
       }
     }
   }
