@@ -509,6 +509,62 @@ __device__ inline void load_diag_block(const T* G, int64_t ld, int k0, int nb,
   }
 }
 
+// 1 / sqrt(x): the hardware estimate (v_rsq) refined by two Newton steps
+// for the reciprocal square root, y <- y + y (1 - x y^2) / 2 (one step in
+// FP32).  Within an ulp of the IEEE square root and division, and a much
+// shorter dependency chain than the two of them.
+__device__ inline double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = __builtin_fma(-x * y, y, 1.0);
+  y = __builtin_fma(y * e, 0.5, y);
+  e = __builtin_fma(-x * y, y, 1.0);
+  return __builtin_fma(y * e, 0.5, y);
+}
+__device__ inline float rsqrt_nr(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float e = __builtin_fmaf(-x * y, y, 1.0f);
+  return __builtin_fmaf(y * e, 0.5f, y);
+}
+
+// The value of lane K of each row of 16 lanes, in every lane of that row
+// (DPP row_newbcast): independent broadcasts for the four rows of a wave.
+template <int K>
+__device__ inline float row_bcast(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x150 + K, 0xf, 0xf, false));
+}
+template <int K>
+__device__ inline double row_bcast(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, 0x150 + K, 0xf, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), 0x150 + K, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (uint64_t)hi << 32 | lo);
+}
+// row_bcast with k known once the caller's loop is unrolled.
+template <typename T>
+__device__ inline T row_bcast_k(T v, int k) {
+  switch (k) {
+    case 0: return row_bcast<0>(v);
+    case 1: return row_bcast<1>(v);
+    case 2: return row_bcast<2>(v);
+    case 3: return row_bcast<3>(v);
+    case 4: return row_bcast<4>(v);
+    case 5: return row_bcast<5>(v);
+    case 6: return row_bcast<6>(v);
+    case 7: return row_bcast<7>(v);
+    case 8: return row_bcast<8>(v);
+    case 9: return row_bcast<9>(v);
+    case 10: return row_bcast<10>(v);
+    case 11: return row_bcast<11>(v);
+    case 12: return row_bcast<12>(v);
+    case 13: return row_bcast<13>(v);
+    case 14: return row_bcast<14>(v);
+    default: return row_bcast<15>(v);
+  }
+}
+
 // In-LDS Cholesky of the 64 x 64 diagonal block fused with the inverse of its
 // factor (all workgroup threads).  On entry S holds the block (lower part;
 // padding = identity from load_diag_block).  On return S holds L (lower,
@@ -541,54 +597,58 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
   }
   for (int pb = 0; pb < 4; ++pb) {
     const int c0 = 16 * pb;
-    // lane r (mod 16) holds row r of D and of X = L_d^{-1}; the values of
-    // other rows come from constant lanes by v_readlane (no LDS)
-    T d[16], x[16];
+    // Wave 0 factors the diagonal 16 x 16 block D = L_d L_d': lane r (mod 16)
+    // holds row r, and each finished entry L[k][j] reaches the other lanes
+    // by v_readlane.  The chain of pivots is the critical path (its reciprocal
+    // square roots are rsqrt_nr); the other waves do not repeat it.
+    if (w == 0) {
+      T d[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      d[k] = k <= r ? S[c0 + r][c0 + k] : T(0);
-      x[k] = T(k == r);
-    }
-    T drr = T(1);  // L[r][r]
+      for (int k = 0; k < 16; ++k) d[k] = k <= r ? S[c0 + r][c0 + k] : T(0);
+      T drr = T(1);  // L[r][r]
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      T piv = lane_value(d[j], j);
-      const bool ok = piv > T(0);
-      if (!ok) {
-        piv = T(1);
-        if (w == 0 && lane == 0 && c0 + j < nb) atomicMin(&first_bad, c0 + j);
+      for (int j = 0; j < 16; ++j) {
+        T piv = lane_value(d[j], j);
+        const bool ok = piv > T(0);
+        if (!ok) {
+          piv = T(1);
+          if (lane == 0 && c0 + j < nb) atomicMin(&first_bad, c0 + j);
+        }
+        const T rj = rsqrt_nr(piv);
+        const T dj = piv * rj;
+        const T lrj = r > j ? d[j] * rj : (r == j ? dj : T(0));  // L[r][j]
+        // rows below j lose L[r][j] L[k][j] (entries above row r's diagonal
+        // are never read back)
+#pragma unroll
+        for (int k = j + 1; k < 16; ++k) d[k] -= lrj * lane_value(lrj, k);
+        d[j] = lrj;
+        if (r == j) drr = dj;
       }
-      const T dj = sqrt(piv);
-      const T rj = T(1) / dj;
-      const T lrj = r > j ? d[j] * rj : (r == j ? dj : T(0));  // L[r][j]
-      // rows below j lose L[r][j] L[k][j] (entries above row r's diagonal
-      // are never read back)
+      if (lane < 16) {
 #pragma unroll
-      for (int k = j + 1; k < 16; ++k) d[k] -= lrj * (lane_value(d[j], k) * rj);
-      d[j] = lrj;
-      if (r == j) drr = dj;
-      // L^{-1} by forward elimination on the identity: finish row j, then
-      // remove it from the rows below (row j of X is zero past column j)
-      if (r == j) {
-#pragma unroll
-        for (int c = 0; c <= j; ++c) x[c] *= rj;
-      }
-#pragma unroll
-      for (int c = 0; c <= j; ++c) {
-        const T xjc = lane_value(x[c], j);
-        if (r > j) x[c] -= lrj * xjc;
+        for (int k = 0; k < 16; ++k) S[c0 + r][c0 + k] = k <= r ? d[k] : T(0);
+        sinv[c0 + r] = T(1) / drr;
       }
     }
-    if (w == 0 && lane < 16) {
+    __syncthreads();  // L_d in S, 1 / diag(L_d) in sinv
+    {
+      // X_d = L_d^{-1} by forward substitution, column by column: wave w's
+      // lane row kq takes column 4w + kq, lane r its row r; row k's value
+      // reaches the other lanes of the row by DPP
+      T lr[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) X[c0 + r][c0 + k] = x[k];
-      sinv[c0 + r] = T(1) / drr;
-    }
-    __syncthreads();  // every wave has read D; L_d^{-1} is in X
-    if (w == 0 && lane < 16) {
+      for (int k = 0; k < 16; ++k) lr[k] = S[c0 + r][c0 + k];
+      const int c = 4 * w + kq;
+      T xv = T(r == c);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) S[c0 + r][c0 + k] = k <= r ? d[k] : T(0);
+      for (int k = 0; k < 16; ++k) {
+        if (r == k) xv *= sinv[c0 + k];
+        const T xk = row_bcast_k(xv, k);
+        if (r > k) xv -= lr[k] * xk;
+      }
+      X[c0 + r][c0 + c] = xv;
     }
+    __syncthreads();  // L_d^{-1} in X
     const int nstrip = 3 - pb;
     if (w < nstrip) {
       const int rs = c0 + 16 + 16 * w;
