@@ -598,9 +598,11 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
   for (int pb = 0; pb < 4; ++pb) {
     const int c0 = 16 * pb;
     // Wave 0 factors the diagonal 16 x 16 block D = L_d L_d': lane r (mod 16)
-    // holds row r, and each finished entry L[k][j] reaches the other lanes
-    // by v_readlane.  The chain of pivots is the critical path (its reciprocal
-    // square roots are rsqrt_nr); the other waves do not repeat it.
+    // holds row r, and the pivot and each finished entry L[k][j] reach the
+    // other lanes of the row by DPP (in vector registers: broadcasts through
+    // scalar registers would outnumber them and spill).  The chain of pivots
+    // is the critical path (its reciprocal square roots are rsqrt_nr); the
+    // other waves do not repeat it.
     if (w == 0) {
       T d[16];
 #pragma unroll
@@ -608,7 +610,7 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
       T drr = T(1);  // L[r][r]
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        T piv = lane_value(d[j], j);
+        T piv = row_bcast_k(d[j], j);
         const bool ok = piv > T(0);
         if (!ok) {
           piv = T(1);
@@ -620,7 +622,7 @@ __device__ inline void potrf_inv_blk(T (*S)[NB + 1], T (*X)[NB + 1], T* sinv, in
         // rows below j lose L[r][j] L[k][j] (entries above row r's diagonal
         // are never read back)
 #pragma unroll
-        for (int k = j + 1; k < 16; ++k) d[k] -= lrj * lane_value(lrj, k);
+        for (int k = j + 1; k < 16; ++k) d[k] -= lrj * row_bcast_k(lrj, k);
         d[j] = lrj;
         if (r == j) drr = dj;
       }
