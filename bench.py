@@ -526,14 +526,16 @@ TRAFFIC_FILE = ROOT / "profiles" / "k_ipm_solve_mpc4096_traffic.md"
 
 
 def measured_traffic():
-  """HBM bytes per k_ipm_solve launch from the committed counter summary, or None."""
+  """(HBM bytes per k_ipm_solve launch, the launch time in ms the counters
+  were taken at) from the committed counter summary, or (None, None)."""
   import re
   try:
     txt = TRAFFIC_FILE.read_text()
   except OSError:
-    return None
+    return None, None
   mt = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
-  return float(mt.group(1)) if mt else None
+  ml = re.search(r"traffic_launch_ms:\s*([0-9.eE+]+)", txt)
+  return (float(mt.group(1)) if mt else None), (float(ml.group(1)) if ml else None)
 
 
 def main():
@@ -718,7 +720,12 @@ def main():
     achieved = (flops32 + flops64) / nl / (iter_ms / nl / 1e3) / 1e12
     roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
-  roof["traffic"] = measured_traffic()
+  # the counters come from a separate rocprofv3 run of the same workload,
+  # committed under profiles/; its launch time shows whether it is current
+  roof["traffic"], prof_ms = measured_traffic()
+  roof["traffic_profile_launch_ms"] = prof_ms
+  roof["traffic_profile_current"] = (
+      prof_ms is not None and abs(prof_ms - iter_ms / nl) <= 0.05 * prof_ms)
   roof.update({"kernel": "k_ipm_solve", "launches": nl, "avg_launch_ms": iter_ms / nl,
                "algorithmic_bytes_per_launch": bytes_ / nl,
                "algorithmic_flops_per_launch": (flops32 + flops64) / nl,
