@@ -86,6 +86,14 @@ int qtqp_batch_create(int32_t count, const int32_t* n, const int32_t* m, const i
  * zero without memory traffic: the bound rows of [I; -I] read one 16-byte
  * piece).  Results are those of the full rows.  This flag reads full rows. */
 #define QTQP_BATCH_NO_ROW_EXTENTS 2
+/* By default, once a QP factors in FP64 and n <= 256, each IPM iteration's
+ * two solves that need only the factor -- K^{-1} [c; b] and the
+ * predictor's (__init__.py:482-503) -- refine together: each keeps its own
+ * residuals, tolerance and step count, and every refinement pass while both
+ * still step reads A and P once for both.  Results are bit for bit those of
+ * the two solves one after the other, which this flag runs instead.  No
+ * reference counterpart (the reference solves them in turn). */
+#define QTQP_BATCH_NO_FUSED_SOLVES 4
 int qtqp_batch_create_ex(int32_t count, const int32_t* n, const int32_t* m, const int32_t* z,
                          const int64_t* a_off, const int64_t* p_off, const int64_t* n_off,
                          const int64_t* m_off, const double* a, const double* p,

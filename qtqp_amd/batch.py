@@ -183,13 +183,23 @@ class BatchQP:
   With ``pair_rows`` (default) rows of A that are exact negations of another
   row are stored once on the device (``stored_rows`` gives the count); with
   ``row_extents`` (default) the passes over A read only each row's nonzero
-  column extent (same results as full rows).
+  column extent (same results as full rows).  With ``fused_solves``
+  (default) a QP that factors in FP64 (n <= 256) refines each iteration's
+  K^{-1} [c; b] and predictor solves together, one pass over A and P
+  serving both (same results, bit for bit, as solving them in turn).
   """
 
   NO_ROW_PAIRS = 1
   NO_ROW_EXTENTS = 2
+  NO_FUSED_SOLVES = 4
 
-  def __init__(self, qps: Sequence, pair_rows: bool = True, row_extents: bool = True):
+  @classmethod
+  def _flags(cls, pair_rows, row_extents, fused_solves):
+    return ((0 if pair_rows else cls.NO_ROW_PAIRS) | (0 if row_extents else cls.NO_ROW_EXTENTS)
+            | (0 if fused_solves else cls.NO_FUSED_SOLVES))
+
+  def __init__(self, qps: Sequence, pair_rows: bool = True, row_extents: bool = True,
+               fused_solves: bool = True):
     self._lib = batch_lib()
     self._h = None
     ns, ms, zs, a_parts, p_parts, b_parts, c_parts = [], [], [], [], [], [], []
@@ -240,14 +250,13 @@ class BatchQP:
         self.count, _ptr(self.n, _i32p), _ptr(self.m, _i32p), _ptr(self.z, _i32p),
         _ptr(self.a_off, _i64p), _ptr(self.p_off, _i64p), _ptr(self.n_off, _i64p),
         _ptr(self.m_off, _i64p), _ptr(a_all, _dp), _ptr(p_all, _dp), _ptr(b_all, _dp),
-        _ptr(c_all, _dp), (0 if pair_rows else self.NO_ROW_PAIRS)
-        | (0 if row_extents else self.NO_ROW_EXTENTS), ctypes.byref(h))
+        _ptr(c_all, _dp), self._flags(pair_rows, row_extents, fused_solves), ctypes.byref(h))
     _check(self._lib, rc)
     self._h = h
 
   @classmethod
-  def family(cls, a, p, z: int, b, c, pair_rows: bool = True,
-             row_extents: bool = True) -> "BatchQP":
+  def family(cls, a, p, z: int, b, c, pair_rows: bool = True, row_extents: bool = True,
+             fused_solves: bool = True) -> "BatchQP":
     """count QPs of one parametric family: the same A (m x n), P (n x n or
     None) and z, with b of shape (count, m) and c of shape (count, n).  A and
     P are held once on the device and every QP reads them (results as for
@@ -277,8 +286,7 @@ class BatchQP:
     h = ctypes.c_void_p()
     rc = self._lib.qtqp_batch_create_family(
         count, n, mk, z, _ptr(a_k, _dp), _ptr(p, _dp), _ptr(bk, _dp), _ptr(ck, _dp),
-        (0 if pair_rows else cls.NO_ROW_PAIRS) | (0 if row_extents else cls.NO_ROW_EXTENTS),
-        ctypes.byref(h))
+        cls._flags(pair_rows, row_extents, fused_solves), ctypes.byref(h))
     _check(self._lib, rc)
     self._h = h
     return self

@@ -88,6 +88,10 @@ struct Inst {
   double *um, *un, *ds, *cm;
   double* pkq;  // P kq_x of this iteration's kinv_q
   double* irb;  // refinement state, two copies: A x (m), A'y, P_off x, g (n each)
+  // a second refined solve run together with the first (solve_ir2): its
+  // right-hand side, solution, KKT right-hand side, refinement state,
+  // residuals, correction and A'(W r_y)
+  double *r2, *sol2, *krhs2, *irb2, *res2, *ires2, *iacc2, *um2;
   double* sd;
   int* si;
   double* trace;  // kTraceFields doubles per IPM iteration (max_iter rows)
@@ -96,6 +100,7 @@ struct Inst {
 struct Opts {
   double atol, rtol, atol_inf, rtol_inf, step_scale, min_reg, ls_atol, ls_rtol;
   int max_iter, max_ir, precision, equilibrate;
+  int fuse;  // the q-solve and the predictor's refine together (solve_ir2)
 };
 
 struct Counters {
@@ -907,13 +912,18 @@ constexpr int kExtPad = 256;
 // full row gives.  A stage's extents are loaded D stages ahead of its rows,
 // before the loads of the stage then issued, so waiting for them never waits
 // for rows not yet needed.
-// scratch: >= max(8 ncol, NOUT kPassChunk) doubles of LDS.
+// With NX = 2 the pass forms the dot products with two vectors (xs, xs2) from
+// the same rows: rowepi(i, t1, t2, in, c, out) with c[COLS], COLS <= 4
+// column sums, colepi(j, s[COLS]); each product and sum is formed as the
+// one-vector pass forms it.
+// scratch: >= max(4 COLS ncol, NOUT CHUNK) doubles of LDS.
 template <int NS, int R, int D, int NOUT, int COLS, bool MAXM = false, int CHUNK = kPassChunk,
-          bool EXT = false, typename RL, typename RE, typename RF, typename CE>
+          bool EXT = false, int NX = 1, typename RL, typename RE, typename RF, typename CE>
 __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol, int ld,
                                 const double* __restrict__ xs, RL rowload, RE rowepi, RF rowflush,
                                 CE colepi, double* scratch,
-                                const uint32_t* __restrict__ ext = nullptr) {
+                                const uint32_t* __restrict__ ext = nullptr,
+                                const double* __restrict__ xs2 = nullptr) {
   using V = VecT<double, 2>::type;
   using In = decltype(rowload(0));
   constexpr int LR = R == 8 ? 3 : (R == 4 ? 2 : 1);
@@ -922,21 +932,25 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
   static_assert(CHUNK % ((D + 1) * STEP) == 0, "chunks hold whole stage rotations");
   static_assert(row_overrun<NS, R, D>() <= kPadRows && 128 * NS <= kPadCols, "slack after M");
   static_assert(3 * D * STEP + R <= kExtPad, "extent words read past the last row");
+  static_assert(NX == 1 || NX == 2, "one or two vectors");
+  static_assert(NX == 1 ? COLS <= 2 : (COLS <= 4 && !MAXM), "column sums");
+  constexpr int NC = COLS > 0 ? COLS : 1;
   M = wave_uniform(M);
   nrow = wave_uniform(nrow);
   ld = wave_uniform(ld);
   const int lane = threadIdx.x & 63, wave = wave_uniform((int)(threadIdx.x >> 6));
   const int lrow = lane >> (6 - LR);  // row of the stage whose total this lane holds
-  double xr[NS][2], c1a[NS][2], c2a[NS][2];
+  double xr[NX][NS][2], ca[NC][NS][2];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int col = s * 128 + lane * 2 + e;
       // (a flat load still pending in the loop would make every wait drain)
-      xr[s][e] = col < ncol ? ldg(xs + col) : 0.0;
-      c1a[s][e] = 0.0;
-      c2a[s][e] = 0.0;
+#pragma unroll
+      for (int v = 0; v < NX; ++v) xr[v][s][e] = col < ncol ? ldg((v == 0 ? xs : xs2) + col) : 0.0;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) ca[q][s][e] = 0.0;
     }
   using E = typename ExtVec<R>::type;
   struct Stage {
@@ -985,24 +999,34 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
 #pragma unroll
       for (int s = 0; s < NS; ++s)
         a[r][s] = (i0 + r < nrow && s * 128 + lane * 2 < ld) ? st.a[r][s] : V(0);
-    double d[R];
+    double d[NX][R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      d[r] = 0.0;
+    for (int v = 0; v < NX; ++v)
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        if constexpr (MAXM)
-          d[r] = fmax(d[r], fmax(fabs(a[r][s][0]) * xr[s][0], fabs(a[r][s][1]) * xr[s][1]));
-        else
-          d[r] += a[r][s][0] * xr[s][0] + a[r][s][1] * xr[s][1];
+      for (int r = 0; r < R; ++r) {
+        d[v][r] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          if constexpr (MAXM)
+            d[v][r] = fmax(d[v][r], fmax(fabs(a[r][s][0]) * xr[v][s][0],
+                                         fabs(a[r][s][1]) * xr[v][s][1]));
+          else
+            d[v][r] += a[r][s][0] * xr[v][s][0] + a[r][s][1] * xr[v][s][1];
+        }
       }
-    }
-    reduce_rows<R, MAXM>(d);
+#pragma unroll
+    for (int v = 0; v < NX; ++v) reduce_rows<R, MAXM>(d[v]);
     const int row = i0 + lrow;
-    double c1 = 0.0, c2 = 0.0;
+    double c[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) c[q] = 0.0;
     if (row < nrow) {
       double out[NOUT > 0 ? NOUT : 1];
-      rowepi(row, d[0], st.in, &c1, &c2, out);
+      double unused = 0.0;
+      if constexpr (NX == 1)
+        rowepi(row, d[0][0], st.in, &c[0], NC > 1 ? &c[1] : &unused, out);
+      else
+        rowepi(row, d[0][0], d[NX - 1][0], st.in, c, out);
       if ((lane & ((64 >> LR) - 1)) == 0)
 #pragma unroll
         for (int k = 0; k < NOUT; ++k) scratch[k * CHUNK + row - c0] = out[k];
@@ -1011,17 +1035,18 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         // row r's coefficients, from the first lane of its group
-        const double k1 = lane_value(c1, r << (6 - LR));
-        const double k2 = COLS > 1 ? lane_value(c2, r << (6 - LR)) : 0.0;
+        double kc[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) kc[q] = lane_value(c[q], r << (6 - LR));
 #pragma unroll
         for (int s = 0; s < NS; ++s)
 #pragma unroll
           for (int e = 0; e < 2; ++e) {
             if constexpr (MAXM) {
-              c1a[s][e] = fmax(c1a[s][e], fabs(a[r][s][e]) * k1);
+              ca[0][s][e] = fmax(ca[0][s][e], fabs(a[r][s][e]) * kc[0]);
             } else {
-              c1a[s][e] += a[r][s][e] * k1;
-              if (COLS > 1) c2a[s][e] += a[r][s][e] * k2;
+#pragma unroll
+              for (int q = 0; q < NC; ++q) ca[q][s][e] += a[r][s][e] * kc[q];
             }
           }
       }
@@ -1071,20 +1096,23 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
       for (int e = 0; e < 2; ++e) {
         const int col = s * 128 + lane * 2 + e;
         if (col < ncol) {
-          scratch[wave * ncol + col] = c1a[s][e];
-          if (COLS > 1) scratch[(4 + wave) * ncol + col] = c2a[s][e];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) scratch[(4 * q + wave) * ncol + col] = ca[q][s][e];
         }
       }
     __syncthreads();
     for (int j = threadIdx.x; j < ncol; j += WG) {
-      if constexpr (MAXM)
-        colepi(j, fmax(fmax(scratch[j], scratch[ncol + j]),
-                       fmax(scratch[2 * ncol + j], scratch[3 * ncol + j])), 0.0);
+      double cs[NC];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const double* b = scratch + 4 * q * ncol + j;
+        cs[q] = MAXM ? fmax(fmax(b[0], b[ncol]), fmax(b[2 * ncol], b[3 * ncol]))
+                     : (b[0] + b[ncol]) + (b[2 * ncol] + b[3 * ncol]);
+      }
+      if constexpr (NX == 1)
+        colepi(j, cs[0], NC > 1 ? cs[1] : 0.0);
       else
-        colepi(j, (scratch[j] + scratch[ncol + j]) + (scratch[2 * ncol + j] + scratch[3 * ncol + j]),
-               COLS > 1 ? (scratch[4 * ncol + j] + scratch[5 * ncol + j]) +
-                              (scratch[6 * ncol + j] + scratch[7 * ncol + j])
-                        : 0.0);
+        colepi(j, cs);
     }
     __syncthreads();
   }
@@ -1116,18 +1144,20 @@ struct IrState {
 
 // Fills S1 from S1.sol (init), or sets S1 = S0 advanced by the correction
 // whose x-part is xc:  y_c = W (A x_c - r0_y), sol1 = sol0 + (x_c, y_c),
-// with the products updated incrementally.  One pass over A (both A x and
-// A' y, A' u in it), one over P.  Returns ||r1||_inf (NaN reported as inf).
+// with the products updated incrementally, for the KKT right-hand side
+// krhs; um (n) receives A'(W r1_y).  One pass over A (both A x and A' y,
+// A' u in it), one over P.  Returns ||r1||_inf (NaN reported as inf).
 template <int NS, int R, int D>
 __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const IrState S1,
-                                       const double* xc, bool init, Smem& sm_in) {
+                                       const double* xc, bool init, const double* krhs,
+                                       double* um, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, mc = I.mc;
   const PairRegs pr(I.pm);
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
   const double* dy = wave_uniform(I.dg + n);
-  const double* ky = wave_uniform(I.krhs + n);
+  const double* ky = wave_uniform(krhs + n);
   const double* cy = wave_uniform(I.corr + n);
   const double* r0y = wave_uniform(S0.r + n);
   const double* ax0 = wave_uniform(S0.ax);
@@ -1212,7 +1242,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
       },
       [&](int j, double s1, double s2) {
         S1.aty[j] = init ? s1 : S0.aty[j] + s1;
-        I.um[j] = s2;  // A'(W r1_y)
+        um[j] = s2;  // A'(W r1_y)
       },
       scratch, I.ext);
   if (I.has_p) {
@@ -1247,9 +1277,9 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   for (int j = threadIdx.x; j < n; j += WG) {
     const double xv = init ? S1.sol[j] : S0.sol[j] + xc[j];
     if (!init) S1.sol[j] = xv;
-    const double rx = I.krhs[j] - (S1.px[j] + I.dg[j] * xv + S1.aty[j]) + I.corr[j] * xv;
+    const double rx = krhs[j] - (S1.px[j] + I.dg[j] * xv + S1.aty[j]) + I.corr[j] * xv;
     S1.r[j] = rx;
-    S1.g[j] = rx + I.um[j];
+    S1.g[j] = rx + um[j];
     const double a = fabs(rx);
     mx = fmax(mx, (a == a) ? a : INFINITY);
   }
@@ -1257,12 +1287,206 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
 }
 
 __device__ inline double ir_pass_any(const Inst& I, const IrState& S0, const IrState& S1,
-                                     const double* xc, bool init, Smem& sm) {
+                                     const double* xc, bool init, const double* krhs, double* um,
+                                     Smem& sm) {
   // R rows x NS 128-column slots = 8 vectors per stage; D stages in flight
-  if (I.ld <= 128) return ir_pass<1, 8, 2>(I, S0, S1, xc, init, sm);
-  if (I.ld <= 256) return ir_pass<2, 4, 2>(I, S0, S1, xc, init, sm);
-  if (I.ld <= 512) return ir_pass<4, 2, 2>(I, S0, S1, xc, init, sm);
-  return ir_pass<8, 2, 1>(I, S0, S1, xc, init, sm);
+  if (I.ld <= 128) return ir_pass<1, 8, 2>(I, S0, S1, xc, init, krhs, um, sm);
+  if (I.ld <= 256) return ir_pass<2, 4, 2>(I, S0, S1, xc, init, krhs, um, sm);
+  if (I.ld <= 512) return ir_pass<4, 2, 2>(I, S0, S1, xc, init, krhs, um, sm);
+  return ir_pass<8, 2, 1>(I, S0, S1, xc, init, krhs, um, sm);
+}
+
+// Widest rows the two-solve pass takes: wider rows would need more
+// registers than a wave has (two solves' x vectors and column sums beside the
+// staged rows), and the spills' waits would stall the row pipeline.
+constexpr int kPass2MaxLd = 256;
+// Rows a chunk of the two-solve pass stages (ten outputs per row).
+constexpr int kPair2Chunk = 768;
+
+struct Res2 {
+  double a, b;
+};
+
+// ir_pass for two refined solves at once: (A0, A1, xca) with the KKT
+// right-hand side I.krhs and A'(W r_y) into I.um, and (B0, B1, xcb) with
+// I.krhs2 and I.um2.  One stream over A and one over P serve both; each
+// solve's arithmetic is ir_pass's, term for term, so each gets ir_pass's
+// results bit for bit.  Returns the two ||r1||_inf.
+template <int NS, int R, int D>
+__device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrState A1,
+                                      const IrState B0, const IrState B1, const double* xca,
+                                      const double* xcb, bool init, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
+  const int n = I.n, mc = I.mc;
+  const PairRegs pr(I.pm);
+  if (threadIdx.x == 0) I.si[I_KMV] += 1;
+  double* scratch = reinterpret_cast<double*>(sm.u);
+  const double* dy = wave_uniform(I.dg + n);
+  const double* cy = wave_uniform(I.corr + n);
+  const double* ky[2] = {wave_uniform(I.krhs + n), wave_uniform(I.krhs2 + n)};
+  const double* r0y[2] = {wave_uniform(A0.r + n), wave_uniform(B0.r + n)};
+  const double* ax0[2] = {wave_uniform(A0.ax), wave_uniform(B0.ax)};
+  const double* sy0[2] = {wave_uniform((init ? A1.sol : A0.sol) + n),
+                          wave_uniform((init ? B1.sol : B0.sol) + n)};
+  double* s1y[2] = {wave_uniform(A1.sol + n), wave_uniform(B1.sol + n)};
+  double* s1ax[2] = {wave_uniform(A1.ax), wave_uniform(B1.ax)};
+  double* s1ry[2] = {wave_uniform(A1.r + n), wave_uniform(B1.r + n)};
+  double mx[2] = {0.0, 0.0};
+  // a stored row's inputs for both solves (row p and its partner q, as in
+  // ir_pass); the diagonal and correction are shared
+  struct RowIn {
+    double sy[2][2], r0[2][2], ky[2][2], ax0[2], dy[2], cy[2];
+    int has_q;
+  };
+  static_assert(10 * kPair2Chunk <= kLdsU && 16 * kPass2MaxLd <= kLdsU,
+                "LDS scratch of the two-solve pass");
+  rowpass2<NS, R, D, 10, 4, false, kPair2Chunk, true, 2>(
+      I.A, mc, n, I.ld, init ? A1.sol : xca,
+      [&](int r) {
+        int p, q;
+        pr.rows(r, &p, &q);
+        RowIn in;
+        in.has_q = q >= 0;
+        const int qq = q >= 0 ? q : p;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          in.sy[v][0] = ldg(sy0[v] + p);
+          in.sy[v][1] = ldg(sy0[v] + qq);
+          in.r0[v][0] = ldg(r0y[v] + p);
+          in.r0[v][1] = ldg(r0y[v] + qq);
+          in.ky[v][0] = ldg(ky[v] + p);
+          in.ky[v][1] = ldg(ky[v] + qq);
+          in.ax0[v] = ldg(ax0[v] + r);
+        }
+        in.dy[0] = ldg(dy + p);
+        in.dy[1] = ldg(dy + qq);
+        in.cy[0] = ldg(cy + p);
+        in.cy[1] = ldg(cy + qq);
+        return in;
+      },
+      [&](int r, double ta, double tb, const RowIn& in, double* c, double* out) {
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          const double t = v == 0 ? ta : tb;
+          const double axv = init ? t : in.ax0[v] + t;
+          double cc1 = 0.0, cc2 = 0.0;
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const double sg = h == 0 ? 1.0 : -1.0;
+            double yv, yc = 0.0;
+            if (init) {
+              yv = in.sy[v][h];
+            } else {
+              yc = -(sg * t - in.r0[v][h]) / in.dy[h];
+              yv = in.sy[v][h] + yc;
+            }
+            const double ry = in.ky[v][h] - (sg * axv + in.dy[h] * yv) + in.cy[h] * yv;
+            if (h == 0 || in.has_q) {
+              cc1 += sg * (init ? yv : yc);
+              cc2 += sg * (-ry / in.dy[h]);
+            }
+            out[5 * v + (h == 0 ? 0 : 3)] = yv;
+            out[5 * v + (h == 0 ? 2 : 4)] = ry;
+          }
+          c[2 * v] = cc1;
+          c[2 * v + 1] = cc2;
+          out[5 * v + 1] = axv;
+        }
+      },
+      [&](int r, const double* out) {
+        int p, q;
+        pr.rows(r, &p, &q);
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+          const double* o = out + 5 * v;
+          if (!init) stg(s1y[v] + p, o[0]);
+          stg(s1ax[v] + r, o[1]);
+          stg(s1ry[v] + p, o[2]);
+          double a = fabs(o[2]);
+          mx[v] = fmax(mx[v], (a == a) ? a : INFINITY);
+          if (q >= 0) {
+            if (!init) stg(s1y[v] + q, o[3]);
+            stg(s1ry[v] + q, o[4]);
+            a = fabs(o[4]);
+            mx[v] = fmax(mx[v], (a == a) ? a : INFINITY);
+          }
+        }
+      },
+      [&](int j, const double* cs) {
+        A1.aty[j] = init ? cs[0] : A0.aty[j] + cs[0];
+        I.um[j] = cs[1];
+        B1.aty[j] = init ? cs[2] : B0.aty[j] + cs[2];
+        I.um2[j] = cs[3];
+      },
+      scratch, I.ext, init ? B1.sol : xcb);
+  if (I.has_p) {
+    struct PIn {
+      double u[2], px0[2], pjj;
+    };
+    const double* u[2] = {wave_uniform(init ? A1.sol : xca), wave_uniform(init ? B1.sol : xcb)};
+    const double* px0[2] = {wave_uniform(A0.px), wave_uniform(B0.px)};
+    double* s1px[2] = {wave_uniform(A1.px), wave_uniform(B1.px)};
+    const double* P = wave_uniform(I.P);
+    const int ld = I.ld;
+    rowpass2<NS, R, D, 2, 0, false, 2 * kPair2Chunk, false, 2>(
+        P, n, n, ld, u[0],
+        [&](int j) {
+          PIn in;
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            in.u[v] = ldg(u[v] + j);
+            in.px0[v] = ldg(px0[v] + j);
+          }
+          in.pjj = ldg(P + (int64_t)j * ld + j);
+          return in;
+        },
+        [&](int j, double ta, double tb, const PIn& in, double* c, double* out) {
+#pragma unroll
+          for (int v = 0; v < 2; ++v) {
+            const double w = (v == 0 ? ta : tb) - in.pjj * in.u[v];
+            out[v] = init ? w : in.px0[v] + w;
+          }
+        },
+        [&](int j, const double* out) {
+          stg(s1px[0] + j, out[0]);
+          stg(s1px[1] + j, out[1]);
+        },
+        [](int, const double*) {}, scratch, nullptr, u[1]);
+  } else {
+    for (int j = threadIdx.x; j < n; j += WG) {
+      A1.px[j] = 0.0;
+      B1.px[j] = 0.0;
+    }
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < n; j += WG) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const IrState& S0 = v == 0 ? A0 : B0;
+      const IrState& S1 = v == 0 ? A1 : B1;
+      const double* xc = v == 0 ? xca : xcb;
+      const double* krhs = v == 0 ? I.krhs : I.krhs2;
+      const double* um = v == 0 ? I.um : I.um2;
+      const double xv = init ? S1.sol[j] : S0.sol[j] + xc[j];
+      if (!init) S1.sol[j] = xv;
+      const double rx = krhs[j] - (S1.px[j] + I.dg[j] * xv + S1.aty[j]) + I.corr[j] * xv;
+      S1.r[j] = rx;
+      S1.g[j] = rx + um[j];
+      const double a = fabs(rx);
+      mx[v] = fmax(mx[v], (a == a) ? a : INFINITY);
+    }
+  }
+  const double ra = bmax(mx[0], sm.red);
+  return Res2{ra, bmax(mx[1], sm.red)};
+}
+
+__device__ inline Res2 ir_pass2_any(const Inst& I, const IrState& A0, const IrState& A1,
+                                    const IrState& B0, const IrState& B1, const double* xca,
+                                    const double* xcb, bool init, Smem& sm) {
+  // one stage in flight: two solves' row inputs take the registers of the
+  // second (ld <= kPass2MaxLd)
+  if (I.ld <= 128) return ir_pass2<1, 8, 1>(I, A0, A1, B0, B1, xca, xcb, init, sm);
+  return ir_pass2<2, 4, 1>(I, A0, A1, B0, B1, xca, xcb, init, sm);
 }
 
 // xc = G^{-1} g with the factor in precision T.
@@ -1340,7 +1564,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   IrState S0{sol, w, w + 2 * m, w + 2 * m + 2 * n, I.res, w + 2 * m + 4 * n};
   IrState S1{I.r, w + m, w + 2 * m + n, w + 2 * m + 3 * n, I.ires, w + 2 * m + 5 * n};
   double* xc = I.iacc;
-  double res = ir_pass_any(I, S1, S0, nullptr, true, sm);
+  double res = ir_pass_any(I, S1, S0, nullptr, true, I.krhs, I.um, sm);
   IrResult out{0, 1, res, true};
   int solves = 0;
   while (solves < o.max_ir) {
@@ -1348,7 +1572,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
     const bool f32 = I.si[I_FORCE64] == 0;
     if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
     factor_solve_current(I, S0.g, xc, sm);
-    const double nr = ir_pass_any(I, S0, S1, xc, false, sm);
+    const double nr = ir_pass_any(I, S0, S1, xc, false, I.krhs, I.um, sm);
     if (f32) {
       if (threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
       if (!(nr < tol) && !(nr <= kPoorContraction * old)) {
@@ -1381,6 +1605,109 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   out.res = res;
   if (out.status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
   return out;
+}
+
+// Two refined solves with the FP64 factor at once: (I.r, warm1) into sol1
+// and (I.r2, warm2) into I.sol2, as two calls of solve_ir would run them
+// (the FP64 factor takes no fallback, so the two are independent).  Each
+// keeps its own residual, tolerance, stall test and step count; while both
+// still step, each step's pass serves both (ir_pass2), and a solve that
+// has stopped leaves the other to ir_pass.  Each result is solve_ir's bit
+// for bit.
+__device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, double* sol1,
+                                       const double* warm2, const Opts o, Smem& sm_in,
+                                       Counters* cnt, IrResult* out1, IrResult* out2) {
+  Smem sm = lds_view(sm_in);
+  const int n = I.n, m = I.m, N = I.n + I.m;
+  double mx1 = 0.0, mx2 = 0.0;
+  double* sol2 = I.sol2;
+  for (int k = threadIdx.x; k < N; k += WG) {
+    const double v1 = k < n ? I.r[k] : -I.r[k];
+    const double v2 = k < n ? I.r2[k] : -I.r2[k];
+    I.krhs[k] = v1;
+    I.krhs2[k] = v2;
+    sol1[k] = warm1[k];
+    sol2[k] = warm2[k];
+    double a = fabs(v1);
+    mx1 = fmax(mx1, (a == a) ? a : INFINITY);
+    a = fabs(v2);
+    mx2 = fmax(mx2, (a == a) ? a : INFINITY);
+  }
+  const double norm1 = bmax(mx1, sm.red);
+  const double norm2 = bmax(mx2, sm.red);
+  struct Run {
+    IrState S0, S1;
+    double* xc;
+    double tol, res;
+    int solves, status;
+    bool on;
+  };
+  double* w = I.irb;
+  double* w2 = I.irb2;
+  Run q[2] = {
+      {IrState{sol1, w, w + 2 * m, w + 2 * m + 2 * n, I.res, w + 2 * m + 4 * n},
+       IrState{I.r, w + m, w + 2 * m + n, w + 2 * m + 3 * n, I.ires, w + 2 * m + 5 * n},
+       I.iacc, o.ls_atol + o.ls_rtol * norm1, 0.0, 0, 1, true},
+      {IrState{sol2, w2, w2 + 2 * m, w2 + 2 * m + 2 * n, I.res2, w2 + 2 * m + 4 * n},
+       IrState{I.r2, w2 + m, w2 + 2 * m + n, w2 + 2 * m + 3 * n, I.ires2, w2 + 2 * m + 5 * n},
+       I.iacc2, o.ls_atol + o.ls_rtol * norm2, 0.0, 0, 1, true}};
+  {
+    const Res2 r = ir_pass2_any(I, q[0].S1, q[0].S0, q[1].S1, q[1].S0, nullptr, nullptr, true, sm);
+    q[0].res = r.a;
+    q[1].res = r.b;
+  }
+  for (;;) {
+    bool act[2];
+#pragma unroll
+    for (int v = 0; v < 2; ++v) act[v] = q[v].on && q[v].solves < o.max_ir;
+    if (!act[0] && !act[1]) break;
+    double nr[2] = {0.0, 0.0};
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+      if (act[v]) {
+        if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
+        factor_solve<double>(I, I.G, I.LiB, q[v].S0.g, q[v].xc, sm);
+      }
+    if (act[0] && act[1]) {
+      const Res2 r = ir_pass2_any(I, q[0].S0, q[0].S1, q[1].S0, q[1].S1, q[0].xc, q[1].xc,
+                                  false, sm);
+      nr[0] = r.a;
+      nr[1] = r.b;
+    } else if (act[0]) {
+      nr[0] = ir_pass_any(I, q[0].S0, q[0].S1, q[0].xc, false, I.krhs, I.um, sm);
+    } else {
+      nr[1] = ir_pass_any(I, q[1].S0, q[1].S1, q[1].xc, false, I.krhs2, I.um2, sm);
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+      if (act[v]) {
+        Run& u = q[v];
+        const double old = u.res;
+        ++u.solves;
+        const IrState t = u.S0;
+        u.S0 = u.S1;
+        u.S1 = t;
+        u.res = nr[v];
+        if (u.res < u.tol) {
+          u.status = 0;
+          u.on = false;
+        } else if (u.res >= old) {
+          u.status = 2;
+          u.on = false;
+        }
+      }
+  }
+  double* sols[2] = {sol1, sol2};
+  IrResult* outs[2] = {out1, out2};
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    if (q[v].S0.sol != sols[v]) {
+      for (int k = threadIdx.x; k < N; k += WG) sols[v][k] = q[v].S0.sol[k];
+      __syncthreads();
+    }
+    *outs[v] = IrResult{max(q[v].solves, 1), q[v].status, q[v].res, true};
+    if (q[v].status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
+  }
 }
 
 // -------------------------------------------------------------------------
@@ -1501,12 +1828,9 @@ __device__ __noinline__ double tau_linearized(const Inst& I, const double* kr, d
   return fmin(fmax(t, 0.1 * tau_c), 10.0 * tau_c);
 }
 
-// Newton step (__init__.py:708-757): writes [x+; y+] into outxy, returns tau+.
-__device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
-                       double tau_anchor, const double* corr_m, double tau, double* outxy,
-                       double* tau_plus, const Opts o, Smem& sm_in, Counters* cnt,
-                       IrResult* irout) {
-  Smem sm = lds_view(sm_in);
+// Right-hand side of a Newton step (__init__.py:708-724) into r.
+__device__ void newton_rhs(const Inst& I, double mu, double mu_t, const double* anchor,
+                           const double* corr_m, double* r) {
   const int n = I.n, z = I.z, N = I.n + I.m;
   for (int k = threadIdx.x; k < N; k += WG) {
     double v = (mu - mu_t) * anchor[k];
@@ -1516,20 +1840,39 @@ __device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const
       v += I.s[i];
       if (corr_m) v += corr_m[i];
     }
-    I.r[k] = v;
+    r[k] = v;
   }
   __syncthreads();
-  const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt);
-  *irout = ir;
-  if (!ir.ok) return false;
+}
+
+// The rest of a Newton step from its solve sol (__init__.py:726-757):
+// tau+ and [x+; y+] = sol - kinv_q tau+ into outxy.
+__device__ __noinline__ void newton_finish(const Inst& I, const IrResult ir, const double* sol,
+                                           double mu, double mu_t, double tau_anchor, double tau,
+                                           double* outxy, double* tau_plus, Smem& sm_in) {
+  Smem sm = lds_view(sm_in);
+  const int N = I.n + I.m;
   double t = 0.0;
   bool have = false;
   if (ir.status == 0 || ir.res < 1e-7)
-    have = tau_quadratic(I, I.sol, mu, mu_t, (mu - mu_t) * tau_anchor, &t, sm);
-  if (!have) t = tau_linearized(I, I.sol, mu, mu_t, tau, tau_anchor, sm);
-  for (int k = threadIdx.x; k < N; k += WG) outxy[k] = I.sol[k] - I.kq[k] * t;
+    have = tau_quadratic(I, sol, mu, mu_t, (mu - mu_t) * tau_anchor, &t, sm);
+  if (!have) t = tau_linearized(I, sol, mu, mu_t, tau, tau_anchor, sm);
+  for (int k = threadIdx.x; k < N; k += WG) outxy[k] = sol[k] - I.kq[k] * t;
   __syncthreads();
   *tau_plus = t;
+}
+
+// Newton step (__init__.py:708-757): writes [x+; y+] into outxy, returns tau+.
+__device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
+                       double tau_anchor, const double* corr_m, double tau, double* outxy,
+                       double* tau_plus, const Opts o, Smem& sm_in, Counters* cnt,
+                       IrResult* irout) {
+  Smem sm = lds_view(sm_in);
+  newton_rhs(I, mu, mu_t, anchor, corr_m, I.r);
+  const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt);
+  *irout = ir;
+  if (!ir.ok) return false;
+  newton_finish(I, ir, I.sol, mu, mu_t, tau_anchor, tau, outxy, tau_plus, sm);
   return true;
 }
 
@@ -1580,31 +1923,47 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   }
 
   IrResult irq{}, irp{}, irc{};
-  // kinv_q = K^{-1} [c; b], warm-started from the previous kinv_q
-  for (int k = tid; k < N; k += WG) I.xp[k] = I.kq[k];
-  __syncthreads();
-  {
-    for (int k = tid; k < N; k += WG) I.r[k] = k < n ? I.c[k] : I.b[k - n];
+  double tau_p;
+  if (o.fuse && I.si[I_FORCE64] && I.ld <= kPass2MaxLd) {
+    // With the FP64 factor, kinv_q and the predictor's solve refine together
+    // (solve_ir2): the same two solves, sharing each pass over A and P.
+    for (int k = tid; k < N; k += WG) {
+      I.r[k] = k < n ? I.c[k] : I.b[k - n];
+      I.xp[k] = k < n ? I.x[k] : I.y[k - n];
+    }
     __syncthreads();
-    const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt);
-    irq = ir;
-    if (!ir.ok) {
+    newton_rhs(I, mu, 0.0, I.xp, nullptr, I.r2);
+    solve_ir2(I, I.kq, I.sol, I.xp, o, sm, cnt, &irq, &irp);
+    for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
+    __syncthreads();
+    if (I.has_p) pmul(I, I.kq, I.pkq, sm);
+    newton_finish(I, irp, I.sol2, mu, 0.0, tau, tau, I.xp, &tau_p, sm);
+  } else {
+    // kinv_q = K^{-1} [c; b], warm-started from the previous kinv_q
+    for (int k = tid; k < N; k += WG) I.xp[k] = I.kq[k];
+    __syncthreads();
+    {
+      for (int k = tid; k < N; k += WG) I.r[k] = k < n ? I.c[k] : I.b[k - n];
+      __syncthreads();
+      const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt);
+      irq = ir;
+      if (!ir.ok) {
+        if (tid == 0) I.si[I_STATUS] = kStatusError;
+        return;
+      }
+      for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
+      __syncthreads();
+      // P kq_x, which both tau quadratics of the iteration use
+      if (I.has_p) pmul(I, I.kq, I.pkq, sm);
+    }
+
+    // predictor (mu_target = 0) anchored at [x; y]
+    for (int k = tid; k < N; k += WG) I.xp[k] = k < n ? I.x[k] : I.y[k - n];
+    __syncthreads();
+    if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp)) {
       if (tid == 0) I.si[I_STATUS] = kStatusError;
       return;
     }
-    for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
-    __syncthreads();
-    // P kq_x, which both tau quadratics of the iteration use
-    if (I.has_p) pmul(I, I.kq, I.pkq, sm);
-  }
-
-  // predictor (mu_target = 0) anchored at [x; y]
-  for (int k = tid; k < N; k += WG) I.xp[k] = k < n ? I.x[k] : I.y[k - n];
-  __syncthreads();
-  double tau_p;
-  if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp)) {
-    if (tid == 0) I.si[I_STATUS] = kStatusError;
-    return;
   }
   // d_s = -y_p s / y ; step, sigma
   for (int i = tid; i < m; i += WG) I.ds[i] = i < z ? 0.0 : -I.xp[n + i] * I.s[i] / I.y[i];
@@ -2224,6 +2583,7 @@ struct qtqp_batch {
   unsigned long long* dclk = nullptr;  // per-QP completion clock, [count] = solve start
   std::vector<double> inst_ms;        // per-QP completion time of the last solve
   bool family = false;  // every QP shares QP 0's A and P (qtqp_batch_create_family)
+  bool fuse_solves = true;  // see QTQP_BATCH_NO_FUSED_SOLVES
   double *d_braw = nullptr, *d_craw = nullptr;  // raw b and c of all QPs, packed by m_off / n_off
 };
 
@@ -2435,6 +2795,8 @@ size_t vector_bytes(int n, int m) {
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
   b += align256(8 * n);                                // pkq
   b += align256(8 * (2 * (int64_t)m + 6 * (int64_t)n));  // refinement state
+  b += 6 * align256(8 * N) + align256(8 * (2 * (int64_t)m + 6 * (int64_t)n)) +
+       align256(8 * n);  // the second solve's r2 sol2 krhs2 res2 ires2 iacc2, irb2, um2
   b += align256(8 * S_NSCAL) + align256(4 * I_NINT);
   b += align256(8 * kTraceFields * kTraceRows);
   return b;
@@ -2501,6 +2863,7 @@ int create_impl(int32_t count, const int32_t* n, const int32_t* m, const int32_t
   std::vector<int> mcs(count);
   std::vector<std::vector<uint32_t>> exts(count);
   {
+    h->fuse_solves = (flags & QTQP_BATCH_NO_FUSED_SOLVES) == 0;
     const bool pairs = (flags & QTQP_BATCH_NO_ROW_PAIRS) == 0;
     const bool extents = (flags & QTQP_BATCH_NO_ROW_EXTENTS) == 0;
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -2640,6 +3003,14 @@ int create_impl(int32_t count, const int32_t* n, const int32_t* m, const int32_t
     I.cm = ar.take<double>(mx);
     I.pkq = ar.take<double>(nq);
     I.irb = ar.take<double>(2 * (int64_t)mq + 6 * (int64_t)nq);
+    I.r2 = ar.take<double>(N);
+    I.sol2 = ar.take<double>(N);
+    I.krhs2 = ar.take<double>(N);
+    I.res2 = ar.take<double>(N);
+    I.ires2 = ar.take<double>(N);
+    I.iacc2 = ar.take<double>(N);
+    I.irb2 = ar.take<double>(2 * (int64_t)mq + 6 * (int64_t)nq);
+    I.um2 = ar.take<double>(nq);
     I.sd = ar.take<double>(S_NSCAL);
     I.si = ar.take<int>(I_NINT);
     I.trace = ar.take<double>(kTraceFields * kTraceRows);
@@ -2772,7 +3143,8 @@ int qtqp_batch_solve(qtqp_batch* h, const qtqp_batch_opts* opts, qtqp_batch_coun
   if (!h || !opts) return fail(QTQP_ERR_ARG, "NULL argument");
   if (opts->max_iter < 1 || opts->max_iterative_refinement_steps < 1)
     return fail(QTQP_ERR_ARG, "max_iter and max_iterative_refinement_steps must be >= 1");
-  const Opts o = to_opts(opts);
+  Opts o = to_opts(opts);
+  o.fuse = h->fuse_solves ? 1 : 0;
   HIPC(hipEventRecord(h->ev0, h->stream));
   HIPC(hipMemsetAsync(h->dcnt, 0, sizeof(Counters), h->stream));
   int dev = 0, cus = 0;

@@ -346,3 +346,37 @@ def test_row_extents_give_the_full_row_results(require_gpu, precision):
   for q, sol in zip(qps, sols):
     ref = O.solve_qp(q.a, q.b, q.c, q.z, q.p)
     _compare(sol, ref, 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_fused_solves_equal_the_solves_in_turn_bitwise(require_gpu, precision):
+  """The two refined solves of an iteration that run together (K^{-1}[c; b]
+  and the predictor's, once a QP factors in FP64) keep each solve's own
+  arithmetic, so every iterate equals that of the solves run in turn, bit for
+  bit: for both row-pass shapes of the two-solve pass (ld up to 128 and
+  256), LPs and QPs, paired rows, and n above 256 where the solves run in
+  turn either way.  Fewer passes over A show that the fused path ran."""
+  qps = [_banded_qp(n, 70 + n) for n in (37, 128, 200, 256, 300, 600)]
+  qps += problems.uneven_mpc_batch(6, base_seed=910, t_range=(4, 40))
+  lp = problems.random_dense_qp(90, 160, 10, seed=31)
+  qps.append((lp.a, lp.b, lp.c, lp.z, None))
+  outs, work = [], []
+  for fused in (True, False):
+    bq = batch.BatchQP(qps, fused_solves=fused)
+    bq.solve_device(precision=precision)
+    outs.append(bq.raw_iterates())
+    work.append(bq.work_counts())
+    bq.free()
+  for k in ("x", "y", "s", "tau", "status", "iters"):
+    assert np.array_equal(outs[0][k], outs[1][k]), k
+  ns = np.array([np.shape(q.a if hasattr(q, "a") else q[0])[1] for q in qps])
+  # K products: fewer where the fused path runs (n <= 256), equal above
+  small = ns <= 256
+  assert np.all(work[0][small, 2] <= work[1][small, 2])
+  assert work[0][small, 2].sum() < work[1][small, 2].sum()
+  if precision == "fp64":
+    assert np.all(work[0][small, 2] < work[1][small, 2])
+  assert np.array_equal(work[0][~small, 2], work[1][~small, 2])
+  # factorisations and factor solves are the same work either way
+  for col in (0, 1, 3, 4):
+    assert np.array_equal(work[0][:, col], work[1][:, col]), col
