@@ -87,6 +87,9 @@ struct Inst {
   double *x, *y, *s, *kq, *dg, *corr, *krhs, *sol, *res, *iacc, *ires, *r, *xp;
   double *um, *un, *ds, *cm;
   double* pkq;  // P kq_x of this iteration's kinv_q
+  // per constraint row, for this iteration: wy = 1 / d_y (d_y the
+  // regularised diagonal) and ey = d_y - corr_y = -(s/y + mu), the true one
+  double *wy, *ey;
   double* irb;  // refinement state, two copies: A x (m), A'y, P_off x, g (n each)
   // a second refined solve run together with the first (solve_ir2): its
   // right-hand side, solution, KKT right-hand side, refinement state,
@@ -837,9 +840,9 @@ constexpr int row_overrun() { return R - 1; }
 // Rows a pipelined row pass takes between two flushes of its staged row
 // outputs: a multiple of every stage rotation (D + 1) * 4R used below, and
 // small enough for three doubles per row in the LDS scratch.
-constexpr int kPassChunk = 2016;
+constexpr int kPassChunk = 1984;
 // The same for passes that stage five outputs per row (a row pair's).
-constexpr int kPairChunk = 1344;
+constexpr int kPairChunk = 1280;
 
 // Row extents of one stage of R rows (R packed words) and the word of row r.
 template <int R>
@@ -1119,8 +1122,8 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
 }
 
 // Row-pass shapes by leading dimension: R rows x NS 128-column slots = 8
-// vectors per stage, D stages in flight (one for the widest rows, whose
-// column sums take the registers of the second).
+// vectors per stage, one stage in flight while the next is processed (with
+// two, measured on the batched MPC workload, every pass ran slower).
 template <int NS_, int R_, int D_>
 struct PassShape {
   static constexpr int NS = NS_, R = R_, D = D_;
@@ -1128,9 +1131,9 @@ struct PassShape {
 
 template <typename F>
 __device__ inline auto with_pass_shape(int ld, F f) {
-  if (ld <= 128) return f(PassShape<1, 8, 2>());
-  if (ld <= 256) return f(PassShape<2, 4, 2>());
-  if (ld <= 512) return f(PassShape<4, 2, 2>());
+  if (ld <= 128) return f(PassShape<1, 8, 1>());
+  if (ld <= 256) return f(PassShape<2, 4, 1>());
+  if (ld <= 512) return f(PassShape<4, 2, 1>());
   return f(PassShape<8, 2, 1>());
 }
 
@@ -1156,9 +1159,9 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   const PairRegs pr(I.pm);
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
-  const double* dy = wave_uniform(I.dg + n);
+  const double* wy = wave_uniform(I.wy);
   const double* ky = wave_uniform(krhs + n);
-  const double* cy = wave_uniform(I.corr + n);
+  const double* ey = wave_uniform(I.ey);
   const double* r0y = wave_uniform(S0.r + n);
   const double* ax0 = wave_uniform(S0.ax);
   double* s1y = wave_uniform(S1.sol + n);
@@ -1171,7 +1174,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   // both, and the column coefficients are c(p) - c(q).  ax (A x of the
   // refinement iterate) is kept per stored row.
   struct RowIn {
-    double sy[2], r0[2], dy[2], ky[2], cy[2], ax0;
+    double sy[2], r0[2], wy[2], ky[2], ey[2], ax0;
     int has_q;
   };
   // Row inputs are loaded whether or not `init` uses them (S0 holds valid
@@ -1190,12 +1193,12 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
         in.sy[1] = ldg(sy0 + qq);
         in.r0[0] = ldg(r0y + p);
         in.r0[1] = ldg(r0y + qq);
-        in.dy[0] = ldg(dy + p);
-        in.dy[1] = ldg(dy + qq);
+        in.wy[0] = ldg(wy + p);
+        in.wy[1] = ldg(wy + qq);
         in.ky[0] = ldg(ky + p);
         in.ky[1] = ldg(ky + qq);
-        in.cy[0] = ldg(cy + p);
-        in.cy[1] = ldg(cy + qq);
+        in.ey[0] = ldg(ey + p);
+        in.ey[1] = ldg(ey + qq);
         in.ax0 = ldg(ax0 + r);
         return in;
       },
@@ -1210,13 +1213,13 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
           if (init) {
             yv = in.sy[h];
           } else {
-            yc = -(sg * t - in.r0[h]) / in.dy[h];
+            yc = -(sg * t - in.r0[h]) * in.wy[h];
             yv = in.sy[h] + yc;
           }
-          const double ry = in.ky[h] - (sg * axv + in.dy[h] * yv) + in.cy[h] * yv;
+          const double ry = in.ky[h] - (sg * axv + in.ey[h] * yv);
           if (h == 0 || in.has_q) {
             cc1 += sg * (init ? yv : yc);
-            cc2 += sg * (-ry / in.dy[h]);
+            cc2 += sg * (-ry * in.wy[h]);
           }
           out[h == 0 ? 0 : 3] = yv;
           out[h == 0 ? 2 : 4] = ry;
@@ -1289,10 +1292,10 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
 __device__ inline double ir_pass_any(const Inst& I, const IrState& S0, const IrState& S1,
                                      const double* xc, bool init, const double* krhs, double* um,
                                      Smem& sm) {
-  // R rows x NS 128-column slots = 8 vectors per stage; D stages in flight
-  if (I.ld <= 128) return ir_pass<1, 8, 2>(I, S0, S1, xc, init, krhs, um, sm);
-  if (I.ld <= 256) return ir_pass<2, 4, 2>(I, S0, S1, xc, init, krhs, um, sm);
-  if (I.ld <= 512) return ir_pass<4, 2, 2>(I, S0, S1, xc, init, krhs, um, sm);
+  // R rows x NS 128-column slots = 8 vectors per stage, one stage in flight
+  if (I.ld <= 128) return ir_pass<1, 8, 1>(I, S0, S1, xc, init, krhs, um, sm);
+  if (I.ld <= 256) return ir_pass<2, 4, 1>(I, S0, S1, xc, init, krhs, um, sm);
+  if (I.ld <= 512) return ir_pass<4, 2, 1>(I, S0, S1, xc, init, krhs, um, sm);
   return ir_pass<8, 2, 1>(I, S0, S1, xc, init, krhs, um, sm);
 }
 
@@ -1321,8 +1324,8 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
   const PairRegs pr(I.pm);
   if (threadIdx.x == 0) I.si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
-  const double* dy = wave_uniform(I.dg + n);
-  const double* cy = wave_uniform(I.corr + n);
+  const double* wy = wave_uniform(I.wy);
+  const double* ey = wave_uniform(I.ey);
   const double* ky[2] = {wave_uniform(I.krhs + n), wave_uniform(I.krhs2 + n)};
   const double* r0y[2] = {wave_uniform(A0.r + n), wave_uniform(B0.r + n)};
   const double* ax0[2] = {wave_uniform(A0.ax), wave_uniform(B0.ax)};
@@ -1333,9 +1336,9 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
   double* s1ry[2] = {wave_uniform(A1.r + n), wave_uniform(B1.r + n)};
   double mx[2] = {0.0, 0.0};
   // a stored row's inputs for both solves (row p and its partner q, as in
-  // ir_pass); the diagonal and correction are shared
+  // ir_pass); the diagonal terms are shared
   struct RowIn {
-    double sy[2][2], r0[2][2], ky[2][2], ax0[2], dy[2], cy[2];
+    double sy[2][2], r0[2][2], ky[2][2], ax0[2], wy[2], ey[2];
     int has_q;
   };
   static_assert(10 * kPair2Chunk <= kLdsU && 16 * kPass2MaxLd <= kLdsU,
@@ -1358,10 +1361,10 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
           in.ky[v][1] = ldg(ky[v] + qq);
           in.ax0[v] = ldg(ax0[v] + r);
         }
-        in.dy[0] = ldg(dy + p);
-        in.dy[1] = ldg(dy + qq);
-        in.cy[0] = ldg(cy + p);
-        in.cy[1] = ldg(cy + qq);
+        in.wy[0] = ldg(wy + p);
+        in.wy[1] = ldg(wy + qq);
+        in.ey[0] = ldg(ey + p);
+        in.ey[1] = ldg(ey + qq);
         return in;
       },
       [&](int r, double ta, double tb, const RowIn& in, double* c, double* out) {
@@ -1377,13 +1380,13 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
             if (init) {
               yv = in.sy[v][h];
             } else {
-              yc = -(sg * t - in.r0[v][h]) / in.dy[h];
+              yc = -(sg * t - in.r0[v][h]) * in.wy[h];
               yv = in.sy[v][h] + yc;
             }
-            const double ry = in.ky[v][h] - (sg * axv + in.dy[h] * yv) + in.cy[h] * yv;
+            const double ry = in.ky[v][h] - (sg * axv + in.ey[h] * yv);
             if (h == 0 || in.has_q) {
               cc1 += sg * (init ? yv : yc);
-              cc2 += sg * (-ry / in.dy[h]);
+              cc2 += sg * (-ry * in.wy[h]);
             }
             out[5 * v + (h == 0 ? 0 : 3)] = yv;
             out[5 * v + (h == 0 ? 2 : 4)] = ry;
@@ -1483,8 +1486,7 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
 __device__ inline Res2 ir_pass2_any(const Inst& I, const IrState& A0, const IrState& A1,
                                     const IrState& B0, const IrState& B1, const double* xca,
                                     const double* xcb, bool init, Smem& sm) {
-  // one stage in flight: two solves' row inputs take the registers of the
-  // second (ld <= kPass2MaxLd)
+  // one stage in flight, as in ir_pass (ld <= kPass2MaxLd)
   if (I.ld <= 128) return ir_pass2<1, 8, 1>(I, A0, A1, B0, B1, xca, xcb, init, sm);
   return ir_pass2<2, 4, 1>(I, A0, A1, B0, B1, xca, xcb, init, sm);
 }
@@ -1914,6 +1916,8 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     } else {
       I.dg[k] = -rg;
       I.corr[k] = -rg + t;
+      I.wy[k - n] = -1.0 / rg;
+      I.ey[k - n] = -t;
     }
   }
   __syncthreads();
@@ -2794,6 +2798,7 @@ size_t vector_bytes(int n, int m) {
   b += 10 * align256(8 * N);                       // kq dg corr krhs sol res iacc ires r xp
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
   b += align256(8 * n);                                // pkq
+  b += 2 * align256(8 * m);                            // wy, ey
   b += align256(8 * (2 * (int64_t)m + 6 * (int64_t)n));  // refinement state
   b += 6 * align256(8 * N) + align256(8 * (2 * (int64_t)m + 6 * (int64_t)n)) +
        align256(8 * n);  // the second solve's r2 sol2 krhs2 res2 ires2 iacc2, irb2, um2
@@ -3002,6 +3007,8 @@ int create_impl(int32_t count, const int32_t* n, const int32_t* m, const int32_t
     I.ds = ar.take<double>(mx);
     I.cm = ar.take<double>(mx);
     I.pkq = ar.take<double>(nq);
+    I.wy = ar.take<double>(mq);
+    I.ey = ar.take<double>(mq);
     I.irb = ar.take<double>(2 * (int64_t)mq + 6 * (int64_t)nq);
     I.r2 = ar.take<double>(N);
     I.sol2 = ar.take<double>(N);

@@ -171,7 +171,7 @@ def test_tile_shapes_give_identical_results_and_autotune(require_gpu, precision)
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_row_pass_chunks_and_slot_boundaries(require_gpu, precision):
-  """Row counts past the 2016-row staging chunk (one, two and three chunks, a
+  """Row counts past the 1984-row staging chunk (one, two and three chunks, a
   partial last stage) and leading dimensions on both sides of the 128/256/512
   column-slot boundaries of the pipelined row pass, against the oracle."""
   qps = [problems.random_dense_qp(128, 2017, 10, seed=21),   # ld 128, 2 chunks
