@@ -90,6 +90,10 @@ struct Inst {
   // per constraint row, for this iteration: wy = 1 / d_y (d_y the
   // regularised diagonal) and ey = d_y - corr_y = -(s/y + mu), the true one
   double *wy, *ey;
+  // products kept for the termination test: A x (stored rows), A'y and
+  // P_off x of the iterate at the start of the iteration, of kinv_q and of
+  // the corrector's solution, each [m | n | n]
+  double* tk;
   double* irb;  // refinement state, two copies: A x (m), A'y, P_off x, g (n each)
   // a second refined solve run together with the first (solve_ir2): its
   // right-hand side, solution, KKT right-hand side, refinement state,
@@ -1529,6 +1533,17 @@ __device__ inline bool fallback_fp64(const Inst& I, Smem& sm_in, Counters* cnt) 
   return factor64_retry(I, sm, cnt);
 }
 
+// Copies a refinement state's products -- A sol_x over the stored rows,
+// A' sol_y and P_off sol_x -- into dst, laid out [m | n | n].
+__device__ inline void keep_products(const Inst& I, const IrState& S, double* dst) {
+  for (int k = threadIdx.x; k < I.mc; k += WG) dst[k] = S.ax[k];
+  for (int j = threadIdx.x; j < I.n; j += WG) {
+    dst[I.m + j] = S.aty[j];
+    dst[I.m + I.n + j] = S.px[j];
+  }
+  __syncthreads();
+}
+
 struct IrResult {
   int solves, status;  // 0 converged, 1 non-converged, 2 stalled
   double res;
@@ -1544,12 +1559,15 @@ struct IrResult {
 // the residual by less than kPoorContraction is dropped and the factor is
 // redone in FP64 (it does not count as a step); FP64 steps follow the
 // reference rule exactly.  rhs may alias I.r (it is consumed into krhs
-// first).
+// first).  keep_warm / keep_sol, where not null, receive the products of
+// warm and of sol (keep_products).
 constexpr double kPoorContraction = 3e-2;
 constexpr int kF32StepBudget = 3;  // refinement steps per IPM iteration kept on an FP32 factor
 
 __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, const double* warm,
-                                          double* sol, const Opts o, Smem& sm_in, Counters* cnt) {
+                                          double* sol, const Opts o, Smem& sm_in, Counters* cnt,
+                                          double* keep_warm = nullptr,
+                                          double* keep_sol = nullptr) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m, N = I.n + I.m;
   double mx = 0.0;
@@ -1567,6 +1585,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   IrState S1{I.r, w + m, w + 2 * m + n, w + 2 * m + 3 * n, I.ires, w + 2 * m + 5 * n};
   double* xc = I.iacc;
   double res = ir_pass_any(I, S1, S0, nullptr, true, I.krhs, I.um, sm);
+  if (keep_warm) keep_products(I, S0, keep_warm);
   IrResult out{0, 1, res, true};
   int solves = 0;
   while (solves < o.max_ir) {
@@ -1603,6 +1622,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
     for (int k = threadIdx.x; k < N; k += WG) sol[k] = S0.sol[k];
     __syncthreads();
   }
+  if (keep_sol) keep_products(I, S0, keep_sol);
   out.solves = max(solves, 1);
   out.res = res;
   if (out.status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
@@ -1615,10 +1635,12 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
 // keeps its own residual, tolerance, stall test and step count; while both
 // still step, each step's pass serves both (ir_pass2), and a solve that
 // has stopped leaves the other to ir_pass.  Each result is solve_ir's bit
-// for bit.
+// for bit; keep_sol1 receives the products of sol1 and keep_warm2 those of
+// warm2 (keep_products).
 __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, double* sol1,
                                        const double* warm2, const Opts o, Smem& sm_in,
-                                       Counters* cnt, IrResult* out1, IrResult* out2) {
+                                       Counters* cnt, IrResult* out1, IrResult* out2,
+                                       double* keep_sol1, double* keep_warm2) {
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m, N = I.n + I.m;
   double mx1 = 0.0, mx2 = 0.0;
@@ -1658,6 +1680,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
     q[0].res = r.a;
     q[1].res = r.b;
   }
+  keep_products(I, q[1].S0, keep_warm2);
   for (;;) {
     bool act[2];
 #pragma unroll
@@ -1707,6 +1730,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
       for (int k = threadIdx.x; k < N; k += WG) sols[v][k] = q[v].S0.sol[k];
       __syncthreads();
     }
+    if (v == 0) keep_products(I, q[0].S0, keep_sol1);
     *outs[v] = IrResult{max(q[v].solves, 1), q[v].status, q[v].res, true};
     if (q[v].status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
   }
@@ -1868,10 +1892,10 @@ __device__ __noinline__ void newton_finish(const Inst& I, const IrResult ir, con
 __device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const double* anchor,
                        double tau_anchor, const double* corr_m, double tau, double* outxy,
                        double* tau_plus, const Opts o, Smem& sm_in, Counters* cnt,
-                       IrResult* irout) {
+                       IrResult* irout, double* keep_warm, double* keep_sol) {
   Smem sm = lds_view(sm_in);
   newton_rhs(I, mu, mu_t, anchor, corr_m, I.r);
-  const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt);
+  const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt, keep_warm, keep_sol);
   *irout = ir;
   if (!ir.ok) return false;
   newton_finish(I, ir, I.sol, mu, mu_t, tau_anchor, tau, outxy, tau_plus, sm);
@@ -1928,6 +1952,12 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 
   IrResult irq{}, irp{}, irc{};
   double tau_p;
+  // products for the termination test (keep_products): of [x; y] as the
+  // iteration starts, of kinv_q, of the corrector's solution
+  const int64_t tks = (int64_t)m + 2 * n;
+  double* tk_start = I.tk;
+  double* tk_q = I.tk + tks;
+  double* tk_c = I.tk + 2 * tks;
   if (o.fuse && I.si[I_FORCE64] && I.ld <= kPass2MaxLd) {
     // With the FP64 factor, kinv_q and the predictor's solve refine together
     // (solve_ir2): the same two solves, sharing each pass over A and P.
@@ -1937,7 +1967,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     }
     __syncthreads();
     newton_rhs(I, mu, 0.0, I.xp, nullptr, I.r2);
-    solve_ir2(I, I.kq, I.sol, I.xp, o, sm, cnt, &irq, &irp);
+    solve_ir2(I, I.kq, I.sol, I.xp, o, sm, cnt, &irq, &irp, tk_q, tk_start);
     for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
     __syncthreads();
     if (I.has_p) pmul(I, I.kq, I.pkq, sm);
@@ -1949,7 +1979,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     {
       for (int k = tid; k < N; k += WG) I.r[k] = k < n ? I.c[k] : I.b[k - n];
       __syncthreads();
-      const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt);
+      const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt, nullptr, tk_q);
       irq = ir;
       if (!ir.ok) {
         if (tid == 0) I.si[I_STATUS] = kStatusError;
@@ -1964,7 +1994,8 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     // predictor (mu_target = 0) anchored at [x; y]
     for (int k = tid; k < N; k += WG) I.xp[k] = k < n ? I.x[k] : I.y[k - n];
     __syncthreads();
-    if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp)) {
+    if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp, tk_start,
+                nullptr)) {
       if (tid == 0) I.si[I_STATUS] = kStatusError;
       return;
     }
@@ -1999,7 +2030,8 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     I.cm[i] = i < z ? 0.0 : -I.ds[i] * (I.xp[n + i] - I.y[i]) / I.y[i];
   __syncthreads();
   double tau_c;
-  if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt, &irc)) {
+  if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt, &irc, nullptr,
+              tk_c)) {
     if (tid == 0) I.si[I_STATUS] = kStatusError;
     return;
   }
@@ -2034,49 +2066,38 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 
   // termination on the unscaled problem (__init__.py:888-965):
   // A_orig x_orig = (A x)/d, A_orig' y_orig = (A' y)/e, P_orig x_orig = (P x)/e.
+  // The products of the updated iterate follow from those the refinement
+  // already formed, by linearity of the update: x+ = x + step (x_c - x) with
+  // x_c = sol_c - tau_c kinv_q, so A x+ = A x + step ((A sol_c - tau_c
+  // A kinv_q) - A x), and likewise A'y+ and P_off x+ (P x+ adds the diagonal
+  // of P times x+).  No pass over A or P is needed.  (The floor of 1e-30 on
+  // y+ changes A'y+ by at most 1e-30 |A|.)
   const double inv_tau = 1.0 / fmax(tau, kEps);
   double* ax = I.um;  // m
   double* aty = I.un; // n
-  {
-    // over the stored rows: row p gets (A x)/d, its partner the negative
-    // (the partner's d equals p's), and the column coefficient is y_p - y_q
-    struct TIn {
-      double y, yq, d;
-    };
-    const PairRegs pr(I.pm);
-    const double* yv = wave_uniform(I.y);
-    const double* dv = wave_uniform(I.d);
-    double* axw = wave_uniform(ax);
-    with_pass_shape(I.ld, [&](auto sh) {
-      using S = decltype(sh);
-      rowpass2<S::NS, S::R, S::D, 1, 1, false, kPassChunk, true>(
-          I.A, I.mc, n, I.ld, I.x,
-          [&](int r) {
-            int p, q;
-            pr.rows(r, &p, &q);
-            TIn in;
-            in.y = ldg(yv + p);
-            const double yq = ldg(yv + (q >= 0 ? q : p));  // unconditional load
-            in.yq = q >= 0 ? yq : 0.0;
-            in.d = ldg(dv + p);
-            return in;
-          },
-          [](int, double t, const TIn& in, double* c1, double*, double* o) {
-            *c1 = in.y - in.yq;
-            o[0] = t / in.d;
-          },
-          [&](int r, const double* o) {
-            int p, q;
-            pr.rows(r, &p, &q);
-            stg(axw + p, o[0]);
-            if (q >= 0) stg(axw + q, -o[0]);
-          },
-          [&](int j, double s1, double) { aty[j] = s1 / I.e[j]; }, reinterpret_cast<double*>(sm.u),
-          I.ext);
-    });
-  }
   double* px = I.iacc;  // n
-  if (I.has_p) pmul(I, I.x, px, sm);
+  {
+    const PairRegs pr(I.pm);
+    for (int r = tid; r < I.mc; r += WG) {
+      const double t0 = tk_start[r];
+      const double t = t0 + step * ((tk_c[r] - tau_c * tk_q[r]) - t0);
+      int p, q;
+      pr.rows(r, &p, &q);
+      const double v = t / I.d[p];
+      ax[p] = v;
+      if (q >= 0) ax[q] = -v;  // the partner's d equals p's
+    }
+    for (int j = tid; j < n; j += WG) {
+      const double a0 = tk_start[m + j];
+      aty[j] = (a0 + step * ((tk_c[m + j] - tau_c * tk_q[m + j]) - a0)) / I.e[j];
+      if (I.has_p) {
+        const double p0 = tk_start[m + n + j];
+        px[j] = (p0 + step * ((tk_c[m + n + j] - tau_c * tk_q[m + n + j]) - p0)) +
+                I.P[(int64_t)j * I.ld + j] * I.x[j];
+      }
+    }
+    __syncthreads();
+  }
   double l_xpx = 0.0, l_ctx = 0.0, l_px = 0.0, l_aty = 0.0, l_dres = 0.0, l_nx = 0.0;
   for (int j = tid; j < n; j += WG) {
     const double pxj = I.has_p ? px[j] : 0.0;
@@ -2799,6 +2820,7 @@ size_t vector_bytes(int n, int m) {
   b += 2 * align256(8 * std::max<int64_t>(m, n)) * 2;  // um un ds cm
   b += align256(8 * n);                                // pkq
   b += 2 * align256(8 * m);                            // wy, ey
+  b += align256(8 * 3 * ((int64_t)m + 2 * (int64_t)n));  // tk
   b += align256(8 * (2 * (int64_t)m + 6 * (int64_t)n));  // refinement state
   b += 6 * align256(8 * N) + align256(8 * (2 * (int64_t)m + 6 * (int64_t)n)) +
        align256(8 * n);  // the second solve's r2 sol2 krhs2 res2 ires2 iacc2, irb2, um2
@@ -3009,6 +3031,7 @@ int create_impl(int32_t count, const int32_t* n, const int32_t* m, const int32_t
     I.pkq = ar.take<double>(nq);
     I.wy = ar.take<double>(mq);
     I.ey = ar.take<double>(mq);
+    I.tk = ar.take<double>(3 * ((int64_t)mq + 2 * (int64_t)nq));
     I.irb = ar.take<double>(2 * (int64_t)mq + 6 * (int64_t)nq);
     I.r2 = ar.take<double>(N);
     I.sol2 = ar.take<double>(N);
