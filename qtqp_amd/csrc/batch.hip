@@ -152,6 +152,28 @@ __device__ inline Smem lds_smem() {
 
 __device__ inline Smem lds_view(const Smem&) { return lds_smem(); }
 
+// Per-phase device clocks of the IPM kernels, compiled in with
+// -DQTQP_PHASE_CLOCKS only: PHASE(k) adds the wall-clock ticks (100 MHz)
+// since the previous mark of this workgroup to slot k of g_phase, PHASE(-1)
+// restarts the interval.  Each workgroup's thread 0 stands for it
+// (every phase ends in a barrier or is uniform).  Read and cleared with
+// qtqp_batch_phase_clocks.
+#ifdef QTQP_PHASE_CLOCKS
+constexpr int kPhaseSlots = 32;
+__device__ unsigned long long g_phase[kPhaseSlots];
+__shared__ unsigned long long g_phase_t;
+__device__ inline void phase_mark(int k) {
+  if (threadIdx.x == 0) {
+    const unsigned long long t = wall_clock64();
+    if (k >= 0) atomicAdd(&g_phase[k], t - g_phase_t);
+    g_phase_t = t;
+  }
+}
+#define PHASE(k) phase_mark(k)
+#else
+#define PHASE(k) ((void)0)
+#endif
+
 // The segment table of a QP's row pairs, in scalar registers.  A row's
 // offsets are formed by adding the steps between consecutive segments
 // selected by comparisons: choosing among the fields themselves lets the
@@ -797,7 +819,10 @@ __device__ bool factor64_retry(const Inst& I, Smem& sm_in, Counters* cnt) {
   const double rel[3] = {1e-14, 1e-12, 1e-10};
   for (int t = 0; t < 3; ++t) {
     gram_syrk<double>(I, I.A, I.P, I.G, sm, rel[t]);
-    if (chol<double>(I.G, I.n, I.LiB, sm) == 0) return true;
+    PHASE(1);
+    const bool ok = chol<double>(I.G, I.n, I.LiB, sm) == 0;
+    PHASE(2);
+    if (ok) return true;
     if (t < 2 && threadIdx.x == 0) atomicAdd(&cnt->jitter_retries, 1ull);
   }
   return false;
@@ -811,7 +836,10 @@ __device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
     float* G = reinterpret_cast<float*>(I.G);
     if (threadIdx.x == 0) I.si[I_F32] += 1;
     gram_syrk<float>(I, I.A32, I.P32, G, sm);
-    if (chol<float>(G, I.n, reinterpret_cast<float*>(I.LiB), sm) == 0) return true;
+    PHASE(1);
+    const bool ok = chol<float>(G, I.n, reinterpret_cast<float*>(I.LiB), sm) == 0;
+    PHASE(2);
+    if (ok) return true;
     if (threadIdx.x == 0) {
       I.si[I_FORCE64] = 1;
       I.si[I_FALLBACK] += 1;
@@ -1000,25 +1028,28 @@ __device__ inline void rowpass2(const double* __restrict__ M, int nrow, int ncol
     st.in = rowload(min(i0 + lrow, nrow - 1));
   };
   auto process = [&](int i0, int c0, const Stage& st) {
+    // With EXT the loads already read zero wherever the mask below would
+    // clear: a row past nrow has extent 0 (and lies past the buffer's end),
+    // and every piece at or past column ncol <= ld lies outside its row's
+    // extent.  Without EXT, rows past nrow and columns past ld hold slack.
     V a[R][NS];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
       for (int s = 0; s < NS; ++s)
-        a[r][s] = (i0 + r < nrow && s * 128 + lane * 2 < ld) ? st.a[r][s] : V(0);
+        a[r][s] = (EXT || (i0 + r < nrow && s * 128 + lane * 2 < ld)) ? st.a[r][s] : V(0);
     double d[NX][R];
 #pragma unroll
     for (int v = 0; v < NX; ++v)
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        d[v][r] = 0.0;
+        // (the first slot's term starts the sum: adding it to 0.0 would cost
+        // an FP64 add per row and change nothing but the sign of a zero)
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-          if constexpr (MAXM)
-            d[v][r] = fmax(d[v][r], fmax(fabs(a[r][s][0]) * xr[v][s][0],
-                                         fabs(a[r][s][1]) * xr[v][s][1]));
-          else
-            d[v][r] += a[r][s][0] * xr[v][s][0] + a[r][s][1] * xr[v][s][1];
+          const double t = MAXM ? fmax(fabs(a[r][s][0]) * xr[v][s][0], fabs(a[r][s][1]) * xr[v][s][1])
+                                : a[r][s][0] * xr[v][s][0] + a[r][s][1] * xr[v][s][1];
+          d[v][r] = s == 0 ? t : (MAXM ? fmax(d[v][r], t) : d[v][r] + t);
         }
       }
 #pragma unroll
@@ -1586,6 +1617,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   double* xc = I.iacc;
   double res = ir_pass_any(I, S1, S0, nullptr, true, I.krhs, I.um, sm);
   if (keep_warm) keep_products(I, S0, keep_warm);
+  PHASE(3);
   IrResult out{0, 1, res, true};
   int solves = 0;
   while (solves < o.max_ir) {
@@ -1593,7 +1625,9 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
     const bool f32 = I.si[I_FORCE64] == 0;
     if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
     factor_solve_current(I, S0.g, xc, sm);
+    PHASE(4);
     const double nr = ir_pass_any(I, S0, S1, xc, false, I.krhs, I.um, sm);
+    PHASE(5);
     if (f32) {
       if (threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
       if (!(nr < tol) && !(nr <= kPoorContraction * old)) {
@@ -1623,6 +1657,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
     __syncthreads();
   }
   if (keep_sol) keep_products(I, S0, keep_sol);
+  PHASE(6);
   out.solves = max(solves, 1);
   out.res = res;
   if (out.status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
@@ -1681,6 +1716,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
     q[1].res = r.b;
   }
   keep_products(I, q[1].S0, keep_warm2);
+  PHASE(3);
   for (;;) {
     bool act[2];
 #pragma unroll
@@ -1693,6 +1729,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
         if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
         factor_solve<double>(I, I.G, I.LiB, q[v].S0.g, q[v].xc, sm);
       }
+    PHASE(4);
     if (act[0] && act[1]) {
       const Res2 r = ir_pass2_any(I, q[0].S0, q[0].S1, q[1].S0, q[1].S1, q[0].xc, q[1].xc,
                                   false, sm);
@@ -1703,6 +1740,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
     } else {
       nr[1] = ir_pass_any(I, q[1].S0, q[1].S1, q[1].xc, false, I.krhs2, I.um2, sm);
     }
+    PHASE(5);
 #pragma unroll
     for (int v = 0; v < 2; ++v)
       if (act[v]) {
@@ -1734,6 +1772,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
     *outs[v] = IrResult{max(q[v].solves, 1), q[v].status, q[v].res, true};
     if (q[v].status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
   }
+  PHASE(6);
 }
 
 // -------------------------------------------------------------------------
@@ -1883,8 +1922,10 @@ __device__ __noinline__ void newton_finish(const Inst& I, const IrResult ir, con
   if (ir.status == 0 || ir.res < 1e-7)
     have = tau_quadratic(I, sol, mu, mu_t, (mu - mu_t) * tau_anchor, &t, sm);
   if (!have) t = tau_linearized(I, sol, mu, mu_t, tau, tau_anchor, sm);
+  PHASE(8);
   for (int k = threadIdx.x; k < N; k += WG) outxy[k] = sol[k] - I.kq[k] * t;
   __syncthreads();
+  PHASE(9);
   *tau_plus = t;
 }
 
@@ -1895,6 +1936,7 @@ __device__ __noinline__ bool newton(const Inst& I, double mu, double mu_t, const
                        IrResult* irout, double* keep_warm, double* keep_sol) {
   Smem sm = lds_view(sm_in);
   newton_rhs(I, mu, mu_t, anchor, corr_m, I.r);
+  PHASE(10);
   const IrResult ir = solve_ir(I, I.r, anchor, I.sol, o, sm, cnt, keep_warm, keep_sol);
   *irout = ir;
   if (!ir.ok) return false;
@@ -1909,6 +1951,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   const int tid = threadIdx.x;
   double* red = sm.red;
   double tau = I.sd[S_TAU];
+  PHASE(-1);
 
   // normalise so ||(x, y, tau)||^2 = m - z + 1
   double nx2 = 0.0;
@@ -1945,6 +1988,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     }
   }
   __syncthreads();
+  PHASE(0);
   if (!factor(I, sm, cnt)) {
     if (tid == 0) I.si[I_STATUS] = kStatusError;
     return;
@@ -1967,10 +2011,12 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     }
     __syncthreads();
     newton_rhs(I, mu, 0.0, I.xp, nullptr, I.r2);
+    PHASE(15);
     solve_ir2(I, I.kq, I.sol, I.xp, o, sm, cnt, &irq, &irp, tk_q, tk_start);
     for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
     __syncthreads();
     if (I.has_p) pmul(I, I.kq, I.pkq, sm);
+    PHASE(7);
     newton_finish(I, irp, I.sol2, mu, 0.0, tau, tau, I.xp, &tau_p, sm);
   } else {
     // kinv_q = K^{-1} [c; b], warm-started from the previous kinv_q
@@ -1989,11 +2035,13 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
       __syncthreads();
       // P kq_x, which both tau quadratics of the iteration use
       if (I.has_p) pmul(I, I.kq, I.pkq, sm);
+      PHASE(7);
     }
 
     // predictor (mu_target = 0) anchored at [x; y]
     for (int k = tid; k < N; k += WG) I.xp[k] = k < n ? I.x[k] : I.y[k - n];
     __syncthreads();
+    PHASE(15);
     if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp, tk_start,
                 nullptr)) {
       if (tid == 0) I.si[I_STATUS] = kStatusError;
@@ -2026,9 +2074,11 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   const double sigma = fmin(fmax(base * base * base, 0.0), 1.0);
 
   // corrector: Mehrotra term, mu_target = sigma mu, anchored at [x_p; y_p]
+  PHASE(11);
   for (int i = tid; i < m; i += WG)
     I.cm[i] = i < z ? 0.0 : -I.ds[i] * (I.xp[n + i] - I.y[i]) / I.y[i];
   __syncthreads();
+  PHASE(12);
   double tau_c;
   if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt, &irc, nullptr,
               tk_c)) {
@@ -2053,6 +2103,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     I.s[i] = sv;
   }
   tau = fmax(tau + step * (tau_c - tau), 1e-30);
+  PHASE(13);
   // An FP32 factor whose three solves took more than kF32StepBudget
   // refinement steps costs more than an FP64 one: every extra step streams A
   // in FP64 once more, more than the FP64 Gram product adds.  Such a QP
@@ -2176,6 +2227,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     I.si[I_ITERS] += 1;
   }
   __syncthreads();
+  PHASE(14);
 }
 
 // -------------------------------------------------------------------------
@@ -3492,4 +3544,13 @@ void qtqp_batch_free(qtqp_batch* h) {
   delete h;
 }
 
+#ifdef QTQP_PHASE_CLOCKS
+// Copies the per-phase clock sums (kPhaseSlots values, 100 MHz ticks summed
+// over workgroups) into out and clears them.
+int qtqp_batch_phase_clocks(unsigned long long* out) {
+  if (hipMemcpyFromSymbol(out, HIP_SYMBOL(g_phase), sizeof(g_phase)) != hipSuccess) return 2;
+  static const unsigned long long zero[kPhaseSlots] = {};
+  return hipMemcpyToSymbol(HIP_SYMBOL(g_phase), zero, sizeof(g_phase)) == hipSuccess ? 0 : 2;
+}
+#endif
 }  // extern "C"
