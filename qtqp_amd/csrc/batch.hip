@@ -141,6 +141,25 @@ constexpr int kLdsU = 2 * NB * (NB + 1) + NB;  // doubles
 constexpr int kLdsV = 1024;
 constexpr int kLdsRed = WG + 8;
 __shared__ double g_lds[kLdsU + kLdsV + kLdsRed];
+// The per-QP ints (status, iterations, FP64 switch, work counts) and scalars
+// (tau, mu, ...) of the QP a workgroup of the IPM kernels holds: loaded from
+// Inst::si / Inst::sd when it takes the QP and written back when it lets it
+// go (qp_take / qp_release), so the iteration reads and counts them in LDS
+// instead of waiting on a global round trip each time.
+__shared__ int g_si[I_NINT];
+__shared__ double g_sd[S_NSCAL];
+
+__device__ inline void qp_take(const Inst& I) {
+  if (threadIdx.x < I_NINT) g_si[threadIdx.x] = I.si[threadIdx.x];
+  if (threadIdx.x < S_NSCAL) g_sd[threadIdx.x] = I.sd[threadIdx.x];
+  __syncthreads();
+}
+__device__ inline void qp_release(const Inst& I) {
+  __syncthreads();
+  if (threadIdx.x < I_NINT) I.si[threadIdx.x] = g_si[threadIdx.x];
+  if (threadIdx.x < S_NSCAL) I.sd[threadIdx.x] = g_sd[threadIdx.x];
+  __syncthreads();
+}
 
 __device__ inline Smem lds_smem() {
   Smem s;
@@ -624,9 +643,11 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
       dst[idx] = Li[idx & (NB - 1)][idx >> 6];
       dst[NB * NB + idx] = Li[idx >> 6][idx & (NB - 1)];
     }
+    PHASE(16);
     if (k0 + NB < n) {
       for (int rb = k0 + NB; rb < n; rb += WG) trsm_mfma_rows(G, n, n, k0, nb, rb, Li);
       __syncthreads();
+      PHASE(17);
       const int t0 = k0 + NB;
       const int tt = (n - t0 + NB - 1) / NB;
       T (*si)[NB + 1] = tile_ptr<T>(sm.u, 0);
@@ -654,6 +675,7 @@ __device__ __noinline__ int chol(T* G, int n, T* LiB, Smem& sm_in) {
       }
     }
     __syncthreads();
+    PHASE(18);
   }
   const int r = info;
   __syncthreads();
@@ -832,22 +854,22 @@ __device__ bool factor64_retry(const Inst& I, Smem& sm_in, Counters* cnt) {
 // FP64.  Returns false on an FP64 breakdown.
 __device__ __noinline__ bool factor(const Inst& I, Smem& sm_in, Counters* cnt) {
   Smem sm = lds_view(sm_in);
-  if (!I.si[I_FORCE64]) {
+  if (!g_si[I_FORCE64]) {
     float* G = reinterpret_cast<float*>(I.G);
-    if (threadIdx.x == 0) I.si[I_F32] += 1;
+    if (threadIdx.x == 0) g_si[I_F32] += 1;
     gram_syrk<float>(I, I.A32, I.P32, G, sm);
     PHASE(1);
     const bool ok = chol<float>(G, I.n, reinterpret_cast<float*>(I.LiB), sm) == 0;
     PHASE(2);
     if (ok) return true;
     if (threadIdx.x == 0) {
-      I.si[I_FORCE64] = 1;
-      I.si[I_FALLBACK] += 1;
+      g_si[I_FORCE64] = 1;
+      g_si[I_FALLBACK] += 1;
       atomicAdd(&cnt->fallbacks, 1ull);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) I.si[I_F64] += 1;
+  if (threadIdx.x == 0) g_si[I_F64] += 1;
   return factor64_retry(I, sm, cnt);
 }
 
@@ -1192,7 +1214,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
   Smem sm = lds_view(sm_in);
   const int n = I.n, mc = I.mc;
   const PairRegs pr(I.pm);
-  if (threadIdx.x == 0) I.si[I_KMV] += 1;
+  if (threadIdx.x == 0) g_si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
   const double* wy = wave_uniform(I.wy);
   const double* ky = wave_uniform(krhs + n);
@@ -1357,7 +1379,7 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
   Smem sm = lds_view(sm_in);
   const int n = I.n, mc = I.mc;
   const PairRegs pr(I.pm);
-  if (threadIdx.x == 0) I.si[I_KMV] += 1;
+  if (threadIdx.x == 0) g_si[I_KMV] += 1;
   double* scratch = reinterpret_cast<double*>(sm.u);
   const double* wy = wave_uniform(I.wy);
   const double* ey = wave_uniform(I.ey);
@@ -1532,19 +1554,22 @@ __device__ __noinline__ void factor_solve(const Inst& I, const T* L, const T* Li
                                           double* xc, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
   const int n = I.n;
-  if (threadIdx.x == 0) I.si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
+  if (threadIdx.x == 0) g_si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
   T* v = reinterpret_cast<T*>(sm.v);
   for (int j = threadIdx.x; j < n; j += WG) v[j] = (T)g[j];
   __syncthreads();
+  PHASE(21);
   trsv_lower_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
+  PHASE(19);
   trsv_upper_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
+  PHASE(20);
   for (int j = threadIdx.x; j < n; j += WG) xc[j] = (double)v[j];
   __syncthreads();
 }
 
 __device__ inline void factor_solve_current(const Inst& I, const double* g, double* xc, Smem& sm_in) {
   Smem sm = lds_view(sm_in);
-  if (I.si[I_FORCE64])
+  if (g_si[I_FORCE64])
     factor_solve<double>(I, I.G, I.LiB, g, xc, sm);
   else
     factor_solve<float>(I, reinterpret_cast<const float*>(I.G),
@@ -1555,9 +1580,9 @@ __device__ inline void factor_solve_current(const Inst& I, const double* g, doub
 __device__ inline bool fallback_fp64(const Inst& I, Smem& sm_in, Counters* cnt) {
   Smem sm = lds_view(sm_in);
   if (threadIdx.x == 0) {
-    I.si[I_FORCE64] = 1;
-    I.si[I_FALLBACK] += 1;
-    I.si[I_F64] += 1;
+    g_si[I_FORCE64] = 1;
+    g_si[I_FALLBACK] += 1;
+    g_si[I_F64] += 1;
     atomicAdd(&cnt->fallbacks, 1ull);
   }
   __syncthreads();
@@ -1622,7 +1647,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   int solves = 0;
   while (solves < o.max_ir) {
     const double old = res;
-    const bool f32 = I.si[I_FORCE64] == 0;
+    const bool f32 = g_si[I_FORCE64] == 0;
     if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
     factor_solve_current(I, S0.g, xc, sm);
     PHASE(4);
@@ -1842,7 +1867,7 @@ __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, doub
     t_c -= v[7];
   }
   if (threadIdx.x == 0 && I.trace) {
-    const int row = I.si[I_ITERS];
+    const int row = g_si[I_ITERS];
     if (row < kTraceRows) {
       double* tr = I.trace + (int64_t)row * kTraceFields + T_DBG0 + (mu_t == 0.0 ? 0 : 4);
       tr[0] = t_a;
@@ -1950,7 +1975,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   const int n = I.n, m = I.m, z = I.z, N = n + m;
   const int tid = threadIdx.x;
   double* red = sm.red;
-  double tau = I.sd[S_TAU];
+  double tau = g_sd[S_TAU];
   PHASE(-1);
 
   // normalise so ||(x, y, tau)||^2 = m - z + 1
@@ -1990,7 +2015,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   __syncthreads();
   PHASE(0);
   if (!factor(I, sm, cnt)) {
-    if (tid == 0) I.si[I_STATUS] = kStatusError;
+    if (tid == 0) g_si[I_STATUS] = kStatusError;
     return;
   }
 
@@ -2002,7 +2027,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   double* tk_start = I.tk;
   double* tk_q = I.tk + tks;
   double* tk_c = I.tk + 2 * tks;
-  if (o.fuse && I.si[I_FORCE64] && I.ld <= kPass2MaxLd) {
+  if (o.fuse && g_si[I_FORCE64] && I.ld <= kPass2MaxLd) {
     // With the FP64 factor, kinv_q and the predictor's solve refine together
     // (solve_ir2): the same two solves, sharing each pass over A and P.
     for (int k = tid; k < N; k += WG) {
@@ -2028,7 +2053,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
       const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt, nullptr, tk_q);
       irq = ir;
       if (!ir.ok) {
-        if (tid == 0) I.si[I_STATUS] = kStatusError;
+        if (tid == 0) g_si[I_STATUS] = kStatusError;
         return;
       }
       for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
@@ -2044,7 +2069,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     PHASE(15);
     if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp, tk_start,
                 nullptr)) {
-      if (tid == 0) I.si[I_STATUS] = kStatusError;
+      if (tid == 0) g_si[I_STATUS] = kStatusError;
       return;
     }
   }
@@ -2082,7 +2107,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   double tau_c;
   if (!newton(I, mu, sigma * mu, I.xp, tau_p, I.cm, tau, I.xp, &tau_c, o, sm, cnt, &irc, nullptr,
               tk_c)) {
-    if (tid == 0) I.si[I_STATUS] = kStatusError;
+    if (tid == 0) g_si[I_STATUS] = kStatusError;
     return;
   }
   for (int i = tid; i < m; i += WG)
@@ -2108,9 +2133,9 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   // refinement steps costs more than an FP64 one: every extra step streams A
   // in FP64 once more, more than the FP64 Gram product adds.  Such a QP
   // factors in FP64 from its next iteration on.
-  if (tid == 0 && !I.si[I_FORCE64] && irq.solves + irp.solves + irc.solves > kF32StepBudget) {
-    I.si[I_FORCE64] = 1;
-    I.si[I_FALLBACK] += 1;
+  if (tid == 0 && !g_si[I_FORCE64] && irq.solves + irp.solves + irc.solves > kF32StepBudget) {
+    g_si[I_FORCE64] = 1;
+    g_si[I_FALLBACK] += 1;
     atomicAdd(&cnt->fallbacks, 1ull);
   }
   __syncthreads();
@@ -2196,7 +2221,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   else if (bty < -kEps && pinf < o.atol_inf + o.rtol_inf * norm_y / fabs(bty))
     status = QTQP_BATCH_INFEASIBLE;
   if (tid == 0) {
-    const int row = I.si[I_ITERS];
+    const int row = g_si[I_ITERS];
     if (row < kTraceRows) {
       double* tr = I.trace + (int64_t)row * kTraceFields;
       tr[T_MU] = mu;
@@ -2216,15 +2241,15 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
       tr[T_DRES] = dres;
       tr[T_GAP] = gap;
     }
-    I.sd[S_TAU] = tau;
-    I.sd[S_MU] = mu;
-    I.sd[S_SIGMA] = sigma;
-    I.sd[S_ALPHA] = alpha;
-    I.sd[S_PRES] = pres;
-    I.sd[S_DRES] = dres;
-    I.sd[S_GAP] = gap;
-    I.si[I_STATUS] = status;
-    I.si[I_ITERS] += 1;
+    g_sd[S_TAU] = tau;
+    g_sd[S_MU] = mu;
+    g_sd[S_SIGMA] = sigma;
+    g_sd[S_ALPHA] = alpha;
+    g_sd[S_PRES] = pres;
+    g_sd[S_DRES] = dres;
+    g_sd[S_GAP] = gap;
+    g_si[I_STATUS] = status;
+    g_si[I_ITERS] += 1;
   }
   __syncthreads();
   PHASE(14);
@@ -2561,17 +2586,18 @@ __global__ __launch_bounds__(WG, 2) void k_ipm_solve(const Inst* insts, const in
     if (q >= count) break;
     const int id = order[q];
     const Inst& I = insts[id];
+    qp_take(I);
     bool ran = false;
-    while (I.si[I_STATUS] == QTQP_BATCH_UNFINISHED && I.si[I_ITERS] < o.max_iter) {
+    while (g_si[I_STATUS] == QTQP_BATCH_UNFINISHED && g_si[I_ITERS] < o.max_iter) {
       ipm_iteration(I, o, sm, cnt);
       ran = true;
     }
     if (threadIdx.x == 0 && ran) {
       atomicAdd(&cnt->done, 1);
-      atomicMax(&cnt->max_iters, I.si[I_ITERS]);
+      atomicMax(&cnt->max_iters, g_si[I_ITERS]);
       clk[id] = wall_clock64();
     }
-    __syncthreads();
+    qp_release(I);
   }
 }
 
@@ -2589,15 +2615,19 @@ __global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int
     __syncthreads();
     if (q >= count) break;
     const Inst& I = insts[order[q]];
-    if (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter) continue;
+    qp_take(I);
+    if (g_si[I_STATUS] != QTQP_BATCH_UNFINISHED || g_si[I_ITERS] >= o.max_iter) {
+      __syncthreads();  // (everyone has read g_si before the next QP's qp_take)
+      continue;
+    }
     ipm_iteration(I, o, sm, cnt);
     if (threadIdx.x == 0 &&
-        (I.si[I_STATUS] != QTQP_BATCH_UNFINISHED || I.si[I_ITERS] >= o.max_iter)) {
+        (g_si[I_STATUS] != QTQP_BATCH_UNFINISHED || g_si[I_ITERS] >= o.max_iter)) {
       atomicAdd(&cnt->done, 1);
-      atomicMax(&cnt->max_iters, I.si[I_ITERS]);
+      atomicMax(&cnt->max_iters, g_si[I_ITERS]);
       if (clk) clk[order[q]] = wall_clock64();
     }
-    __syncthreads();
+    qp_release(I);
   }
 }
 

@@ -166,14 +166,17 @@ class Family:
     self.b0, self.c0 = base.b, base.c
     self.bx = np.stack([q.b - base.b for q in cols], axis=1)
     self.cx = np.stack([q.c - base.c for q in cols], axis=1)
+    # [Bx'; b0] and [Cx'; c0]: b and c of many x0 as one product each
+    self._bxa = np.vstack([self.bx.T, self.b0[None, :]])
+    self._cxa = np.vstack([self.cx.T, self.c0[None, :]])
 
   def bc(self, x0s):
-    """b (count, m) and c (count, n) of the family's QPs at the x0s."""
-    b = x0s @ self.bx.T
-    b += self.b0
-    c = x0s @ self.cx.T
-    c += self.c0
-    return b, c
+    """b (count, m) and c (count, n) of the family's QPs at the x0s:
+    [x0, 1] [Bx'; b0] and [x0, 1] [Cx'; c0]."""
+    x1 = np.empty((x0s.shape[0], self.nx + 1))
+    x1[:, :self.nx] = x0s
+    x1[:, self.nx] = 1.0
+    return x1 @ self._bxa, x1 @ self._cxa
 
   def qps(self, x0s):
     b, c = self.bc(x0s)
@@ -239,7 +242,8 @@ class FamilySolver:
       out["sol_y"].append(res["y"])
       out["sol_s"].append(res["s"])
       out["valid"].append(valid)
-    res = {k: np.concatenate(v) for k, v in out.items()}
+    # (one chunk: its arrays as they are, without another copy of every solution)
+    res = {k: v[0] if len(v) == 1 else np.concatenate(v) for k, v in out.items()}
     res["device_ms"] = t_dev
     return res
 

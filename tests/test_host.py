@@ -307,3 +307,17 @@ def test_family_batch_presolve_rows_must_agree():
     BatchQP._family_keep(np.array([[np.inf, 2.0], [1.0, 2.0]]), 1)
   with pytest.raises(ValueError, match="Inequality RHS"):
     BatchQP._family_keep(np.array([[1.0, -np.inf], [1.0, 2.0]]), 1)
+
+
+def test_family_bc_matches_condensed_mpc():
+  """Family.bc ([x0, 1] times the stacked affine maps) gives each x0's b and c
+  as problems.condensed_mpc builds them for that x0."""
+  from qtqp_amd import problems, warmstart
+  fam = warmstart.Family(horizon=10, nx=4, nu=2, family_seed=3)
+  x0s = problems.parametric_mpc_x0(5, seed=11, nx=4)
+  b, c = fam.bc(x0s)
+  assert b.shape == (5, fam.m) and c.shape == (5, fam.n)
+  for i, x0 in enumerate(x0s):
+    q = problems.condensed_mpc(fam.sys, fam.horizon, x0)
+    np.testing.assert_allclose(b[i], q.b, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(c[i], q.c, rtol=1e-12, atol=1e-12)
