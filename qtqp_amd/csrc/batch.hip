@@ -1305,6 +1305,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
         um[j] = s2;  // A'(W r1_y)
       },
       scratch, I.ext);
+  PHASE(22);
   if (I.has_p) {
     // P_off u with u = sol_x (init) or x_c, added to P_off sol0_x otherwise.
     struct PIn {
@@ -1334,6 +1335,7 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
     for (int j = threadIdx.x; j < n; j += WG) S1.px[j] = 0.0;
     __syncthreads();
   }
+  PHASE(23);
   for (int j = threadIdx.x; j < n; j += WG) {
     const double xv = init ? S1.sol[j] : S0.sol[j] + xc[j];
     if (!init) S1.sol[j] = xv;
@@ -1343,7 +1345,9 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
     const double a = fabs(rx);
     mx = fmax(mx, (a == a) ? a : INFINITY);
   }
-  return bmax(mx, sm.red);
+  const double res = bmax(mx, sm.red);
+  PHASE(24);
+  return res;
 }
 
 __device__ inline double ir_pass_any(const Inst& I, const IrState& S0, const IrState& S1,
