@@ -272,9 +272,35 @@ __device__ inline double bsum(double v, double* red) { return block_sum(v, red +
 __device__ inline double bmax(double v, double* red) { return block_max(v, red + WG); }
 __device__ inline double bmin(double v, double* red) { return -block_max(-v, red + WG); }
 
+// A pointwise loop over k = threadIdx.x, + WG, ... < len, U iterations at a
+// time: ld(k) issues iteration k's loads (through global-qualified pointers;
+// k is clamped to len - 1, so every load is unconditional), the loads of all
+// U iterations go out before the first is used, then body(k, loaded) runs
+// for the iterations in range in the order of the plain loop, so sums
+// accumulate exactly as there.  The plain loop over pointers reached through
+// the QP's table waits for each iteration's loads (flat accesses) before it
+// issues the next iteration's.
+template <int K>
+struct Dv {
+  double v[K];
+};
+template <int U = 4, typename L, typename B>
+__device__ inline void vpass(int len, L ld, B body) {
+  using T = decltype(ld(0));
+  for (int k0 = threadIdx.x; k0 < len; k0 += U * WG) {
+    T t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = ld(min(k0 + u * WG, len - 1));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u * WG < len) body(k0 + u * WG, t[u]);
+  }
+}
+
 __device__ inline double dot(const double* a, const double* b, int len, double* red) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < len; i += WG) s += a[i] * b[i];
+  vpass(len, [&](int i) { return Dv<2>{{ldg(a + i), ldg(b + i)}}; },
+        [&](int, const Dv<2>& t) { s += t.v[0] * t.v[1]; });
   return bsum(s, red);
 }
 
@@ -1811,10 +1837,13 @@ __device__ double max_step(const double* v, const double* dv_a, const double* dv
                            int hi, double* red) {
   // dv = dv_a - dv_b if dv_b else dv_a
   double mn = INFINITY;
-  for (int i = lo + threadIdx.x; i < hi; i += WG) {
-    const double dv = dv_b ? dv_a[i] - dv_b[i] : dv_a[i];
-    if (dv < -kEps) mn = fmin(mn, -v[i] / dv);
-  }
+  const double* dvb = dv_b ? dv_b : dv_a;
+  vpass(hi - lo,
+        [&](int k) { return Dv<3>{{ldg(v + lo + k), ldg(dv_a + lo + k), ldg(dvb + lo + k)}}; },
+        [&](int, const Dv<3>& t) {
+          const double dv = dv_b ? t.v[1] - t.v[2] : t.v[1];
+          if (dv < -kEps) mn = fmin(mn, -t.v[0] / dv);
+        });
   mn = bmin(mn, red);
   return mn == INFINITY ? 1.0 : fmin(1.0, mn);
 }
@@ -1984,38 +2013,44 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 
   // normalise so ||(x, y, tau)||^2 = m - z + 1
   double nx2 = 0.0;
-  for (int k = tid; k < n; k += WG) nx2 += I.x[k] * I.x[k];
+  vpass(n, [&](int k) { return ldg(I.x + k); }, [&](int, double v) { nx2 += v * v; });
   double ny2 = 0.0;
-  for (int k = tid; k < m; k += WG) ny2 += I.y[k] * I.y[k];
+  vpass(m, [&](int k) { return ldg(I.y + k); }, [&](int, double v) { ny2 += v * v; });
   const double nrm = sqrt(bsum(nx2, red) + bsum(ny2, red) + tau * tau);
   const double scale = sqrt((double)(m - z + 1)) / fmax(kEps, nrm);
-  for (int k = tid; k < n; k += WG) I.x[k] *= scale;
-  for (int k = tid; k < m; k += WG) {
-    I.y[k] *= scale;
-    I.s[k] *= scale;
-  }
+  vpass(n, [&](int k) { return ldg(I.x + k); }, [&](int k, double v) { stg(I.x + k, v * scale); });
+  vpass(m, [&](int k) { return Dv<2>{{ldg(I.y + k), ldg(I.s + k)}}; },
+        [&](int k, const Dv<2>& t) {
+          stg(I.y + k, t.v[0] * scale);
+          stg(I.s + k, t.v[1] * scale);
+        });
   tau *= scale;
   __syncthreads();
   const double mu = dot(I.y, I.s, m, red) / (m - z);
 
 
   // regularised diagonal and correction (direct.py:196-216)
-  for (int k = tid; k < N; k += WG) {
-    double t;
-    if (k < n) t = I.P[(int64_t)k * I.ld + k] + mu;
-    else if (k < n + z) t = mu;
-    else t = I.s[k - n] / I.y[k - n] + mu;
-    const double rg = fmax(t, o.min_reg);
-    if (k < n) {
-      I.dg[k] = rg;
-      I.corr[k] = rg - t;
-    } else {
-      I.dg[k] = -rg;
-      I.corr[k] = -rg + t;
-      I.wy[k - n] = -1.0 / rg;
-      I.ey[k - n] = -t;
-    }
-  }
+  vpass(N,
+        [&](int k) {
+          const int j = min(k, n - 1), i = max(k - n, 0);
+          return Dv<3>{{ldg(I.P + (int64_t)j * I.ld + j), ldg(I.s + i), ldg(I.y + i)}};
+        },
+        [&](int k, const Dv<3>& in) {
+          double t;
+          if (k < n) t = in.v[0] + mu;
+          else if (k < n + z) t = mu;
+          else t = in.v[1] / in.v[2] + mu;
+          const double rg = fmax(t, o.min_reg);
+          if (k < n) {
+            stg(I.dg + k, rg);
+            stg(I.corr + k, rg - t);
+          } else {
+            stg(I.dg + k, -rg);
+            stg(I.corr + k, -rg + t);
+            stg(I.wy + (k - n), -1.0 / rg);
+            stg(I.ey + (k - n), -t);
+          }
+        });
   __syncthreads();
   PHASE(0);
   if (!factor(I, sm, cnt)) {
@@ -2034,25 +2069,31 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   if (o.fuse && g_si[I_FORCE64] && I.ld <= kPass2MaxLd) {
     // With the FP64 factor, kinv_q and the predictor's solve refine together
     // (solve_ir2): the same two solves, sharing each pass over A and P.
-    for (int k = tid; k < N; k += WG) {
-      I.r[k] = k < n ? I.c[k] : I.b[k - n];
-      I.xp[k] = k < n ? I.x[k] : I.y[k - n];
-    }
+    vpass(N,
+          [&](int k) {
+            const int j = min(k, n - 1), i = max(k - n, 0);
+            return Dv<4>{{ldg(I.c + j), ldg(I.b + i), ldg(I.x + j), ldg(I.y + i)}};
+          },
+          [&](int k, const Dv<4>& in) {
+            stg(I.r + k, k < n ? in.v[0] : in.v[1]);
+            stg(I.xp + k, k < n ? in.v[2] : in.v[3]);
+          });
     __syncthreads();
     newton_rhs(I, mu, 0.0, I.xp, nullptr, I.r2);
     PHASE(15);
     solve_ir2(I, I.kq, I.sol, I.xp, o, sm, cnt, &irq, &irp, tk_q, tk_start);
-    for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
+    vpass(N, [&](int k) { return ldg(I.sol + k); }, [&](int k, double v) { stg(I.kq + k, v); });
     __syncthreads();
     if (I.has_p) pmul(I, I.kq, I.pkq, sm);
     PHASE(7);
     newton_finish(I, irp, I.sol2, mu, 0.0, tau, tau, I.xp, &tau_p, sm);
   } else {
     // kinv_q = K^{-1} [c; b], warm-started from the previous kinv_q
-    for (int k = tid; k < N; k += WG) I.xp[k] = I.kq[k];
+    vpass(N, [&](int k) { return ldg(I.kq + k); }, [&](int k, double v) { stg(I.xp + k, v); });
     __syncthreads();
     {
-      for (int k = tid; k < N; k += WG) I.r[k] = k < n ? I.c[k] : I.b[k - n];
+      vpass(N, [&](int k) { return Dv<2>{{ldg(I.c + min(k, n - 1)), ldg(I.b + max(k - n, 0))}}; },
+            [&](int k, const Dv<2>& in) { stg(I.r + k, k < n ? in.v[0] : in.v[1]); });
       __syncthreads();
       const IrResult ir = solve_ir(I, I.r, I.xp, I.sol, o, sm, cnt, nullptr, tk_q);
       irq = ir;
@@ -2060,7 +2101,7 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
         if (tid == 0) g_si[I_STATUS] = kStatusError;
         return;
       }
-      for (int k = tid; k < N; k += WG) I.kq[k] = I.sol[k];
+      vpass(N, [&](int k) { return ldg(I.sol + k); }, [&](int k, double v) { stg(I.kq + k, v); });
       __syncthreads();
       // P kq_x, which both tau quadratics of the iteration use
       if (I.has_p) pmul(I, I.kq, I.pkq, sm);
@@ -2068,7 +2109,8 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     }
 
     // predictor (mu_target = 0) anchored at [x; y]
-    for (int k = tid; k < N; k += WG) I.xp[k] = k < n ? I.x[k] : I.y[k - n];
+    vpass(N, [&](int k) { return Dv<2>{{ldg(I.x + min(k, n - 1)), ldg(I.y + max(k - n, 0))}}; },
+          [&](int k, const Dv<2>& in) { stg(I.xp + k, k < n ? in.v[0] : in.v[1]); });
     __syncthreads();
     PHASE(15);
     if (!newton(I, mu, 0.0, I.xp, tau, nullptr, tau, I.xp, &tau_p, o, sm, cnt, &irp, tk_start,
@@ -2078,22 +2120,26 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     }
   }
   // d_s = -y_p s / y ; step, sigma
-  for (int i = tid; i < m; i += WG) I.ds[i] = i < z ? 0.0 : -I.xp[n + i] * I.s[i] / I.y[i];
+  vpass(m, [&](int i) { return Dv<3>{{ldg(I.xp + n + i), ldg(I.s + i), ldg(I.y + i)}}; },
+        [&](int i, const Dv<3>& in) { stg(I.ds + i, i < z ? 0.0 : -in.v[0] * in.v[1] / in.v[2]); });
   __syncthreads();
   const double a_s = max_step(I.s, I.ds, nullptr, z, m, red);
   const double a_y = max_step(I.y, I.xp + n, I.y, z, m, red);
   const double alpha_p = fmin(a_s, a_y);
   double sx = 0.0, sy = 0.0, sys = 0.0;
-  for (int k = tid; k < n; k += WG) {
-    const double xa = I.x[k] + alpha_p * (I.xp[k] - I.x[k]);
-    sx += xa * xa;
-  }
-  for (int i = tid; i < m; i += WG) {
-    const double ya = I.y[i] + alpha_p * (I.xp[n + i] - I.y[i]);
-    const double sa = I.s[i] + alpha_p * I.ds[i];
-    sy += ya * ya;
-    sys += ya * sa;
-  }
+  vpass(n, [&](int k) { return Dv<2>{{ldg(I.x + k), ldg(I.xp + k)}}; },
+        [&](int, const Dv<2>& in) {
+          const double xa = in.v[0] + alpha_p * (in.v[1] - in.v[0]);
+          sx += xa * xa;
+        });
+  vpass(m,
+        [&](int i) { return Dv<4>{{ldg(I.y + i), ldg(I.xp + n + i), ldg(I.s + i), ldg(I.ds + i)}}; },
+        [&](int, const Dv<4>& in) {
+          const double ya = in.v[0] + alpha_p * (in.v[1] - in.v[0]);
+          const double sa = in.v[2] + alpha_p * in.v[3];
+          sy += ya * ya;
+          sys += ya * sa;
+        });
   const double ta = tau + alpha_p * (tau_p - tau);
   double sums[3] = {sx, sy, sys};
   block_reduce_n<3>(sums, red);
@@ -2104,8 +2150,10 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
 
   // corrector: Mehrotra term, mu_target = sigma mu, anchored at [x_p; y_p]
   PHASE(11);
-  for (int i = tid; i < m; i += WG)
-    I.cm[i] = i < z ? 0.0 : -I.ds[i] * (I.xp[n + i] - I.y[i]) / I.y[i];
+  vpass(m, [&](int i) { return Dv<3>{{ldg(I.ds + i), ldg(I.xp + n + i), ldg(I.y + i)}}; },
+        [&](int i, const Dv<3>& in) {
+          stg(I.cm + i, i < z ? 0.0 : -in.v[0] * (in.v[1] - in.v[2]) / in.v[2]);
+        });
   __syncthreads();
   PHASE(12);
   double tau_c;
@@ -2114,23 +2162,30 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
     if (tid == 0) g_si[I_STATUS] = kStatusError;
     return;
   }
-  for (int i = tid; i < m; i += WG)
-    I.ds[i] = i < z ? 0.0 : sigma * mu / I.y[i] + I.cm[i] - I.xp[n + i] * I.s[i] / I.y[i];
+  vpass(m,
+        [&](int i) { return Dv<4>{{ldg(I.y + i), ldg(I.cm + i), ldg(I.xp + n + i), ldg(I.s + i)}}; },
+        [&](int i, const Dv<4>& in) {
+          stg(I.ds + i,
+              i < z ? 0.0 : sigma * mu / in.v[0] + in.v[1] - in.v[2] * in.v[3] / in.v[0]);
+        });
   __syncthreads();
   const double alpha = fmin(max_step(I.s, I.ds, nullptr, z, m, red),
                             max_step(I.y, I.xp + n, I.y, z, m, red));
   const double step = o.step_scale * alpha;
-  for (int k = tid; k < n; k += WG) I.x[k] += step * (I.xp[k] - I.x[k]);
-  for (int i = tid; i < m; i += WG) {
-    double yv = I.y[i] + step * (I.xp[n + i] - I.y[i]);
-    double sv = I.s[i] + step * I.ds[i];
-    if (i >= z) {
-      yv = fmax(yv, 1e-30);
-      sv = fmax(sv, 1e-30);
-    }
-    I.y[i] = yv;
-    I.s[i] = sv;
-  }
+  vpass(n, [&](int k) { return Dv<2>{{ldg(I.x + k), ldg(I.xp + k)}}; },
+        [&](int k, const Dv<2>& in) { stg(I.x + k, in.v[0] + step * (in.v[1] - in.v[0])); });
+  vpass(m,
+        [&](int i) { return Dv<4>{{ldg(I.y + i), ldg(I.xp + n + i), ldg(I.s + i), ldg(I.ds + i)}}; },
+        [&](int i, const Dv<4>& in) {
+          double yv = in.v[0] + step * (in.v[1] - in.v[0]);
+          double sv = in.v[2] + step * in.v[3];
+          if (i >= z) {
+            yv = fmax(yv, 1e-30);
+            sv = fmax(sv, 1e-30);
+          }
+          stg(I.y + i, yv);
+          stg(I.s + i, sv);
+        });
   tau = fmax(tau + step * (tau_c - tau), 1e-30);
   PHASE(13);
   // An FP32 factor whose three solves took more than kF32StepBudget
@@ -2153,52 +2208,74 @@ __device__ __noinline__ void ipm_iteration(const Inst& I, const Opts o, Smem& sm
   // of P times x+).  No pass over A or P is needed.  (The floor of 1e-30 on
   // y+ changes A'y+ by at most 1e-30 |A|.)
   const double inv_tau = 1.0 / fmax(tau, kEps);
+  const bool has_p = I.has_p;
   double* ax = I.um;  // m
   double* aty = I.un; // n
   double* px = I.iacc;  // n
   {
     const PairRegs pr(I.pm);
-    for (int r = tid; r < I.mc; r += WG) {
-      const double t0 = tk_start[r];
-      const double t = t0 + step * ((tk_c[r] - tau_c * tk_q[r]) - t0);
-      int p, q;
-      pr.rows(r, &p, &q);
-      const double v = t / I.d[p];
-      ax[p] = v;
-      if (q >= 0) ax[q] = -v;  // the partner's d equals p's
-    }
-    for (int j = tid; j < n; j += WG) {
-      const double a0 = tk_start[m + j];
-      aty[j] = (a0 + step * ((tk_c[m + j] - tau_c * tk_q[m + j]) - a0)) / I.e[j];
-      if (I.has_p) {
-        const double p0 = tk_start[m + n + j];
-        px[j] = (p0 + step * ((tk_c[m + n + j] - tau_c * tk_q[m + n + j]) - p0)) +
-                I.P[(int64_t)j * I.ld + j] * I.x[j];
-      }
-    }
+    vpass(I.mc,
+          [&](int r) {
+            int p, q;
+            pr.rows(r, &p, &q);
+            return Dv<4>{{ldg(tk_start + r), ldg(tk_c + r), ldg(tk_q + r), ldg(I.d + p)}};
+          },
+          [&](int r, const Dv<4>& in) {
+            const double t0 = in.v[0];
+            const double t = t0 + step * ((in.v[1] - tau_c * in.v[2]) - t0);
+            int p, q;
+            pr.rows(r, &p, &q);
+            const double v = t / in.v[3];
+            stg(ax + p, v);
+            if (q >= 0) stg(ax + q, -v);  // the partner's d equals p's
+          });
+    vpass(n,
+          [&](int j) {
+            return Dv<9>{{ldg(tk_start + m + j), ldg(tk_c + m + j), ldg(tk_q + m + j), ldg(I.e + j),
+                          ldg(tk_start + m + n + j), ldg(tk_c + m + n + j), ldg(tk_q + m + n + j),
+                          ldg(I.P + (int64_t)j * I.ld + j), ldg(I.x + j)}};
+          },
+          [&](int j, const Dv<9>& in) {
+            const double a0 = in.v[0];
+            stg(aty + j, (a0 + step * ((in.v[1] - tau_c * in.v[2]) - a0)) / in.v[3]);
+            if (has_p) {
+              const double p0 = in.v[4];
+              stg(px + j, (p0 + step * ((in.v[5] - tau_c * in.v[6]) - p0)) + in.v[7] * in.v[8]);
+            }
+          });
     __syncthreads();
   }
   double l_xpx = 0.0, l_ctx = 0.0, l_px = 0.0, l_aty = 0.0, l_dres = 0.0, l_nx = 0.0;
-  for (int j = tid; j < n; j += WG) {
-    const double pxj = I.has_p ? px[j] : 0.0;
-    l_xpx += I.x[j] * pxj;
-    l_ctx += I.c[j] * I.x[j];
-    const double pxo = pxj / I.e[j];
-    l_px = fmax(l_px, fabs(pxo));
-    l_aty = fmax(l_aty, fabs(aty[j]));
-    l_dres = fmax(l_dres, fabs((pxo + aty[j]) * inv_tau + I.c[j] / I.e[j]));
-    l_nx = fmax(l_nx, fabs(I.e[j] * I.x[j]));
-  }
+  vpass(n,
+        [&](int j) {
+          return Dv<5>{{ldg(px + j), ldg(I.x + j), ldg(I.c + j), ldg(I.e + j), ldg(aty + j)}};
+        },
+        [&](int, const Dv<5>& in) {
+          const double pxj = has_p ? in.v[0] : 0.0;
+          const double xj = in.v[1], cj = in.v[2], ej = in.v[3], atyj = in.v[4];
+          l_xpx += xj * pxj;
+          l_ctx += cj * xj;
+          const double pxo = pxj / ej;
+          l_px = fmax(l_px, fabs(pxo));
+          l_aty = fmax(l_aty, fabs(atyj));
+          l_dres = fmax(l_dres, fabs((pxo + atyj) * inv_tau + cj / ej));
+          l_nx = fmax(l_nx, fabs(ej * xj));
+        });
   double l_bty = 0.0, l_ax = 0.0, l_s = 0.0, l_pres = 0.0, l_axs = 0.0, l_ny = 0.0;
-  for (int i = tid; i < m; i += WG) {
-    l_bty += I.b[i] * I.y[i];
-    const double so = I.s[i] / I.d[i];
-    l_ax = fmax(l_ax, fabs(ax[i]));
-    l_s = fmax(l_s, fabs(so));
-    l_pres = fmax(l_pres, fabs((ax[i] + so) * inv_tau - I.b[i] / I.d[i]));
-    l_axs = fmax(l_axs, fabs(ax[i] + so));
-    l_ny = fmax(l_ny, fabs(I.d[i] * I.y[i]));
-  }
+  vpass(m,
+        [&](int i) {
+          return Dv<5>{{ldg(I.b + i), ldg(I.y + i), ldg(I.s + i), ldg(I.d + i), ldg(ax + i)}};
+        },
+        [&](int, const Dv<5>& in) {
+          const double bi = in.v[0], yi = in.v[1], di = in.v[3], axi = in.v[4];
+          l_bty += bi * yi;
+          const double so = in.v[2] / di;
+          l_ax = fmax(l_ax, fabs(axi));
+          l_s = fmax(l_s, fabs(so));
+          l_pres = fmax(l_pres, fabs((axi + so) * inv_tau - bi / di));
+          l_axs = fmax(l_axs, fabs(axi + so));
+          l_ny = fmax(l_ny, fabs(di * yi));
+        });
   // three sums (slots 0-2) and eleven maxima, reduced together
   double v[14] = {l_xpx, l_ctx, l_bty, l_px, l_aty, l_dres, l_pres, l_ax,
                   l_s,   l_axs, l_nx,  l_ny, 0.0,  0.0};
