@@ -1622,11 +1622,14 @@ __device__ inline bool fallback_fp64(const Inst& I, Smem& sm_in, Counters* cnt) 
 // Copies a refinement state's products -- A sol_x over the stored rows,
 // A' sol_y and P_off sol_x -- into dst, laid out [m | n | n].
 __device__ inline void keep_products(const Inst& I, const IrState& S, double* dst) {
-  for (int k = threadIdx.x; k < I.mc; k += WG) dst[k] = S.ax[k];
-  for (int j = threadIdx.x; j < I.n; j += WG) {
-    dst[I.m + j] = S.aty[j];
-    dst[I.m + I.n + j] = S.px[j];
-  }
+  const int m = I.m, n = I.n;
+  const double *ax = S.ax, *aty = S.aty, *px = S.px;
+  vpass(I.mc, [&](int k) { return ldg(ax + k); }, [&](int k, double v) { stg(dst + k, v); });
+  vpass(n, [&](int j) { return Dv<2>{{ldg(aty + j), ldg(px + j)}}; },
+        [&](int j, const Dv<2>& in) {
+          stg(dst + m + j, in.v[0]);
+          stg(dst + m + n + j, in.v[1]);
+        });
   __syncthreads();
 }
 
@@ -1657,13 +1660,14 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   Smem sm = lds_view(sm_in);
   const int n = I.n, m = I.m, N = I.n + I.m;
   double mx = 0.0;
-  for (int k = threadIdx.x; k < N; k += WG) {
-    const double v = k < n ? rhs[k] : -rhs[k];
-    I.krhs[k] = v;
-    sol[k] = warm[k];
-    const double a = fabs(v);
-    mx = fmax(mx, (a == a) ? a : INFINITY);
-  }
+  vpass(N, [&](int k) { return Dv<2>{{ldg(rhs + k), ldg(warm + k)}}; },
+        [&](int k, const Dv<2>& in) {
+          const double v = k < n ? in.v[0] : -in.v[0];
+          stg(I.krhs + k, v);
+          stg(sol + k, in.v[1]);
+          const double a = fabs(v);
+          mx = fmax(mx, (a == a) ? a : INFINITY);
+        });
   const double rhs_norm = bmax(mx, sm.red);
   const double tol = o.ls_atol + o.ls_rtol * rhs_norm;
   double* w = I.irb;
@@ -1708,7 +1712,8 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
     }
   }
   if (S0.sol != sol) {
-    for (int k = threadIdx.x; k < N; k += WG) sol[k] = S0.sol[k];
+    const double* src = S0.sol;
+    vpass(N, [&](int k) { return ldg(src + k); }, [&](int k, double v) { stg(sol + k, v); });
     __syncthreads();
   }
   if (keep_sol) keep_products(I, S0, keep_sol);
@@ -1735,18 +1740,22 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
   const int n = I.n, m = I.m, N = I.n + I.m;
   double mx1 = 0.0, mx2 = 0.0;
   double* sol2 = I.sol2;
-  for (int k = threadIdx.x; k < N; k += WG) {
-    const double v1 = k < n ? I.r[k] : -I.r[k];
-    const double v2 = k < n ? I.r2[k] : -I.r2[k];
-    I.krhs[k] = v1;
-    I.krhs2[k] = v2;
-    sol1[k] = warm1[k];
-    sol2[k] = warm2[k];
-    double a = fabs(v1);
-    mx1 = fmax(mx1, (a == a) ? a : INFINITY);
-    a = fabs(v2);
-    mx2 = fmax(mx2, (a == a) ? a : INFINITY);
-  }
+  vpass(N,
+        [&](int k) {
+          return Dv<4>{{ldg(I.r + k), ldg(I.r2 + k), ldg(warm1 + k), ldg(warm2 + k)}};
+        },
+        [&](int k, const Dv<4>& in) {
+          const double v1 = k < n ? in.v[0] : -in.v[0];
+          const double v2 = k < n ? in.v[1] : -in.v[1];
+          stg(I.krhs + k, v1);
+          stg(I.krhs2 + k, v2);
+          stg(sol1 + k, in.v[2]);
+          stg(sol2 + k, in.v[3]);
+          double a = fabs(v1);
+          mx1 = fmax(mx1, (a == a) ? a : INFINITY);
+          a = fabs(v2);
+          mx2 = fmax(mx2, (a == a) ? a : INFINITY);
+        });
   const double norm1 = bmax(mx1, sm.red);
   const double norm2 = bmax(mx2, sm.red);
   struct Run {
@@ -1820,7 +1829,9 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
 #pragma unroll
   for (int v = 0; v < 2; ++v) {
     if (q[v].S0.sol != sols[v]) {
-      for (int k = threadIdx.x; k < N; k += WG) sols[v][k] = q[v].S0.sol[k];
+      const double* src = q[v].S0.sol;
+      double* dst = sols[v];
+      vpass(N, [&](int k) { return ldg(src + k); }, [&](int k, double x) { stg(dst + k, x); });
       __syncthreads();
     }
     if (v == 0) keep_products(I, q[0].S0, keep_sol1);
@@ -1872,22 +1883,29 @@ __device__ __noinline__ bool tau_quadratic(const Inst& I, const double* kr, doub
   // the dot products of the tau quadratic and max|kr|, reduced together:
   // slots kq.c, kq_y.b, kr.c, kr_y.b, kq.Pkq, kr.Pkq, kq.Pkr, kr.Pkr, max|kr|
   double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < n + m; k += WG) v[8] = fmax(v[8], fabs(kr[k]));
-  for (int j = threadIdx.x; j < n; j += WG) {
-    v[0] += I.kq[j] * I.c[j];
-    v[2] += kr[j] * I.c[j];
-  }
-  for (int i = threadIdx.x; i < m; i += WG) {
-    v[1] += I.kq[n + i] * I.b[i];
-    v[3] += kr[n + i] * I.b[i];
-  }
+  vpass(n + m, [&](int k) { return ldg(kr + k); },
+        [&](int, double x) { v[8] = fmax(v[8], fabs(x)); });
+  vpass(n, [&](int j) { return Dv<3>{{ldg(I.kq + j), ldg(kr + j), ldg(I.c + j)}}; },
+        [&](int, const Dv<3>& in) {
+          v[0] += in.v[0] * in.v[2];
+          v[2] += in.v[1] * in.v[2];
+        });
+  vpass(m, [&](int i) { return Dv<3>{{ldg(I.kq + n + i), ldg(kr + n + i), ldg(I.b + i)}}; },
+        [&](int, const Dv<3>& in) {
+          v[1] += in.v[0] * in.v[2];
+          v[3] += in.v[1] * in.v[2];
+        });
   if (I.has_p) {
-    for (int j = threadIdx.x; j < n; j += WG) {
-      v[4] += I.kq[j] * I.pkq[j];
-      v[5] += kr[j] * I.pkq[j];
-      v[6] += I.kq[j] * I.un[j];
-      v[7] += kr[j] * I.un[j];
-    }
+    vpass(n,
+          [&](int j) {
+            return Dv<4>{{ldg(I.kq + j), ldg(kr + j), ldg(I.pkq + j), ldg(I.un + j)}};
+          },
+          [&](int, const Dv<4>& in) {
+            v[4] += in.v[0] * in.v[2];
+            v[5] += in.v[1] * in.v[2];
+            v[6] += in.v[0] * in.v[3];
+            v[7] += in.v[1] * in.v[3];
+          });
   }
   block_reduce_n<9, 0x100u>(v, sm.red);
   const double dot_dbg_kq = v[8];
@@ -1955,6 +1973,9 @@ __device__ __noinline__ double tau_linearized(const Inst& I, const double* kr, d
 __device__ void newton_rhs(const Inst& I, double mu, double mu_t, const double* anchor,
                            const double* corr_m, double* r) {
   const int n = I.n, z = I.z, N = I.n + I.m;
+  // (a plain loop: this function is inlined at two call sites whose results
+  // must agree bit for bit, and with its loads batched the multiply and the
+  // conditional adds below were contracted into FMAs differently in the two)
   for (int k = threadIdx.x; k < N; k += WG) {
     double v = (mu - mu_t) * anchor[k];
     if (k >= n + z) {
@@ -1981,7 +2002,8 @@ __device__ __noinline__ void newton_finish(const Inst& I, const IrResult ir, con
     have = tau_quadratic(I, sol, mu, mu_t, (mu - mu_t) * tau_anchor, &t, sm);
   if (!have) t = tau_linearized(I, sol, mu, mu_t, tau, tau_anchor, sm);
   PHASE(8);
-  for (int k = threadIdx.x; k < N; k += WG) outxy[k] = sol[k] - I.kq[k] * t;
+  vpass(N, [&](int k) { return Dv<2>{{ldg(sol + k), ldg(I.kq + k)}}; },
+        [&](int k, const Dv<2>& in) { stg(outxy + k, in.v[0] - in.v[1] * t); });
   __syncthreads();
   PHASE(9);
   *tau_plus = t;
