@@ -148,16 +148,29 @@ __shared__ double g_lds[kLdsU + kLdsV + kLdsRed];
 // instead of waiting on a global round trip each time.
 __shared__ int g_si[I_NINT];
 __shared__ double g_sd[S_NSCAL];
+// The batch counters a refinement step bumps (backend solves, FP32 inner
+// steps, solves that ended unconverged), counted in LDS while the workgroup
+// holds the QP and added to the launch's Counters when it lets the QP go: a
+// global atomic per step would make the step's next wait for loads wait for
+// the atomic too.
+enum { C_SOLVES = 0, C_INNER, C_IRBAD, C_NCNT };
+__shared__ unsigned g_cnt[C_NCNT];
 
 __device__ inline void qp_take(const Inst& I) {
   if (threadIdx.x < I_NINT) g_si[threadIdx.x] = I.si[threadIdx.x];
   if (threadIdx.x < S_NSCAL) g_sd[threadIdx.x] = I.sd[threadIdx.x];
+  if (threadIdx.x < C_NCNT) g_cnt[threadIdx.x] = 0u;
   __syncthreads();
 }
-__device__ inline void qp_release(const Inst& I) {
+__device__ inline void qp_release(const Inst& I, Counters* cnt) {
   __syncthreads();
   if (threadIdx.x < I_NINT) I.si[threadIdx.x] = g_si[threadIdx.x];
   if (threadIdx.x < S_NSCAL) I.sd[threadIdx.x] = g_sd[threadIdx.x];
+  if (threadIdx.x == 0) {
+    if (g_cnt[C_SOLVES]) atomicAdd(&cnt->backend_solves, (unsigned long long)g_cnt[C_SOLVES]);
+    if (g_cnt[C_INNER]) atomicAdd(&cnt->inner_steps, (unsigned long long)g_cnt[C_INNER]);
+    if (g_cnt[C_IRBAD]) atomicAdd(&cnt->ir_bad, (unsigned long long)g_cnt[C_IRBAD]);
+  }
   __syncthreads();
 }
 
@@ -1586,7 +1599,7 @@ __device__ __noinline__ void factor_solve(const Inst& I, const T* L, const T* Li
   const int n = I.n;
   if (threadIdx.x == 0) g_si[sizeof(T) == 4 ? I_FS32 : I_FS64] += 1;
   T* v = reinterpret_cast<T*>(sm.v);
-  for (int j = threadIdx.x; j < n; j += WG) v[j] = (T)g[j];
+  vpass(n, [&](int j) { return ldg(g + j); }, [&](int j, double x) { v[j] = (T)x; });
   __syncthreads();
   PHASE(21);
   trsv_lower_inv(L, n, LiB, v, sm.red, reinterpret_cast<double*>(sm.u));
@@ -1682,13 +1695,13 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   while (solves < o.max_ir) {
     const double old = res;
     const bool f32 = g_si[I_FORCE64] == 0;
-    if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
+    if (threadIdx.x == 0) g_cnt[C_SOLVES] += 1;
     factor_solve_current(I, S0.g, xc, sm);
     PHASE(4);
     const double nr = ir_pass_any(I, S0, S1, xc, false, I.krhs, I.um, sm);
     PHASE(5);
     if (f32) {
-      if (threadIdx.x == 0) atomicAdd(&cnt->inner_steps, 1ull);
+      if (threadIdx.x == 0) g_cnt[C_INNER] += 1;
       if (!(nr < tol) && !(nr <= kPoorContraction * old)) {
         if (!fallback_fp64(I, sm, cnt)) {
           out.ok = false;
@@ -1720,7 +1733,7 @@ __device__ __noinline__ IrResult solve_ir(const Inst& I, const double* rhs, cons
   PHASE(6);
   out.solves = max(solves, 1);
   out.res = res;
-  if (out.status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
+  if (out.status != 0 && threadIdx.x == 0) g_cnt[C_IRBAD] += 1;
   return out;
 }
 
@@ -1790,7 +1803,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
 #pragma unroll
     for (int v = 0; v < 2; ++v)
       if (act[v]) {
-        if (threadIdx.x == 0) atomicAdd(&cnt->backend_solves, 1ull);
+        if (threadIdx.x == 0) g_cnt[C_SOLVES] += 1;
         factor_solve<double>(I, I.G, I.LiB, q[v].S0.g, q[v].xc, sm);
       }
     PHASE(4);
@@ -1836,7 +1849,7 @@ __device__ __noinline__ void solve_ir2(const Inst& I, const double* warm1, doubl
     }
     if (v == 0) keep_products(I, q[0].S0, keep_sol1);
     *outs[v] = IrResult{max(q[v].solves, 1), q[v].status, q[v].res, true};
-    if (q[v].status != 0 && threadIdx.x == 0) atomicAdd(&cnt->ir_bad, 1ull);
+    if (q[v].status != 0 && threadIdx.x == 0) g_cnt[C_IRBAD] += 1;
   }
   PHASE(6);
 }
@@ -2700,7 +2713,7 @@ __global__ __launch_bounds__(WG, 2) void k_ipm_solve(const Inst* insts, const in
       atomicMax(&cnt->max_iters, g_si[I_ITERS]);
       clk[id] = wall_clock64();
     }
-    qp_release(I);
+    qp_release(I, cnt);
   }
 }
 
@@ -2730,7 +2743,7 @@ __global__ __launch_bounds__(WG, 2) void k_ipm_iter(const Inst* insts, const int
       atomicMax(&cnt->max_iters, g_si[I_ITERS]);
       if (clk) clk[order[q]] = wall_clock64();
     }
-    qp_release(I);
+    qp_release(I, cnt);
   }
 }
 
