@@ -1375,14 +1375,23 @@ __device__ __noinline__ double ir_pass(const Inst& I, const IrState S0, const Ir
     __syncthreads();
   }
   PHASE(23);
-  for (int j = threadIdx.x; j < n; j += WG) {
-    const double xv = init ? S1.sol[j] : S0.sol[j] + xc[j];
-    if (!init) S1.sol[j] = xv;
-    const double rx = krhs[j] - (S1.px[j] + I.dg[j] * xv + S1.aty[j]) + I.corr[j] * xv;
-    S1.r[j] = rx;
-    S1.g[j] = rx + um[j];
-    const double a = fabs(rx);
-    mx = fmax(mx, (a == a) ? a : INFINITY);
+  {
+    const double* solp = init ? S1.sol : S0.sol;
+    const double* xcp = init ? S1.sol : xc;  // (xc is null in init; then unused)
+    vpass(n,
+          [&](int j) {
+            return Dv<8>{{ldg(solp + j), ldg(xcp + j), ldg(krhs + j), ldg(S1.px + j),
+                          ldg(I.dg + j), ldg(S1.aty + j), ldg(I.corr + j), ldg(um + j)}};
+          },
+          [&](int j, const Dv<8>& in) {
+            const double xv = init ? in.v[0] : in.v[0] + in.v[1];
+            if (!init) stg(S1.sol + j, xv);
+            const double rx = in.v[2] - (in.v[3] + in.v[4] * xv + in.v[5]) + in.v[6] * xv;
+            stg(S1.r + j, rx);
+            stg(S1.g + j, rx + in.v[7]);
+            const double a = fabs(rx);
+            mx = fmax(mx, (a == a) ? a : INFINITY);
+          });
   }
   const double res = bmax(mx, sm.red);
   PHASE(24);
@@ -1562,22 +1571,42 @@ __device__ __noinline__ Res2 ir_pass2(const Inst& I, const IrState A0, const IrS
     }
     __syncthreads();
   }
-  for (int j = threadIdx.x; j < n; j += WG) {
+  {
+    // each vector's terms as ir_pass forms them
+    const double* solp[2] = {init ? A1.sol : A0.sol, init ? B1.sol : B0.sol};
+    const double* xcp[2] = {init ? A1.sol : xca, init ? B1.sol : xcb};  // (null in init)
+    const double* krhs[2] = {I.krhs, I.krhs2};
+    const double* um[2] = {I.um, I.um2};
+    const IrState* S1[2] = {&A1, &B1};
+    vpass(n,
+          [&](int j) {
+            Dv<14> t;
 #pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      const IrState& S0 = v == 0 ? A0 : B0;
-      const IrState& S1 = v == 0 ? A1 : B1;
-      const double* xc = v == 0 ? xca : xcb;
-      const double* krhs = v == 0 ? I.krhs : I.krhs2;
-      const double* um = v == 0 ? I.um : I.um2;
-      const double xv = init ? S1.sol[j] : S0.sol[j] + xc[j];
-      if (!init) S1.sol[j] = xv;
-      const double rx = krhs[j] - (S1.px[j] + I.dg[j] * xv + S1.aty[j]) + I.corr[j] * xv;
-      S1.r[j] = rx;
-      S1.g[j] = rx + um[j];
-      const double a = fabs(rx);
-      mx[v] = fmax(mx[v], (a == a) ? a : INFINITY);
-    }
+            for (int v = 0; v < 2; ++v) {
+              t.v[6 * v + 0] = ldg(solp[v] + j);
+              t.v[6 * v + 1] = ldg(xcp[v] + j);
+              t.v[6 * v + 2] = ldg(krhs[v] + j);
+              t.v[6 * v + 3] = ldg(S1[v]->px + j);
+              t.v[6 * v + 4] = ldg(S1[v]->aty + j);
+              t.v[6 * v + 5] = ldg(um[v] + j);
+            }
+            t.v[12] = ldg(I.dg + j);
+            t.v[13] = ldg(I.corr + j);
+            return t;
+          },
+          [&](int j, const Dv<14>& in) {
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              const double* q = in.v + 6 * v;
+              const double xv = init ? q[0] : q[0] + q[1];
+              if (!init) stg(S1[v]->sol + j, xv);
+              const double rx = q[2] - (q[3] + in.v[12] * xv + q[4]) + in.v[13] * xv;
+              stg(S1[v]->r + j, rx);
+              stg(S1[v]->g + j, rx + q[5]);
+              const double a = fabs(rx);
+              mx[v] = fmax(mx[v], (a == a) ? a : INFINITY);
+            }
+          });
   }
   const double ra = bmax(mx[0], sm.red);
   return Res2{ra, bmax(mx[1], sm.red)};
