@@ -638,13 +638,20 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
   // (A[q] = -A[p], so both add w a a' with the same a)
   {
     const PairRegs pr(I.pm);
-    for (int r = threadIdx.x; r < mc; r += WG) {
-      int p, q;
-      pr.rows(r, &p, &q);
-      double w = -1.0 / I.dg[n + p];
-      if (q >= 0) w += -1.0 / I.dg[n + q];
-      I.um[r] = w;
-    }
+    const double* dy = I.dg + n;
+    vpass(mc,
+          [&](int r) {
+            int p, q;
+            pr.rows(r, &p, &q);
+            return Dv<2>{{ldg(dy + p), ldg(dy + (q >= 0 ? q : p))}};
+          },
+          [&](int r, const Dv<2>& in) {
+            int p, q;
+            pr.rows(r, &p, &q);
+            double w = -1.0 / in.v[0];
+            if (q >= 0) w += -1.0 / in.v[1];
+            stg(I.um + r, w);
+          });
   }
   __syncthreads();
   gram_tiles<T>(I, A, P, G, I.um, sm, sizeof(T) == 4 ? I.tile : I.tile64);
@@ -652,10 +659,11 @@ __device__ __noinline__ void gram_syrk(const Inst& I, const T* A, const T* P, T*
   // diag(G) += jit_rel * max(diag(G)), jit_rel = 1e-14 (solvers_dense.py:107-112)
   // or larger on a retry after a breakdown
   double mx = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += WG) mx = fmax(mx, (double)G[(int64_t)i * n + i]);
+  vpass(n, [&](int i) { return (double)ldg(G + (int64_t)i * n + i); },
+        [&](int, double g) { mx = fmax(mx, g); });
   const double jit = jit_rel * bmax(mx, sm.red);
-  for (int i = threadIdx.x; i < n; i += WG)
-    G[(int64_t)i * n + i] = (T)((double)G[(int64_t)i * n + i] + jit);
+  vpass(n, [&](int i) { return (double)ldg(G + (int64_t)i * n + i); },
+        [&](int i, double g) { stg(G + (int64_t)i * n + i, (T)(g + jit)); });
   __syncthreads();
 }
 
