@@ -2455,14 +2455,20 @@ __device__ inline void ruiz(const Inst& I, double* scratch) {
           I.un[j] = fmin(fmax(1.0 / sqrt(cn == 0.0 ? 1.0 : cn), 1e-3), 1e3);
         },
         scratch);
-    for (int r = tid; r < mc; r += WG) {
-      int p, q;
-      pr.rows(r, &p, &q);
-      const double f = I.um[r];
-      I.d[p] *= f;
-      if (q >= 0) I.d[q] *= f;
-    }
-    for (int j = tid; j < n; j += WG) I.e[j] *= I.un[j];
+    vpass(mc,
+          [&](int r) {
+            int p, q;
+            pr.rows(r, &p, &q);
+            return Dv<3>{{ldg(um + r), ldg(dv + p), ldg(dv + (q >= 0 ? q : p))}};
+          },
+          [&](int r, const Dv<3>& in) {
+            int p, q;
+            pr.rows(r, &p, &q);
+            stg(I.d + p, in.v[1] * in.v[0]);
+            if (q >= 0) stg(I.d + q, in.v[2] * in.v[0]);
+          });
+    vpass(n, [&](int j) { return Dv<2>{{ldg(ev + j), ldg(I.un + j)}}; },
+          [&](int j, const Dv<2>& in) { stg(I.e + j, in.v[0] * in.v[1]); });
     __syncthreads();
   }
 }
